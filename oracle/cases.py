@@ -1,0 +1,168 @@
+"""The reference's fingerprint fixture cases, built and digested on the oracle (test infrastructure).
+
+Restates, for the real-storage periodic Cartesian cases, ``array_path_oracle.py``'s
+``CASES`` :427-475, ``_build`` :636-706, ``_stored_volumes`` :709-722, ``_digest_state``
+:1048-1058 and ``_run_case`` :1118-1140. A case's digest is
+
+    sha256("array_path_oracle/real-v6|<case>\\n"
+           + for each step 0..6, for each stored volume in sorted name order:
+                 "<name>|<f4|(nx, ny, nz)\\n" + the float32 bytes, C order
+             and, when E is not stored, "get_E:<Ex|Ey|Ez>|<f4|(..)\\n" + (inv_eps*D) bytes)
+
+The recorded values are in tests/golden/array_path_digests.json (reference
+meep_gpu/cuda_kernels/fingerprints.json, driver_dispatch.array_path_oracle.case_sha256).
+
+``build_state`` returns a plain description (shape, dt, layer faces, material volumes, poles
+and the seeded state words) that both the oracle and the HIP engine's tests load, so the two
+are compared on identical inputs.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass, field
+from typing import Any, Callable, Dict, List, Optional, Tuple
+
+import numpy
+
+from . import fixture, layer
+from .engine import OracleSim, PointDeposit
+
+_E = ("Ex", "Ey", "Ez")
+
+
+@dataclass(frozen=True)
+class Case:
+    name: str
+    cell: Tuple[float, float, float]
+    dimensions: int = 3
+    pml: Any = None
+    offdiagonal: bool = False
+    susceptibilities: Tuple[str, ...] = ()
+
+
+#: The cases this engine steps (periodic, real storage, no symmetry/metal/conductivity).
+CASES: Tuple[Case, ...] = (
+    Case("pml_3d", (1.5, 1.25, 1.0), pml={"x": (2, 3), "y": (2.5, 2), "z": (0, 2)}),
+    Case("pml_2d", (2.0, 1.5, 0.0), dimensions=2, pml={"x": (2.5, 1.5), "y": (1.75, 3)}),
+    Case("no_pml_3d", (1.0, 0.75, 0.5)),
+    Case("offdiag_3d", (1.0, 0.75, 0.75), offdiagonal=True),
+    Case("offdiag_pml_3d", (1.5, 1.25, 1.25), pml=2, offdiagonal=True),
+    Case("dispersive_pml_2d", (1.5, 1.25, 0.0), dimensions=2, pml={"x": 2, "y": 2},
+         susceptibilities=("lorentz", "drude")),
+    Case("dispersive_3d", (1.0, 0.75, 0.75), susceptibilities=("lorentz",)),
+)
+
+CASE_BY_NAME = {case.name: case for case in CASES}
+
+
+@dataclass
+class CaseState:
+    """Everything a stepper needs for one case, as plain arrays."""
+    name: str
+    shape: Tuple[int, int, int]
+    dt: float
+    dx: float
+    invariant: Tuple[bool, bool, bool]
+    faces: Tuple[Tuple[float, float], ...]
+    inv: Dict[str, numpy.ndarray]
+    offdiag: Dict[str, Dict[str, numpy.ndarray]]
+    poles: List[Dict[str, Any]]          # {frequency, gamma, drude, sigma: {Ex: vol}}
+    volumes: Dict[str, numpy.ndarray]    # the seeded stored state, by digest name
+    stores_E: bool
+    pml: bool
+
+
+def _grid_shape(case: Case) -> Tuple[Tuple[int, int, int], Tuple[bool, bool, bool]]:
+    invariant = tuple(case.dimensions == 2 and axis == 2 for axis in range(3))
+    shape = tuple(1 if invariant[a] else layer.cell_count(case.cell[a], fixture.RESOLUTION)
+                  for a in range(3))
+    return shape, invariant
+
+
+def build_state(name: str) -> CaseState:
+    case = CASE_BY_NAME[name]
+    shape, invariant = _grid_shape(case)
+    dx = 1.0 / fixture.RESOLUTION
+    dt = fixture.COURANT / fixture.RESOLUTION
+    epsilon = {c: fixture.volume(fixture.seed(name, "eps", c), shape, 1.0, 3.0) for c in _E}
+    inv = {c: (1.0 / epsilon[c].astype(numpy.float64)).astype(numpy.float32) for c in _E}
+    offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
+    if case.offdiagonal:
+        for row in _E:
+            for partner in _E:
+                if partner == row:
+                    continue
+                vol = fixture.volume(fixture.seed(name, "chi1inv", row, partner), shape,
+                                     -0.0625, 0.0625, zero_fraction=0.4)
+                if numpy.any(vol):
+                    offdiag.setdefault(row, {})[partner] = vol
+    poles = []
+    for index, kind in enumerate(case.susceptibilities):
+        angular = fixture.LORENTZ_ANGULAR if kind == "lorentz" else fixture.DRUDE_ANGULAR
+        poles.append({
+            "frequency": angular / (2.0 * math.pi), "gamma": 0.0625 * (index + 1),
+            "drude": kind == "drude",
+            "sigma": {c: fixture.volume(fixture.seed(name, "pol", str(index), c), shape,
+                                        0.0, 1.0, zero_fraction=0.3) for c in _E}})
+    faces = layer.pml_faces(case.pml, case.dimensions)
+    pml = any(lo > 0 or hi > 0 for lo, hi in faces)
+    stores_E = pml or bool(offdiag) or bool(poles)
+    names = ["D" + c for c in "xyz"] + ["B" + c for c in "xyz"]
+    if stores_E:
+        names += ["E" + c for c in "xyz"]
+    if pml:
+        names += ["H" + c for c in "xyz"]
+        names += [p + f + c for p in ("fu_",) for f in "BD" for c in "xyz"]
+        names += ["f_w_" + f + c for f in "EH" for c in "xyz"]
+    for k, pole in enumerate(poles):
+        for c in _E:
+            if numpy.any(pole["sigma"][c] != 0):
+                names += [f"P[{k}].{c}", f"P_prev[{k}].{c}"]
+    volumes = {n: fixture.state_words(fixture.seed(name, "state", n), shape) for n in names}
+    return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, volumes,
+                     stores_E, pml)
+
+
+def oracle_sim(state: CaseState) -> OracleSim:
+    sim = OracleSim(state.shape, state.dt, state.dx, state.invariant, state.faces,
+                    ln_reflection=fixture.PML_LN_REFLECTION)
+    sim.set_inverse_epsilon(state.inv, state.offdiag)
+    for pole in state.poles:
+        sim.add_pole(pole["frequency"], pole["gamma"], pole["drude"], pole["sigma"])
+    if state.stores_E:
+        sim.enable_field_storage()
+    for n, vol in state.volumes.items():
+        target = sim.stored_volumes()[n]
+        target[...] = vol
+    return sim
+
+
+def digest_update(digest, volumes: Dict[str, numpy.ndarray], stores_E: bool,
+                  get_E: Optional[Callable[[str], numpy.ndarray]] = None) -> None:
+    for n in sorted(volumes):
+        values = numpy.ascontiguousarray(volumes[n])
+        digest.update(f"{n}|{values.dtype.str}|{values.shape}\n".encode("utf-8"))
+        digest.update(values.tobytes())
+    if not stores_E:
+        for c in _E:
+            values = numpy.ascontiguousarray(get_E(c))
+            digest.update(f"get_E:{c}|{values.dtype.str}|{values.shape}\n".encode())
+            digest.update(values.tobytes())
+
+
+def new_digest(name: str):
+    return hashlib.sha256(f"{fixture.FIXTURE_SET}|{name}\n".encode("utf-8"))
+
+
+def run_case(name: str, steps: int = fixture.STEPS) -> str:
+    """The case's digest when the oracle steps it ``steps`` times."""
+    state = build_state(name)
+    sim = oracle_sim(state)
+    digest = new_digest(name)
+    digest_update(digest, sim.stored_volumes(), sim.stores_E, sim.get_E)
+    for _ in range(steps):
+        sim.step()
+        digest_update(digest, sim.stored_volumes(), sim.stores_E, sim.get_E)
+    return digest.hexdigest()

@@ -1,0 +1,314 @@
+"""ctypes driver for the C restatement (ORACLE / test infrastructure only).
+
+``OracleSim`` holds the state the reference's ``Fields`` holds for a real-storage periodic
+Cartesian run (reference fields.py:605-760): D and B always; E when stored (an absorber, a
+susceptibility or an off-diagonal row); H, fu_B, fu_D, f_w_H, f_w_E under an absorber; P and
+P_prev per pole. ``step()`` runs the reference driver's order (driver.py:3419-3596):
+step_B, magnetic deposits, update_H, step_D, electric deposits, update_E, update_P.
+
+Volumes are NumPy float32 C-order (nx, ny, nz) arrays, the reference's own layout.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy
+
+from . import layer
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+class _OGrid(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("present", ctypes.c_int * 3), ("dtdx", ctypes.c_float),
+                ("pml", ctypes.c_int),
+                ("kms", (_FP * 3) * 2), ("sinv", (_FP * 3) * 2), ("kps", (_FP * 3) * 2),
+                ("mem", (_U8P * 3) * 2)]
+
+
+class _OElectric(ctypes.Structure):
+    _fields_ = [("npol", ctypes.c_int), ("p", (_FP * 3) * 4), ("u", (_FP * 3) * 3),
+                ("offdiag", ctypes.c_int)]
+
+
+def build() -> str:
+    """Compile the restatement (make in this directory); returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oracle_deposit.argtypes = [_FP, ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        _lib.oracle_update_p.argtypes = [ctypes.c_void_p, ctypes.c_int, _FP, _FP, _FP, _FP,
+                                         ctypes.c_float, _FP, _FP, _FP]
+    return _lib
+
+
+def _fp(array: Optional[numpy.ndarray]):
+    if array is None:
+        return ctypes.cast(None, _FP)
+    assert array.dtype == numpy.float32 and array.flags.c_contiguous
+    return array.ctypes.data_as(_FP)
+
+
+COMPONENTS = ("x", "y", "z")
+
+
+@dataclass
+class Pole:
+    """One Lorentz/Drude term: frequency and gamma in MEEP units, sigma per component."""
+    frequency: float
+    gamma: float
+    drude: bool
+    sigma: Dict[str, object]          # 'Ex'.. -> float32 volume or scalar
+    P: Dict[str, numpy.ndarray] = field(default_factory=dict)
+    P_prev: Dict[str, numpy.ndarray] = field(default_factory=dict)
+    consts: Optional[numpy.ndarray] = None
+
+
+@dataclass
+class PointDeposit:
+    """A source's deposit table: component, flat C-order cell indices, amplitudes, current(t)."""
+    component: str                    # 'Ex'.. for electric (deposits into D), 'Hx'.. magnetic
+    index: numpy.ndarray              # int64 flat indices
+    amp_re: numpy.ndarray
+    amp_im: numpy.ndarray
+    current: object                   # callable t -> complex
+
+    @property
+    def magnetic(self) -> bool:
+        return self.component[0] in "HB"
+
+
+class OracleSim:
+    """A real-storage periodic Cartesian run of the reference array path, on the CPU."""
+
+    def __init__(self, shape: Sequence[int], dt: float, dx: float,
+                 invariant: Sequence[bool] = (False, False, False),
+                 pml_faces=((0, 0), (0, 0), (0, 0)), pml_order: int = 2,
+                 ln_reflection: Optional[float] = None):
+        self.shape = tuple(int(n) for n in shape)
+        self.dt = float(dt)
+        self.dx = float(dx)
+        self.invariant = tuple(bool(v) for v in invariant)
+        self.faces = tuple(tuple(f) for f in pml_faces)
+        self.pml = any(lo > 0 or hi > 0 for lo, hi in self.faces)
+        self.time = 0.0
+        self.step_count = 0
+        self.h_separated = False
+        zeros = lambda: numpy.zeros(self.shape, dtype=numpy.float32)
+        self.v: Dict[str, numpy.ndarray] = {}
+        for c in COMPONENTS:
+            self.v["D" + c] = zeros()
+            self.v["B" + c] = zeros()
+        self.inv = {("E" + c): numpy.ones(self.shape, dtype=numpy.float32) for c in COMPONENTS}
+        self.offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
+        self.poles: List[Pole] = []
+        self.deposits: List[PointDeposit] = []
+        self.stores_E = False
+        # coefficient vectors and membership lines [half][axis]
+        self._vec = {}
+        for half in (0, 1):
+            for axis in range(3):
+                n = self.shape[axis]
+                faces = self.faces[axis] if not self.invariant[axis] else (0, 0)
+                kms, sinv, kps = layer.axis_coefficients(n, faces, 0.5 * half, self.dt, self.dx,
+                                                         pml_order, ln_reflection)
+                mem = (numpy.zeros(n, dtype=bool) if self.invariant[axis]
+                       else layer.membership_line(n, faces, bool(half)))
+                self._vec[(half, axis)] = (kms, sinv, kps, mem.astype(numpy.uint8))
+        if self.pml:
+            self.enable_field_storage()
+            for c in COMPONENTS:
+                for name in ("H" + c, "fu_B" + c, "fu_D" + c, "f_w_E" + c, "f_w_H" + c):
+                    self.v[name] = zeros()
+
+    # --- configuration -------------------------------------------------------------------
+    def enable_field_storage(self):
+        if not self.stores_E:
+            for c in COMPONENTS:
+                self.v["E" + c] = numpy.zeros(self.shape, dtype=numpy.float32)
+            self.stores_E = True
+
+    def set_inverse_epsilon(self, inv: Dict[str, numpy.ndarray],
+                            offdiag: Optional[Dict[str, Dict[str, numpy.ndarray]]] = None):
+        self.inv = {k: numpy.ascontiguousarray(v, dtype=numpy.float32) for k, v in inv.items()}
+        self.offdiag = {}
+        for row, partners in (offdiag or {}).items():
+            for partner, vol in partners.items():
+                vol = numpy.ascontiguousarray(vol, dtype=numpy.float32)
+                if numpy.any(vol):
+                    self.offdiag.setdefault(row, {})[partner] = vol
+        if self.offdiag:
+            self.enable_field_storage()
+
+    def add_pole(self, frequency: float, gamma: float, drude: bool, sigma: Dict[str, object]):
+        pole = Pole(frequency, gamma, drude, dict(sigma))
+        pole.consts = numpy.array(layer.ade_constants(frequency, gamma, drude, self.dt),
+                                  dtype=numpy.float32)
+        for c in COMPONENTS:
+            s = pole.sigma["E" + c]
+            trivial = (numpy.ndim(s) == 0 and float(s) == 0.0) or \
+                (numpy.ndim(s) > 0 and not numpy.any(numpy.asarray(s) != 0.0))
+            if trivial:
+                continue
+            if numpy.ndim(s) > 0:
+                pole.sigma["E" + c] = numpy.ascontiguousarray(s, dtype=numpy.float32)
+            pole.P["E" + c] = numpy.zeros(self.shape, dtype=numpy.float32)
+            pole.P_prev["E" + c] = numpy.zeros(self.shape, dtype=numpy.float32)
+        self.poles.append(pole)
+        self.enable_field_storage()
+        return pole
+
+    def add_deposit(self, deposit: PointDeposit):
+        self.deposits.append(deposit)
+
+    # --- views ----------------------------------------------------------------------------
+    def stored_volumes(self) -> Dict[str, numpy.ndarray]:
+        out = dict(self.v)
+        for k, pole in enumerate(self.poles):
+            for c, a in pole.P.items():
+                out[f"P[{k}].{c}"] = a
+            for c, a in pole.P_prev.items():
+                out[f"P_prev[{k}].{c}"] = a
+        return out
+
+    def get_E(self, component: str) -> numpy.ndarray:
+        if self.stores_E:
+            return self.v[component]
+        return (self.inv[component] * self.v["D" + component[1]]).astype(numpy.float32)
+
+    # --- the C structures -----------------------------------------------------------------
+    def _grid(self) -> _OGrid:
+        g = _OGrid()
+        for a in range(3):
+            g.n[a] = self.shape[a]
+            g.present[a] = 0 if self.invariant[a] else 1
+        g.dtdx = numpy.float32(self.dt / self.dx)
+        g.pml = 1 if self.pml else 0
+        for half in (0, 1):
+            for a in range(3):
+                kms, sinv, kps, mem = self._vec[(half, a)]
+                g.kms[half][a] = _fp(kms)
+                g.sinv[half][a] = _fp(sinv)
+                g.kps[half][a] = _fp(kps)
+                g.mem[half][a] = mem.ctypes.data_as(_U8P)
+        return g
+
+    def _triple(self, prefix: str):
+        arr = (_FP * 3)()
+        for a, c in enumerate(COMPONENTS):
+            arr[a] = _fp(self.v.get(prefix + c))
+        return arr
+
+    def _electric(self) -> _OElectric:
+        x = _OElectric()
+        x.npol = len(self.poles)
+        for k, pole in enumerate(self.poles):
+            for a, c in enumerate(COMPONENTS):
+                x.p[k][a] = _fp(pole.P.get("E" + c))
+        x.offdiag = 1 if self.offdiag else 0
+        for r, rc in enumerate(COMPONENTS):
+            for p, pc in enumerate(COMPONENTS):
+                x.u[r][p] = _fp(self.offdiag.get("E" + rc, {}).get("E" + pc))
+        return x
+
+    # --- sub-steps ------------------------------------------------------------------------
+    def step_B(self):
+        L = lib()
+        g = self._grid()
+        fu = self._triple("fu_B") if self.pml else None
+        src = self._triple("E") if self.stores_E else None
+        if src is None:
+            src_arrays = [self.get_E("E" + c) for c in COMPONENTS]
+            src = (_FP * 3)(*[_fp(a) for a in src_arrays])
+        L.oracle_step_curl(ctypes.byref(g), 0, self._triple("B"), src, fu)
+
+    def step_D(self):
+        L = lib()
+        g = self._grid()
+        fu = self._triple("fu_D") if self.pml else None
+        L.oracle_step_curl(ctypes.byref(g), 1, self._triple("D"),
+                           self._triple("H") if self.pml else self._triple("B"), fu)
+
+    def update_H(self):
+        if not self.pml:
+            return
+        g = self._grid()
+        lib().oracle_update_h(ctypes.byref(g), self._triple("H"), self._triple("B"),
+                              self._triple("f_w_H"), 1 if self.h_separated else 0)
+        self.h_separated = True
+
+    def update_E(self):
+        if not self.pml and not self.stores_E:
+            return
+        g = self._grid()
+        x = self._electric()
+        inv = (_FP * 3)(*[_fp(self.inv["E" + c]) for c in COMPONENTS])
+        fw = self._triple("f_w_E") if self.pml else None
+        lib().oracle_update_e(ctypes.byref(g), ctypes.byref(x), self._triple("E"),
+                              self._triple("D"), inv, fw)
+
+    def update_P(self):
+        g = self._grid()
+        for pole in self.poles:
+            for c in COMPONENTS:
+                name = "E" + c
+                if name not in pole.P:
+                    continue
+                s = pole.sigma[name]
+                vol = s if numpy.ndim(s) > 0 else None
+                scalar = 0.0 if vol is not None else float(numpy.float32(s))
+                p, pp = pole.P[name], pole.P_prev[name]
+                lib().oracle_update_p(ctypes.byref(g), COMPONENTS.index(c),
+                                      pole.consts.ctypes.data_as(_FP), _fp(p), _fp(pp), _fp(vol),
+                                      ctypes.c_float(scalar), _fp(self.v[name]),
+                                      _fp(self.v.get("f_w_" + name)), _fp(pp))
+                pole.P[name], pole.P_prev[name] = pp, p
+
+    def deposit(self, magnetic: bool, time_value: float):
+        for dep in self.deposits:
+            if dep.magnetic != magnetic:
+                continue
+            cur = complex(dep.current(time_value))
+            if (numpy.copysign(1.0, cur.real) > 0 and cur.real == 0.0
+                    and numpy.copysign(1.0, cur.imag) > 0 and cur.imag == 0.0):
+                continue
+            target = self.v[("B" if magnetic else "D") + dep.component[1]]
+            idx = numpy.ascontiguousarray(dep.index, dtype=numpy.int64)
+            lib().oracle_deposit(_fp(target.reshape(-1)),
+                                 idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 numpy.ascontiguousarray(dep.amp_re, dtype=numpy.float64)
+                                 .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 numpy.ascontiguousarray(dep.amp_im, dtype=numpy.float64)
+                                 .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 int(idx.size), cur.real, cur.imag, self.dt)
+
+    def step(self):
+        self.step_B()
+        self.deposit(True, self.time)
+        self.update_H()
+        self.step_D()
+        self.deposit(False, self.time + 0.5 * self.dt)
+        self.update_E()
+        self.update_P()
+        self.step_count += 1
+        self.time = self.step_count * self.dt

@@ -1,0 +1,328 @@
+/*
+ * CPU restatement of the reference array path's per-step arithmetic (the ORACLE).
+ *
+ * Test infrastructure only: the parity tests, smoke() and bench.py's cpu_baseline leg call
+ * it as the checker / the timed CPU baseline. The product never links or loads it.
+ *
+ * Layout is the reference's: every volume is float32 C-order (nx, ny, nz), z fastest, and
+ * every axis wraps periodically (the fixture cases and the benchmark grids are periodic;
+ * an absorber is a material, not a boundary, reference stepping.py:2984-3040).
+ *
+ * Each function restates one reference routine, operation for operation (one IEEE
+ * rounding per operation, no contraction -- build with -ffp-contract=off):
+ *
+ *   oracle_step_curl  <- stepping.step_B :333 / step_D :463, _curl_operands :2199,
+ *                        _curl_from_operands :2265 (association ((g1s-g1)+g2)-g2s and the
+ *                        absent-partner forms), _apply_curl_regions :2705 with
+ *                        _apply_pml_update :2629 and _apply_dsig_only_update :2662.
+ *                        B: forward differences, half-integer coefficients;
+ *                        D: MEEP's negated strides (backward differences), integer ones.
+ *   oracle_update_h   <- stepping.update_H :1216, separate_h_from_b :2789,
+ *                        _apply_constitutive_regions :2749, _apply_constitutive_pml :2991.
+ *   oracle_update_e   <- stepping.update_E :1243, fields.displacement_minus_polarization
+ *                        fields.py:1173 (D - P0 - P1 ...), _offdiagonal_terms :1594 and
+ *                        _rounded_row :1690 (row summed in double, rounded once).
+ *   oracle_update_p   <- dispersion.PolarizationState.update dispersion.py:689-742 with
+ *                        fields.drive_field fields.py:1234 (w = f_w on the layer's cells).
+ *   oracle_deposit    <- sources._deposit sources.py:1357 + host_writes.subtract_double
+ *                        host_writes.py:140 (double product, one rounding to float32).
+ *   oracle_dft_plane  <- dft.FluxMonitor.update dft.py:3031 with _sliced_component :442 and
+ *                        _apply_yee_interpolation :352 (centre average, then phase*sample).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#define IDX(i, j, k) (((int64_t)(i) * ny + (j)) * nz + (k))
+
+typedef struct {
+    int n[3];              /* nx, ny, nz */
+    int present[3];        /* 1 when the axis is resolved (not translationally invariant) */
+    float dtdx;
+    int pml;               /* 0: no absorber anywhere (plain forms, no auxiliaries) */
+    /* UPML 1-D vectors per [half][axis]: half 0 = integer lattice, 1 = half-integer */
+    const float *kms[2][3];
+    const float *sinv[2][3];
+    const float *kps[2][3];
+    const uint8_t *mem[2][3];  /* one-rank chunk membership lines */
+} ogrid;
+
+static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int axis, int step) {
+    int64_t c[3] = {i, j, k};
+    int n = g->n[axis];
+    c[axis] = (c[axis] + step + n) % n;
+    return ((c[0] * g->n[1]) + c[1]) * g->n[2] + c[2];
+}
+
+/* One curl sub-step. family 0 = B (src = E), 1 = D (src = H). */
+void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *const src[3],
+                      float *const fu[3]) {
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const int step = family == 0 ? +1 : -1;
+    const int half = family == 0 ? 1 : 0;
+    const float dtdx = g->dtdx;
+    for (int c = 0; c < 3; ++c) {
+        const int first = (c + 2) % 3, first_axis = (c + 1) % 3;
+        const int second = (c + 1) % 3, second_axis = (c + 2) % 3;
+        const int dsig = (c + 1) % 3, dsigu = (c + 2) % 3;
+        const int p1 = g->present[first_axis], p2 = g->present[second_axis];
+        float *restrict F = f[c];
+        const float *G1 = src[first], *G2 = src[second];
+        float *U = fu ? fu[c] : 0;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nx; ++i) {
+            for (int j = 0; j < ny; ++j) {
+                for (int k = 0; k < nz; ++k) {
+                    const int64_t at = IDX(i, j, k);
+                    const int ic[3] = {i, j, k};
+                    float curl;
+                    if (p1 && p2) {
+                        float g1s = G1[nb(g, i, j, k, first_axis, step)], g1 = G1[at];
+                        float g2 = G2[at], g2s = G2[nb(g, i, j, k, second_axis, step)];
+                        float t = g1s - g1;
+                        t = t + g2;
+                        t = t - g2s;
+                        curl = dtdx * t;
+                    } else if (p1) {
+                        float t = G1[nb(g, i, j, k, first_axis, step)] - G1[at];
+                        curl = dtdx * t;
+                    } else if (p2) {
+                        float t = G2[nb(g, i, j, k, second_axis, step)] - G2[at];
+                        curl = (-dtdx) * t;
+                    } else {
+                        curl = 0.0f;
+                    }
+                    if (!g->pml) {
+                        F[at] = F[at] - curl;
+                        continue;
+                    }
+                    const float kms = g->kms[half][dsig][ic[dsig]];
+                    const float sinv = g->sinv[half][dsig][ic[dsig]];
+                    if (g->mem[half][dsigu][ic[dsigu]]) {
+                        const float kms_u = g->kms[half][dsigu][ic[dsigu]];
+                        const float sinv_u = g->sinv[half][dsigu][ic[dsigu]];
+                        float prev = U[at];
+                        float u = kms * prev;
+                        u = u - curl;
+                        u = sinv * u;
+                        U[at] = u;
+                        float v = kms_u * F[at];
+                        v = v + u;
+                        v = v - prev;
+                        F[at] = sinv_u * v;
+                    } else {
+                        float v = kms * F[at];
+                        v = v - curl;
+                        F[at] = sinv * v;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* H from B: f_w accumulation on the dsigw (own-axis) layer's cells, H = B elsewhere.
+ * `separated` 0 runs MEEP's first-call start (H and f_w copied from B on those cells). */
+void oracle_update_h(const ogrid *g, float *const h[3], float *const b[3], float *const fw[3],
+                     int separated) {
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    for (int c = 0; c < 3; ++c) {
+        float *H = h[c], *W = fw[c];
+        const float *B = b[c];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nx; ++i) {
+            for (int j = 0; j < ny; ++j) {
+                for (int k = 0; k < nz; ++k) {
+                    const int64_t at = IDX(i, j, k);
+                    const int ic[3] = {i, j, k};
+                    if (g->mem[0][c][ic[c]]) {
+                        if (!separated) { H[at] = B[at]; W[at] = B[at]; }
+                        const float kps = g->kps[0][c][ic[c]], kms = g->kms[0][c][ic[c]];
+                        float prev = W[at];
+                        W[at] = B[at];
+                        float a = kps * W[at];
+                        float m = kms * prev;
+                        H[at] = H[at] + (a - m);
+                    } else {
+                        H[at] = B[at];
+                    }
+                }
+            }
+        }
+    }
+}
+
+typedef struct {
+    int npol;
+    const float *p[4][3];       /* P[k][c], NULL when pole k does not drive component c */
+    const float *u[3][3];       /* off-diagonal chi1inv[row][partner], NULL when absent */
+    int offdiag;                /* any off-diagonal row present */
+} oelectric;
+
+static inline float dmp(const oelectric *x, float *const d[3], int c, int64_t at) {
+    float v = d[c][at];
+    for (int k = 0; k < x->npol; ++k)
+        if (x->p[k][c]) v = v - x->p[k][c][at];
+    return v;
+}
+
+/* E from D: constitutive product (diagonal, or the off-diagonal row in double), then the
+ * f_w accumulation on the own-axis layer's cells under an absorber. */
+void oracle_update_e(const ogrid *g, const oelectric *x, float *const e[3], float *const d[3],
+                     const float *const inv[3], float *const fw[3]) {
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    for (int c = 0; c < 3; ++c) {
+        float *E = e[c];
+        float *W = fw ? fw[c] : 0;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nx; ++i) {
+            for (int j = 0; j < ny; ++j) {
+                for (int k = 0; k < nz; ++k) {
+                    const int64_t at = IDX(i, j, k);
+                    const int ic[3] = {i, j, k};
+                    float cst;
+                    int has_row = 0;
+                    if (x->offdiag)
+                        for (int o = 1; o <= 2; ++o) has_row |= x->u[c][(c + o) % 3] != 0;
+                    if (!has_row) {
+                        cst = inv[c][at] * dmp(x, d, c, at);
+                    } else {
+                        double row = (double)(dmp(x, d, c, at) * inv[c][at]);
+                        for (int o = 1; o <= 2; ++o) {
+                            const int pa = (c + o) % 3;
+                            const float *U = x->u[c][pa];
+                            if (!U) continue;
+                            /* product(q) = u[q] * (g[q] + g[q - s_partner]) */
+                            const int64_t up = nb(g, i, j, k, c, +1);
+                            int64_t q0 = at, q1 = up;
+                            int64_t q0m = nb(g, i, j, k, pa, -1);
+                            int iu[3] = {i, j, k};
+                            iu[c] = (iu[c] + 1) % g->n[c];
+                            int64_t q1m = nb(g, iu[0], iu[1], iu[2], pa, -1);
+                            float pair0 = dmp(x, d, pa, q0) + dmp(x, d, pa, q0m);
+                            float pair1 = dmp(x, d, pa, q1) + dmp(x, d, pa, q1m);
+                            float prod0 = U[q0] * pair0;
+                            float prod1 = U[q1] * pair1;
+                            float summed = prod0 + prod1;
+                            row = row + 0.25 * (double)summed;
+                        }
+                        cst = (float)row;
+                    }
+                    if (g->pml && g->mem[1][c][ic[c]]) {
+                        const float kps = g->kps[1][c][ic[c]], kms = g->kms[1][c][ic[c]];
+                        float prev = W[at];
+                        W[at] = cst;
+                        float a = kps * cst;
+                        float m = kms * prev;
+                        E[at] = E[at] + (a - m);
+                    } else {
+                        E[at] = cst;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* One pole's P update for one component: out = gamma1inv*(((A*p)-(g1*pp))+(od*(s*w))).
+ * `w` is f_w on the own-axis layer's cells under an absorber, E elsewhere. `sigma` NULL
+ * means the scalar `sigma_scalar`. `out` may alias `pp` (the rotation's retired buffer). */
+void oracle_update_p(const ogrid *g, int c, const float consts[4], const float *p,
+                     const float *pp, const float *sigma, float sigma_scalar,
+                     const float *e, const float *fw, float *out) {
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const float A = consts[0], g1 = consts[1], od = consts[2], g1inv = consts[3];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            for (int k = 0; k < nz; ++k) {
+                const int64_t at = IDX(i, j, k);
+                const int ic[3] = {i, j, k};
+                const float w = (g->pml && fw && g->mem[1][c][ic[c]]) ? fw[at] : e[at];
+                const float s = sigma ? sigma[at] : sigma_scalar;
+                float t = A * p[at];
+                t = t - g1 * pp[at];
+                float dr = s * w;
+                dr = od * dr;
+                t = t + dr;
+                out[at] = g1inv * t;
+            }
+        }
+    }
+}
+
+/* f[idx] = float(double(f[idx]) - ((amp_re*c_re - amp_im*c_im) * dt)) at each point. */
+void oracle_deposit(float *f, const int64_t *idx, const double *amp_re, const double *amp_im,
+                    int n, double c_re, double c_im, double dt) {
+    for (int q = 0; q < n; ++q) {
+        double r = amp_re[q] * c_re;
+        r = r - amp_im[q] * c_im;
+        r = r * dt;
+        double w = (double)f[idx[q]];
+        w = w - r;
+        f[idx[q]] = (float)w;
+    }
+}
+
+/* Cell-centred sample of one component over a box [lo, lo+cnt) (MEEP's yee2cent): in each
+ * axis where the component's Yee shift is 0 the field is averaged 0.5*(f[q]+f[q+1]),
+ * x first, then y, then z, each in float32. Periodic wrap past the last row. */
+void oracle_centre_sample(const ogrid *g, const float *f, const int shift[3], const int lo[3],
+                          const int cnt[3], float *out) {
+    const int ny = g->n[1], nz = g->n[2];
+    (void)ny; (void)nz;
+    for (int a = 0; a < cnt[0]; ++a)
+        for (int b = 0; b < cnt[1]; ++b)
+            for (int c = 0; c < cnt[2]; ++c) {
+                /* gather the 2x2x2 (or smaller) neighbourhood in the reference's order */
+                float v[2][2][2];
+                const int ex = shift[0] == 0, ey = shift[1] == 0, ez = shift[2] == 0;
+                for (int da = 0; da <= ex; ++da)
+                    for (int db = 0; db <= ey; ++db)
+                        for (int dc = 0; dc <= ez; ++dc) {
+                            int64_t i = (lo[0] + a + da) % g->n[0];
+                            int64_t j = (lo[1] + b + db) % g->n[1];
+                            int64_t k = (lo[2] + c + dc) % g->n[2];
+                            v[da][db][dc] = f[(i * g->n[1] + j) * g->n[2] + k];
+                        }
+                if (ex)
+                    for (int db = 0; db <= ey; ++db)
+                        for (int dc = 0; dc <= ez; ++dc)
+                            v[0][db][dc] = 0.5f * (v[0][db][dc] + v[1][db][dc]);
+                if (ey)
+                    for (int dc = 0; dc <= ez; ++dc)
+                        v[0][0][dc] = 0.5f * (v[0][0][dc] + v[0][1][dc]);
+                if (ez) v[0][0][0] = 0.5f * (v[0][0][0] + v[0][0][1]);
+                out[((int64_t)a * cnt[1] + b) * cnt[2] + c] = v[0][0][0];
+            }
+}
+
+/* Reference-precision accumulation: acc(complex64) += phase(complex64) * sample(float32),
+ * per part: re += pr*s, im += pi*s (each one float32 rounding). */
+void oracle_dft_accumulate_c64(float *acc, const float *phase, const float *sample, int nf,
+                               int64_t cells) {
+    for (int f = 0; f < nf; ++f) {
+        const float pr = phase[2 * f], pi = phase[2 * f + 1];
+        float *a = acc + 2 * (int64_t)f * cells;
+        for (int64_t q = 0; q < cells; ++q) {
+            float s = sample[q];
+            a[2 * q] = a[2 * q] + pr * s;
+            a[2 * q + 1] = a[2 * q + 1] + pi * s;
+        }
+    }
+}
+
+/* Float64 accumulation against a float64 phase (the oracle for the engine's spectra). */
+void oracle_dft_accumulate_c128(double *acc, const double *phase, const float *sample, int nf,
+                                int64_t cells) {
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < nf; ++f) {
+        const double pr = phase[2 * f], pi = phase[2 * f + 1];
+        double *a = acc + 2 * (int64_t)f * cells;
+        for (int64_t q = 0; q < cells; ++q) {
+            double s = (double)sample[q];
+            a[2 * q] += pr * s;
+            a[2 * q + 1] += pi * s;
+        }
+    }
+}
