@@ -1,0 +1,151 @@
+/*
+ * meep_fdtd.h -- C ABI of the MI355X (gfx950) FDTD time-stepping engine.
+ *
+ * Plain pointers and sizes only. Every volume crosses the boundary in the engine's device
+ * layout, x fastest: element (i, j, k) of an (nx, ny, nz) grid is at ((k*ny)+j)*nx+i. (The
+ * reference stores C-order (nx, ny, nz), z fastest; the Python host transposes on the way
+ * in and out.) Every call returns FDTD_OK or an error code; fdtd_last_error() gives the
+ * text of the calling thread's last failure. Nothing falls back to another path.
+ *
+ * Each entry point names the reference interface it replaces (file:line in the reference
+ * meep_gpu package). The reference reaches its kernels through one seam,
+ * FastPathPlan.dispatch(slot, fields) (fastpath.py:4631) over DRIVER_SLOTS (fastpath.py:575)
+ * = step_B, fill_B, update_H, step_D, fill_D, update_E, update_P; the Python plan object in
+ * meep_gpu.fastpath answers those slots by calling fdtd_step_h / fdtd_step_d below.
+ */
+#ifndef MEEP_FDTD_H
+#define MEEP_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FDTD_OK = 0,
+    FDTD_EINVAL = 1,  /* bad argument or shape */
+    FDTD_EHIP = 2,    /* a HIP runtime call failed */
+    FDTD_ESTATE = 3,  /* call not valid in the simulation's current state */
+    FDTD_ENOMEM = 4   /* device allocation failed */
+};
+
+#define FDTD_MAX_POLES 4
+
+typedef struct fdtd_sim fdtd_sim;
+typedef struct fdtd_batch fdtd_batch;
+
+typedef struct {
+    int32_t n[3];         /* stored cells per axis: nx, ny, nz (grid.py:404-592) */
+    int32_t invariant[3]; /* 1 for a translationally invariant one-cell axis (2-D runs) */
+    float dtdx;           /* dt/dx as float32 (stepping.py:361) */
+    int32_t pml;          /* 1 when an absorber is installed (fields.py:708) */
+    double dt;            /* the timestep, for monitor sample times (grid.py:428) */
+} fdtd_config;
+
+/* Text of the calling thread's last error (empty string when none). */
+const char *fdtd_last_error(void);
+
+/* Engine build and device description, for reports. */
+int fdtd_device_info(char *text, size_t size);
+
+/* Create a simulation: D, B, E always; H, fu_B, fu_D, f_w_H, f_w_E when cfg->pml.
+ * Replaces Fields allocation (fields.py:605-760) and enable_pml_storage (fields.py:708).
+ * All volumes start at +0; inverse epsilon starts at 1. */
+int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out);
+void fdtd_sim_destroy(fdtd_sim *sim);
+
+/* Bytes of device memory the simulation holds. */
+int fdtd_sim_bytes(const fdtd_sim *sim, uint64_t *bytes);
+
+/* One axis of the UPML table and its one-rank chunk membership line: half 0 = integer
+ * lattice, 1 = half-integer; vectors of length n[axis]. Replaces PML._compute_coefficients
+ * (pml.py:673) and RegionPlan membership (chunk_map.py:180, :1093). */
+int fdtd_set_layer(fdtd_sim *sim, int32_t half, int32_t axis, const float *kms,
+                   const float *sinv, const float *kps, const uint8_t *member);
+
+/* Diagonal chi1inv for E component c (0..2), one volume (fields.py:1396, :1523). */
+int fdtd_set_inverse_epsilon(fdtd_sim *sim, int32_t component, const float *volume);
+
+/* Off-diagonal chi1inv[row][partner] volume (fields.py:1396, stepping.py:1594). A row with
+ * any partner switches update_E to the double-precision row form. */
+int fdtd_set_offdiagonal(fdtd_sim *sim, int32_t row, int32_t partner, const float *volume);
+
+/* One Lorentz/Drude pole (dispersion.py:243, :645). consts = {2-omega0dtsqr_denom, gamma1,
+ * omega0dtsqr, gamma1inv}; sigma[c] a volume, or NULL with sigma_scalar[c] a uniform value;
+ * driven[c] 0 leaves component c without a P (a trivial sigma). Returns the pole index. */
+int fdtd_add_pole(fdtd_sim *sim, const float consts[4], const float *const sigma[3],
+                  const float sigma_scalar[3], const int32_t driven[3], int32_t *pole);
+
+/* A source deposit over the box [lo, lo+cnt): amplitudes amp_re/amp_im per box cell (x
+ * fastest), and the complex current J for steps 0..nsteps-1 as (re, im) pairs: the electric
+ * table at (n+1/2)*dt, the magnetic at n*dt. field 0 deposits into D (after step_D), 1 into
+ * B (after step_B). Replaces sources._deposit (sources.py:1357) + host_writes.subtract_double
+ * (host_writes.py:140): f = float(double(f) - (amp_re*J_re - amp_im*J_im)*dt). */
+int fdtd_add_source(fdtd_sim *sim, int32_t field, int32_t component, const int32_t lo[3],
+                    const int32_t cnt[3], const double *amp_re, const double *amp_im,
+                    const double *current, int32_t nsteps, double dt);
+
+/* Upload / download one volume by name: "Dx".."Bz", "Ex".., "Hx".., "fu_Bx".., "fu_Dx"..,
+ * "f_w_Ex".., "f_w_Hx".., "P[k].Ex", "P_prev[k].Ex" (the reference's digest names,
+ * array_path_oracle.py:709). count = nx*ny*nz floats, x fastest. */
+int fdtd_upload(fdtd_sim *sim, const char *name, const float *host, size_t count);
+int fdtd_download(fdtd_sim *sim, const char *name, float *host, size_t count);
+
+/* The step counter (driver.py:3592) and MEEP's one-time H start (stepping.py:2789). */
+int fdtd_get_step(fdtd_sim *sim, int32_t *step);
+int fdtd_set_step(fdtd_sim *sim, int32_t step, int32_t h_separated);
+
+/* E = chi1inv*(D - sum P) everywhere, for a run whose E the reference does not store (no
+ * absorber, pole or off-diagonal row): its get_E (fields.py:1051). Call once after upload. */
+int fdtd_prime_e(fdtd_sim *sim);
+
+/* The fused H pair: step_B + magnetic deposit + update_H (stepping.py:333, :1216;
+ * driver.py:3533-3556). Answers the step_B and update_H slots. */
+int fdtd_step_h(fdtd_sim *sim);
+
+/* The fused D pair: step_D + electric deposit + update_E + update_P, then the step counter
+ * advances (stepping.py:463, :1243, :1962; driver.py:3560-3592). Answers step_D, update_E,
+ * update_P. */
+int fdtd_step_d(fdtd_sim *sim);
+
+/* nsteps whole steps (H pair, D pair, flux accumulation) on the simulation's stream. */
+int fdtd_run(fdtd_sim *sim, int32_t nsteps);
+
+/* Wait for the simulation's stream. */
+int fdtd_synchronize(fdtd_sim *sim);
+
+/* A planar flux monitor over the box [lo, lo+cnt) (one count is 1 along normal_axis):
+ * nf angular frequencies omega, scale = dt*decimation/sqrt(2*pi). The four tangential
+ * components are sampled at cell centres each accumulating step and transformed in float64
+ * (phase exp(i*omega*t) evaluated in double per step; dft.py:316, :3031). Returns its id. */
+int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
+                        const int32_t cnt[3], int32_t nf, const double *omega, double scale,
+                        int32_t decimation, int32_t *plane);
+
+/* Copy one plane's accumulated transforms to the host as float64 (re, im) pairs, laid out
+ * [component: E1, E2, H1, H2 (tangential, cyclic after the normal)][frequency][cell, x
+ * fastest over the plane box]. Flushes buffered samples first. */
+int fdtd_flux_read(fdtd_sim *sim, int32_t plane, double *out, size_t count);
+
+/* Batched form: many simulations advanced together, one persistent fused launch per
+ * half-step over a work list of (simulation, tile) entries. The batch does not own the
+ * simulations. */
+int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out);
+void fdtd_batch_destroy(fdtd_batch *batch);
+int fdtd_batch_run(fdtd_batch *batch, int32_t nsteps);
+int fdtd_batch_synchronize(fdtd_batch *batch);
+
+/* Launch shape knobs (the autotuner sets them): workgroups per CU for the persistent
+ * launch and whether the tile order is remapped XCD-contiguous. */
+int fdtd_set_launch(int32_t blocks_per_cu, int32_t xcd_remap);
+
+/* Measure average kernel durations over `repeats` launches of each fused pair on the
+ * simulation's stream with HIP events: out_ms[0] = H pair, out_ms[1] = D pair. */
+int fdtd_time_pairs(fdtd_sim *sim, int32_t repeats, float out_ms[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

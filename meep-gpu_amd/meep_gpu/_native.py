@@ -1,0 +1,111 @@
+"""ctypes binding of the HIP engine library (include/meep_fdtd.h).
+
+The library is built in-tree (``meep_gpu._build.build``) as ``meep_gpu/_lib/libmeep_fdtd.so``.
+There is no other compute path: if the library is missing or its call fails, the caller gets
+an exception naming the reason, never a CPU substitute.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libmeep_fdtd.so")
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+
+FDTD_MAX_POLES = 4
+
+#: Every entry point include/meep_fdtd.h declares, with its ctypes signature.
+SIGNATURES = {
+    "fdtd_last_error": (ctypes.c_char_p, []),
+    "fdtd_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "fdtd_sim_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "fdtd_sim_destroy": (None, [ctypes.c_void_p]),
+    "fdtd_sim_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "fdtd_set_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _FP, _FP,
+                                      _FP, _U8P]),
+    "fdtd_set_inverse_epsilon": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
+    "fdtd_set_offdiagonal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                            _FP]),
+    "fdtd_add_pole": (ctypes.c_int, [ctypes.c_void_p, _FP, ctypes.POINTER(_FP), _FP, _IP, _IP]),
+    "fdtd_add_source": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _IP, _IP,
+                                       _DP, _DP, _DP, ctypes.c_int32, ctypes.c_double]),
+    "fdtd_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _FP, ctypes.c_size_t]),
+    "fdtd_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _FP, ctypes.c_size_t]),
+    "fdtd_get_step": (ctypes.c_int, [ctypes.c_void_p, _IP]),
+    "fdtd_set_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "fdtd_prime_e": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_step_h": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_step_d": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fdtd_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_add_flux_plane": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _IP, _IP,
+                                           ctypes.c_int32, _DP, ctypes.c_double, ctypes.c_int32,
+                                           _IP]),
+    "fdtd_flux_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _DP, ctypes.c_size_t]),
+    "fdtd_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "fdtd_batch_destroy": (None, [ctypes.c_void_p]),
+    "fdtd_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fdtd_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_set_launch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
+    "fdtd_time_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
+}
+
+
+class FdtdConfig(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("invariant", ctypes.c_int32 * 3),
+                ("dtdx", ctypes.c_float), ("pml", ctypes.c_int32), ("dt", ctypes.c_double)]
+
+
+class EngineError(RuntimeError):
+    """A failed engine call, with the engine's own reason."""
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the engine library and bind every declared symbol (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(
+            f"the HIP engine library is not built ({path} is missing); run "
+            f"`python -c 'import __graft_entry__ as g; g.build()'` from the repository root")
+    lib = ctypes.CDLL(path)
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        text = load().fdtd_last_error().decode("utf-8", "replace")
+        raise EngineError(f"{what}: {text} (code {rc})")
+
+
+def fptr(array) -> ctypes.POINTER(ctypes.c_float):
+    return array.ctypes.data_as(_FP)
+
+
+def dptr(array) -> ctypes.POINTER(ctypes.c_double):
+    return array.ctypes.data_as(_DP)
+
+
+def iptr(array) -> ctypes.POINTER(ctypes.c_int32):
+    return array.ctypes.data_as(_IP)
+
+
+def u8ptr(array) -> ctypes.POINTER(ctypes.c_uint8):
+    return array.ctypes.data_as(_U8P)

@@ -1,0 +1,205 @@
+"""One simulation resident on the GPU: the Python face of an ``fdtd_sim`` handle.
+
+Host arrays use the reference's layout, float32 C-order ``(nx, ny, nz)`` (z fastest); the
+device stores x fastest, so every transfer transposes. Volumes are named as the reference's
+``Fields`` names them (``Dx``, ``fu_Bx``, ``f_w_Ex``, ``P[0].Ex``, ``P_prev[0].Ex``...).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy
+
+from . import _native
+from ._native import check, dptr, fptr, iptr, u8ptr
+from .dispersion import Susceptibility
+from .grid import Grid
+from .pml import PML
+
+_AXES = ("x", "y", "z")
+
+
+def to_device_layout(array: numpy.ndarray) -> numpy.ndarray:
+    """C-order (nx, ny, nz) -> contiguous (nz, ny, nx), i.e. x fastest."""
+    return numpy.ascontiguousarray(numpy.asarray(array, dtype=numpy.float32).transpose(2, 1, 0))
+
+
+def from_device_layout(buffer: numpy.ndarray) -> numpy.ndarray:
+    return numpy.ascontiguousarray(buffer.transpose(2, 1, 0))
+
+
+class HipSimulation:
+    """A handle on device-resident fields, layer table, materials, poles, sources, monitors."""
+
+    def __init__(self, grid: Grid, pml: Optional[PML] = None):
+        self.lib = _native.load()
+        self.grid = grid
+        self.pml = pml if (pml is not None and pml.is_active) else None
+        cfg = _native.FdtdConfig()
+        for a in range(3):
+            cfg.n[a] = grid.shape[a]
+            cfg.invariant[a] = 1 if grid.is_invariant(a) else 0
+        cfg.dtdx = numpy.float32(grid.dt / grid.dx)
+        cfg.pml = 1 if self.pml is not None else 0
+        cfg.dt = grid.dt
+        handle = ctypes.c_void_p()
+        check(self.lib.fdtd_sim_create(ctypes.byref(cfg), ctypes.byref(handle)), "fdtd_sim_create")
+        self.handle = handle
+        self.poles = []
+        self.planes = []
+        if self.pml is not None:
+            for (half, axis), (kms, sinv, kps) in self.pml.vectors.items():
+                member = self.pml.members[(half, axis)].astype(numpy.uint8)
+                check(self.lib.fdtd_set_layer(self.handle, half, axis, fptr(kms), fptr(sinv),
+                                              fptr(kps), u8ptr(member)), "fdtd_set_layer")
+
+    # --- lifetime ----------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.fdtd_sim_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        out = ctypes.c_uint64()
+        check(self.lib.fdtd_sim_bytes(self.handle, ctypes.byref(out)), "fdtd_sim_bytes")
+        return int(out.value)
+
+    # --- materials ---------------------------------------------------------------------------
+    def set_inverse_epsilon(self, component: str, volume: numpy.ndarray):
+        dev = to_device_layout(volume)
+        check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
+                                                fptr(dev)), "fdtd_set_inverse_epsilon")
+
+    def set_offdiagonal(self, row: str, partner: str, volume: numpy.ndarray):
+        dev = to_device_layout(volume)
+        check(self.lib.fdtd_set_offdiagonal(self.handle, _AXES.index(row[1]),
+                                            _AXES.index(partner[1]), fptr(dev)),
+              "fdtd_set_offdiagonal")
+
+    def add_pole(self, term: Susceptibility, sigma: Dict[str, object]) -> Tuple[bool, bool, bool]:
+        """Install one pole; ``sigma[Ex..]`` a volume or a scalar. Returns which components it drives."""
+        consts = numpy.array(term.realnum_constants(self.grid.dt), dtype=numpy.float32)
+        keep = []
+        ptrs = (_native._FP * 3)()
+        scalars = numpy.zeros(3, dtype=numpy.float32)
+        driven = numpy.zeros(3, dtype=numpy.int32)
+        for a, c in enumerate(_AXES):
+            value = sigma.get("E" + c, 0.0)
+            if numpy.ndim(value) == 0:
+                scalars[a] = numpy.float32(value)
+                driven[a] = 1 if float(value) != 0.0 else 0
+                ptrs[a] = ctypes.cast(None, _native._FP)
+            else:
+                vol = to_device_layout(value)
+                keep.append(vol)
+                driven[a] = 1 if numpy.any(vol != 0.0) else 0
+                ptrs[a] = fptr(vol)
+        index = numpy.zeros(1, dtype=numpy.int32)
+        check(self.lib.fdtd_add_pole(self.handle, fptr(consts), ptrs, fptr(scalars), iptr(driven),
+                                     iptr(index)), "fdtd_add_pole")
+        self.poles.append(tuple(bool(v) for v in driven))
+        return self.poles[-1]
+
+    def add_source(self, magnetic: bool, component: int, lo: Sequence[int], cnt: Sequence[int],
+                   amp_re: numpy.ndarray, amp_im: numpy.ndarray, current: numpy.ndarray,
+                   dt: float):
+        """A deposit over box [lo, lo+cnt); amplitudes C-order (cnt), current (nsteps, 2) f64."""
+        re = numpy.ascontiguousarray(numpy.asarray(amp_re, dtype=numpy.float64)
+                                     .reshape(tuple(cnt)).transpose(2, 1, 0))
+        im = numpy.ascontiguousarray(numpy.asarray(amp_im, dtype=numpy.float64)
+                                     .reshape(tuple(cnt)).transpose(2, 1, 0))
+        cur = numpy.ascontiguousarray(current, dtype=numpy.float64).reshape(-1, 2)
+        lo_a = numpy.asarray(lo, dtype=numpy.int32)
+        cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
+        check(self.lib.fdtd_add_source(self.handle, 1 if magnetic else 0, int(component),
+                                       iptr(lo_a), iptr(cnt_a), dptr(re), dptr(im), dptr(cur),
+                                       int(cur.shape[0]), float(dt)), "fdtd_add_source")
+
+    # --- state -------------------------------------------------------------------------------
+    def upload(self, name: str, array: numpy.ndarray):
+        dev = to_device_layout(array)
+        check(self.lib.fdtd_upload(self.handle, name.encode(), fptr(dev), dev.size),
+              f"fdtd_upload({name})")
+
+    def download(self, name: str) -> numpy.ndarray:
+        nx, ny, nz = self.grid.shape
+        buf = numpy.empty((nz, ny, nx), dtype=numpy.float32)
+        check(self.lib.fdtd_download(self.handle, name.encode(), fptr(buf), buf.size),
+              f"fdtd_download({name})")
+        return from_device_layout(buf)
+
+    @property
+    def step_count(self) -> int:
+        out = numpy.zeros(1, dtype=numpy.int32)
+        check(self.lib.fdtd_get_step(self.handle, iptr(out)), "fdtd_get_step")
+        return int(out[0])
+
+    def set_step(self, step: int, h_separated: bool):
+        check(self.lib.fdtd_set_step(self.handle, int(step), 1 if h_separated else 0),
+              "fdtd_set_step")
+
+    def prime_e(self):
+        check(self.lib.fdtd_prime_e(self.handle), "fdtd_prime_e")
+
+    # --- stepping ----------------------------------------------------------------------------
+    def step_h(self):
+        check(self.lib.fdtd_step_h(self.handle), "fdtd_step_h")
+
+    def step_d(self):
+        check(self.lib.fdtd_step_d(self.handle), "fdtd_step_d")
+
+    def run(self, steps: int):
+        check(self.lib.fdtd_run(self.handle, int(steps)), "fdtd_run")
+
+    def synchronize(self):
+        check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
+
+    def time_pairs(self, repeats: int) -> Tuple[float, float]:
+        out = numpy.zeros(2, dtype=numpy.float32)
+        check(self.lib.fdtd_time_pairs(self.handle, int(repeats), fptr(out)), "fdtd_time_pairs")
+        return float(out[0]), float(out[1])
+
+    # --- flux --------------------------------------------------------------------------------
+    def add_flux_plane(self, normal: int, lo: Sequence[int], cnt: Sequence[int],
+                       omegas: numpy.ndarray, scale: float, decimation: int = 1) -> int:
+        om = numpy.ascontiguousarray(omegas, dtype=numpy.float64)
+        lo_a = numpy.asarray(lo, dtype=numpy.int32)
+        cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
+        out = numpy.zeros(1, dtype=numpy.int32)
+        check(self.lib.fdtd_add_flux_plane(self.handle, int(normal), iptr(lo_a), iptr(cnt_a),
+                                           int(om.size), dptr(om), float(scale), int(decimation),
+                                           iptr(out)), "fdtd_add_flux_plane")
+        self.planes.append((int(normal), tuple(int(v) for v in lo), tuple(int(v) for v in cnt),
+                            int(om.size)))
+        return int(out[0])
+
+    def flux_read(self, plane: int) -> numpy.ndarray:
+        """complex128 transforms [4 (E1, E2, H1, H2)][nf][cnt...] in C-order over the box."""
+        normal, lo, cnt, nf = self.planes[plane]
+        cells = int(numpy.prod(cnt))
+        buf = numpy.empty(2 * 4 * nf * cells, dtype=numpy.float64)
+        check(self.lib.fdtd_flux_read(self.handle, int(plane), dptr(buf), buf.size),
+              "fdtd_flux_read")
+        data = buf.view(numpy.complex128).reshape(4, nf, cnt[2], cnt[1], cnt[0])
+        return numpy.ascontiguousarray(data.transpose(0, 1, 4, 3, 2))
+
+
+def device_info() -> str:
+    lib = _native.load()
+    buf = ctypes.create_string_buffer(256)
+    check(lib.fdtd_device_info(buf, 256), "fdtd_device_info")
+    return buf.value.decode()
+
+
+def set_launch(blocks_per_cu: int, xcd_remap: bool):
+    check(_native.load().fdtd_set_launch(int(blocks_per_cu), 1 if xcd_remap else 0),
+          "fdtd_set_launch")
