@@ -1,0 +1,176 @@
+"""Benchmark: the plain single-simulation FDTD step on one MI355X.
+
+Workload (default): the 192^3 dielectric sphere (7,077,888 cells, 24 px/unit, 8-cell UPML on
+all faces, sphere radius 38.4 cells, eps 2.25 with a sub-pixel diagonal chi1inv, Ez Gaussian
+point source fcen 0.15 / df 0.1), 8 flux planes x 100 frequencies transformed in float64.
+A step is one whole time step: the fused H pair, the fused D pair and the flux sampling (and,
+every 32 steps, the float64 transform pass). Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line. ``value`` is Mcell-steps/s = cells * steps / elapsed; ``roofline`` is
+for the D pair (the heavier fused launch): algorithmic bytes per launch over its mean launch
+time from HIP events around every launch in the timed region; ``cpu_baseline`` times the C
+restatement (OpenMP on the host cores) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "meep-gpu_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(sim) -> dict:
+    """Compulsory HBM bytes per launch of each fused pair (every array touched once, neighbour
+    reads free, auxiliaries only on their membership rows)."""
+    grid = sim.grid
+    n = grid.total_cells
+    pml = sim.pml
+    frac = (lambda axis, half: pml.member_fraction(axis, half)) if pml else (lambda a, h: 0.0)
+    # H pair: E r 12, B r+w 24; per component c: fu_B r/w 8 on its dsigu line (half lattice),
+    # H r/w + f_w_H r/w 16 on its own-axis line (integer lattice; H equals B elsewhere and is
+    # not stored there).
+    h = n * (12 + 24)
+    d = n * (12 + 24 + 12 + 12)  # H-or-B r, D r/w, inv r, E w
+    for c in range(3):
+        dsigu = (c + 2) % 3
+        h += n * (8 * frac(dsigu, 1) + 16 * frac(c, 0))
+        d += n * (8 * frac(dsigu, 0) + 12 * frac(c, 1))
+    for driven in sim.poles:
+        d += n * 16 * sum(driven)
+    return {"h": h, "d": d}
+
+
+def cpu_baseline(wl, seconds: float) -> dict:
+    """Time the C restatement (oracle) on the same workload for about ``seconds``."""
+    from meep_gpu.dft import FluxMonitor
+    from meep_gpu.pml import PML, normalize_faces
+    from meep_gpu.sources import current_table
+    from oracle import workload as ow
+
+    grid = wl.grid
+    faces = normalize_faces(wl.pml_faces, grid) if wl.pml_faces else ((0, 0),) * 3
+    steps_cap = 2000
+    deposits = []
+    for box, fn in wl.sources:
+        deposits.append((box.component, box.lo, box.cnt, box.amp_re, box.amp_im,
+                         current_table(fn, grid.dt, steps_cap, box.magnetic)))
+    planes = []
+    for p in wl.planes:
+        mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
+        planes.append((p.normal, p.lo, p.cnt, mon.omegas, mon.scale))
+    poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in wl.poles]
+    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, wl.inv, poles, deposits, planes,
+                            grid.invariant)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ow.run(sim, oplanes, 1, reference_precision=False)  # warm (page in, thread pool)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and steps < steps_cap - 2:
+        ow.run(sim, oplanes, 1, reference_precision=False)
+        steps += 1
+    elapsed = time.perf_counter() - t0
+    return {"value": round(grid.total_cells * steps / elapsed / 1e6, 2), "unit": "Mcell-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{wl.name}: {steps} steps of the C restatement (OpenMP, {threads} threads) "
+                      f"incl. the {len(wl.planes)} flux planes in float64, {elapsed:.1f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="sphere192")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="time a plain device copy of BYTES (profile counter calibration)")
+    args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        from meep_gpu.engine import copy_probe
+
+        ms = copy_probe(args.calibrate_copy, 10)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "mean_ms": ms,
+                          "GB/s": 2 * args.calibrate_copy / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not run here (the engine is a "
+              f"single-GPU engine; how a run would shard is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from meep_gpu import workloads
+    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.engine import device_info
+
+    from meep_gpu import autotune
+    from meep_gpu.pml import PML
+
+    wl = workloads.config(args.workload)
+    shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
+                          wl.poles, use_cache=False)
+    drv = FdtdDriver.from_workload(wl)
+    sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
+    print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
+    sim.run(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    total_ms, h_ms, d_ms = sim.run_timed(args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    cells = wl.grid.total_cells
+    value = cells * args.steps / elapsed / 1e6
+    nbytes = algorithmic_bytes(sim)
+    achieved = nbytes["d"] / (d_ms * 1e-3) / 1e9
+    spectra_ok = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
+    line = {
+        "metric": "Mcell-steps/s (plain single-simulation FDTD step, fused HIP kernels)",
+        "value": round(value, 1),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (fields), f64 (source deposit, flux transform)",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: {cells} cells {wl.grid.shape}, UPML 8, sphere r "
+                               f"{0.2 * wl.grid.shape[0]:.1f} cells eps 2.25, Ez Gaussian point "
+                               f"source, {len(wl.planes)} flux planes x "
+                               f"{wl.planes[0].frequencies.size if wl.planes else 0} freqs",
+                   "cells": cells, "steps_timed": args.steps},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "dpair_kernel (step_D+deposit+update_E+update_P)",
+                     "bytes_per_launch": int(nbytes["d"]), "mean_launch_ms": round(d_ms, 5),
+                     "h_pair": {"bytes_per_launch": int(nbytes["h"]),
+                                "mean_launch_ms": round(h_ms, 5),
+                                "achieved": round(nbytes["h"] / (h_ms * 1e-3) / 1e9, 1)},
+                     "step_model_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2)},
+        "spectra_finite": bool(spectra_ok),
+        "launch_shape": {"cells_per_thread": shape[0], "blocks_per_cu": shape[1],
+                         "xcd_remap": bool(shape[2])},
+    }
+    drv.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -129,21 +129,40 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
  * fastest over the plane box]. Flushes buffered samples first. */
 int fdtd_flux_read(fdtd_sim *sim, int32_t plane, double *out, size_t count);
 
-/* Batched form: many simulations advanced together, one persistent fused launch per
- * half-step over a work list of (simulation, tile) entries. The batch does not own the
- * simulations. */
+/* Batched form: many simulations advanced together, one fused launch per half-step over a
+ * work list of (simulation, tile) entries, and one sampling launch for every flux plane of
+ * every member. The batch does not own the simulations; while one exists its members are
+ * stepped only through it (their own step/upload/download calls return FDTD_ESTATE), and
+ * fdtd_batch_destroy hands their counters back. No counterpart in the reference (it steps
+ * one simulation per process, README.md:589-590). */
 int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out);
 void fdtd_batch_destroy(fdtd_batch *batch);
 int fdtd_batch_run(fdtd_batch *batch, int32_t nsteps);
+/* Run the float64 transform pass over every plane's buffered samples (before a read). */
+int fdtd_batch_flush(fdtd_batch *batch);
 int fdtd_batch_synchronize(fdtd_batch *batch);
 
-/* Launch shape knobs (the autotuner sets them): workgroups per CU for the persistent
- * launch and whether the tile order is remapped XCD-contiguous. */
-int fdtd_set_launch(int32_t blocks_per_cu, int32_t xcd_remap);
+/* Launch shape (the autotuner sets it; process-wide): vec = x-adjacent cells per thread (1, 2
+ * or 4); blocks_per_cu = 0 for one workgroup per tile, else a persistent grid of that many
+ * workgroups per CU; xcd_remap = 1 to give each XCD a contiguous eighth of the tile list;
+ * nt_stores = 1 to stream the fused pairs' outputs with non-temporal stores. No counterpart
+ * in the reference (its launches are one thread per cell). */
+int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores);
+int fdtd_get_launch(int32_t out[4]);
 
 /* Measure average kernel durations over `repeats` launches of each fused pair on the
  * simulation's stream with HIP events: out_ms[0] = H pair, out_ms[1] = D pair. */
 int fdtd_time_pairs(fdtd_sim *sim, int32_t repeats, float out_ms[2]);
+
+/* nsteps whole steps like fdtd_run, with HIP events around every fused-pair launch on the
+ * simulation's stream: out_ms[0] = total elapsed, [1] = mean H pair, [2] = mean D pair. */
+int fdtd_run_timed(fdtd_sim *sim, int32_t nsteps, float out_ms[3]);
+
+/* Diagnostic: mean duration of a device copy of `bytes` (the reference point that scales the
+ * HBM byte counters of a profile): mode 0 one float4 per thread per pass, 1 four loads in
+ * flight per thread, 2 the same with non-temporal stores, 3 read only, 4 write only (the time
+ * is then for `bytes` moved once, not twice). No counterpart in the reference. */
+int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms);
 
 #ifdef __cplusplus
 }

@@ -6,13 +6,16 @@
 // (built with -ffp-contract=off), subnormals kept (-fno-gpu-flush-denormals-to-zero). The
 // deposit and the off-diagonal row are formed in double and rounded once.
 //
-// Device layout: every volume x fastest, element (i,j,k) at (k*ny+j)*nx+i, each axis periodic
-// (an absorber is a material, reference stepping.py:2984). One `SimDev` descriptor per
-// simulation lives in device memory; kernels walk a list of 256-cell tiles, so a single run
-// and a batch of uneven grids run the same persistent kernels. The tile walk is
-// XCD-contiguous: workgroup b runs on XCD b%8, and each XCD sweeps its own contiguous eighth
-// of the tile list so that the neighbour rows and planes a tile reads were fetched into that
-// XCD's L2 by the tiles just before it.
+// Device layout: every volume x fastest, element (i,j,k) at (k*ny+j)*pitch+i with the row
+// pitch nx rounded up to a multiple of 4 floats, every axis periodic (an absorber is a
+// material, reference stepping.py:2984). One `SimDev` descriptor per simulation lives in device
+// memory. Kernels walk a list of tiles of kThreads*V padded cells (each thread owns V
+// x-adjacent cells of one row), so a single run and a batch of uneven grids run the same
+// kernels. The launch is one workgroup per tile or a persistent grid; with the XCD remap,
+// workgroup b (which the dispatcher places on XCD b%8) takes tiles from the b%8-th contiguous
+// eighth of the list, so the neighbour rows and planes a tile reads were fetched into that
+// XCD's own L2 by the tiles just before it. Step counters live on the device and one wave
+// advances them after each D pair, so a step sequence carries no host state.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -20,6 +23,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -28,9 +33,11 @@
 
 namespace {
 
-constexpr int kTile = 256;       // cells per tile = threads per workgroup
+constexpr int kThreads = 256;   // threads per workgroup
+constexpr int kPad = 4;         // row pitch multiple (floats)
 constexpr int kMaxPoles = FDTD_MAX_POLES;
-constexpr int kRing = 32;        // flux samples buffered before one float64 transform pass
+constexpr int kRing = 64;       // flux samples buffered before one float64 transform pass
+constexpr int kMaxRuns = 4;     // runs of a layer membership line held in the descriptor
 
 thread_local std::string g_error;
 
@@ -64,6 +71,7 @@ struct SrcDev {
 
 struct SimDev {
     int n[3];
+    int pitch;      // floats per stored row: nx rounded up to kPad
     int present[3];
     float dtdx;
     int pml;
@@ -73,22 +81,26 @@ struct SimDev {
     int step;       // completed steps (driver.py:3592)
     int hsep;       // MEEP's one-time H start done (stepping.py:2789)
     int ppar;       // which P buffer holds P^n
-    int ncell;
+    int ncell;      // nx*ny*nz
+    int npad;       // pitch*ny*nz, the allocated floats per volume
+    double dt;
     float *D[3], *B[3], *E[3], *H[3];
     float *fuB[3], *fuD[3], *fwE[3], *fwH[3];
     const float *inv[3];
     const float *u[3][3];
     const float4 *line[2][3];  // (kms, sinv, kps, member) per row, [half][axis]
+    int mr[2][3][2 * kMaxRuns];  // the membership lines as up to kMaxRuns [lo, hi) runs
     float pc[kMaxPoles][4];
     int pdriven[kMaxPoles][3];
     float *pbuf[kMaxPoles][3][2];
     const float *psig[kMaxPoles][3];
     float psig_s[kMaxPoles][3];
     const SrcDev *src;
+    int src_lo[2][3], src_hi[2][3];  // per field (D, B): union of the source boxes, [lo, hi)
 };
 
 // ------------------------------------------------------------------------------------------
-// Tile walk
+// Tile walk and the last-workgroup step advance
 // ------------------------------------------------------------------------------------------
 struct TileWalk {
     int next, stride, end;
@@ -97,13 +109,11 @@ struct TileWalk {
 __device__ inline TileWalk tile_walk(int ntiles, int xcd_remap) {
     const int G = gridDim.x, b = blockIdx.x;
     TileWalk w;
-    if (xcd_remap && (G % 8) == 0 && G >= 8) {
+    if (xcd_remap && (G % 8) == 0) {
         const int x = b % 8, s = b / 8, S = G / 8;
-        const int lo = (int)(((long long)ntiles * x) / 8);
-        const int hi = (int)(((long long)ntiles * (x + 1)) / 8);
-        w.next = lo + s;
+        w.next = (int)(((long long)ntiles * x) / 8) + s;
         w.stride = S;
-        w.end = hi;
+        w.end = (int)(((long long)ntiles * (x + 1)) / 8);
     } else {
         w.next = b;
         w.stride = G;
@@ -112,33 +122,111 @@ __device__ inline TileWalk tile_walk(int ntiles, int xcd_remap) {
     return w;
 }
 
-struct Cell {
-    int at, i, j, k;
-    int xp, yp, zp;  // offsets to the +1 neighbour along x, y, z (periodic)
-    int xm, ym, zm;  // offsets to the -1 neighbour
-};
-
-__device__ inline Cell locate(const SimDev &s, int at) {
-    Cell c;
-    const int nx = s.n[0], ny = s.n[1], nz = s.n[2];
-    const int jk = at / nx;
-    c.at = at;
-    c.i = at - jk * nx;
-    c.k = jk / ny;
-    c.j = jk - c.k * ny;
-    const int plane = nx * ny;
-    c.xp = (c.i + 1 < nx) ? 1 : -(nx - 1);
-    c.xm = (c.i > 0) ? -1 : (nx - 1);
-    c.yp = (c.j + 1 < ny) ? nx : -(ny - 1) * nx;
-    c.ym = (c.j > 0) ? -nx : (ny - 1) * nx;
-    c.zp = (c.k + 1 < nz) ? plane : -(nz - 1) * plane;
-    c.zm = (c.k > 0) ? -plane : (nz - 1) * plane;
-    return c;
+// Advance each simulation's step counter, H-start flag and P parity after its D pair (one
+// wave; kept out of the fused kernels so no workgroup of a launch sees a counter change).
+__global__ void advance_kernel(SimDev *sims, int count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    sims[q].step += 1;
+    sims[q].hsep = 1;
+    if (sims[q].npol) sims[q].ppar ^= 1;
 }
 
-__device__ inline int coord(const Cell &c, int axis) { return axis == 0 ? c.i : (axis == 1 ? c.j : c.k); }
-__device__ inline int plus(const Cell &c, int axis) { return axis == 0 ? c.xp : (axis == 1 ? c.yp : c.zp); }
-__device__ inline int minus(const Cell &c, int axis) { return axis == 0 ? c.xm : (axis == 1 ? c.ym : c.zm); }
+// V x-adjacent cells (i0..i0+V-1) of one stored row; cells with i >= nx are row padding,
+// computed on garbage and never read as a neighbour.
+struct Run {
+    int at;   // padded index of element 0
+    int row;  // padded index of the row's cell 0
+    int i0, j, k;
+    int yp, ym, zp, zm;  // offsets to the +-1 rows along y and z (periodic)
+};
+
+__device__ inline Run locate(const SimDev &s, int at) {
+    Run q;
+    const int pitch = s.pitch, ny = s.n[1], nz = s.n[2];
+    const int r = at / pitch;
+    q.at = at;
+    q.i0 = at - r * pitch;
+    q.row = r * pitch;
+    q.k = r / ny;
+    q.j = r - q.k * ny;
+    const int plane = pitch * ny;
+    q.yp = (q.j + 1 < ny) ? pitch : -(ny - 1) * pitch;
+    q.ym = (q.j > 0) ? -pitch : (ny - 1) * pitch;
+    q.zp = (q.k + 1 < nz) ? plane : -(nz - 1) * plane;
+    q.zm = (q.k > 0) ? -plane : (nz - 1) * plane;
+    return q;
+}
+
+template <int V>
+struct VF {
+    float v[V];
+};
+
+template <int V>
+__device__ inline VF<V> ldv(const float *p, int at) {
+    VF<V> r;
+    if constexpr (V == 4) {
+        const float4 t = *reinterpret_cast<const float4 *>(p + at);
+        r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+    } else if constexpr (V == 2) {
+        const float2 t = *reinterpret_cast<const float2 *>(p + at);
+        r.v[0] = t.x; r.v[1] = t.y;
+    } else {
+        r.v[0] = p[at];
+    }
+    return r;
+}
+
+// Store a run; with NT the store is marked non-temporal (streamed past the caches, for the
+// outputs no later read of the same launch wants).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <int V, bool NT = false>
+__device__ inline void stv(float *p, int at, const VF<V> &x) {
+    if constexpr (V == 4) {
+        const f4v v = {x.v[0], x.v[1], x.v[2], x.v[3]};
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(p + at));
+        else *reinterpret_cast<f4v *>(p + at) = v;
+    } else if constexpr (V == 2) {
+        const f2v v = {x.v[0], x.v[1]};
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f2v *>(p + at));
+        else *reinterpret_cast<f2v *>(p + at) = v;
+    } else {
+        if constexpr (NT) __builtin_nontemporal_store(x.v[0], p + at);
+        else p[at] = x.v[0];
+    }
+}
+
+// The +1 (x-up) neighbours: v[m+1] inside the run, the next run's first cell after it, and
+// the row's cell 0 past cell nx-1 (periodic wrap).
+template <int V>
+__device__ inline VF<V> x_up(const SimDev &s, const Run &q, const float *f, const VF<V> &v) {
+    const int nx = s.n[0];
+    VF<V> r;
+#pragma unroll
+    for (int m = 0; m + 1 < V; ++m) r.v[m] = v.v[m + 1];
+    r.v[V - 1] = (q.i0 + V < nx) ? f[q.at + V] : f[q.row];
+    if (q.i0 + V > nx && q.i0 < nx) {
+        const float w0 = f[q.row];
+        const int last = nx - 1 - q.i0;
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (m == last) r.v[m] = w0;
+    }
+    return r;
+}
+
+// The -1 (x-down) neighbours: v[m-1], the previous run's last cell, cell nx-1 before cell 0.
+template <int V>
+__device__ inline VF<V> x_down(const SimDev &s, const Run &q, const float *f, const VF<V> &v) {
+    VF<V> r;
+#pragma unroll
+    for (int m = 1; m < V; ++m) r.v[m] = v.v[m - 1];
+    r.v[0] = q.i0 > 0 ? f[q.at - 1] : f[q.row + s.n[0] - 1];
+    return r;
+}
 
 // Curl with MEEP's partner presence (stepping.py:2265): ((g1s-g1)+g2)-g2s, or the one-partner
 // forms dtdx*(g1s-g1) and (-dtdx)*(g2s-g2).
@@ -177,407 +265,542 @@ __device__ inline float curl_apply_pml(float f, float curl, float4 lsig, float4 
 
 __device__ inline bool plus_zero(double v) { return __double_as_longlong(v) == 0; }
 
-// Sources of one field/component deposited into value `f` at this cell (sources.py:1357).
-__device__ inline float deposit(const SimDev &s, int field, int comp, const Cell &c, float f) {
-    for (int q = 0; q < s.nsrc; ++q) {
-        const SrcDev &d = s.src[q];
+// Sources of one field/component deposited into the run's values (sources.py:1357,
+// host_writes.py:140): f = float(double(f) - ((amp_re*J_re - amp_im*J_im)*dt)), in list order.
+template <int V>
+__device__ inline void deposit_run(const SimDev &s, int field, int comp, const Run &q, VF<V> &x) {
+    // one test against the union of this field's boxes before any per-source work
+    if (q.j < s.src_lo[field][1] || q.j >= s.src_hi[field][1] || q.k < s.src_lo[field][2] ||
+        q.k >= s.src_hi[field][2] || q.i0 + V <= s.src_lo[field][0] || q.i0 >= s.src_hi[field][0])
+        return;
+    for (int n = 0; n < s.nsrc; ++n) {
+        const SrcDev &d = s.src[n];
         if (d.field != field || d.comp != comp) continue;
-        const int a = c.i - d.lo[0], b = c.j - d.lo[1], e = c.k - d.lo[2];
-        if (a < 0 || b < 0 || e < 0 || a >= d.cnt[0] || b >= d.cnt[1] || e >= d.cnt[2]) continue;
+        const int b = q.j - d.lo[1], e = q.k - d.lo[2];
+        if (b < 0 || e < 0 || b >= d.cnt[1] || e >= d.cnt[2]) continue;
+        if (q.i0 + V <= d.lo[0] || q.i0 >= d.lo[0] + d.cnt[0]) continue;
         if (s.step >= d.ncur) continue;
         const double cre = d.cur[2 * s.step], cim = d.cur[2 * s.step + 1];
         if (plus_zero(cre) && plus_zero(cim)) continue;
-        const int p = (e * d.cnt[1] + b) * d.cnt[0] + a;
-        double r = d.amp_re[p] * cre;
-        r = r - d.amp_im[p] * cim;
-        r = r * d.dt;
-        double w = (double)f;
-        w = w - r;
-        f = (float)w;
-    }
-    return f;
-}
-
-// ------------------------------------------------------------------------------------------
-// H pair: step_B -> magnetic deposit -> update_H, one cell, component C.
-// ------------------------------------------------------------------------------------------
-template <int C, bool PML>
-__device__ inline void hpair_component(const SimDev &s, const Cell &c, const float e[3],
-                                       const float en[3][3]) {
-    // en[comp][axis] = E[comp] at the +1 neighbour along axis
-    constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
-    constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
-    constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
-    const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first], en[first][fa],
-                               e[second], en[second][sa]);
-    float *B = s.B[C];
-    float b = B[c.at];
-    if (PML) {
-        const float4 lsig = s.line[1][dsig][coord(c, dsig)];
-        const float4 lsigu = s.line[1][dsigu][coord(c, dsigu)];
-        b = curl_apply_pml(b, curl, lsig, lsigu, s.fuB[C], c.at);
-    } else {
-        b = b - curl;
-    }
-    if (s.nsrc) b = deposit(s, 1, C, c, b);
-    B[c.at] = b;
-    if (PML) {
-        const float4 lw = s.line[0][C][coord(c, C)];
-        float *H = s.H[C];
-        if (lw.w != 0.0f) {
-            float *W = s.fwH[C];
-            float h, prev;
-            if (!s.hsep) {
-                h = b;
-                prev = b;
-            } else {
-                h = H[c.at];
-                prev = W[c.at];
-            }
-            W[c.at] = b;
-            const float a = lw.z * b;
-            const float m = lw.x * prev;
-            H[c.at] = h + (a - m);
-        } else {
-            H[c.at] = b;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const int a = q.i0 + m - d.lo[0];
+            if (a < 0 || a >= d.cnt[0]) continue;
+            const int p = (e * d.cnt[1] + b) * d.cnt[0] + a;
+            double r = d.amp_re[p] * cre;
+            r = r - d.amp_im[p] * cim;
+            r = r * d.dt;
+            double w = (double)x.v[m];
+            w = w - r;
+            x.v[m] = (float)w;
         }
     }
 }
 
-template <bool PML>
-__device__ inline void hpair_cell(const SimDev &s, const Cell &c) {
-    float e[3], en[3][3];
+__device__ inline int coord_of(const Run &q, int axis, int m) {
+    return axis == 0 ? q.i0 + m : (axis == 1 ? q.j : q.k);
+}
+
+// Does the one-rank chunk owning row c of `axis` (sub-lattice `half`) carry that axis's layer?
+// Pure comparisons against the runs in the descriptor: no memory access. Off the layer every
+// coefficient is exactly 1 (the region plan's precondition, chunk_map.py:1031), so the forms
+// there need no coefficient words at all.
+__device__ inline bool member(const SimDev &s, int half, int axis, int c) {
+    const int *r = s.mr[half][axis];
+    bool in = false;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) e[q] = s.E[q][c.at];
-    // neighbours each component's curl partners read (+1 along the partner's derivative axis)
-    en[0][1] = s.E[0][c.at + c.yp];
-    en[0][2] = s.E[0][c.at + c.zp];
-    en[1][2] = s.E[1][c.at + c.zp];
-    en[1][0] = s.E[1][c.at + c.xp];
-    en[2][0] = s.E[2][c.at + c.xp];
-    en[2][1] = s.E[2][c.at + c.yp];
-    en[0][0] = en[1][1] = en[2][2] = 0.0f;
-    hpair_component<0, PML>(s, c, e, en);
-    hpair_component<1, PML>(s, c, e, en);
-    hpair_component<2, PML>(s, c, e, en);
+    for (int t = 0; t < kMaxRuns; ++t) in |= (c >= r[2 * t]) & (c < r[2 * t + 1]);
+    return in;
 }
 
-__global__ void __launch_bounds__(kTile) hpair_kernel(const SimDev *__restrict__ sims,
-                                                      const int2 *__restrict__ tiles, int ntiles,
-                                                      int xcd_remap) {
-    const TileWalk w = tile_walk(ntiles, xcd_remap);
-    for (int t = w.next; t < w.end; t += w.stride) {
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kTile);
-        const SimDev &s = sims[tile.x];
-        const int at = tile.y + (int)threadIdx.x;
-        if (at >= s.ncell) continue;
-        const Cell c = locate(s, at);
-        if (s.pml)
-            hpair_cell<true>(s, c);
-        else
-            hpair_cell<false>(s, c);
-    }
+__device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool in) {
+    return in ? s.line[half][axis][c] : make_float4(1.f, 1.f, 1.f, 0.f);
 }
 
 // ------------------------------------------------------------------------------------------
-// D pair: step_D -> electric deposit -> update_E -> update_P, one cell.
+// H pair: step_B -> magnetic deposit -> update_H.
+// en[comp][axis]: E[comp] at the +1 neighbour along axis (only the pairs a curl reads).
 // ------------------------------------------------------------------------------------------
-template <int C, bool PML>
-__device__ inline float dcurl_component(const SimDev &s, const Cell &c, const float h[3],
-                                        const float hn[3][3]) {
+template <int V, int C, bool PML, bool NT>
+__device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V> (&e)[3],
+                                       const VF<V> (&en)[3][3]) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
     constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
-    const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first], hn[first][fa],
-                               h[second], hn[second][sa]);
-    float d = s.D[C][c.at];
-    if (PML) {
-        const float4 lsig = s.line[0][dsig][coord(c, dsig)];
-        const float4 lsigu = s.line[0][dsigu][coord(c, dsigu)];
-        d = curl_apply_pml(d, curl, lsig, lsigu, s.fuD[C], c.at);
-    } else {
-        d = d - curl;
+    VF<V> b = ldv<V>(s.B[C], q.at);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
+                                   en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
+        if (PML) {
+            const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
+            const bool in_s = member(s, 1, dsig, cs), in_u = member(s, 1, dsigu, cu);
+            if (in_s || in_u)
+                b.v[m] = curl_apply_pml(b.v[m], curl, coeff(s, 1, dsig, cs, in_s),
+                                        coeff(s, 1, dsigu, cu, in_u), s.fuB[C], q.at + m);
+            else
+                b.v[m] = b.v[m] - curl;
+        } else {
+            b.v[m] = b.v[m] - curl;
+        }
     }
-    if (s.nsrc) d = deposit(s, 0, C, c, d);
+    if (s.nsrc) deposit_run<V>(s, 1, C, q, b);
+    stv<V, NT>(s.B[C], q.at, b);
+    if (PML) {
+        // update_H: H accumulates on the own-axis layer's rows; everywhere else H is B's word
+        // (update_eh.cpp:162-170), so nothing is stored there and readers take B.
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const int cw = coord_of(q, C, m);
+            if (member(s, 0, C, cw)) {
+                const float4 lw = s.line[0][C][cw];
+                float hold = b.v[m], prev = b.v[m];
+                if (s.hsep) {
+                    hold = s.H[C][q.at + m];
+                    prev = s.fwH[C][q.at + m];
+                }
+                s.fwH[C][q.at + m] = b.v[m];
+                const float a = lw.z * b.v[m];
+                const float mm = lw.x * prev;
+                s.H[C][q.at + m] = hold + (a - mm);
+            }
+        }
+    }
+}
+
+template <int V, bool PML, bool NT>
+__device__ inline void hpair_run(const SimDev &s, const Run &q) {
+    VF<V> e[3], en[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[c] = ldv<V>(s.E[c], q.at);
+    en[0][1] = ldv<V>(s.E[0], q.at + q.yp);
+    en[0][2] = ldv<V>(s.E[0], q.at + q.zp);
+    en[1][2] = ldv<V>(s.E[1], q.at + q.zp);
+    en[1][0] = x_up<V>(s, q, s.E[1], e[1]);
+    en[2][0] = x_up<V>(s, q, s.E[2], e[2]);
+    en[2][1] = ldv<V>(s.E[2], q.at + q.yp);
+    hpair_component<V, 0, PML, NT>(s, q, e, en);
+    hpair_component<V, 1, PML, NT>(s, q, e, en);
+    hpair_component<V, 2, PML, NT>(s, q, e, en);
+}
+
+template <int V, bool NT>
+__global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restrict__ sims,
+                                                         const int2 *__restrict__ tiles,
+                                                         int ntiles, int xcd_remap) {
+    const TileWalk w = tile_walk(ntiles, xcd_remap);
+    for (int t = w.next; t < w.end; t += w.stride) {
+        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
+        const SimDev &s = sims[tile.x];
+        const int at = tile.y + V * (int)threadIdx.x;
+        if (at >= s.npad) continue;
+        const Run q = locate(s, at);
+        if (s.pml)
+            hpair_run<V, true, NT>(s, q);
+        else
+            hpair_run<V, false, NT>(s, q);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// D pair: step_D -> electric deposit -> update_E -> update_P.
+// ------------------------------------------------------------------------------------------
+template <int V, int C, bool PML>
+__device__ inline VF<V> dcurl_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
+                                        const VF<V> (&hn)[3][3]) {
+    constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
+    constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
+    constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
+    VF<V> d = ldv<V>(s.D[C], q.at);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
+                                   hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
+        if (PML) {
+            const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
+            const bool in_s = member(s, 0, dsig, cs), in_u = member(s, 0, dsigu, cu);
+            if (in_s || in_u)
+                d.v[m] = curl_apply_pml(d.v[m], curl, coeff(s, 0, dsig, cs, in_s),
+                                        coeff(s, 0, dsigu, cu, in_u), s.fuD[C], q.at + m);
+            else
+                d.v[m] = d.v[m] - curl;
+        } else {
+            d.v[m] = d.v[m] - curl;
+        }
+    }
+    if (s.nsrc) deposit_run<V>(s, 0, C, q, d);
     return d;
 }
 
-// E from the constitutive product `cst`, then every pole's P update for component C.
-template <int C, bool PML>
-__device__ inline void e_and_p(const SimDev &s, const Cell &c, float cst, const float p[kMaxPoles]) {
-    float *E = s.E[C];
-    float w;
+// E from the constitutive product `cst`, then every pole's P update for component C. The
+// drive W is f_w on the own-axis layer's cells and E elsewhere: the constitutive product both.
+template <int V, int C, bool PML, bool NT>
+__device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
+                               const VF<V> (&p)[kMaxPoles]) {
+    VF<V> e;
     if (PML) {
-        const float4 lw = s.line[1][C][coord(c, C)];
-        if (lw.w != 0.0f) {
-            float *W = s.fwE[C];
-            const float prev = W[c.at];
-            W[c.at] = cst;
-            const float a = lw.z * cst;
-            const float m = lw.x * prev;
-            E[c.at] = E[c.at] + (a - m);
-            w = cst;
-        } else {
-            E[c.at] = cst;
-            w = cst;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const int cw = coord_of(q, C, m);
+            float ev = cst.v[m];
+            if (member(s, 1, C, cw)) {
+                const float4 lw = s.line[1][C][cw];
+                const float prev = s.fwE[C][q.at + m];
+                s.fwE[C][q.at + m] = cst.v[m];
+                const float a = lw.z * cst.v[m];
+                const float mm = lw.x * prev;
+                ev = s.E[C][q.at + m] + (a - mm);
+            }
+            e.v[m] = ev;
         }
     } else {
-        E[c.at] = cst;
-        w = cst;
+        e = cst;
     }
+    stv<V, NT>(s.E[C], q.at, e);
     const int par = s.ppar;
-    for (int q = 0; q < s.npol; ++q) {
-        if (!s.pdriven[q][C]) continue;
-        float *out = s.pbuf[q][C][par ^ 1];
-        const float pp = out[c.at];
-        const float sg = s.psig[q][C] ? s.psig[q][C][c.at] : s.psig_s[q][C];
-        float t = s.pc[q][0] * p[q];
-        t = t - s.pc[q][1] * pp;
-        float dr = sg * w;
-        dr = s.pc[q][2] * dr;
-        t = t + dr;
-        out[c.at] = s.pc[q][3] * t;
-    }
-}
-
-template <int C, bool PML>
-__device__ inline void dpair_component(const SimDev &s, const Cell &c, const float h[3],
-                                       const float hn[3][3]) {
-    const float d = dcurl_component<C, PML>(s, c, h, hn);
-    s.D[C][c.at] = d;
-    float p[kMaxPoles];
-    float dmp = d;
-    const int par = s.ppar;
-    for (int q = 0; q < s.npol; ++q) {
-        p[q] = 0.0f;
-        if (!s.pdriven[q][C]) continue;
-        p[q] = s.pbuf[q][C][par][c.at];
-        dmp = dmp - p[q];
-    }
-    const float cst = s.inv[C][c.at] * dmp;
-    e_and_p<C, PML>(s, c, cst, p);
-}
-
-template <bool PML>
-__device__ inline void load_h(const SimDev &s, const Cell &c, float h[3], float hn[3][3]) {
-    float *const *Hs = PML ? s.H : s.B;  // without an absorber H is B (fields.py:1380)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) h[q] = Hs[q][c.at];
-    // MEEP's negated strides: the D curl reads its partners one cell DOWN
-    hn[0][1] = Hs[0][c.at + c.ym];
-    hn[0][2] = Hs[0][c.at + c.zm];
-    hn[1][2] = Hs[1][c.at + c.zm];
-    hn[1][0] = Hs[1][c.at + c.xm];
-    hn[2][0] = Hs[2][c.at + c.xm];
-    hn[2][1] = Hs[2][c.at + c.ym];
-    hn[0][0] = hn[1][1] = hn[2][2] = 0.0f;
+    for (int pq = 0; pq < kMaxPoles; ++pq) {
+        if (pq >= s.npol) break;
+        if (!s.pdriven[pq][C]) continue;
+        float *out = s.pbuf[pq][C][par ^ 1];
+        const VF<V> pp = ldv<V>(out, q.at);
+        const float *sv = s.psig[pq][C];
+        VF<V> sg;
+        if (sv) {
+            sg = ldv<V>(sv, q.at);
+        } else {
+#pragma unroll
+            for (int m = 0; m < V; ++m) sg.v[m] = s.psig_s[pq][C];
+        }
+        VF<V> r;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            float t = s.pc[pq][0] * p[pq].v[m];
+            t = t - s.pc[pq][1] * pp.v[m];
+            float dr = sg.v[m] * cst.v[m];
+            dr = s.pc[pq][2] * dr;
+            t = t + dr;
+            r.v[m] = s.pc[pq][3] * t;
+        }
+        stv<V>(out, q.at, r);
+    }
 }
 
-template <bool PML>
-__device__ inline void dpair_cell(const SimDev &s, const Cell &c) {
-    float h[3], hn[3][3];
-    load_h<PML>(s, c, h, hn);
-    dpair_component<0, PML>(s, c, h, hn);
-    dpair_component<1, PML>(s, c, h, hn);
-    dpair_component<2, PML>(s, c, h, hn);
+// D - sum P for component C from the D values in `d`, loading each driving pole's P^n.
+template <int V, int C>
+__device__ inline VF<V> minus_poles(const SimDev &s, const Run &q, const VF<V> &d,
+                                    VF<V> (&p)[kMaxPoles]) {
+    VF<V> dmp = d;
+    const int par = s.ppar;
+#pragma unroll
+    for (int pq = 0; pq < kMaxPoles; ++pq) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) p[pq].v[m] = 0.0f;
+        if (pq >= s.npol) continue;
+        if (!s.pdriven[pq][C]) continue;
+        p[pq] = ldv<V>(s.pbuf[pq][C][par], q.at);
+#pragma unroll
+        for (int m = 0; m < V; ++m) dmp.v[m] = dmp.v[m] - p[pq].v[m];
+    }
+    return dmp;
 }
 
-__global__ void __launch_bounds__(kTile) dpair_kernel(const SimDev *__restrict__ sims,
-                                                      const int2 *__restrict__ tiles, int ntiles,
-                                                      int xcd_remap) {
+template <int V, int C, bool PML, bool NT>
+__device__ inline void dpair_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
+                                       const VF<V> (&hn)[3][3]) {
+    const VF<V> d = dcurl_component<V, C, PML>(s, q, h, hn);
+    stv<V, NT>(s.D[C], q.at, d);
+    VF<V> p[kMaxPoles];
+    const VF<V> dmp = minus_poles<V, C>(s, q, d, p);
+    const VF<V> iv = ldv<V>(s.inv[C], q.at);
+    VF<V> cst;
+#pragma unroll
+    for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
+    e_and_p<V, C, PML, NT>(s, q, cst, p);
+}
+
+// H component c at the run shifted by `off` (a shift across c's own axis): the stored H on
+// c's own-axis layer rows, B elsewhere. Only Hx's rows vary along a run.
+template <int V>
+__device__ inline VF<V> ld_hb(const SimDev &s, const Run &q, int c, int off) {
+    if (c != 0 || V == 1) {
+        const bool in = member(s, 0, c, coord_of(q, c, 0));
+        return ldv<V>(in ? s.H[c] : s.B[c], q.at + off);
+    }
+    VF<V> r = ldv<V>(s.B[0], q.at + off);
+#pragma unroll
+    for (int m = 0; m < V; ++m)
+        if (member(s, 0, 0, q.i0 + m)) r.v[m] = s.H[0][q.at + off + m];
+    return r;
+}
+
+template <int V, bool PML>
+__device__ inline void load_h(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V> (&hn)[3][3]) {
+    if (PML) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h[c] = ld_hb<V>(s, q, c, 0);
+        // MEEP's negated strides: the D curl reads its partners one cell DOWN
+        hn[0][1] = ld_hb<V>(s, q, 0, q.ym);
+        hn[0][2] = ld_hb<V>(s, q, 0, q.zm);
+        hn[1][2] = ld_hb<V>(s, q, 1, q.zm);
+        hn[2][1] = ld_hb<V>(s, q, 2, q.ym);
+        const float *hy = member(s, 0, 1, q.j) ? s.H[1] : s.B[1];
+        const float *hz = member(s, 0, 2, q.k) ? s.H[2] : s.B[2];
+        hn[1][0] = x_down<V>(s, q, hy, h[1]);
+        hn[2][0] = x_down<V>(s, q, hz, h[2]);
+        return;
+    }
+    float *const *Hs = s.B;  // without an absorber H is B (fields.py:1380)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) h[c] = ldv<V>(Hs[c], q.at);
+    hn[0][1] = ldv<V>(Hs[0], q.at + q.ym);
+    hn[0][2] = ldv<V>(Hs[0], q.at + q.zm);
+    hn[1][2] = ldv<V>(Hs[1], q.at + q.zm);
+    hn[1][0] = x_down<V>(s, q, Hs[1], h[1]);
+    hn[2][0] = x_down<V>(s, q, Hs[2], h[2]);
+    hn[2][1] = ldv<V>(Hs[2], q.at + q.ym);
+}
+
+template <int V, bool PML, bool NT>
+__device__ inline void dpair_run(const SimDev &s, const Run &q) {
+    VF<V> h[3], hn[3][3];
+    load_h<V, PML>(s, q, h, hn);
+    dpair_component<V, 0, PML, NT>(s, q, h, hn);
+    dpair_component<V, 1, PML, NT>(s, q, h, hn);
+    dpair_component<V, 2, PML, NT>(s, q, h, hn);
+}
+
+template <int V, bool NT>
+__global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
+                                                         const int2 *__restrict__ tiles,
+                                                         int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int t = w.next; t < w.end; t += w.stride) {
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kTile);
+        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (s.offdiag) continue;  // stepped by the two-kernel row path below
-        const int at = tile.y + (int)threadIdx.x;
-        if (at >= s.ncell) continue;
-        const Cell c = locate(s, at);
+        const int at = tile.y + V * (int)threadIdx.x;
+        if (at >= s.npad) continue;
+        const Run q = locate(s, at);
         if (s.pml)
-            dpair_cell<true>(s, c);
+            dpair_run<V, true, NT>(s, q);
         else
-            dpair_cell<false>(s, c);
+            dpair_run<V, false, NT>(s, q);
     }
 }
 
 // Off-diagonal chi1inv: the row reads the partners' D - sum P at neighbouring cells, so the
 // D curl (with its deposit) completes over the whole grid first, then E and P.
-template <bool PML>
-__device__ inline void dcurl_cell(const SimDev &s, const Cell &c) {
-    float h[3], hn[3][3];
-    load_h<PML>(s, c, h, hn);
-    s.D[0][c.at] = dcurl_component<0, PML>(s, c, h, hn);
-    s.D[1][c.at] = dcurl_component<1, PML>(s, c, h, hn);
-    s.D[2][c.at] = dcurl_component<2, PML>(s, c, h, hn);
+template <int V, bool PML>
+__device__ inline void dcurl_run(const SimDev &s, const Run &q) {
+    VF<V> h[3], hn[3][3];
+    load_h<V, PML>(s, q, h, hn);
+    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, q, h, hn));
+    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, q, h, hn));
+    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, q, h, hn));
 }
 
-__global__ void __launch_bounds__(kTile) dcurl_kernel(const SimDev *__restrict__ sims,
-                                                      const int2 *__restrict__ tiles, int ntiles,
-                                                      int xcd_remap) {
+template <int V>
+__global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restrict__ sims,
+                                                         const int2 *__restrict__ tiles,
+                                                         int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int t = w.next; t < w.end; t += w.stride) {
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kTile);
+        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
-        const int at = tile.y + (int)threadIdx.x;
-        if (at >= s.ncell) continue;
-        const Cell c = locate(s, at);
+        const int at = tile.y + V * (int)threadIdx.x;
+        if (at >= s.npad) continue;
+        const Run q = locate(s, at);
         if (s.pml)
-            dcurl_cell<true>(s, c);
+            dcurl_run<V, true>(s, q);
         else
-            dcurl_cell<false>(s, c);
+            dcurl_run<V, false>(s, q);
     }
 }
 
 __device__ inline float dmp_at(const SimDev &s, int comp, int at) {
     float v = s.D[comp][at];
-    for (int q = 0; q < s.npol; ++q)
-        if (s.pdriven[q][comp]) v = v - s.pbuf[q][comp][s.ppar][at];
+#pragma unroll
+    for (int pq = 0; pq < kMaxPoles; ++pq) {
+        if (pq >= s.npol) break;
+        if (s.pdriven[pq][comp]) v = v - s.pbuf[pq][comp][s.ppar][at];
+    }
     return v;
 }
 
-// The row of stepping._offdiagonal_terms (stepping.py:1594-1687), summed in double.
-template <int C, bool PML>
-__device__ inline void erow_component(const SimDev &s, const Cell &c) {
-    float p[kMaxPoles];
-    float dmp = s.D[C][c.at];
-    for (int q = 0; q < s.npol; ++q) {
-        p[q] = 0.0f;
-        if (!s.pdriven[q][C]) continue;
-        p[q] = s.pbuf[q][C][s.ppar][c.at];
-        dmp = dmp - p[q];
+// Padded-index offset of the +1 / -1 neighbour of cell (i, j, k) along `axis` (periodic).
+__device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, int dir) {
+    if (axis == 0) {
+        if (dir > 0) return (i + 1 < s.n[0]) ? 1 : -(s.n[0] - 1);
+        return (i > 0) ? -1 : (s.n[0] - 1);
     }
-    float cst;
+    if (axis == 1) return dir > 0 ? q.yp : q.ym;
+    return dir > 0 ? q.zp : q.zm;
+}
+
+// The row of stepping._offdiagonal_terms (stepping.py:1594-1687), summed in double.
+template <int V, int C, bool PML>
+__device__ inline void erow_component(const SimDev &s, const Run &q) {
+    VF<V> p[kMaxPoles];
+    const VF<V> dmp = minus_poles<V, C>(s, q, ldv<V>(s.D[C], q.at), p);
+    const VF<V> iv = ldv<V>(s.inv[C], q.at);
+    VF<V> cst;
     const float *u1 = s.u[C][(C + 1) % 3], *u2 = s.u[C][(C + 2) % 3];
-    if (!u1 && !u2) {
-        cst = s.inv[C][c.at] * dmp;
-    } else {
-        double row = (double)(dmp * s.inv[C][c.at]);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const int at = q.at + m, i = q.i0 + m;
+        if (i >= s.n[0] || (!u1 && !u2)) {
+            cst.v[m] = iv.v[m] * dmp.v[m];
+            continue;
+        }
+        double row = (double)(dmp.v[m] * iv.v[m]);
 #pragma unroll
         for (int o = 1; o <= 2; ++o) {
             const int pa = (C + o) % 3;
             const float *U = o == 1 ? u1 : u2;
             if (!U) continue;
-            // u[q]*(g[q] + g[q - s_partner]) at q = here and q = here + s_own; a shift along
+            // u[r]*(g[r] + g[r - s_partner]) at r = here and r = here + s_own; a shift along
             // the own axis leaves the coordinate along the partner's axis unchanged
-            const int up = c.at + plus(c, C);
-            const int down = minus(c, pa);
-            const float pair0 = dmp_at(s, pa, c.at) + dmp_at(s, pa, c.at + down);
-            const float pair1 = dmp_at(s, pa, up) + dmp_at(s, pa, up + down);
-            const float prod0 = U[c.at] * pair0;
+            const int up = at + step_off(s, q, i, C, +1);
+            const int iu = (C == 0) ? ((i + 1 < s.n[0]) ? i + 1 : 0) : i;
+            const int down0 = step_off(s, q, i, pa, -1);
+            const int down1 = step_off(s, q, iu, pa, -1);
+            const float pair0 = dmp_at(s, pa, at) + dmp_at(s, pa, at + down0);
+            const float pair1 = dmp_at(s, pa, up) + dmp_at(s, pa, up + down1);
+            const float prod0 = U[at] * pair0;
             const float prod1 = U[up] * pair1;
             const float summed = prod0 + prod1;
             row = row + 0.25 * (double)summed;
         }
-        cst = (float)row;
+        cst.v[m] = (float)row;
     }
-    e_and_p<C, PML>(s, c, cst, p);
+    e_and_p<V, C, PML, false>(s, q, cst, p);
 }
 
-__global__ void __launch_bounds__(kTile) erow_kernel(const SimDev *__restrict__ sims,
-                                                     const int2 *__restrict__ tiles, int ntiles,
-                                                     int xcd_remap) {
+template <int V>
+__global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict__ sims,
+                                                        const int2 *__restrict__ tiles,
+                                                        int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int t = w.next; t < w.end; t += w.stride) {
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kTile);
+        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
-        const int at = tile.y + (int)threadIdx.x;
-        if (at >= s.ncell) continue;
-        const Cell c = locate(s, at);
+        const int at = tile.y + V * (int)threadIdx.x;
+        if (at >= s.npad) continue;
+        const Run q = locate(s, at);
         if (s.pml) {
-            erow_component<0, true>(s, c);
-            erow_component<1, true>(s, c);
-            erow_component<2, true>(s, c);
+            erow_component<V, 0, true>(s, q);
+            erow_component<V, 1, true>(s, q);
+            erow_component<V, 2, true>(s, q);
         } else {
-            erow_component<0, false>(s, c);
-            erow_component<1, false>(s, c);
-            erow_component<2, false>(s, c);
+            erow_component<V, 0, false>(s, q);
+            erow_component<V, 1, false>(s, q);
+            erow_component<V, 2, false>(s, q);
         }
     }
 }
 
 // E = chi1inv * (D - sum P): the reference's on-demand get_E for runs that do not store E.
-__global__ void prime_e_kernel(SimDev *sims, int ntiles) {
-    const int t = blockIdx.x;
-    if (t >= ntiles) return;
+__global__ void prime_e_kernel(SimDev *sims, int count) {
     const SimDev &s = sims[0];
-    const int at = t * kTile + (int)threadIdx.x;
-    if (at >= s.ncell) return;
-    for (int q = 0; q < 3; ++q) s.E[q][at] = s.inv[q][at] * dmp_at(s, q, at);
-}
-
-// Advance each simulation's step counter, H-start flag and P parity after its D pair.
-__global__ void advance_kernel(SimDev *sims, int count) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= count) return;
-    sims[q].step += 1;
-    sims[q].hsep = 1;
-    if (sims[q].npol) sims[q].ppar ^= 1;
+    const int at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= count) return;
+    for (int c = 0; c < 3; ++c) s.E[c][at] = s.inv[c][at] * dmp_at(s, c, at);
 }
 
 // ------------------------------------------------------------------------------------------
 // Flux planes: cell-centred tangential samples into a ring, then a float64 transform pass.
 // ------------------------------------------------------------------------------------------
 struct PlaneDev {
+    const SimDev *sim;  // the descriptor the plane's simulation is stepped through
     int normal;
     int lo[3], cnt[3];
     int cells;
+    int row;            // ring row stride: cells rounded up to kCB
     int nf;
     int dec;
+    int first_step;     // sampling counts steps after this one
     double scale;
-    double dt;
     const double *omega;
-    float *ring;     // [kRing][4][cells]
-    double *times;   // [kRing] electric sample time t (magnetic is t - dt/2)
-    double *phase;   // [kRing][2][nf] (re, im) scratch of one transform pass
-    double *acc;     // [4][nf][cells] (re, im)
+    float *ring;        // [kRing][4][row]
+    double *times;      // [kRing] electric sample time t (magnetic is t - dt/2)
+    double *phase;      // [kRing][2][nf] (re, im)
+    double *acc;        // [4][nf][cells] (re, im)
 };
 
-// MEEP yee2cent over the box (dft.py:352): average along every axis where the component's
-// Yee shift is 0, x first, then y, then z, each 0.5f*(a+b) in float32.
-__device__ inline float centred(const SimDev &s, const float *f, int comp_shift_mask, int i, int j,
-                                int k) {
-    const int nx = s.n[0], ny = s.n[1], nz = s.n[2];
-    const int ex = !(comp_shift_mask & 1), ey = !(comp_shift_mask & 2), ez = !(comp_shift_mask & 4);
-    const int i1 = (i + 1) % nx, j1 = (j + 1) % ny, k1 = (k + 1) % nz;
-    float v[2][2][2];
-    for (int a = 0; a <= ex; ++a)
-        for (int b = 0; b <= ey; ++b)
-            for (int c = 0; c <= ez; ++c) {
-                const int ii = a ? i1 : i, jj = b ? j1 : j, kk = c ? k1 : k;
-                v[a][b][c] = f[(kk * ny + jj) * nx + ii];
-            }
-    if (ex)
-        for (int b = 0; b <= ey; ++b)
-            for (int c = 0; c <= ez; ++c) v[0][b][c] = 0.5f * (v[0][b][c] + v[1][b][c]);
-    if (ey)
-        for (int c = 0; c <= ez; ++c) v[0][0][c] = 0.5f * (v[0][0][c] + v[0][1][c]);
-    if (ez) v[0][0][0] = 0.5f * (v[0][0][0] + v[0][0][1]);
-    return v[0][0][0];
+// MEEP yee2cent (dft.py:352): average along every axis where the component's Yee shift is 0,
+// x first, then y, then z, each 0.5f*(a+b) in float32. `mask` bit a = shift 1 on axis a.
+__device__ inline float centred(const SimDev &s, const float *f, int mask, int i, int j, int k) {
+    const int ex = !(mask & 1), ey = !(mask & 2), ez = !(mask & 4);
+    const int i1 = (i + 1 < s.n[0]) ? i + 1 : 0;
+    const int j1 = (j + 1 < s.n[1]) ? j + 1 : 0;
+    const int k1 = (k + 1 < s.n[2]) ? k + 1 : 0;
+    const int py = s.pitch, pz = s.pitch * s.n[1];
+    auto at = [&](int a, int b, int c) { return f[c * pz + b * py + a]; };
+    float v000 = at(i, j, k), v100 = 0.f, v010 = 0.f, v110 = 0.f;
+    float v001 = 0.f, v101 = 0.f, v011 = 0.f, v111 = 0.f;
+    if (ex) v100 = at(i1, j, k);
+    if (ey) v010 = at(i, j1, k);
+    if (ex && ey) v110 = at(i1, j1, k);
+    if (ez) v001 = at(i, j, k1);
+    if (ex && ez) v101 = at(i1, j, k1);
+    if (ey && ez) v011 = at(i, j1, k1);
+    if (ex && ey && ez) v111 = at(i1, j1, k1);
+    if (ex) {
+        v000 = 0.5f * (v000 + v100);
+        v010 = 0.5f * (v010 + v110);
+        v001 = 0.5f * (v001 + v101);
+        v011 = 0.5f * (v011 + v111);
+    }
+    if (ey) {
+        v000 = 0.5f * (v000 + v010);
+        v001 = 0.5f * (v001 + v011);
+    }
+    if (ez) v000 = 0.5f * (v000 + v001);
+    return v000;
 }
 
-// Yee shift bit masks (bit a set = shift 1 along axis a), fields.py:218-223.
-__device__ __host__ inline int e_shift(int c) { return 1 << c; }
-__device__ __host__ inline int h_shift(int c) { return 7 & ~(1 << c); }
+// The H component c at cell (i, j, k): stored H on c's own-axis layer rows, B elsewhere.
+__device__ inline float h_at(const SimDev &s, int c, int i, int j, int k) {
+    const int idx = (k * s.n[1] + j) * s.pitch + i;
+    if (!s.pml) return s.B[c][idx];
+    const int cc = c == 0 ? i : (c == 1 ? j : k);
+    return member(s, 0, c, cc) ? s.H[c][idx] : s.B[c][idx];
+}
 
-__global__ void flux_sample_kernel(const SimDev *sims, PlaneDev *planes, int nplanes, int slot) {
-    const PlaneDev &p = planes[blockIdx.y];
-    const SimDev &s = sims[0];
-    if (s.step % p.dec) return;
+// yee2cent for an H component: its Yee shift is 0 only along its own axis c, so the average
+// is of the cell and its +1 neighbour along c (which may sit on the other side of the layer).
+__device__ inline float centred_h(const SimDev &s, int c, int i, int j, int k) {
+    int i1 = i, j1 = j, k1 = k;
+    if (c == 0) i1 = (i + 1 < s.n[0]) ? i + 1 : 0;
+    else if (c == 1) j1 = (j + 1 < s.n[1]) ? j + 1 : 0;
+    else k1 = (k + 1 < s.n[2]) ? k + 1 : 0;
+    return 0.5f * (h_at(s, c, i, j, k) + h_at(s, c, i1, j1, k1));
+}
+
+// One launch samples every plane of a simulation (or of a batch) that accumulates this step.
+__global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes) {
+    const int pi = blockIdx.y;
+    if (pi >= nplanes) return;
+    const PlaneDev &p = planes[pi];
+    const SimDev &s = *p.sim;
+    const int since = s.step - p.first_step;
+    if (since <= 0 || since % p.dec) return;
+    const int slot = (since / p.dec - 1) % kRing;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.times[slot] = (double)s.step * p.dt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.times[slot] = (double)s.step * s.dt;
     if (q >= p.cells) return;
     const int a = q % p.cnt[0], r = q / p.cnt[0];
     const int b = r % p.cnt[1], e = r / p.cnt[1];
     const int i = (p.lo[0] + a) % s.n[0], j = (p.lo[1] + b) % s.n[1], k = (p.lo[2] + e) % s.n[2];
     const int t1 = (p.normal + 1) % 3, t2 = (p.normal + 2) % 3;
-    float *out = p.ring + (size_t)slot * 4 * p.cells;
-    float *const *Hs = s.pml ? s.H : s.B;
-    out[0 * p.cells + q] = centred(s, s.E[t1], e_shift(t1), i, j, k);
-    out[1 * p.cells + q] = centred(s, s.E[t2], e_shift(t2), i, j, k);
-    out[2 * p.cells + q] = centred(s, Hs[t1], h_shift(t1), i, j, k);
-    out[3 * p.cells + q] = centred(s, Hs[t2], h_shift(t2), i, j, k);
+    float *out = p.ring + (size_t)slot * 4 * p.row;
+    out[0 * p.row + q] = centred(s, s.E[t1], 1 << t1, i, j, k);
+    out[1 * p.row + q] = centred(s, s.E[t2], 1 << t2, i, j, k);
+    out[2 * p.row + q] = centred_h(s, t1, i, j, k);
+    out[3 * p.row + q] = centred_h(s, t2, i, j, k);
 }
 
 // exp(i*omega*t)*scale in double for each buffered sample time, E at t and H at t - dt/2.
@@ -589,7 +812,7 @@ __global__ void flux_phase_kernel(PlaneDev *planes, int nslots) {
     const double t = p.times[m];
     double sE, cE, sH, cH;
     sincos(p.omega[f] * t, &sE, &cE);
-    sincos(p.omega[f] * (t - 0.5 * p.dt), &sH, &cH);
+    sincos(p.omega[f] * (t - 0.5 * p.sim->dt), &sH, &cH);
     double *ph = p.phase + (size_t)m * 4 * p.nf;
     ph[2 * f] = cE * p.scale;
     ph[2 * f + 1] = sE * p.scale;
@@ -597,24 +820,124 @@ __global__ void flux_phase_kernel(PlaneDev *planes, int nslots) {
     ph[2 * p.nf + 2 * f + 1] = sH * p.scale;
 }
 
-// acc[comp][f][cell] += sum_m phase[m][E|H][f] * ring[m][comp][cell], in float64.
-__global__ void __launch_bounds__(256) flux_accumulate_kernel(PlaneDev *planes, int nslots) {
-    PlaneDev &p = planes[blockIdx.z];
-    const int comp = blockIdx.y / p.nf, f = blockIdx.y % p.nf;
-    if (comp >= 4) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p.cells) return;
+// acc[comp][f][cell] += sum_m phase[m][E|H][f] * ring[m][comp][cell], in float64: per plane and
+// component a (2*nf x nslots) by (nslots x cells) product. Each thread owns kCB consecutive
+// cells and kFB frequencies of accumulators; the workgroup stages the phase words of its
+// frequencies in LDS (every lane reads them as broadcasts) and the frequency range is split
+// over blockIdx.z so that small planes still fill the machine.
+constexpr int kCB = 4;   // cells per thread
+constexpr int kFB = 8;   // frequencies per register block
+constexpr int kFZ = 16;  // frequencies per workgroup
+
+__global__ void __launch_bounds__(256) flux_accumulate_kernel(
+    const float *__restrict__ ring, const double *__restrict__ phase, double *__restrict__ acc,
+    int cells, int row, int nf, int nslots) {
+    __shared__ double2 ph_s[kRing][kFZ];
+    const int comp = blockIdx.y;
     const int side = comp < 2 ? 0 : 1;
-    double re = 0.0, im = 0.0;
-    for (int m = 0; m < nslots; ++m) {
-        const double v = (double)p.ring[((size_t)m * 4 + comp) * p.cells + q];
-        const double *ph = p.phase + (size_t)m * 4 * p.nf + side * 2 * p.nf + 2 * f;
-        re += ph[0] * v;
-        im += ph[1] * v;
+    const int fz0 = blockIdx.z * kFZ;
+    const int fzn = nf - fz0 < kFZ ? nf - fz0 : kFZ;
+    for (int x = threadIdx.x; x < kRing * kFZ; x += blockDim.x) {
+        const int m = x / kFZ, fi = x % kFZ;
+        double2 v = make_double2(0.0, 0.0);
+        if (m < nslots && fi < fzn) {
+            const double *ph = phase + (size_t)m * 4 * nf + side * 2 * nf + 2 * (fz0 + fi);
+            v = make_double2(ph[0], ph[1]);
+        }
+        ph_s[m][fi] = v;
     }
-    double *a = p.acc + (((size_t)comp * p.nf + f) * p.cells + q) * 2;
-    a[0] += re;
-    a[1] += im;
+    __syncthreads();
+    const int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * kCB;
+    if (q0 >= cells) return;
+    double *a = acc + (size_t)comp * nf * cells * 2;
+    for (int f0 = 0; f0 < fzn; f0 += kFB) {
+        double re[kFB][kCB], im[kFB][kCB];
+#pragma unroll
+        for (int fi = 0; fi < kFB; ++fi)
+#pragma unroll
+            for (int c = 0; c < kCB; ++c) re[fi][c] = im[fi][c] = 0.0;
+        for (int m = 0; m < nslots; ++m) {
+            const float4 sv = *reinterpret_cast<const float4 *>(ring + ((size_t)m * 4 + comp) * row + q0);
+            const double sm[kCB] = {(double)sv.x, (double)sv.y, (double)sv.z, (double)sv.w};
+#pragma unroll
+            for (int fi = 0; fi < kFB; ++fi) {
+                const double2 w = ph_s[m][f0 + fi];
+#pragma unroll
+                for (int c = 0; c < kCB; ++c) {
+                    re[fi][c] = fma(w.x, sm[c], re[fi][c]);  // float64 by design: fused is closer
+                    im[fi][c] = fma(w.y, sm[c], im[fi][c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int fi = 0; fi < kFB; ++fi) {
+            if (f0 + fi >= fzn) break;
+            double2 *cell = reinterpret_cast<double2 *>(a + ((size_t)(fz0 + f0 + fi) * cells + q0) * 2);
+#pragma unroll
+            for (int c = 0; c < kCB; ++c) {
+                if (q0 + c >= cells) break;
+                double2 old = cell[c];
+                old.x += re[fi][c];
+                old.y += im[fi][c];
+                cell[c] = old;
+            }
+        }
+    }
+}
+
+__global__ void fill_kernel(float *p, size_t n, float v) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+__global__ void copy_probe_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                  size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+
+// Read-only and write-only streams (the two halves of a copy, measured apart).
+__global__ void read_probe_kernel(const float4 *__restrict__ src, float *__restrict__ out, size_t n) {
+    float acc = 0.f;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = src[q];
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    if (acc == 12345.678f) out[blockIdx.x] = acc;  // keeps the loads; never true for zeros
+}
+
+__global__ void write_probe_kernel(float4 *__restrict__ dst, size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = make_float4(0.f, 1.f, 2.f, 3.f);
+}
+
+// Four independent loads in flight per thread before the stores; `nt` marks the stores
+// non-temporal (streamed past the caches).
+template <bool NT>
+__global__ void copy_probe4_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                   size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (q + r * stride < n) v[r] = src[q + r * stride];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (q + r * stride < n) {
+                if (NT) {
+                    __builtin_nontemporal_store(v[r].x, &dst[q + r * stride].x);
+                    __builtin_nontemporal_store(v[r].y, &dst[q + r * stride].y);
+                    __builtin_nontemporal_store(v[r].z, &dst[q + r * stride].z);
+                    __builtin_nontemporal_store(v[r].w, &dst[q + r * stride].w);
+                } else {
+                    dst[q + r * stride] = v[r];
+                }
+            }
+    }
 }
 
 }  // namespace
@@ -623,8 +946,13 @@ __global__ void __launch_bounds__(256) flux_accumulate_kernel(PlaneDev *planes, 
 // Host side
 // ==========================================================================================
 namespace {
-int g_blocks_per_cu = 8;
-int g_xcd_remap = 1;
+struct LaunchShape {
+    int vec = 1;            // cells per thread: 1, 2 or 4
+    int blocks_per_cu = 8;  // 0: one workgroup per tile; else a persistent grid
+    int xcd_remap = 0;
+    int nt = 0;             // non-temporal stores of the fused pairs' outputs
+};
+LaunchShape g_shape;
 int g_num_cu = 0;
 
 int num_cu() {
@@ -638,21 +966,81 @@ int num_cu() {
     return g_num_cu;
 }
 
-int persistent_grid(int ntiles) {
-    int g = num_cu() * g_blocks_per_cu;
-    g = (g / 8) * 8;
-    if (g < 8) g = 8;
-    if (ntiles < g) {
-        g = ((ntiles + 7) / 8) * 8;
-        if (g < 8) g = 8;
-    }
-    return g;
+int grid_for(int ntiles) {
+    int g = g_shape.blocks_per_cu > 0 ? num_cu() * g_shape.blocks_per_cu : ntiles;
+    if (g > ntiles) g = ntiles;
+    g = ((g + 7) / 8) * 8;
+    return g < 8 ? 8 : g;
 }
+
+int tile_cells() { return kThreads * g_shape.vec; }
 
 struct PlaneHost {
     PlaneDev dev;
-    int filled = 0;  // samples in the ring
+    int filled = 0;  // samples in the ring since the last transform pass
 };
+
+// A set of simulations stepped by the same launches: one, or a batch.
+struct Group {
+    SimDev *dev = nullptr;          // descriptors, contiguous
+    int nsims = 0;
+    std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
+    std::map<int, int> ntiles;
+    bool any_offdiag = false, all_offdiag = false;
+};
+
+int launch_h(const Group &g, hipStream_t st) {
+    const int V = g_shape.vec;
+    const int nt = g.ntiles.at(V);
+    const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
+    const int G = grid_for(nt);
+    const int rm = g_shape.xcd_remap;
+    if (g_shape.nt) {
+        if (V == 4) hpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        else if (V == 2) hpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        else hpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+    } else {
+        if (V == 4) hpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        else if (V == 2) hpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        else hpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+    }
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int launch_d(const Group &g, hipStream_t st) {
+    const int V = g_shape.vec;
+    const int nt = g.ntiles.at(V);
+    const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
+    const int G = grid_for(nt);
+    const int rm = g_shape.xcd_remap;
+    if (!g.all_offdiag) {
+        if (g_shape.nt) {
+            if (V == 4) dpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            else if (V == 2) dpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            else dpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        } else {
+            if (V == 4) dpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            else if (V == 2) dpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            else dpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        }
+    }
+    if (g.any_offdiag) {
+        if (V == 4) {
+            dcurl_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            erow_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        } else if (V == 2) {
+            dcurl_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            erow_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        } else {
+            dcurl_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            erow_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        }
+    }
+    advance_kernel<<<(g.nsims + 63) / 64, 64, 0, st>>>(g.dev, g.nsims);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
 }  // namespace
 
 struct fdtd_sim {
@@ -662,13 +1050,13 @@ struct fdtd_sim {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     std::vector<SrcDev> srcs;
-    SrcDev *dev_src = nullptr;
     std::vector<PlaneHost> planes;
     PlaneDev *dev_planes = nullptr;
+    int max_plane_cells = 0;
+    std::vector<uint8_t> hmember[3];  // integer-lattice layer membership: where H is stored
     uint64_t bytes = 0;
-    int ntiles = 0;
     bool dirty = true;
-    int host_step = 0;      // host mirror of the device counter
+    bool in_batch = false;
 
     int alloc(void **p, size_t bytes_) {
         hipError_t e = hipMalloc(p, bytes_);
@@ -677,7 +1065,18 @@ struct fdtd_sim {
         bytes += bytes_;
         return FDTD_OK;
     }
-    int volume(float **p, float fill_one = 0.0f);
+    int volume(float **p, float fill = 0.0f) {
+        const size_t n = (size_t)host.npad;
+        int rc = alloc((void **)p, n * sizeof(float));
+        if (rc) return rc;
+        if (fill == 0.0f) {
+            HIP_TRY(hipMemsetAsync(*p, 0, n * sizeof(float), stream));
+        } else {
+            fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(*p, n, fill);
+            HIP_TRY(hipGetLastError());
+        }
+        return FDTD_OK;
+    }
     int sync_desc() {
         if (!dirty) return FDTD_OK;
         HIP_TRY(hipMemcpyAsync(dev, &host, sizeof(SimDev), hipMemcpyHostToDevice, stream));
@@ -692,35 +1091,93 @@ struct fdtd_sim {
         HIP_TRY(hipStreamSynchronize(stream));
         return FDTD_OK;
     }
+    Group group() const {
+        Group g;
+        g.dev = dev;
+        g.nsims = 1;
+        const int64_t nt = ((int64_t)host.npad + tile_cells() - 1) / tile_cells();
+        g.ntiles[g_shape.vec] = (int)nt;
+        g.any_offdiag = g.all_offdiag = host.offdiag != 0;
+        return g;
+    }
+    int push_planes() {
+        if (planes.empty()) return FDTD_OK;
+        std::vector<PlaneDev> devs;
+        for (auto &x : planes) devs.push_back(x.dev);
+        HIP_TRY(hipMemcpyAsync(dev_planes, devs.data(), sizeof(PlaneDev) * devs.size(),
+                               hipMemcpyHostToDevice, stream));
+        return FDTD_OK;
+    }
+};
+
+struct fdtd_batch {
+    std::vector<fdtd_sim *> sims;
+    Group group;
+    std::vector<void *> allocs;
+    PlaneDev *planes = nullptr;     // every simulation's planes, pointing at group.dev
+    int nplanes = 0;
+    int max_plane_cells = 0;
+    hipStream_t stream = nullptr;
 };
 
 namespace {
-__global__ void fill_kernel(float *p, size_t n, float v) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) p[q] = v;
+int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
+    if (ph.filled == 0) return FDTD_OK;
+    const PlaneDev &p = ph.dev;
+    flux_phase_kernel<<<dim3((ph.filled * p.nf + 255) / 256, 1), 256, 0, st>>>(dev_plane, ph.filled);
+    const int per_block = 256 * kCB;
+    flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, 4, (p.nf + kFZ - 1) / kFZ),
+                             256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+    HIP_TRY(hipGetLastError());
+    ph.filled = 0;
+    return FDTD_OK;
 }
-}  // namespace
 
-int fdtd_sim::volume(float **p, float fill) {
-    const size_t n = (size_t)host.ncell;
-    int rc = alloc((void **)p, n * sizeof(float));
-    if (rc) return rc;
-    if (fill == 0.0f) {
-        HIP_TRY(hipMemsetAsync(*p, 0, n * sizeof(float), stream));
-    } else {
-        fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(*p, n, fill);
-        HIP_TRY(hipGetLastError());
+bool samples_now(const PlaneHost &ph, int step_after) {
+    const int since = step_after - ph.dev.first_step;
+    return since > 0 && since % ph.dev.dec == 0;
+}
+
+// Flux bookkeeping after a step: sample every plane in one launch, then a transform pass for
+// each plane whose ring is full.
+int flux_after_step(fdtd_sim *s, hipStream_t st) {
+    if (s->planes.empty()) return FDTD_OK;
+    bool any = false;
+    for (auto &ph : s->planes) any |= samples_now(ph, s->host.step);
+    if (!any) return FDTD_OK;
+    const int np = (int)s->planes.size();
+    flux_sample_kernel<<<dim3((s->max_plane_cells + 255) / 256, np), 256, 0, st>>>(s->dev_planes, np);
+    HIP_TRY(hipGetLastError());
+    for (int q = 0; q < np; ++q) {
+        PlaneHost &ph = s->planes[q];
+        if (!samples_now(ph, s->host.step)) continue;
+        ph.filled += 1;
+        if (ph.filled == kRing) {
+            int rc = flux_flush_plane(s->dev_planes + q, ph, st);
+            if (rc) return rc;
+        }
     }
     return FDTD_OK;
 }
 
-struct fdtd_batch {
-    std::vector<fdtd_sim *> sims;
-    SimDev *dev = nullptr;  // contiguous descriptors
-    int2 *tiles = nullptr;
-    int ntiles = 0;
-    hipStream_t stream = nullptr;
-};
+void mirror_advance(fdtd_sim *s) {
+    s->host.step += 1;  // mirror of the device counter the D pair's last workgroup advanced
+    s->host.hsep = 1;
+    if (s->host.npol) s->host.ppar ^= 1;
+}
+
+int sim_step(fdtd_sim *s) {
+    int rc = s->sync_desc();
+    if (rc) return rc;
+    const Group g = s->group();
+    rc = launch_h(g, s->stream);
+    if (rc) return rc;
+    rc = launch_d(g, s->stream);
+    if (rc) return rc;
+    mirror_advance(s);
+    return flux_after_step(s, s->stream);
+}
+}  // namespace
 
 extern "C" {
 
@@ -736,10 +1193,22 @@ int fdtd_device_info(char *text, size_t size) {
     return FDTD_OK;
 }
 
-int fdtd_set_launch(int32_t blocks_per_cu, int32_t xcd_remap) {
-    if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
-    g_blocks_per_cu = blocks_per_cu;
-    g_xcd_remap = xcd_remap ? 1 : 0;
+int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores) {
+    if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
+    g_shape.vec = vec;
+    g_shape.blocks_per_cu = blocks_per_cu;
+    g_shape.xcd_remap = xcd_remap ? 1 : 0;
+    g_shape.nt = nt_stores ? 1 : 0;
+    return FDTD_OK;
+}
+
+int fdtd_get_launch(int32_t out[4]) {
+    if (!out) return fail(FDTD_EINVAL, "null argument");
+    out[0] = g_shape.vec;
+    out[1] = g_shape.blocks_per_cu;
+    out[2] = g_shape.xcd_remap;
+    out[3] = g_shape.nt;
     return FDTD_OK;
 }
 
@@ -752,7 +1221,11 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
             return fail(FDTD_EINVAL, "invariant axis %d must have one cell", a);
         ncell *= cfg->n[a];
     }
-    if (ncell >= (int64_t)1 << 31) return fail(FDTD_EINVAL, "grid of %lld cells exceeds 2^31", (long long)ncell);
+    const int64_t pitch = ((cfg->n[0] + kPad - 1) / kPad) * kPad;
+    const int64_t npad = pitch * cfg->n[1] * cfg->n[2];
+    if (npad >= ((int64_t)1 << 31) - 4 * kThreads * 4)
+        return fail(FDTD_EINVAL, "grid of %lld padded cells exceeds the 2^31 index range", (long long)npad);
+    if (!(cfg->dt > 0.0)) return fail(FDTD_EINVAL, "dt must be positive");
     fdtd_sim *s = new (std::nothrow) fdtd_sim();
     if (!s) return fail(FDTD_ENOMEM, "host allocation");
     s->cfg = *cfg;
@@ -762,10 +1235,12 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
         h.n[a] = cfg->n[a];
         h.present[a] = cfg->invariant[a] ? 0 : 1;
     }
+    h.pitch = (int)pitch;
     h.dtdx = cfg->dtdx;
+    h.dt = cfg->dt;
     h.pml = cfg->pml ? 1 : 0;
     h.ncell = (int)ncell;
-    s->ntiles = (int)((ncell + kTile - 1) / kTile);
+    h.npad = (int)npad;
     int rc = FDTD_OK;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -791,12 +1266,12 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
     for (int half = 0; half < 2; ++half)
         for (int a = 0; a < 3; ++a) {
             void *p;
-            TRY(s->alloc(&p, sizeof(float4) * h.n[a]));
-            std::vector<float4> ones(h.n[a], make_float4(1.f, 1.f, 1.f, 0.f));
-            e = hipMemcpyAsync(p, ones.data(), sizeof(float4) * h.n[a], hipMemcpyHostToDevice, s->stream);
-            if (e != hipSuccess) { fdtd_sim_destroy(s); return fail(FDTD_EHIP, "hipMemcpy: %s", hipGetErrorString(e)); }
-            e = hipStreamSynchronize(s->stream);
-            if (e != hipSuccess) { fdtd_sim_destroy(s); return fail(FDTD_EHIP, "sync: %s", hipGetErrorString(e)); }
+            const int len = a == 0 ? h.pitch : h.n[a];  // the x line covers the row padding
+            TRY(s->alloc(&p, sizeof(float4) * len));
+            std::vector<float4> ones(len, make_float4(1.f, 1.f, 1.f, 0.f));
+            e = hipMemcpyAsync(p, ones.data(), sizeof(float4) * len, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            if (e != hipSuccess) { fdtd_sim_destroy(s); return fail(FDTD_EHIP, "line upload: %s", hipGetErrorString(e)); }
             h.line[half][a] = (const float4 *)p;
         }
 #undef TRY
@@ -826,6 +1301,24 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
     const int n = s->host.n[axis];
     std::vector<float4> v(n);
     for (int q = 0; q < n; ++q) v[q] = make_float4(kms[q], sinv[q], kps[q], member[q] ? 1.f : 0.f);
+    if (half == 0) s->hmember[axis].assign(member, member + n);
+    int runs[2 * kMaxRuns];
+    for (int t = 0; t < 2 * kMaxRuns; ++t) runs[t] = 0;  // empty runs [0, 0)
+    int nr = 0;
+    for (int q = 0; q < n;) {
+        if (!member[q]) { ++q; continue; }
+        int e = q;
+        while (e < n && member[e]) ++e;
+        if (nr == kMaxRuns) return fail(FDTD_EINVAL, "layer membership of axis %d has more than %d runs", axis, kMaxRuns);
+        runs[2 * nr] = q;
+        runs[2 * nr + 1] = e;
+        ++nr;
+        q = e;
+    }
+    int rc = s->pull_desc();
+    if (rc) return rc;
+    for (int t = 0; t < 2 * kMaxRuns; ++t) s->host.mr[half][axis][t] = runs[t];
+    s->dirty = true;
     HIP_TRY(hipMemcpyAsync((void *)s->host.line[half][axis], v.data(), sizeof(float4) * n,
                            hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -833,7 +1326,10 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
 }
 
 static int upload_volume(fdtd_sim *s, float *dst, const float *src) {
-    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)s->host.ncell, hipMemcpyHostToDevice, s->stream));
+    const SimDev &h = s->host;
+    HIP_TRY(hipMemcpy2DAsync(dst, sizeof(float) * h.pitch, src, sizeof(float) * h.n[0],
+                             sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
+                             hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return FDTD_OK;
 }
@@ -846,6 +1342,7 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
 int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float *vol) {
     if (!s || row < 0 || row > 2 || partner < 0 || partner > 2 || row == partner || !vol)
         return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->pull_desc();
     if (rc) return rc;
     if (!s->host.u[row][partner]) {
@@ -862,6 +1359,7 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
 int fdtd_add_pole(fdtd_sim *s, const float consts[4], const float *const sigma[3],
                   const float sigma_scalar[3], const int32_t driven[3], int32_t *pole) {
     if (!s || !consts || !sigma || !sigma_scalar || !driven) return fail(FDTD_EINVAL, "fdtd_add_pole: null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->pull_desc();
     if (rc) return rc;
     SimDev &h = s->host;
@@ -898,6 +1396,7 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
     if (!s || field < 0 || field > 1 || comp < 0 || comp > 2 || !lo || !cnt || !amp_re || !amp_im ||
         !current || nsteps < 0)
         return fail(FDTD_EINVAL, "fdtd_add_source: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int64_t npts = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] + cnt[a] > s->host.n[a])
@@ -932,6 +1431,16 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->host.src = (const SrcDev *)p;
     s->host.nsrc = (int)s->srcs.size();
+    for (int f = 0; f < 2; ++f)
+        for (int a = 0; a < 3; ++a) {
+            s->host.src_lo[f][a] = 1 << 30;
+            s->host.src_hi[f][a] = -1;
+        }
+    for (const SrcDev &x : s->srcs)
+        for (int a = 0; a < 3; ++a) {
+            s->host.src_lo[x.field][a] = std::min(s->host.src_lo[x.field][a], x.lo[a]);
+            s->host.src_hi[x.field][a] = std::max(s->host.src_hi[x.field][a], x.lo[a] + x.cnt[a]);
+        }
     s->dirty = true;
     return FDTD_OK;
 }
@@ -963,12 +1472,12 @@ static float *volume_by_name(fdtd_sim *s, const char *name) {
     }
     int k = -1;
     char e = 0, cc = 0;
-    if (sscanf(name, "P[%d].%c%c", &k, &e, &cc) == 3 && !strncmp(name, "P[", 2)) {
+    if (!strncmp(name, "P[", 2) && sscanf(name, "P[%d].%c%c", &k, &e, &cc) == 3) {
         const int c = comp(cc);
         if (k < 0 || k >= h.npol || e != 'E' || c < 0 || !h.pdriven[k][c]) return nullptr;
         return h.pbuf[k][c][h.ppar];
     }
-    if (sscanf(name, "P_prev[%d].%c%c", &k, &e, &cc) == 3) {
+    if (!strncmp(name, "P_prev[", 7) && sscanf(name, "P_prev[%d].%c%c", &k, &e, &cc) == 3) {
         const int c = comp(cc);
         if (k < 0 || k >= h.npol || e != 'E' || c < 0 || !h.pdriven[k][c]) return nullptr;
         return h.pbuf[k][c][h.ppar ^ 1];
@@ -978,6 +1487,7 @@ static float *volume_by_name(fdtd_sim *s, const char *name) {
 
 int fdtd_upload(fdtd_sim *s, const char *name, const float *host, size_t count) {
     if (!s || !name || !host) return fail(FDTD_EINVAL, "fdtd_upload: null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->pull_desc();
     if (rc) return rc;
     if (count != (size_t)s->host.ncell) return fail(FDTD_EINVAL, "upload %s: %zu floats, grid has %d", name, count, s->host.ncell);
@@ -988,18 +1498,45 @@ int fdtd_upload(fdtd_sim *s, const char *name, const float *host, size_t count) 
 
 int fdtd_download(fdtd_sim *s, const char *name, float *host, size_t count) {
     if (!s || !name || !host) return fail(FDTD_EINVAL, "fdtd_download: null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->pull_desc();
     if (rc) return rc;
     if (count != (size_t)s->host.ncell) return fail(FDTD_EINVAL, "download %s: %zu floats, grid has %d", name, count, s->host.ncell);
     float *src = volume_by_name(s, name);
     if (!src) return fail(FDTD_EINVAL, "no volume named '%s' in this simulation", name);
-    HIP_TRY(hipMemcpyAsync(host, src, sizeof(float) * count, hipMemcpyDeviceToHost, s->stream));
+    const SimDev &h = s->host;
+    HIP_TRY(hipMemcpy2DAsync(host, sizeof(float) * h.n[0], src, sizeof(float) * h.pitch,
+                             sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
+                             hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    if (name[0] == 'H' && name[2] == 0 && h.pml && h.hsep) {
+        // after the first update_H the stored H is current only on its own-axis layer rows;
+        // everywhere else H is B's word
+        const int c = name[1] - 'x';
+        std::vector<float> b((size_t)h.ncell);
+        HIP_TRY(hipMemcpy2DAsync(b.data(), sizeof(float) * h.n[0], h.B[c], sizeof(float) * h.pitch,
+                                 sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        const std::vector<uint8_t> &line = s->hmember[c];
+        for (int k = 0; k < h.n[2]; ++k)
+            for (int j = 0; j < h.n[1]; ++j)
+                for (int i = 0; i < h.n[0]; ++i) {
+                    const int cc = c == 0 ? i : (c == 1 ? j : k);
+                    const bool member = !line.empty() && line[cc];
+                    const size_t idx = ((size_t)k * h.n[1] + j) * h.n[0] + i;
+                    if (!member) host[idx] = b[idx];
+                }
+    }
     return FDTD_OK;
 }
 
 int fdtd_get_step(fdtd_sim *s, int32_t *step) {
     if (!s || !step) return fail(FDTD_EINVAL, "null argument");
+    if (s->in_batch) {
+        *step = s->host.step;  // the batch mirrors every step it runs
+        return FDTD_OK;
+    }
     int rc = s->pull_desc();
     if (rc) return rc;
     *step = s->host.step;
@@ -1008,6 +1545,8 @@ int fdtd_get_step(fdtd_sim *s, int32_t *step) {
 
 int fdtd_set_step(fdtd_sim *s, int32_t step, int32_t h_separated) {
     if (!s || step < 0) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (!s->planes.empty()) return fail(FDTD_ESTATE, "the step count is fixed once a flux plane samples");
     int rc = s->pull_desc();
     if (rc) return rc;
     s->host.step = step;
@@ -1020,76 +1559,35 @@ int fdtd_prime_e(fdtd_sim *s) {
     if (!s) return fail(FDTD_EINVAL, "null argument");
     int rc = s->sync_desc();
     if (rc) return rc;
-    prime_e_kernel<<<s->ntiles, kTile, 0, s->stream>>>(s->dev, s->ntiles);
-    HIP_TRY(hipGetLastError());
-    return FDTD_OK;
-}
-
-static int launch_h(SimDev *dev, const int2 *tiles, int ntiles, hipStream_t st) {
-    const int g = persistent_grid(ntiles);
-    hpair_kernel<<<g, kTile, 0, st>>>(dev, tiles, ntiles, g_xcd_remap);
-    HIP_TRY(hipGetLastError());
-    return FDTD_OK;
-}
-
-static int launch_d(SimDev *dev, const int2 *tiles, int ntiles, int count, bool any_offdiag,
-                    bool all_offdiag, hipStream_t st) {
-    const int g = persistent_grid(ntiles);
-    if (!all_offdiag) dpair_kernel<<<g, kTile, 0, st>>>(dev, tiles, ntiles, g_xcd_remap);
-    if (any_offdiag) {
-        dcurl_kernel<<<g, kTile, 0, st>>>(dev, tiles, ntiles, g_xcd_remap);
-        erow_kernel<<<g, kTile, 0, st>>>(dev, tiles, ntiles, g_xcd_remap);
-    }
-    advance_kernel<<<(count + 63) / 64, 64, 0, st>>>(dev, count);
+    prime_e_kernel<<<(s->host.npad + 255) / 256, 256, 0, s->stream>>>(s->dev, s->host.npad);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
 
 int fdtd_step_h(fdtd_sim *s) {
     if (!s) return fail(FDTD_EINVAL, "null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->sync_desc();
     if (rc) return rc;
-    return launch_h(s->dev, nullptr, s->ntiles, s->stream);
-}
-
-static int flux_flush(fdtd_sim *s, PlaneHost &ph, int plane_index);
-
-static int flux_after_step(fdtd_sim *s, int step_after) {
-    if (s->planes.empty()) return FDTD_OK;
-    // every plane shares the ring slot index when decimations agree; sample each on its own
-    for (size_t q = 0; q < s->planes.size(); ++q) {
-        PlaneHost &ph = s->planes[q];
-        if (step_after % ph.dev.dec) continue;
-        dim3 grid((ph.dev.cells + 255) / 256, 1);
-        flux_sample_kernel<<<grid, 256, 0, s->stream>>>(s->dev, s->dev_planes + q, 1, ph.filled);
-        HIP_TRY(hipGetLastError());
-        ph.filled += 1;
-        if (ph.filled == kRing) {
-            int rc = flux_flush(s, ph, (int)q);
-            if (rc) return rc;
-        }
-    }
-    return FDTD_OK;
+    return launch_h(s->group(), s->stream);
 }
 
 int fdtd_step_d(fdtd_sim *s) {
     if (!s) return fail(FDTD_EINVAL, "null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->sync_desc();
     if (rc) return rc;
-    rc = launch_d(s->dev, nullptr, s->ntiles, 1, s->host.offdiag, s->host.offdiag, s->stream);
+    rc = launch_d(s->group(), s->stream);
     if (rc) return rc;
-    s->host.step += 1;  // mirror (the device counter is authoritative)
-    s->host.hsep = 1;
-    if (s->host.npol) s->host.ppar ^= 1;
-    return flux_after_step(s, s->host.step);
+    mirror_advance(s);
+    return flux_after_step(s, s->stream);
 }
 
 int fdtd_run(fdtd_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     for (int q = 0; q < nsteps; ++q) {
-        int rc = fdtd_step_h(s);
-        if (rc) return rc;
-        rc = fdtd_step_d(s);
+        int rc = sim_step(s);
         if (rc) return rc;
     }
     return FDTD_OK;
@@ -1106,29 +1604,34 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     if (!s || normal < 0 || normal > 2 || !lo || !cnt || nf < 1 || !omega || dec < 1)
         return fail(FDTD_EINVAL, "fdtd_add_flux_plane: bad argument");
     if (cnt[normal] != 1) return fail(FDTD_EINVAL, "a flux plane is one cell thick along its normal");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int cells = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] >= s->host.n[a] || cnt[a] > s->host.n[a])
             return fail(FDTD_EINVAL, "flux box axis %d [%d,+%d) outside the grid", a, lo[a], cnt[a]);
         cells *= cnt[a];
     }
+    int rc = s->pull_desc();
+    if (rc) return rc;
     PlaneHost ph;
     PlaneDev &p = ph.dev;
+    p.sim = s->dev;
     p.normal = normal;
     for (int a = 0; a < 3; ++a) { p.lo[a] = lo[a]; p.cnt[a] = cnt[a]; }
     p.cells = cells;
+    p.row = ((cells + kCB - 1) / kCB) * kCB;
     p.nf = nf;
     p.dec = dec;
+    p.first_step = s->host.step;
     p.scale = scale;
-    p.dt = s->cfg.dt;
-    int rc;
     void *m;
     rc = s->alloc(&m, sizeof(double) * nf);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(m, omega, sizeof(double) * nf, hipMemcpyHostToDevice, s->stream));
     p.omega = (const double *)m;
-    rc = s->alloc(&m, sizeof(float) * (size_t)kRing * 4 * cells);
+    rc = s->alloc(&m, sizeof(float) * (size_t)kRing * 4 * p.row);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(m, 0, sizeof(float) * (size_t)kRing * 4 * p.row, s->stream));
     p.ring = (float *)m;
     rc = s->alloc(&m, sizeof(double) * kRing);
     if (rc) return rc;
@@ -1143,33 +1646,22 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     s->planes.push_back(ph);
     rc = s->alloc(&m, sizeof(PlaneDev) * s->planes.size());
     if (rc) return rc;
-    std::vector<PlaneDev> devs;
-    for (auto &x : s->planes) devs.push_back(x.dev);
-    HIP_TRY(hipMemcpyAsync(m, devs.data(), sizeof(PlaneDev) * devs.size(), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
     s->dev_planes = (PlaneDev *)m;
+    if (cells > s->max_plane_cells) s->max_plane_cells = cells;
+    rc = s->push_planes();
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
     if (plane) *plane = (int)s->planes.size() - 1;
-    return FDTD_OK;
-}
-
-static int flux_flush(fdtd_sim *s, PlaneHost &ph, int q) {
-    if (ph.filled == 0) return FDTD_OK;
-    PlaneDev &p = ph.dev;
-    dim3 g1((ph.filled * p.nf + 255) / 256, 1);
-    flux_phase_kernel<<<g1, 256, 0, s->stream>>>(s->dev_planes + q, ph.filled);
-    dim3 g2((p.cells + 255) / 256, 4 * p.nf, 1);
-    flux_accumulate_kernel<<<g2, 256, 0, s->stream>>>(s->dev_planes + q, ph.filled);
-    HIP_TRY(hipGetLastError());
-    ph.filled = 0;
     return FDTD_OK;
 }
 
 int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
     if (!s || plane < 0 || plane >= (int)s->planes.size() || !out) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     PlaneHost &ph = s->planes[plane];
     const size_t need = (size_t)2 * 4 * ph.dev.nf * ph.dev.cells;
     if (count != need) return fail(FDTD_EINVAL, "flux read needs %zu doubles, got %zu", need, count);
-    int rc = flux_flush(s, ph, plane);
+    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(out, ph.dev.acc, sizeof(double) * need, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1178,8 +1670,11 @@ int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
 
 int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
     if (!s || repeats < 1 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (!s->planes.empty()) return fail(FDTD_ESTATE, "fdtd_time_pairs steps a simulation without flux planes");
     int rc = s->sync_desc();
     if (rc) return rc;
+    const Group g = s->group();
     hipEvent_t a, b, c;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
@@ -1187,24 +1682,20 @@ int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
     float th = 0.f, td = 0.f;
     for (int r = 0; r < repeats; ++r) {
         HIP_TRY(hipEventRecord(a, s->stream));
-        rc = launch_h(s->dev, nullptr, s->ntiles, s->stream);
+        rc = launch_h(g, s->stream);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(b, s->stream));
-        const int g = persistent_grid(s->ntiles);
-        dpair_kernel<<<g, kTile, 0, s->stream>>>(s->dev, nullptr, s->ntiles, g_xcd_remap);
+        rc = launch_d(g, s->stream);
+        if (rc) return rc;
         HIP_TRY(hipEventRecord(c, s->stream));
-        advance_kernel<<<1, 64, 0, s->stream>>>(s->dev, 1);
         HIP_TRY(hipEventSynchronize(c));
         float x = 0.f, y = 0.f;
         HIP_TRY(hipEventElapsedTime(&x, a, b));
         HIP_TRY(hipEventElapsedTime(&y, b, c));
         th += x;
         td += y;
+        mirror_advance(s);
     }
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    s->host.step += repeats;
-    s->host.hsep = 1;
-    if (s->host.npol && (repeats & 1)) s->host.ppar ^= 1;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipEventDestroy(c);
@@ -1213,38 +1704,156 @@ int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
     return FDTD_OK;
 }
 
+int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
+    if (!s || nsteps < 0 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int rc = s->sync_desc();
+    if (rc) return rc;
+    const Group g = s->group();
+    constexpr int kChunk = 128;
+    std::vector<hipEvent_t> ev(3 * kChunk + 2);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    double th = 0.0, td = 0.0;
+    HIP_TRY(hipEventRecord(ev[3 * kChunk], s->stream));
+    for (int base = 0; base < nsteps; base += kChunk) {
+        const int n = nsteps - base < kChunk ? nsteps - base : kChunk;
+        for (int q = 0; q < n; ++q) {
+            HIP_TRY(hipEventRecord(ev[3 * q], s->stream));
+            rc = launch_h(g, s->stream);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(ev[3 * q + 1], s->stream));
+            rc = launch_d(g, s->stream);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(ev[3 * q + 2], s->stream));
+            mirror_advance(s);
+            rc = flux_after_step(s, s->stream);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventSynchronize(ev[3 * (n - 1) + 2]));
+        for (int q = 0; q < n; ++q) {
+            float x = 0.f, y = 0.f;
+            HIP_TRY(hipEventElapsedTime(&x, ev[3 * q], ev[3 * q + 1]));
+            HIP_TRY(hipEventElapsedTime(&y, ev[3 * q + 1], ev[3 * q + 2]));
+            th += x;
+            td += y;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev[3 * kChunk + 1], s->stream));
+    HIP_TRY(hipEventSynchronize(ev[3 * kChunk + 1]));
+    float total = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total, ev[3 * kChunk], ev[3 * kChunk + 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    out_ms[0] = total;
+    out_ms[1] = nsteps ? (float)(th / nsteps) : 0.f;
+    out_ms[2] = nsteps ? (float)(td / nsteps) : 0.f;
+    return FDTD_OK;
+}
+
+int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms) {
+    if (bytes < 1024 || repeats < 1 || !ms || mode < 0 || mode > 4) return fail(FDTD_EINVAL, "bad argument");
+    const size_t n = (size_t)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc((void **)&a, n * 16));
+    if (hipMalloc((void **)&b, n * 16) != hipSuccess) { (void)hipFree(a); return fail(FDTD_ENOMEM, "copy probe"); }
+    HIP_TRY(hipMemset(a, 0, n * 16));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int grid = num_cu() * 8;
+    auto go = [&]() {
+        if (mode == 0) copy_probe_kernel<<<grid, 256>>>(a, b, n);
+        else if (mode == 1) copy_probe4_kernel<false><<<grid, 256>>>(a, b, n);
+        else if (mode == 2) copy_probe4_kernel<true><<<grid, 256>>>(a, b, n);
+        else if (mode == 3) read_probe_kernel<<<grid, 256>>>(a, (float *)b, n);
+        else write_probe_kernel<<<grid, 256>>>(b, n);
+    };
+    go();
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; ++r) go();
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return FDTD_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // Batch
 // ------------------------------------------------------------------------------------------
+static int batch_tiles(fdtd_batch *b, int V) {
+    if (b->group.tiles.count(V)) return FDTD_OK;
+    std::vector<int2> tiles;
+    const int tc = kThreads * V;
+    for (int q = 0; q < (int)b->sims.size(); ++q) {
+        const int nt = (b->sims[q]->host.npad + tc - 1) / tc;
+        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(q, t * tc));
+    }
+    int2 *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(int2) * tiles.size()));
+    b->allocs.push_back(d);
+    HIP_TRY(hipMemcpy(d, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+    b->group.tiles[V] = d;
+    b->group.ntiles[V] = (int)tiles.size();
+    return FDTD_OK;
+}
+
 int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     if (!sims || count < 1 || !out) return fail(FDTD_EINVAL, "fdtd_batch_create: bad argument");
+    for (int q = 0; q < count; ++q) {
+        if (!sims[q]) return fail(FDTD_EINVAL, "null simulation %d", q);
+        if (sims[q]->in_batch) return fail(FDTD_ESTATE, "simulation %d is already in a batch", q);
+        for (int r = 0; r < q; ++r)
+            if (sims[r] == sims[q]) return fail(FDTD_EINVAL, "simulation %d appears twice", q);
+    }
     fdtd_batch *b = new (std::nothrow) fdtd_batch();
     if (!b) return fail(FDTD_ENOMEM, "host allocation");
     std::vector<SimDev> descs;
-    std::vector<int2> tiles;
     for (int q = 0; q < count; ++q) {
         fdtd_sim *s = sims[q];
-        if (!s) { delete b; return fail(FDTD_EINVAL, "null simulation %d", q); }
-        if (!s->planes.empty()) { delete b; return fail(FDTD_EINVAL, "batched simulations carry no flux planes here"); }
         int rc = s->pull_desc();
         if (rc) { delete b; return rc; }
+        HIP_TRY(hipStreamSynchronize(s->stream));
         b->sims.push_back(s);
         descs.push_back(s->host);
-        for (int t = 0; t < s->ntiles; ++t) tiles.push_back(make_int2(q, t * kTile));
+        b->group.any_offdiag |= s->host.offdiag != 0;
     }
+    b->group.all_offdiag = true;
+    for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;
+    b->group.nsims = count;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
-    if (hipMalloc((void **)&b->dev, sizeof(SimDev) * count) != hipSuccess ||
-        hipMalloc((void **)&b->tiles, sizeof(int2) * tiles.size()) != hipSuccess) {
+    auto cleanup = [&](int code, const char *what) {
         fdtd_batch_destroy(b);
-        return fail(FDTD_ENOMEM, "batch descriptors");
+        return fail(code, "%s", what);
+    };
+    if (hipMalloc((void **)&b->group.dev, sizeof(SimDev) * count) != hipSuccess) return cleanup(FDTD_ENOMEM, "batch descriptors");
+    b->allocs.push_back(b->group.dev);
+    if (hipMemcpy(b->group.dev, descs.data(), sizeof(SimDev) * count, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(FDTD_EHIP, "batch descriptor upload");
+    // every simulation's planes, re-pointed at the batch's descriptors
+    std::vector<PlaneDev> planes;
+    for (int q = 0; q < count; ++q)
+        for (size_t r = 0; r < b->sims[q]->planes.size(); ++r) {
+            PlaneDev p = b->sims[q]->planes[r].dev;
+            p.sim = b->group.dev + q;
+            planes.push_back(p);
+            if (p.cells > b->max_plane_cells) b->max_plane_cells = p.cells;
+        }
+    b->nplanes = (int)planes.size();
+    if (!planes.empty()) {
+        if (hipMalloc((void **)&b->planes, sizeof(PlaneDev) * planes.size()) != hipSuccess) return cleanup(FDTD_ENOMEM, "batch planes");
+        b->allocs.push_back(b->planes);
+        if (hipMemcpy(b->planes, planes.data(), sizeof(PlaneDev) * planes.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(FDTD_EHIP, "batch plane upload");
     }
-    if (hipMemcpy(b->dev, descs.data(), sizeof(SimDev) * count, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess) {
-        fdtd_batch_destroy(b);
-        return fail(FDTD_EHIP, "batch descriptor upload");
-    }
-    b->ntiles = (int)tiles.size();
+    int rc = batch_tiles(b, g_shape.vec);
+    if (rc) { fdtd_batch_destroy(b); return rc; }
+    for (auto *s : b->sims) s->in_batch = true;
     *out = b;
     return FDTD_OK;
 }
@@ -1253,33 +1862,62 @@ void fdtd_batch_destroy(fdtd_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     // hand the advanced counters back to each simulation's own descriptor
-    if (b->dev) {
+    if (b->group.dev && !b->sims.empty()) {
         std::vector<SimDev> descs(b->sims.size());
-        if (hipMemcpy(descs.data(), b->dev, sizeof(SimDev) * descs.size(), hipMemcpyDeviceToHost) == hipSuccess)
+        if (hipMemcpy(descs.data(), b->group.dev, sizeof(SimDev) * descs.size(), hipMemcpyDeviceToHost) == hipSuccess)
             for (size_t q = 0; q < descs.size(); ++q) {
-                b->sims[q]->host = descs[q];
+                b->sims[q]->host.step = descs[q].step;
+                b->sims[q]->host.hsep = descs[q].hsep;
+                b->sims[q]->host.ppar = descs[q].ppar;
                 b->sims[q]->dirty = true;
             }
-        (void)hipFree(b->dev);
+        for (auto *s : b->sims) s->in_batch = false;
     }
-    if (b->tiles) (void)hipFree(b->tiles);
+    for (void *p : b->allocs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
 int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
-    bool any_off = false, all_off = true;
-    for (auto *s : b->sims) {
-        any_off |= s->host.offdiag != 0;
-        all_off &= s->host.offdiag != 0;
-    }
+    int rc = batch_tiles(b, g_shape.vec);
+    if (rc) return rc;
     for (int q = 0; q < nsteps; ++q) {
-        int rc = launch_h(b->dev, b->tiles, b->ntiles, b->stream);
+        rc = launch_h(b->group, b->stream);
         if (rc) return rc;
-        rc = launch_d(b->dev, b->tiles, b->ntiles, (int)b->sims.size(), any_off, all_off, b->stream);
+        rc = launch_d(b->group, b->stream);
         if (rc) return rc;
+        bool sample = false;
+        for (auto *s : b->sims) {
+            mirror_advance(s);
+            for (auto &ph : s->planes) sample |= samples_now(ph, s->host.step);
+        }
+        if (!sample) continue;
+        flux_sample_kernel<<<dim3((b->max_plane_cells + 255) / 256, b->nplanes), 256, 0, b->stream>>>(b->planes, b->nplanes);
+        HIP_TRY(hipGetLastError());
+        int pi = 0;
+        for (auto *s : b->sims)
+            for (size_t r = 0; r < s->planes.size(); ++r, ++pi) {
+                PlaneHost &ph = s->planes[r];
+                if (!samples_now(ph, s->host.step)) continue;
+                ph.filled += 1;
+                if (ph.filled == kRing) {
+                    rc = flux_flush_plane(b->planes + pi, ph, b->stream);
+                    if (rc) return rc;
+                }
+            }
     }
+    return FDTD_OK;
+}
+
+int fdtd_batch_flush(fdtd_batch *b) {
+    if (!b) return fail(FDTD_EINVAL, "null argument");
+    int pi = 0;
+    for (auto *s : b->sims)
+        for (size_t r = 0; r < s->planes.size(); ++r, ++pi) {
+            int rc = flux_flush_plane(b->planes + pi, s->planes[r], b->stream);
+            if (rc) return rc;
+        }
     return FDTD_OK;
 }
 

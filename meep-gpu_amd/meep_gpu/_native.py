@@ -54,8 +54,13 @@ SIGNATURES = {
     "fdtd_batch_destroy": (None, [ctypes.c_void_p]),
     "fdtd_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fdtd_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
-    "fdtd_set_launch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
+    "fdtd_set_launch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32]),
+    "fdtd_get_launch": (ctypes.c_int, [_IP]),
+    "fdtd_batch_flush": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_time_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
+    "fdtd_run_timed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
+    "fdtd_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _FP]),
 }
 
 

@@ -1,0 +1,95 @@
+"""Launch-shape autotuner: pick the tile shape the fused pairs run fastest with on this GPU.
+
+The candidates are the engine's launch shapes (``fdtd_set_launch``): cells per thread along x
+(1, 2 or 4 -- wider runs mean fewer index computations and wider loads, narrower ones fewer
+registers and more resident waves), one workgroup per tile or a persistent grid of 4/8
+workgroups per CU, the XCD-contiguous tile order on or off, and plain or non-temporal output
+stores. Each is timed on a scratch
+simulation with the same grid, absorber and poles as the run (no sources, no monitors), with
+HIP events around every launch, and the fastest mean H+D pair wins. Results are cached per
+(grid shape, absorber, poles, device) in a JSON file, so a served or batched run tunes once.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, Iterable, List, Optional, Tuple
+
+import numpy
+
+from .engine import HipSimulation, device_info, get_launch, set_launch
+from .grid import Grid
+from .pml import PML
+
+Shape = Tuple[int, int, bool, bool]
+
+CANDIDATES: Tuple[Shape, ...] = tuple(
+    (vec, bpc, remap, nt) for vec in (1, 2, 4) for bpc in (0, 4, 8) for remap in (True, False)
+    for nt in (False, True))
+
+_CACHE = os.environ.get("MEEP_GPU_TUNE_CACHE",
+                        os.path.join(os.path.expanduser("~"), ".cache", "meep_gpu_tune.json"))
+
+
+def _key(grid: Grid, pml: Optional[PML], npoles: int) -> str:
+    faces = pml.faces if pml is not None else None
+    return json.dumps([list(grid.shape), faces, npoles, device_info()])
+
+
+def _load_cache() -> Dict[str, list]:
+    try:
+        with open(_CACHE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(cache: Dict[str, list]) -> None:
+    try:
+        os.makedirs(os.path.dirname(_CACHE), exist_ok=True)
+        with open(_CACHE, "w") as fh:
+            json.dump(cache, fh)
+    except OSError:
+        pass
+
+
+def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
+            candidates: Iterable[Shape] = CANDIDATES, repeats: int = 10
+            ) -> List[Tuple[Shape, float, float]]:
+    """(shape, mean H-pair ms, mean D-pair ms) for every candidate, on a scratch simulation."""
+    sim = HipSimulation(grid, pml)
+    for term, sigma in poles:
+        sim.add_pole(term, sigma)
+    saved = get_launch()
+    out = []
+    try:
+        for shape in candidates:
+            set_launch(*shape)
+            sim.time_pairs(2)  # first launch of a shape builds its tile walk
+            h, d = sim.time_pairs(repeats)
+            out.append((shape, h, d))
+    finally:
+        set_launch(*saved)
+        sim.close()
+    return out
+
+
+def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
+         use_cache: bool = True) -> Shape:
+    """Choose and install the fastest launch shape for this grid; returns it."""
+    poles = list(poles)
+    key = _key(grid, pml, len(poles))
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        shape = tuple(cache[key])
+        set_launch(*shape)
+        return shape  # type: ignore[return-value]
+    results = measure(grid, pml, poles, repeats=repeats)
+    best = min(results, key=lambda r: r[1] + r[2])[0]
+    set_launch(*best)
+    if use_cache:
+        cache[key] = list(best)
+        _store_cache(cache)
+    return best

@@ -1,0 +1,204 @@
+"""FdtdDriver: the reference driver's API over the HIP engine.
+
+Mirrors reference ``meep_gpu.driver.FdtdDriver`` (driver.py:796) for the configurations the
+engine steps: build the material (``set_epsilon_components`` :1220), poles
+(``add_susceptibility`` :1531), the absorber (``setup_pml`` :2155), sources (``add_source``
+:2567), flux monitors (``add_flux_monitor`` :3001), then ``step`` (:3419) / ``run`` (:3667).
+
+The configuration freezes at the first step, as the reference's does (driver.py:3505-3521):
+the device simulation is created then, with every table uploaded once (source currents for
+the whole planned run included), and from there a step is two fused launches and the flux
+sampling, with no host work in between. ``fast_path_report`` names the engine that ran.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy
+
+from .dft import FluxMonitor, meep_frequency_span
+from .dispersion import Susceptibility
+from .engine import HipSimulation
+from .grid import Grid
+from .pml import PML
+from .sources import DepositBox, current_table
+from .workloads import Workload
+
+_E = ("Ex", "Ey", "Ez")
+
+
+class FdtdDriver:
+    def __init__(self, cell_size: Sequence[float], resolution: float, courant: float = 0.5,
+                 dimensions: int = 3, prefer_gpu: bool = True, max_steps: int = 100000):
+        if not prefer_gpu:
+            raise ValueError("this package steps on the GPU only; prefer_gpu=False has no engine")
+        self.grid = Grid(resolution=resolution, cell_size=tuple(cell_size), courant=courant,
+                         dimensions=dimensions)
+        shape = self.grid.shape
+        self._inv = {c: numpy.ones(shape, dtype=numpy.float32) for c in _E}
+        self._offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
+        self._poles: List[Tuple[Susceptibility, Dict[str, object]]] = []
+        self._pml_spec = None
+        self._pml_kwargs = {}
+        self._sources: List[Tuple[DepositBox, object]] = []
+        self.flux_monitors: List[FluxMonitor] = []
+        self.max_steps = int(max_steps)
+        self.sim: Optional[HipSimulation] = None
+        self._planned = 0
+
+    # --- configuration (before the first step) ------------------------------------------------
+    def _require_unfrozen(self, what: str):
+        if self.sim is not None:
+            raise RuntimeError(f"{what} must be called before the first step")
+
+    def set_epsilon_components(self, epsilon: Dict[str, object],
+                               chi1inv_offdiagonal: Optional[Dict[str, Dict[str, object]]] = None):
+        self._require_unfrozen("set_epsilon_components")
+        if set(epsilon) != set(_E):
+            raise ValueError("epsilon must map exactly Ex, Ey, Ez")
+        for c in _E:
+            eps = numpy.broadcast_to(numpy.asarray(epsilon[c], dtype=numpy.float64),
+                                     self.grid.shape)
+            if not numpy.all(numpy.isfinite(eps)) or numpy.any(eps <= 0):
+                raise ValueError(f"epsilon[{c}] must be finite and positive")
+            # chi1inv is the reciprocal formed in double and rounded once (driver.py:1305)
+            self._inv[c] = numpy.ascontiguousarray((1.0 / eps).astype(numpy.float32))
+        self._offdiag = {}
+        for row, partners in (chi1inv_offdiagonal or {}).items():
+            for partner, vol in (partners or {}).items():
+                arr = numpy.ascontiguousarray(numpy.broadcast_to(
+                    numpy.asarray(vol, dtype=numpy.float32), self.grid.shape))
+                if numpy.any(arr):
+                    self._offdiag.setdefault(row, {})[partner] = arr
+
+    def set_inverse_epsilon(self, inv: Dict[str, numpy.ndarray]):
+        self._require_unfrozen("set_inverse_epsilon")
+        for c in _E:
+            self._inv[c] = numpy.ascontiguousarray(inv[c], dtype=numpy.float32)
+
+    def add_susceptibility(self, term: Susceptibility, sigma):
+        self._require_unfrozen("add_susceptibility")
+        if not term.stable(self.grid.dt):
+            raise ValueError(f"{term} is unstable at dt={self.grid.dt}")
+        if not isinstance(sigma, dict):
+            sigma = {c: sigma for c in _E}
+        self._poles.append((term, dict(sigma)))
+
+    def setup_pml(self, thickness_cells, order: int = 2, R_asymptotic: float = 1e-15):
+        self._require_unfrozen("setup_pml")
+        probe = PML(self.grid, thickness_cells, order=order, R_asymptotic=R_asymptotic)
+        if not probe.is_active:
+            raise ValueError("a PML request that absorbs nowhere")
+        self._pml_spec = thickness_cells
+        self._pml_kwargs = dict(order=order, R_asymptotic=R_asymptotic)
+
+    def add_source(self, box: DepositBox, time_function):
+        self._require_unfrozen("add_source")
+        self._sources.append((box, time_function))
+
+    def add_flux_monitor(self, normal: int, lo: Sequence[int], cnt: Sequence[int],
+                         fcen: float = None, df: float = None, nfreq: int = None,
+                         frequencies: Sequence[float] = None, decimation: int = 1) -> FluxMonitor:
+        self._require_unfrozen("add_flux_monitor")
+        freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
+                 else meep_frequency_span(fcen, df, nfreq))
+        mon = FluxMonitor(int(normal), tuple(lo), tuple(cnt), freqs, self.grid.dx, self.grid.dt,
+                          decimation)
+        self.flux_monitors.append(mon)
+        return mon
+
+    @classmethod
+    def from_workload(cls, wl: Workload) -> "FdtdDriver":
+        drv = cls.__new__(cls)
+        drv.grid = wl.grid
+        drv._inv = dict(wl.inv)
+        drv._offdiag = {}
+        drv._poles = list(wl.poles)
+        drv._pml_spec = wl.pml_faces
+        drv._pml_kwargs = {}
+        drv._sources = list(wl.sources)
+        drv.flux_monitors = []
+        drv.max_steps = max(wl.steps, 1)
+        drv.sim = None
+        drv._planned = 0
+        for p in wl.planes:
+            drv.add_flux_monitor(p.normal, p.lo, p.cnt, frequencies=p.frequencies)
+        return drv
+
+    # --- the run -------------------------------------------------------------------------------
+    @property
+    def step_count(self) -> int:
+        return 0 if self.sim is None else self.sim.step_count
+
+    @property
+    def time(self) -> float:
+        return self.step_count * self.grid.dt
+
+    def freeze(self, planned_steps: Optional[int] = None):
+        """Create the device simulation and upload every table (the reference's first-step freeze)."""
+        if self.sim is not None:
+            return self.sim
+        steps = int(planned_steps or self.max_steps)
+        pml = PML(self.grid, self._pml_spec, **self._pml_kwargs) if self._pml_spec else None
+        sim = HipSimulation(self.grid, pml)
+        for c in _E:
+            sim.set_inverse_epsilon(c, self._inv[c])
+        for row, partners in self._offdiag.items():
+            for partner, vol in partners.items():
+                sim.set_offdiagonal(row, partner, vol)
+        for term, sigma in self._poles:
+            sim.add_pole(term, sigma)
+        for box, fn in self._sources:
+            table = current_table(fn, self.grid.dt, steps, box.magnetic)
+            axis = "xyz".index(box.component[1])
+            sim.add_source(box.magnetic, axis, box.lo, box.cnt, box.amp_re, box.amp_im, table,
+                           self.grid.dt)
+        for mon in self.flux_monitors:
+            mon.attach(sim)
+        if pml is None and not self._poles and not self._offdiag:
+            sim.prime_e()
+        self.sim = sim
+        self._planned = steps
+        return sim
+
+    def step(self):
+        sim = self.freeze()
+        if sim.step_count >= self._planned:
+            raise RuntimeError("the run is past its planned length; set max_steps higher")
+        sim.step_h()
+        sim.step_d()
+
+    def run(self, steps: Optional[int] = None, until: Optional[float] = None):
+        if (steps is None) == (until is None):
+            raise ValueError("give exactly one of steps or until")
+        if until is not None:
+            steps = int(math.ceil(until / self.grid.dt - 1e-9))
+        sim = self.freeze(max(int(steps) + self.step_count, self.max_steps))
+        if sim.step_count + steps > self._planned:
+            raise RuntimeError("the run is longer than the source tables planned at freeze")
+        sim.run(int(steps))
+
+    def synchronize(self):
+        if self.sim is not None:
+            self.sim.synchronize()
+
+    def get_array(self, component: str) -> numpy.ndarray:
+        """A stored volume in the reference's layout (C-order (nx, ny, nz))."""
+        sim = self.freeze()
+        if component[0] == "H" and sim.pml is None:
+            component = "B" + component[1]
+        return sim.download(component)
+
+    def fast_path_report(self) -> Dict[str, object]:
+        from .engine import device_info
+
+        return {"engine": "hip", "device": device_info(), "fused": ["step_B+update_H",
+                "step_D+update_E+update_P"], "library": "meep_gpu/_lib/libmeep_fdtd.so"}
+
+    def close(self):
+        if self.sim is not None:
+            self.sim.close()
+            self.sim = None

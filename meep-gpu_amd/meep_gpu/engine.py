@@ -163,6 +163,12 @@ class HipSimulation:
     def synchronize(self):
         check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
 
+    def run_timed(self, steps: int) -> Tuple[float, float, float]:
+        """(total ms, mean H-pair ms, mean D-pair ms) of ``steps`` steps, HIP events per launch."""
+        out = numpy.zeros(3, dtype=numpy.float32)
+        check(self.lib.fdtd_run_timed(self.handle, int(steps), fptr(out)), "fdtd_run_timed")
+        return float(out[0]), float(out[1]), float(out[2])
+
     def time_pairs(self, repeats: int) -> Tuple[float, float]:
         out = numpy.zeros(2, dtype=numpy.float32)
         check(self.lib.fdtd_time_pairs(self.handle, int(repeats), fptr(out)), "fdtd_time_pairs")
@@ -200,6 +206,22 @@ def device_info() -> str:
     return buf.value.decode()
 
 
-def set_launch(blocks_per_cu: int, xcd_remap: bool):
-    check(_native.load().fdtd_set_launch(int(blocks_per_cu), 1 if xcd_remap else 0),
-          "fdtd_set_launch")
+def copy_probe(nbytes: int, repeats: int = 20, mode: int = 0) -> float:
+    """Mean ms of a device copy of ``nbytes`` (profile calibration; mode as fdtd_copy_probe)."""
+    out = numpy.zeros(1, dtype=numpy.float32)
+    check(_native.load().fdtd_copy_probe(int(nbytes), int(repeats), int(mode), fptr(out)),
+          "fdtd_copy_probe")
+    return float(out[0])
+
+
+def set_launch(vec: int, blocks_per_cu: int, xcd_remap: bool, nt_stores: bool = False):
+    """Process-wide launch shape: cells per thread, workgroups per CU (0 = one per tile), XCD
+    remap, non-temporal output stores."""
+    check(_native.load().fdtd_set_launch(int(vec), int(blocks_per_cu), 1 if xcd_remap else 0,
+                                         1 if nt_stores else 0), "fdtd_set_launch")
+
+
+def get_launch() -> Tuple[int, int, bool, bool]:
+    out = numpy.zeros(4, dtype=numpy.int32)
+    check(_native.load().fdtd_get_launch(iptr(out)), "fdtd_get_launch")
+    return int(out[0]), int(out[1]), bool(out[2]), bool(out[3])

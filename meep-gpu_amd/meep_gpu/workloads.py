@@ -7,8 +7,8 @@ component, poles, deposit boxes with their time functions, flux planes -- that e
 * ``sphere``: a dielectric sphere (eps inside, vacuum outside) centred in a cubic periodic cell
   under a UPML on all six faces, an Ez Gaussian point source, x-normal or box-face flux planes.
   The chi1inv of each E component is sampled at that component's own Yee position with a
-  sub-pixel average: the fraction ``f`` of an 4x4x4 sub-sample of the component's cell that
-  falls inside the sphere gives ``chi1inv = f/eps + (1 - f)/1`` (diagonal only).
+  sub-pixel fill fraction ``f`` (from the signed distance to the surface) as
+  ``chi1inv = f/eps + (1 - f)/1`` (diagonal only).
 * ``slab``: a Lorentz-dispersive slab, normal x, spanning the transverse cell, a planar Ez
   Gaussian source, a reflection and a transmission plane.
 """
@@ -66,31 +66,25 @@ def meep_frequency_span(fcen: float, df: float, nfreq: int) -> numpy.ndarray:
     return numpy.linspace(fcen - 0.5 * df, fcen + 0.5 * df, nfreq)
 
 
-def _fill_fraction(grid: Grid, component: str, inside, samples: int = 4) -> numpy.ndarray:
-    """Fraction of each Yee cell of ``component`` inside a region, by sub-sampling."""
-    nx, ny, nz = grid.shape
-    dx = grid.dx
+def _yee_coordinates(grid: Grid, component: str):
+    """Physical coordinates (origin at the cell centre) of every sample of a component."""
     shift = IYEE[component]
-    origin = [-0.5 * n * dx for n in grid.shape]
-    sub = (numpy.arange(samples) + 0.5) / samples - 0.5
-    frac = numpy.zeros(grid.shape, dtype=numpy.float64)
-    coords = []
+    axes = []
     for axis, n in enumerate(grid.shape):
-        centre = origin[axis] + (numpy.arange(n) + 0.5 * shift[axis]) * dx
-        coords.append(centre)
-    X, Y, Z = numpy.meshgrid(*coords, indexing="ij")
-    for a in sub:
-        for b in sub:
-            for c in sub:
-                frac += inside(X + a * dx, Y + b * dx, Z + c * dx)
-    return frac / samples ** 3
+        axes.append(-0.5 * n * grid.dx + (numpy.arange(n) + 0.5 * shift[axis]) * grid.dx)
+    return axes
 
 
-def _inverse_epsilon(grid: Grid, inside, eps_in: float, eps_out: float = 1.0
-                     ) -> Dict[str, numpy.ndarray]:
+def _sphere_inverse_epsilon(grid: Grid, radius: float, eps_in: float, eps_out: float = 1.0
+                            ) -> Dict[str, numpy.ndarray]:
+    """Sub-pixel chi1inv of a sphere: the fill fraction of each Yee cell from the signed
+    distance to the surface, f = clip(1/2 - d/dx, 0, 1) (a locally planar interface through
+    the cell), then chi1inv = f/eps_in + (1 - f)/eps_out."""
     out = {}
     for c in _E:
-        f = _fill_fraction(grid, c, inside)
+        x, y, z = _yee_coordinates(grid, c)
+        r = numpy.sqrt(x[:, None, None] ** 2 + y[None, :, None] ** 2 + z[None, None, :] ** 2)
+        f = numpy.clip(0.5 - (r - radius) / grid.dx, 0.0, 1.0)
         out[c] = (f / eps_in + (1.0 - f) / eps_out).astype(numpy.float32)
     return out
 
@@ -109,13 +103,13 @@ def _ez_point(grid: Grid, x: float, y: float, z: float) -> DepositBox:
 def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
            radius_cells: float = 16.0, eps: float = 2.25, fcen: float = 0.15, df: float = 0.1,
            nfreq: int = 50, planes: str = "one", steps: int = 2000,
-           source_x: float = -2.0) -> Workload:
+           source_x: float = -2.0, plane_x: float = 2.0) -> Workload:
     """The dielectric sphere under a UPML (configs 1 and 2 of the benchmark table)."""
     L = edge / resolution
     grid = Grid(resolution=resolution, cell_size=(L, L, L))
     assert grid.shape == (edge, edge, edge), grid.shape
     r = radius_cells / resolution
-    inv = _inverse_epsilon(grid, lambda X, Y, Z: (X * X + Y * Y + Z * Z) < r * r, eps)
+    inv = _sphere_inverse_epsilon(grid, r, eps)
     freqs = meep_frequency_span(fcen, df, nfreq)
     src = GaussianSource(fcen, df)
     wl = Workload(f"sphere_{edge}", grid, pml_cells, inv, steps=steps)
@@ -123,7 +117,9 @@ def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
     c = edge // 2
     if planes == "one":
         half = edge // 4
-        i = int(round((2.0 + L / 2) * resolution))
+        i = int(round((plane_x + L / 2) * resolution))
+        if not pml_cells <= i < edge - pml_cells:
+            raise ValueError(f"flux plane at x={plane_x} falls outside the interior")
         wl.planes.append(FluxPlane(0, (i, c - half, c - half), (1, 2 * half, 2 * half), freqs))
     elif planes == "box8":
         b = int(round(radius_cells * 1.25))

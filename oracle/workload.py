@@ -1,0 +1,114 @@
+"""Step a synthetic workload on the oracle, with sources and flux planes (test infrastructure).
+
+Takes the plain description a workload carries (grid shape, dt/dx, absorber faces, inverse
+permittivity volumes, poles, deposit boxes with their current tables, flux planes) and
+runs the reference driver's order (driver.py:3419-3596) on the C restatement, accumulating
+each flux plane after every step as ``FluxMonitor.update`` does (dft.py:3031-3063): the four
+tangential components sampled at cell centres (dft.py:352), E at t and H at t - dt/2 with
+``t = step_count*dt`` after the increment, phase ``exp(i*omega*t)*dt/sqrt(2*pi)``.
+
+Two accumulations are kept per plane: the reference's own precision (complex64 phase times
+float32 sample added in float32, per part) and a float64 one, the yardstick for the engine's
+float64 transform. The flux is ``sum Re(E1*conj(H2) - E2*conj(H1)) * dx**2`` over the plane
+(dft.py:3107-3130 with unit interior weights).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, List, Sequence
+
+import numpy
+
+from . import engine as oe
+
+_IYEE_MASK = {"Ex": 1, "Ey": 2, "Ez": 4, "Hx": 6, "Hy": 5, "Hz": 3}  # bit a: Yee shift 1 on axis a
+_AX = "xyz"
+
+
+class OraclePlane:
+    def __init__(self, normal: int, lo, cnt, omegas: numpy.ndarray, scale: float):
+        self.normal = int(normal)
+        self.lo = tuple(int(v) for v in lo)
+        self.cnt = tuple(int(v) for v in cnt)
+        self.omegas = numpy.asarray(omegas, dtype=numpy.float64)
+        self.scale = float(scale)
+        t1, t2 = (self.normal + 1) % 3, (self.normal + 2) % 3
+        self.components = ("E" + _AX[t1], "E" + _AX[t2], "H" + _AX[t1], "H" + _AX[t2])
+        cells = int(numpy.prod(self.cnt))
+        nf = self.omegas.size
+        self.acc64 = numpy.zeros((4, nf, cells), dtype=numpy.complex128)
+        self.acc32 = numpy.zeros((4, nf, cells), dtype=numpy.complex64)
+
+    def sample(self, sim: oe.OracleSim, name: str, field: numpy.ndarray) -> numpy.ndarray:
+        g = sim._grid()
+        out = numpy.empty(self.cnt, dtype=numpy.float32)
+        shift = (ctypes.c_int * 3)(*[(_IYEE_MASK[name] >> a) & 1 for a in range(3)])
+        lo = (ctypes.c_int * 3)(*self.lo)
+        cnt = (ctypes.c_int * 3)(*self.cnt)
+        oe.lib().oracle_centre_sample(ctypes.byref(g), oe._fp(field), shift, lo, cnt, oe._fp(out))
+        return out.reshape(-1)
+
+    def update(self, sim: oe.OracleSim, time_value: float, reference_precision: bool = True):
+        dt = sim.dt
+        cells = int(numpy.prod(self.cnt))
+        for q, name in enumerate(self.components):
+            field = sim.get_E(name) if name[0] == "E" else sim.v["H" + name[1]] if sim.pml \
+                else sim.v["B" + name[1]]
+            s = self.sample(sim, name, numpy.ascontiguousarray(field))
+            t = time_value if name[0] == "E" else time_value - 0.5 * dt
+            phase = numpy.exp(1j * self.omegas * t) * self.scale
+            ph64 = numpy.ascontiguousarray(phase.astype(numpy.complex128)).view(numpy.float64)
+            acc = self.acc64[q]
+            oe.lib().oracle_dft_accumulate_c128(
+                acc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                ph64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), oe._fp(s),
+                ctypes.c_int(self.omegas.size), ctypes.c_int64(cells))
+            if reference_precision:
+                p32 = numpy.ascontiguousarray(phase.astype(numpy.complex64)).view(numpy.float32)
+                a32 = self.acc32[q]
+                oe.lib().oracle_dft_accumulate_c64(
+                    a32.view(numpy.float32).ctypes.data_as(oe._FP), p32.ctypes.data_as(oe._FP),
+                    oe._fp(s), ctypes.c_int(self.omegas.size), ctypes.c_int64(cells))
+
+    def flux(self, dx: float, precision: str = "c128") -> numpy.ndarray:
+        acc = self.acc64 if precision == "c128" else self.acc32.astype(numpy.complex128)
+        e1, e2, h1, h2 = acc
+        return numpy.sum(numpy.real(e1 * numpy.conj(h2) - e2 * numpy.conj(h1)), axis=1) * dx * dx
+
+
+def build(shape, dt, dx, faces, inv: Dict[str, numpy.ndarray], poles: Sequence, deposits,
+          planes, invariant=(False, False, False), ln_reflection=None):
+    """An ``OracleSim`` plus its planes. ``poles``: (frequency, gamma, drude, sigma dict);
+    ``deposits``: (component, lo, cnt, amp_re, amp_im, current (steps, 2)); ``planes``:
+    (normal, lo, cnt, omegas, scale)."""
+    sim = oe.OracleSim(shape, dt, dx, invariant, faces, ln_reflection=ln_reflection)
+    sim.set_inverse_epsilon(inv)
+    for frequency, gamma, drude, sigma in poles:
+        sim.add_pole(frequency, gamma, drude, sigma)
+    nx, ny, nz = shape
+    for component, lo, cnt, amp_re, amp_im, current in deposits:
+        ii, jj, kk = numpy.meshgrid(*[numpy.arange(l, l + c) for l, c in zip(lo, cnt)],
+                                    indexing="ij")
+        flat = ((ii * ny + jj) * nz + kk).reshape(-1).astype(numpy.int64)
+        table = numpy.asarray(current, dtype=numpy.float64).reshape(-1, 2)
+
+        def cur(t, table=table, magnetic=component[0] in "HB"):
+            n = int(math.floor(t / dt + 0.25))
+            return complex(table[n, 0], table[n, 1]) if 0 <= n < len(table) else 0j
+
+        sim.add_deposit(oe.PointDeposit(component, flat,
+                                        numpy.asarray(amp_re, dtype=numpy.float64).reshape(-1),
+                                        numpy.asarray(amp_im, dtype=numpy.float64).reshape(-1),
+                                        cur))
+    out_planes = [OraclePlane(*p) for p in planes]
+    return sim, out_planes
+
+
+def run(sim: oe.OracleSim, planes: List[OraclePlane], steps: int,
+        reference_precision: bool = True):
+    for _ in range(steps):
+        sim.step()
+        for p in planes:
+            p.update(sim, sim.time, reference_precision)

@@ -1,0 +1,69 @@
+"""Batched and launch-shape invariance (GPU): a simulation steps to the same words alone, in a
+batch of uneven grids, and under every launch shape the autotuner may pick."""
+
+import numpy
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 0, True, False), (2, 0, True, True), (4, 0, False, False), (2, 8, True, False),
+          (4, 4, True, True), (1, 8, False, True)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"v{s[0]}b{s[1]}r{int(s[2])}n{int(s[3])}")
+def test_every_launch_shape_reproduces_the_recorded_digests(shape, golden_digests):
+    from meep_gpu.engine import get_launch, set_launch
+    from oracle import cases
+    from tests import hip_cases
+
+    saved = get_launch()
+    set_launch(*shape)
+    try:
+        for name in ("pml_3d", "offdiag_pml_3d", "dispersive_pml_2d", "no_pml_3d"):
+            print(f"shape {shape} case {name}", flush=True)
+            digest, _ = hip_cases.hip_case_digest(cases.build_state(name))
+            assert digest == golden_digests["case_sha256"][name], (shape, name)
+    finally:
+        set_launch(*saved)
+
+
+def _members():
+    from meep_gpu.batch import synthetic_batch
+
+    return synthetic_batch(count=5, seed=7, edge_range=(28, 44), steps=150, nfreq=6,
+                           lorentz_fraction=0.5)
+
+
+def test_each_member_steps_the_same_alone_and_in_the_batch():
+    from meep_gpu.batch import BatchedRun
+    from meep_gpu.driver import FdtdDriver
+
+    steps = 150
+    alone = []
+    for wl in _members():
+        drv = FdtdDriver.from_workload(wl)
+        drv.freeze(planned_steps=steps + 1)
+        drv.run(steps=steps)
+        drv.synchronize()
+        alone.append(drv)
+    together = []
+    for wl in _members():
+        drv = FdtdDriver.from_workload(wl)
+        drv.freeze(planned_steps=steps + 1)
+        together.append(drv)
+    batch = BatchedRun(together)
+    batch.run(100)
+    batch.run(steps - 100)
+    batch.synchronize()
+    batch.close()
+    for a, b in zip(alone, together):
+        assert b.step_count == steps
+        for comp in ("Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Bx", "By", "Bz", "Hx", "Hy", "Hz"):
+            x, y = a.sim.download(comp), b.sim.download(comp)
+            assert x.view(numpy.uint32).tobytes() == y.view(numpy.uint32).tobytes(), comp
+        for ma, mb in zip(a.flux_monitors, b.flux_monitors):
+            fa, fb = ma.get_flux_spectrum(), mb.get_flux_spectrum()
+            assert numpy.array_equal(fa, fb)
+            assert numpy.max(numpy.abs(fa)) > 0
+    for d in alone + together:
+        d.close()

@@ -1,0 +1,50 @@
+"""Small workloads stepped on the HIP engine and on the oracle from identical inputs."""
+
+from __future__ import annotations
+
+import numpy
+
+from meep_gpu import workloads
+from meep_gpu.dft import FluxMonitor
+from meep_gpu.driver import FdtdDriver
+from meep_gpu.pml import normalize_faces
+from meep_gpu.sources import current_table
+from oracle import workload as ow
+
+
+def small(name: str):
+    if name == "sphere":
+        return workloads.sphere(edge=36, resolution=4.0, pml_cells=6, radius_cells=7.0,
+                                eps=2.25, fcen=0.15, df=0.1, nfreq=8, planes="one",
+                                steps=160, source_x=-2.0)
+    if name == "slab":
+        return workloads.slab(edge=40, resolution=8.0, pml_cells=6, thickness=6, nfreq=9,
+                              steps=160)
+    if name == "sphere_box":
+        return workloads.sphere(edge=40, resolution=5.0, pml_cells=6, radius_cells=8.0,
+                                nfreq=6, planes="box8", steps=120)
+    raise KeyError(name)
+
+
+def oracle_run(wl, steps: int, reference_precision: bool = True):
+    grid = wl.grid
+    faces = normalize_faces(wl.pml_faces, grid) if wl.pml_faces else ((0, 0),) * 3
+    deposits = [(b.component, b.lo, b.cnt, b.amp_re, b.amp_im,
+                 current_table(fn, grid.dt, steps + 1, b.magnetic)) for b, fn in wl.sources]
+    planes = []
+    for p in wl.planes:
+        mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
+        planes.append((p.normal, p.lo, p.cnt, mon.omegas, mon.scale))
+    poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in wl.poles]
+    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, wl.inv, poles, deposits, planes,
+                            grid.invariant)
+    ow.run(sim, oplanes, steps, reference_precision)
+    return sim, oplanes
+
+
+def hip_run(wl, steps: int):
+    drv = FdtdDriver.from_workload(wl)
+    drv.freeze(planned_steps=steps + 1)
+    drv.run(steps=steps)
+    drv.synchronize()
+    return drv
