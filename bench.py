@@ -86,6 +86,88 @@ def cpu_baseline(wl, seconds: float) -> dict:
                       f"incl. the {len(wl.planes)} flux planes in float64, {elapsed:.1f} s"}
 
 
+def step_bytes(sim) -> float:
+    b = algorithmic_bytes(sim)
+    return float(b["h"] + b["d"])
+
+
+def batched_line(count: int, steps: int, seed: int) -> dict:
+    """The batched form: ``count`` uneven synthetic simulations stepped by shared launches,
+    against the same simulations stepped one at a time."""
+    from meep_gpu import autotune
+    from meep_gpu.batch import BatchedRun, synthetic_batch
+    from meep_gpu.driver import FdtdDriver
+
+    t_build = time.perf_counter()
+    wls = synthetic_batch(count=count, seed=seed, steps=steps)
+    drivers = []
+    for wl in wls:
+        drv = FdtdDriver.from_workload(wl)
+        drv.freeze(planned_steps=2 * steps + 2 * 64 + 2 * 16 * len(autotune.CANDIDATES) + 100)
+        drivers.append(drv)
+    build_s = time.perf_counter() - t_build
+    cells = sum(d.grid.total_cells for d in drivers)
+    batch = BatchedRun(drivers)
+    shape = autotune.tune_batch(batch, steps=16)
+    batch.run(64)  # warm: first capture of the step graph
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    batch.synchronize()
+    batched_s = time.perf_counter() - t0
+    batch.close()
+    nbytes = sum(step_bytes(d.sim) for d in drivers)
+    # one at a time: each simulation alone, in turn, with its own autotuned shape
+    alone_s = 0.0
+    for q, d in enumerate(drivers):
+        if q % 16 == 0:
+            print(f"bench: one at a time, simulation {q}/{len(drivers)}", file=sys.stderr,
+                  flush=True)
+        autotune.tune(d.grid, d.sim.pml, [], repeats=3)
+        d.sim.run(64)
+        d.sim.synchronize()
+        t0 = time.perf_counter()
+        d.sim.run(steps)
+        d.sim.synchronize()
+        alone_s += time.perf_counter() - t0
+    finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum()))
+                 for d in drivers for m in d.flux_monitors)
+    for d in drivers:
+        d.close()
+    value = cells * steps / batched_s / 1e6
+    alone = cells * steps / alone_s / 1e6
+    achieved = nbytes * steps / batched_s / 1e9
+    return {"workload": f"{count} synthetic sphere simulations (seed {seed}), edges 32-112 "
+                        f"log-uniform, 25% Lorentz, UPML 8, 1 flux plane x 32 freqs",
+            "cells": cells, "steps": steps, "value": round(value, 1), "unit": "Mcell-steps/s",
+            "ms_per_step": round(batched_s * 1e3 / steps, 4),
+            "one_at_a_time": round(alone, 1), "speedup_vs_one_at_a_time": round(value / alone, 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "note": "algorithmic bytes of both fused pairs per step, all members"},
+            "launch_shape": list(shape), "graphs": True, "spectra_finite": bool(finite),
+            "host_build_s": round(build_s, 2)}
+
+
+def served_line(count: int, rate: float, seed: int) -> dict:
+    """The served form: a Poisson stream of solves under continuous batching, beside the same
+    stream served one solve at a time (one slot)."""
+    from meep_gpu.serve import SolveServer, poisson_requests
+
+    reqs = poisson_requests(count=count, rate=rate, seed=seed)
+    rep = SolveServer(max_slots=64, quantum=256).run(reqs)
+    one = SolveServer(max_slots=1, quantum=256).run(reqs)
+    s, o = rep.summary(), one.summary()
+    return {"workload": f"{count} Poisson arrivals at {rate}/s (seed {seed}): sphere solves, "
+                        f"edge 32-160 log-uniform, 30% Lorentz, 1-8 planes x 16-100 freqs, "
+                        f"1000-8000 steps",
+            "cell_steps": rep.cell_steps, **s,
+            "one_at_a_time": {k: o[k] for k in ("p50_latency_s", "p99_latency_s",
+                                                 "aggregate_mcell_steps_per_s", "makespan_s")},
+            "speedup_vs_one_at_a_time": round(s["aggregate_mcell_steps_per_s"]
+                                              / max(o["aggregate_mcell_steps_per_s"], 1e-9), 2)}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +176,14 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="sphere192")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=3000)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT",
+                    help="fixed launch shape instead of the autotuned one (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="time a plain device copy of BYTES (profile counter calibration)")
     args = ap.parse_args(argv)
@@ -117,9 +207,16 @@ def main(argv=None) -> int:
     from meep_gpu import autotune
     from meep_gpu.pml import PML
 
+    from meep_gpu.engine import set_launch
+
     wl = workloads.config(args.workload)
-    shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
-                          wl.poles, use_cache=False)
+    if args.launch:
+        v, bpc, remap, nt = (int(x) for x in args.launch.split(","))
+        shape = (v, bpc, bool(remap), bool(nt))
+        set_launch(*shape)
+    else:
+        shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
+                              wl.poles, use_cache=False)
     drv = FdtdDriver.from_workload(wl)
     sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
     print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
@@ -163,9 +260,15 @@ def main(argv=None) -> int:
                      "step_model_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2)},
         "spectra_finite": bool(spectra_ok),
         "launch_shape": {"cells_per_thread": shape[0], "blocks_per_cu": shape[1],
-                         "xcd_remap": bool(shape[2])},
+                         "xcd_remap": bool(shape[2]), "nt_stores": bool(shape[3])},
     }
     drv.close()
+    if not args.no_batched:
+        print("bench: batched form", file=sys.stderr, flush=True)
+        line["batched"] = batched_line(args.batch_count, args.batch_steps, 20261203)
+    if not args.no_served:
+        print("bench: served form", file=sys.stderr, flush=True)
+        line["served"] = served_line(args.serve_count, args.serve_rate, 20261203)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(line), flush=True)

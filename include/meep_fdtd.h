@@ -50,6 +50,10 @@ const char *fdtd_last_error(void);
 /* Engine build and device description, for reports. */
 int fdtd_device_info(char *text, size_t size);
 
+/* Free and total device memory of the current GPU (hipMemGetInfo), for footprint admission of
+ * served solves. No counterpart in the reference (it runs one simulation per process). */
+int fdtd_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* Create a simulation: D, B, E always; H, fu_B, fu_D, f_w_H, f_w_E when cfg->pml.
  * Replaces Fields allocation (fields.py:605-760) and enable_pml_storage (fields.py:708).
  * All volumes start at +0; inverse epsilon starts at 1. */
@@ -149,6 +153,13 @@ int fdtd_batch_synchronize(fdtd_batch *batch);
  * in the reference (its launches are one thread per cell). */
 int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores);
 int fdtd_get_launch(int32_t out[4]);
+
+/* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 64 whole
+ * steps (both fused pairs, the counter advance, flux sampling and the transform passes that fall
+ * due) wherever every flux plane samples each step; 0 launches every step eagerly. Results are
+ * the same words either way. Process-wide. No counterpart in the reference (its host loop
+ * launches each sub-step). */
+int fdtd_set_graphs(int32_t enable);
 
 /* Measure average kernel durations over `repeats` launches of each fused pair on the
  * simulation's stream with HIP events: out_ms[0] = H pair, out_ms[1] = D pair. */

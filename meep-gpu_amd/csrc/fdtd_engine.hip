@@ -37,7 +37,7 @@ constexpr int kThreads = 256;   // threads per workgroup
 constexpr int kPad = 4;         // row pitch multiple (floats)
 constexpr int kMaxPoles = FDTD_MAX_POLES;
 constexpr int kRing = 64;       // flux samples buffered before one float64 transform pass
-constexpr int kMaxRuns = 4;     // runs of a layer membership line held in the descriptor
+constexpr int kMaxRuns = 2;     // runs of a layer membership line held in the descriptor
 
 thread_local std::string g_error;
 
@@ -323,11 +323,10 @@ __device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool 
 // ------------------------------------------------------------------------------------------
 template <int V, int C, bool PML, bool NT>
 __device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V> (&e)[3],
-                                       const VF<V> (&en)[3][3]) {
+                                       const VF<V> (&en)[3][3], VF<V> b) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
     constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
-    VF<V> b = ldv<V>(s.B[C], q.at);
 #pragma unroll
     for (int m = 0; m < V; ++m) {
         const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
@@ -379,9 +378,14 @@ __device__ inline void hpair_run(const SimDev &s, const Run &q) {
     en[1][0] = x_up<V>(s, q, s.E[1], e[1]);
     en[2][0] = x_up<V>(s, q, s.E[2], e[2]);
     en[2][1] = ldv<V>(s.E[2], q.at + q.yp);
-    hpair_component<V, 0, PML, NT>(s, q, e, en);
-    hpair_component<V, 1, PML, NT>(s, q, e, en);
-    hpair_component<V, 2, PML, NT>(s, q, e, en);
+    // every stream's load issued before the first store: the stores may alias as far as the
+    // compiler knows, so loads left behind them would wait a memory round trip each
+    VF<V> b[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[c] = ldv<V>(s.B[c], q.at);
+    hpair_component<V, 0, PML, NT>(s, q, e, en, b[0]);
+    hpair_component<V, 1, PML, NT>(s, q, e, en, b[1]);
+    hpair_component<V, 2, PML, NT>(s, q, e, en, b[2]);
 }
 
 template <int V, bool NT>
@@ -407,11 +411,10 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
 // ------------------------------------------------------------------------------------------
 template <int V, int C, bool PML>
 __device__ inline VF<V> dcurl_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
-                                        const VF<V> (&hn)[3][3]) {
+                                        const VF<V> (&hn)[3][3], VF<V> d) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
     constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
-    VF<V> d = ldv<V>(s.D[C], q.at);
 #pragma unroll
     for (int m = 0; m < V; ++m) {
         const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
@@ -507,12 +510,12 @@ __device__ inline VF<V> minus_poles(const SimDev &s, const Run &q, const VF<V> &
 
 template <int V, int C, bool PML, bool NT>
 __device__ inline void dpair_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
-                                       const VF<V> (&hn)[3][3]) {
-    const VF<V> d = dcurl_component<V, C, PML>(s, q, h, hn);
+                                       const VF<V> (&hn)[3][3], const VF<V> &d0,
+                                       const VF<V> &iv) {
+    const VF<V> d = dcurl_component<V, C, PML>(s, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
     VF<V> p[kMaxPoles];
     const VF<V> dmp = minus_poles<V, C>(s, q, d, p);
-    const VF<V> iv = ldv<V>(s.inv[C], q.at);
     VF<V> cst;
 #pragma unroll
     for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
@@ -565,9 +568,16 @@ template <int V, bool PML, bool NT>
 __device__ inline void dpair_run(const SimDev &s, const Run &q) {
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
-    dpair_component<V, 0, PML, NT>(s, q, h, hn);
-    dpair_component<V, 1, PML, NT>(s, q, h, hn);
-    dpair_component<V, 2, PML, NT>(s, q, h, hn);
+    // every stream's load issued before the first store (see hpair_run)
+    VF<V> d[3], iv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        d[c] = ldv<V>(s.D[c], q.at);
+        iv[c] = ldv<V>(s.inv[c], q.at);
+    }
+    dpair_component<V, 0, PML, NT>(s, q, h, hn, d[0], iv[0]);
+    dpair_component<V, 1, PML, NT>(s, q, h, hn, d[1], iv[1]);
+    dpair_component<V, 2, PML, NT>(s, q, h, hn, d[2], iv[2]);
 }
 
 template <int V, bool NT>
@@ -595,9 +605,12 @@ template <int V, bool PML>
 __device__ inline void dcurl_run(const SimDev &s, const Run &q) {
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
-    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, q, h, hn));
-    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, q, h, hn));
-    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, q, h, hn));
+    VF<V> d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
+    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, q, h, hn, d[0]));
+    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, q, h, hn, d[1]));
+    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, q, h, hn, d[2]));
 }
 
 template <int V>
@@ -1041,6 +1054,28 @@ int launch_d(const Group &g, hipStream_t st) {
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
+// A captured sequence of kRing steps (fused pairs, counter advance, flux sampling and the
+// transform passes that fall due), replayed while the launch shape and every plane's ring
+// phase are what they were at capture. Plane rings fill once per kRing steps when every plane
+// samples each step, so one capture serves every later period.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> phase;
+    LaunchShape shape;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        phase.clear();
+    }
+    bool matches(const std::vector<int> &ph) const {
+        return exec && ph == phase && shape.vec == g_shape.vec &&
+               shape.blocks_per_cu == g_shape.blocks_per_cu &&
+               shape.xcd_remap == g_shape.xcd_remap && shape.nt == g_shape.nt;
+    }
+    ~StepGraph() { reset(); }
+};
+
+int g_graphs = 1;  // replay captured step sequences where eligible
 }  // namespace
 
 struct fdtd_sim {
@@ -1057,6 +1092,7 @@ struct fdtd_sim {
     uint64_t bytes = 0;
     bool dirty = true;
     bool in_batch = false;
+    StepGraph graph;
 
     int alloc(void **p, size_t bytes_) {
         hipError_t e = hipMalloc(p, bytes_);
@@ -1118,6 +1154,7 @@ struct fdtd_batch {
     int nplanes = 0;
     int max_plane_cells = 0;
     hipStream_t stream = nullptr;
+    StepGraph graph;
 };
 
 namespace {
@@ -1138,44 +1175,168 @@ bool samples_now(const PlaneHost &ph, int step_after) {
     return since > 0 && since % ph.dev.dec == 0;
 }
 
-// Flux bookkeeping after a step: sample every plane in one launch, then a transform pass for
-// each plane whose ring is full.
-int flux_after_step(fdtd_sim *s, hipStream_t st) {
-    if (s->planes.empty()) return FDTD_OK;
+void mirror_advance(fdtd_sim *s) {
+    s->host.step += 1;  // mirror of the device counter the advance launch moved
+    s->host.hsep = 1;
+    if (s->host.npol) s->host.ppar ^= 1;
+}
+
+// The simulations one launch sequence steps (one, or a batch) with their flux planes, in
+// descriptor order: simulation q's planes follow simulation q-1's in `planes`.
+struct Stepper {
+    Group g;
+    fdtd_sim *const *sims = nullptr;
+    int nsims = 0;
+    PlaneDev *planes = nullptr;
+    int nplanes = 0;
+    int max_plane_cells = 0;
+    hipStream_t st = nullptr;
+    StepGraph *graph = nullptr;
+};
+
+// Flux bookkeeping after a step: one sampling launch for every plane that samples now, then a
+// transform pass for each ring that is full. With launch == false only the host mirrors move
+// (the step itself runs inside a replayed graph).
+int flux_after_step(const Stepper &k, bool launch) {
+    if (!k.nplanes) return FDTD_OK;
     bool any = false;
-    for (auto &ph : s->planes) any |= samples_now(ph, s->host.step);
+    for (int q = 0; q < k.nsims; ++q)
+        for (auto &ph : k.sims[q]->planes) any |= samples_now(ph, k.sims[q]->host.step);
     if (!any) return FDTD_OK;
-    const int np = (int)s->planes.size();
-    flux_sample_kernel<<<dim3((s->max_plane_cells + 255) / 256, np), 256, 0, st>>>(s->dev_planes, np);
-    HIP_TRY(hipGetLastError());
-    for (int q = 0; q < np; ++q) {
-        PlaneHost &ph = s->planes[q];
-        if (!samples_now(ph, s->host.step)) continue;
-        ph.filled += 1;
-        if (ph.filled == kRing) {
-            int rc = flux_flush_plane(s->dev_planes + q, ph, st);
+    if (launch) {
+        flux_sample_kernel<<<dim3((k.max_plane_cells + 255) / 256, k.nplanes), 256, 0, k.st>>>(
+            k.planes, k.nplanes);
+        HIP_TRY(hipGetLastError());
+    }
+    int pi = 0;
+    for (int q = 0; q < k.nsims; ++q)
+        for (size_t r = 0; r < k.sims[q]->planes.size(); ++r, ++pi) {
+            PlaneHost &ph = k.sims[q]->planes[r];
+            if (!samples_now(ph, k.sims[q]->host.step)) continue;
+            ph.filled += 1;
+            if (ph.filled < kRing) continue;
+            if (!launch) {
+                ph.filled = 0;
+                continue;
+            }
+            int rc = flux_flush_plane(k.planes + pi, ph, k.st);
             if (rc) return rc;
         }
+    return FDTD_OK;
+}
+
+int step_once(const Stepper &k, bool launch) {
+    if (launch) {
+        int rc = launch_h(k.g, k.st);
+        if (rc) return rc;
+        rc = launch_d(k.g, k.st);
+        if (rc) return rc;
+    }
+    for (int q = 0; q < k.nsims; ++q) mirror_advance(k.sims[q]);
+    return flux_after_step(k, launch);
+}
+
+struct Mirror {
+    std::vector<SimDev> descs;
+    std::vector<int> fills;
+};
+
+Mirror save_mirror(const Stepper &k) {
+    Mirror m;
+    for (int q = 0; q < k.nsims; ++q) {
+        m.descs.push_back(k.sims[q]->host);
+        for (auto &ph : k.sims[q]->planes) m.fills.push_back(ph.filled);
+    }
+    return m;
+}
+
+void restore_mirror(const Stepper &k, const Mirror &m) {
+    size_t f = 0;
+    for (int q = 0; q < k.nsims; ++q) {
+        k.sims[q]->host = m.descs[q];
+        for (auto &ph : k.sims[q]->planes) ph.filled = m.fills[f++];
+    }
+}
+
+// Capture kRing steps into k.graph and launch it once (the host mirrors advance as the
+// capture walks the steps). On failure nothing has run and the mirrors are put back.
+int capture_period(const Stepper &k, const std::vector<int> &phase) {
+    k.graph->reset();
+    (void)num_cu();  // device query outside the capture
+    const Mirror before = save_mirror(k);
+    HIP_TRY(hipStreamBeginCapture(k.st, hipStreamCaptureModeThreadLocal));
+    int rc = FDTD_OK;
+    for (int t = 0; t < kRing && rc == FDTD_OK; ++t) rc = step_once(k, true);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(k.st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (rc == FDTD_OK && e != hipSuccess)
+        rc = fail(FDTD_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc == FDTD_OK) {
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(FDTD_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != FDTD_OK) {
+        restore_mirror(k, before);
+        return rc;
+    }
+    k.graph->exec = exec;
+    k.graph->phase = phase;
+    k.graph->shape = g_shape;
+    HIP_TRY(hipGraphLaunch(exec, k.st));
+    return FDTD_OK;
+}
+
+// nsteps whole steps: replayed kRing-step graphs where every plane samples each step, eager
+// launches for the rest.
+int run_steps(const Stepper &k, int nsteps) {
+    int done = 0;
+    bool eligible = g_graphs && k.graph && nsteps >= kRing;
+    std::vector<int> phase;
+    for (int q = 0; q < k.nsims && eligible; ++q)
+        for (auto &ph : k.sims[q]->planes) {
+            eligible &= ph.dev.dec == 1;
+            phase.push_back(ph.filled);
+        }
+    if (eligible) {
+        if (!k.graph->matches(phase)) {
+            int rc = capture_period(k, phase);
+            if (rc) return rc;
+            done = kRing;
+        }
+        for (; done + kRing <= nsteps; done += kRing) {
+            for (int t = 0; t < kRing; ++t) {
+                int rc = step_once(k, false);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipGraphLaunch(k.graph->exec, k.st));
+        }
+    }
+    for (; done < nsteps; ++done) {
+        int rc = step_once(k, true);
+        if (rc) return rc;
     }
     return FDTD_OK;
 }
 
-void mirror_advance(fdtd_sim *s) {
-    s->host.step += 1;  // mirror of the device counter the D pair's last workgroup advanced
-    s->host.hsep = 1;
-    if (s->host.npol) s->host.ppar ^= 1;
+Stepper lone(fdtd_sim *&s) {
+    Stepper k;
+    k.g = s->group();
+    k.sims = &s;
+    k.nsims = 1;
+    k.planes = s->dev_planes;
+    k.nplanes = (int)s->planes.size();
+    k.max_plane_cells = s->max_plane_cells;
+    k.st = s->stream;
+    k.graph = &s->graph;
+    return k;
 }
 
 int sim_step(fdtd_sim *s) {
     int rc = s->sync_desc();
     if (rc) return rc;
-    const Group g = s->group();
-    rc = launch_h(g, s->stream);
-    if (rc) return rc;
-    rc = launch_d(g, s->stream);
-    if (rc) return rc;
-    mirror_advance(s);
-    return flux_after_step(s, s->stream);
+    return step_once(lone(s), true);
 }
 }  // namespace
 
@@ -1190,6 +1351,15 @@ int fdtd_device_info(char *text, size_t size) {
     HIP_TRY(hipGetDeviceProperties(&prop, dev));
     snprintf(text, size, "%s gcn=%s cu=%d l2=%d", prop.name, prop.gcnArchName,
              prop.multiProcessorCount, prop.l2CacheSize);
+    return FDTD_OK;
+}
+
+int fdtd_device_memory(uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(FDTD_EINVAL, "null argument");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
     return FDTD_OK;
 }
 
@@ -1580,16 +1750,19 @@ int fdtd_step_d(fdtd_sim *s) {
     rc = launch_d(s->group(), s->stream);
     if (rc) return rc;
     mirror_advance(s);
-    return flux_after_step(s, s->stream);
+    return flux_after_step(lone(s), true);
 }
 
 int fdtd_run(fdtd_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    for (int q = 0; q < nsteps; ++q) {
-        int rc = sim_step(s);
-        if (rc) return rc;
-    }
+    int rc = s->sync_desc();
+    if (rc) return rc;
+    return run_steps(lone(s), nsteps);
+}
+
+int fdtd_set_graphs(int32_t enable) {
+    g_graphs = enable ? 1 : 0;
     return FDTD_OK;
 }
 
@@ -1605,6 +1778,7 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
         return fail(FDTD_EINVAL, "fdtd_add_flux_plane: bad argument");
     if (cnt[normal] != 1) return fail(FDTD_EINVAL, "a flux plane is one cell thick along its normal");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    s->graph.reset();  // a captured sequence samples the planes it was captured with
     int cells = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] >= s->host.n[a] || cnt[a] > s->host.n[a])
@@ -1726,7 +1900,7 @@ int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
             if (rc) return rc;
             HIP_TRY(hipEventRecord(ev[3 * q + 2], s->stream));
             mirror_advance(s);
-            rc = flux_after_step(s, s->stream);
+            rc = flux_after_step(lone(s), true);
             if (rc) return rc;
         }
         HIP_TRY(hipEventSynchronize(ev[3 * (n - 1) + 2]));
@@ -1882,32 +2056,16 @@ int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     int rc = batch_tiles(b, g_shape.vec);
     if (rc) return rc;
-    for (int q = 0; q < nsteps; ++q) {
-        rc = launch_h(b->group, b->stream);
-        if (rc) return rc;
-        rc = launch_d(b->group, b->stream);
-        if (rc) return rc;
-        bool sample = false;
-        for (auto *s : b->sims) {
-            mirror_advance(s);
-            for (auto &ph : s->planes) sample |= samples_now(ph, s->host.step);
-        }
-        if (!sample) continue;
-        flux_sample_kernel<<<dim3((b->max_plane_cells + 255) / 256, b->nplanes), 256, 0, b->stream>>>(b->planes, b->nplanes);
-        HIP_TRY(hipGetLastError());
-        int pi = 0;
-        for (auto *s : b->sims)
-            for (size_t r = 0; r < s->planes.size(); ++r, ++pi) {
-                PlaneHost &ph = s->planes[r];
-                if (!samples_now(ph, s->host.step)) continue;
-                ph.filled += 1;
-                if (ph.filled == kRing) {
-                    rc = flux_flush_plane(b->planes + pi, ph, b->stream);
-                    if (rc) return rc;
-                }
-            }
-    }
-    return FDTD_OK;
+    Stepper k;
+    k.g = b->group;
+    k.sims = b->sims.data();
+    k.nsims = (int)b->sims.size();
+    k.planes = b->planes;
+    k.nplanes = b->nplanes;
+    k.max_plane_cells = b->max_plane_cells;
+    k.st = b->stream;
+    k.graph = &b->graph;
+    return run_steps(k, nsteps);
 }
 
 int fdtd_batch_flush(fdtd_batch *b) {

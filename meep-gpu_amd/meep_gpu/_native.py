@@ -25,6 +25,8 @@ FDTD_MAX_POLES = 4
 SIGNATURES = {
     "fdtd_last_error": (ctypes.c_char_p, []),
     "fdtd_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "fdtd_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
     "fdtd_sim_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "fdtd_sim_destroy": (None, [ctypes.c_void_p]),
     "fdtd_sim_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
@@ -57,6 +59,7 @@ SIGNATURES = {
     "fdtd_set_launch": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32]),
     "fdtd_get_launch": (ctypes.c_int, [_IP]),
+    "fdtd_set_graphs": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_batch_flush": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_time_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
     "fdtd_run_timed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),

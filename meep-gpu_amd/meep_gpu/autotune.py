@@ -93,3 +93,24 @@ def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10
         cache[key] = list(best)
         _store_cache(cache)
     return best
+
+
+def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16) -> Shape:
+    """Choose and install the fastest launch shape for a ``BatchedRun`` by stepping it.
+
+    Each candidate runs ``steps`` warm steps (its tile list is built on first use), then
+    ``steps`` timed ones; the batch really advances, so its source tables must cover them.
+    """
+    best, best_t = None, float("inf")
+    for shape in candidates:
+        set_launch(*shape)
+        batch.run(steps)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        batch.run(steps)
+        batch.synchronize()
+        t = time.perf_counter() - t0
+        if t < best_t:
+            best, best_t = shape, t
+    set_launch(*best)
+    return best

@@ -22,6 +22,7 @@ import numpy
 from .dft import FluxMonitor, meep_frequency_span
 from .dispersion import Susceptibility
 from .engine import HipSimulation
+from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
 from .grid import Grid
 from .pml import PML
 from .sources import DepositBox, current_table
@@ -47,6 +48,7 @@ class FdtdDriver:
         self.flux_monitors: List[FluxMonitor] = []
         self.max_steps = int(max_steps)
         self.sim: Optional[HipSimulation] = None
+        self._fast_path = None
         self._planned = 0
 
     # --- configuration (before the first step) ------------------------------------------------
@@ -123,6 +125,7 @@ class FdtdDriver:
         drv.flux_monitors = []
         drv.max_steps = max(wl.steps, 1)
         drv.sim = None
+        drv._fast_path = None
         drv._planned = 0
         for p in wl.planes:
             drv.add_flux_monitor(p.normal, p.lo, p.cnt, frequencies=p.frequencies)
@@ -160,16 +163,29 @@ class FdtdDriver:
             mon.attach(sim)
         if pml is None and not self._poles and not self._offdiag:
             sim.prime_e()
+        plan = plan_fast_path(sim, pml, self.grid, self._sources)
+        if plan is None:
+            sim.close()
+            raise FastPathError(f"no engine plan: {last_dispatch_report()['reasons']}")
         self.sim = sim
+        self._fast_path = plan
         self._planned = steps
         return sim
 
+    #: The consults of one step in the reference driver's order (driver.py:3505-3596).
+    CONSULTS = ("step_B", "fill_B", "fill_folded_far_ghosts_B", "update_H", "step_D", "fill_D",
+                "fill_folded_far_ghosts_D", "update_E", "update_P")
+
     def step(self):
+        """One time step through the dispatch seam: each consult must be answered by the plan
+        (there is no array path to fall back on), the fused launches at step_B and step_D."""
         sim = self.freeze()
         if sim.step_count >= self._planned:
             raise RuntimeError("the run is past its planned length; set max_steps higher")
-        sim.step_h()
-        sim.step_d()
+        fast = self._fast_path
+        for slot in self.CONSULTS:
+            if not fast.dispatch(slot, sim):
+                raise FastPathError(f"the engine plan declined {slot}")
 
     def run(self, steps: Optional[int] = None, until: Optional[float] = None):
         if (steps is None) == (until is None):
@@ -195,8 +211,11 @@ class FdtdDriver:
     def fast_path_report(self) -> Dict[str, object]:
         from .engine import device_info
 
-        return {"engine": "hip", "device": device_info(), "fused": ["step_B+update_H",
-                "step_D+update_E+update_P"], "library": "meep_gpu/_lib/libmeep_fdtd.so"}
+        rep = last_dispatch_report()
+        rep.update({"engine": "hip", "device": device_info(),
+                    "library": "meep_gpu/_lib/libmeep_fdtd.so",
+                    "launches": self._fast_path.launches[0] if self._fast_path else 0})
+        return rep
 
     def close(self):
         if self.sim is not None:

@@ -225,3 +225,47 @@ def get_launch() -> Tuple[int, int, bool, bool]:
     out = numpy.zeros(4, dtype=numpy.int32)
     check(_native.load().fdtd_get_launch(iptr(out)), "fdtd_get_launch")
     return int(out[0]), int(out[1]), bool(out[2]), bool(out[3])
+
+
+def set_graphs(enable: bool):
+    """Replay captured 64-step hipGraphs in ``run`` and batched runs (default on)."""
+    check(_native.load().fdtd_set_graphs(1 if enable else 0), "fdtd_set_graphs")
+
+
+def device_memory() -> Tuple[int, int]:
+    """(free, total) bytes of device memory on the current GPU."""
+    free, total = ctypes.c_uint64(), ctypes.c_uint64()
+    check(_native.load().fdtd_device_memory(ctypes.byref(free), ctypes.byref(total)),
+          "fdtd_device_memory")
+    return int(free.value), int(total.value)
+
+
+#: flux samples buffered per plane before a transform pass (kRing in the engine)
+RING = 64
+
+
+def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tuple[int, int]] = (),
+                    offdiagonal_volumes: int = 0,
+                    planes: Sequence[Tuple[int, int]] = (), source_steps: int = 0,
+                    source_points: int = 0) -> int:
+    """Device bytes a simulation allocates (an upper bound on ``HipSimulation.device_bytes``).
+
+    ``driven_poles``: per pole (driven components, sigma volumes); ``planes``: per flux plane
+    (cells, frequencies). Mirrors the engine's allocations: 12 float32 volumes (D, B, E and
+    inverse epsilon), 15 more under an absorber (H, fu_B, fu_D, f_w_E, f_w_H), two per driven
+    pole component plus its sigma volume, the ring, phase and float64 accumulators per plane.
+    """
+    nx, ny, nz = (int(v) for v in shape)
+    npad = ((nx + 3) // 4) * 4 * ny * nz
+    vol = 4 * npad
+    total = 4096 + vol * (12 + (15 if pml else 0))
+    total += 16 * 2 * (npad // max(ny * nz, 1) + ny + nz)  # the six layer coefficient lines
+    for comps, sig in driven_poles:
+        total += vol * (2 * int(comps) + int(sig))
+    total += vol * int(offdiagonal_volumes)
+    for cells, nf in planes:
+        row = ((int(cells) + 3) // 4) * 4
+        total += 8 * nf + 4 * RING * 4 * row + 8 * RING + 8 * RING * 4 * nf + 64 * nf * cells
+        total += 512 * (len(planes) + 1)
+    total += 16 * source_points + 16 * (source_steps + 1) + 256
+    return int(total)

@@ -133,6 +133,8 @@ def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
         inner = edge - 2 * pml_cells
         for pos in (pml_cells + 4, edge - pml_cells - 4):
             wl.planes.append(FluxPlane(0, (pos, pml_cells, pml_cells), (1, inner, inner), freqs))
+    elif planes != "none":
+        raise ValueError(f"planes must be 'one', 'box8' or 'none', got {planes!r}")
     return wl
 
 

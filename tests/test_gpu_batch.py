@@ -34,18 +34,59 @@ def _members():
                            lorentz_fraction=0.5)
 
 
+def _same_words(a, b):
+    for comp in ("Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Bx", "By", "Bz", "Hx", "Hy", "Hz"):
+        x, y = a.sim.download(comp), b.sim.download(comp)
+        assert x.view(numpy.uint32).tobytes() == y.view(numpy.uint32).tobytes(), comp
+    for ma, mb in zip(a.flux_monitors, b.flux_monitors):
+        fa, fb = ma.get_flux_spectrum(), mb.get_flux_spectrum()
+        assert numpy.array_equal(fa, fb)
+        assert numpy.max(numpy.abs(fa)) > 0
+
+
+def test_graph_replay_steps_the_same_words_as_eager_launches():
+    """Captured 64-step sequences (re-captured when a call starts at another ring phase)
+    against one eager launch per step."""
+    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.engine import set_graphs
+    from tests.workload_runs import small
+
+    steps = 230
+    runs = []
+    for graphs, chunks in ((False, (steps,)), (True, (70, 130, 30))):
+        set_graphs(graphs)
+        try:
+            drv = FdtdDriver.from_workload(small("sphere_box"))
+            drv.freeze(planned_steps=steps + 1)
+            for c in chunks:
+                drv.run(steps=c)
+            drv.synchronize()
+        finally:
+            set_graphs(True)
+        assert drv.step_count == steps
+        runs.append(drv)
+    _same_words(*runs)
+    for d in runs:
+        d.close()
+
+
 def test_each_member_steps_the_same_alone_and_in_the_batch():
     from meep_gpu.batch import BatchedRun
     from meep_gpu.driver import FdtdDriver
+    from meep_gpu.engine import set_graphs
 
     steps = 150
     alone = []
-    for wl in _members():
-        drv = FdtdDriver.from_workload(wl)
-        drv.freeze(planned_steps=steps + 1)
-        drv.run(steps=steps)
-        drv.synchronize()
-        alone.append(drv)
+    set_graphs(False)  # alone: eager launches; the batch below replays graphs
+    try:
+        for wl in _members():
+            drv = FdtdDriver.from_workload(wl)
+            drv.freeze(planned_steps=steps + 1)
+            drv.run(steps=steps)
+            drv.synchronize()
+            alone.append(drv)
+    finally:
+        set_graphs(True)
     together = []
     for wl in _members():
         drv = FdtdDriver.from_workload(wl)
@@ -58,12 +99,6 @@ def test_each_member_steps_the_same_alone_and_in_the_batch():
     batch.close()
     for a, b in zip(alone, together):
         assert b.step_count == steps
-        for comp in ("Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Bx", "By", "Bz", "Hx", "Hy", "Hz"):
-            x, y = a.sim.download(comp), b.sim.download(comp)
-            assert x.view(numpy.uint32).tobytes() == y.view(numpy.uint32).tobytes(), comp
-        for ma, mb in zip(a.flux_monitors, b.flux_monitors):
-            fa, fb = ma.get_flux_spectrum(), mb.get_flux_spectrum()
-            assert numpy.array_equal(fa, fb)
-            assert numpy.max(numpy.abs(fa)) > 0
+        _same_words(a, b)
     for d in alone + together:
         d.close()
