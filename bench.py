@@ -103,7 +103,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     drivers = []
     for wl in wls:
         drv = FdtdDriver.from_workload(wl)
-        drv.freeze(planned_steps=2 * steps + 2 * 64 + 2 * 16 * len(autotune.CANDIDATES) + 100)
+        drv.freeze(planned_steps=2 * steps + 2 * 64 + 4 * 16 * len(autotune.CANDIDATES) + 100)
         drivers.append(drv)
     build_s = time.perf_counter() - t_build
     cells = sum(d.grid.total_cells for d in drivers)
@@ -145,7 +145,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "note": "algorithmic bytes of both fused pairs per step, all members"},
-            "launch_shape": list(shape), "graphs": True, "spectra_finite": bool(finite),
+            "launch_shape": {"h_pair": list(shape[0]), "d_pair": list(shape[1])}, "graphs": True, "spectra_finite": bool(finite),
             "host_build_s": round(build_s, 2)}
 
 
@@ -182,8 +182,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-count", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT",
-                    help="fixed launch shape instead of the autotuned one (profiling runs)")
+    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[/V,BPC,REMAP,NT]",
+                    help="fixed launch shapes (H pair[/D pair]) instead of the autotuned ones "
+                         "(profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="time a plain device copy of BYTES (profile counter calibration)")
     args = ap.parse_args(argv)
@@ -207,13 +208,15 @@ def main(argv=None) -> int:
     from meep_gpu import autotune
     from meep_gpu.pml import PML
 
-    from meep_gpu.engine import set_launch
-
     wl = workloads.config(args.workload)
     if args.launch:
-        v, bpc, remap, nt = (int(x) for x in args.launch.split(","))
-        shape = (v, bpc, bool(remap), bool(nt))
-        set_launch(*shape)
+        parts = args.launch.split("/")
+        shapes = []
+        for part in (parts[0], parts[-1]):
+            v, bpc, remap, nt = (int(x) for x in part.split(","))
+            shapes.append((v, bpc, bool(remap), bool(nt)))
+        shape = tuple(shapes)
+        autotune.install(shape)
     else:
         shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
                               wl.poles, use_cache=False)
@@ -259,8 +262,9 @@ def main(argv=None) -> int:
                                 "achieved": round(nbytes["h"] / (h_ms * 1e-3) / 1e9, 1)},
                      "step_model_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2)},
         "spectra_finite": bool(spectra_ok),
-        "launch_shape": {"cells_per_thread": shape[0], "blocks_per_cu": shape[1],
-                         "xcd_remap": bool(shape[2]), "nt_stores": bool(shape[3])},
+        "launch_shape": {name: {"cells_per_thread": sh[0], "blocks_per_cu": sh[1],
+                                "xcd_remap": bool(sh[2]), "nt_stores": bool(sh[3])}
+                         for name, sh in (("h_pair", shape[0]), ("d_pair", shape[1]))},
     }
     drv.close()
     if not args.no_batched:

@@ -152,7 +152,13 @@ int fdtd_batch_synchronize(fdtd_batch *batch);
  * nt_stores = 1 to stream the fused pairs' outputs with non-temporal stores. No counterpart
  * in the reference (its launches are one thread per cell). */
 int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores);
-int fdtd_get_launch(int32_t out[4]);
+int fdtd_get_launch(int32_t out[4]);  /* the H pair's shape */
+
+/* The same per fused pair: pair 0 = the H pair, 1 = the D pair (each has its own best shape:
+ * the D pair streams more arrays and prefers fewer resident workgroups). */
+int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap,
+                         int32_t nt_stores);
+int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
 
 /* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 64 whole
  * steps (both fused pairs, the counter advance, flux sampling and the transform passes that fall

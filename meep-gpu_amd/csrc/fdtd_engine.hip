@@ -965,7 +965,7 @@ struct LaunchShape {
     int xcd_remap = 0;
     int nt = 0;             // non-temporal stores of the fused pairs' outputs
 };
-LaunchShape g_shape;
+LaunchShape g_shape[2];  // [0]: the H pair's launches, [1]: the D pair's (and its row path)
 int g_num_cu = 0;
 
 int num_cu() {
@@ -979,14 +979,17 @@ int num_cu() {
     return g_num_cu;
 }
 
-int grid_for(int ntiles) {
-    int g = g_shape.blocks_per_cu > 0 ? num_cu() * g_shape.blocks_per_cu : ntiles;
+int grid_for(const LaunchShape &sh, int ntiles) {
+    int g = sh.blocks_per_cu > 0 ? num_cu() * sh.blocks_per_cu : ntiles;
     if (g > ntiles) g = ntiles;
     g = ((g + 7) / 8) * 8;
     return g < 8 ? 8 : g;
 }
 
-int tile_cells() { return kThreads * g_shape.vec; }
+bool same_shape(const LaunchShape &a, const LaunchShape &b) {
+    return a.vec == b.vec && a.blocks_per_cu == b.blocks_per_cu && a.xcd_remap == b.xcd_remap &&
+           a.nt == b.nt;
+}
 
 struct PlaneHost {
     PlaneDev dev;
@@ -1003,12 +1006,13 @@ struct Group {
 };
 
 int launch_h(const Group &g, hipStream_t st) {
-    const int V = g_shape.vec;
+    const LaunchShape &sh = g_shape[0];
+    const int V = sh.vec;
     const int nt = g.ntiles.at(V);
     const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
-    const int G = grid_for(nt);
-    const int rm = g_shape.xcd_remap;
-    if (g_shape.nt) {
+    const int G = grid_for(sh, nt);
+    const int rm = sh.xcd_remap;
+    if (sh.nt) {
         if (V == 4) hpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
         else if (V == 2) hpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
         else hpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
@@ -1022,13 +1026,14 @@ int launch_h(const Group &g, hipStream_t st) {
 }
 
 int launch_d(const Group &g, hipStream_t st) {
-    const int V = g_shape.vec;
+    const LaunchShape &sh = g_shape[1];
+    const int V = sh.vec;
     const int nt = g.ntiles.at(V);
     const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
-    const int G = grid_for(nt);
-    const int rm = g_shape.xcd_remap;
+    const int G = grid_for(sh, nt);
+    const int rm = sh.xcd_remap;
     if (!g.all_offdiag) {
-        if (g_shape.nt) {
+        if (sh.nt) {
             if (V == 4) dpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
             else if (V == 2) dpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
             else dpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
@@ -1061,16 +1066,15 @@ int launch_d(const Group &g, hipStream_t st) {
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
     std::vector<int> phase;
-    LaunchShape shape;
+    LaunchShape shape[2];
     void reset() {
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
         phase.clear();
     }
     bool matches(const std::vector<int> &ph) const {
-        return exec && ph == phase && shape.vec == g_shape.vec &&
-               shape.blocks_per_cu == g_shape.blocks_per_cu &&
-               shape.xcd_remap == g_shape.xcd_remap && shape.nt == g_shape.nt;
+        return exec && ph == phase && same_shape(shape[0], g_shape[0]) &&
+               same_shape(shape[1], g_shape[1]);
     }
     ~StepGraph() { reset(); }
 };
@@ -1131,8 +1135,10 @@ struct fdtd_sim {
         Group g;
         g.dev = dev;
         g.nsims = 1;
-        const int64_t nt = ((int64_t)host.npad + tile_cells() - 1) / tile_cells();
-        g.ntiles[g_shape.vec] = (int)nt;
+        for (const LaunchShape &sh : g_shape) {
+            const int64_t tc = (int64_t)kThreads * sh.vec;
+            g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
+        }
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
         return g;
     }
@@ -1283,7 +1289,8 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
     }
     k.graph->exec = exec;
     k.graph->phase = phase;
-    k.graph->shape = g_shape;
+    k.graph->shape[0] = g_shape[0];
+    k.graph->shape[1] = g_shape[1];
     HIP_TRY(hipGraphLaunch(exec, k.st));
     return FDTD_OK;
 }
@@ -1366,19 +1373,38 @@ int fdtd_device_memory(uint64_t *free_bytes, uint64_t *total_bytes) {
 int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores) {
     if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
-    g_shape.vec = vec;
-    g_shape.blocks_per_cu = blocks_per_cu;
-    g_shape.xcd_remap = xcd_remap ? 1 : 0;
-    g_shape.nt = nt_stores ? 1 : 0;
+    for (int pair = 0; pair < 2; ++pair) {
+        int rc = fdtd_set_launch_pair(pair, vec, blocks_per_cu, xcd_remap, nt_stores);
+        if (rc) return rc;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap,
+                         int32_t nt_stores) {
+    if (pair != 0 && pair != 1) return fail(FDTD_EINVAL, "pair must be 0 (H) or 1 (D), got %d", pair);
+    if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
+    LaunchShape &sh = g_shape[pair];
+    sh.vec = vec;
+    sh.blocks_per_cu = blocks_per_cu;
+    sh.xcd_remap = xcd_remap ? 1 : 0;
+    sh.nt = nt_stores ? 1 : 0;
+    return FDTD_OK;
+}
+
+int fdtd_get_launch_pair(int32_t pair, int32_t out[4]) {
+    if (!out || (pair != 0 && pair != 1)) return fail(FDTD_EINVAL, "bad argument");
+    out[0] = g_shape[pair].vec;
+    out[1] = g_shape[pair].blocks_per_cu;
+    out[2] = g_shape[pair].xcd_remap;
+    out[3] = g_shape[pair].nt;
     return FDTD_OK;
 }
 
 int fdtd_get_launch(int32_t out[4]) {
     if (!out) return fail(FDTD_EINVAL, "null argument");
-    out[0] = g_shape.vec;
-    out[1] = g_shape.blocks_per_cu;
-    out[2] = g_shape.xcd_remap;
-    out[3] = g_shape.nt;
+    return fdtd_get_launch_pair(0, out);
     return FDTD_OK;
 }
 
@@ -1495,12 +1521,38 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
     return FDTD_OK;
 }
 
+// Host <-> device volume transfers. The device rows are padded to the pitch; the rows are
+// re-pitched on the host and moved as ONE contiguous copy (a pitched 2-D copy from pageable
+// memory goes row by row and costs ~10 us per row).
 static int upload_volume(fdtd_sim *s, float *dst, const float *src) {
     const SimDev &h = s->host;
-    HIP_TRY(hipMemcpy2DAsync(dst, sizeof(float) * h.pitch, src, sizeof(float) * h.n[0],
-                             sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
-                             hipMemcpyHostToDevice, s->stream));
+    const size_t rows = (size_t)h.n[1] * h.n[2];
+    if (h.pitch == h.n[0]) {
+        HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float) * rows * h.n[0], hipMemcpyHostToDevice, s->stream));
+    } else {
+        std::vector<float> stage(rows * h.pitch, 0.0f);
+        for (size_t r = 0; r < rows; ++r)
+            std::memcpy(stage.data() + r * h.pitch, src + r * h.n[0], sizeof(float) * h.n[0]);
+        HIP_TRY(hipMemcpyAsync(dst, stage.data(), sizeof(float) * stage.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));  // before `stage` goes
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+static int download_volume(fdtd_sim *s, float *host, const float *src) {
+    const SimDev &h = s->host;
+    const size_t rows = (size_t)h.n[1] * h.n[2];
+    if (h.pitch == h.n[0]) {
+        HIP_TRY(hipMemcpyAsync(host, src, sizeof(float) * rows * h.n[0], hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        return FDTD_OK;
+    }
+    std::vector<float> stage(rows * h.pitch);
+    HIP_TRY(hipMemcpyAsync(stage.data(), src, sizeof(float) * stage.size(), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t r = 0; r < rows; ++r)
+        std::memcpy(host + r * h.n[0], stage.data() + r * h.pitch, sizeof(float) * h.n[0]);
     return FDTD_OK;
 }
 
@@ -1675,19 +1727,15 @@ int fdtd_download(fdtd_sim *s, const char *name, float *host, size_t count) {
     float *src = volume_by_name(s, name);
     if (!src) return fail(FDTD_EINVAL, "no volume named '%s' in this simulation", name);
     const SimDev &h = s->host;
-    HIP_TRY(hipMemcpy2DAsync(host, sizeof(float) * h.n[0], src, sizeof(float) * h.pitch,
-                             sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
-                             hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    rc = download_volume(s, host, src);
+    if (rc) return rc;
     if (name[0] == 'H' && name[2] == 0 && h.pml && h.hsep) {
         // after the first update_H the stored H is current only on its own-axis layer rows;
         // everywhere else H is B's word
         const int c = name[1] - 'x';
         std::vector<float> b((size_t)h.ncell);
-        HIP_TRY(hipMemcpy2DAsync(b.data(), sizeof(float) * h.n[0], h.B[c], sizeof(float) * h.pitch,
-                                 sizeof(float) * h.n[0], (size_t)h.n[1] * h.n[2],
-                                 hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
+        rc = download_volume(s, b.data(), h.B[c]);
+        if (rc) return rc;
         const std::vector<uint8_t> &line = s->hmember[c];
         for (int k = 0; k < h.n[2]; ++k)
             for (int j = 0; j < h.n[1]; ++j)
@@ -2025,7 +2073,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         if (hipMemcpy(b->planes, planes.data(), sizeof(PlaneDev) * planes.size(), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(FDTD_EHIP, "batch plane upload");
     }
-    int rc = batch_tiles(b, g_shape.vec);
+    int rc = batch_tiles(b, g_shape[0].vec);
+    if (rc == FDTD_OK) rc = batch_tiles(b, g_shape[1].vec);
     if (rc) { fdtd_batch_destroy(b); return rc; }
     for (auto *s : b->sims) s->in_batch = true;
     *out = b;
@@ -2054,7 +2103,8 @@ void fdtd_batch_destroy(fdtd_batch *b) {
 
 int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
-    int rc = batch_tiles(b, g_shape.vec);
+    int rc = batch_tiles(b, g_shape[0].vec);
+    if (rc == FDTD_OK) rc = batch_tiles(b, g_shape[1].vec);
     if (rc) return rc;
     Stepper k;
     k.g = b->group;

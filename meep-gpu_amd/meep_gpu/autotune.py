@@ -6,8 +6,9 @@ registers and more resident waves), one workgroup per tile or a persistent grid 
 workgroups per CU, the XCD-contiguous tile order on or off, and plain or non-temporal output
 stores. Each is timed on a scratch
 simulation with the same grid, absorber and poles as the run (no sources, no monitors), with
-HIP events around every launch, and the fastest mean H+D pair wins. Results are cached per
-(grid shape, absorber, poles, device) in a JSON file, so a served or batched run tunes once.
+HIP events around every launch; the H pair and the D pair each get the shape they ran fastest
+with (the D pair streams more arrays than the H pair and their best shapes differ). Results
+are cached per (grid shape, absorber, poles, device) in a JSON file, so a run tunes once.
 """
 
 from __future__ import annotations
@@ -19,11 +20,13 @@ from typing import Dict, Iterable, List, Optional, Tuple
 
 import numpy
 
-from .engine import HipSimulation, device_info, get_launch, set_launch
+from .engine import (HipSimulation, device_info, get_launch_pair, set_launch,
+                     set_launch_pair)
 from .grid import Grid
 from .pml import PML
 
 Shape = Tuple[int, int, bool, bool]
+PairShapes = Tuple[Shape, Shape]  # (H pair, D pair)
 
 CANDIDATES: Tuple[Shape, ...] = tuple(
     (vec, bpc, remap, nt) for vec in (1, 2, 4) for bpc in (0, 4, 8) for remap in (True, False)
@@ -62,7 +65,7 @@ def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
     sim = HipSimulation(grid, pml)
     for term, sigma in poles:
         sim.add_pole(term, sigma)
-    saved = get_launch()
+    saved = (get_launch_pair(0), get_launch_pair(1))
     out = []
     try:
         for shape in candidates:
@@ -71,46 +74,64 @@ def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
             h, d = sim.time_pairs(repeats)
             out.append((shape, h, d))
     finally:
-        set_launch(*saved)
+        install(saved)
         sim.close()
     return out
 
 
+def install(shapes: PairShapes) -> None:
+    set_launch_pair(0, *shapes[0])
+    set_launch_pair(1, *shapes[1])
+
+
+def best_pair_shapes(results) -> PairShapes:
+    """The fastest shape for each fused pair among ``measure``'s results."""
+    h = min(results, key=lambda r: r[1])[0]
+    d = min(results, key=lambda r: r[2])[0]
+    return tuple(h), tuple(d)
+
+
 def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
-         use_cache: bool = True) -> Shape:
-    """Choose and install the fastest launch shape for this grid; returns it."""
+         use_cache: bool = True) -> PairShapes:
+    """Choose and install the fastest launch shape of each pair for this grid; returns them."""
     poles = list(poles)
     key = _key(grid, pml, len(poles))
     cache = _load_cache() if use_cache else {}
-    if key in cache:
-        shape = tuple(cache[key])
-        set_launch(*shape)
-        return shape  # type: ignore[return-value]
-    results = measure(grid, pml, poles, repeats=repeats)
-    best = min(results, key=lambda r: r[1] + r[2])[0]
-    set_launch(*best)
+    entry = cache.get(key)
+    if isinstance(entry, list) and len(entry) == 2 and all(len(x) == 4 for x in entry):
+        shapes = (tuple(entry[0]), tuple(entry[1]))
+        install(shapes)
+        return shapes  # type: ignore[return-value]
+    shapes = best_pair_shapes(measure(grid, pml, poles, repeats=repeats))
+    install(shapes)
     if use_cache:
-        cache[key] = list(best)
+        cache[key] = [list(shapes[0]), list(shapes[1])]
         _store_cache(cache)
-    return best
+    return shapes
 
 
-def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16) -> Shape:
-    """Choose and install the fastest launch shape for a ``BatchedRun`` by stepping it.
+def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16
+               ) -> PairShapes:
+    """Choose and install launch shapes for a ``BatchedRun`` by stepping it.
 
     Each candidate runs ``steps`` warm steps (its tile list is built on first use), then
-    ``steps`` timed ones; the batch really advances, so its source tables must cover them.
+    ``steps`` timed ones, for both pairs at once; then the best shape's D pair is held while
+    the H pair tries every candidate. The batch really advances, so its source tables must
+    cover 2 * steps * (2 * len(candidates)) steps.
     """
-    best, best_t = None, float("inf")
-    for shape in candidates:
-        set_launch(*shape)
+    candidates = list(candidates)
+
+    def timed() -> float:
         batch.run(steps)
         batch.synchronize()
         t0 = time.perf_counter()
         batch.run(steps)
         batch.synchronize()
-        t = time.perf_counter() - t0
-        if t < best_t:
-            best, best_t = shape, t
+        return time.perf_counter() - t0
+
+    best = min(candidates, key=lambda sh: (set_launch(*sh), timed())[1])
     set_launch(*best)
-    return best
+    h_best = min(candidates, key=lambda sh: (set_launch_pair(0, *sh), timed())[1])
+    shapes = (tuple(h_best), tuple(best))
+    install(shapes)
+    return shapes

@@ -112,6 +112,20 @@ class FdtdDriver:
         self.flux_monitors.append(mon)
         return mon
 
+    def add_flux_region(self, center: Sequence[float], size: Sequence[float],
+                        fcen: float = None, df: float = None, nfreq: int = None,
+                        frequencies: Sequence[float] = None, decimation: int = 1) -> FluxMonitor:
+        """``sim.add_flux(fcen, df, nfreq, mp.FluxRegion(center, size))``: the plane placed and
+        weighted as MEEP places it (coordinates with their origin at the cell centre).
+        ``decimation`` is the accumulation stride MEEP resolved for the monitor."""
+        self._require_unfrozen("add_flux_region")
+        freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
+                 else meep_frequency_span(fcen, df, nfreq))
+        mon = FluxMonitor.from_region(self.grid, tuple(center), tuple(size), freqs,
+                                      int(decimation))
+        self.flux_monitors.append(mon)
+        return mon
+
     @classmethod
     def from_workload(cls, wl: Workload) -> "FdtdDriver":
         drv = cls.__new__(cls)

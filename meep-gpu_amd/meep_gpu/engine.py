@@ -222,8 +222,21 @@ def set_launch(vec: int, blocks_per_cu: int, xcd_remap: bool, nt_stores: bool = 
 
 
 def get_launch() -> Tuple[int, int, bool, bool]:
+    """The H pair's launch shape (``set_launch`` sets both pairs alike)."""
+    return get_launch_pair(0)
+
+
+def set_launch_pair(pair: int, vec: int, blocks_per_cu: int, xcd_remap: bool,
+                    nt_stores: bool = False):
+    """One fused pair's launch shape: pair 0 = H pair, 1 = D pair."""
+    check(_native.load().fdtd_set_launch_pair(int(pair), int(vec), int(blocks_per_cu),
+                                              1 if xcd_remap else 0, 1 if nt_stores else 0),
+          "fdtd_set_launch_pair")
+
+
+def get_launch_pair(pair: int) -> Tuple[int, int, bool, bool]:
     out = numpy.zeros(4, dtype=numpy.int32)
-    check(_native.load().fdtd_get_launch(iptr(out)), "fdtd_get_launch")
+    check(_native.load().fdtd_get_launch_pair(int(pair), iptr(out)), "fdtd_get_launch_pair")
     return int(out[0]), int(out[1]), bool(out[2]), bool(out[3])
 
 

@@ -12,8 +12,10 @@ frequencies, 1000..8000 steps). ``SolveServer.run`` is continuous batching, not 
   device from admission to completion, and all of them advance together through the same
   fused launches (one batch, ``fdtd_batch_run``), in chunks of at most ``quantum`` steps that
   end where the next resident solve completes;
-* while the device runs a chunk, the host builds the inputs (materials, source tables) of the
-  requests that have arrived, so admission costs the device little;
+* while the device runs a chunk, the host admits what has arrived and fits -- allocates its
+  device state and uploads its materials and source tables -- and builds the inputs of the
+  requests still waiting, so admission costs the device nothing: the new solves join the
+  shared launches at the end of the chunk;
 * a solve that completes is read out (spectra), freed, and its slot and bytes go to the queue.
 
 Each solve steps exactly the words it would step alone (a batch member's launches are the
@@ -209,7 +211,8 @@ class SolveServer:
     """Continuous batching of FDTD solves under an HBM budget (see the module docstring)."""
 
     def __init__(self, budget_bytes: Optional[int] = None, max_slots: int = 64,
-                 quantum: int = 256, backend=None, footprint: Callable = request_footprint,
+                 quantum: int = 256, max_batch_cells: Optional[int] = None, backend=None,
+                 footprint: Callable = request_footprint,
                  clock: Callable[[], float] = time.perf_counter,
                  sleep: Callable[[float], None] = time.sleep, checksums: bool = False):
         if max_slots < 1 or quantum < 1:
@@ -220,6 +223,9 @@ class SolveServer:
             budget_bytes = int(0.85 * free)
         self.budget = int(budget_bytes)
         self.max_slots = int(max_slots)
+        # resident cells past which a batch already saturates the device: admitting more only
+        # shares it thinner (every resident solve's latency grows), so further requests wait
+        self.max_batch_cells = int(max_batch_cells) if max_batch_cells else None
         self.quantum = int(quantum)
         self.footprint = footprint
         self.clock = clock
@@ -249,29 +255,44 @@ class SolveServer:
                 queue.append(reqs[nxt])
                 nxt += 1
 
-        while nxt < len(reqs) or queue or running:
-            now = self.clock() - t0
-            arrivals(now)
-            # admit first come, first served while a slot and the bytes are there
-            while queue and len(running) < self.max_slots:
+        staged: List[_Slot] = []  # admitted while the device ran; they join at the next regroup
+
+        def admit(into: List[_Slot]):
+            """Admit from the head of the queue while a slot and the bytes are there."""
+            nonlocal used, peak_n, peak_b
+            while queue and len(running) + len(staged) < self.max_slots:
                 r = queue[0]
                 need = self.footprint(r)
                 if used + need > self.budget:
+                    break
+                resident = sum(x.request.cells for x in running) + sum(
+                    x.request.cells for x in staged)
+                if (self.max_batch_cells is not None and resident > 0
+                        and resident + r.cells > self.max_batch_cells):
                     break
                 queue.popleft()
                 inputs = prepared.pop(r.rid, None)
                 if inputs is None:
                     inputs = be.prepare(r)
                 handle = be.admit(r, inputs)
-                running.append(_Slot(r, handle, r.steps, self.clock() - t0, need))
+                into.append(_Slot(r, handle, r.steps, self.clock() - t0, need))
                 used += need
-                regroup = True
-            peak_n = max(peak_n, len(running))
+            peak_n = max(peak_n, len(running) + len(staged))
             peak_b = max(peak_b, used)
+
+        while nxt < len(reqs) or queue or running or staged:
+            arrivals(self.clock() - t0)
+            if staged:
+                running.extend(staged)
+                staged.clear()
+                regroup = True
             if not running:
-                if nxt < len(reqs):
-                    self.sleep(max(0.0, reqs[nxt].arrival - (self.clock() - t0)))
-                continue
+                admit(running)
+                regroup = True
+                if not running:
+                    if nxt < len(reqs):
+                        self.sleep(max(0.0, reqs[nxt].arrival - (self.clock() - t0)))
+                    continue
             if regroup:
                 if batch is not None:
                     be.ungroup(batch)
@@ -280,8 +301,10 @@ class SolveServer:
                 regroupings += 1
             chunk = min(self.quantum, min(s.remaining for s in running))
             be.advance(batch, chunk)
-            # overlapped with the device: inputs of requests that have arrived meanwhile
+            # overlapped with the device: admit (allocate and upload) what has arrived and fits,
+            # and build the host inputs of the requests still waiting
             arrivals(self.clock() - t0)
+            admit(staged)
             for r in list(queue)[: self.max_slots]:
                 if r.rid not in prepared:
                     prepared[r.rid] = be.prepare(r)

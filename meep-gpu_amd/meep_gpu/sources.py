@@ -132,3 +132,63 @@ def plane_box(component: str, axis: int, index: int, shape: Sequence[int],
     return DepositBox(component, tuple(lo), tuple(cnt),
                       numpy.full(tuple(cnt), complex(amplitude).real),
                       numpy.full(tuple(cnt), complex(amplitude).imag))
+
+
+#: Yee offsets in half cells per component (reference fields.py:218-223).
+_IYEE = {"Ex": (1, 0, 0), "Ey": (0, 1, 0), "Ez": (0, 0, 1),
+         "Hx": (0, 1, 1), "Hy": (1, 0, 1), "Hz": (1, 1, 0)}
+
+
+def _zero_extent_weights(coord: float, n: int, dx: float, shift: int) -> Tuple[int, Tuple[float, ...]]:
+    """The two samples a zero-size source direction lands between and their linear weights.
+
+    MEEP's boundary weights for a zero-extent direction (``where_min == where_max``,
+    loop_in_chunks.cpp:257-297, reference sources.py:657-696): the point at ``coord`` (origin
+    at the cell centre) splits between the component's samples below and above it as
+    ``(1 - f, f)``. A point exactly on a sample keeps both, the upper with weight 0.
+    """
+    pos = (coord + 0.5 * n * dx) / dx - 0.5 * shift  # in sample units of this component
+    i0 = int(math.floor(pos + 1e-12))
+    f = pos - i0
+    if abs(f) < 1e-12:
+        f = 0.0
+    if not 0 <= i0 < n - 1:
+        raise ValueError(f"a source at {coord} is not between two interior samples")
+    return i0, (1.0 - f, f)
+
+
+def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequence[float],
+                       amplitude: complex = 1.0) -> DepositBox:
+    """A MEEP ``mp.Source`` whose size is zero or the whole cell along each axis.
+
+    Zero-size directions interpolate linearly between the two nearest samples of the
+    component and scale the amplitude by ``a`` (the resolution, MEEP sources.cpp:481-484,
+    reference sources.py:860-866): a point source in 3-D deposits ``amp * a^3 * w`` -- the
+    discrete delta of a current density. A direction the source spans whole takes every
+    sample with weight 1 (a periodic axis has no boundary sample to halve).
+    """
+    iyee = _IYEE[component]
+    a = grid.resolution
+    lo, cnt, weights = [], [], []
+    amp = complex(amplitude)
+    for d in range(3):
+        n = grid.shape[d]
+        if grid.is_invariant(d):
+            lo.append(0)
+            cnt.append(1)
+            weights.append(numpy.ones(1))
+            continue
+        if size[d] == 0.0:
+            i0, w = _zero_extent_weights(float(center[d]), n, grid.dx, iyee[d])
+            lo.append(i0)
+            cnt.append(2)
+            weights.append(numpy.array(w))
+            amp = amp * a
+        elif abs(size[d] - n * grid.dx) < 1e-9 * n * grid.dx:
+            lo.append(0)
+            cnt.append(n)
+            weights.append(numpy.ones(n))
+        else:
+            raise ValueError("a source extent must be zero or the whole cell along each axis")
+    w = weights[0][:, None, None] * weights[1][None, :, None] * weights[2][None, None, :]
+    return DepositBox(component, tuple(lo), tuple(cnt), w * amp.real, w * amp.imag)

@@ -23,7 +23,7 @@ import numpy
 
 from .dispersion import LORENTZIAN, Susceptibility
 from .grid import Grid
-from .sources import DepositBox, GaussianSource, plane_box
+from .sources import DepositBox, GaussianSource, meep_volume_source
 
 _E = ("Ex", "Ey", "Ez")
 #: Yee offsets in half cells per component (reference fields.py:218-223).
@@ -90,14 +90,14 @@ def _sphere_inverse_epsilon(grid: Grid, radius: float, eps_in: float, eps_out: f
 
 
 def _ez_point(grid: Grid, x: float, y: float, z: float) -> DepositBox:
-    """An Ez point source split over the two Ez samples straddling z (weights 1/2 each)."""
-    dx = grid.dx
-    idx = [int(round((v + 0.5 * n * dx) / dx)) % n for v, n in zip((x, y, z), grid.shape)]
-    nz = grid.shape[2]
-    k0 = (idx[2] - 1) % nz
-    lo = (idx[0], idx[1], min(k0, nz - 2))
-    cnt = (1, 1, 2)
-    return DepositBox("Ez", lo, cnt, numpy.full(cnt, 0.5), numpy.zeros(cnt))
+    """An Ez point source at (x, y, z), deposited as MEEP deposits a zero-size source."""
+    return meep_volume_source(grid, "Ez", (x, y, z), (0.0, 0.0, 0.0))
+
+
+def _plane_index(x: float, edge: int, dx: float) -> int:
+    """The cell whose centre is nearest the coordinate ``x`` (origin at the cell centre): flux
+    planes sample cell-centred fields."""
+    return int(math.floor((x + 0.5 * edge * dx) / dx))
 
 
 def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
@@ -117,7 +117,7 @@ def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
     c = edge // 2
     if planes == "one":
         half = edge // 4
-        i = int(round((plane_x + L / 2) * resolution))
+        i = _plane_index(plane_x, edge, grid.dx)
         if not pml_cells <= i < edge - pml_cells:
             raise ValueError(f"flux plane at x={plane_x} falls outside the interior")
         wl.planes.append(FluxPlane(0, (i, c - half, c - half), (1, 2 * half, 2 * half), freqs))
@@ -155,7 +155,9 @@ def slab(edge: int = 160, resolution: float = 20.0, pml_cells: int = 10, thickne
     wl = Workload(f"slab_{edge}", grid, pml_cells, inv, steps=steps)
     wl.poles.append((Susceptibility(f0, gamma, LORENTZIAN), {c: sig for c in _E}))
     src_x = pml_cells + 5
-    wl.sources.append((plane_box("Ez", 0, src_x, grid.shape), GaussianSource(fcen, df)))
+    x = -0.5 * L + (src_x + 0.25) * grid.dx  # a quarter cell past the sample: both weights live
+    wl.sources.append((meep_volume_source(grid, "Ez", (x, 0.0, 0.0), (0.0, L, L)),
+                       GaussianSource(fcen, df)))
     freqs = meep_frequency_span(fcen, df, nfreq)
     inner = edge - 2 * pml_cells
     for pos in (src_x + 5, lo_x + thickness + 10):

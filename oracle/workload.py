@@ -28,8 +28,10 @@ _AX = "xyz"
 
 
 class OraclePlane:
-    def __init__(self, normal: int, lo, cnt, omegas: numpy.ndarray, scale: float):
+    def __init__(self, normal: int, lo, cnt, omegas: numpy.ndarray, scale: float,
+                 decimation: int = 1):
         self.normal = int(normal)
+        self.decimation = int(decimation)  # accumulate when step_count % decimation == 0
         self.lo = tuple(int(v) for v in lo)
         self.cnt = tuple(int(v) for v in cnt)
         self.omegas = numpy.asarray(omegas, dtype=numpy.float64)
@@ -78,6 +80,79 @@ class OraclePlane:
         return numpy.sum(numpy.real(e1 * numpy.conj(h2) - e2 * numpy.conj(h1)), axis=1) * dx * dx
 
 
+def meep_ladder(where_min: float, where_max: float, a: float):
+    """The oracle's restatement of MEEP's monitor ladder on the centred grid (reference
+    dft.py:604-690: vec2diel_floor/ceil, then compute_boundary_weights' case ladder; MEEP
+    loop_in_chunks.cpp:233-297). Returns (first doubled site, per-site weights)."""
+    lo_site = 1 + 2 * math.floor(where_min * a - 0.5)
+    hi_site = 1 + 2 * math.ceil(where_max * a - 0.5)
+    w0 = 1.0 - where_min * a + 0.5 * lo_site
+    w1 = 1.0 + where_max * a - 0.5 * hi_site
+    n = (hi_site - lo_site) // 2 + 1
+    if hi_site - lo_site >= 6:
+        ends = (w0 * w0 / 2, 1 - (1 - w0) ** 2 / 2, w1 * w1 / 2, 1 - (1 - w1) ** 2 / 2)
+    elif hi_site - lo_site == 4:
+        mid = 1 - (1 - w0) ** 2 / 2 - (1 - w1) ** 2 / 2
+        ends = (w0 * w0 / 2, mid, w1 * w1 / 2, mid)
+    elif where_min == where_max:
+        ends = (w0, w1, w1, w0)
+    elif hi_site - lo_site == 2:
+        first = w0 * w0 / 2 - (1 - w1) ** 2 / 2
+        last = w1 * w1 / 2 - (1 - w0) ** 2 / 2
+        ends = (first, last, last, first)
+    else:
+        ends = (1.0, 1.0, 1.0, 1.0)
+    # IVEC_LOOP_WEIGHT1x: site 0 -> s0, site 1 -> s1, last -> e0, second last -> e1, in that
+    # order of precedence; interior sites 1
+    w = numpy.ones(n)
+    for q in range(n):
+        for site, value in ((0, ends[0]), (1, ends[1]), (n - 1, ends[2]), (n - 2, ends[3])):
+            if q == site and not 1 < q < n - 2:
+                w[q] = value
+                break
+    return lo_site, w
+
+
+def region_planes(shape, dx, center, size, invariant=(False, False, False)):
+    """Oracle planes for a MEEP FluxRegion: [(lo, cnt, normal weight)], the transverse weights
+    (C-order over cnt) and the normal axis. Cell i of an axis of n cells is centred at
+    (i + 1/2 - n/2) * dx, MEEP's doubled site 2i + 1 - n."""
+    a = 1.0 / dx
+    normal = [d for d in range(3) if not invariant[d] and size[d] == 0][0]
+    lo, cnt, tw, nplanes = [0, 0, 0], [1, 1, 1], [], []
+    for d in range(3):
+        if invariant[d]:
+            tw.append(numpy.ones(1))
+            continue
+        site, w = meep_ladder(center[d] - size[d] / 2, center[d] + size[d] / 2, a)
+        first = (site - 1 + shape[d]) // 2
+        if d == normal:
+            nplanes = [(first + q, w[q]) for q in range(w.size)]
+            tw.append(numpy.ones(1))
+            lo[d] = first
+        else:
+            lo[d], cnt[d] = first, w.size
+            tw.append(w)
+    weights = tw[0][:, None, None] * tw[1][None, :, None] * tw[2][None, None, :]
+    out = []
+    for index, w in nplanes:
+        l = list(lo)
+        l[normal] = index
+        out.append((tuple(l), tuple(cnt), w))
+    return out, weights, normal
+
+
+def region_flux(oracle_planes, weights, normal_weights, area):
+    """sum_planes w_n * sum_cells w * Re(E1 conj(H2) - E2 conj(H1)) * dA from the float64
+    accumulations (dft.py:3107-3130)."""
+    total = 0.0
+    for p, wn in zip(oracle_planes, normal_weights):
+        e1, e2, h1, h2 = p.acc64
+        s = numpy.real(e1 * numpy.conj(h2) - e2 * numpy.conj(h1)) * weights.reshape(1, -1)
+        total = total + wn * s.sum(axis=1) * area
+    return total
+
+
 def build(shape, dt, dx, faces, inv: Dict[str, numpy.ndarray], poles: Sequence, deposits,
           planes, invariant=(False, False, False), ln_reflection=None):
     """An ``OracleSim`` plus its planes. ``poles``: (frequency, gamma, drude, sigma dict);
@@ -111,4 +186,5 @@ def run(sim: oe.OracleSim, planes: List[OraclePlane], steps: int,
     for _ in range(steps):
         sim.step()
         for p in planes:
-            p.update(sim, sim.time, reference_precision)
+            if sim.step_count % p.decimation == 0:  # dft.py:3057
+                p.update(sim, sim.time, reference_precision)

@@ -6,25 +6,32 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(1, 0, True, False), (2, 0, True, True), (4, 0, False, False), (2, 8, True, False),
-          (4, 4, True, True), (1, 8, False, True)]
+# (H pair shape, D pair shape): cells per thread, workgroups per CU, XCD remap, NT stores
+SHAPES = [((1, 0, True, False),) * 2, ((2, 0, True, True),) * 2, ((4, 0, False, False),) * 2,
+          ((2, 8, True, False),) * 2, ((4, 4, True, True),) * 2, ((1, 8, False, True),) * 2,
+          ((1, 8, False, True), (2, 0, False, True)), ((4, 0, True, False), (1, 4, False, True))]
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"v{s[0]}b{s[1]}r{int(s[2])}n{int(s[3])}")
-def test_every_launch_shape_reproduces_the_recorded_digests(shape, golden_digests):
-    from meep_gpu.engine import get_launch, set_launch
+def _sid(pair):
+    return "_".join(f"v{s[0]}b{s[1]}r{int(s[2])}n{int(s[3])}" for s in pair)
+
+
+@pytest.mark.parametrize("shapes", SHAPES, ids=_sid)
+def test_every_launch_shape_reproduces_the_recorded_digests(shapes, golden_digests):
+    from meep_gpu.autotune import install
+    from meep_gpu.engine import get_launch_pair
     from oracle import cases
     from tests import hip_cases
 
-    saved = get_launch()
-    set_launch(*shape)
+    saved = (get_launch_pair(0), get_launch_pair(1))
+    install(shapes)
     try:
         for name in ("pml_3d", "offdiag_pml_3d", "dispersive_pml_2d", "no_pml_3d"):
-            print(f"shape {shape} case {name}", flush=True)
+            print(f"shapes {shapes} case {name}", flush=True)
             digest, _ = hip_cases.hip_case_digest(cases.build_state(name))
-            assert digest == golden_digests["case_sha256"][name], (shape, name)
+            assert digest == golden_digests["case_sha256"][name], (shapes, name)
     finally:
-        set_launch(*saved)
+        install(saved)
 
 
 def _members():

@@ -156,3 +156,15 @@ def test_the_served_stream_matches_its_stated_distribution():
     assert all(1000 <= r.steps <= 8000 for r in reqs)
     gaps = [b.arrival - a.arrival for a, b in zip(reqs, reqs[1:])]
     assert 0.2 < sum(gaps) / len(gaps) < 0.3
+
+
+def test_resident_cells_cap_holds_requests_back_until_completions():
+    reqs = [SolveRequest(rid=i, arrival=0.0, edge=40, steps=200 + 100 * i) for i in range(6)]
+    clock = FakeClock()
+    be = FakeBackend(clock)
+    srv = SolveServer(budget_bytes=10 ** 12, max_batch_cells=2 * 40 ** 3, backend=be,
+                      footprint=lambda r: r.cells * 100, clock=clock, sleep=clock.sleep)
+    rep = srv.run(reqs)
+    assert rep.peak_resident == 2
+    assert max(len(g) for g in be.groups) == 2
+    assert [r.rid for r in sorted(rep.results, key=lambda r: r.admitted)] == list(range(6))
