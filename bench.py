@@ -86,6 +86,23 @@ def cpu_baseline(wl, seconds: float) -> dict:
                       f"incl. the {len(wl.planes)} flux planes in float64, {elapsed:.1f} s"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_pairs_sphere192.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by the copy calibration it states), or None when it has no line for it."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def step_bytes(sim) -> float:
     b = algorithmic_bytes(sim)
     return float(b["h"] + b["d"])
@@ -103,7 +120,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     drivers = []
     for wl in wls:
         drv = FdtdDriver.from_workload(wl)
-        drv.freeze(planned_steps=2 * steps + 2 * 64 + 4 * 16 * len(autotune.CANDIDATES) + 100)
+        drv.freeze(planned_steps=3 * steps + 3 * 64 + 4 * 16 * len(autotune.CANDIDATES) + 400)
         drivers.append(drv)
     build_s = time.perf_counter() - t_build
     cells = sum(d.grid.total_cells for d in drivers)
@@ -117,31 +134,43 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     batched_s = time.perf_counter() - t0
     batch.close()
     nbytes = sum(step_bytes(d.sim) for d in drivers)
-    # one at a time: each simulation alone, in turn, with its own autotuned shape
-    alone_s = 0.0
-    for q, d in enumerate(drivers):
-        if q % 16 == 0:
-            print(f"bench: one at a time, simulation {q}/{len(drivers)}", file=sys.stderr,
-                  flush=True)
-        autotune.tune(d.grid, d.sim.pml, [], repeats=3)
-        d.sim.run(64)
-        d.sim.synchronize()
-        t0 = time.perf_counter()
-        d.sim.run(steps)
-        d.sim.synchronize()
-        alone_s += time.perf_counter() - t0
+    # one at a time: each simulation alone, in turn, with its own autotuned shapes; first as
+    # the plain form launches a step (eagerly), then with the step graphs of the batched form
+    from meep_gpu.engine import set_graphs
+
+    alone = {}
+    for graphs in (False, True):
+        set_graphs(graphs)
+        alone_s = 0.0
+        for q, d in enumerate(drivers):
+            if q % 16 == 0:
+                print(f"bench: one at a time (graphs {graphs}), simulation {q}/{len(drivers)}",
+                      file=sys.stderr, flush=True)
+            autotune.tune(d.grid, d.sim.pml, [], repeats=3)
+            d.sim.run(64)
+            d.sim.synchronize()
+            t0 = time.perf_counter()
+            d.sim.run(steps)
+            d.sim.synchronize()
+            alone_s += time.perf_counter() - t0
+        alone[graphs] = cells * steps / alone_s / 1e6
+    set_graphs(True)
     finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum()))
                  for d in drivers for m in d.flux_monitors)
     for d in drivers:
         d.close()
     value = cells * steps / batched_s / 1e6
-    alone = cells * steps / alone_s / 1e6
     achieved = nbytes * steps / batched_s / 1e9
     return {"workload": f"{count} synthetic sphere simulations (seed {seed}), edges 32-112 "
                         f"log-uniform, 25% Lorentz, UPML 8, 1 flux plane x 32 freqs",
             "cells": cells, "steps": steps, "value": round(value, 1), "unit": "Mcell-steps/s",
             "ms_per_step": round(batched_s * 1e3 / steps, 4),
-            "one_at_a_time": round(alone, 1), "speedup_vs_one_at_a_time": round(value / alone, 2),
+            "one_at_a_time": round(alone[False], 1),
+            "speedup_vs_one_at_a_time": round(value / alone[False], 2),
+            "one_at_a_time_graphed": round(alone[True], 1),
+            "speedup_vs_one_at_a_time_graphed": round(value / alone[True], 2),
+            "one_at_a_time_note": ("each simulation alone with its own autotuned shapes: plain "
+                                   "form (one launch sequence per step) and with 64-step graphs"),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "note": "algorithmic bytes of both fused pairs per step, all members"},
@@ -154,10 +183,16 @@ def served_line(count: int, rate: float, seed: int) -> dict:
     stream served one solve at a time (one slot)."""
     from meep_gpu.serve import SolveServer, poisson_requests
 
+    from meep_gpu.engine import set_graphs
+
     reqs = poisson_requests(count=count, rate=rate, seed=seed)
     rep = SolveServer(max_slots=64, quantum=256).run(reqs)
-    one = SolveServer(max_slots=1, quantum=256).run(reqs)
-    s, o = rep.summary(), one.summary()
+    one = {}
+    for graphs in (False, True):
+        set_graphs(graphs)
+        one[graphs] = SolveServer(max_slots=1, quantum=256).run(reqs).summary()
+    set_graphs(True)
+    s, o, og = rep.summary(), one[False], one[True]
     return {"workload": f"{count} Poisson arrivals at {rate}/s (seed {seed}): sphere solves, "
                         f"edge 32-160 log-uniform, 30% Lorentz, 1-8 planes x 16-100 freqs, "
                         f"1000-8000 steps",
@@ -165,7 +200,12 @@ def served_line(count: int, rate: float, seed: int) -> dict:
             "one_at_a_time": {k: o[k] for k in ("p50_latency_s", "p99_latency_s",
                                                  "aggregate_mcell_steps_per_s", "makespan_s")},
             "speedup_vs_one_at_a_time": round(s["aggregate_mcell_steps_per_s"]
-                                              / max(o["aggregate_mcell_steps_per_s"], 1e-9), 2)}
+                                              / max(o["aggregate_mcell_steps_per_s"], 1e-9), 2),
+            "one_at_a_time_graphed": {k: og[k] for k in ("p50_latency_s", "p99_latency_s",
+                                                         "aggregate_mcell_steps_per_s",
+                                                         "makespan_s")},
+            "speedup_vs_one_at_a_time_graphed": round(
+                s["aggregate_mcell_steps_per_s"] / max(og["aggregate_mcell_steps_per_s"], 1e-9), 2)}
 
 
 def main(argv=None) -> int:
@@ -182,6 +222,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-count", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-saturated-rate", type=float, default=32.0,
+                    help="a second stream whose arrivals outpace the device (capacity)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every step eagerly (profiling the batched form)")
     ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[/V,BPC,REMAP,NT]",
                     help="fixed launch shapes (H pair[/D pair]) instead of the autotuned ones "
                          "(profiling runs)")
@@ -208,13 +252,17 @@ def main(argv=None) -> int:
     from meep_gpu import autotune
     from meep_gpu.pml import PML
 
+    if args.no_graphs:
+        from meep_gpu.engine import set_graphs
+
+        set_graphs(False)
     wl = workloads.config(args.workload)
     if args.launch:
         parts = args.launch.split("/")
         shapes = []
         for part in (parts[0], parts[-1]):
             v, bpc, remap, nt = (int(x) for x in part.split(","))
-            shapes.append((v, bpc, bool(remap), bool(nt)))
+            shapes.append((v, bpc, int(remap), bool(nt)))
         shape = tuple(shapes)
         autotune.install(shape)
     else:
@@ -233,6 +281,9 @@ def main(argv=None) -> int:
     value = cells * args.steps / elapsed / 1e6
     nbytes = algorithmic_bytes(sim)
     achieved = nbytes["d"] / (d_ms * 1e-3) / 1e9
+    dshape = shape[1]
+    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'}>"
+    traffic = measured_traffic(dkernel)
     spectra_ok = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
     line = {
         "metric": "Mcell-steps/s (plain single-simulation FDTD step, fused HIP kernels)",
@@ -254,7 +305,10 @@ def main(argv=None) -> int:
                    "cells": cells, "steps_timed": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": (f"profiles/traffic_pairs_sphere192.txt ({dkernel}: "
+                                        "FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
+                                        if traffic else None),
                      "kernel": "dpair_kernel (step_D+deposit+update_E+update_P)",
                      "bytes_per_launch": int(nbytes["d"]), "mean_launch_ms": round(d_ms, 5),
                      "h_pair": {"bytes_per_launch": int(nbytes["h"]),
@@ -263,7 +317,7 @@ def main(argv=None) -> int:
                      "step_model_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2)},
         "spectra_finite": bool(spectra_ok),
         "launch_shape": {name: {"cells_per_thread": sh[0], "blocks_per_cu": sh[1],
-                                "xcd_remap": bool(sh[2]), "nt_stores": bool(sh[3])}
+                                "xcd_remap": int(sh[2]), "nt_stores": bool(sh[3])}
                          for name, sh in (("h_pair", shape[0]), ("d_pair", shape[1]))},
     }
     drv.close()
@@ -273,6 +327,10 @@ def main(argv=None) -> int:
     if not args.no_served:
         print("bench: served form", file=sys.stderr, flush=True)
         line["served"] = served_line(args.serve_count, args.serve_rate, 20261203)
+        if args.serve_saturated_rate > 0:
+            print("bench: served form, saturated", file=sys.stderr, flush=True)
+            line["served_saturated"] = served_line(args.serve_count, args.serve_saturated_rate,
+                                                   20261203)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(line), flush=True)

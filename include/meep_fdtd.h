@@ -148,7 +148,8 @@ int fdtd_batch_synchronize(fdtd_batch *batch);
 
 /* Launch shape (the autotuner sets it; process-wide): vec = x-adjacent cells per thread (1, 2
  * or 4); blocks_per_cu = 0 for one workgroup per tile, else a persistent grid of that many
- * workgroups per CU; xcd_remap = 1 to give each XCD a contiguous eighth of the tile list;
+ * workgroups per CU; xcd_remap = 0 for the plain order, 1 to give each XCD a contiguous eighth of
+ * the tile list, sb >= 2 to deal super-blocks of sb tiles round-robin over the 8 XCDs;
  * nt_stores = 1 to stream the fused pairs' outputs with non-temporal stores. No counterpart
  * in the reference (its launches are one thread per cell). */
 int fdtd_set_launch(int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap, int32_t nt_stores);
@@ -180,6 +181,13 @@ int fdtd_run_timed(fdtd_sim *sim, int32_t nsteps, float out_ms[3]);
  * flight per thread, 2 the same with non-temporal stores, 3 read only, 4 write only (the time
  * is then for `bytes` moved once, not twice). No counterpart in the reference. */
 int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms);
+
+/* The fused pairs' access pattern without their arithmetic: `nread` read streams and `nwrite`
+ * write streams (<= 16 in all) of n floats, each touched once per thread at the same index,
+ * stream q based at q * (n*4 rounded to 2 MiB) + q * `stagger` bytes in one allocation. Mean
+ * ms over `repeats` launches. A measurement aid with no counterpart in the reference. */
+int fdtd_stream_probe(int32_t nread, int32_t nwrite, int32_t n, uint64_t stagger,
+                      int32_t repeats, float *ms);
 
 #ifdef __cplusplus
 }

@@ -104,22 +104,41 @@ struct SimDev {
 // ------------------------------------------------------------------------------------------
 struct TileWalk {
     int next, stride, end;
+    int xcd, sb;  // super-block order: tiles in blocks of `sb`, block q on XCD q % 8
 };
 
-__device__ inline TileWalk tile_walk(int ntiles, int xcd_remap) {
+// Which tiles this workgroup takes, in order. `remap` 0: tile b, b+G, ...; 1: each XCD takes a
+// contiguous eighth of the list; sb >= 2: the list is cut into super-blocks of sb tiles dealt
+// round-robin to the 8 XCDs, so an XCD's next tile usually follows its last one (the j-1 rows
+// it reads are in its own L2) while the eight XCDs still sweep neighbouring memory together.
+// With sb = tiles per plane / 8 the k-1 plane is on the same XCD too.
+__device__ inline TileWalk tile_walk(int ntiles, int remap) {
     const int G = gridDim.x, b = blockIdx.x;
     TileWalk w;
-    if (xcd_remap && (G % 8) == 0) {
+    w.xcd = 0;
+    w.sb = 0;
+    if (remap == 1 && (G % 8) == 0) {
         const int x = b % 8, s = b / 8, S = G / 8;
         w.next = (int)(((long long)ntiles * x) / 8) + s;
         w.stride = S;
         w.end = (int)(((long long)ntiles * (x + 1)) / 8);
+    } else if (remap >= 2 && (G % 8) == 0) {
+        w.xcd = b % 8;
+        w.sb = remap;
+        w.next = b / 8;
+        w.stride = G / 8;
+        w.end = ((ntiles + 8 * remap - 1) / (8 * remap)) * remap;  // slots per XCD
     } else {
         w.next = b;
         w.stride = G;
         w.end = ntiles;
     }
     return w;
+}
+
+// The tile of walk position g (may be >= ntiles in super-block order: skipped).
+__device__ inline int tile_at(const TileWalk &w, int g) {
+    return w.sb ? ((g / w.sb) * 8 + w.xcd) * w.sb + g % w.sb : g;
 }
 
 // Advance each simulation's step counter, H-start flag and P parity after its D pair (one
@@ -393,7 +412,9 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
-    for (int t = w.next; t < w.end; t += w.stride) {
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= ntiles) continue;
         const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         const int at = tile.y + V * (int)threadIdx.x;
@@ -585,7 +606,9 @@ __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restric
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
-    for (int t = w.next; t < w.end; t += w.stride) {
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= ntiles) continue;
         const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (s.offdiag) continue;  // stepped by the two-kernel row path below
@@ -618,7 +641,9 @@ __global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restric
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
-    for (int t = w.next; t < w.end; t += w.stride) {
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= ntiles) continue;
         const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
@@ -696,7 +721,9 @@ __global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict
                                                         const int2 *__restrict__ tiles,
                                                         int ntiles, int xcd_remap) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
-    for (int t = w.next; t < w.end; t += w.stride) {
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= ntiles) continue;
         const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
@@ -908,6 +935,24 @@ __global__ void copy_probe_kernel(const float4 *__restrict__ src, float4 *__rest
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x)
         dst[q] = src[q];
+}
+
+// The fused pairs' access pattern without the arithmetic: `nr` read streams and `nw` write
+// streams of n floats, every stream touched once at the same index by one thread.
+struct StreamSet {
+    float *p[16];
+};
+
+__global__ void stream_probe_kernel(StreamSet set, int nr, int nw, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        if (r < nr) acc += set.p[r][i];
+#pragma unroll
+    for (int w = 0; w < 16; ++w)
+        if (w < nw) set.p[nr + w][i] = acc + (float)w;
 }
 
 // Read-only and write-only streams (the two halves of a copy, measured apart).
@@ -1385,10 +1430,11 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
     if (pair != 0 && pair != 1) return fail(FDTD_EINVAL, "pair must be 0 (H) or 1 (D), got %d", pair);
     if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
+    if (xcd_remap < 0 || xcd_remap > 4096) return fail(FDTD_EINVAL, "xcd_remap %d", xcd_remap);
     LaunchShape &sh = g_shape[pair];
     sh.vec = vec;
     sh.blocks_per_cu = blocks_per_cu;
-    sh.xcd_remap = xcd_remap ? 1 : 0;
+    sh.xcd_remap = xcd_remap;
     sh.nt = nt_stores ? 1 : 0;
     return FDTD_OK;
 }
@@ -2001,6 +2047,38 @@ int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms) {
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
+    return FDTD_OK;
+}
+
+int fdtd_stream_probe(int32_t nread, int32_t nwrite, int32_t n, uint64_t stagger,
+                      int32_t repeats, float *ms) {
+    if (nread < 0 || nwrite < 1 || nread + nwrite > 16 || n < 256 || repeats < 1 || !ms ||
+        stagger > (1u << 20))
+        return fail(FDTD_EINVAL, "fdtd_stream_probe: bad argument");
+    const int q = nread + nwrite;
+    const size_t span = (((size_t)n * 4 + (2u << 20) - 1) / (2u << 20)) * (2u << 20);
+    const size_t total = q * span + (size_t)q * stagger + 4096;
+    char *arena = nullptr;
+    HIP_TRY(hipMalloc((void **)&arena, total));
+    HIP_TRY(hipMemset(arena, 0, total));
+    StreamSet set;
+    for (int r = 0; r < 16; ++r) set.p[r] = nullptr;
+    for (int r = 0; r < q; ++r) set.p[r] = (float *)(arena + r * span + (size_t)r * stagger);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int grid = (n + 255) / 256;
+    stream_probe_kernel<<<grid, 256>>>(set, nread, nwrite, n);
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; ++r) stream_probe_kernel<<<grid, 256>>>(set, nread, nwrite, n);
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(arena);
     return FDTD_OK;
 }
 

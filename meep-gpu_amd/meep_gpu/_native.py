@@ -67,6 +67,8 @@ SIGNATURES = {
     "fdtd_time_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
     "fdtd_run_timed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
     "fdtd_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _FP]),
+    "fdtd_stream_probe": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_uint64, ctypes.c_int32, _FP]),
 }
 
 

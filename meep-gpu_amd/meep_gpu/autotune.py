@@ -25,12 +25,34 @@ from .engine import (HipSimulation, device_info, get_launch_pair, set_launch,
 from .grid import Grid
 from .pml import PML
 
-Shape = Tuple[int, int, bool, bool]
+Shape = Tuple[int, int, int, bool]
 PairShapes = Tuple[Shape, Shape]  # (H pair, D pair)
 
 CANDIDATES: Tuple[Shape, ...] = tuple(
-    (vec, bpc, remap, nt) for vec in (1, 2, 4) for bpc in (0, 4, 8) for remap in (True, False)
+    (vec, bpc, remap, nt) for vec in (1, 2, 4) for bpc in (0, 4, 8) for remap in (0, 1)
     for nt in (False, True))
+
+
+def plane_superblock(grid: Grid, vec: int) -> int:
+    """Super-block size that keeps a tile's j-1 rows and k-1 plane on its own XCD: an eighth of
+    the tiles per z plane when that is a whole number (else 0)."""
+    nx, ny, _ = grid.shape
+    plane = ((nx + 3) // 4) * 4 * ny
+    tile = 256 * vec
+    if plane % tile:
+        return 0
+    per_plane = plane // tile
+    return per_plane // 8 if per_plane % 8 == 0 and per_plane // 8 >= 2 else 0
+
+
+def candidates_for(grid: Grid) -> Tuple[Shape, ...]:
+    """The generic candidates plus, per vector width, the plane-aligned super-block order."""
+    extra = []
+    for vec in (1, 2, 4):
+        sb = plane_superblock(grid, vec)
+        if sb:
+            extra += [(vec, bpc, sb, nt) for bpc in (0, 4, 8) for nt in (False, True)]
+    return CANDIDATES + tuple(extra)
 
 _CACHE = os.environ.get("MEEP_GPU_TUNE_CACHE",
                         os.path.join(os.path.expanduser("~"), ".cache", "meep_gpu_tune.json"))
@@ -59,9 +81,11 @@ def _store_cache(cache: Dict[str, list]) -> None:
 
 
 def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
-            candidates: Iterable[Shape] = CANDIDATES, repeats: int = 10
+            candidates: Optional[Iterable[Shape]] = None, repeats: int = 10
             ) -> List[Tuple[Shape, float, float]]:
     """(shape, mean H-pair ms, mean D-pair ms) for every candidate, on a scratch simulation."""
+    if candidates is None:
+        candidates = candidates_for(grid)
     sim = HipSimulation(grid, pml)
     for term, sigma in poles:
         sim.add_pole(term, sigma)

@@ -9,8 +9,8 @@ from for grids of 32^3..112^3 cells.
 
 ``synthetic_batch`` builds the benchmark's batch: edges log-uniform over [32, 112], sphere
 radius uniform in [0.1, 0.35] x edge, eps uniform in [1.5, 12], a quarter of them with a
-Lorentz pole (eps_inf 2.0, sigma 1.5, f0 1.1, gamma 0.05), 8-cell UPML, one flux plane x 32
-frequencies each, from one seed.
+Lorentz pole (eps_inf 2.0, sigma 1.5, f0 1.1, gamma 0.05) where the sphere lies outside the
+absorber, 8-cell UPML, one flux plane x 32 frequencies each, from one seed.
 """
 
 from __future__ import annotations
@@ -25,7 +25,7 @@ from . import _native
 from ._native import check
 from .dispersion import LORENTZIAN, Susceptibility
 from .driver import FdtdDriver
-from .workloads import FluxPlane, Workload, meep_frequency_span, sphere
+from .workloads import FluxPlane, Workload, meep_frequency_span, outside_absorber, sphere
 
 
 class BatchedRun:
@@ -86,8 +86,9 @@ def synthetic_sim(rng: numpy.random.Generator, edge_range=(32, 112), steps: int 
                 eps=eps, fcen=0.15, df=0.1, nfreq=nfreq, planes="one", steps=steps,
                 source_x=-0.2 * edge / resolution,
                 plane_x=0.2 * edge / resolution)
-    if dispersive:
-        sig = numpy.where(wl.inv["Ez"] < 1.0, numpy.float32(1.5), numpy.float32(0.0))
+    if dispersive:  # the pole inside the sphere and outside the absorber
+        inside = (wl.inv["Ez"] < 1.0) & outside_absorber(wl.grid, pml_cells)
+        sig = numpy.where(inside, numpy.float32(1.5), numpy.float32(0.0))
         wl.poles.append((Susceptibility(1.1, 0.05, LORENTZIAN),
                          {c: sig.astype(numpy.float32) for c in ("Ex", "Ey", "Ez")}))
         for c in ("Ex", "Ey", "Ez"):  # eps_inf 2.0 inside the sphere

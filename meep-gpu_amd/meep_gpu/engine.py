@@ -214,30 +214,31 @@ def copy_probe(nbytes: int, repeats: int = 20, mode: int = 0) -> float:
     return float(out[0])
 
 
-def set_launch(vec: int, blocks_per_cu: int, xcd_remap: bool, nt_stores: bool = False):
-    """Process-wide launch shape: cells per thread, workgroups per CU (0 = one per tile), XCD
-    remap, non-temporal output stores."""
-    check(_native.load().fdtd_set_launch(int(vec), int(blocks_per_cu), 1 if xcd_remap else 0,
+def set_launch(vec: int, blocks_per_cu: int, xcd_remap: int, nt_stores: bool = False):
+    """Process-wide launch shape: cells per thread, workgroups per CU (0 = one per tile), tile
+    order over the XCDs (0 plain, 1 contiguous eighths, sb >= 2 super-blocks of sb tiles),
+    non-temporal output stores."""
+    check(_native.load().fdtd_set_launch(int(vec), int(blocks_per_cu), int(xcd_remap),
                                          1 if nt_stores else 0), "fdtd_set_launch")
 
 
-def get_launch() -> Tuple[int, int, bool, bool]:
+def get_launch() -> Tuple[int, int, int, bool]:
     """The H pair's launch shape (``set_launch`` sets both pairs alike)."""
     return get_launch_pair(0)
 
 
-def set_launch_pair(pair: int, vec: int, blocks_per_cu: int, xcd_remap: bool,
+def set_launch_pair(pair: int, vec: int, blocks_per_cu: int, xcd_remap: int,
                     nt_stores: bool = False):
     """One fused pair's launch shape: pair 0 = H pair, 1 = D pair."""
     check(_native.load().fdtd_set_launch_pair(int(pair), int(vec), int(blocks_per_cu),
-                                              1 if xcd_remap else 0, 1 if nt_stores else 0),
+                                              int(xcd_remap), 1 if nt_stores else 0),
           "fdtd_set_launch_pair")
 
 
-def get_launch_pair(pair: int) -> Tuple[int, int, bool, bool]:
+def get_launch_pair(pair: int) -> Tuple[int, int, int, bool]:
     out = numpy.zeros(4, dtype=numpy.int32)
     check(_native.load().fdtd_get_launch_pair(int(pair), iptr(out)), "fdtd_get_launch_pair")
-    return int(out[0]), int(out[1]), bool(out[2]), bool(out[3])
+    return int(out[0]), int(out[1]), int(out[2]), bool(out[3])
 
 
 def set_graphs(enable: bool):
@@ -282,3 +283,11 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
         total += 512 * (len(planes) + 1)
     total += 16 * source_points + 16 * (source_steps + 1) + 256
     return int(total)
+
+
+def stream_probe(nread: int, nwrite: int, n: int, stagger: int = 0, repeats: int = 10) -> float:
+    """Mean ms of one launch touching ``nread`` + ``nwrite`` float streams of ``n`` cells."""
+    out = numpy.zeros(1, dtype=numpy.float32)
+    check(_native.load().fdtd_stream_probe(int(nread), int(nwrite), int(n), int(stagger),
+                                           int(repeats), fptr(out)), "fdtd_stream_probe")
+    return float(out[0])

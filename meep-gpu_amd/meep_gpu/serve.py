@@ -38,7 +38,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 import numpy
 
 from .dispersion import LORENTZIAN, Susceptibility
-from .workloads import FluxPlane, Workload, meep_frequency_span, sphere
+from .workloads import FluxPlane, Workload, meep_frequency_span, outside_absorber, sphere
 
 # --- requests --------------------------------------------------------------------------------
 
@@ -93,8 +93,8 @@ def request_workload(r: SolveRequest) -> Workload:
                 planes="none", steps=r.steps, source_x=-0.2 * r.edge / r.resolution)
     wl.planes = _request_planes(r.edge, r.pml_cells, radius, r.nplanes,
                                 meep_frequency_span(0.15, 0.1, r.nfreq))
-    if r.dispersive:  # the Lorentz pole inside the sphere, eps_inf 2.0 there
-        inside = wl.inv["Ez"] < 1.0
+    if r.dispersive:  # the Lorentz pole inside the sphere (out of the absorber), eps_inf 2.0
+        inside = (wl.inv["Ez"] < 1.0) & outside_absorber(wl.grid, r.pml_cells)
         sig = numpy.where(inside, numpy.float32(1.5), numpy.float32(0.0)).astype(numpy.float32)
         wl.poles.append((Susceptibility(1.1, 0.05, LORENTZIAN), {c: sig for c in ("Ex", "Ey", "Ez")}))
         for c in ("Ex", "Ey", "Ez"):

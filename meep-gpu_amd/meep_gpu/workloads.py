@@ -94,6 +94,26 @@ def _ez_point(grid: Grid, x: float, y: float, z: float) -> DepositBox:
     return meep_volume_source(grid, "Ez", (x, y, z), (0.0, 0.0, 0.0))
 
 
+def outside_absorber(grid: Grid, pml_cells: int) -> numpy.ndarray:
+    """Cells at least ``pml_cells`` from every face of a resolved axis.
+
+    A Lorentz susceptibility must stay out of the UPML: with the pole's sigma inside the layer
+    the coupled update grows without bound (the oracle, which restates the reference's array
+    path, blows up the same way on those inputs), so workloads keep their dispersive material
+    inside this mask.
+    """
+    mask = numpy.ones(grid.shape, dtype=bool)
+    for axis, n in enumerate(grid.shape):
+        if grid.is_invariant(axis) or pml_cells <= 0:
+            continue
+        idx = numpy.arange(n)
+        keep = (idx >= pml_cells) & (idx < n - pml_cells)
+        shape = [1, 1, 1]
+        shape[axis] = n
+        mask &= keep.reshape(shape)
+    return mask
+
+
 def _plane_index(x: float, edge: int, dx: float) -> int:
     """The cell whose centre is nearest the coordinate ``x`` (origin at the cell centre): flux
     planes sample cell-centred fields."""

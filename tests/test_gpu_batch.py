@@ -9,11 +9,12 @@ pytestmark = pytest.mark.gpu
 # (H pair shape, D pair shape): cells per thread, workgroups per CU, XCD remap, NT stores
 SHAPES = [((1, 0, True, False),) * 2, ((2, 0, True, True),) * 2, ((4, 0, False, False),) * 2,
           ((2, 8, True, False),) * 2, ((4, 4, True, True),) * 2, ((1, 8, False, True),) * 2,
-          ((1, 8, False, True), (2, 0, False, True)), ((4, 0, True, False), (1, 4, False, True))]
+          ((1, 8, False, True), (2, 0, False, True)), ((4, 0, True, False), (1, 4, False, True)),
+          ((1, 8, 18, True), (2, 0, 9, True)), ((2, 4, 5, False),) * 2, ((4, 0, 3, True),) * 2]
 
 
 def _sid(pair):
-    return "_".join(f"v{s[0]}b{s[1]}r{int(s[2])}n{int(s[3])}" for s in pair)
+    return "_".join(f"v{s[0]}b{s[1]}r{int(s[2])}n{int(s[3])}" for s in pair)  # r: tile order
 
 
 @pytest.mark.parametrize("shapes", SHAPES, ids=_sid)

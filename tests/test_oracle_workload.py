@@ -39,3 +39,23 @@ def test_flux_transform_matches_a_direct_dft_of_the_sampled_fields():
     direct = numpy.einsum("nf,nc->fc", numpy.exp(1j * numpy.outer(times, plane.omegas)),
                           samples) * plane.scale
     assert numpy.allclose(plane.acc64[0], direct, rtol=1e-12, atol=1e-14 * numpy.abs(direct).max())
+
+
+def test_synthetic_dispersive_member_stays_bounded():
+    """The synthetic batch keeps the Lorentz pole out of the UPML: with it inside the layer the
+    coupled update grows without bound (x1e4 by step 1500 on this member), without it the
+    field after the pulse stays at the pulse's own scale."""
+    import numpy
+
+    from meep_gpu.batch import synthetic_batch
+    from oracle import workload as ow
+    from tests.workload_runs import oracle_run
+
+    members = [w for w in synthetic_batch(count=64, seed=20261203, steps=3000) if w.poles]
+    w = min(members, key=lambda m: m.grid.total_cells)
+    sim, planes = oracle_run(w, 0, reference_precision=False)
+    ow.run(sim, planes, 600, False)
+    early = float(numpy.abs(sim.get_E("Ez")).max())
+    ow.run(sim, planes, 1400, False)
+    late = float(numpy.abs(sim.get_E("Ez")).max())
+    assert numpy.isfinite(late) and late < 10 * early
