@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 
 def algorithmic_bytes(sim) -> dict:
     """Compulsory HBM bytes per launch of each fused pair (every array touched once, neighbour
-    reads free, auxiliaries only on their membership rows)."""
+    reads free, auxiliaries only on their membership rows, inverse epsilon at the width the
+    engine stores it: 1 or 2 byte table indices where it has few distinct words)."""
     grid = sim.grid
     n = grid.total_cells
     pml = sim.pml
@@ -41,13 +42,14 @@ def algorithmic_bytes(sim) -> dict:
     # H r/w + f_w_H r/w 16 on its own-axis line (integer lattice; H equals B elsewhere and is
     # not stored there).
     h = n * (12 + 24)
-    d = n * (12 + 24 + 12 + 12)  # H-or-B r, D r/w, inv r, E w
+    d = n * (12 + 24 + 12 + sum(sim.inverse_epsilon_bytes()))  # H-or-B r, D r/w, E w, inv r
     for c in range(3):
         dsigu = (c + 2) % 3
         h += n * (8 * frac(dsigu, 1) + 16 * frac(c, 0))
         d += n * (8 * frac(dsigu, 0) + 12 * frac(c, 1))
-    for driven in sim.poles:
-        d += n * 16 * sum(driven)
+    rows = sim.pole_row_fraction()
+    for driven in sim.poles:  # P r, P_prev r, P' w, sigma r on the rows that carry pole work
+        d += n * 16 * sum(driven) * rows
     return {"h": h, "d": d}
 
 
@@ -106,6 +108,44 @@ def measured_traffic(kernel: str):
 def step_bytes(sim) -> float:
     b = algorithmic_bytes(sim)
     return float(b["h"] + b["d"])
+
+
+def stream_ceiling(cells: int) -> dict:
+    """The HBM rate the D pair's stream pattern reaches with no arithmetic: 9 read and 6 write
+    float32 streams of ``cells`` (H-or-B, D, inv in; D, E out), one touch per thread."""
+    from meep_gpu.engine import stream_probe
+
+    ms = stream_probe(9, 6, cells, 0, 20)
+    gbs = 15 * cells * 4 / (ms * 1e-3) / 1e9
+    return {"GB/s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "probe": f"fdtd_stream_probe 9 reads + 6 writes x {cells} floats, no arithmetic"}
+
+
+def config_line(name: str, steps: int) -> dict:
+    """Mcell-steps/s of another benchmark configuration (autotuned, graphs, flux every step)."""
+    from meep_gpu import autotune, workloads
+    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.pml import PML
+
+    wl = workloads.config(name)
+    shapes = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
+                           wl.poles, use_cache=False, inv=wl.inv)
+    drv = FdtdDriver.from_workload(wl)
+    sim = drv.freeze(planned_steps=steps + 101)
+    sim.run(100)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
+    drv.close()
+    return {"workload": f"{wl.name}: {wl.grid.total_cells} cells, {len(wl.poles)} Lorentz pole(s), "
+                        f"{len(wl.planes)} flux planes x {wl.planes[0].frequencies.size} freqs",
+            "steps_timed": steps, "value": round(wl.grid.total_cells * steps / elapsed / 1e6, 1),
+            "unit": "Mcell-steps/s", "ms_per_step": round(elapsed * 1e3 / steps, 4),
+            "launch_shape": {"h_pair": list(shapes[0]), "d_pair": list(shapes[1])},
+            "spectra_finite": bool(finite)}
 
 
 def batched_line(count: int, steps: int, seed: int) -> dict:
@@ -217,6 +257,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the sphere80 and slab160 lines")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-served", action="store_true")
@@ -267,7 +309,7 @@ def main(argv=None) -> int:
         autotune.install(shape)
     else:
         shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
-                              wl.poles, use_cache=False)
+                              wl.poles, use_cache=False, inv=wl.inv)
     drv = FdtdDriver.from_workload(wl)
     sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
     print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
@@ -321,6 +363,11 @@ def main(argv=None) -> int:
                          for name, sh in (("h_pair", shape[0]), ("d_pair", shape[1]))},
     }
     drv.close()
+    line["roofline"]["stream_ceiling"] = stream_ceiling(2 * cells)
+    if not args.no_configs:
+        print("bench: other configurations", file=sys.stderr, flush=True)
+        line["configs"] = {"sphere80": config_line("sphere80", 2000),
+                           "slab160": config_line("slab160", 2000)}
     if not args.no_batched:
         print("bench: batched form", file=sys.stderr, flush=True)
         line["batched"] = batched_line(args.batch_count, args.batch_steps, 20261203)

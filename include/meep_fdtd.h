@@ -72,6 +72,17 @@ int fdtd_set_layer(fdtd_sim *sim, int32_t half, int32_t axis, const float *kms,
 /* Diagonal chi1inv for E component c (0..2), one volume (fields.py:1396, :1523). */
 int fdtd_set_inverse_epsilon(fdtd_sim *sim, int32_t component, const float *volume);
 
+/* Bytes per cell the D pair streams for each component's inverse epsilon: 1 or 2 when the
+ * volume has at most 256 or 65536 distinct words (stored as indices into a table of them, the
+ * same words), else 4. No counterpart in the reference (its chi1inv is a float32 volume). */
+int fdtd_inverse_epsilon_bytes(fdtd_sim *sim, int32_t out[3]);
+
+/* Cells (of nx*ny*nz, in `total`) on which the D pair runs pole work: per row, the x range
+ * covering every cell where some pole's sigma or a stored polarization is nonzero (elsewhere
+ * every P stays +0 and D - sum P is D). No counterpart in the reference (it updates P over the
+ * whole grid, dispersion.py:689). */
+int fdtd_pole_rows(fdtd_sim *sim, int64_t *active, int64_t *total);
+
 /* Off-diagonal chi1inv[row][partner] volume (fields.py:1396, stepping.py:1594). A row with
  * any partner switches update_E to the double-precision row form. */
 int fdtd_set_offdiagonal(fdtd_sim *sim, int32_t row, int32_t partner, const float *volume);

@@ -27,6 +27,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/meep_fdtd.h"
@@ -87,9 +88,16 @@ struct SimDev {
     float *D[3], *B[3], *E[3], *H[3];
     float *fuB[3], *fuD[3], *fwE[3], *fwH[3];
     const float *inv[3];
+    // inverse epsilon as an index into a table of its distinct words, when it has few: the D
+    // pair streams 1 or 2 bytes per cell instead of 4 (the words are the same)
+    const uint8_t *inv8[3];
+    const uint16_t *inv16[3];
+    const float *invtab[3];
     const float *u[3][3];
     const float4 *line[2][3];  // (kms, sinv, kps, member) per row, [half][axis]
     int mr[2][3][2 * kMaxRuns];  // the membership lines as up to kMaxRuns [lo, hi) runs
+    const int2 *prow;  // per row (j + k*ny): the x range [lo, hi) where a pole may hold or
+                       // gain polarization (empty: lo == hi)
     float pc[kMaxPoles][4];
     int pdriven[kMaxPoles][3];
     float *pbuf[kMaxPoles][3][2];
@@ -182,39 +190,51 @@ struct VF {
     float v[V];
 };
 
+// Every field pointer the kernels follow is device memory: say so, so that the accesses are
+// global (vector-memory only) rather than flat ones, which also count against the scalar
+// memory counter and make each wait for a descriptor word wait for every field load in flight.
+#define GMEM __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GMEM T *gp(T *p) { return (GMEM T *)p; }
+template <class T>
+__device__ __forceinline__ const GMEM T *gp(const T *p) { return (const GMEM T *)p; }
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
 template <int V>
 __device__ inline VF<V> ldv(const float *p, int at) {
     VF<V> r;
+    const GMEM float *g = gp(p) + at;
     if constexpr (V == 4) {
-        const float4 t = *reinterpret_cast<const float4 *>(p + at);
+        const f4v t = *reinterpret_cast<const GMEM f4v *>(g);
         r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
     } else if constexpr (V == 2) {
-        const float2 t = *reinterpret_cast<const float2 *>(p + at);
+        const f2v t = *reinterpret_cast<const GMEM f2v *>(g);
         r.v[0] = t.x; r.v[1] = t.y;
     } else {
-        r.v[0] = p[at];
+        r.v[0] = *g;
     }
     return r;
 }
 
 // Store a run; with NT the store is marked non-temporal (streamed past the caches, for the
 // outputs no later read of the same launch wants).
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef float f2v __attribute__((ext_vector_type(2)));
 
 template <int V, bool NT = false>
 __device__ inline void stv(float *p, int at, const VF<V> &x) {
+    GMEM float *g = gp(p) + at;
     if constexpr (V == 4) {
         const f4v v = {x.v[0], x.v[1], x.v[2], x.v[3]};
-        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(p + at));
-        else *reinterpret_cast<f4v *>(p + at) = v;
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<GMEM f4v *>(g));
+        else *reinterpret_cast<GMEM f4v *>(g) = v;
     } else if constexpr (V == 2) {
         const f2v v = {x.v[0], x.v[1]};
-        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f2v *>(p + at));
-        else *reinterpret_cast<f2v *>(p + at) = v;
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<GMEM f2v *>(g));
+        else *reinterpret_cast<GMEM f2v *>(g) = v;
     } else {
-        if constexpr (NT) __builtin_nontemporal_store(x.v[0], p + at);
-        else p[at] = x.v[0];
+        if constexpr (NT) __builtin_nontemporal_store(x.v[0], g);
+        else *g = x.v[0];
     }
 }
 
@@ -226,9 +246,10 @@ __device__ inline VF<V> x_up(const SimDev &s, const Run &q, const float *f, cons
     VF<V> r;
 #pragma unroll
     for (int m = 0; m + 1 < V; ++m) r.v[m] = v.v[m + 1];
-    r.v[V - 1] = (q.i0 + V < nx) ? f[q.at + V] : f[q.row];
+    const GMEM float *gf = gp(f);
+    r.v[V - 1] = (q.i0 + V < nx) ? gf[q.at + V] : gf[q.row];
     if (q.i0 + V > nx && q.i0 < nx) {
-        const float w0 = f[q.row];
+        const float w0 = gf[q.row];
         const int last = nx - 1 - q.i0;
 #pragma unroll
         for (int m = 0; m < V; ++m)
@@ -243,7 +264,8 @@ __device__ inline VF<V> x_down(const SimDev &s, const Run &q, const float *f, co
     VF<V> r;
 #pragma unroll
     for (int m = 1; m < V; ++m) r.v[m] = v.v[m - 1];
-    r.v[0] = q.i0 > 0 ? f[q.at - 1] : f[q.row + s.n[0] - 1];
+    const GMEM float *gf = gp(f);
+    r.v[0] = q.i0 > 0 ? gf[q.at - 1] : gf[q.row + s.n[0] - 1];
     return r;
 }
 
@@ -267,11 +289,12 @@ __device__ inline float curl_of(float dtdx, int p1, int p2, float g1, float g1s,
 __device__ inline float curl_apply_pml(float f, float curl, float4 lsig, float4 lsigu, float *fu,
                                        int at) {
     if (lsigu.w != 0.0f) {
-        const float prev = fu[at];
+        GMEM float *gfu = gp(fu);
+        const float prev = gfu[at];
         float u = lsig.x * prev;
         u = u - curl;
         u = lsig.y * u;
-        fu[at] = u;
+        gfu[at] = u;
         float v = lsigu.x * f;
         v = v + u;
         v = v - prev;
@@ -293,21 +316,21 @@ __device__ inline void deposit_run(const SimDev &s, int field, int comp, const R
         q.k >= s.src_hi[field][2] || q.i0 + V <= s.src_lo[field][0] || q.i0 >= s.src_hi[field][0])
         return;
     for (int n = 0; n < s.nsrc; ++n) {
-        const SrcDev &d = s.src[n];
+        const GMEM SrcDev &d = gp(s.src)[n];
         if (d.field != field || d.comp != comp) continue;
         const int b = q.j - d.lo[1], e = q.k - d.lo[2];
         if (b < 0 || e < 0 || b >= d.cnt[1] || e >= d.cnt[2]) continue;
         if (q.i0 + V <= d.lo[0] || q.i0 >= d.lo[0] + d.cnt[0]) continue;
         if (s.step >= d.ncur) continue;
-        const double cre = d.cur[2 * s.step], cim = d.cur[2 * s.step + 1];
+        const double cre = gp(d.cur)[2 * s.step], cim = gp(d.cur)[2 * s.step + 1];
         if (plus_zero(cre) && plus_zero(cim)) continue;
 #pragma unroll
         for (int m = 0; m < V; ++m) {
             const int a = q.i0 + m - d.lo[0];
             if (a < 0 || a >= d.cnt[0]) continue;
             const int p = (e * d.cnt[1] + b) * d.cnt[0] + a;
-            double r = d.amp_re[p] * cre;
-            r = r - d.amp_im[p] * cim;
+            double r = gp(d.amp_re)[p] * cre;
+            r = r - gp(d.amp_im)[p] * cim;
             r = r * d.dt;
             double w = (double)x.v[m];
             w = w - r;
@@ -333,7 +356,9 @@ __device__ inline bool member(const SimDev &s, int half, int axis, int c) {
 }
 
 __device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool in) {
-    return in ? s.line[half][axis][c] : make_float4(1.f, 1.f, 1.f, 0.f);
+    if (!in) return make_float4(1.f, 1.f, 1.f, 0.f);
+    const GMEM float *l = reinterpret_cast<const GMEM float *>(gp(s.line[half][axis]) + c);
+    return make_float4(l[0], l[1], l[2], l[3]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -371,16 +396,16 @@ __device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V
         for (int m = 0; m < V; ++m) {
             const int cw = coord_of(q, C, m);
             if (member(s, 0, C, cw)) {
-                const float4 lw = s.line[0][C][cw];
+                const float4 lw = coeff(s, 0, C, cw, true);
                 float hold = b.v[m], prev = b.v[m];
                 if (s.hsep) {
-                    hold = s.H[C][q.at + m];
-                    prev = s.fwH[C][q.at + m];
+                    hold = gp(s.H[C])[q.at + m];
+                    prev = gp(s.fwH[C])[q.at + m];
                 }
-                s.fwH[C][q.at + m] = b.v[m];
+                gp(s.fwH[C])[q.at + m] = b.v[m];
                 const float a = lw.z * b.v[m];
                 const float mm = lw.x * prev;
-                s.H[C][q.at + m] = hold + (a - mm);
+                gp(s.H[C])[q.at + m] = hold + (a - mm);
             }
         }
     }
@@ -460,7 +485,7 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Run &q, const VF<
 // drive W is f_w on the own-axis layer's cells and E elsewhere: the constitutive product both.
 template <int V, int C, bool PML, bool NT>
 __device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
-                               const VF<V> (&p)[kMaxPoles]) {
+                               const VF<V> (&p)[kMaxPoles], bool active) {
     VF<V> e;
     if (PML) {
 #pragma unroll
@@ -468,12 +493,12 @@ __device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
             const int cw = coord_of(q, C, m);
             float ev = cst.v[m];
             if (member(s, 1, C, cw)) {
-                const float4 lw = s.line[1][C][cw];
-                const float prev = s.fwE[C][q.at + m];
-                s.fwE[C][q.at + m] = cst.v[m];
+                const float4 lw = coeff(s, 1, C, cw, true);
+                const float prev = gp(s.fwE[C])[q.at + m];
+                gp(s.fwE[C])[q.at + m] = cst.v[m];
                 const float a = lw.z * cst.v[m];
                 const float mm = lw.x * prev;
-                ev = s.E[C][q.at + m] + (a - mm);
+                ev = gp(s.E[C])[q.at + m] + (a - mm);
             }
             e.v[m] = ev;
         }
@@ -481,6 +506,7 @@ __device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
         e = cst;
     }
     stv<V, NT>(s.E[C], q.at, e);
+    if (!active) return;  // no polarization on this row, now or later: P stays +0
     const int par = s.ppar;
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
@@ -511,22 +537,79 @@ __device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
 }
 
 // D - sum P for component C from the D values in `d`, loading each driving pole's P^n.
+__device__ inline bool pole_run_active(const SimDev &s, const Run &q, int V) {
+    if (!s.npol) return false;
+    if (!s.prow) return true;
+    const GMEM int *r = reinterpret_cast<const GMEM int *>(gp(s.prow) + q.j + q.k * s.n[1]);
+    return q.i0 < r[1] && q.i0 + V > r[0];
+}
+
 template <int V, int C>
 __device__ inline VF<V> minus_poles(const SimDev &s, const Run &q, const VF<V> &d,
-                                    VF<V> (&p)[kMaxPoles]) {
+                                    VF<V> (&p)[kMaxPoles], bool active) {
     VF<V> dmp = d;
     const int par = s.ppar;
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
 #pragma unroll
         for (int m = 0; m < V; ++m) p[pq].v[m] = 0.0f;
-        if (pq >= s.npol) continue;
+        if (pq >= s.npol || !active) continue;
         if (!s.pdriven[pq][C]) continue;
         p[pq] = ldv<V>(s.pbuf[pq][C][par], q.at);
 #pragma unroll
         for (int m = 0; m < V; ++m) dmp.v[m] = dmp.v[m] - p[pq].v[m];
     }
     return dmp;
+}
+
+// Inverse epsilon in two parts: the index words (or nothing, for a float volume) are loaded
+// first of everything the D pair reads, so the table look-ups that depend on them go out while
+// the field loads are still in flight instead of after all of them.
+struct InvWords {
+    uint32_t w[2];
+};
+
+// INV: the storage every simulation of the launch uses for all three components -- 0 float,
+// 1 byte indices, 2 16-bit indices -- or 3, mixed (chosen per component at run time).
+template <int V, int INV>
+__device__ inline InvWords load_inv_words(const SimDev &s, int c, int at) {
+    InvWords r;
+    r.w[0] = r.w[1] = 0u;
+    if (INV == 0) return r;
+    if ((INV == 1) || (INV == 3 && s.inv8[c])) {
+        const GMEM uint8_t *g = gp(s.inv8[c]) + at;
+        if (V == 1) r.w[0] = *g;
+        else if (V == 2) r.w[0] = *reinterpret_cast<const GMEM uint16_t *>(g);
+        else r.w[0] = *reinterpret_cast<const GMEM uint32_t *>(g);
+    } else if ((INV == 2) || (INV == 3 && s.inv16[c])) {
+        const GMEM uint16_t *g = gp(s.inv16[c]) + at;
+        if (V == 1) r.w[0] = *g;
+        else if (V == 2) r.w[0] = *reinterpret_cast<const GMEM uint32_t *>(g);
+        else {
+            typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+            const u2v x = *reinterpret_cast<const GMEM u2v *>(g);
+            r.w[0] = x.x;
+            r.w[1] = x.y;
+        }
+    }
+    return r;
+}
+
+template <int V, int INV>
+__device__ inline VF<V> inv_values(const SimDev &s, int c, int at, const InvWords &iw) {
+    VF<V> r;
+    if (INV == 0) return ldv<V>(s.inv[c], at);
+    if ((INV == 1) || (INV == 3 && s.inv8[c])) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) r.v[m] = gp(s.invtab[c])[(iw.w[0] >> (8 * m)) & 0xffu];
+        return r;
+    }
+    if ((INV == 2) || (INV == 3 && s.inv16[c])) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) r.v[m] = gp(s.invtab[c])[(iw.w[m / 2] >> (16 * (m % 2))) & 0xffffu];
+        return r;
+    }
+    return ldv<V>(s.inv[c], at);
 }
 
 template <int V, int C, bool PML, bool NT>
@@ -536,11 +619,15 @@ __device__ inline void dpair_component(const SimDev &s, const Run &q, const VF<V
     const VF<V> d = dcurl_component<V, C, PML>(s, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
     VF<V> p[kMaxPoles];
-    const VF<V> dmp = minus_poles<V, C>(s, q, d, p);
+    // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
+    // is g1inv*((A*0 - g1*0) + od*(0*w)) = +0), and D - (+0) is D: runs with no such cell
+    // skip the pole work (a run straddling the range updates its +0 cells to +0)
+    const bool active = pole_run_active(s, q, V);
+    const VF<V> dmp = minus_poles<V, C>(s, q, d, p, active);
     VF<V> cst;
 #pragma unroll
     for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
-    e_and_p<V, C, PML, NT>(s, q, cst, p);
+    e_and_p<V, C, PML, NT>(s, q, cst, p, active);
 }
 
 // H component c at the run shifted by `off` (a shift across c's own axis): the stored H on
@@ -554,7 +641,7 @@ __device__ inline VF<V> ld_hb(const SimDev &s, const Run &q, int c, int off) {
     VF<V> r = ldv<V>(s.B[0], q.at + off);
 #pragma unroll
     for (int m = 0; m < V; ++m)
-        if (member(s, 0, 0, q.i0 + m)) r.v[m] = s.H[0][q.at + off + m];
+        if (member(s, 0, 0, q.i0 + m)) r.v[m] = gp(s.H[0])[q.at + off + m];
     return r;
 }
 
@@ -585,23 +672,25 @@ __device__ inline void load_h(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V
     hn[2][1] = ldv<V>(Hs[2], q.at + q.ym);
 }
 
-template <int V, bool PML, bool NT>
+template <int V, bool PML, bool NT, int INV>
 __device__ inline void dpair_run(const SimDev &s, const Run &q) {
+    InvWords iw[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV>(s, c, q.at);
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
     // every stream's load issued before the first store (see hpair_run)
     VF<V> d[3], iv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        d[c] = ldv<V>(s.D[c], q.at);
-        iv[c] = ldv<V>(s.inv[c], q.at);
-    }
+    for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
     dpair_component<V, 0, PML, NT>(s, q, h, hn, d[0], iv[0]);
     dpair_component<V, 1, PML, NT>(s, q, h, hn, d[1], iv[1]);
     dpair_component<V, 2, PML, NT>(s, q, h, hn, d[2], iv[2]);
 }
 
-template <int V, bool NT>
+template <int V, bool NT, int INV>
 __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap) {
@@ -616,9 +705,9 @@ __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restric
         if (at >= s.npad) continue;
         const Run q = locate(s, at);
         if (s.pml)
-            dpair_run<V, true, NT>(s, q);
+            dpair_run<V, true, NT, INV>(s, q);
         else
-            dpair_run<V, false, NT>(s, q);
+            dpair_run<V, false, NT, INV>(s, q);
     }
 }
 
@@ -681,7 +770,8 @@ __device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, i
 template <int V, int C, bool PML>
 __device__ inline void erow_component(const SimDev &s, const Run &q) {
     VF<V> p[kMaxPoles];
-    const VF<V> dmp = minus_poles<V, C>(s, q, ldv<V>(s.D[C], q.at), p);
+    const bool active = pole_run_active(s, q, V);
+    const VF<V> dmp = minus_poles<V, C>(s, q, ldv<V>(s.D[C], q.at), p, active);
     const VF<V> iv = ldv<V>(s.inv[C], q.at);
     VF<V> cst;
     const float *u1 = s.u[C][(C + 1) % 3], *u2 = s.u[C][(C + 2) % 3];
@@ -713,7 +803,7 @@ __device__ inline void erow_component(const SimDev &s, const Run &q) {
         }
         cst.v[m] = (float)row;
     }
-    e_and_p<V, C, PML, false>(s, q, cst, p);
+    e_and_p<V, C, PML, false>(s, q, cst, p, active);
 }
 
 template <int V>
@@ -1048,7 +1138,29 @@ struct Group {
     std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
     std::map<int, int> ntiles;
     bool any_offdiag = false, all_offdiag = false;
+    int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
 };
+
+// The one inverse-epsilon storage all three components of a descriptor use, or 3 (mixed).
+int inv_mode_of(const SimDev &h) {
+    int m = -1;
+    for (int c = 0; c < 3; ++c) {
+        const int mc = h.inv8[c] ? 1 : h.inv16[c] ? 2 : 0;
+        if (m < 0) m = mc;
+        else if (m != mc) return 3;
+    }
+    return m;
+}
+
+template <int V, bool NT>
+void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, hipStream_t st) {
+    switch (g.inv_mode) {
+        case 0: dpair_kernel<V, NT, 0><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
+        case 1: dpair_kernel<V, NT, 1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
+        case 2: dpair_kernel<V, NT, 2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
+        default: dpair_kernel<V, NT, 3><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
+    }
+}
 
 int launch_h(const Group &g, hipStream_t st) {
     const LaunchShape &sh = g_shape[0];
@@ -1079,13 +1191,13 @@ int launch_d(const Group &g, hipStream_t st) {
     const int rm = sh.xcd_remap;
     if (!g.all_offdiag) {
         if (sh.nt) {
-            if (V == 4) dpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            else if (V == 2) dpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            else dpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            if (V == 4) launch_dpair<4, true>(g, G, tl, nt, rm, st);
+            else if (V == 2) launch_dpair<2, true>(g, G, tl, nt, rm, st);
+            else launch_dpair<1, true>(g, G, tl, nt, rm, st);
         } else {
-            if (V == 4) dpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            else if (V == 2) dpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            else dpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            if (V == 4) launch_dpair<4, false>(g, G, tl, nt, rm, st);
+            else if (V == 2) launch_dpair<2, false>(g, G, tl, nt, rm, st);
+            else launch_dpair<1, false>(g, G, tl, nt, rm, st);
         }
     }
     if (g.any_offdiag) {
@@ -1138,6 +1250,7 @@ struct fdtd_sim {
     PlaneDev *dev_planes = nullptr;
     int max_plane_cells = 0;
     std::vector<uint8_t> hmember[3];  // integer-lattice layer membership: where H is stored
+    std::vector<int2> prow;           // per row (j + k*ny): x range of pole work; see mark_rows
     uint64_t bytes = 0;
     bool dirty = true;
     bool in_batch = false;
@@ -1185,6 +1298,7 @@ struct fdtd_sim {
             g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
         }
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
+        g.inv_mode = inv_mode_of(host);
         return g;
     }
     int push_planes() {
@@ -1604,13 +1718,75 @@ static int download_volume(fdtd_sim *s, float *host, const float *src) {
 
 int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     if (!s || c < 0 || c > 2 || !vol) return fail(FDTD_EINVAL, "fdtd_set_inverse_epsilon: bad argument");
-    return upload_volume(s, (float *)s->host.inv[c], vol);
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int rc = upload_volume(s, (float *)s->host.inv[c], vol);
+    if (rc) return rc;
+    // the distinct words, by bit pattern (so -0 and NaN payloads stay themselves)
+    SimDev &h = s->host;
+    std::unordered_map<uint32_t, uint32_t> index;
+    std::vector<float> table;
+    const size_t ncell = (size_t)h.ncell;
+    for (size_t q = 0; q < ncell && table.size() <= 65536; ++q) {
+        uint32_t w;
+        std::memcpy(&w, vol + q, 4);
+        if (index.emplace(w, (uint32_t)table.size()).second) table.push_back(vol[q]);
+    }
+    rc = s->pull_desc();
+    if (rc) return rc;
+    h.inv8[c] = nullptr;
+    h.inv16[c] = nullptr;
+    h.invtab[c] = nullptr;
+    s->dirty = true;
+    s->graph.reset();  // a captured sequence launched the kernels of the old storage
+    if (table.size() > 65536) return FDTD_OK;  // the float volume serves
+    const bool narrow = table.size() <= 256;
+    const size_t rows = (size_t)h.n[1] * h.n[2], esize = narrow ? 1 : 2;
+    std::vector<uint8_t> idx((size_t)h.npad * esize, 0);
+    for (size_t r = 0; r < rows; ++r)
+        for (int i = 0; i < h.n[0]; ++i) {
+            uint32_t w;
+            std::memcpy(&w, vol + r * h.n[0] + i, 4);
+            const uint32_t k = index[w];
+            const size_t at = r * h.pitch + i;
+            if (narrow) idx[at] = (uint8_t)k;
+            else std::memcpy(&idx[at * 2], &k, 2);
+        }
+    void *dev_idx = nullptr, *dev_tab = nullptr;
+    rc = s->alloc(&dev_idx, idx.size());
+    if (rc) return rc;
+    rc = s->alloc(&dev_tab, sizeof(float) * table.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dev_idx, idx.data(), idx.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(dev_tab, table.data(), sizeof(float) * table.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (narrow) h.inv8[c] = (const uint8_t *)dev_idx;
+    else h.inv16[c] = (const uint16_t *)dev_idx;
+    h.invtab[c] = (const float *)dev_tab;
+    return FDTD_OK;
+}
+
+int fdtd_pole_rows(fdtd_sim *s, int64_t *active, int64_t *rows) {
+    if (!s || !active || !rows) return fail(FDTD_EINVAL, "null argument");
+    *rows = (int64_t)s->host.n[1] * s->host.n[2];
+    int64_t a = 0;
+    for (const int2 &v : s->prow) a += v.y > v.x ? v.y - v.x : 0;
+    *active = s->host.npol ? a : 0;  // cells, in units of rows
+    *rows = *rows * s->host.n[0];
+    return FDTD_OK;
+}
+
+int fdtd_inverse_epsilon_bytes(fdtd_sim *s, int32_t out[3]) {
+    if (!s || !out) return fail(FDTD_EINVAL, "null argument");
+    for (int c = 0; c < 3; ++c)
+        out[c] = s->host.inv8[c] ? 1 : s->host.inv16[c] ? 2 : 4;
+    return FDTD_OK;
 }
 
 int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float *vol) {
     if (!s || row < 0 || row > 2 || partner < 0 || partner > 2 || row == partner || !vol)
         return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    s->graph.reset();  // the off-diagonal path launches other kernels
     int rc = s->pull_desc();
     if (rc) return rc;
     if (!s->host.u[row][partner]) {
@@ -1622,6 +1798,52 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
     s->host.offdiag = 1;
     s->dirty = true;
     return upload_volume(s, (float *)s->host.u[row][partner], vol);
+}
+
+// Per row (j + k*ny), the x range [lo, hi) where pole work must run: it covers every nonzero
+// word (bitwise, so -0 counts) of `vol` (host layout, x fastest, unpadded), or the whole row
+// when `all`. Ranges only grow. Outside them every pole's P stays +0 -- exact only while the
+// pole constants A, gamma1 and 1/gamma1 are non-negative, which the caller checks.
+static int mark_rows(fdtd_sim *s, const float *vol, bool all) {
+    SimDev &h = s->host;
+    const size_t rows = (size_t)h.n[1] * h.n[2];
+    if (s->prow.empty()) s->prow.assign(rows, make_int2(0, 0));
+    for (size_t r = 0; r < rows; ++r) {
+        int lo = h.n[0], hi = 0;
+        if (all) {
+            lo = 0;
+            hi = h.n[0];
+        } else if (vol) {
+            for (int i = 0; i < h.n[0]; ++i) {
+                uint32_t w;
+                std::memcpy(&w, vol + r * h.n[0] + i, 4);
+                if (w) {
+                    lo = std::min(lo, i);
+                    hi = i + 1;
+                }
+            }
+        }
+        if (lo >= hi) continue;
+        int2 &cur = s->prow[r];
+        if (cur.x >= cur.y) cur = make_int2(lo, hi);
+        else cur = make_int2(std::min(cur.x, lo), std::max(cur.y, hi));
+    }
+    if (!h.prow) {
+        void *p = nullptr;
+        int rc = s->alloc(&p, rows * sizeof(int2));
+        if (rc) return rc;
+        h.prow = (const int2 *)p;
+        s->dirty = true;
+    }
+    HIP_TRY(hipMemcpyAsync((void *)h.prow, s->prow.data(), rows * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+static bool nonnegative(float v) {
+    uint32_t w;
+    std::memcpy(&w, &v, 4);
+    return !(w >> 31) && std::isfinite(v);
 }
 
 int fdtd_add_pole(fdtd_sim *s, const float consts[4], const float *const sigma[3],
@@ -1654,6 +1876,14 @@ int fdtd_add_pole(fdtd_sim *s, const float consts[4], const float *const sigma[3
     }
     h.npol = q + 1;
     s->dirty = true;
+    const bool exact_skip = nonnegative(consts[0]) && nonnegative(consts[1]) && nonnegative(consts[3]);
+    for (int c = 0; c < 3; ++c) {
+        if (!driven[c]) continue;
+        // a scalar sigma drives every row; a volume the rows where it is nonzero
+        const bool all = !exact_skip || !sigma[c];
+        rc = mark_rows(s, sigma[c], all);  // the volume is not read when `all`
+        if (rc) return rc;
+    }
     if (pole) *pole = q;
     return FDTD_OK;
 }
@@ -1761,6 +1991,10 @@ int fdtd_upload(fdtd_sim *s, const char *name, const float *host, size_t count) 
     if (count != (size_t)s->host.ncell) return fail(FDTD_EINVAL, "upload %s: %zu floats, grid has %d", name, count, s->host.ncell);
     float *dst = volume_by_name(s, name);
     if (!dst) return fail(FDTD_EINVAL, "no volume named '%s' in this simulation", name);
+    if (name[0] == 'P') {  // a stored polarization: its nonzero rows need the pole work
+        rc = mark_rows(s, host, false);
+        if (rc) return rc;
+    }
     return upload_volume(s, dst, host);
 }
 
@@ -2124,6 +2358,9 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     }
     b->group.all_offdiag = true;
     for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;
+    b->group.inv_mode = inv_mode_of(b->sims[0]->host);
+    for (auto *s : b->sims)
+        if (inv_mode_of(s->host) != b->group.inv_mode) b->group.inv_mode = 3;
     b->group.nsims = count;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }

@@ -33,6 +33,9 @@ SIGNATURES = {
     "fdtd_set_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _FP, _FP,
                                       _FP, _U8P]),
     "fdtd_set_inverse_epsilon": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
+    "fdtd_pole_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64)]),
+    "fdtd_inverse_epsilon_bytes": (ctypes.c_int, [ctypes.c_void_p, _IP]),
     "fdtd_set_offdiagonal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                             _FP]),
     "fdtd_add_pole": (ctypes.c_int, [ctypes.c_void_p, _FP, ctypes.POINTER(_FP), _FP, _IP, _IP]),

@@ -81,12 +81,15 @@ def _store_cache(cache: Dict[str, list]) -> None:
 
 
 def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
-            candidates: Optional[Iterable[Shape]] = None, repeats: int = 10
-            ) -> List[Tuple[Shape, float, float]]:
-    """(shape, mean H-pair ms, mean D-pair ms) for every candidate, on a scratch simulation."""
+            candidates: Optional[Iterable[Shape]] = None, repeats: int = 10,
+            inv: Optional[dict] = None) -> List[Tuple[Shape, float, float]]:
+    """(shape, mean H-pair ms, mean D-pair ms) for every candidate, on a scratch simulation
+    (with the run's inverse epsilon when given: its storage form changes the D pair)."""
     if candidates is None:
         candidates = candidates_for(grid)
     sim = HipSimulation(grid, pml)
+    for c, vol in (inv or {}).items():
+        sim.set_inverse_epsilon(c, vol)
     for term, sigma in poles:
         sim.add_pole(term, sigma)
     saved = (get_launch_pair(0), get_launch_pair(1))
@@ -116,7 +119,7 @@ def best_pair_shapes(results) -> PairShapes:
 
 
 def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
-         use_cache: bool = True) -> PairShapes:
+         use_cache: bool = True, inv: Optional[dict] = None) -> PairShapes:
     """Choose and install the fastest launch shape of each pair for this grid; returns them."""
     poles = list(poles)
     key = _key(grid, pml, len(poles))
@@ -126,7 +129,7 @@ def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10
         shapes = (tuple(entry[0]), tuple(entry[1]))
         install(shapes)
         return shapes  # type: ignore[return-value]
-    shapes = best_pair_shapes(measure(grid, pml, poles, repeats=repeats))
+    shapes = best_pair_shapes(measure(grid, pml, poles, repeats=repeats, inv=inv))
     install(shapes)
     if use_cache:
         cache[key] = [list(shapes[0]), list(shapes[1])]

@@ -79,6 +79,20 @@ class HipSimulation:
         check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
                                                 fptr(dev)), "fdtd_set_inverse_epsilon")
 
+    def pole_row_fraction(self) -> float:
+        """Fraction of the grid's cells inside the per-row ranges where pole work runs."""
+        active, rows = ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.fdtd_pole_rows(self.handle, ctypes.byref(active), ctypes.byref(rows)),
+              "fdtd_pole_rows")
+        return active.value / rows.value if rows.value else 0.0
+
+    def inverse_epsilon_bytes(self) -> Tuple[int, int, int]:
+        """Bytes per cell the D pair reads for each component's inverse epsilon (1, 2 or 4)."""
+        out = numpy.zeros(3, dtype=numpy.int32)
+        check(self.lib.fdtd_inverse_epsilon_bytes(self.handle, iptr(out)),
+              "fdtd_inverse_epsilon_bytes")
+        return int(out[0]), int(out[1]), int(out[2])
+
     def set_offdiagonal(self, row: str, partner: str, volume: numpy.ndarray):
         dev = to_device_layout(volume)
         check(self.lib.fdtd_set_offdiagonal(self.handle, _AXES.index(row[1]),
@@ -274,6 +288,10 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
     vol = 4 * npad
     total = 4096 + vol * (12 + (15 if pml else 0))
     total += 16 * 2 * (npad // max(ny * nz, 1) + ny + nz)  # the six layer coefficient lines
+    # inverse epsilon as table indices (at most 2 bytes a cell) and tables (<= 65536 words)
+    total += 3 * (2 * npad + 4 * min(65536, nx * ny * nz) + 512)
+    if driven_poles:
+        total += ny * nz + 256  # the rows where pole work runs
     for comps, sig in driven_poles:
         total += vol * (2 * int(comps) + int(sig))
     total += vol * int(offdiagonal_volumes)

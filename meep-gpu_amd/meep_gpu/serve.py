@@ -211,7 +211,7 @@ class SolveServer:
     """Continuous batching of FDTD solves under an HBM budget (see the module docstring)."""
 
     def __init__(self, budget_bytes: Optional[int] = None, max_slots: int = 64,
-                 quantum: int = 256, max_batch_cells: Optional[int] = None, backend=None,
+                 quantum: int = 256, max_batch_cells: Optional[int] = 4 << 20, backend=None,
                  footprint: Callable = request_footprint,
                  clock: Callable[[], float] = time.perf_counter,
                  sleep: Callable[[float], None] = time.sleep, checksums: bool = False):

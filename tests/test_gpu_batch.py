@@ -110,3 +110,49 @@ def test_each_member_steps_the_same_alone_and_in_the_batch():
         _same_words(a, b)
     for d in alone + together:
         d.close()
+
+
+def _inv_driver(kind, steps):
+    """A small PML run whose inverse epsilon is stored as float words (kind 'float': more than
+    65536 distinct), 16-bit indices ('u16') or byte indices ('u8')."""
+    from meep_gpu import workloads
+    from meep_gpu.driver import FdtdDriver
+
+    # 44^3 = 85,184 cells: enough for more than 65536 distinct words
+    wl = workloads.sphere(edge=44, resolution=4.0, pml_cells=6, radius_cells=8.0, nfreq=4,
+                          planes="one", steps=steps, source_x=-2.0, plane_x=2.0)
+    drv = FdtdDriver.from_workload(wl)
+    rng = numpy.random.default_rng({"float": 1, "u16": 2, "u8": 3}[kind])
+    if kind == "float":
+        inv = {c: rng.uniform(0.3, 1.0, wl.grid.shape).astype(numpy.float32) for c in wl.inv}
+    elif kind == "u16":
+        table = rng.uniform(0.3, 1.0, 3000).astype(numpy.float32)
+        inv = {c: table[rng.integers(0, 3000, wl.grid.shape)] for c in wl.inv}
+    else:
+        inv = wl.inv
+    drv.set_inverse_epsilon(inv)
+    drv.freeze(planned_steps=steps + 1)
+    return drv
+
+
+def test_every_inverse_epsilon_storage_steps_the_same_alone_and_mixed_in_a_batch():
+    from meep_gpu.batch import BatchedRun
+
+    steps = 90
+    kinds = ("float", "u16", "u8")
+    alone = []
+    for k in kinds:
+        d = _inv_driver(k, steps)
+        d.run(steps=steps)
+        d.synchronize()
+        alone.append(d)
+    assert [d.sim.inverse_epsilon_bytes()[0] for d in alone] == [4, 2, 1]
+    mixed = [_inv_driver(k, steps) for k in kinds]
+    batch = BatchedRun(mixed)
+    batch.run(steps)
+    batch.synchronize()
+    batch.close()
+    for a, b in zip(alone, mixed):
+        _same_words(a, b)
+    for d in alone + mixed:
+        d.close()

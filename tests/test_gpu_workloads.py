@@ -34,3 +34,20 @@ def test_workload_fields_and_spectra_match_the_oracle(name):
         assert numpy.max(numpy.abs(got - ref64)) <= 1e-9 * peak
         assert numpy.max(numpy.abs(ref32 - ref64)) <= 1e-4 * peak
     drv.close()
+
+
+def test_compact_inverse_epsilon_and_pole_rows_are_in_use():
+    """The workloads above run with inverse epsilon as 1-byte table indices (the sphere has 78
+    distinct words per component) and, for the slab, pole work only over the slab's x range: their
+    word-for-word agreement with the oracle covers both forms."""
+    from meep_gpu.driver import FdtdDriver
+    from tests.workload_runs import small
+
+    drv = FdtdDriver.from_workload(small("sphere"))
+    drv.freeze(planned_steps=2)
+    assert drv.sim.inverse_epsilon_bytes() == (1, 1, 1)
+    drv.close()
+    drv = FdtdDriver.from_workload(small("slab"))
+    drv.freeze(planned_steps=2)
+    assert 0.0 < drv.sim.pole_row_fraction() < 0.5
+    drv.close()

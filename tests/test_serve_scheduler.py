@@ -68,8 +68,9 @@ class FakeBackend:
 def _server(budget, slots=64, quantum=256, rate=1e9):
     clock = FakeClock()
     be = FakeBackend(clock, rate)
-    srv = SolveServer(budget_bytes=budget, max_slots=slots, quantum=quantum, backend=be,
-                      footprint=lambda r: r.cells * 100, clock=clock, sleep=clock.sleep)
+    srv = SolveServer(budget_bytes=budget, max_slots=slots, quantum=quantum, max_batch_cells=None,
+                      backend=be, footprint=lambda r: r.cells * 100, clock=clock,
+                      sleep=clock.sleep)
     return srv, be
 
 
