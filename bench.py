@@ -324,7 +324,9 @@ def main(argv=None) -> int:
     nbytes = algorithmic_bytes(sim)
     achieved = nbytes["d"] / (d_ms * 1e-3) / 1e9
     dshape = shape[1]
-    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'}>"
+    widths = set(sim.inverse_epsilon_bytes())
+    inv_mode = {1: 1, 2: 2, 4: 0}[widths.pop()] if len(widths) == 1 else 3
+    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'},{inv_mode}>"
     traffic = measured_traffic(dkernel)
     spectra_ok = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
     line = {
@@ -350,8 +352,8 @@ def main(argv=None) -> int:
                      "traffic": traffic,
                      "traffic_source": (f"profiles/traffic_pairs_sphere192.txt ({dkernel}: "
                                         "FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
-                                        if traffic else None),
-                     "kernel": "dpair_kernel (step_D+deposit+update_E+update_P)",
+                                        if traffic else f"no profile line for {dkernel}"),
+                     "kernel": f"{dkernel} (step_D+deposit+update_E+update_P)",
                      "bytes_per_launch": int(nbytes["d"]), "mean_launch_ms": round(d_ms, 5),
                      "h_pair": {"bytes_per_launch": int(nbytes["h"]),
                                 "mean_launch_ms": round(h_ms, 5),

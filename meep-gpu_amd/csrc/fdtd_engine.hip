@@ -747,11 +747,11 @@ __global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restric
 }
 
 __device__ inline float dmp_at(const SimDev &s, int comp, int at) {
-    float v = s.D[comp][at];
+    float v = gp(s.D[comp])[at];
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
         if (pq >= s.npol) break;
-        if (s.pdriven[pq][comp]) v = v - s.pbuf[pq][comp][s.ppar][at];
+        if (s.pdriven[pq][comp]) v = v - gp(s.pbuf[pq][comp][s.ppar])[at];
     }
     return v;
 }
@@ -796,8 +796,8 @@ __device__ inline void erow_component(const SimDev &s, const Run &q) {
             const int down1 = step_off(s, q, iu, pa, -1);
             const float pair0 = dmp_at(s, pa, at) + dmp_at(s, pa, at + down0);
             const float pair1 = dmp_at(s, pa, up) + dmp_at(s, pa, up + down1);
-            const float prod0 = U[at] * pair0;
-            const float prod1 = U[up] * pair1;
+            const float prod0 = gp(U)[at] * pair0;
+            const float prod1 = gp(U)[up] * pair1;
             const float summed = prod0 + prod1;
             row = row + 0.25 * (double)summed;
         }
@@ -868,7 +868,8 @@ __device__ inline float centred(const SimDev &s, const float *f, int mask, int i
     const int j1 = (j + 1 < s.n[1]) ? j + 1 : 0;
     const int k1 = (k + 1 < s.n[2]) ? k + 1 : 0;
     const int py = s.pitch, pz = s.pitch * s.n[1];
-    auto at = [&](int a, int b, int c) { return f[c * pz + b * py + a]; };
+    const GMEM float *gf = gp(f);
+    auto at = [&](int a, int b, int c) { return gf[c * pz + b * py + a]; };
     float v000 = at(i, j, k), v100 = 0.f, v010 = 0.f, v110 = 0.f;
     float v001 = 0.f, v101 = 0.f, v011 = 0.f, v111 = 0.f;
     if (ex) v100 = at(i1, j, k);
@@ -895,9 +896,9 @@ __device__ inline float centred(const SimDev &s, const float *f, int mask, int i
 // The H component c at cell (i, j, k): stored H on c's own-axis layer rows, B elsewhere.
 __device__ inline float h_at(const SimDev &s, int c, int i, int j, int k) {
     const int idx = (k * s.n[1] + j) * s.pitch + i;
-    if (!s.pml) return s.B[c][idx];
+    if (!s.pml) return gp(s.B[c])[idx];
     const int cc = c == 0 ? i : (c == 1 ? j : k);
-    return member(s, 0, c, cc) ? s.H[c][idx] : s.B[c][idx];
+    return member(s, 0, c, cc) ? gp(s.H[c])[idx] : gp(s.B[c])[idx];
 }
 
 // yee2cent for an H component: its Yee shift is 0 only along its own axis c, so the average
@@ -920,13 +921,13 @@ __global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes) {
     if (since <= 0 || since % p.dec) return;
     const int slot = (since / p.dec - 1) % kRing;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.times[slot] = (double)s.step * s.dt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) gp(p.times)[slot] = (double)s.step * s.dt;
     if (q >= p.cells) return;
     const int a = q % p.cnt[0], r = q / p.cnt[0];
     const int b = r % p.cnt[1], e = r / p.cnt[1];
     const int i = (p.lo[0] + a) % s.n[0], j = (p.lo[1] + b) % s.n[1], k = (p.lo[2] + e) % s.n[2];
     const int t1 = (p.normal + 1) % 3, t2 = (p.normal + 2) % 3;
-    float *out = p.ring + (size_t)slot * 4 * p.row;
+    GMEM float *out = gp(p.ring) + (size_t)slot * 4 * p.row;
     out[0 * p.row + q] = centred(s, s.E[t1], 1 << t1, i, j, k);
     out[1 * p.row + q] = centred(s, s.E[t2], 1 << t2, i, j, k);
     out[2 * p.row + q] = centred_h(s, t1, i, j, k);
