@@ -210,7 +210,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             "one_at_a_time_graphed": round(alone[True], 1),
             "speedup_vs_one_at_a_time_graphed": round(value / alone[True], 2),
             "one_at_a_time_note": ("each simulation alone with its own autotuned shapes: plain "
-                                   "form (one launch sequence per step) and with 64-step graphs"),
+                                   "form (one launch sequence per step) and with 128-step graphs"),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "note": "algorithmic bytes of both fused pairs per step, all members"},

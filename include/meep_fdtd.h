@@ -172,12 +172,17 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
                          int32_t nt_stores);
 int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
 
-/* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 64 whole
+/* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 128 whole
  * steps (both fused pairs, the counter advance, flux sampling and the transform passes that fall
  * due) wherever every flux plane samples each step; 0 launches every step eagerly. Results are
  * the same words either way. Process-wide. No counterpart in the reference (its host loop
  * launches each sub-step). */
 int fdtd_set_graphs(int32_t enable);
+
+/* mfma = 1 (the default): the float64 flux transform runs as 16x16x4 float64 MFMA products
+ * (phase words times ring samples); 0: the vector-FMA kernel. The sums associate differently
+ * (both are float64; they agree to rounding). Process-wide. No counterpart in the reference. */
+int fdtd_set_flux_transform(int32_t mfma);
 
 /* Measure average kernel durations over `repeats` launches of each fused pair on the
  * simulation's stream with HIP events: out_ms[0] = H pair, out_ms[1] = D pair. */

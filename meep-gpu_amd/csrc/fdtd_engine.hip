@@ -37,7 +37,7 @@ namespace {
 constexpr int kThreads = 256;   // threads per workgroup
 constexpr int kPad = 4;         // row pitch multiple (floats)
 constexpr int kMaxPoles = FDTD_MAX_POLES;
-constexpr int kRing = 64;       // flux samples buffered before one float64 transform pass
+constexpr int kRing = 128;      // flux samples buffered before one float64 transform pass
 constexpr int kMaxRuns = 2;     // runs of a layer membership line held in the descriptor
 
 thread_local std::string g_error;
@@ -1016,6 +1016,55 @@ __global__ void __launch_bounds__(256) flux_accumulate_kernel(
     }
 }
 
+// The same product on the matrix cores: per component, C[n][cell] += sum_m P[n][m] R[m][cell]
+// with n = 2f + (re, im), P the phase words and R the ring's samples widened to double, in
+// 16x16x4 float64 MFMA steps. V_MFMA_F64_16X16X4F64 on a wave: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and holds D[l/16 + 4r][l%16] in accumulator r. Each wave owns 16 cells and
+// kMT tiles of 16 n, so one ring word feeds kMT MFMAs; the ring rows are padded to 16 cells.
+typedef double d4v __attribute__((ext_vector_type(4)));
+constexpr int kMT = 4;  // 16-row n tiles per wave
+
+__global__ void __launch_bounds__(256) flux_accumulate_mfma_kernel(
+    const float *__restrict__ ring, const double *__restrict__ phase, double *__restrict__ acc,
+    int cells, int row, int nf, int nslots) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int comp = blockIdx.y;
+    const int side = comp < 2 ? 0 : 1;
+    const int cell0 = (blockIdx.x * 4 + wave) * 16;
+    if (cell0 >= cells) return;
+    const int nn = 2 * nf;
+    const int n0 = blockIdx.z * 16 * kMT;
+    const int cell = cell0 + (lane & 15);
+    const int kk = lane >> 4;      // this lane's k (slot) within a 4-slot step
+    const int ni = lane & 15;      // this lane's n within a tile (A operand row)
+    d4v c[kMT];
+#pragma unroll
+    for (int t = 0; t < kMT; ++t) c[t] = (d4v){0.0, 0.0, 0.0, 0.0};
+    for (int m0 = 0; m0 < nslots; m0 += 4) {
+        const int m = m0 + kk;
+        double b = 0.0;
+        if (m < nslots) b = (double)ring[((size_t)m * 4 + comp) * row + cell];  // row >= cell0+16
+#pragma unroll
+        for (int t = 0; t < kMT; ++t) {
+            const int n = n0 + t * 16 + ni;
+            double a = 0.0;
+            if (m < nslots && n < nn) a = phase[(size_t)m * 4 * nf + side * nn + n];
+            c[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c[t], 0, 0, 0);
+        }
+    }
+    if (cell >= cells) return;
+    double *a = acc + (size_t)comp * nf * cells * 2;
+#pragma unroll
+    for (int t = 0; t < kMT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = n0 + t * 16 + kk + 4 * r;
+            if (n >= nn) continue;
+            double *w = a + ((size_t)(n >> 1) * cells + cell) * 2 + (n & 1);
+            *w += c[t][r];
+        }
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = v;
@@ -1238,6 +1287,7 @@ struct StepGraph {
 };
 
 int g_graphs = 1;  // replay captured step sequences where eligible
+int g_flux_mfma = 1;  // the flux transform on the matrix cores (else the vector FMA kernel)
 }  // namespace
 
 struct fdtd_sim {
@@ -1328,9 +1378,14 @@ int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
     if (ph.filled == 0) return FDTD_OK;
     const PlaneDev &p = ph.dev;
     flux_phase_kernel<<<dim3((ph.filled * p.nf + 255) / 256, 1), 256, 0, st>>>(dev_plane, ph.filled);
-    const int per_block = 256 * kCB;
-    flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, 4, (p.nf + kFZ - 1) / kFZ),
-                             256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+    if (g_flux_mfma) {
+        flux_accumulate_mfma_kernel<<<dim3((p.cells + 63) / 64, 4, (2 * p.nf + 16 * kMT - 1) / (16 * kMT)),
+                                      256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+    } else {
+        const int per_block = 256 * kCB;
+        flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, 4, (p.nf + kFZ - 1) / kFZ),
+                                 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+    }
     HIP_TRY(hipGetLastError());
     ph.filled = 0;
     return FDTD_OK;
@@ -2090,6 +2145,11 @@ int fdtd_run(fdtd_sim *s, int32_t nsteps) {
     return run_steps(lone(s), nsteps);
 }
 
+int fdtd_set_flux_transform(int32_t mfma) {
+    g_flux_mfma = mfma ? 1 : 0;
+    return FDTD_OK;
+}
+
 int fdtd_set_graphs(int32_t enable) {
     g_graphs = enable ? 1 : 0;
     return FDTD_OK;
@@ -2122,7 +2182,7 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     p.normal = normal;
     for (int a = 0; a < 3; ++a) { p.lo[a] = lo[a]; p.cnt[a] = cnt[a]; }
     p.cells = cells;
-    p.row = ((cells + kCB - 1) / kCB) * kCB;
+    p.row = ((cells + 15) / 16) * 16;  // whole 16-cell tiles for the matrix-core transform
     p.nf = nf;
     p.dec = dec;
     p.first_step = s->host.step;

@@ -66,6 +66,7 @@ SIGNATURES = {
                                             ctypes.c_int32, ctypes.c_int32]),
     "fdtd_get_launch_pair": (ctypes.c_int, [ctypes.c_int32, _IP]),
     "fdtd_set_graphs": (ctypes.c_int, [ctypes.c_int32]),
+    "fdtd_set_flux_transform": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_batch_flush": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_time_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),
     "fdtd_run_timed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _FP]),

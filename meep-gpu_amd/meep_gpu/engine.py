@@ -255,8 +255,13 @@ def get_launch_pair(pair: int) -> Tuple[int, int, int, bool]:
     return int(out[0]), int(out[1]), int(out[2]), bool(out[3])
 
 
+def set_flux_transform(mfma: bool):
+    """Run the float64 flux transform on the matrix cores (default) or as vector FMAs."""
+    check(_native.load().fdtd_set_flux_transform(1 if mfma else 0), "fdtd_set_flux_transform")
+
+
 def set_graphs(enable: bool):
-    """Replay captured 64-step hipGraphs in ``run`` and batched runs (default on)."""
+    """Replay captured 128-step hipGraphs in ``run`` and batched runs (default on)."""
     check(_native.load().fdtd_set_graphs(1 if enable else 0), "fdtd_set_graphs")
 
 
@@ -269,7 +274,7 @@ def device_memory() -> Tuple[int, int]:
 
 
 #: flux samples buffered per plane before a transform pass (kRing in the engine)
-RING = 64
+RING = 128
 
 
 def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tuple[int, int]] = (),
@@ -296,7 +301,7 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
         total += vol * (2 * int(comps) + int(sig))
     total += vol * int(offdiagonal_volumes)
     for cells, nf in planes:
-        row = ((int(cells) + 3) // 4) * 4
+        row = ((int(cells) + 15) // 16) * 16
         total += 8 * nf + 4 * RING * 4 * row + 8 * RING + 8 * RING * 4 * nf + 64 * nf * cells
         total += 512 * (len(planes) + 1)
     total += 16 * source_points + 16 * (source_steps + 1) + 256

@@ -38,7 +38,7 @@ def test_every_launch_shape_reproduces_the_recorded_digests(shapes, golden_diges
 def _members():
     from meep_gpu.batch import synthetic_batch
 
-    return synthetic_batch(count=5, seed=7, edge_range=(28, 44), steps=150, nfreq=6,
+    return synthetic_batch(count=5, seed=7, edge_range=(28, 44), steps=300, nfreq=6,
                            lorentz_fraction=0.5)
 
 
@@ -53,15 +53,15 @@ def _same_words(a, b):
 
 
 def test_graph_replay_steps_the_same_words_as_eager_launches():
-    """Captured 64-step sequences (re-captured when a call starts at another ring phase)
+    """Captured 128-step sequences (re-captured when a call starts at another ring phase)
     against one eager launch per step."""
     from meep_gpu.driver import FdtdDriver
     from meep_gpu.engine import set_graphs
     from tests.workload_runs import small
 
-    steps = 230
+    steps = 600
     runs = []
-    for graphs, chunks in ((False, (steps,)), (True, (70, 130, 30))):
+    for graphs, chunks in ((False, (steps,)), (True, (70, 300, 230))):
         set_graphs(graphs)
         try:
             drv = FdtdDriver.from_workload(small("sphere_box"))
@@ -83,7 +83,7 @@ def test_each_member_steps_the_same_alone_and_in_the_batch():
     from meep_gpu.driver import FdtdDriver
     from meep_gpu.engine import set_graphs
 
-    steps = 150
+    steps = 300
     alone = []
     set_graphs(False)  # alone: eager launches; the batch below replays graphs
     try:
@@ -101,8 +101,8 @@ def test_each_member_steps_the_same_alone_and_in_the_batch():
         drv.freeze(planned_steps=steps + 1)
         together.append(drv)
     batch = BatchedRun(together)
-    batch.run(100)
-    batch.run(steps - 100)
+    batch.run(20)
+    batch.run(steps - 20)  # a captured 128-step sequence, replayed, then eager steps
     batch.synchronize()
     batch.close()
     for a, b in zip(alone, together):

@@ -51,3 +51,24 @@ def test_compact_inverse_epsilon_and_pole_rows_are_in_use():
     drv.freeze(planned_steps=2)
     assert 0.0 < drv.sim.pole_row_fraction() < 0.5
     drv.close()
+
+
+def test_matrix_core_and_vector_flux_transforms_agree():
+    """The float64 transform on the matrix cores (MFMA 16x16x4 f64) and as vector FMAs: the same
+    products summed in another order, so the spectra agree to float64 rounding."""
+    from meep_gpu.engine import set_flux_transform
+    from tests.workload_runs import hip_run, small
+
+    spectra = []
+    for mfma in (True, False):
+        set_flux_transform(mfma)
+        try:
+            drv = hip_run(small("sphere_box"), 150)
+            spectra.append([m.get_flux_spectrum() for m in drv.flux_monitors])
+            drv.close()
+        finally:
+            set_flux_transform(True)
+    for a, b in zip(*spectra):
+        scale = numpy.abs(b).max()
+        assert scale > 0
+        assert numpy.abs(a - b).max() <= 1e-12 * scale
