@@ -218,6 +218,24 @@ __device__ inline VF<V> ldv(const float *p, int at) {
     return r;
 }
 
+// Load a run read exactly once in the launch (no neighbour reuses it): non-temporal, so it
+// does not push out of the caches the neighbour rows and planes other loads come back for.
+template <int V>
+__device__ inline VF<V> ldv_once(const float *p, int at) {
+    VF<V> r;
+    const GMEM float *g = gp(p) + at;
+    if constexpr (V == 4) {
+        const f4v t = __builtin_nontemporal_load(reinterpret_cast<const GMEM f4v *>(g));
+        r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+    } else if constexpr (V == 2) {
+        const f2v t = __builtin_nontemporal_load(reinterpret_cast<const GMEM f2v *>(g));
+        r.v[0] = t.x; r.v[1] = t.y;
+    } else {
+        r.v[0] = __builtin_nontemporal_load(g);
+    }
+    return r;
+}
+
 // Store a run; with NT the store is marked non-temporal (streamed past the caches, for the
 // outputs no later read of the same launch wants).
 
@@ -426,7 +444,7 @@ __device__ inline void hpair_run(const SimDev &s, const Run &q) {
     // compiler knows, so loads left behind them would wait a memory round trip each
     VF<V> b[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) b[c] = ldv<V>(s.B[c], q.at);
+    for (int c = 0; c < 3; ++c) b[c] = NT ? ldv_once<V>(s.B[c], q.at) : ldv<V>(s.B[c], q.at);
     hpair_component<V, 0, PML, NT>(s, q, e, en, b[0]);
     hpair_component<V, 1, PML, NT>(s, q, e, en, b[1]);
     hpair_component<V, 2, PML, NT>(s, q, e, en, b[2]);
@@ -571,16 +589,21 @@ struct InvWords {
 
 // INV: the storage every simulation of the launch uses for all three components -- 0 float,
 // 1 byte indices, 2 16-bit indices -- or 3, mixed (chosen per component at run time).
-template <int V, int INV>
+template <int V, int INV, bool NT>
 __device__ inline InvWords load_inv_words(const SimDev &s, int c, int at) {
     InvWords r;
     r.w[0] = r.w[1] = 0u;
     if (INV == 0) return r;
     if ((INV == 1) || (INV == 3 && s.inv8[c])) {
         const GMEM uint8_t *g = gp(s.inv8[c]) + at;
-        if (V == 1) r.w[0] = *g;
-        else if (V == 2) r.w[0] = *reinterpret_cast<const GMEM uint16_t *>(g);
-        else r.w[0] = *reinterpret_cast<const GMEM uint32_t *>(g);
+        if (V == 1) r.w[0] = NT ? __builtin_nontemporal_load(g) : *g;
+        else if (V == 2) {
+            const GMEM uint16_t *h = reinterpret_cast<const GMEM uint16_t *>(g);
+            r.w[0] = NT ? __builtin_nontemporal_load(h) : *h;
+        } else {
+            const GMEM uint32_t *h = reinterpret_cast<const GMEM uint32_t *>(g);
+            r.w[0] = NT ? __builtin_nontemporal_load(h) : *h;
+        }
     } else if ((INV == 2) || (INV == 3 && s.inv16[c])) {
         const GMEM uint16_t *g = gp(s.inv16[c]) + at;
         if (V == 1) r.w[0] = *g;
@@ -676,13 +699,13 @@ template <int V, bool PML, bool NT, int INV>
 __device__ inline void dpair_run(const SimDev &s, const Run &q) {
     InvWords iw[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV>(s, c, q.at);
+    for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV, NT>(s, c, q.at);
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
     // every stream's load issued before the first store (see hpair_run)
     VF<V> d[3], iv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
+    for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<V>(s.D[c], q.at) : ldv<V>(s.D[c], q.at);
 #pragma unroll
     for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
     dpair_component<V, 0, PML, NT>(s, q, h, hn, d[0], iv[0]);
