@@ -173,8 +173,9 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
 int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
 
 /* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 128 whole
- * steps (both fused pairs, the counter advance, flux sampling and the transform passes that fall
- * due) wherever every flux plane samples each step; 0 launches every step eagerly. Results are
+ * steps (both fused pairs, flux sampling and the transform passes that fall due, each launch
+ * reading the step counters at its step's offset, and one counter advance at the end) wherever
+ * every flux plane samples each step; 0 launches every step eagerly. Results are
  * the same words either way. Process-wide. No counterpart in the reference (its host loop
  * launches each sub-step). */
 int fdtd_set_graphs(int32_t enable);
@@ -188,8 +189,11 @@ int fdtd_set_flux_transform(int32_t mfma);
  * simulation's stream with HIP events: out_ms[0] = H pair, out_ms[1] = D pair. */
 int fdtd_time_pairs(fdtd_sim *sim, int32_t repeats, float out_ms[2]);
 
-/* nsteps whole steps like fdtd_run, with HIP events around every fused-pair launch on the
- * simulation's stream: out_ms[0] = total elapsed, [1] = mean H pair, [2] = mean D pair. */
+/* nsteps whole steps launched eagerly in chunks of 128, as fdtd_run launches the steps it does
+ * not replay from a graph (each launch reads the step counters at its step's offset in the
+ * chunk; the counters advance once per chunk), with HIP events around every fused-pair launch
+ * on the simulation's stream, read after each chunk. out_ms[0] = total elapsed, [1] = mean H
+ * pair, [2] = mean D pair (per step). */
 int fdtd_run_timed(fdtd_sim *sim, int32_t nsteps, float out_ms[3]);
 
 /* Diagnostic: mean duration of a device copy of `bytes` (the reference point that scales the

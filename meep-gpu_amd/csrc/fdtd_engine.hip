@@ -108,7 +108,7 @@ struct SimDev {
 };
 
 // ------------------------------------------------------------------------------------------
-// Tile walk and the last-workgroup step advance
+// Tile walk and the step counters
 // ------------------------------------------------------------------------------------------
 struct TileWalk {
     int next, stride, end;
@@ -149,14 +149,31 @@ __device__ inline int tile_at(const TileWalk &w, int g) {
     return w.sb ? ((g / w.sb) * 8 + w.xcd) * w.sb + g % w.sb : g;
 }
 
-// Advance each simulation's step counter, H-start flag and P parity after its D pair (one
-// wave; kept out of the fused kernels so no workgroup of a launch sees a counter change).
-__global__ void advance_kernel(SimDev *sims, int count) {
+// Advance each simulation's step counter, H-start flag and P parity by `by` steps (one wave;
+// kept out of the fused kernels so no workgroup of a launch sees a counter change). Eager steps
+// advance by 1 after each D pair; a captured sequence of steps advances once, at its end, by
+// its length, and its kernels read the counters through the step offset they were launched
+// with (Clock).
+__global__ void advance_kernel(SimDev *sims, int count, int by) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= count) return;
-    sims[q].step += 1;
+    if (q >= count || by <= 0) return;
+    sims[q].step += by;
     sims[q].hsep = 1;
-    if (sims[q].npol) sims[q].ppar ^= 1;
+    if (sims[q].npol) sims[q].ppar ^= by & 1;
+}
+
+// The counters a kernel launched `adv` steps after the descriptor's last advance works with:
+// the step, whether H has started (update_H has run once), and the P buffer parity.
+struct Clock {
+    int step, hsep, ppar;
+};
+
+__device__ inline Clock clock_of(const SimDev &s, int adv) {
+    Clock c;
+    c.step = s.step + adv;
+    c.hsep = (s.hsep || adv > 0) ? 1 : 0;
+    c.ppar = s.npol ? (s.ppar ^ (adv & 1)) : s.ppar;
+    return c;
 }
 
 // V x-adjacent cells (i0..i0+V-1) of one stored row; cells with i >= nx are row padding,
@@ -328,7 +345,8 @@ __device__ inline bool plus_zero(double v) { return __double_as_longlong(v) == 0
 // Sources of one field/component deposited into the run's values (sources.py:1357,
 // host_writes.py:140): f = float(double(f) - ((amp_re*J_re - amp_im*J_im)*dt)), in list order.
 template <int V>
-__device__ inline void deposit_run(const SimDev &s, int field, int comp, const Run &q, VF<V> &x) {
+__device__ inline void deposit_run(const SimDev &s, int step, int field, int comp, const Run &q,
+                                   VF<V> &x) {
     // one test against the union of this field's boxes before any per-source work
     if (q.j < s.src_lo[field][1] || q.j >= s.src_hi[field][1] || q.k < s.src_lo[field][2] ||
         q.k >= s.src_hi[field][2] || q.i0 + V <= s.src_lo[field][0] || q.i0 >= s.src_hi[field][0])
@@ -339,8 +357,8 @@ __device__ inline void deposit_run(const SimDev &s, int field, int comp, const R
         const int b = q.j - d.lo[1], e = q.k - d.lo[2];
         if (b < 0 || e < 0 || b >= d.cnt[1] || e >= d.cnt[2]) continue;
         if (q.i0 + V <= d.lo[0] || q.i0 >= d.lo[0] + d.cnt[0]) continue;
-        if (s.step >= d.ncur) continue;
-        const double cre = gp(d.cur)[2 * s.step], cim = gp(d.cur)[2 * s.step + 1];
+        if (step >= d.ncur) continue;
+        const double cre = gp(d.cur)[2 * step], cim = gp(d.cur)[2 * step + 1];
         if (plus_zero(cre) && plus_zero(cim)) continue;
 #pragma unroll
         for (int m = 0; m < V; ++m) {
@@ -384,8 +402,8 @@ __device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool 
 // en[comp][axis]: E[comp] at the +1 neighbour along axis (only the pairs a curl reads).
 // ------------------------------------------------------------------------------------------
 template <int V, int C, bool PML, bool NT>
-__device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V> (&e)[3],
-                                       const VF<V> (&en)[3][3], VF<V> b) {
+__device__ inline void hpair_component(const SimDev &s, const Clock &ck, const Run &q,
+                                       const VF<V> (&e)[3], const VF<V> (&en)[3][3], VF<V> b) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
     constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
@@ -405,7 +423,7 @@ __device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V
             b.v[m] = b.v[m] - curl;
         }
     }
-    if (s.nsrc) deposit_run<V>(s, 1, C, q, b);
+    if (s.nsrc) deposit_run<V>(s, ck.step, 1, C, q, b);
     stv<V, NT>(s.B[C], q.at, b);
     if (PML) {
         // update_H: H accumulates on the own-axis layer's rows; everywhere else H is B's word
@@ -416,7 +434,7 @@ __device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V
             if (member(s, 0, C, cw)) {
                 const float4 lw = coeff(s, 0, C, cw, true);
                 float hold = b.v[m], prev = b.v[m];
-                if (s.hsep) {
+                if (ck.hsep) {
                     hold = gp(s.H[C])[q.at + m];
                     prev = gp(s.fwH[C])[q.at + m];
                 }
@@ -430,7 +448,7 @@ __device__ inline void hpair_component(const SimDev &s, const Run &q, const VF<V
 }
 
 template <int V, bool PML, bool NT>
-__device__ inline void hpair_run(const SimDev &s, const Run &q) {
+__device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> e[3], en[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) e[c] = ldv<V>(s.E[c], q.at);
@@ -445,15 +463,15 @@ __device__ inline void hpair_run(const SimDev &s, const Run &q) {
     VF<V> b[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) b[c] = NT ? ldv_once<V>(s.B[c], q.at) : ldv<V>(s.B[c], q.at);
-    hpair_component<V, 0, PML, NT>(s, q, e, en, b[0]);
-    hpair_component<V, 1, PML, NT>(s, q, e, en, b[1]);
-    hpair_component<V, 2, PML, NT>(s, q, e, en, b[2]);
+    hpair_component<V, 0, PML, NT>(s, ck, q, e, en, b[0]);
+    hpair_component<V, 1, PML, NT>(s, ck, q, e, en, b[1]);
+    hpair_component<V, 2, PML, NT>(s, ck, q, e, en, b[2]);
 }
 
 template <int V, bool NT>
 __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
-                                                         int ntiles, int xcd_remap) {
+                                                         int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
@@ -463,10 +481,11 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
         const int at = tile.y + V * (int)threadIdx.x;
         if (at >= s.npad) continue;
         const Run q = locate(s, at);
+        const Clock ck = clock_of(s, adv);
         if (s.pml)
-            hpair_run<V, true, NT>(s, q);
+            hpair_run<V, true, NT>(s, ck, q);
         else
-            hpair_run<V, false, NT>(s, q);
+            hpair_run<V, false, NT>(s, ck, q);
     }
 }
 
@@ -474,8 +493,8 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
 // D pair: step_D -> electric deposit -> update_E -> update_P.
 // ------------------------------------------------------------------------------------------
 template <int V, int C, bool PML>
-__device__ inline VF<V> dcurl_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
-                                        const VF<V> (&hn)[3][3], VF<V> d) {
+__device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const Run &q,
+                                        const VF<V> (&h)[3], const VF<V> (&hn)[3][3], VF<V> d) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
     constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
@@ -495,14 +514,14 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Run &q, const VF<
             d.v[m] = d.v[m] - curl;
         }
     }
-    if (s.nsrc) deposit_run<V>(s, 0, C, q, d);
+    if (s.nsrc) deposit_run<V>(s, ck.step, 0, C, q, d);
     return d;
 }
 
 // E from the constitutive product `cst`, then every pole's P update for component C. The
 // drive W is f_w on the own-axis layer's cells and E elsewhere: the constitutive product both.
 template <int V, int C, bool PML, bool NT>
-__device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
+__device__ inline void e_and_p(const SimDev &s, const Clock &ck, const Run &q, const VF<V> &cst,
                                const VF<V> (&p)[kMaxPoles], bool active) {
     VF<V> e;
     if (PML) {
@@ -525,7 +544,7 @@ __device__ inline void e_and_p(const SimDev &s, const Run &q, const VF<V> &cst,
     }
     stv<V, NT>(s.E[C], q.at, e);
     if (!active) return;  // no polarization on this row, now or later: P stays +0
-    const int par = s.ppar;
+    const int par = ck.ppar;
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
         if (pq >= s.npol) break;
@@ -563,10 +582,10 @@ __device__ inline bool pole_run_active(const SimDev &s, const Run &q, int V) {
 }
 
 template <int V, int C>
-__device__ inline VF<V> minus_poles(const SimDev &s, const Run &q, const VF<V> &d,
-                                    VF<V> (&p)[kMaxPoles], bool active) {
+__device__ inline VF<V> minus_poles(const SimDev &s, const Clock &ck, const Run &q,
+                                    const VF<V> &d, VF<V> (&p)[kMaxPoles], bool active) {
     VF<V> dmp = d;
-    const int par = s.ppar;
+    const int par = ck.ppar;
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
 #pragma unroll
@@ -636,21 +655,21 @@ __device__ inline VF<V> inv_values(const SimDev &s, int c, int at, const InvWord
 }
 
 template <int V, int C, bool PML, bool NT>
-__device__ inline void dpair_component(const SimDev &s, const Run &q, const VF<V> (&h)[3],
-                                       const VF<V> (&hn)[3][3], const VF<V> &d0,
-                                       const VF<V> &iv) {
-    const VF<V> d = dcurl_component<V, C, PML>(s, q, h, hn, d0);
+__device__ inline void dpair_component(const SimDev &s, const Clock &ck, const Run &q,
+                                       const VF<V> (&h)[3], const VF<V> (&hn)[3][3],
+                                       const VF<V> &d0, const VF<V> &iv) {
+    const VF<V> d = dcurl_component<V, C, PML>(s, ck, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
     VF<V> p[kMaxPoles];
     // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
     // is g1inv*((A*0 - g1*0) + od*(0*w)) = +0), and D - (+0) is D: runs with no such cell
     // skip the pole work (a run straddling the range updates its +0 cells to +0)
     const bool active = pole_run_active(s, q, V);
-    const VF<V> dmp = minus_poles<V, C>(s, q, d, p, active);
+    const VF<V> dmp = minus_poles<V, C>(s, ck, q, d, p, active);
     VF<V> cst;
 #pragma unroll
     for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
-    e_and_p<V, C, PML, NT>(s, q, cst, p, active);
+    e_and_p<V, C, PML, NT>(s, ck, q, cst, p, active);
 }
 
 // H component c at the run shifted by `off` (a shift across c's own axis): the stored H on
@@ -696,7 +715,7 @@ __device__ inline void load_h(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V
 }
 
 template <int V, bool PML, bool NT, int INV>
-__device__ inline void dpair_run(const SimDev &s, const Run &q) {
+__device__ inline void dpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     InvWords iw[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV, NT>(s, c, q.at);
@@ -708,15 +727,15 @@ __device__ inline void dpair_run(const SimDev &s, const Run &q) {
     for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<V>(s.D[c], q.at) : ldv<V>(s.D[c], q.at);
 #pragma unroll
     for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
-    dpair_component<V, 0, PML, NT>(s, q, h, hn, d[0], iv[0]);
-    dpair_component<V, 1, PML, NT>(s, q, h, hn, d[1], iv[1]);
-    dpair_component<V, 2, PML, NT>(s, q, h, hn, d[2], iv[2]);
+    dpair_component<V, 0, PML, NT>(s, ck, q, h, hn, d[0], iv[0]);
+    dpair_component<V, 1, PML, NT>(s, ck, q, h, hn, d[1], iv[1]);
+    dpair_component<V, 2, PML, NT>(s, ck, q, h, hn, d[2], iv[2]);
 }
 
 template <int V, bool NT, int INV>
 __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
-                                                         int ntiles, int xcd_remap) {
+                                                         int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
@@ -727,31 +746,32 @@ __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restric
         const int at = tile.y + V * (int)threadIdx.x;
         if (at >= s.npad) continue;
         const Run q = locate(s, at);
+        const Clock ck = clock_of(s, adv);
         if (s.pml)
-            dpair_run<V, true, NT, INV>(s, q);
+            dpair_run<V, true, NT, INV>(s, ck, q);
         else
-            dpair_run<V, false, NT, INV>(s, q);
+            dpair_run<V, false, NT, INV>(s, ck, q);
     }
 }
 
 // Off-diagonal chi1inv: the row reads the partners' D - sum P at neighbouring cells, so the
 // D curl (with its deposit) completes over the whole grid first, then E and P.
 template <int V, bool PML>
-__device__ inline void dcurl_run(const SimDev &s, const Run &q) {
+__device__ inline void dcurl_run(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
     VF<V> d[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
-    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, q, h, hn, d[0]));
-    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, q, h, hn, d[1]));
-    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, q, h, hn, d[2]));
+    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, ck, q, h, hn, d[0]));
+    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, ck, q, h, hn, d[1]));
+    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, ck, q, h, hn, d[2]));
 }
 
 template <int V>
 __global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
-                                                         int ntiles, int xcd_remap) {
+                                                         int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
@@ -762,19 +782,20 @@ __global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restric
         const int at = tile.y + V * (int)threadIdx.x;
         if (at >= s.npad) continue;
         const Run q = locate(s, at);
+        const Clock ck = clock_of(s, adv);
         if (s.pml)
-            dcurl_run<V, true>(s, q);
+            dcurl_run<V, true>(s, ck, q);
         else
-            dcurl_run<V, false>(s, q);
+            dcurl_run<V, false>(s, ck, q);
     }
 }
 
-__device__ inline float dmp_at(const SimDev &s, int comp, int at) {
+__device__ inline float dmp_at(const SimDev &s, int ppar, int comp, int at) {
     float v = gp(s.D[comp])[at];
 #pragma unroll
     for (int pq = 0; pq < kMaxPoles; ++pq) {
         if (pq >= s.npol) break;
-        if (s.pdriven[pq][comp]) v = v - gp(s.pbuf[pq][comp][s.ppar])[at];
+        if (s.pdriven[pq][comp]) v = v - gp(s.pbuf[pq][comp][ppar])[at];
     }
     return v;
 }
@@ -791,10 +812,10 @@ __device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, i
 
 // The row of stepping._offdiagonal_terms (stepping.py:1594-1687), summed in double.
 template <int V, int C, bool PML>
-__device__ inline void erow_component(const SimDev &s, const Run &q) {
+__device__ inline void erow_component(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> p[kMaxPoles];
     const bool active = pole_run_active(s, q, V);
-    const VF<V> dmp = minus_poles<V, C>(s, q, ldv<V>(s.D[C], q.at), p, active);
+    const VF<V> dmp = minus_poles<V, C>(s, ck, q, ldv<V>(s.D[C], q.at), p, active);
     const VF<V> iv = ldv<V>(s.inv[C], q.at);
     VF<V> cst;
     const float *u1 = s.u[C][(C + 1) % 3], *u2 = s.u[C][(C + 2) % 3];
@@ -817,8 +838,8 @@ __device__ inline void erow_component(const SimDev &s, const Run &q) {
             const int iu = (C == 0) ? ((i + 1 < s.n[0]) ? i + 1 : 0) : i;
             const int down0 = step_off(s, q, i, pa, -1);
             const int down1 = step_off(s, q, iu, pa, -1);
-            const float pair0 = dmp_at(s, pa, at) + dmp_at(s, pa, at + down0);
-            const float pair1 = dmp_at(s, pa, up) + dmp_at(s, pa, up + down1);
+            const float pair0 = dmp_at(s, ck.ppar, pa, at) + dmp_at(s, ck.ppar, pa, at + down0);
+            const float pair1 = dmp_at(s, ck.ppar, pa, up) + dmp_at(s, ck.ppar, pa, up + down1);
             const float prod0 = gp(U)[at] * pair0;
             const float prod1 = gp(U)[up] * pair1;
             const float summed = prod0 + prod1;
@@ -826,13 +847,13 @@ __device__ inline void erow_component(const SimDev &s, const Run &q) {
         }
         cst.v[m] = (float)row;
     }
-    e_and_p<V, C, PML, false>(s, q, cst, p, active);
+    e_and_p<V, C, PML, false>(s, ck, q, cst, p, active);
 }
 
 template <int V>
 __global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict__ sims,
                                                         const int2 *__restrict__ tiles,
-                                                        int ntiles, int xcd_remap) {
+                                                        int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
@@ -843,14 +864,15 @@ __global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict
         const int at = tile.y + V * (int)threadIdx.x;
         if (at >= s.npad) continue;
         const Run q = locate(s, at);
+        const Clock ck = clock_of(s, adv);
         if (s.pml) {
-            erow_component<V, 0, true>(s, q);
-            erow_component<V, 1, true>(s, q);
-            erow_component<V, 2, true>(s, q);
+            erow_component<V, 0, true>(s, ck, q);
+            erow_component<V, 1, true>(s, ck, q);
+            erow_component<V, 2, true>(s, ck, q);
         } else {
-            erow_component<V, 0, false>(s, q);
-            erow_component<V, 1, false>(s, q);
-            erow_component<V, 2, false>(s, q);
+            erow_component<V, 0, false>(s, ck, q);
+            erow_component<V, 1, false>(s, ck, q);
+            erow_component<V, 2, false>(s, ck, q);
         }
     }
 }
@@ -860,7 +882,7 @@ __global__ void prime_e_kernel(SimDev *sims, int count) {
     const SimDev &s = sims[0];
     const int at = blockIdx.x * blockDim.x + threadIdx.x;
     if (at >= count) return;
-    for (int c = 0; c < 3; ++c) s.E[c][at] = s.inv[c][at] * dmp_at(s, c, at);
+    for (int c = 0; c < 3; ++c) s.E[c][at] = s.inv[c][at] * dmp_at(s, s.ppar, c, at);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -935,16 +957,19 @@ __device__ inline float centred_h(const SimDev &s, int c, int i, int j, int k) {
 }
 
 // One launch samples every plane of a simulation (or of a batch) that accumulates this step.
-__global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes) {
+// `adv`: steps since the descriptors' last counter advance (the sample follows the step's
+// advance, so a captured step t samples at the descriptor's step + t + 1).
+__global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes, int adv) {
     const int pi = blockIdx.y;
     if (pi >= nplanes) return;
     const PlaneDev &p = planes[pi];
     const SimDev &s = *p.sim;
-    const int since = s.step - p.first_step;
+    const int step = s.step + adv;
+    const int since = step - p.first_step;
     if (since <= 0 || since % p.dec) return;
     const int slot = (since / p.dec - 1) % kRing;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) gp(p.times)[slot] = (double)s.step * s.dt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) gp(p.times)[slot] = (double)step * s.dt;
     if (q >= p.cells) return;
     const int a = q % p.cnt[0], r = q / p.cnt[0];
     const int b = r % p.cnt[1], e = r / p.cnt[1];
@@ -1226,16 +1251,18 @@ int inv_mode_of(const SimDev &h) {
 }
 
 template <int V, bool NT>
-void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, hipStream_t st) {
+void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
     switch (g.inv_mode) {
-        case 0: dpair_kernel<V, NT, 0><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
-        case 1: dpair_kernel<V, NT, 1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
-        case 2: dpair_kernel<V, NT, 2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
-        default: dpair_kernel<V, NT, 3><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm); break;
+        case 0: dpair_kernel<V, NT, 0><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 1: dpair_kernel<V, NT, 1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 2: dpair_kernel<V, NT, 2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        default: dpair_kernel<V, NT, 3><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
     }
 }
 
-int launch_h(const Group &g, hipStream_t st) {
+// The fused pairs' launches `adv` steps after the descriptors' last counter advance (0 for an
+// eager step; a captured step's offset within its sequence).
+int launch_h(const Group &g, hipStream_t st, int adv = 0) {
     const LaunchShape &sh = g_shape[0];
     const int V = sh.vec;
     const int nt = g.ntiles.at(V);
@@ -1243,19 +1270,21 @@ int launch_h(const Group &g, hipStream_t st) {
     const int G = grid_for(sh, nt);
     const int rm = sh.xcd_remap;
     if (sh.nt) {
-        if (V == 4) hpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-        else if (V == 2) hpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-        else hpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        if (V == 4) hpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else if (V == 2) hpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else hpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
     } else {
-        if (V == 4) hpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-        else if (V == 2) hpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-        else hpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+        if (V == 4) hpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else if (V == 2) hpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else hpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
     }
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
 
-int launch_d(const Group &g, hipStream_t st) {
+// ... then the counter advance by `advance_by` steps (0 inside a captured sequence, which
+// advances once at its end).
+int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
     const LaunchShape &sh = g_shape[1];
     const int V = sh.vec;
     const int nt = g.ntiles.at(V);
@@ -1264,33 +1293,34 @@ int launch_d(const Group &g, hipStream_t st) {
     const int rm = sh.xcd_remap;
     if (!g.all_offdiag) {
         if (sh.nt) {
-            if (V == 4) launch_dpair<4, true>(g, G, tl, nt, rm, st);
-            else if (V == 2) launch_dpair<2, true>(g, G, tl, nt, rm, st);
-            else launch_dpair<1, true>(g, G, tl, nt, rm, st);
+            if (V == 4) launch_dpair<4, true>(g, G, tl, nt, rm, adv, st);
+            else if (V == 2) launch_dpair<2, true>(g, G, tl, nt, rm, adv, st);
+            else launch_dpair<1, true>(g, G, tl, nt, rm, adv, st);
         } else {
-            if (V == 4) launch_dpair<4, false>(g, G, tl, nt, rm, st);
-            else if (V == 2) launch_dpair<2, false>(g, G, tl, nt, rm, st);
-            else launch_dpair<1, false>(g, G, tl, nt, rm, st);
+            if (V == 4) launch_dpair<4, false>(g, G, tl, nt, rm, adv, st);
+            else if (V == 2) launch_dpair<2, false>(g, G, tl, nt, rm, adv, st);
+            else launch_dpair<1, false>(g, G, tl, nt, rm, adv, st);
         }
     }
     if (g.any_offdiag) {
         if (V == 4) {
-            dcurl_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            erow_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            dcurl_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+            erow_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
         } else if (V == 2) {
-            dcurl_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            erow_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            dcurl_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+            erow_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
         } else {
-            dcurl_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
-            erow_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm);
+            dcurl_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+            erow_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
         }
     }
-    advance_kernel<<<(g.nsims + 63) / 64, 64, 0, st>>>(g.dev, g.nsims);
+    if (advance_by > 0) advance_kernel<<<(g.nsims + 63) / 64, 64, 0, st>>>(g.dev, g.nsims, advance_by);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
-// A captured sequence of kRing steps (fused pairs, counter advance, flux sampling and the
-// transform passes that fall due), replayed while the launch shape and every plane's ring
+// A captured sequence of kRing steps (fused pairs, flux sampling and the transform passes that
+// fall due, each launched with its step's offset; one counter advance by kRing at the end),
+// replayed while the launch shape and every plane's ring
 // phase are what they were at capture. Plane rings fill once per kRing steps when every plane
 // samples each step, so one capture serves every later period.
 struct StepGraph {
@@ -1329,6 +1359,7 @@ struct fdtd_sim {
     bool dirty = true;
     bool in_batch = false;
     StepGraph graph;
+    std::vector<hipEvent_t> timed_ev;  // fdtd_run_timed: around each fused pair of a chunk
 
     int alloc(void **p, size_t bytes_) {
         hipError_t e = hipMalloc(p, bytes_);
@@ -1427,6 +1458,13 @@ void mirror_advance(fdtd_sim *s) {
 
 // The simulations one launch sequence steps (one, or a batch) with their flux planes, in
 // descriptor order: simulation q's planes follow simulation q-1's in `planes`.
+// HIP events around every fused-pair launch of the steps a Stepper runs: ev[3t..3t+2] around
+// step t of a chunk.
+struct Timing {
+    hipEvent_t *ev = nullptr;
+    double h_ms = 0.0, d_ms = 0.0;
+};
+
 struct Stepper {
     Group g;
     fdtd_sim *const *sims = nullptr;
@@ -1436,12 +1474,26 @@ struct Stepper {
     int max_plane_cells = 0;
     hipStream_t st = nullptr;
     StepGraph *graph = nullptr;
+    Timing *tm = nullptr;
 };
+
+// Add the pair durations of `nsteps` recorded steps (waits for the last one).
+int add_timing(Timing *tm, int nsteps) {
+    HIP_TRY(hipEventSynchronize(tm->ev[3 * (nsteps - 1) + 2]));
+    for (int t = 0; t < nsteps; ++t) {
+        float x = 0.f, y = 0.f;
+        HIP_TRY(hipEventElapsedTime(&x, tm->ev[3 * t], tm->ev[3 * t + 1]));
+        HIP_TRY(hipEventElapsedTime(&y, tm->ev[3 * t + 1], tm->ev[3 * t + 2]));
+        tm->h_ms += x;
+        tm->d_ms += y;
+    }
+    return FDTD_OK;
+}
 
 // Flux bookkeeping after a step: one sampling launch for every plane that samples now, then a
 // transform pass for each ring that is full. With launch == false only the host mirrors move
 // (the step itself runs inside a replayed graph).
-int flux_after_step(const Stepper &k, bool launch) {
+int flux_after_step(const Stepper &k, bool launch, int adv = 0) {
     if (!k.nplanes) return FDTD_OK;
     bool any = false;
     for (int q = 0; q < k.nsims; ++q)
@@ -1449,7 +1501,7 @@ int flux_after_step(const Stepper &k, bool launch) {
     if (!any) return FDTD_OK;
     if (launch) {
         flux_sample_kernel<<<dim3((k.max_plane_cells + 255) / 256, k.nplanes), 256, 0, k.st>>>(
-            k.planes, k.nplanes);
+            k.planes, k.nplanes, adv);
         HIP_TRY(hipGetLastError());
     }
     int pi = 0;
@@ -1469,15 +1521,41 @@ int flux_after_step(const Stepper &k, bool launch) {
     return FDTD_OK;
 }
 
-int step_once(const Stepper &k, bool launch) {
+// One whole step. `t` < 0: a lone step (the counters advance after its D pair); else step t of a
+// chunk (eager or being captured) whose launches read the counters at offset t and whose
+// counters advance once, by the chunk's length, at its end (chunk_end).
+int step_once(const Stepper &k, bool launch, int t = -1) {
+    const int adv = t < 0 ? 0 : t;
     if (launch) {
-        int rc = launch_h(k.g, k.st);
+        hipEvent_t *ev = (k.tm && t >= 0) ? k.tm->ev + 3 * t : nullptr;
+        if (ev) HIP_TRY(hipEventRecord(ev[0], k.st));
+        int rc = launch_h(k.g, k.st, adv);
         if (rc) return rc;
-        rc = launch_d(k.g, k.st);
+        if (ev) HIP_TRY(hipEventRecord(ev[1], k.st));
+        rc = launch_d(k.g, k.st, adv, t < 0 ? 1 : 0);
         if (rc) return rc;
+        if (ev) HIP_TRY(hipEventRecord(ev[2], k.st));
     }
     for (int q = 0; q < k.nsims; ++q) mirror_advance(k.sims[q]);
-    return flux_after_step(k, launch);
+    return flux_after_step(k, launch, t < 0 ? 0 : t + 1);
+}
+
+// The counter advance that closes a chunk of n steps.
+int chunk_end(const Stepper &k, int n) {
+    advance_kernel<<<(k.g.nsims + 63) / 64, 64, 0, k.st>>>(k.g.dev, k.g.nsims, n);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+// n <= kRing steps launched eagerly as one chunk (then their pair durations, when timed).
+int eager_chunk(const Stepper &k, int n) {
+    for (int t = 0; t < n; ++t) {
+        int rc = step_once(k, true, t);
+        if (rc) return rc;
+    }
+    int rc = chunk_end(k, n);
+    if (rc == FDTD_OK && k.tm) rc = add_timing(k.tm, n);
+    return rc;
 }
 
 struct Mirror {
@@ -1510,7 +1588,8 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
     const Mirror before = save_mirror(k);
     HIP_TRY(hipStreamBeginCapture(k.st, hipStreamCaptureModeThreadLocal));
     int rc = FDTD_OK;
-    for (int t = 0; t < kRing && rc == FDTD_OK; ++t) rc = step_once(k, true);
+    for (int t = 0; t < kRing && rc == FDTD_OK; ++t) rc = step_once(k, true, t);
+    if (rc == FDTD_OK) rc = chunk_end(k, kRing);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(k.st, &graph);
     hipGraphExec_t exec = nullptr;
@@ -1537,7 +1616,7 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
 // launches for the rest.
 int run_steps(const Stepper &k, int nsteps) {
     int done = 0;
-    bool eligible = g_graphs && k.graph && nsteps >= kRing;
+    bool eligible = g_graphs && k.graph && !k.tm && nsteps >= kRing;
     std::vector<int> phase;
     for (int q = 0; q < k.nsims && eligible; ++q)
         for (auto &ph : k.sims[q]->planes) {
@@ -1558,9 +1637,11 @@ int run_steps(const Stepper &k, int nsteps) {
             HIP_TRY(hipGraphLaunch(k.graph->exec, k.st));
         }
     }
-    for (; done < nsteps; ++done) {
-        int rc = step_once(k, true);
+    while (done < nsteps) {
+        const int n = nsteps - done < kRing ? nsteps - done : kRing;
+        int rc = eager_chunk(k, n);
         if (rc) return rc;
+        done += n;
     }
     return FDTD_OK;
 }
@@ -1718,6 +1799,7 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
 void fdtd_sim_destroy(fdtd_sim *s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (hipEvent_t e : s->timed_ev) (void)hipEventDestroy(e);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -2295,43 +2377,28 @@ int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->sync_desc();
     if (rc) return rc;
-    const Group g = s->group();
-    constexpr int kChunk = 128;
-    std::vector<hipEvent_t> ev(3 * kChunk + 2);
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    double th = 0.0, td = 0.0;
-    HIP_TRY(hipEventRecord(ev[3 * kChunk], s->stream));
-    for (int base = 0; base < nsteps; base += kChunk) {
-        const int n = nsteps - base < kChunk ? nsteps - base : kChunk;
-        for (int q = 0; q < n; ++q) {
-            HIP_TRY(hipEventRecord(ev[3 * q], s->stream));
-            rc = launch_h(g, s->stream);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(ev[3 * q + 1], s->stream));
-            rc = launch_d(g, s->stream);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(ev[3 * q + 2], s->stream));
-            mirror_advance(s);
-            rc = flux_after_step(lone(s), true);
-            if (rc) return rc;
-        }
-        HIP_TRY(hipEventSynchronize(ev[3 * (n - 1) + 2]));
-        for (int q = 0; q < n; ++q) {
-            float x = 0.f, y = 0.f;
-            HIP_TRY(hipEventElapsedTime(&x, ev[3 * q], ev[3 * q + 1]));
-            HIP_TRY(hipEventElapsedTime(&y, ev[3 * q + 1], ev[3 * q + 2]));
-            th += x;
-            td += y;
-        }
+    if (s->timed_ev.empty()) {
+        s->timed_ev.resize(3 * kRing + 2);
+        for (auto &e : s->timed_ev) HIP_TRY(hipEventCreate(&e));
     }
-    HIP_TRY(hipEventRecord(ev[3 * kChunk + 1], s->stream));
-    HIP_TRY(hipEventSynchronize(ev[3 * kChunk + 1]));
+    // eager chunks of kRing steps (launched as fdtd_run launches the steps it does not replay)
+    // with events around every fused pair, read after each chunk
+    Timing tm;
+    tm.ev = s->timed_ev.data();
+    fdtd_sim *self = s;
+    Stepper k = lone(self);
+    k.tm = &tm;
+    hipEvent_t t0 = s->timed_ev[3 * kRing], t1 = s->timed_ev[3 * kRing + 1];
+    HIP_TRY(hipEventRecord(t0, s->stream));
+    rc = run_steps(k, nsteps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(t1, s->stream));
+    HIP_TRY(hipEventSynchronize(t1));
     float total = 0.f;
-    HIP_TRY(hipEventElapsedTime(&total, ev[3 * kChunk], ev[3 * kChunk + 1]));
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    HIP_TRY(hipEventElapsedTime(&total, t0, t1));
     out_ms[0] = total;
-    out_ms[1] = nsteps ? (float)(th / nsteps) : 0.f;
-    out_ms[2] = nsteps ? (float)(td / nsteps) : 0.f;
+    out_ms[1] = nsteps ? (float)(tm.h_ms / nsteps) : 0.f;
+    out_ms[2] = nsteps ? (float)(tm.d_ms / nsteps) : 0.f;
     return FDTD_OK;
 }
 

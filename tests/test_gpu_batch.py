@@ -53,15 +53,16 @@ def _same_words(a, b):
 
 
 def test_graph_replay_steps_the_same_words_as_eager_launches():
-    """Captured 128-step sequences (re-captured when a call starts at another ring phase)
-    against one eager launch per step."""
+    """Captured 128-step sequences (re-captured when a call starts at another ring phase; one
+    run's first sequence starts at step 0, where update_H has not yet run) against one eager
+    launch per step."""
     from meep_gpu.driver import FdtdDriver
     from meep_gpu.engine import set_graphs
     from tests.workload_runs import small
 
     steps = 600
     runs = []
-    for graphs, chunks in ((False, (steps,)), (True, (70, 300, 230))):
+    for graphs, chunks in ((False, (steps,)), (True, (70, 300, 230)), (True, (256, 344))):
         set_graphs(graphs)
         try:
             drv = FdtdDriver.from_workload(small("sphere_box"))
@@ -72,6 +73,31 @@ def test_graph_replay_steps_the_same_words_as_eager_launches():
         finally:
             set_graphs(True)
         assert drv.step_count == steps
+        runs.append(drv)
+    _same_words(runs[0], runs[1])
+    _same_words(runs[0], runs[2])
+    for d in runs:
+        d.close()
+
+
+def test_timed_steps_are_the_steps_run_takes():
+    """fdtd_run_timed (eager chunks with events around every pair) against fdtd_run (a
+    captured sequence, a replay, eager steps): the same words, and sane per-launch means."""
+    from meep_gpu.driver import FdtdDriver
+    from tests.workload_runs import small
+
+    steps = 300
+    runs = []
+    for timed in (False, True):
+        drv = FdtdDriver.from_workload(small("sphere_box"))
+        sim = drv.freeze(planned_steps=steps + 1)
+        if timed:
+            sim.run(20)
+            total, h_ms, d_ms = sim.run_timed(steps - 20)  # chunks of 128, 128 and 24
+            assert 0 < h_ms and 0 < d_ms and (h_ms + d_ms) * (steps - 20) <= total * 1.001
+        else:
+            sim.run(steps)
+        sim.synchronize()
         runs.append(drv)
     _same_words(*runs)
     for d in runs:
