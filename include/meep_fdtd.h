@@ -143,6 +143,15 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
  * [component: E1, E2, H1, H2 (tangential, cyclic after the normal)][frequency][cell, x
  * fastest over the plane box]. Flushes buffered samples first. */
 int fdtd_flux_read(fdtd_sim *sim, int32_t plane, double *out, size_t count);
+/* Replace a plane's transforms with `in` (the layout fdtd_flux_read writes), after the samples
+ * still buffered are transformed in: MEEP's load_flux_data, which assigns every element and
+ * does not add (FluxMonitor.load_dft_data, reference dft.py:3288-3295, _load_flux_planes
+ * dft.py:661-733). The steps that follow accumulate on top. */
+int fdtd_flux_write(fdtd_sim *sim, int32_t plane, const double *in, size_t count);
+/* Multiply a plane's transforms (every component, E and H sides alike) by re + i*im, after the
+ * buffered samples are transformed in: FluxMonitor.scale_dfts (reference dft.py:3274-3286);
+ * -1 after fdtd_flux_write is MEEP's load_minus_flux_data (dft.py:3297-3316). */
+int fdtd_flux_scale(fdtd_sim *sim, int32_t plane, double re, double im);
 
 /* Batched form: many simulations advanced together, one fused launch per half-step over a
  * work list of (simulation, tile) entries, and one sampling launch for every flux plane of

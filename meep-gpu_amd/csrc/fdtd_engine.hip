@@ -1113,6 +1113,21 @@ __global__ void __launch_bounds__(256) flux_accumulate_mfma_kernel(
         }
 }
 
+// acc *= (re + i im) over n complex float64 words (dft_chunk::scale_dft as reference
+// dft.py:3274-3286 restates it: every stored element, E and H sides alike).
+__global__ void flux_scale_kernel(double *acc, size_t n, double re, double im) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    GMEM double *a = gp(acc) + 2 * q;
+    const double x = a[0], y = a[1];
+    double r = x * re;
+    r = r - y * im;
+    double i = x * im;
+    i = i + y * re;
+    a[0] = r;
+    a[1] = i;
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = v;
@@ -2333,6 +2348,38 @@ int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(out, ph.dev.acc, sizeof(double) * need, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+// Bring a plane's accumulators up to date (the ring's pending samples transformed in) before
+// they are replaced or scaled: the pending samples belong to the run before the edit.
+static int flux_plane_for_edit(fdtd_sim *s, int32_t plane, size_t count, bool check_count) {
+    if (!s || plane < 0 || plane >= (int)s->planes.size()) return fail(FDTD_EINVAL, "bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    PlaneHost &ph = s->planes[plane];
+    const size_t need = (size_t)2 * 4 * ph.dev.nf * ph.dev.cells;
+    if (check_count && count != need)
+        return fail(FDTD_EINVAL, "flux write needs %zu doubles, got %zu", need, count);
+    return flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+}
+
+int fdtd_flux_write(fdtd_sim *s, int32_t plane, const double *in, size_t count) {
+    if (!in) return fail(FDTD_EINVAL, "null argument");
+    int rc = flux_plane_for_edit(s, plane, count, true);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s->planes[plane].dev.acc, in, sizeof(double) * count,
+                           hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
+int fdtd_flux_scale(fdtd_sim *s, int32_t plane, double re, double im) {
+    int rc = flux_plane_for_edit(s, plane, 0, false);
+    if (rc) return rc;
+    const PlaneDev &p = s->planes[plane].dev;
+    const size_t n = (size_t)4 * p.nf * p.cells;
+    flux_scale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(p.acc, n, re, im);
+    HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
 

@@ -97,6 +97,75 @@ def axis_cells(n: int, dx: float, where_min: float, where_max: float) -> Tuple[i
     return first, w
 
 
+_AXES = "xyz"
+
+
+def tangential_names(normal: int) -> Tuple[Tuple[str, str], Tuple[str, str]]:
+    """(E names, H names) of the two tangential components, in cyclic order after the normal."""
+    t1, t2 = (normal + 1) % 3, (normal + 2) % 3
+    return ("E" + _AXES[t1], "E" + _AXES[t2]), ("H" + _AXES[t1], "H" + _AXES[t2])
+
+
+@dataclass(frozen=True)
+class FluxPlaneDft:
+    """One sample plane's accumulated transforms, detached from the run (host copies), as
+    reference ``FluxPlaneDft`` (dft.py:2618-2635): a named ``(nf, nx, ny, nz)`` complex block per
+    tangential component on each side, and the frequencies they were accumulated at."""
+    E: dict
+    H: dict
+    frequencies: tuple
+
+
+@dataclass(frozen=True)
+class FluxDftData:
+    """What ``FluxMonitor.get_dft_data`` returns (MEEP's ``sim.get_flux_data(flux)``), as
+    reference ``FluxDftData`` (dft.py:2638-2658). Opaque: the only supported use is handing it
+    back to ``load_dft_data`` / ``load_minus_dft_data`` of a monitor built the same way in a
+    later run. One record per sample plane of the monitor, in the monitor's own order."""
+    planes: tuple
+
+
+def _load_flux_planes(data, monitor: "FluxMonitor", description: str) -> None:
+    """Replace each sample plane's transforms from ``data`` (reference ``_load_flux_planes``,
+    dft.py:661-733, after MEEP's ``load_flux_data``: every element is assigned, nothing is
+    added). Checked plane by plane -- plane count, frequencies, component sets, block shapes --
+    before anything is written, so a refusal leaves the monitor untouched."""
+    if not isinstance(data, FluxDftData):
+        raise TypeError(f"{description} takes the object a flux monitor's get_dft_data() "
+                        f"returned (FluxDftData), got {type(data).__name__}")
+    ids = monitor._attached_ids(description)
+    if len(data.planes) != len(ids):
+        raise ValueError(f"{description} was given data for {len(data.planes)} flux plane(s) "
+                         f"but this monitor has {len(ids)}")
+    e_names, h_names = tangential_names(monitor.normal)
+    blocks = []
+    for index, (record, (lo, cnt, _)) in enumerate(zip(data.planes, monitor.plane_boxes())):
+        where = f"{description}, plane {index}"
+        if len(record.frequencies) != monitor.num_frequencies:
+            raise ValueError(f"{where}: the data holds {len(record.frequencies)} frequencies "
+                             f"and this monitor accumulates {monitor.num_frequencies}")
+        for saved, mine in zip(record.frequencies, monitor.frequencies):
+            if not math.isclose(float(saved), float(mine), rel_tol=1e-12, abs_tol=0.0):
+                raise ValueError(f"{where}: the data was accumulated at frequencies "
+                                 f"{record.frequencies}, this monitor at "
+                                 f"{tuple(monitor.frequencies)}")
+        shape = (monitor.num_frequencies,) + tuple(cnt)
+        stack = []
+        for side, names, block in (("E", e_names, record.E), ("H", h_names, record.H)):
+            if set(block) != set(names):
+                raise ValueError(f"{where}: the data's {side} components are {sorted(block)}, "
+                                 f"expected {sorted(names)}")
+            for name in names:
+                arr = numpy.asarray(block[name])
+                if arr.shape != shape:
+                    raise ValueError(f"{where}: {name} in the data has shape {arr.shape} and "
+                                     f"this monitor's plane is {shape}")
+                stack.append(arr)
+        blocks.append(numpy.stack(stack))
+    for pid, block in zip(ids, blocks):
+        monitor._sim.flux_write(pid, block)
+
+
 @dataclass
 class FluxMonitor:
     """A planar flux monitor: one or two sample planes of cell-centred fields with weights.
@@ -194,6 +263,42 @@ class FluxMonitor:
         if self._sim is None:
             raise RuntimeError("the monitor is not attached to a running simulation")
         return self._sim.flux_read(self.plane_id)
+
+    def _attached_ids(self, what: str) -> list:
+        if self._sim is None or self.plane_ids is None:
+            raise RuntimeError(f"{what}: the monitor is not attached to a running simulation "
+                               f"(freeze the driver first)")
+        return list(self.plane_ids)
+
+    def get_dft_data(self) -> FluxDftData:
+        """Every sample plane's accumulated transforms as host copies, MEEP's
+        ``sim.get_flux_data(flux)`` (reference ``FluxMonitor.get_dft_data``, dft.py:3261)."""
+        e_names, h_names = tangential_names(self.normal)
+        planes = []
+        for pid in self._attached_ids("get_dft_data"):
+            t = self._sim.flux_read(pid)
+            planes.append(FluxPlaneDft(E={e_names[0]: t[0].copy(), e_names[1]: t[1].copy()},
+                                       H={h_names[0]: t[2].copy(), h_names[1]: t[3].copy()},
+                                       frequencies=tuple(float(f) for f in self.frequencies)))
+        return FluxDftData(planes=tuple(planes))
+
+    def load_dft_data(self, data: FluxDftData) -> None:
+        """Replace the transforms with ``data``: MEEP's ``load_flux_data`` (reference
+        dft.py:3288-3295). The accumulation that follows adds on top."""
+        _load_flux_planes(data, self, "FluxMonitor.load_dft_data")
+
+    def scale_dfts(self, scale: complex) -> None:
+        """Multiply every transform, E and H sides alike, by ``scale`` (reference
+        dft.py:3274-3286, MEEP's dft_flux::scale_dfts)."""
+        for pid in self._attached_ids("scale_dfts"):
+            self._sim.flux_scale(pid, complex(scale))
+
+    def load_minus_dft_data(self, data: FluxDftData) -> None:
+        """``load_dft_data`` then negate: MEEP's ``load_minus_flux_data`` (reference
+        dft.py:3297-3316). The run then ends holding E2 - E1 and H2 - H1, and the flux it
+        reports is Re[(E2 - E1) x conj(H2 - H1)]: the fields are subtracted, not the powers."""
+        self.load_dft_data(data)
+        self.scale_dfts(-1.0)
 
     def get_flux_spectrum(self) -> numpy.ndarray:
         if self._sim is None:

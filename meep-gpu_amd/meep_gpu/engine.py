@@ -212,6 +212,23 @@ class HipSimulation:
         data = buf.view(numpy.complex128).reshape(4, nf, cnt[2], cnt[1], cnt[0])
         return numpy.ascontiguousarray(data.transpose(0, 1, 4, 3, 2))
 
+    def flux_write(self, plane: int, transforms: numpy.ndarray) -> None:
+        """Replace a plane's transforms: complex [4][nf][cnt...] as ``flux_read`` returns them."""
+        normal, lo, cnt, nf = self.planes[plane]
+        data = numpy.asarray(transforms)
+        if data.shape != (4, nf) + tuple(cnt):
+            raise ValueError(f"plane {plane} takes transforms of shape {(4, nf) + tuple(cnt)}, "
+                             f"got {data.shape}")
+        buf = numpy.ascontiguousarray(data.astype(numpy.complex128).transpose(0, 1, 4, 3, 2))
+        flat = buf.view(numpy.float64).reshape(-1)
+        check(self.lib.fdtd_flux_write(self.handle, int(plane), dptr(flat), flat.size),
+              "fdtd_flux_write")
+
+    def flux_scale(self, plane: int, scale: complex) -> None:
+        """Multiply a plane's transforms by ``scale``."""
+        z = complex(scale)
+        check(self.lib.fdtd_flux_scale(self.handle, int(plane), z.real, z.imag), "fdtd_flux_scale")
+
 
 def device_info() -> str:
     lib = _native.load()

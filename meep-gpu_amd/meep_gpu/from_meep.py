@@ -3,9 +3,10 @@
 
 ``GpuRunResult`` is the object a run hands back: the stepped driver plus what the run cost,
 with MEEP's read-out names (``get_flux_spectrum`` is ``mp.get_fluxes``; ``get_flux_data`` the
-transforms for the normalization idiom; ``get_array`` one field component; ``close``).
+opaque transforms for the normalization idiom; ``get_array`` one field component; ``close``).
 ``run_driver`` steps an ``FdtdDriver`` built through the driver API (materials, poles,
-absorber, sources, monitors) and returns one.
+absorber, sources, monitors) and returns one; its ``flux_data`` / ``minus_flux_data`` load an
+earlier run's transforms first (MEEP's ``load_flux_data`` / ``load_minus_flux_data``).
 
 Lifting an ``mp.Simulation`` (``lift_simulation``, ``gpu_compatibility``, ``run_on_gpu`` on a
 MEEP object) needs MEEP itself: MEEP computes the sub-pixel chi1inv, the chunk division and
@@ -23,7 +24,7 @@ from typing import Any, Optional, Sequence
 
 import numpy
 
-from .dft import FluxMonitor
+from .dft import FluxDftData, FluxMonitor
 from .driver import FdtdDriver
 
 MEEP_VERSION = "1.35.0"  # the MEEP release the reference certifies its lift on (README.md:468)
@@ -104,9 +105,11 @@ class GpuRunResult:
         """The flux spectrum of a monitor (``mp.get_fluxes``), float64 per frequency."""
         return self.monitor_for(monitor).get_flux_spectrum()
 
-    def get_flux_data(self, monitor) -> numpy.ndarray:
-        """The monitor's transforms, complex128 [E1, E2, H1, H2][frequency][plane cells]."""
-        return self.monitor_for(monitor).transforms()
+    def get_flux_data(self, monitor) -> FluxDftData:
+        """The monitor's accumulated transforms, MEEP's ``sim.get_flux_data(flux)`` (reference
+        from_meep.py:575): opaque, for ``run_driver(..., minus_flux_data=[(monitor, data)])`` on
+        a later run carrying the same monitor -- the two-run normalization idiom."""
+        return self.monitor_for(monitor).get_dft_data()
 
     def get_array(self, component: str) -> numpy.ndarray:
         """One stored field component, float32 C-order (nx, ny, nz), a private host copy."""
@@ -120,13 +123,24 @@ class GpuRunResult:
 
 
 def run_driver(driver: FdtdDriver, until: Optional[float] = None,
-               steps: Optional[int] = None) -> GpuRunResult:
+               steps: Optional[int] = None, flux_data: Sequence = (),
+               minus_flux_data: Sequence = ()) -> GpuRunResult:
     """Step ``driver`` for ``steps`` steps or until simulation time ``until`` (MEEP's
-    ``sim.run(until=...)``: ceil(until / dt) steps) and return the result object."""
+    ``sim.run(until=...)``: ceil(until / dt) steps) and return the result object.
+
+    ``flux_data`` / ``minus_flux_data``: ``(monitor, data)`` pairs loaded into the monitors
+    before the first step, MEEP's ``sim.load_flux_data`` / ``sim.load_minus_flux_data`` (data
+    from an earlier run's ``get_flux_data``; reference from_meep.py:575-620)."""
     if (until is None) == (steps is None):
         raise ValueError("give exactly one of until or steps")
     if until is not None:
         steps = int(math.ceil(until / driver.grid.dt - 1e-9))
+    if flux_data or minus_flux_data:
+        driver.freeze(max(int(steps) + driver.step_count, driver.max_steps))
+        for monitor, data in flux_data:
+            monitor.load_dft_data(data)
+        for monitor, data in minus_flux_data:
+            monitor.load_minus_dft_data(data)
     t0 = time.perf_counter()
     driver.run(steps=int(steps))
     driver.synchronize()
@@ -138,6 +152,8 @@ def run_on_gpu(sim, until: Optional[float] = None, **kwargs) -> GpuRunResult:
     """MEEP's ``sim.run(until=...)`` with the loop on the GPU; an ``FdtdDriver`` runs directly,
     an ``mp.Simulation`` is lifted first (needs MEEP)."""
     if isinstance(sim, FdtdDriver):
-        return run_driver(sim, until=until, steps=kwargs.get("steps"))
+        return run_driver(sim, until=until, steps=kwargs.get("steps"),
+                          flux_data=kwargs.get("flux_data", ()),
+                          minus_flux_data=kwargs.get("minus_flux_data", ()))
     driver = lift_simulation(sim, **kwargs)
     return run_driver(driver, until=until)
