@@ -139,9 +139,20 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
                         const int32_t cnt[3], int32_t nf, const double *omega, double scale,
                         int32_t decimation, int32_t *plane);
 
+/* A DFT field region, MEEP's add_dft_fields (DFTMonitor, reference dft.py:1960; the region's
+ * block as FdtdDriver.get_dft returns it): the cells [lo, lo+cnt) of the grid, ncomp (1..4)
+ * components (codes 0..5 = Ex, Ey, Ez, Hx, Hy, Hz) sampled each accumulating step at the cell
+ * centre (yee_grid = 0: MEEP's yee2cent, as for flux planes) or at each component's own Yee site
+ * (yee_grid = 1), E at t and H at t - dt/2, and transformed in float64 like a flux plane. It
+ * shares the flux planes' ids and read/write/scale calls (its components in place of E1..H2). */
+int fdtd_add_dft_region(fdtd_sim *sim, const int32_t lo[3], const int32_t cnt[3], int32_t ncomp,
+                        const int32_t *comps, int32_t yee_grid, int32_t nf, const double *omega,
+                        double scale, int32_t decimation, int32_t *region);
+
 /* Copy one plane's accumulated transforms to the host as float64 (re, im) pairs, laid out
- * [component: E1, E2, H1, H2 (tangential, cyclic after the normal)][frequency][cell, x
- * fastest over the plane box]. Flushes buffered samples first. */
+ * [component: E1, E2, H1, H2 (tangential, cyclic after the normal), or a DFT region's own
+ * components in order][frequency][cell, x fastest over the box]. Flushes buffered samples
+ * first. */
 int fdtd_flux_read(fdtd_sim *sim, int32_t plane, double *out, size_t count);
 /* Replace a plane's transforms with `in` (the layout fdtd_flux_read writes), after the samples
  * still buffered are transformed in: MEEP's load_flux_data, which assigns every element and

@@ -888,9 +888,15 @@ __global__ void prime_e_kernel(SimDev *sims, int count) {
 // ------------------------------------------------------------------------------------------
 // Flux planes: cell-centred tangential samples into a ring, then a float64 transform pass.
 // ------------------------------------------------------------------------------------------
+// A DFT region: the cells [lo, lo+cnt) of a simulation, with nslot (1..4) sampled components.
+// A flux plane is the region one cell thick along its normal with the four tangential
+// components E1, E2, H1, H2, cell-centred.
 struct PlaneDev {
     const SimDev *sim;  // the descriptor the plane's simulation is stepped through
-    int normal;
+    int nslot;
+    int comp[4];        // per slot: 0..2 = Ex..Ez, 3..5 = Hx..Hz
+    int yee;            // 1: the component's own Yee site; 0: the cell centre (yee2cent)
+    int hmask;          // bit r: slot r is magnetic (sampled at t - dt/2)
     int lo[3], cnt[3];
     int cells;
     int row;            // ring row stride: cells rounded up to kCB
@@ -899,10 +905,10 @@ struct PlaneDev {
     int first_step;     // sampling counts steps after this one
     double scale;
     const double *omega;
-    float *ring;        // [kRing][4][row]
+    float *ring;        // [kRing][nslot][row]
     double *times;      // [kRing] electric sample time t (magnetic is t - dt/2)
     double *phase;      // [kRing][2][nf] (re, im)
-    double *acc;        // [4][nf][cells] (re, im)
+    double *acc;        // [nslot][nf][cells] (re, im)
 };
 
 // MEEP yee2cent (dft.py:352): average along every axis where the component's Yee shift is 0,
@@ -956,7 +962,17 @@ __device__ inline float centred_h(const SimDev &s, int c, int i, int j, int k) {
     return 0.5f * (h_at(s, c, i, j, k) + h_at(s, c, i1, j1, k1));
 }
 
-// One launch samples every plane of a simulation (or of a batch) that accumulates this step.
+// Component c of a region at cell (i, j, k): at the component's own Yee site, or centred.
+__device__ inline float sample_of(const SimDev &s, int c, int yee, int i, int j, int k) {
+    if (c < 3) {
+        if (yee) return gp(s.E[c])[(k * s.n[1] + j) * s.pitch + i];
+        return centred(s, s.E[c], 1 << c, i, j, k);  // E_c: Yee shift 1 along its own axis
+    }
+    if (yee) return h_at(s, c - 3, i, j, k);
+    return centred_h(s, c - 3, i, j, k);
+}
+
+// One launch samples every region of a simulation (or of a batch) that accumulates this step.
 // `adv`: steps since the descriptors' last counter advance (the sample follows the step's
 // advance, so a captured step t samples at the descriptor's step + t + 1).
 __global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes, int adv) {
@@ -974,12 +990,8 @@ __global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes, int adv)
     const int a = q % p.cnt[0], r = q / p.cnt[0];
     const int b = r % p.cnt[1], e = r / p.cnt[1];
     const int i = (p.lo[0] + a) % s.n[0], j = (p.lo[1] + b) % s.n[1], k = (p.lo[2] + e) % s.n[2];
-    const int t1 = (p.normal + 1) % 3, t2 = (p.normal + 2) % 3;
-    GMEM float *out = gp(p.ring) + (size_t)slot * 4 * p.row;
-    out[0 * p.row + q] = centred(s, s.E[t1], 1 << t1, i, j, k);
-    out[1 * p.row + q] = centred(s, s.E[t2], 1 << t2, i, j, k);
-    out[2 * p.row + q] = centred_h(s, t1, i, j, k);
-    out[3 * p.row + q] = centred_h(s, t2, i, j, k);
+    GMEM float *out = gp(p.ring) + (size_t)slot * p.nslot * p.row;
+    for (int r = 0; r < p.nslot; ++r) out[r * p.row + q] = sample_of(s, p.comp[r], p.yee, i, j, k);
 }
 
 // exp(i*omega*t)*scale in double for each buffered sample time, E at t and H at t - dt/2.
@@ -1010,10 +1022,10 @@ constexpr int kFZ = 16;  // frequencies per workgroup
 
 __global__ void __launch_bounds__(256) flux_accumulate_kernel(
     const float *__restrict__ ring, const double *__restrict__ phase, double *__restrict__ acc,
-    int cells, int row, int nf, int nslots) {
+    int cells, int row, int nf, int nslots, int nslot, int hmask) {
     __shared__ double2 ph_s[kRing][kFZ];
     const int comp = blockIdx.y;
-    const int side = comp < 2 ? 0 : 1;
+    const int side = (hmask >> comp) & 1;
     const int fz0 = blockIdx.z * kFZ;
     const int fzn = nf - fz0 < kFZ ? nf - fz0 : kFZ;
     for (int x = threadIdx.x; x < kRing * kFZ; x += blockDim.x) {
@@ -1036,7 +1048,7 @@ __global__ void __launch_bounds__(256) flux_accumulate_kernel(
 #pragma unroll
             for (int c = 0; c < kCB; ++c) re[fi][c] = im[fi][c] = 0.0;
         for (int m = 0; m < nslots; ++m) {
-            const float4 sv = *reinterpret_cast<const float4 *>(ring + ((size_t)m * 4 + comp) * row + q0);
+            const float4 sv = *reinterpret_cast<const float4 *>(ring + ((size_t)m * nslot + comp) * row + q0);
             const double sm[kCB] = {(double)sv.x, (double)sv.y, (double)sv.z, (double)sv.w};
 #pragma unroll
             for (int fi = 0; fi < kFB; ++fi) {
@@ -1074,10 +1086,10 @@ constexpr int kMT = 4;  // 16-row n tiles per wave
 
 __global__ void __launch_bounds__(256) flux_accumulate_mfma_kernel(
     const float *__restrict__ ring, const double *__restrict__ phase, double *__restrict__ acc,
-    int cells, int row, int nf, int nslots) {
+    int cells, int row, int nf, int nslots, int nslot, int hmask) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int comp = blockIdx.y;
-    const int side = comp < 2 ? 0 : 1;
+    const int side = (hmask >> comp) & 1;
     const int cell0 = (blockIdx.x * 4 + wave) * 16;
     if (cell0 >= cells) return;
     const int nn = 2 * nf;
@@ -1091,7 +1103,7 @@ __global__ void __launch_bounds__(256) flux_accumulate_mfma_kernel(
     for (int m0 = 0; m0 < nslots; m0 += 4) {
         const int m = m0 + kk;
         double b = 0.0;
-        if (m < nslots) b = (double)ring[((size_t)m * 4 + comp) * row + cell];  // row >= cell0+16
+        if (m < nslots) b = (double)ring[((size_t)m * nslot + comp) * row + cell];  // row >= cell0+16
 #pragma unroll
         for (int t = 0; t < kMT; ++t) {
             const int n = n0 + t * 16 + ni;
@@ -1448,12 +1460,14 @@ int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
     const PlaneDev &p = ph.dev;
     flux_phase_kernel<<<dim3((ph.filled * p.nf + 255) / 256, 1), 256, 0, st>>>(dev_plane, ph.filled);
     if (g_flux_mfma) {
-        flux_accumulate_mfma_kernel<<<dim3((p.cells + 63) / 64, 4, (2 * p.nf + 16 * kMT - 1) / (16 * kMT)),
-                                      256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+        flux_accumulate_mfma_kernel<<<dim3((p.cells + 63) / 64, p.nslot, (2 * p.nf + 16 * kMT - 1) / (16 * kMT)),
+                                      256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled,
+                                                    p.nslot, p.hmask);
     } else {
         const int per_block = 256 * kCB;
-        flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, 4, (p.nf + kFZ - 1) / kFZ),
-                                 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled);
+        flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, p.nslot, (p.nf + kFZ - 1) / kFZ),
+                                 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled,
+                                               p.nslot, p.hmask);
     }
     HIP_TRY(hipGetLastError());
     ph.filled = 0;
@@ -2281,17 +2295,17 @@ int fdtd_synchronize(fdtd_sim *s) {
     return FDTD_OK;
 }
 
-int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const int32_t cnt[3],
-                        int32_t nf, const double *omega, double scale, int32_t dec, int32_t *plane) {
-    if (!s || normal < 0 || normal > 2 || !lo || !cnt || nf < 1 || !omega || dec < 1)
-        return fail(FDTD_EINVAL, "fdtd_add_flux_plane: bad argument");
-    if (cnt[normal] != 1) return fail(FDTD_EINVAL, "a flux plane is one cell thick along its normal");
-    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    s->graph.reset();  // a captured sequence samples the planes it was captured with
+// A DFT region's device state: the ring, the sample times, the phase table and the float64
+// accumulators, registered after the simulation's other regions.
+static int add_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], int nslot,
+                      const int32_t *comps, int yee, int32_t nf, const double *omega,
+                      double scale, int32_t dec, int32_t *plane) {
+    s->graph.reset();  // a captured sequence samples the regions it was captured with
     int cells = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] >= s->host.n[a] || cnt[a] > s->host.n[a])
-            return fail(FDTD_EINVAL, "flux box axis %d [%d,+%d) outside the grid", a, lo[a], cnt[a]);
+            return fail(FDTD_EINVAL, "region box axis %d [%d,+%d) outside the grid", a, lo[a], cnt[a]);
+        if ((int64_t)cells * cnt[a] > (1 << 30)) return fail(FDTD_EINVAL, "region too large");
         cells *= cnt[a];
     }
     int rc = s->pull_desc();
@@ -2299,7 +2313,11 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     PlaneHost ph;
     PlaneDev &p = ph.dev;
     p.sim = s->dev;
-    p.normal = normal;
+    p.nslot = nslot;
+    p.hmask = 0;
+    for (int r = 0; r < 4; ++r) p.comp[r] = r < nslot ? comps[r] : 0;
+    for (int r = 0; r < nslot; ++r) p.hmask |= (comps[r] >= 3 ? 1 : 0) << r;
+    p.yee = yee ? 1 : 0;
     for (int a = 0; a < 3; ++a) { p.lo[a] = lo[a]; p.cnt[a] = cnt[a]; }
     p.cells = cells;
     p.row = ((cells + 15) / 16) * 16;  // whole 16-cell tiles for the matrix-core transform
@@ -2312,9 +2330,9 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(m, omega, sizeof(double) * nf, hipMemcpyHostToDevice, s->stream));
     p.omega = (const double *)m;
-    rc = s->alloc(&m, sizeof(float) * (size_t)kRing * 4 * p.row);
+    rc = s->alloc(&m, sizeof(float) * (size_t)kRing * nslot * p.row);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(m, 0, sizeof(float) * (size_t)kRing * 4 * p.row, s->stream));
+    HIP_TRY(hipMemsetAsync(m, 0, sizeof(float) * (size_t)kRing * nslot * p.row, s->stream));
     p.ring = (float *)m;
     rc = s->alloc(&m, sizeof(double) * kRing);
     if (rc) return rc;
@@ -2322,9 +2340,9 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     rc = s->alloc(&m, sizeof(double) * (size_t)kRing * 4 * nf);
     if (rc) return rc;
     p.phase = (double *)m;
-    rc = s->alloc(&m, sizeof(double) * 2 * 4 * (size_t)nf * cells);
+    rc = s->alloc(&m, sizeof(double) * 2 * (size_t)nslot * nf * cells);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(m, 0, sizeof(double) * 2 * 4 * (size_t)nf * cells, s->stream));
+    HIP_TRY(hipMemsetAsync(m, 0, sizeof(double) * 2 * (size_t)nslot * nf * cells, s->stream));
     p.acc = (double *)m;
     s->planes.push_back(ph);
     rc = s->alloc(&m, sizeof(PlaneDev) * s->planes.size());
@@ -2338,11 +2356,33 @@ int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const 
     return FDTD_OK;
 }
 
+int fdtd_add_flux_plane(fdtd_sim *s, int32_t normal, const int32_t lo[3], const int32_t cnt[3],
+                        int32_t nf, const double *omega, double scale, int32_t dec, int32_t *plane) {
+    if (!s || normal < 0 || normal > 2 || !lo || !cnt || nf < 1 || !omega || dec < 1)
+        return fail(FDTD_EINVAL, "fdtd_add_flux_plane: bad argument");
+    if (cnt[normal] != 1) return fail(FDTD_EINVAL, "a flux plane is one cell thick along its normal");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    const int t1 = (normal + 1) % 3, t2 = (normal + 2) % 3;
+    const int32_t comps[4] = {t1, t2, 3 + t1, 3 + t2};
+    return add_region(s, lo, cnt, 4, comps, 0, nf, omega, scale, dec, plane);
+}
+
+int fdtd_add_dft_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], int32_t ncomp,
+                        const int32_t *comps, int32_t yee_grid, int32_t nf, const double *omega,
+                        double scale, int32_t dec, int32_t *region) {
+    if (!s || !lo || !cnt || ncomp < 1 || ncomp > 4 || !comps || nf < 1 || !omega || dec < 1)
+        return fail(FDTD_EINVAL, "fdtd_add_dft_region: bad argument");
+    for (int r = 0; r < ncomp; ++r)
+        if (comps[r] < 0 || comps[r] > 5) return fail(FDTD_EINVAL, "component code %d (0..5: Ex..Hz)", comps[r]);
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    return add_region(s, lo, cnt, ncomp, comps, yee_grid, nf, omega, scale, dec, region);
+}
+
 int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
     if (!s || plane < 0 || plane >= (int)s->planes.size() || !out) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     PlaneHost &ph = s->planes[plane];
-    const size_t need = (size_t)2 * 4 * ph.dev.nf * ph.dev.cells;
+    const size_t need = (size_t)2 * ph.dev.nslot * ph.dev.nf * ph.dev.cells;
     if (count != need) return fail(FDTD_EINVAL, "flux read needs %zu doubles, got %zu", need, count);
     int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
     if (rc) return rc;
@@ -2357,7 +2397,7 @@ static int flux_plane_for_edit(fdtd_sim *s, int32_t plane, size_t count, bool ch
     if (!s || plane < 0 || plane >= (int)s->planes.size()) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     PlaneHost &ph = s->planes[plane];
-    const size_t need = (size_t)2 * 4 * ph.dev.nf * ph.dev.cells;
+    const size_t need = (size_t)2 * ph.dev.nslot * ph.dev.nf * ph.dev.cells;
     if (check_count && count != need)
         return fail(FDTD_EINVAL, "flux write needs %zu doubles, got %zu", need, count);
     return flux_flush_plane(s->dev_planes + plane, ph, s->stream);
@@ -2377,7 +2417,7 @@ int fdtd_flux_scale(fdtd_sim *s, int32_t plane, double re, double im) {
     int rc = flux_plane_for_edit(s, plane, 0, false);
     if (rc) return rc;
     const PlaneDev &p = s->planes[plane].dev;
-    const size_t n = (size_t)4 * p.nf * p.cells;
+    const size_t n = (size_t)p.nslot * p.nf * p.cells;
     flux_scale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(p.acc, n, re, im);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;

@@ -12,6 +12,12 @@ with (E1, E2, H1, H2) the tangential components in cyclic order after the normal
 the normal Poynting component comes out with MEEP's sign for x, y and z normals alike.
 A plane given as a whole-cell index box has unit weights; ``FluxMonitor.from_region`` places
 a MEEP ``FluxRegion`` as MEEP does (its ladder weights, dft.py:604/:2753).
+
+``DftFieldsMonitor`` is MEEP's ``add_dft_fields`` (reference ``DFTMonitor``, dft.py:1960): the
+transform of up to any number of components over a box of cells, cell-centred (yee2cent) or at
+each component's own Yee site, on the same device machinery as the flux planes (at most four
+components per engine region); ``get_dft_array`` reduces a zero-thickness axis with MEEP's
+interpolation weights as ``sim.get_dft_array`` does (reference from_meep.py:886-955).
 """
 
 from __future__ import annotations
@@ -320,3 +326,126 @@ def flux_spectrum(transforms: numpy.ndarray, area_element: float, weight: float 
     if cell_weights is not None:
         s = s * cell_weights.reshape((1,) + s.shape[1:])
     return numpy.sum(s, axis=axes) * area_element * weight
+
+
+_FIELD_NAMES = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+@dataclass
+class DftFieldsMonitor:
+    """The DFT of field components over the cells [lo, lo+cnt) (reference ``DFTMonitor``,
+    dft.py:1960-2300): every accumulating step each component is sampled -- at the cell centre
+    (MEEP's yee2cent, dft.py:352, the default) or at its own Yee site (``yee_grid``) -- E at t
+    and H at t - dt/2, times exp(i omega t) dt*decimation/sqrt(2 pi), summed in float64.
+
+    ``empty_axes`` maps an axis the requested volume is flat along to the interpolation weights
+    of the one or two cells registered there (the ladder's, ``boundary_weight_ladder``);
+    ``get_dft_array`` sums that axis away with them, and drops invariant axes, as MEEP's
+    ``get_dft_array`` does (process_dft_component and collapse_array, reference
+    from_meep.py:886-955).
+    """
+    components: Tuple[str, ...]
+    lo: Tuple[int, int, int]
+    cnt: Tuple[int, int, int]
+    frequencies: numpy.ndarray
+    dt: float
+    decimation: int = 1
+    yee_grid: bool = False
+    empty_axes: dict = field(default_factory=dict)
+    invariant: Tuple[bool, bool, bool] = (False, False, False)
+    _sim: object = field(default=None, repr=False)
+    _slots: dict = field(default_factory=dict, repr=False)
+
+    def __post_init__(self):
+        self.components = tuple(str(c) for c in self.components)
+        if not self.components:
+            raise ValueError("a DFT field monitor needs at least one component")
+        unknown = [c for c in self.components if c not in _FIELD_NAMES]
+        if unknown or len(set(self.components)) != len(self.components):
+            raise ValueError(f"DFT field components must be distinct names of {_FIELD_NAMES}, "
+                             f"got {self.components}")
+        self.frequencies = numpy.atleast_1d(numpy.asarray(self.frequencies, dtype=numpy.float64))
+        if self.frequencies.size < 1:
+            raise ValueError("a DFT field monitor needs at least one frequency")
+        if int(self.decimation) < 1:
+            raise ValueError("decimation must be a positive integer")
+        self.lo = tuple(int(v) for v in self.lo)
+        self.cnt = tuple(int(v) for v in self.cnt)
+        for axis, w in self.empty_axes.items():
+            if len(w) != self.cnt[axis]:
+                raise ValueError(f"axis {axis}: {len(w)} weights for {self.cnt[axis]} cells")
+
+    @classmethod
+    def from_region(cls, grid, components, center, size, frequencies, decimation: int = 1,
+                    yee_grid: bool = False) -> "DftFieldsMonitor":
+        """The cells MEEP's axis ladder registers for the volume (center, size) -- coordinates
+        with their origin at the cell centre -- as for a flux region: an extended axis takes
+        the ladder's cells, a flat one the one or two cell-centre planes bracketing it, with
+        their interpolation weights (reference ``set_region_from_volume`` /
+        ``_register_volume``, dft.py:604-706)."""
+        lo, cnt, empty = [0, 0, 0], [1, 1, 1], {}
+        for a in range(3):
+            if grid.is_invariant(a):
+                if size[a] != 0:
+                    raise ValueError(f"axis {a} is invariant: the region must be flat along it")
+                continue
+            first, w = axis_cells(grid.shape[a], grid.dx, center[a] - 0.5 * size[a],
+                                  center[a] + 0.5 * size[a])
+            if size[a] == 0:
+                if yee_grid:
+                    raise ValueError("a yee_grid region flat along an axis interpolates on the "
+                                     "component's own lattice, which this build does not place; "
+                                     "give it one cell of thickness instead")
+                keep = [q for q in range(w.size) if w[q] != 0.0] or [0]
+                first, w = first + keep[0], w[keep[0]:keep[-1] + 1]
+                empty[a] = tuple(float(x) for x in w)
+            lo[a], cnt[a] = first, w.size
+        return cls(tuple(components), tuple(lo), tuple(cnt), frequencies, grid.dt, decimation,
+                   yee_grid, empty, tuple(grid.is_invariant(a) for a in range(3)))
+
+    @property
+    def omegas(self) -> numpy.ndarray:
+        return 2.0 * math.pi * self.frequencies
+
+    @property
+    def scale(self) -> float:
+        return self.dt * self.decimation / math.sqrt(2.0 * math.pi)
+
+    @property
+    def num_frequencies(self) -> int:
+        return int(self.frequencies.size)
+
+    def attach(self, sim) -> None:
+        """Register the engine regions: the components four at a time."""
+        self._sim, self._slots = sim, {}
+        for g in range(0, len(self.components), 4):
+            group = self.components[g:g + 4]
+            rid = sim.add_dft_region(self.lo, self.cnt, group, self.omegas, self.scale,
+                                     self.decimation, self.yee_grid)
+            for slot, name in enumerate(group):
+                self._slots[name] = (rid, slot)
+
+    def get_dft(self, component: str, freq_index: int = 0) -> numpy.ndarray:
+        """The component's transform over the box, complex128 (cnt_x, cnt_y, cnt_z), at one
+        frequency (reference ``DFTMonitor.get_dft``, dft.py:2259)."""
+        if self._sim is None:
+            raise RuntimeError("the monitor is not attached to a running simulation")
+        if component not in self._slots:
+            raise ValueError(f"component {component!r} is not monitored (monitored: "
+                             f"{self.components})")
+        if not 0 <= int(freq_index) < self.num_frequencies:
+            raise IndexError(f"frequency index {freq_index} of {self.num_frequencies}")
+        rid, slot = self._slots[component]
+        return self._sim.flux_read(rid)[slot, int(freq_index)].copy()
+
+    def get_dft_array(self, component: str, freq_index: int = 0) -> numpy.ndarray:
+        """``sim.get_dft_array(dft_fields, component, freq_index)``: the block with every flat
+        axis summed away under its interpolation weights and every invariant axis dropped."""
+        block = self.get_dft(component, freq_index)
+        for axis in (2, 1, 0):
+            if axis in self.empty_axes:
+                w = numpy.asarray(self.empty_axes[axis], dtype=numpy.float64)
+                block = numpy.tensordot(block, w, axes=([axis], [0]))
+            elif self.invariant[axis]:
+                block = block.take(0, axis=axis)
+        return block

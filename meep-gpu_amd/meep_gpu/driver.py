@@ -19,7 +19,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy
 
-from .dft import FluxMonitor, meep_frequency_span
+from .dft import DftFieldsMonitor, FluxMonitor, meep_frequency_span
 from .dispersion import Susceptibility
 from .engine import HipSimulation
 from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
@@ -46,6 +46,8 @@ class FdtdDriver:
         self._pml_kwargs = {}
         self._sources: List[Tuple[DepositBox, object]] = []
         self.flux_monitors: List[FluxMonitor] = []
+        self.dft_monitors: List[DftFieldsMonitor] = []
+        self._default_dft_regions: List[DftFieldsMonitor] = []
         self.max_steps = int(max_steps)
         self.sim: Optional[HipSimulation] = None
         self._fast_path = None
@@ -126,6 +128,47 @@ class FdtdDriver:
         self.flux_monitors.append(mon)
         return mon
 
+    def add_dft_monitor(self, frequencies=None, components: Sequence[str] = ("Ex", "Ey", "Ez"),
+                        center: Optional[Sequence[float]] = None,
+                        size: Optional[Sequence[float]] = None, *, fcen: float = None,
+                        df: float = None, nfreq: int = None, decimation: int = 1,
+                        yee_grid: bool = False) -> DftFieldsMonitor:
+        """``sim.add_dft_fields(components, fcen, df, nfreq, center=, size=)`` (reference
+        ``FdtdDriver.add_dft_monitor``, driver.py:2783): the DFT of ``components`` over the
+        volume (center, size), or, without one, over the absorber's interior (the whole grid
+        without an absorber), resolved when stepping starts. ``decimation`` is an explicit
+        positive accumulation stride."""
+        self._require_unfrozen("add_dft_monitor")
+        freqs = (numpy.atleast_1d(numpy.asarray(frequencies, dtype=numpy.float64))
+                 if frequencies is not None else meep_frequency_span(fcen, df, nfreq))
+        if (center is None) != (size is None):
+            raise ValueError("a DFT monitor region needs both center and size, or neither")
+        if size is not None and any(float(v) < 0 for v in size):
+            raise ValueError(f"monitor size must be non-negative, got {tuple(size)}")
+        if center is not None:
+            mon = DftFieldsMonitor.from_region(self.grid, tuple(components), tuple(center),
+                                               tuple(size), freqs, int(decimation), yee_grid)
+        else:  # placed at freeze, when the absorber is known
+            mon = DftFieldsMonitor(tuple(components), (0, 0, 0), tuple(self.grid.shape), freqs,
+                                   self.grid.dt, int(decimation), yee_grid,
+                                   invariant=tuple(self.grid.is_invariant(a) for a in range(3)))
+            self._default_dft_regions.append(mon)
+        self.dft_monitors.append(mon)
+        return mon
+
+    def _place_default_dft_regions(self, pml) -> None:
+        """The absorber's interior: each face trimmed by the whole cells it grades."""
+        for mon in self._default_dft_regions:
+            lo, cnt = [0, 0, 0], list(self.grid.shape)
+            if pml is not None:
+                for a in range(3):
+                    low, high = (int(math.ceil(float(t) - 1e-9)) for t in pml.faces[a])
+                    lo[a], cnt[a] = low, self.grid.shape[a] - low - high
+                    if cnt[a] < 1:
+                        raise ValueError(f"axis {a}: the absorber leaves no interior")
+            mon.lo, mon.cnt = tuple(lo), tuple(cnt)
+        self._default_dft_regions = []
+
     @classmethod
     def from_workload(cls, wl: Workload) -> "FdtdDriver":
         drv = cls.__new__(cls)
@@ -137,6 +180,8 @@ class FdtdDriver:
         drv._pml_kwargs = {}
         drv._sources = list(wl.sources)
         drv.flux_monitors = []
+        drv.dft_monitors = []
+        drv._default_dft_regions = []
         drv.max_steps = max(wl.steps, 1)
         drv.sim = None
         drv._fast_path = None
@@ -174,6 +219,9 @@ class FdtdDriver:
             sim.add_source(box.magnetic, axis, box.lo, box.cnt, box.amp_re, box.amp_im, table,
                            self.grid.dt)
         for mon in self.flux_monitors:
+            mon.attach(sim)
+        self._place_default_dft_regions(pml)
+        for mon in self.dft_monitors:
             mon.attach(sim)
         if pml is None and not self._poles and not self._offdiag:
             sim.prime_e()

@@ -199,26 +199,48 @@ class HipSimulation:
                                            int(om.size), dptr(om), float(scale), int(decimation),
                                            iptr(out)), "fdtd_add_flux_plane")
         self.planes.append((int(normal), tuple(int(v) for v in lo), tuple(int(v) for v in cnt),
-                            int(om.size)))
+                            int(om.size), 4))
+        return int(out[0])
+
+    def add_dft_region(self, lo: Sequence[int], cnt: Sequence[int], components: Sequence[str],
+                       omegas: numpy.ndarray, scale: float, decimation: int = 1,
+                       yee_grid: bool = False) -> int:
+        """A DFT field region over the cells [lo, lo+cnt): up to 4 components, cell-centred or
+        at their own Yee sites (``fdtd_add_dft_region``). Read back with ``flux_read``."""
+        names = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+        if not 1 <= len(components) <= 4 or any(c not in names for c in components):
+            raise ValueError(f"a DFT region takes 1 to 4 of {names}, got {tuple(components)}")
+        om = numpy.ascontiguousarray(omegas, dtype=numpy.float64)
+        lo_a = numpy.asarray(lo, dtype=numpy.int32)
+        cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
+        codes = numpy.asarray([names.index(c) for c in components], dtype=numpy.int32)
+        out = numpy.zeros(1, dtype=numpy.int32)
+        check(self.lib.fdtd_add_dft_region(self.handle, iptr(lo_a), iptr(cnt_a), int(codes.size),
+                                           iptr(codes), 1 if yee_grid else 0, int(om.size),
+                                           dptr(om), float(scale), int(decimation), iptr(out)),
+              "fdtd_add_dft_region")
+        self.planes.append((-1, tuple(int(v) for v in lo), tuple(int(v) for v in cnt),
+                            int(om.size), int(codes.size)))
         return int(out[0])
 
     def flux_read(self, plane: int) -> numpy.ndarray:
-        """complex128 transforms [4 (E1, E2, H1, H2)][nf][cnt...] in C-order over the box."""
-        normal, lo, cnt, nf = self.planes[plane]
+        """complex128 transforms [4 (E1, E2, H1, H2), or a region's components][nf][cnt...] in
+        C-order over the box."""
+        normal, lo, cnt, nf, nslot = self.planes[plane]
         cells = int(numpy.prod(cnt))
-        buf = numpy.empty(2 * 4 * nf * cells, dtype=numpy.float64)
+        buf = numpy.empty(2 * nslot * nf * cells, dtype=numpy.float64)
         check(self.lib.fdtd_flux_read(self.handle, int(plane), dptr(buf), buf.size),
               "fdtd_flux_read")
-        data = buf.view(numpy.complex128).reshape(4, nf, cnt[2], cnt[1], cnt[0])
+        data = buf.view(numpy.complex128).reshape(nslot, nf, cnt[2], cnt[1], cnt[0])
         return numpy.ascontiguousarray(data.transpose(0, 1, 4, 3, 2))
 
     def flux_write(self, plane: int, transforms: numpy.ndarray) -> None:
         """Replace a plane's transforms: complex [4][nf][cnt...] as ``flux_read`` returns them."""
-        normal, lo, cnt, nf = self.planes[plane]
+        normal, lo, cnt, nf, nslot = self.planes[plane]
         data = numpy.asarray(transforms)
-        if data.shape != (4, nf) + tuple(cnt):
-            raise ValueError(f"plane {plane} takes transforms of shape {(4, nf) + tuple(cnt)}, "
-                             f"got {data.shape}")
+        if data.shape != (nslot, nf) + tuple(cnt):
+            raise ValueError(f"plane {plane} takes transforms of shape "
+                             f"{(nslot, nf) + tuple(cnt)}, got {data.shape}")
         buf = numpy.ascontiguousarray(data.astype(numpy.complex128).transpose(0, 1, 4, 3, 2))
         flat = buf.view(numpy.float64).reshape(-1)
         check(self.lib.fdtd_flux_write(self.handle, int(plane), dptr(flat), flat.size),

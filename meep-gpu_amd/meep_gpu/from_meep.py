@@ -3,7 +3,8 @@
 
 ``GpuRunResult`` is the object a run hands back: the stepped driver plus what the run cost,
 with MEEP's read-out names (``get_flux_spectrum`` is ``mp.get_fluxes``; ``get_flux_data`` the
-opaque transforms for the normalization idiom; ``get_array`` one field component; ``close``).
+opaque transforms for the normalization idiom; ``get_dft_array`` a DFT field monitor's reduced
+block; ``get_array`` one field component; ``close``).
 ``run_driver`` steps an ``FdtdDriver`` built through the driver API (materials, poles,
 absorber, sources, monitors) and returns one; its ``flux_data`` / ``minus_flux_data`` load an
 earlier run's transforms first (MEEP's ``load_flux_data`` / ``load_minus_flux_data``).
@@ -110,6 +111,16 @@ class GpuRunResult:
         from_meep.py:575): opaque, for ``run_driver(..., minus_flux_data=[(monitor, data)])`` on
         a later run carrying the same monitor -- the two-run normalization idiom."""
         return self.monitor_for(monitor).get_dft_data()
+
+    def get_dft_array(self, monitor, component: str, freq_index: int = 0) -> numpy.ndarray:
+        """MEEP's ``sim.get_dft_array(dft_fields, component, freq_index)`` for a monitor of
+        ``driver.add_dft_monitor`` (reference from_meep.py:886): the region's transform with
+        its flat axes reduced by their interpolation weights and its invariant axes dropped."""
+        if isinstance(monitor, int):
+            monitor = self.driver.dft_monitors[monitor]
+        if monitor not in self.driver.dft_monitors:
+            raise KeyError("not a DFT field monitor of this run")
+        return monitor.get_dft_array(component, freq_index)
 
     def get_array(self, component: str) -> numpy.ndarray:
         """One stored field component, float32 C-order (nx, ny, nz), a private host copy."""

@@ -28,22 +28,33 @@ _AX = "xyz"
 
 
 class OraclePlane:
+    """A flux plane (the four tangential components, cell-centred), or, with ``components``, a
+    DFT field region (reference DFTMonitor, dft.py:1960-2300): those components over the box,
+    cell-centred or (``yee``) the stored word at each cell's own Yee site."""
+
     def __init__(self, normal: int, lo, cnt, omegas: numpy.ndarray, scale: float,
-                 decimation: int = 1):
+                 decimation: int = 1, components=None, yee: bool = False):
         self.normal = int(normal)
         self.decimation = int(decimation)  # accumulate when step_count % decimation == 0
         self.lo = tuple(int(v) for v in lo)
         self.cnt = tuple(int(v) for v in cnt)
         self.omegas = numpy.asarray(omegas, dtype=numpy.float64)
         self.scale = float(scale)
-        t1, t2 = (self.normal + 1) % 3, (self.normal + 2) % 3
-        self.components = ("E" + _AX[t1], "E" + _AX[t2], "H" + _AX[t1], "H" + _AX[t2])
+        self.yee = bool(yee)
+        if components is None:
+            t1, t2 = (self.normal + 1) % 3, (self.normal + 2) % 3
+            self.components = ("E" + _AX[t1], "E" + _AX[t2], "H" + _AX[t1], "H" + _AX[t2])
+        else:
+            self.components = tuple(components)
         cells = int(numpy.prod(self.cnt))
         nf = self.omegas.size
-        self.acc64 = numpy.zeros((4, nf, cells), dtype=numpy.complex128)
-        self.acc32 = numpy.zeros((4, nf, cells), dtype=numpy.complex64)
+        self.acc64 = numpy.zeros((len(self.components), nf, cells), dtype=numpy.complex128)
+        self.acc32 = numpy.zeros((len(self.components), nf, cells), dtype=numpy.complex64)
 
     def sample(self, sim: oe.OracleSim, name: str, field: numpy.ndarray) -> numpy.ndarray:
+        if self.yee:  # the stored words of the box, C order (x slowest), periodic indices
+            idx = [numpy.arange(l, l + c) % n for l, c, n in zip(self.lo, self.cnt, field.shape)]
+            return numpy.ascontiguousarray(field[numpy.ix_(*idx)], dtype=numpy.float32).reshape(-1)
         g = sim._grid()
         out = numpy.empty(self.cnt, dtype=numpy.float32)
         shift = (ctypes.c_int * 3)(*[(_IYEE_MASK[name] >> a) & 1 for a in range(3)])

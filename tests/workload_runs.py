@@ -26,7 +26,9 @@ def small(name: str):
     raise KeyError(name)
 
 
-def oracle_run(wl, steps: int, reference_precision: bool = True):
+def oracle_run(wl, steps: int, reference_precision: bool = True, dft_monitors=()):
+    """The oracle's run of ``wl``: its flux planes, then one oracle region per DFT field
+    monitor in ``dft_monitors`` (after them in the returned list)."""
     grid = wl.grid
     faces = normalize_faces(wl.pml_faces, grid) if wl.pml_faces else ((0, 0),) * 3
     deposits = [(b.component, b.lo, b.cnt, b.amp_re, b.amp_im,
@@ -35,6 +37,9 @@ def oracle_run(wl, steps: int, reference_precision: bool = True):
     for p in wl.planes:
         mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
         planes.append((p.normal, p.lo, p.cnt, mon.omegas, mon.scale))
+    for m in dft_monitors:
+        planes.append((-1, m.lo, m.cnt, m.omegas, m.scale, m.decimation, m.components,
+                       m.yee_grid))
     poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in wl.poles]
     sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, wl.inv, poles, deposits, planes,
                             grid.invariant)
