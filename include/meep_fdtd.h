@@ -139,6 +139,15 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
                         const int32_t cnt[3], int32_t nf, const double *omega, double scale,
                         int32_t decimation, int32_t *plane);
 
+/* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of
+ * D (field 0, MEEP's D_conductivity) or B (field 1, B_conductivity); null or all zero removes it.
+ * Stores condfac = 1 - sigma*dt2 (float32, dt2 = float32(dt)/2) and condinv = 1/(1 + sigma*dt/2)
+ * (double, rounded once) and, under an absorber, an f_cond history ("f_cond_Dx" .. "f_cond_Bz"
+ * for fdtd_upload/fdtd_download). The curl of that component then takes MEEP's conductive forms.
+ * Replaces Fields.set_d_conductivity / set_b_conductivity (reference fields.py:775-870) and
+ * stepping._apply_conductive_update / _apply_conductive_pml_update (stepping.py:2820-2983). */
+int fdtd_set_conductivity(fdtd_sim *sim, int32_t field, int32_t axis, const float *sigma);
+
 /* A DFT field region, MEEP's add_dft_fields (DFTMonitor, reference dft.py:1960; the region's
  * block as FdtdDriver.get_dft returns it): the cells [lo, lo+cnt) of the grid, ncomp (1..4)
  * components (codes 0..5 = Ex, Ey, Ez, Hx, Hy, Hz) sampled each accumulating step at the cell

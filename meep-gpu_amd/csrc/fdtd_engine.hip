@@ -105,6 +105,11 @@ struct SimDev {
     float psig_s[kMaxPoles][3];
     const SrcDev *src;
     int src_lo[2][3], src_hi[2][3];  // per field (D, B): union of the source boxes, [lo, hi)
+    // per field (0 = D, 1 = B) and component: a conductivity's condfac and condinv words and,
+    // under an absorber, its f_cond history; null where the component is lossless
+    const float *cf[2][3], *ci[2][3];
+    float *fcond[2][3];
+    int cond;  // any conductivity
 };
 
 // ------------------------------------------------------------------------------------------
@@ -340,6 +345,55 @@ __device__ inline float curl_apply_pml(float f, float curl, float4 lsig, float4 
     return lsig.y * v;
 }
 
+// A component with a conductivity (stepping.py:2820-2841 without an absorber, :2844-2983 under
+// one; MEEP step_generic.cpp:84-210): X = curl, cf = condfac, ci = condinv. On the dsig layer's
+// cells the f_cond stage fcnd' = ((cf*fcnd) - X)*ci runs first and its increment fcnd' - fcnd,
+// formed before it is added, replaces -X in the layer's recurrences; on the dsigu layer's cells
+// alone the split field is fu' = ((cf*fu) - X)*ci; elsewhere f' = ((cf*f) - X)*ci.
+__device__ inline float curl_apply_cond(float f, float curl, float cfv, float civ, bool in_s,
+                                        bool in_u, float4 ls, float4 lu, float *fu, float *fc,
+                                        int at) {
+    if (in_s) {
+        GMEM float *gfc = gp(fc);
+        const float old = gfc[at];
+        float n = cfv * old;
+        n = n - curl;
+        n = n * civ;
+        gfc[at] = n;
+        const float inc = n - old;
+        if (in_u) {
+            GMEM float *gfu = gp(fu);
+            const float prev = gfu[at];
+            float u = ls.x * prev;
+            u = u + inc;
+            u = u * ls.y;
+            gfu[at] = u;
+            float v = lu.x * f;
+            v = v + u;
+            v = v - prev;
+            return v * lu.y;
+        }
+        float v = ls.x * f;
+        v = v + inc;
+        return v * ls.y;
+    }
+    if (in_u) {
+        GMEM float *gfu = gp(fu);
+        const float prev = gfu[at];
+        float u = cfv * prev;
+        u = u - curl;
+        u = u * civ;
+        gfu[at] = u;
+        float v = lu.x * f;
+        v = v + u;
+        v = v - prev;
+        return v * lu.y;
+    }
+    float v = cfv * f;
+    v = v - curl;
+    return v * civ;
+}
+
 __device__ inline bool plus_zero(double v) { return __double_as_longlong(v) == 0; }
 
 // Sources of one field/component deposited into the run's values (sources.py:1357,
@@ -401,7 +455,7 @@ __device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool 
 // H pair: step_B -> magnetic deposit -> update_H.
 // en[comp][axis]: E[comp] at the +1 neighbour along axis (only the pairs a curl reads).
 // ------------------------------------------------------------------------------------------
-template <int V, int C, bool PML, bool NT>
+template <int V, int C, bool PML, bool NT, bool COND>
 __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const Run &q,
                                        const VF<V> (&e)[3], const VF<V> (&en)[3][3], VF<V> b) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
@@ -411,6 +465,17 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
     for (int m = 0; m < V; ++m) {
         const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
                                    en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
+        if constexpr (COND) {
+            if (s.cf[1][C]) {
+                const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
+                const bool in_s = PML && member(s, 1, dsig, cs), in_u = PML && member(s, 1, dsigu, cu);
+                b.v[m] = curl_apply_cond(b.v[m], curl, gp(s.cf[1][C])[q.at + m],
+                                         gp(s.ci[1][C])[q.at + m], in_s, in_u,
+                                         coeff(s, 1, dsig, cs, in_s), coeff(s, 1, dsigu, cu, in_u),
+                                         s.fuB[C], s.fcond[1][C], q.at + m);
+                continue;
+            }
+        }
         if (PML) {
             const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
             const bool in_s = member(s, 1, dsig, cs), in_u = member(s, 1, dsigu, cu);
@@ -447,7 +512,7 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
     }
 }
 
-template <int V, bool PML, bool NT>
+template <int V, bool PML, bool NT, bool COND>
 __device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> e[3], en[3][3];
 #pragma unroll
@@ -463,12 +528,12 @@ __device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q)
     VF<V> b[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) b[c] = NT ? ldv_once<V>(s.B[c], q.at) : ldv<V>(s.B[c], q.at);
-    hpair_component<V, 0, PML, NT>(s, ck, q, e, en, b[0]);
-    hpair_component<V, 1, PML, NT>(s, ck, q, e, en, b[1]);
-    hpair_component<V, 2, PML, NT>(s, ck, q, e, en, b[2]);
+    hpair_component<V, 0, PML, NT, COND>(s, ck, q, e, en, b[0]);
+    hpair_component<V, 1, PML, NT, COND>(s, ck, q, e, en, b[1]);
+    hpair_component<V, 2, PML, NT, COND>(s, ck, q, e, en, b[2]);
 }
 
-template <int V, bool NT>
+template <int V, bool NT, bool COND>
 __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
@@ -483,16 +548,16 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
         const Run q = locate(s, at);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
-            hpair_run<V, true, NT>(s, ck, q);
+            hpair_run<V, true, NT, COND>(s, ck, q);
         else
-            hpair_run<V, false, NT>(s, ck, q);
+            hpair_run<V, false, NT, COND>(s, ck, q);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // D pair: step_D -> electric deposit -> update_E -> update_P.
 // ------------------------------------------------------------------------------------------
-template <int V, int C, bool PML>
+template <int V, int C, bool PML, bool COND>
 __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const Run &q,
                                         const VF<V> (&h)[3], const VF<V> (&hn)[3][3], VF<V> d) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
@@ -502,6 +567,17 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
     for (int m = 0; m < V; ++m) {
         const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
                                    hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
+        if constexpr (COND) {
+            if (s.cf[0][C]) {
+                const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
+                const bool in_s = PML && member(s, 0, dsig, cs), in_u = PML && member(s, 0, dsigu, cu);
+                d.v[m] = curl_apply_cond(d.v[m], curl, gp(s.cf[0][C])[q.at + m],
+                                         gp(s.ci[0][C])[q.at + m], in_s, in_u,
+                                         coeff(s, 0, dsig, cs, in_s), coeff(s, 0, dsigu, cu, in_u),
+                                         s.fuD[C], s.fcond[0][C], q.at + m);
+                continue;
+            }
+        }
         if (PML) {
             const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
             const bool in_s = member(s, 0, dsig, cs), in_u = member(s, 0, dsigu, cu);
@@ -654,11 +730,11 @@ __device__ inline VF<V> inv_values(const SimDev &s, int c, int at, const InvWord
     return ldv<V>(s.inv[c], at);
 }
 
-template <int V, int C, bool PML, bool NT>
+template <int V, int C, bool PML, bool NT, bool COND>
 __device__ inline void dpair_component(const SimDev &s, const Clock &ck, const Run &q,
                                        const VF<V> (&h)[3], const VF<V> (&hn)[3][3],
                                        const VF<V> &d0, const VF<V> &iv) {
-    const VF<V> d = dcurl_component<V, C, PML>(s, ck, q, h, hn, d0);
+    const VF<V> d = dcurl_component<V, C, PML, COND>(s, ck, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
     VF<V> p[kMaxPoles];
     // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
@@ -714,7 +790,7 @@ __device__ inline void load_h(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V
     hn[2][1] = ldv<V>(Hs[2], q.at + q.ym);
 }
 
-template <int V, bool PML, bool NT, int INV>
+template <int V, bool PML, bool NT, int INV, bool COND>
 __device__ inline void dpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     InvWords iw[3];
 #pragma unroll
@@ -727,12 +803,12 @@ __device__ inline void dpair_run(const SimDev &s, const Clock &ck, const Run &q)
     for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<V>(s.D[c], q.at) : ldv<V>(s.D[c], q.at);
 #pragma unroll
     for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
-    dpair_component<V, 0, PML, NT>(s, ck, q, h, hn, d[0], iv[0]);
-    dpair_component<V, 1, PML, NT>(s, ck, q, h, hn, d[1], iv[1]);
-    dpair_component<V, 2, PML, NT>(s, ck, q, h, hn, d[2], iv[2]);
+    dpair_component<V, 0, PML, NT, COND>(s, ck, q, h, hn, d[0], iv[0]);
+    dpair_component<V, 1, PML, NT, COND>(s, ck, q, h, hn, d[1], iv[1]);
+    dpair_component<V, 2, PML, NT, COND>(s, ck, q, h, hn, d[2], iv[2]);
 }
 
-template <int V, bool NT, int INV>
+template <int V, bool NT, int INV, bool COND>
 __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
@@ -748,9 +824,9 @@ __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restric
         const Run q = locate(s, at);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
-            dpair_run<V, true, NT, INV>(s, ck, q);
+            dpair_run<V, true, NT, INV, COND>(s, ck, q);
         else
-            dpair_run<V, false, NT, INV>(s, ck, q);
+            dpair_run<V, false, NT, INV, COND>(s, ck, q);
     }
 }
 
@@ -763,9 +839,9 @@ __device__ inline void dcurl_run(const SimDev &s, const Clock &ck, const Run &q)
     VF<V> d[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
-    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML>(s, ck, q, h, hn, d[0]));
-    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML>(s, ck, q, h, hn, d[1]));
-    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML>(s, ck, q, h, hn, d[2]));
+    stv<V>(s.D[0], q.at, dcurl_component<V, 0, PML, true>(s, ck, q, h, hn, d[0]));
+    stv<V>(s.D[1], q.at, dcurl_component<V, 1, PML, true>(s, ck, q, h, hn, d[1]));
+    stv<V>(s.D[2], q.at, dcurl_component<V, 2, PML, true>(s, ck, q, h, hn, d[2]));
 }
 
 template <int V>
@@ -1263,6 +1339,7 @@ struct Group {
     std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
     std::map<int, int> ntiles;
     bool any_offdiag = false, all_offdiag = false;
+    bool any_cond = false;  // some member has a conductivity: the pairs' conductive forms
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
 };
 
@@ -1277,14 +1354,28 @@ int inv_mode_of(const SimDev &h) {
     return m;
 }
 
+template <int V, bool NT, bool COND>
+void launch_dpair_c(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
+    switch (g.inv_mode) {
+        case 0: dpair_kernel<V, NT, 0, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 1: dpair_kernel<V, NT, 1, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 2: dpair_kernel<V, NT, 2, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        default: dpair_kernel<V, NT, 3, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+    }
+}
+
+// The conductive instantiations run only where a member has a conductivity: the lossless
+// pairs' code stays what it is.
 template <int V, bool NT>
 void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
-    switch (g.inv_mode) {
-        case 0: dpair_kernel<V, NT, 0><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        case 1: dpair_kernel<V, NT, 1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        case 2: dpair_kernel<V, NT, 2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        default: dpair_kernel<V, NT, 3><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-    }
+    if (g.any_cond) launch_dpair_c<V, NT, true>(g, G, tl, nt, rm, adv, st);
+    else launch_dpair_c<V, NT, false>(g, G, tl, nt, rm, adv, st);
+}
+
+template <int V, bool NT>
+void launch_hpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
+    if (g.any_cond) hpair_kernel<V, NT, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+    else hpair_kernel<V, NT, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
 }
 
 // The fused pairs' launches `adv` steps after the descriptors' last counter advance (0 for an
@@ -1297,13 +1388,13 @@ int launch_h(const Group &g, hipStream_t st, int adv = 0) {
     const int G = grid_for(sh, nt);
     const int rm = sh.xcd_remap;
     if (sh.nt) {
-        if (V == 4) hpair_kernel<4, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        else if (V == 2) hpair_kernel<2, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        else hpair_kernel<1, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        if (V == 4) launch_hpair<4, true>(g, G, tl, nt, rm, adv, st);
+        else if (V == 2) launch_hpair<2, true>(g, G, tl, nt, rm, adv, st);
+        else launch_hpair<1, true>(g, G, tl, nt, rm, adv, st);
     } else {
-        if (V == 4) hpair_kernel<4, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        else if (V == 2) hpair_kernel<2, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        else hpair_kernel<1, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        if (V == 4) launch_hpair<4, false>(g, G, tl, nt, rm, adv, st);
+        else if (V == 2) launch_hpair<2, false>(g, G, tl, nt, rm, adv, st);
+        else launch_hpair<1, false>(g, G, tl, nt, rm, adv, st);
     }
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
@@ -1430,6 +1521,7 @@ struct fdtd_sim {
             g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
         }
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
+        g.any_cond = host.cond != 0;
         g.inv_mode = inv_mode_of(host);
         return g;
     }
@@ -1906,6 +1998,54 @@ static int download_volume(fdtd_sim *s, float *host, const float *src) {
     return FDTD_OK;
 }
 
+int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *sigma) {
+    if (!s || (field != 0 && field != 1) || c < 0 || c > 2)
+        return fail(FDTD_EINVAL, "fdtd_set_conductivity: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    SimDev &h = s->host;
+    const size_t ncell = (size_t)h.ncell;
+    bool any = false;
+    for (size_t q = 0; sigma && q < ncell && !any; ++q) any = sigma[q] != 0.0f;
+    s->graph.reset();
+    if (!any) {  // lossless: the plain forms, word for word (the volumes stay allocated)
+        h.cf[field][c] = h.ci[field][c] = nullptr;
+        h.fcond[field][c] = nullptr;
+    } else {
+        // condfac = 1 - sigma*dt2 in float32, dt2 = float32(dt)*0.5 (step_generic.cpp:88);
+        // condinv = 1/(1 + (sigma*dt)*0.5) in double from the float32 word, rounded once
+        // (structure.cpp:699) -- fields.set_d_conductivity, reference fields.py:836-864
+        const float dt2 = (float)h.dt * 0.5f;
+        std::vector<float> cf(ncell), ci(ncell);
+        for (size_t q = 0; q < ncell; ++q) {
+            const float prod = sigma[q] * dt2;
+            cf[q] = 1.0f - prod;
+            const double wide = (double)sigma[q] * h.dt;
+            ci[q] = (float)(1.0 / (1.0 + wide * 0.5));
+        }
+        float *pf = const_cast<float *>(h.cf[field][c]), *pi = const_cast<float *>(h.ci[field][c]);
+        int rc = FDTD_OK;
+        if (!pf) rc = s->volume(&pf);
+        if (rc == FDTD_OK && !pi) rc = s->volume(&pi);
+        if (rc) return rc;
+        rc = upload_volume(s, pf, cf.data());
+        if (rc == FDTD_OK) rc = upload_volume(s, pi, ci.data());
+        if (rc) return rc;
+        h.cf[field][c] = pf;
+        h.ci[field][c] = pi;
+        if (h.pml && !h.fcond[field][c]) {  // f_cond history (fields.py:754-773)
+            float *fc = nullptr;
+            rc = s->volume(&fc);
+            if (rc) return rc;
+            h.fcond[field][c] = fc;
+        }
+    }
+    h.cond = 0;
+    for (int f = 0; f < 2; ++f)
+        for (int a = 0; a < 3; ++a) h.cond |= h.cf[f][a] != nullptr;
+    s->dirty = true;
+    return FDTD_OK;
+}
+
 int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     if (!s || c < 0 || c > 2 || !vol) return fail(FDTD_EINVAL, "fdtd_set_inverse_epsilon: bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
@@ -2157,6 +2297,12 @@ static float *volume_by_name(fdtd_sim *s, const char *name) {
         const int c = comp(name[5]);
         if (c < 0 || !h.pml) return nullptr;
         return name[4] == 'E' ? h.fwE[c] : name[4] == 'H' ? h.fwH[c] : nullptr;
+    }
+    if (L == 9 && !strncmp(name, "f_cond_", 7)) {
+        const int c = comp(name[8]);
+        const int f = name[7] == 'D' ? 0 : name[7] == 'B' ? 1 : -1;
+        if (c < 0 || f < 0) return nullptr;
+        return h.fcond[f][c];
     }
     int k = -1;
     char e = 0, cc = 0;
@@ -2593,6 +2739,7 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         b->sims.push_back(s);
         descs.push_back(s->host);
         b->group.any_offdiag |= s->host.offdiag != 0;
+        b->group.any_cond |= s->host.cond != 0;
     }
     b->group.all_offdiag = true;
     for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;

@@ -19,6 +19,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy
 
+from .absorber import AbsorberLayer, absorber_conductivity, layers_for
 from .dft import DftFieldsMonitor, FluxMonitor, meep_frequency_span
 from .dispersion import Susceptibility
 from .engine import HipSimulation
@@ -41,6 +42,8 @@ class FdtdDriver:
         shape = self.grid.shape
         self._inv = {c: numpy.ones(shape, dtype=numpy.float32) for c in _E}
         self._offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
+        self._conductivity: Dict[str, numpy.ndarray] = {}
+        self._absorber_layers: List[AbsorberLayer] = []
         self._poles: List[Tuple[Susceptibility, Dict[str, object]]] = []
         self._pml_spec = None
         self._pml_kwargs = {}
@@ -77,6 +80,51 @@ class FdtdDriver:
                     numpy.asarray(vol, dtype=numpy.float32), self.grid.shape))
                 if numpy.any(arr):
                     self._offdiag.setdefault(row, {})[partner] = arr
+
+    def set_conductivity(self, conductivity: Dict[str, object]):
+        """MEEP's D conductivity (``mp.Medium(D_conductivity=...)``; reference
+        ``Fields.set_d_conductivity``, fields.py:775): sigma per component "Dx".."Dz", a
+        volume or a scalar; an all-zero one is none."""
+        self._set_conductivity_side("D", conductivity)
+
+    def set_b_conductivity(self, conductivity: Dict[str, object]):
+        """MEEP's B conductivity, the magnetic half of an ``mp.Absorber`` (reference
+        ``Fields.set_b_conductivity``, fields.py:809): sigma per component "Bx".."Bz"."""
+        self._set_conductivity_side("B", conductivity)
+
+    def _set_conductivity_side(self, side: str, conductivity: Dict[str, object]):
+        self._require_unfrozen("set_conductivity")
+        for name in [n for n in self._conductivity if n[0] == side]:
+            del self._conductivity[name]
+        for name, value in (conductivity or {}).items():
+            if len(name) != 2 or name[0] != side or name[1] not in "xyz":
+                raise ValueError(f"{side} conductivity components are {side}x, {side}y, {side}z; "
+                                 f"got {name!r}")
+            # kept in double until it is added to an absorber's and rounded once at the freeze
+            vol = numpy.ascontiguousarray(numpy.broadcast_to(
+                numpy.asarray(value, dtype=numpy.float64), self.grid.shape))
+            if not numpy.all(numpy.isfinite(vol)):
+                raise ValueError(f"{name} conductivity must be finite")
+            if numpy.any(vol):
+                self._conductivity[name] = vol
+
+    def set_absorber(self, thickness: float, direction: Optional[int] = None,
+                     side: Optional[str] = None, R_asymptotic: float = 1e-15):
+        """``mp.Absorber(thickness, direction, side, R_asymptotic)``: a graded D and B
+        conductivity inside the face(s) (reference absorber.py; MEEP meepgeom.cpp:744-763,
+        :1596-1625), added to any material conductivity in double and rounded once. Every
+        resolved axis when ``direction`` is None, both sides when ``side`` is None."""
+        self._require_unfrozen("set_absorber")
+        self._absorber_layers += layers_for(thickness, direction, side, R_asymptotic,
+                                            [not self.grid.is_invariant(a) for a in range(3)])
+
+    def conductivity_volumes(self) -> Dict[str, numpy.ndarray]:
+        """The float32 sigma each conductive component is stepped with: the material's plus
+        the absorber's, summed in double and rounded once."""
+        total = {n: v.copy() for n, v in self._conductivity.items()}
+        for name, vol in absorber_conductivity(self.grid, self._absorber_layers).items():
+            total[name] = total[name] + vol if name in total else vol
+        return {n: v.astype(numpy.float32) for n, v in total.items() if numpy.any(v)}
 
     def set_inverse_epsilon(self, inv: Dict[str, numpy.ndarray]):
         self._require_unfrozen("set_inverse_epsilon")
@@ -175,6 +223,8 @@ class FdtdDriver:
         drv.grid = wl.grid
         drv._inv = dict(wl.inv)
         drv._offdiag = {}
+        drv._conductivity = {}
+        drv._absorber_layers = []
         drv._poles = list(wl.poles)
         drv._pml_spec = wl.pml_faces
         drv._pml_kwargs = {}
@@ -213,6 +263,8 @@ class FdtdDriver:
                 sim.set_offdiagonal(row, partner, vol)
         for term, sigma in self._poles:
             sim.add_pole(term, sigma)
+        for name, vol in self.conductivity_volumes().items():
+            sim.set_conductivity(name, vol)
         for box, fn in self._sources:
             table = current_table(fn, self.grid.dt, steps, box.magnetic)
             axis = "xyz".index(box.component[1])

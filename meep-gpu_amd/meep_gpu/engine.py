@@ -79,6 +79,18 @@ class HipSimulation:
         check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
                                                 fptr(dev)), "fdtd_set_inverse_epsilon")
 
+    def set_conductivity(self, component: str, volume: Optional[numpy.ndarray]):
+        """MEEP's D or B conductivity of one component ("Dx".."Bz"): sigma as a float32 volume
+        in the reference layout, or None / all zero for none (``fdtd_set_conductivity``)."""
+        if len(component) != 2 or component[0] not in "DB" or component[1] not in _AXES:
+            raise ValueError(f"a conductivity belongs to Dx..Dz or Bx..Bz, not {component!r}")
+        dev = None if volume is None else to_device_layout(
+            numpy.broadcast_to(numpy.asarray(volume, dtype=numpy.float32), self.grid.shape))
+        check(self.lib.fdtd_set_conductivity(self.handle, 0 if component[0] == "D" else 1,
+                                             _AXES.index(component[1]),
+                                             fptr(dev) if dev is not None else None),
+              "fdtd_set_conductivity")
+
     def pole_row_fraction(self) -> float:
         """Fraction of the grid's cells inside the per-row ranges where pole work runs."""
         active, rows = ctypes.c_int64(), ctypes.c_int64()
@@ -319,7 +331,7 @@ RING = 128
 def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tuple[int, int]] = (),
                     offdiagonal_volumes: int = 0,
                     planes: Sequence[Tuple[int, int]] = (), source_steps: int = 0,
-                    source_points: int = 0) -> int:
+                    source_points: int = 0, conductive_components: int = 0) -> int:
     """Device bytes a simulation allocates (an upper bound on ``HipSimulation.device_bytes``).
 
     ``driven_poles``: per pole (driven components, sigma volumes); ``planes``: per flux plane
@@ -339,6 +351,8 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
     for comps, sig in driven_poles:
         total += vol * (2 * int(comps) + int(sig))
     total += vol * int(offdiagonal_volumes)
+    # per conductive component: condfac, condinv and, under an absorber, f_cond
+    total += vol * (3 if pml else 2) * int(conductive_components)
     for cells, nf in planes:
         row = ((int(cells) + 15) // 16) * 16
         total += 8 * nf + 4 * RING * 4 * row + 8 * RING + 8 * RING * 4 * nf + 64 * nf * cells

@@ -40,9 +40,12 @@ class Case:
     pml: Any = None
     offdiagonal: bool = False
     susceptibilities: Tuple[str, ...] = ()
+    d_conductivity: bool = False
+    b_conductivity: bool = False
 
 
-#: The cases this engine steps (periodic, real storage, no symmetry/metal/conductivity).
+#: The cases this engine steps (periodic, real storage, no symmetry or metal, the conductivity
+#: everywhere: every one-rank chunk keeps it).
 CASES: Tuple[Case, ...] = (
     Case("pml_3d", (1.5, 1.25, 1.0), pml={"x": (2, 3), "y": (2.5, 2), "z": (0, 2)}),
     Case("pml_2d", (2.0, 1.5, 0.0), dimensions=2, pml={"x": (2.5, 1.5), "y": (1.75, 3)}),
@@ -52,6 +55,9 @@ CASES: Tuple[Case, ...] = (
     Case("dispersive_pml_2d", (1.5, 1.25, 0.0), dimensions=2, pml={"x": 2, "y": 2},
          susceptibilities=("lorentz", "drude")),
     Case("dispersive_3d", (1.0, 0.75, 0.75), susceptibilities=("lorentz",)),
+    Case("conductive_3d", (1.0, 0.75, 0.75), d_conductivity=True, b_conductivity=True),
+    Case("conductive_pml_3d", (1.5, 1.25, 1.25), pml=2, d_conductivity=True,
+         b_conductivity=True),
 )
 
 CASE_BY_NAME = {case.name: case for case in CASES}
@@ -69,6 +75,7 @@ class CaseState:
     inv: Dict[str, numpy.ndarray]
     offdiag: Dict[str, Dict[str, numpy.ndarray]]
     poles: List[Dict[str, Any]]          # {frequency, gamma, drude, sigma: {Ex: vol}}
+    conductivity: Dict[str, numpy.ndarray]  # sigma by component, "Dx".."Bz" (absent: none)
     volumes: Dict[str, numpy.ndarray]    # the seeded stored state, by digest name
     stores_E: bool
     pml: bool
@@ -106,6 +113,14 @@ def build_state(name: str) -> CaseState:
             "drude": kind == "drude",
             "sigma": {c: fixture.volume(fixture.seed(name, "pol", str(index), c), shape,
                                         0.0, 1.0, zero_fraction=0.3) for c in _E}})
+    conductivity: Dict[str, numpy.ndarray] = {}
+    for side, flag in (("D", case.d_conductivity), ("B", case.b_conductivity)):
+        if flag:  # array_path_oracle.py:662-670
+            for c in "xyz":
+                vol = fixture.volume(fixture.seed(name, "sigma_" + side.lower(), side + c),
+                                     shape, 0.0, 2.0, zero_fraction=0.3)
+                if numpy.any(vol):
+                    conductivity[side + c] = vol
     faces = layer.pml_faces(case.pml, case.dimensions)
     pml = any(lo > 0 or hi > 0 for lo, hi in faces)
     stores_E = pml or bool(offdiag) or bool(poles)
@@ -116,13 +131,14 @@ def build_state(name: str) -> CaseState:
         names += ["H" + c for c in "xyz"]
         names += [p + f + c for p in ("fu_",) for f in "BD" for c in "xyz"]
         names += ["f_w_" + f + c for f in "EH" for c in "xyz"]
+        names += ["f_cond_" + n for n in conductivity]
     for k, pole in enumerate(poles):
         for c in _E:
             if numpy.any(pole["sigma"][c] != 0):
                 names += [f"P[{k}].{c}", f"P_prev[{k}].{c}"]
     volumes = {n: fixture.state_words(fixture.seed(name, "state", n), shape) for n in names}
-    return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, volumes,
-                     stores_E, pml)
+    return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, conductivity,
+                     volumes, stores_E, pml)
 
 
 def oracle_sim(state: CaseState) -> OracleSim:
@@ -131,6 +147,7 @@ def oracle_sim(state: CaseState) -> OracleSim:
     sim.set_inverse_epsilon(state.inv, state.offdiag)
     for pole in state.poles:
         sim.add_pole(pole["frequency"], pole["gamma"], pole["drude"], pole["sigma"])
+    sim.set_conductivity(state.conductivity)
     if state.stores_E:
         sim.enable_field_storage()
     for n, vol in state.volumes.items():

@@ -124,6 +124,8 @@ class OracleSim:
         self.poles: List[Pole] = []
         self.deposits: List[PointDeposit] = []
         self.stores_E = False
+        self.condfac: Dict[str, numpy.ndarray] = {}  # "Dx".."Bz": MEEP's conductivity words
+        self.condinv: Dict[str, numpy.ndarray] = {}
         # coefficient vectors and membership lines [half][axis]
         self._vec = {}
         for half in (0, 1):
@@ -177,6 +179,29 @@ class OracleSim:
         self.poles.append(pole)
         self.enable_field_storage()
         return pole
+
+    def set_conductivity(self, sigma: Dict[str, numpy.ndarray]):
+        """D and B conductivities by component ("Dx".."Bz"), as fields.set_d_conductivity /
+        set_b_conductivity form their words (reference fields.py:775-870): condfac =
+        1 - sigma*dt2 in float32 with dt2 = float32(dt)*0.5, condinv = 1/(1 + (sigma*dt)*0.5)
+        in double from the float32 word, rounded once. An all-zero volume is no conductivity.
+        Under an absorber each conductive component gets an f_cond history (fields.py:754)."""
+        dt2 = numpy.float32(numpy.float32(self.dt) * numpy.float32(0.5))
+        for name, vol in sigma.items():
+            vol = numpy.ascontiguousarray(vol, dtype=numpy.float32)
+            if not numpy.any(vol):
+                continue
+            self.condfac[name] = (numpy.float32(1.0) - vol * dt2).astype(numpy.float32)
+            wide = vol.astype(numpy.float64)
+            self.condinv[name] = (1.0 / (1.0 + (wide * self.dt) * 0.5)).astype(numpy.float32)
+            if self.pml:
+                self.v["f_cond_" + name] = numpy.zeros(self.shape, dtype=numpy.float32)
+
+    def _conductive(self, field: str):
+        cf = (_FP * 3)(*[_fp(self.condfac.get(field + c)) for c in COMPONENTS])
+        ci = (_FP * 3)(*[_fp(self.condinv.get(field + c)) for c in COMPONENTS])
+        fc = self._triple("f_cond_" + field)
+        return cf, ci, fc
 
     def add_deposit(self, deposit: PointDeposit):
         self.deposits.append(deposit)
@@ -240,14 +265,15 @@ class OracleSim:
         if src is None:
             src_arrays = [self.get_E("E" + c) for c in COMPONENTS]
             src = (_FP * 3)(*[_fp(a) for a in src_arrays])
-        L.oracle_step_curl(ctypes.byref(g), 0, self._triple("B"), src, fu)
+        L.oracle_step_curl(ctypes.byref(g), 0, self._triple("B"), src, fu, *self._conductive("B"))
 
     def step_D(self):
         L = lib()
         g = self._grid()
         fu = self._triple("fu_D") if self.pml else None
         L.oracle_step_curl(ctypes.byref(g), 1, self._triple("D"),
-                           self._triple("H") if self.pml else self._triple("B"), fu)
+                           self._triple("H") if self.pml else self._triple("B"), fu,
+                           *self._conductive("D"))
 
     def update_H(self):
         if not self.pml:

@@ -17,6 +17,10 @@
  *                        _apply_pml_update :2629 and _apply_dsig_only_update :2662.
  *                        B: forward differences, half-integer coefficients;
  *                        D: MEEP's negated strides (backward differences), integer ones.
+ *                        A component with a conductivity takes MEEP's conductive forms:
+ *                        _apply_conductive_update :2820-2841 and _apply_conductive_pml_update
+ *                        :2844-2983 (condfac/condinv from fields.set_d_conductivity
+ *                        fields.py:775-870; the f_cond stage on the dsig layer's cells).
  *   oracle_update_h   <- stepping.update_H :1216, separate_h_from_b :2789,
  *                        _apply_constitutive_regions :2749, _apply_constitutive_pml :2991.
  *   oracle_update_e   <- stepping.update_E :1243, fields.displacement_minus_polarization
@@ -54,9 +58,11 @@ static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int ax
     return ((c[0] * g->n[1]) + c[1]) * g->n[2] + c[2];
 }
 
-/* One curl sub-step. family 0 = B (src = E), 1 = D (src = H). */
+/* One curl sub-step. family 0 = B (src = E), 1 = D (src = H). cf/ci/fc: per component, the
+ * conductivity's condfac, condinv and (under an absorber) f_cond volumes, or NULL (lossless). */
 void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *const src[3],
-                      float *const fu[3]) {
+                      float *const fu[3], const float *const cf[3], const float *const ci[3],
+                      float *const fc[3]) {
     const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
     const int step = family == 0 ? +1 : -1;
     const int half = family == 0 ? 1 : 0;
@@ -69,6 +75,8 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
         float *restrict F = f[c];
         const float *G1 = src[first], *G2 = src[second];
         float *U = fu ? fu[c] : 0;
+        const float *CF = cf ? cf[c] : 0, *CI = ci ? ci[c] : 0;
+        float *FC = fc ? fc[c] : 0;
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < nx; ++i) {
             for (int j = 0; j < ny; ++j) {
@@ -91,6 +99,56 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
                         curl = (-dtdx) * t;
                     } else {
                         curl = 0.0f;
+                    }
+                    if (CF) {
+                        /* X = curl, cf, ci: MEEP's eight step_curl forms reduce, per cell,
+                         * to the four below (step_generic.cpp:84-210 as the reference
+                         * restates them); the increment fcnd' - fcnd is formed first */
+                        const float cfv = CF[at], civ = CI[at];
+                        const int in_s = g->pml && g->mem[half][dsig][ic[dsig]];
+                        const int in_u = g->pml && g->mem[half][dsigu][ic[dsigu]];
+                        const float kms = g->pml ? g->kms[half][dsig][ic[dsig]] : 1.0f;
+                        const float sinv = g->pml ? g->sinv[half][dsig][ic[dsig]] : 1.0f;
+                        const float kms_u = g->pml ? g->kms[half][dsigu][ic[dsigu]] : 1.0f;
+                        const float sinv_u = g->pml ? g->sinv[half][dsigu][ic[dsigu]] : 1.0f;
+                        if (in_s) {
+                            const float old = FC[at];
+                            float n = cfv * old;
+                            n = n - curl;
+                            n = n * civ;
+                            FC[at] = n;
+                            const float inc = n - old;
+                            if (in_u) {
+                                const float prev = U[at];
+                                float u = kms * prev;
+                                u = u + inc;
+                                u = u * sinv;
+                                U[at] = u;
+                                float v = kms_u * F[at];
+                                v = v + u;
+                                v = v - prev;
+                                F[at] = v * sinv_u;
+                            } else {
+                                float v = kms * F[at];
+                                v = v + inc;
+                                F[at] = v * sinv;
+                            }
+                        } else if (in_u) {
+                            const float prev = U[at];
+                            float u = cfv * prev;
+                            u = u - curl;
+                            u = u * civ;
+                            U[at] = u;
+                            float v = kms_u * F[at];
+                            v = v + u;
+                            v = v - prev;
+                            F[at] = v * sinv_u;
+                        } else {
+                            float v = cfv * F[at];
+                            v = v - curl;
+                            F[at] = v * civ;
+                        }
+                        continue;
                     }
                     if (!g->pml) {
                         F[at] = F[at] - curl;

@@ -35,6 +35,8 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
         term = Susceptibility(pole["frequency"], pole["gamma"],
                               DRUDE if pole["drude"] else LORENTZIAN)
         sim.add_pole(term, pole["sigma"])
+    for comp, vol in state.conductivity.items():
+        sim.set_conductivity(comp, vol)
     for n, vol in state.volumes.items():
         sim.upload(n, vol)
     if not state.stores_E:

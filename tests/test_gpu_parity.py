@@ -17,7 +17,8 @@ def test_hip_reproduces_recorded_digest(name, golden_digests):
     assert digest == golden_digests["case_sha256"][name]
 
 
-@pytest.mark.parametrize("name", ["pml_3d", "dispersive_pml_2d", "offdiag_pml_3d"])
+@pytest.mark.parametrize("name", ["pml_3d", "dispersive_pml_2d", "offdiag_pml_3d",
+                                  "conductive_pml_3d"])
 def test_hip_matches_oracle_over_forty_steps(name):
     from tests import hip_cases
 

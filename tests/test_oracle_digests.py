@@ -25,3 +25,33 @@ def test_a_planted_mistake_in_the_oracle_moves_the_digest(golden_digests, monkey
 
     monkeypatch.setattr(layer, "axis_coefficients", skewed)
     assert cases.run_case("pml_3d") != golden_digests["case_sha256"]["pml_3d"]
+
+
+@pytest.mark.parametrize("mistake", ["condinv_in_float32", "condfac_without_the_half"])
+def test_a_planted_conductivity_mistake_moves_the_digests(golden_digests, monkeypatch, mistake):
+    """condinv formed in float32 (rounded twice), or condfac missing the 1/2 of sigma*dt/2,
+    moves both conductive digests. (condfac formed in double instead of float32 would not:
+    with dt = 2**-4 the fixture's products are exact either way.)"""
+    import numpy
+
+    from oracle import engine
+
+    original = engine.OracleSim.set_conductivity
+
+    def planted(self, sigma):
+        original(self, sigma)
+        for name, vol in sigma.items():
+            if name not in self.condinv:
+                continue
+            v = numpy.ascontiguousarray(vol, dtype=numpy.float32)
+            if mistake == "condinv_in_float32":
+                dt = numpy.float32(self.dt)
+                self.condinv[name] = (numpy.float32(1.0)
+                                      / (numpy.float32(1.0) + v * dt * numpy.float32(0.5)))
+            else:
+                self.condfac[name] = (numpy.float32(1.0) - v * numpy.float32(self.dt)
+                                      ).astype(numpy.float32)
+
+    monkeypatch.setattr(engine.OracleSim, "set_conductivity", planted)
+    for name in ("conductive_3d", "conductive_pml_3d"):
+        assert cases.run_case(name) != golden_digests["case_sha256"][name], (mistake, name)
