@@ -139,6 +139,15 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
                         const int32_t cnt[3], int32_t nf, const double *omega, double scale,
                         int32_t decimation, int32_t *plane);
 
+/* The boundary of one axis: kind 0 periodic (the default), 1 metallic -- a perfect electric
+ * conductor on both faces: curl operands past a wall read zero, and the samples on a wall
+ * (stored cell 0 of a component whose Yee shift there is 0: tangential D, normal B) are not
+ * stepped and are held at zero after the sub-step's sources. Replaces the reference's METALLIC
+ * ghost rule and zero_metal_D / zero_metal_B (stepping.py:140-146, :2429-2626, :3094-3183).
+ * Refused on an invariant axis, on an axis with an absorber layer and beside off-diagonal
+ * chi1inv (not built). */
+int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
+
 /* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of
  * D (field 0, MEEP's D_conductivity) or B (field 1, B_conductivity); null or all zero removes it.
  * Stores condfac = 1 - sigma*dt2 (float32, dt2 = float32(dt)/2) and condinv = 1/(1 + sigma*dt/2)

@@ -109,7 +109,8 @@ struct SimDev {
     // under an absorber, its f_cond history; null where the component is lossless
     const float *cf[2][3], *ci[2][3];
     float *fcond[2][3];
-    int cond;  // any conductivity
+    int cond;      // any conductivity
+    int metal[3];  // metallic (PEC) axes
 };
 
 // ------------------------------------------------------------------------------------------
@@ -451,11 +452,50 @@ __device__ inline float4 coeff(const SimDev &s, int half, int axis, int c, bool 
     return make_float4(l[0], l[1], l[2], l[3]);
 }
 
+// Metallic walls (stepping.py:140-146, :2429-2530, :2589-2626, :3094-3183). Past a perfect
+// conductor there is no field, so a curl operand beyond the wall reads zero; the samples ON a
+// wall -- stored cell 0 along the axis of a component whose Yee shift there is 0 (B_c along c,
+// D_c along the other two) -- are not stepped (their curl is dropped) and are held at zero after
+// the sub-step's sources.
+__device__ inline bool on_wall(const SimDev &s, const Run &q, int axis, int m) {
+    return s.metal[axis] && coord_of(q, axis, m) == 0;
+}
+
+template <int V>
+__device__ inline void zero_vf(VF<V> &x) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) x.v[m] = 0.0f;
+}
+
+// The H pair's +1 operands past the high wall.
+template <int V>
+__device__ inline void metal_up(const SimDev &s, const Run &q, VF<V> (&en)[3][3]) {
+    if (s.metal[1] && q.j == s.n[1] - 1) { zero_vf(en[0][1]); zero_vf(en[2][1]); }
+    if (s.metal[2] && q.k == s.n[2] - 1) { zero_vf(en[0][2]); zero_vf(en[1][2]); }
+    if (s.metal[0]) {
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (q.i0 + m == s.n[0] - 1) en[1][0].v[m] = en[2][0].v[m] = 0.0f;
+    }
+}
+
+// The D pair's -1 operands past the low wall.
+template <int V>
+__device__ inline void metal_down(const SimDev &s, const Run &q, VF<V> (&hn)[3][3]) {
+    if (s.metal[1] && q.j == 0) { zero_vf(hn[0][1]); zero_vf(hn[2][1]); }
+    if (s.metal[2] && q.k == 0) { zero_vf(hn[0][2]); zero_vf(hn[1][2]); }
+    if (s.metal[0]) {
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (q.i0 + m == 0) hn[1][0].v[m] = hn[2][0].v[m] = 0.0f;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // H pair: step_B -> magnetic deposit -> update_H.
 // en[comp][axis]: E[comp] at the +1 neighbour along axis (only the pairs a curl reads).
 // ------------------------------------------------------------------------------------------
-template <int V, int C, bool PML, bool NT, bool COND>
+template <int V, int C, bool PML, bool NT, bool GEN>
 __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const Run &q,
                                        const VF<V> (&e)[3], const VF<V> (&en)[3][3], VF<V> b) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
@@ -463,9 +503,10 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
-                                   en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
-        if constexpr (COND) {
+        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
+                             en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
+        if constexpr (GEN) {
+            if (on_wall(s, q, C, m)) curl = 0.0f;
             if (s.cf[1][C]) {
                 const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
                 const bool in_s = PML && member(s, 1, dsig, cs), in_u = PML && member(s, 1, dsigu, cu);
@@ -489,6 +530,11 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
         }
     }
     if (s.nsrc) deposit_run<V>(s, ck.step, 1, C, q, b);
+    if constexpr (GEN) {
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (on_wall(s, q, C, m)) b.v[m] = 0.0f;
+    }
     stv<V, NT>(s.B[C], q.at, b);
     if (PML) {
         // update_H: H accumulates on the own-axis layer's rows; everywhere else H is B's word
@@ -512,7 +558,7 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
     }
 }
 
-template <int V, bool PML, bool NT, bool COND>
+template <int V, bool PML, bool NT, bool GEN>
 __device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> e[3], en[3][3];
 #pragma unroll
@@ -523,17 +569,18 @@ __device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q)
     en[1][0] = x_up<V>(s, q, s.E[1], e[1]);
     en[2][0] = x_up<V>(s, q, s.E[2], e[2]);
     en[2][1] = ldv<V>(s.E[2], q.at + q.yp);
+    if constexpr (GEN) metal_up<V>(s, q, en);
     // every stream's load issued before the first store: the stores may alias as far as the
     // compiler knows, so loads left behind them would wait a memory round trip each
     VF<V> b[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) b[c] = NT ? ldv_once<V>(s.B[c], q.at) : ldv<V>(s.B[c], q.at);
-    hpair_component<V, 0, PML, NT, COND>(s, ck, q, e, en, b[0]);
-    hpair_component<V, 1, PML, NT, COND>(s, ck, q, e, en, b[1]);
-    hpair_component<V, 2, PML, NT, COND>(s, ck, q, e, en, b[2]);
+    hpair_component<V, 0, PML, NT, GEN>(s, ck, q, e, en, b[0]);
+    hpair_component<V, 1, PML, NT, GEN>(s, ck, q, e, en, b[1]);
+    hpair_component<V, 2, PML, NT, GEN>(s, ck, q, e, en, b[2]);
 }
 
-template <int V, bool NT, bool COND>
+template <int V, bool NT, bool GEN>
 __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
@@ -548,16 +595,16 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
         const Run q = locate(s, at);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
-            hpair_run<V, true, NT, COND>(s, ck, q);
+            hpair_run<V, true, NT, GEN>(s, ck, q);
         else
-            hpair_run<V, false, NT, COND>(s, ck, q);
+            hpair_run<V, false, NT, GEN>(s, ck, q);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // D pair: step_D -> electric deposit -> update_E -> update_P.
 // ------------------------------------------------------------------------------------------
-template <int V, int C, bool PML, bool COND>
+template <int V, int C, bool PML, bool GEN>
 __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const Run &q,
                                         const VF<V> (&h)[3], const VF<V> (&hn)[3][3], VF<V> d) {
     constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
@@ -565,9 +612,10 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
-                                   hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
-        if constexpr (COND) {
+        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
+                             hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
+        if constexpr (GEN) {
+            if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m)) curl = 0.0f;
             if (s.cf[0][C]) {
                 const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
                 const bool in_s = PML && member(s, 0, dsig, cs), in_u = PML && member(s, 0, dsigu, cu);
@@ -591,6 +639,11 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
         }
     }
     if (s.nsrc) deposit_run<V>(s, ck.step, 0, C, q, d);
+    if constexpr (GEN) {
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+            if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m)) d.v[m] = 0.0f;
+    }
     return d;
 }
 
@@ -730,11 +783,11 @@ __device__ inline VF<V> inv_values(const SimDev &s, int c, int at, const InvWord
     return ldv<V>(s.inv[c], at);
 }
 
-template <int V, int C, bool PML, bool NT, bool COND>
+template <int V, int C, bool PML, bool NT, bool GEN>
 __device__ inline void dpair_component(const SimDev &s, const Clock &ck, const Run &q,
                                        const VF<V> (&h)[3], const VF<V> (&hn)[3][3],
                                        const VF<V> &d0, const VF<V> &iv) {
-    const VF<V> d = dcurl_component<V, C, PML, COND>(s, ck, q, h, hn, d0);
+    const VF<V> d = dcurl_component<V, C, PML, GEN>(s, ck, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
     VF<V> p[kMaxPoles];
     // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
@@ -790,25 +843,26 @@ __device__ inline void load_h(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V
     hn[2][1] = ldv<V>(Hs[2], q.at + q.ym);
 }
 
-template <int V, bool PML, bool NT, int INV, bool COND>
+template <int V, bool PML, bool NT, int INV, bool GEN>
 __device__ inline void dpair_run(const SimDev &s, const Clock &ck, const Run &q) {
     InvWords iw[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV, NT>(s, c, q.at);
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
+    if constexpr (GEN) metal_down<V>(s, q, hn);
     // every stream's load issued before the first store (see hpair_run)
     VF<V> d[3], iv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<V>(s.D[c], q.at) : ldv<V>(s.D[c], q.at);
 #pragma unroll
     for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
-    dpair_component<V, 0, PML, NT, COND>(s, ck, q, h, hn, d[0], iv[0]);
-    dpair_component<V, 1, PML, NT, COND>(s, ck, q, h, hn, d[1], iv[1]);
-    dpair_component<V, 2, PML, NT, COND>(s, ck, q, h, hn, d[2], iv[2]);
+    dpair_component<V, 0, PML, NT, GEN>(s, ck, q, h, hn, d[0], iv[0]);
+    dpair_component<V, 1, PML, NT, GEN>(s, ck, q, h, hn, d[1], iv[1]);
+    dpair_component<V, 2, PML, NT, GEN>(s, ck, q, h, hn, d[2], iv[2]);
 }
 
-template <int V, bool NT, int INV, bool COND>
+template <int V, bool NT, int INV, bool GEN>
 __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
                                                          const int2 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
@@ -824,9 +878,9 @@ __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restric
         const Run q = locate(s, at);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
-            dpair_run<V, true, NT, INV, COND>(s, ck, q);
+            dpair_run<V, true, NT, INV, GEN>(s, ck, q);
         else
-            dpair_run<V, false, NT, INV, COND>(s, ck, q);
+            dpair_run<V, false, NT, INV, GEN>(s, ck, q);
     }
 }
 
@@ -1339,7 +1393,8 @@ struct Group {
     std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
     std::map<int, int> ntiles;
     bool any_offdiag = false, all_offdiag = false;
-    bool any_cond = false;  // some member has a conductivity: the pairs' conductive forms
+    bool general = false;  // some member has a conductivity or a metallic wall: the pairs'
+                           // general instantiation (the others keep the lossless periodic code)
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
 };
 
@@ -1354,27 +1409,27 @@ int inv_mode_of(const SimDev &h) {
     return m;
 }
 
-template <int V, bool NT, bool COND>
+template <int V, bool NT, bool GEN>
 void launch_dpair_c(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
     switch (g.inv_mode) {
-        case 0: dpair_kernel<V, NT, 0, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        case 1: dpair_kernel<V, NT, 1, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        case 2: dpair_kernel<V, NT, 2, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
-        default: dpair_kernel<V, NT, 3, COND><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 0: dpair_kernel<V, NT, 0, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 1: dpair_kernel<V, NT, 1, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        case 2: dpair_kernel<V, NT, 2, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
+        default: dpair_kernel<V, NT, 3, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
     }
 }
 
-// The conductive instantiations run only where a member has a conductivity: the lossless
-// pairs' code stays what it is.
+// The general instantiations run only where a member has a conductivity or a metallic wall:
+// the lossless periodic pairs' code stays what it is.
 template <int V, bool NT>
 void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
-    if (g.any_cond) launch_dpair_c<V, NT, true>(g, G, tl, nt, rm, adv, st);
+    if (g.general) launch_dpair_c<V, NT, true>(g, G, tl, nt, rm, adv, st);
     else launch_dpair_c<V, NT, false>(g, G, tl, nt, rm, adv, st);
 }
 
 template <int V, bool NT>
 void launch_hpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
-    if (g.any_cond) hpair_kernel<V, NT, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+    if (g.general) hpair_kernel<V, NT, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
     else hpair_kernel<V, NT, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
 }
 
@@ -1521,7 +1576,7 @@ struct fdtd_sim {
             g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
         }
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
-        g.any_cond = host.cond != 0;
+        g.general = host.cond != 0 || host.metal[0] || host.metal[1] || host.metal[2];
         g.inv_mode = inv_mode_of(host);
         return g;
     }
@@ -1937,6 +1992,9 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
     if (!s || half < 0 || half > 1 || axis < 0 || axis > 2 || !kms || !sinv || !kps || !member)
         return fail(FDTD_EINVAL, "fdtd_set_layer: bad argument");
     const int n = s->host.n[axis];
+    if (s->host.metal[axis])
+        for (int i = 0; i < n; ++i)
+            if (member[i]) return fail(FDTD_EINVAL, "an absorber on metallic axis %d is not built", axis);
     std::vector<float4> v(n);
     for (int q = 0; q < n; ++q) v[q] = make_float4(kms[q], sinv[q], kps[q], member[q] ? 1.f : 0.f);
     if (half == 0) s->hmember[axis].assign(member, member + n);
@@ -1998,10 +2056,33 @@ static int download_volume(fdtd_sim *s, float *host, const float *src) {
     return FDTD_OK;
 }
 
+int fdtd_set_boundary(fdtd_sim *s, int32_t axis, int32_t kind) {
+    if (!s || axis < 0 || axis > 2 || (kind != 0 && kind != 1))
+        return fail(FDTD_EINVAL, "fdtd_set_boundary: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int rc = s->pull_desc();
+    if (rc) return rc;
+    SimDev &h = s->host;
+    if (kind == 1) {
+        if (!h.present[axis]) return fail(FDTD_EINVAL, "axis %d is invariant: it has no wall", axis);
+        for (int half = 0; half < 2; ++half)
+            for (int t = 0; t < kMaxRuns; ++t)
+                if (h.mr[half][axis][2 * t] < h.mr[half][axis][2 * t + 1])
+                    return fail(FDTD_EINVAL, "an absorber on metallic axis %d is not built", axis);
+        if (h.offdiag) return fail(FDTD_EINVAL, "off-diagonal chi1inv beside a metallic wall is not built");
+    }
+    h.metal[axis] = kind;
+    s->graph.reset();
+    s->dirty = true;
+    return FDTD_OK;
+}
+
 int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *sigma) {
     if (!s || (field != 0 && field != 1) || c < 0 || c > 2)
         return fail(FDTD_EINVAL, "fdtd_set_conductivity: bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int prc = s->pull_desc();
+    if (prc) return prc;
     SimDev &h = s->host;
     const size_t ncell = (size_t)h.ncell;
     bool any = false;
@@ -2119,6 +2200,8 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
     s->graph.reset();  // the off-diagonal path launches other kernels
     int rc = s->pull_desc();
     if (rc) return rc;
+    if (s->host.metal[0] || s->host.metal[1] || s->host.metal[2])
+        return fail(FDTD_EINVAL, "off-diagonal chi1inv beside a metallic wall is not built");
     if (!s->host.u[row][partner]) {
         float *p;
         rc = s->volume(&p);
@@ -2739,7 +2822,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         b->sims.push_back(s);
         descs.push_back(s->host);
         b->group.any_offdiag |= s->host.offdiag != 0;
-        b->group.any_cond |= s->host.cond != 0;
+        b->group.general |= s->host.cond != 0 || s->host.metal[0] || s->host.metal[1] ||
+                            s->host.metal[2];
     }
     b->group.all_offdiag = true;
     for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;

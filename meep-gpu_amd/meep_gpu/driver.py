@@ -44,6 +44,7 @@ class FdtdDriver:
         self._offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
         self._conductivity: Dict[str, numpy.ndarray] = {}
         self._absorber_layers: List[AbsorberLayer] = []
+        self._boundaries: Dict[int, str] = {}
         self._poles: List[Tuple[Susceptibility, Dict[str, object]]] = []
         self._pml_spec = None
         self._pml_kwargs = {}
@@ -107,6 +108,22 @@ class FdtdDriver:
                 raise ValueError(f"{name} conductivity must be finite")
             if numpy.any(vol):
                 self._conductivity[name] = vol
+
+    def set_boundaries(self, boundaries: Dict[str, str]):
+        """Per-axis boundaries, ``{"x": "metallic", ...}`` (the reference driver's
+        ``boundaries``): "periodic" (the default) or "metallic" -- perfect electric conductor
+        walls on both faces of the axis (stepping.py:140-146). An absorber on a metallic axis
+        and off-diagonal chi1inv beside one are not built and are refused at the freeze."""
+        self._require_unfrozen("set_boundaries")
+        kinds = {}
+        for key, kind in (boundaries or {}).items():
+            axis = "xyz".index(str(key).lower())
+            if kind not in ("periodic", "metallic"):
+                raise ValueError(f"boundary {key!r} must be 'periodic' or 'metallic', got {kind!r}")
+            if kind == "metallic" and self.grid.is_invariant(axis):
+                raise ValueError(f"axis {key!r} is invariant: it has no wall")
+            kinds[axis] = kind
+        self._boundaries = kinds
 
     def set_absorber(self, thickness: float, direction: Optional[int] = None,
                      side: Optional[str] = None, R_asymptotic: float = 1e-15):
@@ -225,6 +242,7 @@ class FdtdDriver:
         drv._offdiag = {}
         drv._conductivity = {}
         drv._absorber_layers = []
+        drv._boundaries = {}
         drv._poles = list(wl.poles)
         drv._pml_spec = wl.pml_faces
         drv._pml_kwargs = {}
@@ -256,6 +274,12 @@ class FdtdDriver:
         steps = int(planned_steps or self.max_steps)
         pml = PML(self.grid, self._pml_spec, **self._pml_kwargs) if self._pml_spec else None
         sim = HipSimulation(self.grid, pml)
+        try:
+            for axis, kind in self._boundaries.items():
+                sim.set_boundary(axis, kind)
+        except Exception:
+            sim.close()
+            raise
         for c in _E:
             sim.set_inverse_epsilon(c, self._inv[c])
         for row, partners in self._offdiag.items():

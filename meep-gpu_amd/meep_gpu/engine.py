@@ -79,6 +79,14 @@ class HipSimulation:
         check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
                                                 fptr(dev)), "fdtd_set_inverse_epsilon")
 
+    def set_boundary(self, axis: int, kind: str):
+        """One axis's boundary: "periodic" or "metallic" (``fdtd_set_boundary``)."""
+        kinds = ("periodic", "metallic")
+        if kind not in kinds:
+            raise ValueError(f"boundary kind must be one of {kinds}, got {kind!r}")
+        check(self.lib.fdtd_set_boundary(self.handle, int(axis), kinds.index(kind)),
+              "fdtd_set_boundary")
+
     def set_conductivity(self, component: str, volume: Optional[numpy.ndarray]):
         """MEEP's D or B conductivity of one component ("Dx".."Bz"): sigma as a float32 volume
         in the reference layout, or None / all zero for none (``fdtd_set_conductivity``)."""

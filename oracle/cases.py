@@ -42,10 +42,11 @@ class Case:
     susceptibilities: Tuple[str, ...] = ()
     d_conductivity: bool = False
     b_conductivity: bool = False
+    metallic: Tuple[bool, bool, bool] = (False, False, False)
 
 
-#: The cases this engine steps (periodic, real storage, no symmetry or metal, the conductivity
-#: everywhere: every one-rank chunk keeps it).
+#: The cases this engine steps: real storage, no symmetry; periodic or metallic axes (no
+#: absorber on a metallic one); conductivity everywhere (every one-rank chunk keeps it).
 CASES: Tuple[Case, ...] = (
     Case("pml_3d", (1.5, 1.25, 1.0), pml={"x": (2, 3), "y": (2.5, 2), "z": (0, 2)}),
     Case("pml_2d", (2.0, 1.5, 0.0), dimensions=2, pml={"x": (2.5, 1.5), "y": (1.75, 3)}),
@@ -58,6 +59,8 @@ CASES: Tuple[Case, ...] = (
     Case("conductive_3d", (1.0, 0.75, 0.75), d_conductivity=True, b_conductivity=True),
     Case("conductive_pml_3d", (1.5, 1.25, 1.25), pml=2, d_conductivity=True,
          b_conductivity=True),
+    Case("metallic_2d", (1.5, 1.25, 0.0), dimensions=2, pml={"y": 2},
+         metallic=(True, False, False)),
 )
 
 CASE_BY_NAME = {case.name: case for case in CASES}
@@ -148,6 +151,7 @@ def oracle_sim(state: CaseState) -> OracleSim:
     for pole in state.poles:
         sim.add_pole(pole["frequency"], pole["gamma"], pole["drude"], pole["sigma"])
     sim.set_conductivity(state.conductivity)
+    sim.set_metallic(CASE_BY_NAME[state.name].metallic)
     if state.stores_E:
         sim.enable_field_storage()
     for n, vol in state.volumes.items():

@@ -33,7 +33,7 @@ class _OGrid(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 3), ("present", ctypes.c_int * 3), ("dtdx", ctypes.c_float),
                 ("pml", ctypes.c_int),
                 ("kms", (_FP * 3) * 2), ("sinv", (_FP * 3) * 2), ("kps", (_FP * 3) * 2),
-                ("mem", (_U8P * 3) * 2)]
+                ("mem", (_U8P * 3) * 2), ("metal", ctypes.c_int * 3)]
 
 
 class _OElectric(ctypes.Structure):
@@ -124,6 +124,7 @@ class OracleSim:
         self.poles: List[Pole] = []
         self.deposits: List[PointDeposit] = []
         self.stores_E = False
+        self.metal = (False, False, False)  # metallic (PEC) axes
         self.condfac: Dict[str, numpy.ndarray] = {}  # "Dx".."Bz": MEEP's conductivity words
         self.condinv: Dict[str, numpy.ndarray] = {}
         # coefficient vectors and membership lines [half][axis]
@@ -197,6 +198,25 @@ class OracleSim:
             if self.pml:
                 self.v["f_cond_" + name] = numpy.zeros(self.shape, dtype=numpy.float32)
 
+    def set_metallic(self, axes: Sequence[bool]):
+        """Metallic (PEC) walls on the given axes (stepping.py:140-146): zero ghosts past the
+        wall, the wall's own samples unstepped and held at zero."""
+        self.metal = tuple(bool(a) for a in axes)
+
+    def zero_metal(self, field: str):
+        """zero_metal_D / zero_metal_B (stepping.py:3094-3183): stored cell 0 of every
+        component whose Yee shift on a metallic axis is 0 (B_c along c, D_c along the other two)
+        is written zero, after the sub-step's sources."""
+        for a in range(3):
+            if not self.metal[a]:
+                continue
+            for c, name in enumerate(COMPONENTS):
+                on_wall = (a == c) if field == "B" else (a != c)
+                if on_wall:
+                    index = [slice(None)] * 3
+                    index[a] = 0
+                    self.v[field + name][tuple(index)] = 0.0
+
     def _conductive(self, field: str):
         cf = (_FP * 3)(*[_fp(self.condfac.get(field + c)) for c in COMPONENTS])
         ci = (_FP * 3)(*[_fp(self.condinv.get(field + c)) for c in COMPONENTS])
@@ -229,6 +249,8 @@ class OracleSim:
             g.present[a] = 0 if self.invariant[a] else 1
         g.dtdx = numpy.float32(self.dt / self.dx)
         g.pml = 1 if self.pml else 0
+        for a in range(3):
+            g.metal[a] = 1 if self.metal[a] else 0
         for half in (0, 1):
             for a in range(3):
                 kms, sinv, kps, mem = self._vec[(half, a)]
@@ -331,9 +353,11 @@ class OracleSim:
     def step(self):
         self.step_B()
         self.deposit(True, self.time)
+        self.zero_metal("B")
         self.update_H()
         self.step_D()
         self.deposit(False, self.time + 0.5 * self.dt)
+        self.zero_metal("D")
         self.update_E()
         self.update_P()
         self.step_count += 1

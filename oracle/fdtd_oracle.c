@@ -49,6 +49,7 @@ typedef struct {
     const float *sinv[2][3];
     const float *kps[2][3];
     const uint8_t *mem[2][3];  /* one-rank chunk membership lines */
+    int metal[3];          /* 1: a metallic (PEC) axis -- stepping.py:140-146, :2429-2530 */
 } ogrid;
 
 static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int axis, int step) {
@@ -56,6 +57,17 @@ static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int ax
     int n = g->n[axis];
     c[axis] = (c[axis] + step + n) % n;
     return ((c[0] * g->n[1]) + c[1]) * g->n[2] + c[2];
+}
+
+/* A curl operand one cell along `axis` (step = +1 forward, -1 backward): the wrapped neighbour,
+ * or, past the wall of a metallic axis, zero -- no field beyond a perfect conductor
+ * (_shift_up / _shift_down, stepping.py:2429-2530). */
+static inline float rd(const ogrid *g, const float *G, int64_t i, int64_t j, int64_t k, int axis,
+                       int step) {
+    const int64_t c[3] = {i, j, k};
+    const int64_t m = c[axis] + step;
+    if (g->metal[axis] && (m < 0 || m >= g->n[axis])) return 0.0f;
+    return G[nb(g, i, j, k, axis, step)];
 }
 
 /* One curl sub-step. family 0 = B (src = E), 1 = D (src = H). cf/ci/fc: per component, the
@@ -85,20 +97,27 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
                     const int ic[3] = {i, j, k};
                     float curl;
                     if (p1 && p2) {
-                        float g1s = G1[nb(g, i, j, k, first_axis, step)], g1 = G1[at];
-                        float g2 = G2[at], g2s = G2[nb(g, i, j, k, second_axis, step)];
+                        float g1s = rd(g, G1, i, j, k, first_axis, step), g1 = G1[at];
+                        float g2 = G2[at], g2s = rd(g, G2, i, j, k, second_axis, step);
                         float t = g1s - g1;
                         t = t + g2;
                         t = t - g2s;
                         curl = dtdx * t;
                     } else if (p1) {
-                        float t = G1[nb(g, i, j, k, first_axis, step)] - G1[at];
+                        float t = rd(g, G1, i, j, k, first_axis, step) - G1[at];
                         curl = dtdx * t;
                     } else if (p2) {
-                        float t = G2[nb(g, i, j, k, second_axis, step)] - G2[at];
+                        float t = rd(g, G2, i, j, k, second_axis, step) - G2[at];
                         curl = (-dtdx) * t;
                     } else {
                         curl = 0.0f;
+                    }
+                    /* a component sitting ON a metallic wall (Yee shift 0 on that axis: B_c
+                     * along c, D_c along the other two) at stored cell 0 is not stepped: its
+                     * curl is dropped (_mask_non_owned_cells, stepping.py:2589-2626) */
+                    for (int a = 0; a < 3; ++a) {
+                        const int shift0 = family == 0 ? (a == c) : (a != c);
+                        if (g->metal[a] && shift0 && ic[a] == 0) curl = 0.0f;
                     }
                     if (CF) {
                         /* X = curl, cf, ci: MEEP's eight step_curl forms reduce, per cell,

@@ -37,6 +37,9 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
         sim.add_pole(term, pole["sigma"])
     for comp, vol in state.conductivity.items():
         sim.set_conductivity(comp, vol)
+    for axis, metal in enumerate(case.metallic):
+        if metal:
+            sim.set_boundary(axis, "metallic")
     for n, vol in state.volumes.items():
         sim.upload(n, vol)
     if not state.stores_E:

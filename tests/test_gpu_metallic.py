@@ -1,0 +1,75 @@
+"""Metallic (PEC) walls on the GPU against the oracle: a 3-D sphere run with walls on x and an
+absorber on y and z (fields word for word, spectra to 1e-9 of their peak), and the refusals
+the engine makes by name."""
+
+import numpy
+import pytest
+
+from tests import workload_runs as wr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["sphere", "slab"])
+def test_metallic_walls_match_the_oracle(name):
+    from meep_gpu.dft import FluxMonitor
+    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.pml import normalize_faces
+    from meep_gpu.sources import current_table
+    from oracle import workload as ow
+
+    wl = wr.small(name)
+    cells = 6
+    wl.pml_faces = {"y": cells, "z": cells}
+    steps = 160
+    drv = FdtdDriver.from_workload(wl)
+    drv.set_boundaries({"x": "metallic"})
+    print(f"{name} with metallic x walls", flush=True)
+    drv.freeze(planned_steps=steps + 1)
+    drv.run(steps=steps)
+    drv.synchronize()
+
+    grid = wl.grid
+    deposits = [(b.component, b.lo, b.cnt, b.amp_re, b.amp_im,
+                 current_table(fn, grid.dt, steps + 1, b.magnetic)) for b, fn in wl.sources]
+    planes = []
+    for p in wl.planes:
+        mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
+        planes.append((p.normal, p.lo, p.cnt, mon.omegas, mon.scale))
+    poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in wl.poles]
+    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, normalize_faces(wl.pml_faces, grid),
+                            wl.inv, poles, deposits, planes, grid.invariant)
+    sim.set_metallic((True, False, False))
+    ow.run(sim, oplanes, steps, reference_precision=False)
+    vols = sim.stored_volumes()
+    for comp in ("Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Bx", "By", "Bz", "Hx", "Hy", "Hz"):
+        got = drv.sim.download(comp)
+        want = vols[comp]
+        assert got.view(numpy.uint32).tobytes() == want.view(numpy.uint32).tobytes(), comp
+    # the walls hold: tangential D and normal B at x = 0
+    assert not numpy.any(drv.sim.download("Dy")[0]) and not numpy.any(drv.sim.download("Bx")[0])
+    for mon, plane in zip(drv.flux_monitors, oplanes):
+        got = mon.get_flux_spectrum()
+        ref = plane.flux(grid.dx, "c128")
+        peak = numpy.max(numpy.abs(ref))
+        assert peak > 0 and numpy.max(numpy.abs(got - ref)) <= 1e-9 * peak
+    drv.close()
+
+
+def test_what_metallic_walls_refuse():
+    from meep_gpu._native import EngineError
+    from meep_gpu.driver import FdtdDriver
+
+    wl = wr.small("sphere")  # an absorber on every face
+    drv = FdtdDriver.from_workload(wl)
+    drv.set_boundaries({"x": "metallic"})
+    with pytest.raises(EngineError, match="absorber on metallic axis 0"):
+        drv.freeze(planned_steps=4)
+    wl = wr.small("sphere")
+    wl.pml_faces = {"y": 6, "z": 6}
+    drv = FdtdDriver.from_workload(wl)
+    drv.set_boundaries({"x": "metallic"})
+    drv.set_epsilon_components({c: 2.0 for c in ("Ex", "Ey", "Ez")},
+                               chi1inv_offdiagonal={"Ex": {"Ey": 0.01}})
+    with pytest.raises(EngineError, match="off-diagonal"):
+        drv.freeze(planned_steps=4)
