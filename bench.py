@@ -326,7 +326,8 @@ def main(argv=None) -> int:
     dshape = shape[1]
     widths = set(sim.inverse_epsilon_bytes())
     inv_mode = {1: 1, 2: 2, 4: 0}[widths.pop()] if len(widths) == 1 else 3
-    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'},{inv_mode}>"
+    # the lossless periodic instantiation (the workload has no conductivity and no metallic wall)
+    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'},{inv_mode},false>"
     traffic = measured_traffic(dkernel)
     spectra_ok = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
     line = {
