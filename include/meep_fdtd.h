@@ -218,9 +218,10 @@ int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
  * launches each sub-step). */
 int fdtd_set_graphs(int32_t enable);
 
-/* mfma = 1 (the default): the float64 flux transform runs as 16x16x4 float64 MFMA products
- * (phase words times ring samples); 0: the vector-FMA kernel. The sums associate differently
- * (both are float64; they agree to rounding). Process-wide. No counterpart in the reference. */
+/* mfma = 1 (the default), 2 or 4: the float64 flux transform runs as 16x16x4 float64 MFMA
+ * products (phase words times ring samples), each wave taking that many 16-cell tiles; 0: the
+ * vector-FMA kernel. The sums associate differently (both are float64; they agree to rounding).
+ * Process-wide. No counterpart in the reference. */
 int fdtd_set_flux_transform(int32_t mfma);
 
 /* Measure average kernel durations over `repeats` launches of each fused pair on the

@@ -1209,49 +1209,78 @@ __global__ void __launch_bounds__(256) flux_accumulate_kernel(
 // The same product on the matrix cores: per component, C[n][cell] += sum_m P[n][m] R[m][cell]
 // with n = 2f + (re, im), P the phase words and R the ring's samples widened to double, in
 // 16x16x4 float64 MFMA steps. V_MFMA_F64_16X16X4F64 on a wave: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16] and holds D[l/16 + 4r][l%16] in accumulator r. Each wave owns 16 cells and
-// kMT tiles of 16 n, so one ring word feeds kMT MFMAs; the ring rows are padded to 16 cells.
+// and B[l/16][l%16] and holds D[l/16 + 4r][l%16] in accumulator r. Each wave owns CT tiles of
+// 16 cells and kMT tiles of 16 n, so each phase word it loads feeds CT MFMAs and each ring word
+// kMT; tiles wholly past the frequencies or the cells are skipped (with 100 frequencies, 3 of
+// the 16 tiles a wave range covers). The ring rows are padded to 16 cells.
 typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int kMT = 4;  // 16-row n tiles per wave
 
+// CT: 16-cell tiles per wave (each operand load feeds kMT or CT products). Measured on the
+// 192^3 sphere's planes: one tile per wave, 129 us a pass; two, 183 us; four, 296 us (fewer
+// waves resident: 8, 3 and 1 per SIMD).
+template <int CT>
 __global__ void __launch_bounds__(256) flux_accumulate_mfma_kernel(
     const float *__restrict__ ring, const double *__restrict__ phase, double *__restrict__ acc,
     int cells, int row, int nf, int nslots, int nslot, int hmask) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int comp = blockIdx.y;
     const int side = (hmask >> comp) & 1;
-    const int cell0 = (blockIdx.x * 4 + wave) * 16;
+    const int cell0 = (blockIdx.x * 4 + wave) * 16 * CT;
     if (cell0 >= cells) return;
     const int nn = 2 * nf;
     const int n0 = blockIdx.z * 16 * kMT;
-    const int cell = cell0 + (lane & 15);
     const int kk = lane >> 4;      // this lane's k (slot) within a 4-slot step
-    const int ni = lane & 15;      // this lane's n within a tile (A operand row)
-    d4v c[kMT];
-#pragma unroll
-    for (int t = 0; t < kMT; ++t) c[t] = (d4v){0.0, 0.0, 0.0, 0.0};
-    for (int m0 = 0; m0 < nslots; m0 += 4) {
-        const int m = m0 + kk;
-        double b = 0.0;
-        if (m < nslots) b = (double)ring[((size_t)m * nslot + comp) * row + cell];  // row >= cell0+16
-#pragma unroll
-        for (int t = 0; t < kMT; ++t) {
-            const int n = n0 + t * 16 + ni;
-            double a = 0.0;
-            if (m < nslots && n < nn) a = phase[(size_t)m * 4 * nf + side * nn + n];
-            c[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c[t], 0, 0, 0);
-        }
-    }
-    if (cell >= cells) return;
-    double *a = acc + (size_t)comp * nf * cells * 2;
+    const int ni = lane & 15;      // this lane's n within a tile (A row) and cell (B column)
+    // the tiles this wave multiplies: wave-uniform counts, so every MFMA runs on a full wave
+    int mt = (nn - n0 + 15) / 16, ct = (cells - cell0 + 15) / 16;
+    mt = mt < kMT ? mt : kMT;
+    ct = ct < CT ? ct : CT;
+    d4v c[kMT][CT];
 #pragma unroll
     for (int t = 0; t < kMT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int n = n0 + t * 16 + kk + 4 * r;
-            if (n >= nn) continue;
-            double *w = a + ((size_t)(n >> 1) * cells + cell) * 2 + (n & 1);
-            *w += c[t][r];
+        for (int u = 0; u < CT; ++u) c[t][u] = (d4v){0.0, 0.0, 0.0, 0.0};
+    const GMEM float *rg = gp(ring) + (size_t)comp * row;
+    const GMEM double *ph = gp(phase) + (size_t)side * nn;
+    for (int m0 = 0; m0 < nslots; m0 += 4) {
+        const int m = m0 + kk;
+        const bool live = m < nslots;
+        double b[CT], av[kMT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int cell = cell0 + u * 16 + ni;  // < row (a multiple of 16) for u < ct
+            b[u] = (live && u < ct) ? (double)rg[(size_t)m * nslot * row + cell] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kMT; ++t) {
+            const int n = n0 + t * 16 + ni;
+            av[t] = (live && n < nn) ? ph[(size_t)m * 4 * nf + n] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kMT; ++t) {
+            if (t >= mt) break;
+#pragma unroll
+            for (int u = 0; u < CT; ++u) {
+                if (u >= ct) break;
+                c[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], b[u], c[t][u], 0, 0, 0);
+            }
+        }
+    }
+    GMEM double *out = gp(acc) + (size_t)comp * nf * cells * 2;
+#pragma unroll
+    for (int t = 0; t < kMT; ++t)
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int cell = cell0 + u * 16 + ni;
+            if (t >= mt || u >= ct || cell >= cells) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int n = n0 + t * 16 + kk + 4 * r;
+                if (n >= nn) continue;
+                GMEM double *w = out + ((size_t)(n >> 1) * cells + cell) * 2 + (n & 1);
+                *w += c[t][u][r];
+            }
         }
 }
 
@@ -1496,10 +1525,15 @@ int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
 // replayed while the launch shape and every plane's ring
 // phase are what they were at capture. Plane rings fill once per kRing steps when every plane
 // samples each step, so one capture serves every later period.
+int g_graphs = 1;  // replay captured step sequences where eligible
+int g_flux_mfma = 1;  // the flux transform on the matrix cores with this many 16-cell tiles per
+                      // wave (1, 2 or 4), or 0: the vector FMA kernel
+
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
     std::vector<int> phase;
     LaunchShape shape[2];
+    int flux_mode = -1;  // the transform kernel the sequence was captured with
     void reset() {
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
@@ -1507,13 +1541,10 @@ struct StepGraph {
     }
     bool matches(const std::vector<int> &ph) const {
         return exec && ph == phase && same_shape(shape[0], g_shape[0]) &&
-               same_shape(shape[1], g_shape[1]);
+               same_shape(shape[1], g_shape[1]) && flux_mode == g_flux_mfma;
     }
     ~StepGraph() { reset(); }
 };
-
-int g_graphs = 1;  // replay captured step sequences where eligible
-int g_flux_mfma = 1;  // the flux transform on the matrix cores (else the vector FMA kernel)
 }  // namespace
 
 struct fdtd_sim {
@@ -1607,9 +1638,19 @@ int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
     const PlaneDev &p = ph.dev;
     flux_phase_kernel<<<dim3((ph.filled * p.nf + 255) / 256, 1), 256, 0, st>>>(dev_plane, ph.filled);
     if (g_flux_mfma) {
-        flux_accumulate_mfma_kernel<<<dim3((p.cells + 63) / 64, p.nslot, (2 * p.nf + 16 * kMT - 1) / (16 * kMT)),
-                                      256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled,
-                                                    p.nslot, p.hmask);
+        const int ct = g_flux_mfma;  // 16-cell tiles per wave: 1, 2 or 4
+        const int per_block = 4 * 16 * ct;
+        const dim3 grid((p.cells + per_block - 1) / per_block, p.nslot,
+                        (2 * p.nf + 16 * kMT - 1) / (16 * kMT));
+        if (ct == 4)
+            flux_accumulate_mfma_kernel<4><<<grid, 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row,
+                                                                 p.nf, ph.filled, p.nslot, p.hmask);
+        else if (ct == 2)
+            flux_accumulate_mfma_kernel<2><<<grid, 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row,
+                                                                 p.nf, ph.filled, p.nslot, p.hmask);
+        else
+            flux_accumulate_mfma_kernel<1><<<grid, 256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row,
+                                                                 p.nf, ph.filled, p.nslot, p.hmask);
     } else {
         const int per_block = 256 * kCB;
         flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, p.nslot, (p.nf + kFZ - 1) / kFZ),
@@ -1784,6 +1825,7 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
     k.graph->phase = phase;
     k.graph->shape[0] = g_shape[0];
     k.graph->shape[1] = g_shape[1];
+    k.graph->flux_mode = g_flux_mfma;
     HIP_TRY(hipGraphLaunch(exec, k.st));
     return FDTD_OK;
 }
@@ -2509,7 +2551,9 @@ int fdtd_run(fdtd_sim *s, int32_t nsteps) {
 }
 
 int fdtd_set_flux_transform(int32_t mfma) {
-    g_flux_mfma = mfma ? 1 : 0;
+    if (mfma != 0 && mfma != 1 && mfma != 2 && mfma != 4)
+        return fail(FDTD_EINVAL, "flux transform: 0 (vector), or 1, 2, 4 (matrix cores), got %d", mfma);
+    g_flux_mfma = mfma;
     return FDTD_OK;
 }
 

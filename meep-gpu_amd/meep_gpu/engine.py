@@ -314,9 +314,11 @@ def get_launch_pair(pair: int) -> Tuple[int, int, int, bool]:
     return int(out[0]), int(out[1]), int(out[2]), bool(out[3])
 
 
-def set_flux_transform(mfma: bool):
-    """Run the float64 flux transform on the matrix cores (default) or as vector FMAs."""
-    check(_native.load().fdtd_set_flux_transform(1 if mfma else 0), "fdtd_set_flux_transform")
+def set_flux_transform(mfma: bool, cell_tiles: int = 1):
+    """Run the float64 flux transform on the matrix cores (default; ``cell_tiles`` 16-cell tiles
+    per wave: 1, 2 or 4) or as vector FMAs."""
+    check(_native.load().fdtd_set_flux_transform(int(cell_tiles) if mfma else 0),
+          "fdtd_set_flux_transform")
 
 
 def set_graphs(enable: bool):
