@@ -31,7 +31,9 @@ enum {
     FDTD_ENOMEM = 4   /* device allocation failed */
 };
 
-#define FDTD_MAX_POLES 4
+/* Lorentz/Drude poles one simulation holds (up to 4 step in the lossless kernels; more select
+ * the general ones, as a conductivity or a metallic wall does). */
+#define FDTD_MAX_POLES 8
 
 typedef struct fdtd_sim fdtd_sim;
 typedef struct fdtd_batch fdtd_batch;
@@ -63,9 +65,12 @@ void fdtd_sim_destroy(fdtd_sim *sim);
 /* Bytes of device memory the simulation holds. */
 int fdtd_sim_bytes(const fdtd_sim *sim, uint64_t *bytes);
 
-/* One axis of the UPML table and its one-rank chunk membership line: half 0 = integer
- * lattice, 1 = half-integer; vectors of length n[axis]. Replaces PML._compute_coefficients
- * (pml.py:673) and RegionPlan membership (chunk_map.py:180, :1093). */
+/* One axis of the UPML table and its chunk membership line: half 0 = integer lattice,
+ * 1 = half-integer; vectors of length n[axis]. The line is whatever the chunk division says
+ * (the one-rank rule, or a lifted division's ChunkLayout.membership; at most 2 runs of
+ * member rows); on a metallic axis the unowned wall row reads 0. Replaces
+ * PML._compute_coefficients (pml.py:673) and RegionPlan membership (chunk_map.py:180, :534,
+ * :1093). */
 int fdtd_set_layer(fdtd_sim *sim, int32_t half, int32_t axis, const float *kms,
                    const float *sinv, const float *kps, const uint8_t *member);
 
@@ -144,8 +149,9 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
  * (stored cell 0 of a component whose Yee shift there is 0: tangential D, normal B) are not
  * stepped and are held at zero after the sub-step's sources. Replaces the reference's METALLIC
  * ghost rule and zero_metal_D / zero_metal_B (stepping.py:140-146, :2429-2626, :3094-3183).
- * Refused on an invariant axis, on an axis with an absorber layer and beside off-diagonal
- * chi1inv (not built). */
+ * A layer on the axis (MEEP's PML inside metallic walls) steps its forms with the wall cells'
+ * curl masked to zero (stepping.py:2589-2626). Refused on an invariant axis and beside
+ * off-diagonal chi1inv (not built). */
 int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
 
 /* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of

@@ -37,6 +37,8 @@ namespace {
 constexpr int kThreads = 256;   // threads per workgroup
 constexpr int kPad = 4;         // row pitch multiple (floats)
 constexpr int kMaxPoles = FDTD_MAX_POLES;
+constexpr int kLosslessPoles = 4;  // poles the lossless periodic instantiations step; a
+                                   // simulation with more runs the general ones
 constexpr int kRing = 128;      // flux samples buffered before one float64 transform pass
 constexpr int kMaxRuns = 2;     // runs of a layer membership line held in the descriptor
 
@@ -649,9 +651,9 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
 
 // E from the constitutive product `cst`, then every pole's P update for component C. The
 // drive W is f_w on the own-axis layer's cells and E elsewhere: the constitutive product both.
-template <int V, int C, bool PML, bool NT>
+template <int V, int C, bool PML, bool NT, int NP>
 __device__ inline void e_and_p(const SimDev &s, const Clock &ck, const Run &q, const VF<V> &cst,
-                               const VF<V> (&p)[kMaxPoles], bool active) {
+                               const VF<V> (&p)[NP], bool active) {
     VF<V> e;
     if (PML) {
 #pragma unroll
@@ -675,7 +677,7 @@ __device__ inline void e_and_p(const SimDev &s, const Clock &ck, const Run &q, c
     if (!active) return;  // no polarization on this row, now or later: P stays +0
     const int par = ck.ppar;
 #pragma unroll
-    for (int pq = 0; pq < kMaxPoles; ++pq) {
+    for (int pq = 0; pq < NP; ++pq) {
         if (pq >= s.npol) break;
         if (!s.pdriven[pq][C]) continue;
         float *out = s.pbuf[pq][C][par ^ 1];
@@ -710,13 +712,13 @@ __device__ inline bool pole_run_active(const SimDev &s, const Run &q, int V) {
     return q.i0 < r[1] && q.i0 + V > r[0];
 }
 
-template <int V, int C>
+template <int V, int C, int NP>
 __device__ inline VF<V> minus_poles(const SimDev &s, const Clock &ck, const Run &q,
-                                    const VF<V> &d, VF<V> (&p)[kMaxPoles], bool active) {
+                                    const VF<V> &d, VF<V> (&p)[NP], bool active) {
     VF<V> dmp = d;
     const int par = ck.ppar;
 #pragma unroll
-    for (int pq = 0; pq < kMaxPoles; ++pq) {
+    for (int pq = 0; pq < NP; ++pq) {
 #pragma unroll
         for (int m = 0; m < V; ++m) p[pq].v[m] = 0.0f;
         if (pq >= s.npol || !active) continue;
@@ -789,16 +791,17 @@ __device__ inline void dpair_component(const SimDev &s, const Clock &ck, const R
                                        const VF<V> &d0, const VF<V> &iv) {
     const VF<V> d = dcurl_component<V, C, PML, GEN>(s, ck, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
-    VF<V> p[kMaxPoles];
+    constexpr int NP = GEN ? kMaxPoles : kLosslessPoles;
+    VF<V> p[NP];
     // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
     // is g1inv*((A*0 - g1*0) + od*(0*w)) = +0), and D - (+0) is D: runs with no such cell
     // skip the pole work (a run straddling the range updates its +0 cells to +0)
     const bool active = pole_run_active(s, q, V);
-    const VF<V> dmp = minus_poles<V, C>(s, ck, q, d, p, active);
+    const VF<V> dmp = minus_poles<V, C, NP>(s, ck, q, d, p, active);
     VF<V> cst;
 #pragma unroll
     for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
-    e_and_p<V, C, PML, NT>(s, ck, q, cst, p, active);
+    e_and_p<V, C, PML, NT, NP>(s, ck, q, cst, p, active);
 }
 
 // H component c at the run shifted by `off` (a shift across c's own axis): the stored H on
@@ -945,7 +948,7 @@ template <int V, int C, bool PML>
 __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> p[kMaxPoles];
     const bool active = pole_run_active(s, q, V);
-    const VF<V> dmp = minus_poles<V, C>(s, ck, q, ldv<V>(s.D[C], q.at), p, active);
+    const VF<V> dmp = minus_poles<V, C, kMaxPoles>(s, ck, q, ldv<V>(s.D[C], q.at), p, active);
     const VF<V> iv = ldv<V>(s.inv[C], q.at);
     VF<V> cst;
     const float *u1 = s.u[C][(C + 1) % 3], *u2 = s.u[C][(C + 2) % 3];
@@ -977,7 +980,7 @@ __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Ru
         }
         cst.v[m] = (float)row;
     }
-    e_and_p<V, C, PML, false>(s, ck, q, cst, p, active);
+    e_and_p<V, C, PML, false, kMaxPoles>(s, ck, q, cst, p, active);
 }
 
 template <int V>
@@ -1422,8 +1425,9 @@ struct Group {
     std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
     std::map<int, int> ntiles;
     bool any_offdiag = false, all_offdiag = false;
-    bool general = false;  // some member has a conductivity or a metallic wall: the pairs'
-                           // general instantiation (the others keep the lossless periodic code)
+    bool general = false;  // some member has a conductivity, a metallic wall or more than
+                           // kLosslessPoles poles: the pairs' general instantiation (the others
+                           // keep the lossless periodic code)
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
 };
 
@@ -1448,8 +1452,8 @@ void launch_dpair_c(const Group &g, int G, const int2 *tl, int nt, int rm, int a
     }
 }
 
-// The general instantiations run only where a member has a conductivity or a metallic wall:
-// the lossless periodic pairs' code stays what it is.
+// The general instantiations run only where a member has a conductivity, a metallic wall or
+// more than kLosslessPoles poles: the lossless periodic pairs' code stays what it is.
 template <int V, bool NT>
 void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
     if (g.general) launch_dpair_c<V, NT, true>(g, G, tl, nt, rm, adv, st);
@@ -1607,7 +1611,8 @@ struct fdtd_sim {
             g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
         }
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
-        g.general = host.cond != 0 || host.metal[0] || host.metal[1] || host.metal[2];
+        g.general = host.cond != 0 || host.metal[0] || host.metal[1] || host.metal[2] ||
+                    host.npol > kLosslessPoles;
         g.inv_mode = inv_mode_of(host);
         return g;
     }
@@ -2034,9 +2039,6 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
     if (!s || half < 0 || half > 1 || axis < 0 || axis > 2 || !kms || !sinv || !kps || !member)
         return fail(FDTD_EINVAL, "fdtd_set_layer: bad argument");
     const int n = s->host.n[axis];
-    if (s->host.metal[axis])
-        for (int i = 0; i < n; ++i)
-            if (member[i]) return fail(FDTD_EINVAL, "an absorber on metallic axis %d is not built", axis);
     std::vector<float4> v(n);
     for (int q = 0; q < n; ++q) v[q] = make_float4(kms[q], sinv[q], kps[q], member[q] ? 1.f : 0.f);
     if (half == 0) s->hmember[axis].assign(member, member + n);
@@ -2107,10 +2109,6 @@ int fdtd_set_boundary(fdtd_sim *s, int32_t axis, int32_t kind) {
     SimDev &h = s->host;
     if (kind == 1) {
         if (!h.present[axis]) return fail(FDTD_EINVAL, "axis %d is invariant: it has no wall", axis);
-        for (int half = 0; half < 2; ++half)
-            for (int t = 0; t < kMaxRuns; ++t)
-                if (h.mr[half][axis][2 * t] < h.mr[half][axis][2 * t + 1])
-                    return fail(FDTD_EINVAL, "an absorber on metallic axis %d is not built", axis);
         if (h.offdiag) return fail(FDTD_EINVAL, "off-diagonal chi1inv beside a metallic wall is not built");
     }
     h.metal[axis] = kind;
@@ -2867,7 +2865,7 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         descs.push_back(s->host);
         b->group.any_offdiag |= s->host.offdiag != 0;
         b->group.general |= s->host.cond != 0 || s->host.metal[0] || s->host.metal[1] ||
-                            s->host.metal[2];
+                            s->host.metal[2] || s->host.npol > kLosslessPoles;
     }
     b->group.all_offdiag = true;
     for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;

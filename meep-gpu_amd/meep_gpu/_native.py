@@ -19,7 +19,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int32)
 _U8P = ctypes.POINTER(ctypes.c_uint8)
 
-FDTD_MAX_POLES = 4
+FDTD_MAX_POLES = 8
 
 #: Every entry point include/meep_fdtd.h declares, with its ctypes signature.
 SIGNATURES = {

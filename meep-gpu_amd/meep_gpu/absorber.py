@@ -17,7 +17,8 @@ The rule, as the reference restates MEEP (absorber.py:1-316; meepgeom.cpp:744-76
   ``prof[i+1]``, clamped to ``prof[N]`` once ``i >= N``; the layers' contributions add;
 * each D and B component is sampled at its own Yee position, formed in float64 from one
   doubled integer ``(-n + 2 i + shift) * dx/2``; on a wrapping axis an integer-lattice
-  component's index 0 is MEEP's seam copy of index n, and takes that coordinate;
+  component's index 0 is MEEP's seam copy of index n, and takes that coordinate; a metallic
+  axis does not wrap, and its index 0 is the wall at ``-L/2`` (absorber.py:221-246);
 * the sigma stays float64 until it is added to any material conductivity, then is rounded
   to float32 once.
 """
@@ -107,20 +108,22 @@ def sigma_along_axis(coordinates: numpy.ndarray, layer: AbsorberLayer, half_leng
     return numpy.where(inside, numpy.where(saturated, profile[count], interp), 0.0)
 
 
-def component_coordinates(n: int, dx: float, shift: int) -> numpy.ndarray:
-    """Coordinates of one Yee sub-lattice along a wrapping axis of n cells centred on 0: the
-    doubled integer -n + 2 i + shift times dx/2, with index 0 of the integer lattice (shift 0)
-    taking the seam copy's index n."""
+def component_coordinates(n: int, dx: float, shift: int, wraps: bool = True) -> numpy.ndarray:
+    """Coordinates of one Yee sub-lattice along an axis of n cells centred on 0: the doubled
+    integer -n + 2 i + shift times dx/2, with index 0 of the integer lattice (shift 0) taking
+    the seam copy's index n on a wrapping axis."""
     index = numpy.arange(n, dtype=numpy.int64)
-    if shift == 0:
+    if shift == 0 and wraps:
         index[0] = n
     return (-n + 2 * index + shift) * (0.5 * float(dx))
 
 
-def absorber_conductivity(grid, layers: Sequence[AbsorberLayer]) -> Dict[str, numpy.ndarray]:
+def absorber_conductivity(grid, layers: Sequence[AbsorberLayer],
+                          wraps: Sequence[bool] = (True, True, True)) -> Dict[str, numpy.ndarray]:
     """The six float64 sigma volumes ("Dx".."Bz", C order (nx, ny, nz)) of a set of layers;
     empty for none. Sigma is a sum over the layers of a function of one axis's coordinate, so
-    each volume is the broadcast sum of per-axis lines at the component's own sub-lattice."""
+    each volume is the broadcast sum of per-axis lines at the component's own sub-lattice.
+    ``wraps`` is False on a metallic axis."""
     if not layers:
         return {}
     resolution = 1.0 / float(grid.dx)
@@ -131,8 +134,9 @@ def absorber_conductivity(grid, layers: Sequence[AbsorberLayer]) -> Dict[str, nu
         n = grid.shape[layer.axis]
         half_length = 0.5 * float(grid.cell_size[layer.axis])
         for shift in (0, 1):
-            line = sigma_along_axis(component_coordinates(n, grid.dx, shift), layer,
-                                    half_length, resolution)
+            line = sigma_along_axis(component_coordinates(n, grid.dx, shift,
+                                                          bool(wraps[layer.axis])),
+                                    layer, half_length, resolution)
             key = (layer.axis, shift)
             lines[key] = line if key not in lines else lines[key] + line
     shape = tuple(grid.shape)

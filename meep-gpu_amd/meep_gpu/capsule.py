@@ -1,0 +1,561 @@
+"""Lifted MEEP simulations from capsules: load one with no MEEP and step it on the GPU.
+
+A capsule (reference capsule.py:1-60) is the reference's lifted driver saved as data: a zip
+holding ``capsule.json`` (the construction log and the whole object graph below the driver,
+every float scalar as the hex of its IEEE-754 bytes) and one ``.npy`` member per array. The
+MEEP-identity fixtures (reference meep_identity.py:1-60) pair each capsule with the currents
+its sources returned (``currents.json``) and the SHA-256 of MEEP's own float32 words per
+compared step and component (``digests.json``). Stepping the capsule here and hashing the
+engine's words against those digests is the reference's Tier A replay, with the HIP engine in
+place of its array path.
+
+What is read, and how:
+
+* only JSON and ``numpy.load(..., allow_pickle=False)``: nothing in a capsule executes;
+* every array member is checked against the sha256 of its bytes the document records;
+* the object graph is decoded by tag: ``a`` array, ``f`` float64 bits, ``c`` complex,
+  ``t`` tuple, ``l`` list, ``d`` dict, ``i``/``o``/``s`` an object (id, class, attributes),
+  ``r`` a reference to an object (written before or after it), ``b``/``g`` a NumPy scalar, ``nt``/``v`` a
+  named tuple;
+* the lifted state is pasted, not recomputed: the UPML vectors, inverse epsilon, ADE words,
+  conductivity and the sources' point amplitudes go to the engine as the capsule holds them,
+  and the PML membership lines come from the lifted chunk division (chunk_map.py:497-570).
+
+What is refused, by name (``CapsuleRefusal``): symmetry folding, cylindrical and complex
+(Bloch) storage, chi2/chi3, off-diagonal chi1inv, monitors, a chunk division no membership
+line describes, more poles than the engine holds, a source without point tables, and a
+currents table whose times are not the driver's (electric sources at ``n*dt + 0.5*dt``,
+magnetic at ``n*dt``; driver.py:3536, :3565).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import io
+import json
+import struct
+import zipfile
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy
+
+from . import _native
+from .grid import Grid
+
+CAPSULE_SCHEMA = ("meep_gpu.capsule", 1)
+CURRENTS_SCHEMA = ("meep_gpu.capsule_currents", 1)
+DIGESTS_SCHEMA = ("meep_gpu.meep_identity_digests", 1)
+
+#: The twelve components a digest set covers (meep_identity.py COMPONENTS).
+COMPONENTS = ("Ex", "Ey", "Ez", "Dx", "Dy", "Dz", "Hx", "Hy", "Hz", "Bx", "By", "Bz")
+_AXES = ("x", "y", "z")
+
+
+class CapsuleRefusal(ValueError):
+    """A capsule this engine cannot load, with the reason."""
+
+
+# --- decoding -------------------------------------------------------------------------------
+@dataclass
+class Node:
+    """One decoded object of the graph: its class path and attributes."""
+    cls: str
+    attrs: Dict[str, object] = field(default_factory=dict)
+
+    def __getitem__(self, name: str):
+        return self.attrs[name]
+
+    def get(self, name: str, default=None):
+        return self.attrs.get(name, default)
+
+
+def _float_bits(text: str) -> float:
+    return struct.unpack("<d", bytes.fromhex(text))[0]
+
+
+def _index_objects(node, raw: Dict[int, dict]):
+    """Every object node of the document by id (attributes are written in sorted order, so a
+    reference can come before the object it names)."""
+    if isinstance(node, list):
+        for v in node:
+            _index_objects(v, raw)
+    elif isinstance(node, dict):
+        if set(node) == {"i", "o", "s"}:
+            raw[node["i"]] = node
+        for v in node.values():
+            _index_objects(v, raw)
+
+
+def _decode(node, arrays: Sequence[numpy.ndarray], seen: Dict[int, Node],
+            raw: Dict[int, dict]):
+    if node is None or isinstance(node, (bool, int, float, str)):
+        return node
+    if isinstance(node, list):
+        return [_decode(v, arrays, seen, raw) for v in node]
+    if not isinstance(node, dict):
+        raise CapsuleRefusal(f"unknown capsule node {node!r}")
+    keys = set(node)
+    if keys == {"a"}:
+        return arrays[node["a"]]
+    if keys == {"f"}:
+        return _float_bits(node["f"])
+    if keys == {"c"}:
+        return complex(_float_bits(node["c"][0]), _float_bits(node["c"][1]))
+    if keys == {"t"}:
+        return tuple(_decode(v, arrays, seen, raw) for v in node["t"])
+    if keys == {"l"}:
+        return [_decode(v, arrays, seen, raw) for v in node["l"]]
+    if keys == {"d"}:
+        out = {}
+        for key, value in node["d"]:
+            key = _decode(key, arrays, seen, raw)
+            out[tuple(key) if isinstance(key, list) else key] = _decode(value, arrays, seen, raw)
+        return out
+    if keys == {"r"}:
+        ref = node["r"]
+        if ref not in seen:
+            if ref not in raw:
+                raise CapsuleRefusal(f"a reference to object {ref}, which the capsule lacks")
+            return _decode(raw[ref], arrays, seen, raw)
+        return seen[ref]
+    if keys == {"b", "g"}:
+        return numpy.frombuffer(bytes.fromhex(node["b"]), dtype=numpy.dtype(node["g"]))[0]
+    if keys == {"nt", "v"}:
+        return Node(node["nt"], {"values": tuple(_decode(v, arrays, seen, raw) for v in node["v"])})
+    if keys == {"i", "o", "s"}:
+        if node["i"] in seen:
+            return seen[node["i"]]
+        obj = Node(node["o"])
+        seen[node["i"]] = obj  # registered first: attributes may refer back to it
+        obj.attrs = {k: _decode(v, arrays, seen, raw) for k, v in node["s"].items()}
+        return obj
+    raise CapsuleRefusal(f"unknown capsule node tags {sorted(keys)}")
+
+
+def read_capsule_document(path) -> Tuple[dict, Node]:
+    """The capsule's JSON document and its decoded driver object, every array checked."""
+    with zipfile.ZipFile(path) as archive:
+        doc = json.loads(archive.read("capsule.json"))
+        if (doc.get("schema"), doc.get("version")) != CAPSULE_SCHEMA:
+            raise CapsuleRefusal(f"capsule schema {doc.get('schema')!r} version "
+                                 f"{doc.get('version')!r}; this reader takes {CAPSULE_SCHEMA}")
+        arrays = []
+        for entry in doc["arrays"]:
+            array = numpy.load(io.BytesIO(archive.read(entry["member"])), allow_pickle=False)
+            if (array.dtype != numpy.dtype(entry["dtype"])
+                    or list(array.shape) != list(entry["shape"])):
+                raise CapsuleRefusal(f"{entry['path']}: member is {array.dtype} {array.shape}, "
+                                     f"the document says {entry['dtype']} {entry['shape']}")
+            if hashlib.sha256(numpy.ascontiguousarray(array).tobytes()).hexdigest() != entry["sha256"]:
+                raise CapsuleRefusal(f"{entry['path']}: bytes differ from the recorded sha256")
+            arrays.append(array)
+    raw: Dict[int, dict] = {}
+    _index_objects(doc["state"], raw)
+    return doc, _decode(doc["state"], arrays, {}, raw)
+
+
+# --- the lifted state -----------------------------------------------------------------------
+@dataclass
+class LayerTables:
+    """A lifted UPML: the coefficient vectors as stored, and the division's membership lines,
+    in the shape ``HipSimulation`` takes a ``PML`` (keys ``(half, axis)``)."""
+    vectors: Dict[Tuple[int, int], Tuple[numpy.ndarray, numpy.ndarray, numpy.ndarray]]
+    members: Dict[Tuple[int, int], numpy.ndarray]
+    faces: Tuple[Tuple[float, float], ...]
+    order: int
+    R_asymptotic: float
+
+    @property
+    def is_active(self) -> bool:
+        return True
+
+
+@dataclass
+class LiftedPole:
+    kind: str
+    frequency: float
+    gamma: float
+    constants: numpy.ndarray            # the four float32 words, as lifted
+    sigma: Dict[str, object]            # "Ex".. -> float32 volume or float (driven only)
+
+
+@dataclass
+class LiftedSource:
+    component: str                      # "Ex".."Ez" (into D) or "Hx".."Hz" (into B)
+    index: numpy.ndarray                # (npoints, 3) int64 stored-cell indices
+    amps: numpy.ndarray                 # (npoints,) complex128
+
+    @property
+    def magnetic(self) -> bool:
+        return self.component[0] in "HB"
+
+
+@dataclass
+class LiftedState:
+    shape: Tuple[int, int, int]
+    dimensions: int
+    resolution: float
+    courant: float
+    dx: float
+    dt: float
+    invariant: Tuple[bool, bool, bool]
+    metallic: Tuple[bool, bool, bool]
+    inv_eps: Dict[str, numpy.ndarray]
+    pml: Optional[LayerTables]
+    conductivity: Dict[str, numpy.ndarray]      # "Dx".."Bz" float32 sigma, nonzero only
+    condfac: Dict[str, numpy.ndarray]
+    condinv: Dict[str, numpy.ndarray]
+    absorber_sigma: Dict[str, numpy.ndarray]    # float64, before rounding (provenance)
+    absorber_layers: List[Tuple[float, int, str, float]]
+    poles: List[LiftedPole]
+    sources: List[LiftedSource]
+    volumes: Dict[str, numpy.ndarray]           # every stored float32 field volume, engine names
+    step: int
+    h_separated: bool
+    absent: Tuple[str, ...]
+
+    def grid(self) -> Grid:
+        """The engine grid; refuses when its dx, dt or cell counts are not the lifted words."""
+        size = [0.0 if self.invariant[a] else self.shape[a] / self.resolution for a in range(3)]
+        grid = Grid(resolution=self.resolution, cell_size=tuple(size), courant=self.courant,
+                    dimensions=self.dimensions)
+        if tuple(grid.shape) != tuple(self.shape) or grid.invariant != self.invariant:
+            raise CapsuleRefusal(f"grid {grid.shape} {grid.invariant} is not the lifted "
+                                 f"{self.shape} {self.invariant}")
+        if grid.dx != self.dx or grid.dt != self.dt:
+            raise CapsuleRefusal(f"dx/dt {grid.dx!r}/{grid.dt!r} differ from the lifted "
+                                 f"{self.dx!r}/{self.dt!r}")
+        return grid
+
+
+def _axis_frame(frame: Node) -> Tuple[int, int, int, bool, bool]:
+    for ghost in ("near_ghost_row", "far_ghost_row"):
+        if frame.get(ghost) is not None:
+            raise CapsuleRefusal("a folded axis (symmetry ghost rows) is not built")
+    return (int(frame["origin"]), int(frame["stored"]), int(frame["owned"]),
+            bool(frame["wraps"]), bool(frame["invariant"]))
+
+
+def layout_membership(layout: Node, axis: int, half: bool) -> numpy.ndarray:
+    """Per stored row: does the chunk owning it carry this axis's layer? A restatement of
+    ``ChunkLayout.membership`` (chunk_map.py:497-570): every stored row's doubled coordinate
+    ``origin + 2 i + half``, moved into the window on a wrapping axis; owned when it lies in
+    ``(origin, origin + 2 owned]``; a member when every box covering it carries the axis and
+    one does. Refuses a division no line describes (boxes that disagree on one row)."""
+    lines = layout.get("_lines") or {}
+    if (axis, bool(half)) in lines:
+        return numpy.asarray(lines[(axis, bool(half))], dtype=bool)
+    origin, stored, owned, wraps, invariant = _axis_frame(layout["frames"][axis])
+    if invariant:
+        return numpy.zeros(stored, dtype=bool)
+    points = origin + 2 * numpy.arange(stored, dtype=numpy.int64) + (1 if half else 0)
+    lo, hi = origin, origin + 2 * owned
+    if wraps:
+        points = numpy.where(points <= lo, points + (hi - lo), points)
+        points = numpy.where(points > hi, points - (hi - lo), points)
+    is_owned = (points > lo) & (points <= hi)
+    any_true = numpy.zeros(stored, dtype=bool)
+    any_false = numpy.zeros(stored, dtype=bool)
+    for (box_lo, box_hi), flags in zip(layout["boxes"], layout["pml_flags"]):
+        covered = (points > box_lo[axis]) & (points <= box_hi[axis])
+        if flags[axis]:
+            any_true |= covered
+        else:
+            any_false |= covered
+    if numpy.any(any_true & any_false & is_owned):
+        raise CapsuleRefusal(f"the chunk division is not separable along axis {axis}")
+    return any_true & is_owned
+
+
+def _flat(vector) -> numpy.ndarray:
+    return numpy.ascontiguousarray(numpy.asarray(vector, dtype=numpy.float32).reshape(-1))
+
+
+def _lift_pml(pml: Node, layout: Node, shape) -> LayerTables:
+    vectors, members = {}, {}
+    for half in (0, 1):
+        suffix = "_h" if half else ""
+        for axis, name in enumerate(_AXES):
+            kms = _flat(pml[f"kms_{name}{suffix}"])
+            sinv = _flat(pml[f"sinv_{name}{suffix}"])
+            kps = _flat(pml[f"kps_{name}{suffix}"])
+            if not kms.size == sinv.size == kps.size == shape[axis]:
+                raise CapsuleRefusal(f"PML vectors of axis {name} have {kms.size} words, the "
+                                     f"grid {shape[axis]} rows")
+            vectors[(half, axis)] = (kms, sinv, kps)
+            members[(half, axis)] = layout_membership(layout, axis, bool(half))
+    faces = tuple(tuple(float(v) for v in f) for f in pml["thickness_by_face"])
+    return LayerTables(vectors, members, faces, int(pml["order"]), float(pml["R_asymptotic"]))
+
+
+def _volume(value) -> numpy.ndarray:
+    array = numpy.asarray(value)
+    if array.dtype != numpy.float32:
+        raise CapsuleRefusal(f"a field volume is {array.dtype}, not float32")
+    return numpy.ascontiguousarray(array)
+
+
+def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedState:
+    """The engine's view of a decoded lifted driver, refusing what the engine does not step."""
+    if driver.cls != "meep_gpu.driver:FdtdDriver":
+        raise CapsuleRefusal(f"the capsule holds a {driver.cls}, not an FdtdDriver")
+    fields = driver["fields"]
+    grid = fields["grid"]
+    if grid["symmetry"] or any(m is not None for m in grid["mirrors_by_axis"]):
+        raise CapsuleRefusal("symmetry folding is not built")
+    if grid["cylindrical"]:
+        raise CapsuleRefusal("cylindrical coordinates are not built")
+    if grid["has_bloch"] or fields["force_complex_fields"] or driver.get("force_complex_fields"):
+        raise CapsuleRefusal("complex (Bloch) field storage is not built")
+    if float(driver["_chi2"]) != 0.0 or float(driver["_chi3"]) != 0.0 \
+            or fields["_chi2_components"] or fields["_chi3_components"]:
+        raise CapsuleRefusal("chi2/chi3 nonlinearity is not built")
+    if fields["_chi1inv_offdiagonal"]:
+        raise CapsuleRefusal("off-diagonal chi1inv in a lifted run is not built")
+    if driver["_flux_monitors"] or driver["_dft_monitors"]:
+        raise CapsuleRefusal("a capsule carries no monitors")
+    if any(fields.get(f"f_bfast_{c}") is not None for c in ("Bx", "By", "Bz", "Dx", "Dy", "Dz")):
+        raise CapsuleRefusal("the special-kz (bfast) form is not built")
+
+    shape = tuple(int(n) for n in grid["shape"])
+    invariant = tuple(bool(v) for v in grid["invariant_axes"])
+    bounds = grid["boundaries"]
+    metallic = []
+    for axis in range(3):
+        low, high = bounds[axis]
+        if low != high or low not in ("periodic", "metallic"):
+            raise CapsuleRefusal(f"boundaries {low!r}/{high!r} on axis {axis} are not built")
+        metallic.append(low == "metallic")
+
+    inv_eps = {}
+    for c in ("Ex", "Ey", "Ez"):
+        vol = fields["_inv_eps_components"].get(c, fields["inv_eps"])
+        inv_eps[c] = _volume(vol)
+
+    layout = driver["_chunk_layout"]
+    pml = None
+    if driver["pml"] is not None and fields["_pml_active"]:
+        pml = _lift_pml(driver["pml"], layout, shape)
+
+    def side(table) -> Dict[str, numpy.ndarray]:
+        return {k: _volume(v) for k, v in (table or {}).items()}
+
+    conductivity = {k: v for k, v in side(fields["_conductivity"]).items() if numpy.any(v)}
+    absorber_sigma = {k: numpy.asarray(v, dtype=numpy.float64)
+                      for k, v in (driver["_absorber_sigma"] or {}).items()}
+    layers = [(float(l["thickness"]), int(l["axis"]), str(l["side"]), float(l["R_asymptotic"]))
+              for l in driver["_absorber_layers"]]
+
+    poles = []
+    for state in fields["polarizations"]:
+        term = state["susceptibility"]
+        words = numpy.array(state["constants"]["values"], dtype=numpy.float32)
+        driven = tuple(state["_driven"])
+        sigma = {}
+        for c, value in state["sigma"].items():
+            if c not in driven:
+                continue
+            sigma[c] = _volume(value) if numpy.ndim(value) else float(value)
+        poles.append(LiftedPole(str(term["kind"]), float(term["frequency"]),
+                                float(term["gamma"]), words, sigma))
+    if len(poles) > max_poles:
+        raise CapsuleRefusal(f"{len(poles)} poles; the engine holds {max_poles}")
+
+    sources = []
+    for src in driver["_sources"]:
+        ix = src.get("_point_ix")
+        if ix is None or src.get("_point_amps") is None:
+            raise CapsuleRefusal(f"source {src.cls} has no point tables")
+        index = numpy.stack([numpy.asarray(src[f"_point_i{a}"], dtype=numpy.int64).reshape(-1)
+                             for a in _AXES], axis=1)
+        amps = numpy.asarray(src["_point_amps"], dtype=numpy.complex128).reshape(-1)
+        component = str(src["component"])
+        if component not in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+            raise CapsuleRefusal(f"source component {component!r} is not built")
+        if numpy.any(index < 0) or numpy.any(index >= numpy.array(shape)):
+            raise CapsuleRefusal(f"source {src.cls} has points outside the grid")
+        sources.append(LiftedSource(component, index, amps))
+
+    volumes = {}
+    for prefix in ("B", "D", "E", "H", "fu_B", "fu_D", "f_w_E", "f_w_H", "f_cond_D", "f_cond_B"):
+        for a in _AXES:
+            value = fields.get(prefix + a)
+            if value is not None:
+                volumes[prefix + a] = _volume(value)
+    for k, state in enumerate(fields["polarizations"]):
+        for c, value in state["P"].items():
+            volumes[f"P[{k}].{c}"] = _volume(value)
+        for c, value in state["P_prev"].items():
+            volumes[f"P_prev[{k}].{c}"] = _volume(value)
+
+    return LiftedState(
+        shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
+        courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),
+        invariant=invariant, metallic=tuple(metallic), inv_eps=inv_eps, pml=pml,
+        conductivity=conductivity, condfac=side(fields["_condfac"]),
+        condinv=side(fields["_condinv"]), absorber_sigma=absorber_sigma, absorber_layers=layers,
+        poles=poles, sources=sources, volumes=volumes, step=int(driver["step_count"]),
+        h_separated=bool(fields["h_separated"]), absent=tuple(fields["absent_components"]))
+
+
+def read_capsule(path, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedState:
+    return lift_state(read_capsule_document(path)[1], max_poles=max_poles)
+
+
+# --- currents and digests -------------------------------------------------------------------
+def read_currents(path, state: LiftedState, steps: int) -> List[numpy.ndarray]:
+    """Each source's current per step, ``(steps, 2)`` float64 (re, im), from a recorded
+    currents table; its times must be the ones the driver asks for at each step."""
+    doc = json.loads(Path(path).read_text())
+    if (doc.get("schema"), doc.get("version")) != CURRENTS_SCHEMA:
+        raise CapsuleRefusal(f"currents schema {doc.get('schema')!r}; this reader takes "
+                             f"{CURRENTS_SCHEMA}")
+    if len(doc["sources"]) != len(state.sources):
+        raise CapsuleRefusal(f"{len(doc['sources'])} current tables for "
+                             f"{len(state.sources)} sources")
+    start = int(doc["provenance"].get("start_step", 0))
+    tables = []
+    for rec, src in zip(doc["sources"], state.sources):
+        if rec["component"] != src.component:
+            raise CapsuleRefusal(f"currents of {rec['component']} for a {src.component} source")
+        calls = rec["calls"]["current"]
+        if len(calls) < steps:
+            raise CapsuleRefusal(f"{len(calls)} recorded currents for {steps} steps")
+        table = numpy.zeros((steps, 2), dtype=numpy.float64)
+        for n in range(steps):
+            key, value = calls[n]
+            t_text, dt_text = key.split("|")
+            t, dt = _float_bits(t_text), _float_bits(dt_text)
+            step = start + n
+            want = step * state.dt + (0.0 if src.magnetic else 0.5 * state.dt)
+            if dt != state.dt or t != want:
+                raise CapsuleRefusal(f"current call {n} asks t={t!r}, dt={dt!r}; step {step} "
+                                     f"deposits at t={want!r}")
+            table[n] = (_float_bits(value["c"][0]), _float_bits(value["c"][1]))
+        tables.append(table)
+    return tables
+
+
+@dataclass
+class DigestSet:
+    """MEEP's words per compared step and component (meep_identity.py:17-30)."""
+    case: str
+    mode: str
+    n_steps: int
+    steps: List[int]
+    stored: Dict[int, Dict[str, str]]
+    ghost_cells: Dict[str, int]
+
+
+def read_digests(path) -> DigestSet:
+    doc = json.loads(Path(path).read_text())
+    if (doc.get("schema"), doc.get("version")) != DIGESTS_SCHEMA:
+        raise CapsuleRefusal(f"digests schema {doc.get('schema')!r}; this reader takes "
+                             f"{DIGESTS_SCHEMA}")
+    stored = {int(step): dict(words) for step, words in doc["stored"].items()}
+    return DigestSet(doc["case"], doc["mode"], int(doc["n_steps"]),
+                     [int(s) for s in doc["steps"]], stored, dict(doc["ghost_cells"]))
+
+
+def word_digest(words: numpy.ndarray) -> str:
+    """SHA-256 of one component's words: little-endian float32 in C order (x, y, z)."""
+    return hashlib.sha256(numpy.ascontiguousarray(words, dtype="<f4").tobytes()).hexdigest()
+
+
+# --- stepping a lifted state on the GPU -----------------------------------------------------
+def _source_boxes(src: LiftedSource):
+    """One deposit box per point, in the table's order (MEEP's src_vol order)."""
+    for (i, j, k), amp in zip(src.index, src.amps):
+        yield (int(i), int(j), int(k)), amp
+
+
+class LiftedRun:
+    """A lifted state resident on the GPU: every word pasted, stepped by the engine."""
+
+    def __init__(self, state: LiftedState, currents: Sequence[numpy.ndarray]):
+        from .engine import HipSimulation
+
+        self.state = state
+        grid = state.grid()
+        self.sim = HipSimulation(grid, state.pml)
+        try:
+            self._configure(state, currents)
+        except Exception:
+            self.sim.close()
+            raise
+
+    def _configure(self, state: LiftedState, currents):
+        sim = self.sim
+        for axis in range(3):
+            if state.metallic[axis]:
+                sim.set_boundary(axis, "metallic")
+        for c, vol in state.inv_eps.items():
+            sim.set_inverse_epsilon(c, vol)
+        for pole in state.poles:
+            sim.add_pole(None, pole.sigma, constants=pole.constants)
+        for name, vol in state.conductivity.items():
+            sim.set_conductivity(name, vol)
+        for src, table in zip(state.sources, currents):
+            axis = "xyz".index(src.component[1])
+            for lo, amp in _source_boxes(src):
+                sim.add_source(src.magnetic, axis, lo, (1, 1, 1), numpy.array([amp.real]),
+                               numpy.array([amp.imag]), table, state.dt)
+        stores_e = state.pml is not None or bool(state.poles)
+        for name, vol in state.volumes.items():
+            if not numpy.any(vol):
+                continue  # the engine's volumes start at +0
+            if name[0] == "H" and state.pml is None:
+                raise CapsuleRefusal("a stored H apart from B without an absorber")
+            if name[0] == "E" and not stores_e:
+                continue  # E is formed from D below
+            sim.upload(name, vol)
+        sim.set_step(state.step, state.h_separated)
+        if not stores_e and not state.conductivity:
+            sim.prime_e()
+
+    @property
+    def step_count(self) -> int:
+        return self.sim.step_count
+
+    def advance(self, steps: int):
+        if steps > 0:
+            self.sim.run(int(steps))
+
+    def words(self, component: str) -> numpy.ndarray:
+        """One component's stored words in the reference layout (H is B's word off the
+        layer's H rows, and everywhere without an absorber)."""
+        if component[0] == "H" and self.state.pml is None:
+            component = "B" + component[1]
+        return self.sim.download(component)
+
+    def close(self):
+        self.sim.close()
+
+
+def compare_step(run, digests: DigestSet, step: int) -> List[str]:
+    """The components whose words differ from MEEP's at one compared step."""
+    want = digests.stored[step]
+    return [c for c in COMPONENTS if c in want and word_digest(run.words(c)) != want[c]]
+
+
+def replay(capsule_dir, max_poles: int = _native.FDTD_MAX_POLES) -> Dict[int, List[str]]:
+    """Step a MEEP-identity case (``capsule.capsule``, ``currents.json``, ``digests.json`` in
+    one folder) on the GPU and judge MEEP's stored words at every compared step. Returns the
+    differing components per compared step (empty lists when every word is MEEP's)."""
+    folder = Path(capsule_dir)
+    state = read_capsule(folder / "capsule.capsule", max_poles=max_poles)
+    digests = read_digests(folder / "digests.json")
+    if any(digests.ghost_cells.values()):
+        raise CapsuleRefusal("the digests compare a folded grid's cells")
+    currents = read_currents(folder / "currents.json", state, digests.n_steps)
+    run = LiftedRun(state, currents)
+    try:
+        out = {}
+        for step in digests.steps:
+            run.advance(step - run.step_count)
+            out[step] = compare_step(run, digests, step)
+        return out
+    finally:
+        run.close()

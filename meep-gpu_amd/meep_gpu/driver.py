@@ -112,8 +112,9 @@ class FdtdDriver:
     def set_boundaries(self, boundaries: Dict[str, str]):
         """Per-axis boundaries, ``{"x": "metallic", ...}`` (the reference driver's
         ``boundaries``): "periodic" (the default) or "metallic" -- perfect electric conductor
-        walls on both faces of the axis (stepping.py:140-146). An absorber on a metallic axis
-        and off-diagonal chi1inv beside one are not built and are refused at the freeze."""
+        walls on both faces of the axis (stepping.py:140-146). A PML or an absorber on a
+        metallic axis grades from the walls without a wrap; off-diagonal chi1inv beside a wall
+        is not built and is refused at the freeze."""
         self._require_unfrozen("set_boundaries")
         kinds = {}
         for key, kind in (boundaries or {}).items():
@@ -124,6 +125,12 @@ class FdtdDriver:
                 raise ValueError(f"axis {key!r} is invariant: it has no wall")
             kinds[axis] = kind
         self._boundaries = kinds
+
+    @property
+    def wraps(self) -> Tuple[bool, bool, bool]:
+        """Per axis: periodic (True) or terminated by metallic walls (reference
+        grid.py:1267-1289)."""
+        return tuple(self._boundaries.get(a, "periodic") != "metallic" for a in range(3))
 
     def set_absorber(self, thickness: float, direction: Optional[int] = None,
                      side: Optional[str] = None, R_asymptotic: float = 1e-15):
@@ -139,7 +146,8 @@ class FdtdDriver:
         """The float32 sigma each conductive component is stepped with: the material's plus
         the absorber's, summed in double and rounded once."""
         total = {n: v.copy() for n, v in self._conductivity.items()}
-        for name, vol in absorber_conductivity(self.grid, self._absorber_layers).items():
+        for name, vol in absorber_conductivity(self.grid, self._absorber_layers,
+                                               self.wraps).items():
             total[name] = total[name] + vol if name in total else vol
         return {n: v.astype(numpy.float32) for n, v in total.items() if numpy.any(v)}
 
@@ -272,7 +280,8 @@ class FdtdDriver:
         if self.sim is not None:
             return self.sim
         steps = int(planned_steps or self.max_steps)
-        pml = PML(self.grid, self._pml_spec, **self._pml_kwargs) if self._pml_spec else None
+        pml = (PML(self.grid, self._pml_spec, wraps=self.wraps, **self._pml_kwargs)
+               if self._pml_spec else None)
         sim = HipSimulation(self.grid, pml)
         try:
             for axis, kind in self._boundaries.items():

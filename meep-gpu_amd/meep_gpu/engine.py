@@ -119,9 +119,15 @@ class HipSimulation:
                                             _AXES.index(partner[1]), fptr(dev)),
               "fdtd_set_offdiagonal")
 
-    def add_pole(self, term: Susceptibility, sigma: Dict[str, object]) -> Tuple[bool, bool, bool]:
-        """Install one pole; ``sigma[Ex..]`` a volume or a scalar. Returns which components it drives."""
-        consts = numpy.array(term.realnum_constants(self.grid.dt), dtype=numpy.float32)
+    def add_pole(self, term: Optional[Susceptibility], sigma: Dict[str, object],
+                 constants: Optional[Sequence[float]] = None) -> Tuple[bool, bool, bool]:
+        """Install one pole; ``sigma[Ex..]`` a volume or a scalar. ``constants`` (the four
+        float32 words of ``realnum_constants``) replaces the words formed from ``term``, as a
+        lifted run carries them. Returns which components it drives."""
+        words = constants if constants is not None else term.realnum_constants(self.grid.dt)
+        consts = numpy.array(words, dtype=numpy.float32)
+        if consts.shape != (4,):
+            raise ValueError(f"a pole has 4 constant words, got {consts.shape}")
         keep = []
         ptrs = (_native._FP * 3)()
         scalars = numpy.zeros(3, dtype=numpy.float32)

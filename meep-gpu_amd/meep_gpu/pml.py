@@ -10,14 +10,16 @@ with ``sig = (dt/2) * prefac * u**order``. The grading follows MEEP's ``pml_x``
 (structure.cpp:625-628 via pml.py:469-598): distance from each wall counted in half cells,
 with the extent snapped to ``int(2*cells + 0.5)`` half cells and ``u`` normalised by the raw
 request. On a wrapping axis the integer sample stored at index 0 is MEEP's owned lattice
-point at 2N, graded from the upper wall. ``prefac = -ln(R) / (4 * cells*dx / (order+1))``,
-per face (pml.py:436).
+point at 2N, graded from the upper wall; on a metallic axis, which does not wrap (reference
+grid.py:1267-1289), it is the low wall itself at 0 (pml.py:578-582).
+``prefac = -ln(R) / (4 * cells*dx / (order+1))``, per face (pml.py:436).
 
 The membership line of an axis says whether the one-rank chunk owning a stored row carries
 that axis's layer (MEEP picks the PML form per chunk, step_db.cpp:56-59): a face of ``t``
 cells owns ``int(t + 1.5)`` rows from its wall, rows ``1..span`` on the integer lattice at the
 low wall and ``0..span-1`` on the half lattice; the top ``span`` on the half lattice at the
-high wall and the top ``span-1`` plus row 0 (the wrap of the wall plane) on the integer one.
+high wall and the top ``span-1`` plus row 0 (the wrap of the wall plane) on the integer one;
+a metallic axis has no such wrap, and its integer row 0 (the wall) is owned by no chunk.
 """
 
 from __future__ import annotations
@@ -80,6 +82,8 @@ class PML:
     R_asymptotic: float = 1e-15
     #: Replaces -ln(R) when set (a fixture keeps libm out of its table this way).
     ln_reflection: Optional[float] = None
+    #: Per axis: periodic (True) or terminated by a metallic wall (False).
+    wraps: Tuple[bool, bool, bool] = (True, True, True)
     faces: Faces = field(init=False)
     vectors: Dict[Tuple[int, int], Tuple[numpy.ndarray, numpy.ndarray, numpy.ndarray]] = \
         field(init=False, repr=False)
@@ -91,18 +95,22 @@ class PML:
         if not 0.0 < self.R_asymptotic < 1.0:
             raise ValueError("R_asymptotic must lie in (0, 1)")
         self.faces = normalize_faces(self.thickness, self.grid)
+        self.wraps = tuple(bool(w) for w in self.wraps)
+        if len(self.wraps) != 3:
+            raise ValueError("wraps needs one flag per axis")
         self.vectors = {}
         self.members = {}
         for half in (0, 1):
             for axis in range(3):
                 n = self.grid.shape[axis]
-                sig = self._sigma(n, self.faces[axis], half)
+                sig = self._sigma(n, self.faces[axis], half, self.wraps[axis])
                 one = numpy.ones(n, dtype=numpy.float32)
                 total = one + sig
                 self.vectors[(half, axis)] = ((one - sig).astype(numpy.float32),
                                               (numpy.float32(1.0) / total).astype(numpy.float32),
                                               total.astype(numpy.float32))
-                self.members[(half, axis)] = self._line(n, self.faces[axis], half)
+                self.members[(half, axis)] = self._line(n, self.faces[axis], half,
+                                                        self.wraps[axis])
 
     @property
     def is_active(self) -> bool:
@@ -116,11 +124,14 @@ class PML:
             else float(-numpy.log(self.R_asymptotic))
         return float(ln_r / (4.0 * depth * (1.0 / (self.order + 1))))
 
-    def _sigma(self, n: int, faces: Tuple[float, float], half: int) -> numpy.ndarray:
+    def _sigma(self, n: int, faces: Tuple[float, float], half: int,
+               wraps: bool = True) -> numpy.ndarray:
         out = numpy.zeros(n, dtype=numpy.float32)
-        # half-cell index of every stored row, index 0 of the integer lattice wrapped to 2n
+        # half-cell index of every stored row; on a wrapping axis index 0 of the integer
+        # lattice is the image of 2n
         pos = 2 * numpy.arange(n) + half
-        pos = (pos - 1) % (2 * n) + 1
+        if wraps:
+            pos = (pos - 1) % (2 * n) + 1
         for wall, cells in ((0, faces[0]), (2 * n, faces[1])):
             if cells <= 0:
                 continue
@@ -132,7 +143,7 @@ class PML:
         return out
 
     @staticmethod
-    def _line(n: int, faces: Tuple[float, float], half: int) -> numpy.ndarray:
+    def _line(n: int, faces: Tuple[float, float], half: int, wraps: bool = True) -> numpy.ndarray:
         line = numpy.zeros(n, dtype=bool)
         low, high = faces
         if low > 0:
@@ -147,7 +158,8 @@ class PML:
                 line[n - span:] = True
             else:
                 line[max(0, n - span + 1):] = True
-                line[0] = True
+                if wraps:
+                    line[0] = True
         return line
 
     def member_fraction(self, axis: int, half: int) -> float:

@@ -36,8 +36,11 @@ class _OGrid(ctypes.Structure):
                 ("mem", (_U8P * 3) * 2), ("metal", ctypes.c_int * 3)]
 
 
+MAX_POLES = 8  # ORACLE_MAX_POLES of the C restatement
+
+
 class _OElectric(ctypes.Structure):
-    _fields_ = [("npol", ctypes.c_int), ("p", (_FP * 3) * 4), ("u", (_FP * 3) * 3),
+    _fields_ = [("npol", ctypes.c_int), ("p", (_FP * 3) * MAX_POLES), ("u", (_FP * 3) * 3),
                 ("offdiag", ctypes.c_int)]
 
 
@@ -56,7 +59,7 @@ def lib():
         _lib.oracle_deposit.argtypes = [_FP, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int,
-                                        ctypes.c_double, ctypes.c_double, ctypes.c_double]
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _FP]
         _lib.oracle_update_p.argtypes = [ctypes.c_void_p, ctypes.c_int, _FP, _FP, _FP, _FP,
                                          ctypes.c_float, _FP, _FP, _FP]
     return _lib
@@ -106,6 +109,7 @@ class OracleSim:
                  pml_faces=((0, 0), (0, 0), (0, 0)), pml_order: int = 2,
                  ln_reflection: Optional[float] = None):
         self.shape = tuple(int(n) for n in shape)
+        self._layer_args = (pml_order, ln_reflection)
         self.dt = float(dt)
         self.dx = float(dx)
         self.invariant = tuple(bool(v) for v in invariant)
@@ -200,8 +204,31 @@ class OracleSim:
 
     def set_metallic(self, axes: Sequence[bool]):
         """Metallic (PEC) walls on the given axes (stepping.py:140-146): zero ghosts past the
-        wall, the wall's own samples unstepped and held at zero."""
+        wall, the wall's own samples unstepped and held at zero. A metallic axis does not wrap
+        (grid.py:1267-1289), so its layer is graded from the walls without a wrap of the
+        integer row 0 and that row (the wall) is owned by no chunk (pml.py:578-582)."""
         self.metal = tuple(bool(a) for a in axes)
+        order, ln_r = self._layer_args
+        for axis in range(3):
+            if not self.metal[axis] or self.invariant[axis]:
+                continue
+            for half in (0, 1):
+                n = self.shape[axis]
+                kms, sinv, kps = layer.axis_coefficients(n, self.faces[axis], 0.5 * half,
+                                                         self.dt, self.dx, order, ln_r,
+                                                         wraps=False)
+                mem = layer.membership_line(n, self.faces[axis], bool(half), wraps=False)
+                self._vec[(half, axis)] = (kms, sinv, kps, mem.astype(numpy.uint8))
+
+    def set_layer_tables(self, vectors, members):
+        """Paste a layer table as lifted (``{(half, axis): (kms, sinv, kps)}`` float32 and
+        ``{(half, axis): bool line}``) in place of the one this run graded itself."""
+        if not self.pml:
+            raise ValueError("this run has no absorber to paste a table into")
+        for key, (kms, sinv, kps) in vectors.items():
+            mem = numpy.asarray(members[key], dtype=numpy.uint8)
+            self._vec[key] = tuple(numpy.ascontiguousarray(v, dtype=numpy.float32)
+                                   for v in (kms, sinv, kps)) + (numpy.ascontiguousarray(mem),)
 
     def zero_metal(self, field: str):
         """zero_metal_D / zero_metal_B (stepping.py:3094-3183): stored cell 0 of every
@@ -267,6 +294,8 @@ class OracleSim:
         return arr
 
     def _electric(self) -> _OElectric:
+        if len(self.poles) > MAX_POLES:
+            raise ValueError(f"{len(self.poles)} poles; the restatement holds {MAX_POLES}")
         x = _OElectric()
         x.npol = len(self.poles)
         for k, pole in enumerate(self.poles):
@@ -340,15 +369,19 @@ class OracleSim:
             if (numpy.copysign(1.0, cur.real) > 0 and cur.real == 0.0
                     and numpy.copysign(1.0, cur.imag) > 0 and cur.imag == 0.0):
                 continue
-            target = self.v[("B" if magnetic else "D") + dep.component[1]]
+            name = ("B" if magnetic else "D") + dep.component[1]
+            target = self.v[name]
             idx = numpy.ascontiguousarray(dep.index, dtype=numpy.int64)
+            condinv = self.condinv.get(name)
+            if condinv is not None:
+                condinv = numpy.ascontiguousarray(condinv).reshape(-1)
             lib().oracle_deposit(_fp(target.reshape(-1)),
                                  idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                  numpy.ascontiguousarray(dep.amp_re, dtype=numpy.float64)
                                  .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                  numpy.ascontiguousarray(dep.amp_im, dtype=numpy.float64)
                                  .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                 int(idx.size), cur.real, cur.imag, self.dt)
+                                 int(idx.size), cur.real, cur.imag, self.dt, _fp(condinv))
 
     def step(self):
         self.step_B()

@@ -229,9 +229,11 @@ void oracle_update_h(const ogrid *g, float *const h[3], float *const b[3], float
     }
 }
 
+#define ORACLE_MAX_POLES 8
+
 typedef struct {
-    int npol;
-    const float *p[4][3];       /* P[k][c], NULL when pole k does not drive component c */
+    int npol;                   /* at most ORACLE_MAX_POLES */
+    const float *p[ORACLE_MAX_POLES][3];  /* P[k][c], NULL when pole k does not drive c */
     const float *u[3][3];       /* off-diagonal chi1inv[row][partner], NULL when absent */
     int offdiag;                /* any off-diagonal row present */
 } oelectric;
@@ -328,13 +330,16 @@ void oracle_update_p(const ogrid *g, int c, const float consts[4], const float *
     }
 }
 
-/* f[idx] = float(double(f[idx]) - ((amp_re*c_re - amp_im*c_im) * dt)) at each point. */
+/* f[idx] = float(double(f[idx]) - ((amp_re*c_re - amp_im*c_im) * dt) [* double(condinv)])
+ * at each point; condinv (the written component's conductivity word, sources.py:1357-1400,
+ * step.cpp:299-307) is null where that component is lossless. */
 void oracle_deposit(float *f, const int64_t *idx, const double *amp_re, const double *amp_im,
-                    int n, double c_re, double c_im, double dt) {
+                    int n, double c_re, double c_im, double dt, const float *condinv) {
     for (int q = 0; q < n; ++q) {
         double r = amp_re[q] * c_re;
         r = r - amp_im[q] * c_im;
         r = r * dt;
+        if (condinv) r = r * (double)condinv[idx[q]];
         double w = (double)f[idx[q]];
         w = w - r;
         f[idx[q]] = (float)w;

@@ -1,0 +1,107 @@
+"""MEEP-identity capsules (tests/golden/meep_identity): the oracle built from a lifted state,
+and the replay loop both tiers judge MEEP's stored words with.
+
+The capsules, currents and digests are the reference's Tier A keep-set fixtures, copied as data
+(tests/golden/meep_identity/provenance.json). The product reads them with
+``meep_gpu.capsule``; the oracle here steps the same pasted state on the CPU.
+"""
+
+from __future__ import annotations
+
+import os
+from typing import Dict, List
+
+import numpy
+
+from meep_gpu import capsule
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = os.path.join(HERE, "golden", "meep_identity", "keep")
+#: The keep-set cases the engine steps (the others need symmetry folding, cylindrical or
+#: complex storage, and are refused by name).
+CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d")
+
+
+def case_dir(case: str) -> str:
+    return os.path.join(FIXTURES, case)
+
+
+def load(case: str, max_poles: int = 8):
+    """(state, digests, currents) of one case."""
+    folder = case_dir(case)
+    state = capsule.read_capsule(os.path.join(folder, "capsule.capsule"), max_poles=max_poles)
+    digests = capsule.read_digests(os.path.join(folder, "digests.json"))
+    currents = capsule.read_currents(os.path.join(folder, "currents.json"), state,
+                                     digests.n_steps)
+    return state, digests, currents
+
+
+class OracleRun:
+    """A lifted state stepped by the CPU restatement, every word pasted as the engine pastes it."""
+
+    def __init__(self, state: capsule.LiftedState, currents):
+        from oracle.engine import OracleSim, PointDeposit
+
+        faces = state.pml.faces if state.pml is not None else ((0, 0), (0, 0), (0, 0))
+        sim = OracleSim(state.shape, state.dt, state.dx, invariant=state.invariant,
+                        pml_faces=faces)
+        sim.set_metallic(state.metallic)
+        if state.pml is not None:
+            sim.set_layer_tables(state.pml.vectors, state.pml.members)
+        sim.set_inverse_epsilon(dict(state.inv_eps))
+        for pole in state.poles:
+            sigma = {c: pole.sigma.get(c, 0.0) for c in ("Ex", "Ey", "Ez")}
+            added = sim.add_pole(pole.frequency, pole.gamma, pole.kind == "drude", sigma)
+            added.consts = numpy.array(pole.constants, dtype=numpy.float32)
+        if state.conductivity:
+            sim.set_conductivity(dict(state.conductivity))
+        for src, table in zip(state.sources, currents):
+            flat = numpy.ravel_multi_index(tuple(src.index.T), state.shape).astype(numpy.int64)
+            start = state.step
+            times = {}
+            for n in range(table.shape[0]):
+                t = (start + n) * state.dt + (0.0 if src.magnetic else 0.5 * state.dt)
+                times[t] = complex(table[n, 0], table[n, 1])
+            sim.add_deposit(PointDeposit(src.component, flat, src.amps.real.copy(),
+                                         src.amps.imag.copy(), times.__getitem__))
+        for name, vol in state.volumes.items():
+            if not numpy.any(vol):
+                continue
+            if name.startswith("P"):
+                raise NotImplementedError("pasting a nonzero polarization")
+            sim.v[name][...] = vol
+        sim.step_count = state.step
+        sim.time = state.step * state.dt
+        sim.h_separated = state.h_separated
+        self.sim = sim
+        self.state = state
+
+    @property
+    def step_count(self) -> int:
+        return self.sim.step_count
+
+    def advance(self, steps: int):
+        for _ in range(steps):
+            self.sim.step()
+
+    def words(self, component: str) -> numpy.ndarray:
+        sim = self.sim
+        if component[0] == "E":
+            return sim.get_E(component)
+        if component[0] == "H" and not sim.pml:
+            return sim.v["B" + component[1]]
+        return sim.v[component]
+
+    def close(self):
+        pass
+
+
+def replay_digests(run, digests: capsule.DigestSet, steps=None) -> Dict[int, List[str]]:
+    """Advance ``run`` through the compared steps; the components differing from MEEP's words
+    at each (all compared steps when ``steps`` is None)."""
+    out = {}
+    for step in (digests.steps if steps is None else steps):
+        run.advance(step - run.step_count)
+        print(f"compared step {step}", flush=True)
+        out[step] = capsule.compare_step(run, digests, step)
+    return out

@@ -1,0 +1,198 @@
+"""MEEP identity on the CPU: the lifted capsules decode, their tables are this package's own
+word for word, and the oracle steps them to MEEP 1.35.0's stored words at every compared step.
+
+Fixtures: the reference's Tier A keep set (tests/golden/meep_identity, copied as data; the
+reference's replay is meep_identity.py ``replay``). Each digest is the SHA-256 of MEEP's own
+float32 words of one component at one step (steps 0-20, then every tenth of the run).
+"""
+
+import hashlib
+import json
+import os
+
+import numpy
+import pytest
+
+from meep_gpu import capsule
+from meep_gpu.absorber import AbsorberLayer, absorber_conductivity
+from meep_gpu.dispersion import Susceptibility
+from meep_gpu.pml import PML
+from tests import capsule_cases as cc
+
+
+
+def test_fixture_files_are_the_reference_ledger_s():
+    with open(os.path.join(cc.FIXTURES, "..", "provenance.json")) as fh:
+        prov = json.load(fh)
+    assert set(prov["cases"]) == set(cc.CASES)
+    for case, files in prov["cases"].items():
+        for name, digest in files.items():
+            with open(os.path.join(cc.case_dir(case), name), "rb") as fh:
+                assert hashlib.sha256(fh.read()).hexdigest() == digest, (case, name)
+
+
+@pytest.mark.parametrize("case", cc.CASES)
+def test_capsules_decode(case):
+    state, digests, currents = cc.load(case)
+    assert digests.mode == "keep" and digests.case == case
+    assert state.step == 0 and not state.h_separated
+    assert len(currents) == len(state.sources) == 1
+    assert currents[0].shape == (digests.n_steps, 2)
+    # the source is live: MEEP's words move off zero
+    last = digests.stored[digests.steps[-1]]
+    comp = state.sources[0].component
+    assert last["D" + comp[1]] != digests.stored[0]["D" + comp[1]]
+
+
+def _decoded(case):
+    return capsule.read_capsule_document(os.path.join(cc.case_dir(case), "capsule.capsule"))[1]
+
+
+@pytest.mark.parametrize("change, reason", [
+    (lambda d: d["fields"]["grid"].attrs.__setitem__("symmetry", ("mirror",)), "symmetry"),
+    (lambda d: d["fields"]["grid"].attrs.__setitem__("cylindrical", True), "cylindrical"),
+    (lambda d: d["fields"].attrs.__setitem__("force_complex_fields", True), "complex"),
+    (lambda d: d.attrs.__setitem__("_chi3", 1.0), "chi2/chi3"),
+    (lambda d: d["fields"]["grid"].attrs.__setitem__(
+        "boundaries", (("metallic", "periodic"), ("metallic", "metallic"),
+                       ("periodic", "periodic"))), "boundaries"),
+    (lambda d: d["_sources"][0].attrs.__setitem__("_point_ix", None), "point tables"),
+])
+def test_what_the_engine_does_not_step_is_refused_by_name(change, reason):
+    driver = _decoded("a")
+    change(driver)
+    with pytest.raises(capsule.CapsuleRefusal, match=reason):
+        capsule.lift_state(driver)
+
+
+def test_more_poles_than_the_engine_holds_are_refused():
+    driver = _decoded("ex.absorber-1d")
+    assert len(capsule.lift_state(driver).poles) == 5
+    with pytest.raises(capsule.CapsuleRefusal, match="5 poles"):
+        capsule.lift_state(driver, max_poles=4)
+
+
+def test_a_currents_table_at_other_times_is_refused(tmp_path):
+    state, digests, _ = cc.load("a")
+    with open(os.path.join(cc.case_dir("a"), "currents.json")) as fh:
+        doc = json.load(fh)
+    calls = doc["sources"][0]["calls"]["current"]
+    calls[0], calls[1] = calls[1], calls[0]
+    path = tmp_path / "currents.json"
+    path.write_text(json.dumps(doc))
+    with pytest.raises(capsule.CapsuleRefusal, match="deposits at t="):
+        capsule.read_currents(path, state, digests.n_steps)
+
+
+@pytest.mark.parametrize("case", ("a", "b"))
+def test_the_lifted_pml_is_this_package_s_inside_metallic_walls(case):
+    """MEEP's UPML words on a metallic (non-wrapping) axis, and the chunk division's lines,
+    equal the table this package grades itself (pml.py with wraps=False)."""
+    state, _, _ = cc.load(case)
+    assert state.metallic == (True, True, False)
+    grid = state.grid()
+    faces = {"x": state.pml.faces[0], "y": state.pml.faces[1]}
+    own = PML(grid, faces, order=state.pml.order, R_asymptotic=state.pml.R_asymptotic,
+              wraps=(False, False, True))
+    for key, vectors in state.pml.vectors.items():
+        for mine, lifted in zip(own.vectors[key], vectors):
+            assert numpy.array_equal(mine.view(numpy.uint32), lifted.view(numpy.uint32)), key
+        assert numpy.array_equal(own.members[key], state.pml.members[key]), key
+    # the wrapping table owns the wall row (the image of the top plane); with equal faces its
+    # sigma there is the same maximum either way
+    wrapped = PML(grid, faces, order=state.pml.order, R_asymptotic=state.pml.R_asymptotic)
+    assert wrapped.members[(0, 0)][0] and not state.pml.members[(0, 0)][0]
+
+
+def test_a_metallic_axis_grades_row_zero_from_the_low_wall():
+    """Unequal faces tell the two gradings apart: on a wrapping axis integer row 0 is the top
+    wall's image (the high face's sigma), on a metallic one it is the low wall (pml.py:578-582)."""
+    from meep_gpu.grid import Grid
+
+    grid = Grid(resolution=10, cell_size=(4.0, 4.0, 0.0), dimensions=2)
+    faces = {"x": (6, 3)}
+    walls = PML(grid, faces, wraps=(False, True, True))
+    periodic = PML(grid, faces)
+    kms_w, kms_p = walls.vectors[(0, 0)][0], periodic.vectors[(0, 0)][0]
+    assert (kms_w != kms_p).nonzero()[0].tolist() == [0]
+    low_face = PML(grid, {"x": (6, 0)}, wraps=(False, True, True)).vectors[(0, 0)][0]
+    high_face = PML(grid, {"x": (0, 3)}).vectors[(0, 0)][0]
+    assert kms_w[0] == low_face[0] and kms_p[0] == high_face[0]
+
+
+def test_the_lifted_absorber_is_this_package_s_on_a_metallic_axis():
+    """mp.Absorber on a metallic z: the float64 sigma MEEP's lift stored equals
+    absorber.absorber_conductivity with no seam wrap, bit for bit, and so does its float32."""
+    state, _, _ = cc.load("ex.absorber-1d")
+    assert state.metallic == (False, False, True) and state.dimensions == 1
+    layers = [AbsorberLayer(t, axis, side, r) for t, axis, side, r in state.absorber_layers]
+    mine = absorber_conductivity(state.grid(), layers, wraps=(True, True, False))
+    assert set(mine) == set(state.absorber_sigma)
+    for name, lifted in state.absorber_sigma.items():
+        assert numpy.array_equal(mine[name].view(numpy.uint64), lifted.view(numpy.uint64)), name
+        assert numpy.array_equal(mine[name].astype(numpy.float32), state.conductivity[name])
+    # a one-sided layer tells the seam rule apart: integer row 0 is the far wall's image on a
+    # wrapping axis (inside a high layer), the near wall on a metallic one (outside it)
+    high = [AbsorberLayer(1.0, 2, "high")]
+    walls = absorber_conductivity(state.grid(), high, wraps=(True, True, False))["Dx"]
+    periodic = absorber_conductivity(state.grid(), high)["Dx"]
+    assert walls[0, 0, 0] == 0.0 and periodic[0, 0, 0] > 0.0
+    assert numpy.array_equal(walls[0, 0, 1:], periodic[0, 0, 1:])
+
+
+def test_the_lifted_conductivity_words_are_the_oracle_s():
+    """condfac = 1 - sigma*(float32(dt)*0.5) in float32 and condinv = 1/(1 + (sigma*dt)*0.5)
+    in double, rounded once (fields.py:775-870): MEEP's words, as the oracle forms them."""
+    from oracle.engine import OracleSim
+
+    state, _, _ = cc.load("ex.absorber-1d")
+    sim = OracleSim(state.shape, state.dt, state.dx, invariant=state.invariant)
+    sim.set_conductivity(dict(state.conductivity))
+    for name in state.conductivity:
+        assert numpy.array_equal(sim.condfac[name], state.condfac[name]), name
+        assert numpy.array_equal(sim.condinv[name], state.condinv[name]), name
+
+
+@pytest.mark.parametrize("case", ("ex.material-dispersion", "ex.absorber-1d"))
+def test_the_lifted_ade_words_are_this_package_s(case):
+    state, _, _ = cc.load(case)
+    assert state.poles
+    for pole in state.poles:
+        words = Susceptibility(pole.frequency, pole.gamma, pole.kind).realnum_constants(state.dt)
+        assert numpy.array_equal(numpy.array(words, dtype=numpy.float32), pole.constants)
+
+
+@pytest.mark.parametrize("case", cc.CASES)
+def test_the_oracle_steps_meep_s_words(case):
+    """Every compared step's stored words of all twelve components are MEEP's."""
+    state, digests, currents = cc.load(case)
+    got = cc.replay_digests(cc.OracleRun(state, currents), digests)
+    assert set(got) == set(digests.steps)
+    assert {step: d for step, d in got.items() if d} == {}
+
+
+def test_the_replay_catches_planted_mistakes():
+    """Two mistakes a restatement could make, each caught by MEEP's words at the last step:
+    the walls dropped (the axes wrap), and one UPML word inside the layer one ulp off."""
+    state, digests, currents = cc.load("a")
+    last = digests.steps[-1]
+    state.metallic = (False, False, False)
+    got = cc.replay_digests(cc.OracleRun(state, currents), digests, steps=[last])
+    assert got[last]
+
+    state, digests, currents = cc.load("a")
+    kms, sinv, kps = state.pml.vectors[(1, 0)]
+    kms = kms.copy()
+    kms[185] = numpy.nextafter(kms[185], numpy.float32(0))
+    state.pml.vectors[(1, 0)] = (kms, sinv, kps)
+    got = cc.replay_digests(cc.OracleRun(state, currents), digests, steps=[last])
+    assert got[last]
+
+
+def test_a_planted_pole_word_is_caught():
+    state, digests, currents = cc.load("ex.material-dispersion")
+    words = state.poles[0].constants.copy()
+    words[1] = numpy.nextafter(words[1], numpy.float32(2))
+    state.poles[0].constants = words
+    got = cc.replay_digests(cc.OracleRun(state, currents), digests, steps=[40])
+    assert "Ez" in got[40]
