@@ -401,9 +401,12 @@ __device__ inline bool plus_zero(double v) { return __double_as_longlong(v) == 0
 
 // Sources of one field/component deposited into the run's values (sources.py:1357,
 // host_writes.py:140): f = float(double(f) - ((amp_re*J_re - amp_im*J_im)*dt)), in list order.
-template <int V>
+template <int V, bool GEN>
 __device__ inline void deposit_run(const SimDev &s, int step, int field, int comp, const Run &q,
                                    VF<V> &x) {
+    // GEN: a conductive component scales the current by its condinv word, in the same double
+    // expression (MEEP step.cpp:299-307, sources.py:1357-1400)
+    const float *ci = GEN ? s.ci[field][comp] : nullptr;
     // one test against the union of this field's boxes before any per-source work
     if (q.j < s.src_lo[field][1] || q.j >= s.src_hi[field][1] || q.k < s.src_lo[field][2] ||
         q.k >= s.src_hi[field][2] || q.i0 + V <= s.src_lo[field][0] || q.i0 >= s.src_hi[field][0])
@@ -425,6 +428,9 @@ __device__ inline void deposit_run(const SimDev &s, int step, int field, int com
             double r = gp(d.amp_re)[p] * cre;
             r = r - gp(d.amp_im)[p] * cim;
             r = r * d.dt;
+            if constexpr (GEN) {
+                if (ci) r = r * (double)gp(ci)[q.at + m];
+            }
             double w = (double)x.v[m];
             w = w - r;
             x.v[m] = (float)w;
@@ -531,7 +537,7 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
             b.v[m] = b.v[m] - curl;
         }
     }
-    if (s.nsrc) deposit_run<V>(s, ck.step, 1, C, q, b);
+    if (s.nsrc) deposit_run<V, GEN>(s, ck.step, 1, C, q, b);
     if constexpr (GEN) {
 #pragma unroll
         for (int m = 0; m < V; ++m)
@@ -640,7 +646,7 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
             d.v[m] = d.v[m] - curl;
         }
     }
-    if (s.nsrc) deposit_run<V>(s, ck.step, 0, C, q, d);
+    if (s.nsrc) deposit_run<V, GEN>(s, ck.step, 0, C, q, d);
     if constexpr (GEN) {
 #pragma unroll
         for (int m = 0; m < V; ++m)
