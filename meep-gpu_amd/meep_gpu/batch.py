@@ -29,7 +29,8 @@ from .workloads import FluxPlane, Workload, meep_frequency_span, outside_absorbe
 
 
 class BatchedRun:
-    """Step several ``FdtdDriver`` runs (already frozen) through one batch."""
+    """Step several runs through one batch: frozen ``FdtdDriver`` runs, or anything holding a
+    ``HipSimulation`` as ``.sim`` and its grid as ``.grid`` (a ``capsule.LiftedRun``)."""
 
     def __init__(self, drivers: Sequence[FdtdDriver]):
         self.lib = _native.load()

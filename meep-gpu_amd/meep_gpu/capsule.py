@@ -516,6 +516,10 @@ class LiftedRun:
             sim.prime_e()
 
     @property
+    def grid(self) -> Grid:
+        return self.sim.grid
+
+    @property
     def step_count(self) -> int:
         return self.sim.step_count
 

@@ -37,3 +37,34 @@ def test_the_replay_entry_point():
     print("capsule.replay ex.material-dispersion", flush=True)
     out = capsule.replay(cc.case_dir("ex.material-dispersion"))
     assert out and all(not diff for diff in out.values())
+
+
+def test_meep_s_words_in_a_batch():
+    """The four lifted cases stepped together by shared batch launches each keep MEEP's words
+    (walls and five poles put the whole batch on the general instantiation; 1-D, 2-D and a
+    single cell share its tile list). The batch is read at compared steps by closing it, and
+    re-formed to go on."""
+    from meep_gpu.batch import BatchedRun
+
+    loaded = {case: cc.load(case) for case in cc.CASES}
+    runs = {case: capsule.LiftedRun(state, currents)
+            for case, (state, _, currents) in loaded.items()}
+    try:
+        done = 0
+        for target, cases in ((1, cc.CASES), (2, cc.CASES), (7, cc.CASES), (20, cc.CASES),
+                              (100, ("a", "b", "ex.material-dispersion")),
+                              (200, ("a", "b", "ex.material-dispersion"))):
+            print(f"batch of {len(runs)} to step {target}", flush=True)
+            batch = BatchedRun(list(runs.values()))
+            batch.run(target - done)
+            batch.synchronize()
+            batch.close()
+            done = target
+            for case in cases:
+                digests = loaded[case][1]
+                assert runs[case].step_count == target
+                if target in digests.stored:
+                    assert capsule.compare_step(runs[case], digests, target) == [], (case, target)
+    finally:
+        for run in runs.values():
+            run.close()
