@@ -150,8 +150,9 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
  * stepped and are held at zero after the sub-step's sources. Replaces the reference's METALLIC
  * ghost rule and zero_metal_D / zero_metal_B (stepping.py:140-146, :2429-2626, :3094-3183).
  * A layer on the axis (MEEP's PML inside metallic walls) steps its forms with the wall cells'
- * curl masked to zero (stepping.py:2589-2626). Refused on an invariant axis and beside
- * off-diagonal chi1inv (not built). */
+ * curl masked to zero (stepping.py:2589-2626); an off-diagonal chi1inv row reads zero past the
+ * wall and drops its coupling on the wall planes (stepping.py:1594-1745). Refused on an
+ * invariant axis. */
 int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
 
 /* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of

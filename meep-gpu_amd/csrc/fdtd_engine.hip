@@ -637,8 +637,13 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
         if (PML) {
             const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
             const bool in_s = member(s, 0, dsig, cs), in_u = member(s, 0, dsigu, cu);
+            // the dsigu recurrence reads the dsig row's words wherever it runs: 1 off the line,
+            // except on a metallic axis's wall row, which no chunk owns but whose words are
+            // graded from the wall (the region plan broadcasts the whole vector there)
+            bool read_s = in_s;
+            if constexpr (GEN) read_s |= cs == 0 && s.metal[dsig];
             if (in_s || in_u)
-                d.v[m] = curl_apply_pml(d.v[m], curl, coeff(s, 0, dsig, cs, in_s),
+                d.v[m] = curl_apply_pml(d.v[m], curl, coeff(s, 0, dsig, cs, read_s),
                                         coeff(s, 0, dsigu, cu, in_u), s.fuD[C], q.at + m);
             else
                 d.v[m] = d.v[m] - curl;
@@ -899,6 +904,7 @@ template <int V, bool PML>
 __device__ inline void dcurl_run(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> h[3], hn[3][3];
     load_h<V, PML>(s, q, h, hn);
+    metal_down<V>(s, q, hn);  // the general forms: zero below a metallic wall
     VF<V> d[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = ldv<V>(s.D[c], q.at);
@@ -972,16 +978,29 @@ __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Ru
             const float *U = o == 1 ? u1 : u2;
             if (!U) continue;
             // u[r]*(g[r] + g[r - s_partner]) at r = here and r = here + s_own; a shift along
-            // the own axis leaves the coordinate along the partner's axis unchanged
+            // the own axis leaves the coordinate along the partner's axis unchanged. Past a
+            // metallic wall the partner's lower neighbour and the product above read zero, and
+            // the summed term is zero on a wall plane of a metallic axis other than the own
+            // one (stepping.py:1594-1687, _mask_metallic_wall_coupling :1718-1745)
+            const bool low_pa = s.metal[pa] && coord_of(q, pa, m) == 0;
+            const bool top_c = s.metal[C] && coord_of(q, C, m) == s.n[C] - 1;
             const int up = at + step_off(s, q, i, C, +1);
             const int iu = (C == 0) ? ((i + 1 < s.n[0]) ? i + 1 : 0) : i;
             const int down0 = step_off(s, q, i, pa, -1);
             const int down1 = step_off(s, q, iu, pa, -1);
-            const float pair0 = dmp_at(s, ck.ppar, pa, at) + dmp_at(s, ck.ppar, pa, at + down0);
-            const float pair1 = dmp_at(s, ck.ppar, pa, up) + dmp_at(s, ck.ppar, pa, up + down1);
+            const float pair0 = dmp_at(s, ck.ppar, pa, at) +
+                                (low_pa ? 0.0f : dmp_at(s, ck.ppar, pa, at + down0));
             const float prod0 = gp(U)[at] * pair0;
-            const float prod1 = gp(U)[up] * pair1;
-            const float summed = prod0 + prod1;
+            float prod1 = 0.0f;
+            if (!top_c) {
+                const float pair1 = dmp_at(s, ck.ppar, pa, up) +
+                                    (low_pa ? 0.0f : dmp_at(s, ck.ppar, pa, up + down1));
+                prod1 = gp(U)[up] * pair1;
+            }
+            float summed = prod0 + prod1;
+            if ((s.metal[(C + 1) % 3] && coord_of(q, (C + 1) % 3, m) == 0) ||
+                (s.metal[(C + 2) % 3] && coord_of(q, (C + 2) % 3, m) == 0))
+                summed = 0.0f;
             row = row + 0.25 * (double)summed;
         }
         cst.v[m] = (float)row;
@@ -2115,7 +2134,6 @@ int fdtd_set_boundary(fdtd_sim *s, int32_t axis, int32_t kind) {
     SimDev &h = s->host;
     if (kind == 1) {
         if (!h.present[axis]) return fail(FDTD_EINVAL, "axis %d is invariant: it has no wall", axis);
-        if (h.offdiag) return fail(FDTD_EINVAL, "off-diagonal chi1inv beside a metallic wall is not built");
     }
     h.metal[axis] = kind;
     s->graph.reset();
@@ -2246,8 +2264,6 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
     s->graph.reset();  // the off-diagonal path launches other kernels
     int rc = s->pull_desc();
     if (rc) return rc;
-    if (s->host.metal[0] || s->host.metal[1] || s->host.metal[2])
-        return fail(FDTD_EINVAL, "off-diagonal chi1inv beside a metallic wall is not built");
     if (!s->host.u[row][partner]) {
         float *p;
         rc = s->volume(&p);

@@ -113,8 +113,8 @@ class FdtdDriver:
         """Per-axis boundaries, ``{"x": "metallic", ...}`` (the reference driver's
         ``boundaries``): "periodic" (the default) or "metallic" -- perfect electric conductor
         walls on both faces of the axis (stepping.py:140-146). A PML or an absorber on a
-        metallic axis grades from the walls without a wrap; off-diagonal chi1inv beside a wall
-        is not built and is refused at the freeze."""
+        metallic axis grades from the walls without a wrap; an off-diagonal chi1inv row reads
+        zero past a wall and couples nothing on its planes (stepping.py:1594-1745)."""
         self._require_unfrozen("set_boundaries")
         kinds = {}
         for key, kind in (boundaries or {}).items():

@@ -45,8 +45,8 @@ class Case:
     metallic: Tuple[bool, bool, bool] = (False, False, False)
 
 
-#: The cases this engine steps: real storage, no symmetry; periodic or metallic axes (no
-#: absorber on a metallic one); conductivity everywhere (every one-rank chunk keeps it).
+#: The fixture cases this engine steps: real storage, no symmetry; periodic or metallic axes;
+#: conductivity everywhere (every one-rank chunk keeps it).
 CASES: Tuple[Case, ...] = (
     Case("pml_3d", (1.5, 1.25, 1.0), pml={"x": (2, 3), "y": (2.5, 2), "z": (0, 2)}),
     Case("pml_2d", (2.0, 1.5, 0.0), dimensions=2, pml={"x": (2.5, 1.5), "y": (1.75, 3)}),
@@ -63,7 +63,16 @@ CASES: Tuple[Case, ...] = (
          metallic=(True, False, False)),
 )
 
-CASE_BY_NAME = {case.name: case for case in CASES}
+#: Cases of this build's own, in the fixture cases' form (seeded the same way, by name) but with
+#: no recorded digest: the engine is checked against the oracle word for word on them.
+#: ``offdiag_pml_metallic_3d`` puts MEEP's default metallic walls around an off-diagonal
+#: chi1inv and a PML on every face, two of them metallic.
+VARIANTS: Tuple[Case, ...] = (
+    Case("offdiag_pml_metallic_3d", (1.5, 1.25, 1.25), pml=2, offdiagonal=True,
+         metallic=(True, False, True)),
+)
+
+CASE_BY_NAME = {case.name: case for case in CASES + VARIANTS}
 
 
 @dataclass

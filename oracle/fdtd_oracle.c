@@ -271,18 +271,30 @@ void oracle_update_e(const ogrid *g, const oelectric *x, float *const e[3], floa
                             const int pa = (c + o) % 3;
                             const float *U = x->u[c][pa];
                             if (!U) continue;
-                            /* product(q) = u[q] * (g[q] + g[q - s_partner]) */
+                            /* product(q) = u[q] * (g[q] + g[q - s_partner]); past a metallic
+                             * wall the partner's lower neighbour and the product above read
+                             * the shift helpers' zero ghost, and the summed term is zero on a
+                             * wall plane of a metallic axis where this component's Yee shift
+                             * is 0 (stepping.py:1594-1687, _mask_metallic_wall_coupling
+                             * :1718-1745) */
+                            const int low_pa = g->metal[pa] && ic[pa] == 0;
+                            const int top_c = g->metal[c] && ic[c] == g->n[c] - 1;
                             const int64_t up = nb(g, i, j, k, c, +1);
                             int64_t q0 = at, q1 = up;
                             int64_t q0m = nb(g, i, j, k, pa, -1);
                             int iu[3] = {i, j, k};
                             iu[c] = (iu[c] + 1) % g->n[c];
                             int64_t q1m = nb(g, iu[0], iu[1], iu[2], pa, -1);
-                            float pair0 = dmp(x, d, pa, q0) + dmp(x, d, pa, q0m);
-                            float pair1 = dmp(x, d, pa, q1) + dmp(x, d, pa, q1m);
+                            float pair0 = dmp(x, d, pa, q0) + (low_pa ? 0.0f : dmp(x, d, pa, q0m));
                             float prod0 = U[q0] * pair0;
-                            float prod1 = U[q1] * pair1;
+                            float prod1 = 0.0f;
+                            if (!top_c) {
+                                float pair1 = dmp(x, d, pa, q1) + (low_pa ? 0.0f : dmp(x, d, pa, q1m));
+                                prod1 = U[q1] * pair1;
+                            }
                             float summed = prod0 + prod1;
+                            for (int a = 0; a < 3; ++a)
+                                if (a != c && g->metal[a] && ic[a] == 0) summed = 0.0f;
                             row = row + 0.25 * (double)summed;
                         }
                         cst = (float)row;

@@ -24,7 +24,8 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
     grid = Grid(resolution=fixture.RESOLUTION, cell_size=case.cell, courant=fixture.COURANT,
                 dimensions=case.dimensions)
     assert grid.shape == state.shape, (grid.shape, state.shape)
-    pml = PML(grid, case.pml, ln_reflection=fixture.PML_LN_REFLECTION) if case.pml else None
+    pml = (PML(grid, case.pml, ln_reflection=fixture.PML_LN_REFLECTION,
+               wraps=tuple(not m for m in case.metallic)) if case.pml else None)
     sim = HipSimulation(grid, pml)
     for c, vol in state.inv.items():
         sim.set_inverse_epsilon(c, vol)

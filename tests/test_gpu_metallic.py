@@ -1,7 +1,8 @@
 """Metallic (PEC) walls on the GPU against the oracle: 3-D sphere and slab runs with walls on
 x and an absorber on y and z, and a sphere with the absorber on x too -- MEEP's PML inside
 metallic walls, graded from the walls with no wrap (fields word for word, spectra to 1e-9 of
-their peak) -- and the refusal the engine makes by name."""
+their peak) -- and the refusal the engine makes by name. Off-diagonal chi1inv inside walls is
+checked against the oracle in test_gpu_parity.py."""
 
 import numpy
 import pytest
@@ -60,14 +61,14 @@ def test_metallic_walls_match_the_oracle(name, layer_on_walls):
 
 
 def test_what_metallic_walls_refuse():
+    """An invariant axis has no wall (the engine's own refusal, beneath the driver's)."""
     from meep_gpu._native import EngineError
-    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.engine import HipSimulation
+    from meep_gpu.grid import Grid
 
-    wl = wr.small("sphere")
-    wl.pml_faces = {"y": 6, "z": 6}
-    drv = FdtdDriver.from_workload(wl)
-    drv.set_boundaries({"x": "metallic"})
-    drv.set_epsilon_components({c: 2.0 for c in ("Ex", "Ey", "Ez")},
-                               chi1inv_offdiagonal={"Ex": {"Ey": 0.01}})
-    with pytest.raises(EngineError, match="off-diagonal"):
-        drv.freeze(planned_steps=4)
+    sim = HipSimulation(Grid(resolution=4, cell_size=(4.0, 4.0, 0.0), dimensions=2))
+    try:
+        with pytest.raises(EngineError, match="invariant"):
+            sim.set_boundary(2, "metallic")
+    finally:
+        sim.close()
