@@ -161,8 +161,16 @@ int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
  * (double, rounded once) and, under an absorber, an f_cond history ("f_cond_Dx" .. "f_cond_Bz"
  * for fdtd_upload/fdtd_download). The curl of that component then takes MEEP's conductive forms.
  * Replaces Fields.set_d_conductivity / set_b_conductivity (reference fields.py:775-870) and
- * stepping._apply_conductive_update / _apply_conductive_pml_update (stepping.py:2820-2983). */
+ * stepping._apply_conductive_update / _apply_conductive_pml_update (stepping.py:2820-2983).
+ * A non-finite sigma is refused. */
 int fdtd_set_conductivity(fdtd_sim *sim, int32_t field, int32_t axis, const float *sigma);
+
+/* Confine a conductivity set above to the cells whose structure chunk keeps it: cells[q] 0
+ * (x fastest, nx*ny*nz bytes) steps the lossless forms there and deposits without the condinv
+ * factor, as MEEP does in a chunk whose conductivity samples are all zero (structure.cpp:
+ * 883-900). Replaces ChunkLayout.conductive_mask (reference chunk_map.py:720-755) as the
+ * stepping reads it. Call after fdtd_set_conductivity for the component. */
+int fdtd_set_conductive_cells(fdtd_sim *sim, int32_t field, int32_t axis, const uint8_t *cells);
 
 /* A DFT field region, MEEP's add_dft_fields (DFTMonitor, reference dft.py:1960; the region's
  * block as FdtdDriver.get_dft returns it): the cells [lo, lo+cnt) of the grid, ncomp (1..4)

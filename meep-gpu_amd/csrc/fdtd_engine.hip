@@ -429,7 +429,10 @@ __device__ inline void deposit_run(const SimDev &s, int step, int field, int com
             r = r - gp(d.amp_im)[p] * cim;
             r = r * d.dt;
             if constexpr (GEN) {
-                if (ci) r = r * (double)gp(ci)[q.at + m];
+                if (ci) {  // a 0 word: the cell's chunk keeps no conductivity (MEEP's NULL)
+                    const float cw = gp(ci)[q.at + m];
+                    if (cw != 0.0f) r = r * (double)cw;
+                }
             }
             double w = (double)x.v[m];
             w = w - r;
@@ -515,11 +518,13 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
                              en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
         if constexpr (GEN) {
             if (on_wall(s, q, C, m)) curl = 0.0f;
-            if (s.cf[1][C]) {
+            // a condinv word of 0 marks a cell whose chunk keeps no conductivity array: it
+            // steps the lossless forms (structure.cpp:883-900)
+            const float civ = s.cf[1][C] ? gp(s.ci[1][C])[q.at + m] : 0.0f;
+            if (civ != 0.0f) {
                 const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
                 const bool in_s = PML && member(s, 1, dsig, cs), in_u = PML && member(s, 1, dsigu, cu);
-                b.v[m] = curl_apply_cond(b.v[m], curl, gp(s.cf[1][C])[q.at + m],
-                                         gp(s.ci[1][C])[q.at + m], in_s, in_u,
+                b.v[m] = curl_apply_cond(b.v[m], curl, gp(s.cf[1][C])[q.at + m], civ, in_s, in_u,
                                          coeff(s, 1, dsig, cs, in_s), coeff(s, 1, dsigu, cu, in_u),
                                          s.fuB[C], s.fcond[1][C], q.at + m);
                 continue;
@@ -624,11 +629,11 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
                              hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
         if constexpr (GEN) {
             if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m)) curl = 0.0f;
-            if (s.cf[0][C]) {
+            const float civ = s.cf[0][C] ? gp(s.ci[0][C])[q.at + m] : 0.0f;
+            if (civ != 0.0f) {  // 0: the cell's chunk keeps no conductivity (lossless forms)
                 const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
                 const bool in_s = PML && member(s, 0, dsig, cs), in_u = PML && member(s, 0, dsigu, cu);
-                d.v[m] = curl_apply_cond(d.v[m], curl, gp(s.cf[0][C])[q.at + m],
-                                         gp(s.ci[0][C])[q.at + m], in_s, in_u,
+                d.v[m] = curl_apply_cond(d.v[m], curl, gp(s.cf[0][C])[q.at + m], civ, in_s, in_u,
                                          coeff(s, 0, dsig, cs, in_s), coeff(s, 0, dsigu, cu, in_u),
                                          s.fuD[C], s.fcond[0][C], q.at + m);
                 continue;
@@ -2150,7 +2155,10 @@ int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *si
     SimDev &h = s->host;
     const size_t ncell = (size_t)h.ncell;
     bool any = false;
-    for (size_t q = 0; sigma && q < ncell && !any; ++q) any = sigma[q] != 0.0f;
+    for (size_t q = 0; sigma && q < ncell; ++q) {
+        if (!std::isfinite(sigma[q])) return fail(FDTD_EINVAL, "a conductivity must be finite");
+        any |= sigma[q] != 0.0f;
+    }
     s->graph.reset();
     if (!any) {  // lossless: the plain forms, word for word (the volumes stay allocated)
         h.cf[field][c] = h.ci[field][c] = nullptr;
@@ -2187,6 +2195,28 @@ int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *si
     h.cond = 0;
     for (int f = 0; f < 2; ++f)
         for (int a = 0; a < 3; ++a) h.cond |= h.cf[f][a] != nullptr;
+    s->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_set_conductive_cells(fdtd_sim *s, int32_t field, int32_t c, const uint8_t *cells) {
+    if (!s || (field != 0 && field != 1) || c < 0 || c > 2 || !cells)
+        return fail(FDTD_EINVAL, "fdtd_set_conductive_cells: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int rc = s->pull_desc();
+    if (rc) return rc;
+    SimDev &h = s->host;
+    if (!h.ci[field][c])
+        return fail(FDTD_ESTATE, "%c%c has no conductivity to confine", field ? 'B' : 'D', 'x' + c);
+    // a condinv word of 0 (no finite sigma gives it) marks a cell stepped by the lossless forms
+    std::vector<float> ci((size_t)h.ncell);
+    rc = download_volume(s, ci.data(), h.ci[field][c]);
+    if (rc) return rc;
+    for (size_t q = 0; q < ci.size(); ++q)
+        if (!cells[q]) ci[q] = 0.0f;
+    rc = upload_volume(s, (float *)h.ci[field][c], ci.data());
+    if (rc) return rc;
+    s->graph.reset();
     s->dirty = true;
     return FDTD_OK;
 }

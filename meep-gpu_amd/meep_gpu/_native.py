@@ -58,6 +58,8 @@ SIGNATURES = {
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_conductivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                              _FP]),
+    "fdtd_set_conductive_cells": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.c_int32, _U8P]),
     "fdtd_add_dft_region": (ctypes.c_int, [ctypes.c_void_p, _IP, _IP, ctypes.c_int32, _IP,
                                            ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_double,
                                            ctypes.c_int32, _IP]),

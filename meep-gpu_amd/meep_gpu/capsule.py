@@ -42,6 +42,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy
 
 from . import _native
+from .chunks import IYEE, ChunkDivision, Frame
 from .grid import Grid
 
 CAPSULE_SCHEMA = ("meep_gpu.capsule", 1)
@@ -215,6 +216,7 @@ class LiftedState:
     step: int
     h_separated: bool
     absent: Tuple[str, ...]
+    division: ChunkDivision = None  # the lifted chunk division (per-chunk conductivity)
 
     def grid(self) -> Grid:
         """The engine grid; refuses when its dx, dt or cell counts are not the lifted words."""
@@ -390,8 +392,11 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         for c, value in state["P_prev"].items():
             volumes[f"P_prev[{k}].{c}"] = _volume(value)
 
+    division = ChunkDivision(
+        tuple(Frame(*_axis_frame(frame)) for frame in layout["frames"]),
+        [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]])
     return LiftedState(
-        shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
+        division=division, shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
         courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),
         invariant=invariant, metallic=tuple(metallic), inv_eps=inv_eps, pml=pml,
         conductivity=conductivity, condfac=side(fields["_condfac"]),
@@ -496,7 +501,7 @@ class LiftedRun:
         for pole in state.poles:
             sim.add_pole(None, pole.sigma, constants=pole.constants)
         for name, vol in state.conductivity.items():
-            sim.set_conductivity(name, vol)
+            sim.set_conductivity(name, vol, state.division.conductive_mask(vol, IYEE[name]))
         for src, table in zip(state.sources, currents):
             axis = "xyz".index(src.component[1])
             for lo, amp in _source_boxes(src):

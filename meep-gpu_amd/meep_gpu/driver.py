@@ -20,6 +20,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy
 
 from .absorber import AbsorberLayer, absorber_conductivity, layers_for
+from .chunks import IYEE, ChunkDivision
 from .dft import DftFieldsMonitor, FluxMonitor, meep_frequency_span
 from .dispersion import Susceptibility
 from .engine import HipSimulation
@@ -296,8 +297,11 @@ class FdtdDriver:
                 sim.set_offdiagonal(row, partner, vol)
         for term, sigma in self._poles:
             sim.add_pole(term, sigma)
+        division = ChunkDivision.one_rank(
+            self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)), self.wraps)
         for name, vol in self.conductivity_volumes().items():
-            sim.set_conductivity(name, vol)
+            # MEEP keeps a chunk's conductivity only where it samples a nonzero sigma
+            sim.set_conductivity(name, vol, division.conductive_mask(vol, IYEE[name]))
         for box, fn in self._sources:
             table = current_table(fn, self.grid.dt, steps, box.magnetic)
             axis = "xyz".index(box.component[1])

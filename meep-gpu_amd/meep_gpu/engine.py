@@ -87,9 +87,12 @@ class HipSimulation:
         check(self.lib.fdtd_set_boundary(self.handle, int(axis), kinds.index(kind)),
               "fdtd_set_boundary")
 
-    def set_conductivity(self, component: str, volume: Optional[numpy.ndarray]):
+    def set_conductivity(self, component: str, volume: Optional[numpy.ndarray],
+                         cells: Optional[numpy.ndarray] = None):
         """MEEP's D or B conductivity of one component ("Dx".."Bz"): sigma as a float32 volume
-        in the reference layout, or None / all zero for none (``fdtd_set_conductivity``)."""
+        in the reference layout, or None / all zero for none (``fdtd_set_conductivity``).
+        ``cells`` (bool, reference layout) confines it to the cells whose chunk keeps it
+        (``fdtd_set_conductive_cells``); None keeps it everywhere."""
         if len(component) != 2 or component[0] not in "DB" or component[1] not in _AXES:
             raise ValueError(f"a conductivity belongs to Dx..Dz or Bx..Bz, not {component!r}")
         dev = None if volume is None else to_device_layout(
@@ -98,6 +101,14 @@ class HipSimulation:
                                              _AXES.index(component[1]),
                                              fptr(dev) if dev is not None else None),
               "fdtd_set_conductivity")
+        if cells is not None and dev is not None and numpy.any(dev):
+            mask = numpy.ascontiguousarray(numpy.broadcast_to(
+                numpy.asarray(cells, dtype=bool), self.grid.shape).astype(numpy.uint8)
+                .transpose(2, 1, 0))  # x fastest
+            check(self.lib.fdtd_set_conductive_cells(self.handle,
+                                                     0 if component[0] == "D" else 1,
+                                                     _AXES.index(component[1]), u8ptr(mask)),
+                  "fdtd_set_conductive_cells")
 
     def pole_row_fraction(self) -> float:
         """Fraction of the grid's cells inside the per-row ranges where pole work runs."""

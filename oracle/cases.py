@@ -26,10 +26,13 @@ from typing import Any, Callable, Dict, List, Optional, Tuple
 
 import numpy
 
-from . import fixture, layer
+from . import chunks, fixture, layer
 from .engine import OracleSim, PointDeposit
 
 _E = ("Ex", "Ey", "Ez")
+#: Yee shifts of the conductive components (D like E, B like H; MEEP's iyee_shift)
+_IYEE = {"Dx": (1, 0, 0), "Dy": (0, 1, 0), "Dz": (0, 0, 1),
+         "Bx": (0, 1, 1), "By": (1, 0, 1), "Bz": (1, 1, 0)}
 
 
 @dataclass(frozen=True)
@@ -43,10 +46,13 @@ class Case:
     d_conductivity: bool = False
     b_conductivity: bool = False
     metallic: Tuple[bool, bool, bool] = (False, False, False)
+    #: A conductivity kept only inside this block of fractions of each axis
+    #: (array_path_oracle.py:626-633).
+    conductivity_block: Any = None
 
 
 #: The fixture cases this engine steps: real storage, no symmetry; periodic or metallic axes;
-#: conductivity everywhere (every one-rank chunk keeps it).
+#: a conductivity everywhere or in a block (kept per one-rank chunk).
 CASES: Tuple[Case, ...] = (
     Case("pml_3d", (1.5, 1.25, 1.0), pml={"x": (2, 3), "y": (2.5, 2), "z": (0, 2)}),
     Case("pml_2d", (2.0, 1.5, 0.0), dimensions=2, pml={"x": (2.5, 1.5), "y": (1.75, 3)}),
@@ -61,6 +67,11 @@ CASES: Tuple[Case, ...] = (
          b_conductivity=True),
     Case("metallic_2d", (1.5, 1.25, 0.0), dimensions=2, pml={"y": 2},
          metallic=(True, False, False)),
+    # the block conductivity reaches the interior chunk, the x-high and y-high edge chunks
+    # and their corner; the other five chunks keep none and step the lossless PML forms
+    Case("conductive_chunks_pml_2d", (2.0, 2.0, 0.0), dimensions=2, pml={"x": 2, "y": 2},
+         d_conductivity=True, b_conductivity=True, metallic=(True, True, False),
+         conductivity_block=((0.5, 1.0), (0.5, 1.0), (0.0, 1.0))),
 )
 
 #: Cases of this build's own, in the fixture cases' form (seeded the same way, by name) but with
@@ -100,6 +111,17 @@ def _grid_shape(case: Case) -> Tuple[Tuple[int, int, int], Tuple[bool, bool, boo
     return shape, invariant
 
 
+def _within(case: Case, values: numpy.ndarray) -> numpy.ndarray:
+    """``values`` inside the case's conductivity block, exact zero outside
+    (array_path_oracle.py:626-633: the block's rows are ``round(fraction * n)``)."""
+    if case.conductivity_block is None:
+        return values
+    kept = numpy.zeros(values.shape, dtype=bool)
+    kept[tuple(slice(int(round(lo * n)), int(round(hi * n)))
+               for (lo, hi), n in zip(case.conductivity_block, values.shape))] = True
+    return numpy.where(kept, values, numpy.float32(0.0)).astype(numpy.float32)
+
+
 def build_state(name: str) -> CaseState:
     case = CASE_BY_NAME[name]
     shape, invariant = _grid_shape(case)
@@ -129,8 +151,9 @@ def build_state(name: str) -> CaseState:
     for side, flag in (("D", case.d_conductivity), ("B", case.b_conductivity)):
         if flag:  # array_path_oracle.py:662-670
             for c in "xyz":
-                vol = fixture.volume(fixture.seed(name, "sigma_" + side.lower(), side + c),
-                                     shape, 0.0, 2.0, zero_fraction=0.3)
+                vol = _within(case, fixture.volume(
+                    fixture.seed(name, "sigma_" + side.lower(), side + c), shape, 0.0, 2.0,
+                    zero_fraction=0.3))
                 if numpy.any(vol):
                     conductivity[side + c] = vol
     faces = layer.pml_faces(case.pml, case.dimensions)
@@ -159,8 +182,12 @@ def oracle_sim(state: CaseState) -> OracleSim:
     sim.set_inverse_epsilon(state.inv, state.offdiag)
     for pole in state.poles:
         sim.add_pole(pole["frequency"], pole["gamma"], pole["drude"], pole["sigma"])
-    sim.set_conductivity(state.conductivity)
-    sim.set_metallic(CASE_BY_NAME[state.name].metallic)
+    case = CASE_BY_NAME[state.name]
+    cells = {n: chunks.conductive_cells(state.shape, state.invariant, case.metallic,
+                                        state.faces, vol, _IYEE[n])
+             for n, vol in state.conductivity.items()}
+    sim.set_conductivity(state.conductivity, cells)
+    sim.set_metallic(case.metallic)
     if state.stores_E:
         sim.enable_field_storage()
     for n, vol in state.volumes.items():

@@ -185,12 +185,15 @@ class OracleSim:
         self.enable_field_storage()
         return pole
 
-    def set_conductivity(self, sigma: Dict[str, numpy.ndarray]):
+    def set_conductivity(self, sigma: Dict[str, numpy.ndarray],
+                         cells: Optional[Dict[str, numpy.ndarray]] = None):
         """D and B conductivities by component ("Dx".."Bz"), as fields.set_d_conductivity /
         set_b_conductivity form their words (reference fields.py:775-870): condfac =
         1 - sigma*dt2 in float32 with dt2 = float32(dt)*0.5, condinv = 1/(1 + (sigma*dt)*0.5)
         in double from the float32 word, rounded once. An all-zero volume is no conductivity.
-        Under an absorber each conductive component gets an f_cond history (fields.py:754)."""
+        Under an absorber each conductive component gets an f_cond history (fields.py:754).
+        ``cells[name]``, a bool volume, marks the cells whose chunk keeps the array; the others
+        step the lossless forms (structure.cpp:883-900), held as a condinv word of 0."""
         dt2 = numpy.float32(numpy.float32(self.dt) * numpy.float32(0.5))
         for name, vol in sigma.items():
             vol = numpy.ascontiguousarray(vol, dtype=numpy.float32)
@@ -199,6 +202,8 @@ class OracleSim:
             self.condfac[name] = (numpy.float32(1.0) - vol * dt2).astype(numpy.float32)
             wide = vol.astype(numpy.float64)
             self.condinv[name] = (1.0 / (1.0 + (wide * self.dt) * 0.5)).astype(numpy.float32)
+            if cells is not None and name in cells:
+                self.condinv[name][~numpy.asarray(cells[name], dtype=bool)] = 0.0
             if self.pml:
                 self.v["f_cond_" + name] = numpy.zeros(self.shape, dtype=numpy.float32)
 

@@ -119,7 +119,10 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
                         const int shift0 = family == 0 ? (a == c) : (a != c);
                         if (g->metal[a] && shift0 && ic[a] == 0) curl = 0.0f;
                     }
-                    if (CF) {
+                    /* a condinv word of 0 (no finite sigma gives it) marks a cell whose chunk
+                     * keeps no conductivity array: it steps the lossless forms below
+                     * (structure.cpp:883-900) */
+                    if (CF && CI[at] != 0.0f) {
                         /* X = curl, cf, ci: MEEP's eight step_curl forms reduce, per cell,
                          * to the four below (step_generic.cpp:84-210 as the reference
                          * restates them); the increment fcnd' - fcnd is formed first */
@@ -344,14 +347,15 @@ void oracle_update_p(const ogrid *g, int c, const float consts[4], const float *
 
 /* f[idx] = float(double(f[idx]) - ((amp_re*c_re - amp_im*c_im) * dt) [* double(condinv)])
  * at each point; condinv (the written component's conductivity word, sources.py:1357-1400,
- * step.cpp:299-307) is null where that component is lossless. */
+ * step.cpp:299-307) is null where that component is lossless, and 0 at a cell whose chunk
+ * keeps no conductivity array (MEEP's NULL there). */
 void oracle_deposit(float *f, const int64_t *idx, const double *amp_re, const double *amp_im,
                     int n, double c_re, double c_im, double dt, const float *condinv) {
     for (int q = 0; q < n; ++q) {
         double r = amp_re[q] * c_re;
         r = r - amp_im[q] * c_im;
         r = r * dt;
-        if (condinv) r = r * (double)condinv[idx[q]];
+        if (condinv && condinv[idx[q]] != 0.0f) r = r * (double)condinv[idx[q]];
         double w = (double)f[idx[q]];
         w = w - r;
         f[idx[q]] = (float)w;

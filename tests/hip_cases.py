@@ -12,6 +12,7 @@ from typing import Dict, Tuple
 
 import numpy
 
+from meep_gpu.chunks import IYEE, ChunkDivision
 from meep_gpu.dispersion import DRUDE, LORENTZIAN, Susceptibility
 from meep_gpu.engine import HipSimulation
 from meep_gpu.grid import Grid
@@ -36,8 +37,10 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
         term = Susceptibility(pole["frequency"], pole["gamma"],
                               DRUDE if pole["drude"] else LORENTZIAN)
         sim.add_pole(term, pole["sigma"])
+    division = ChunkDivision.one_rank(grid, pml.faces if pml else ((0, 0),) * 3,
+                                      tuple(not m for m in case.metallic))
     for comp, vol in state.conductivity.items():
-        sim.set_conductivity(comp, vol)
+        sim.set_conductivity(comp, vol, division.conductive_mask(vol, IYEE[comp]))
     for axis, metal in enumerate(case.metallic):
         if metal:
             sim.set_boundary(axis, "metallic")

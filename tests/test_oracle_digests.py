@@ -38,8 +38,8 @@ def test_a_planted_conductivity_mistake_moves_the_digests(golden_digests, monkey
 
     original = engine.OracleSim.set_conductivity
 
-    def planted(self, sigma):
-        original(self, sigma)
+    def planted(self, sigma, cells=None):
+        original(self, sigma, cells)
         for name, vol in sigma.items():
             if name not in self.condinv:
                 continue
@@ -55,3 +55,18 @@ def test_a_planted_conductivity_mistake_moves_the_digests(golden_digests, monkey
     monkeypatch.setattr(engine.OracleSim, "set_conductivity", planted)
     for name in ("conductive_3d", "conductive_pml_3d"):
         assert cases.run_case(name) != golden_digests["case_sha256"][name], (mistake, name)
+
+
+def test_keeping_the_conductivity_in_every_chunk_moves_the_block_digest(golden_digests,
+                                                                        monkeypatch):
+    """conductive_chunks_pml_2d: stepping the conductive forms in the five chunks that keep no
+    conductivity (sigma 0 there, so only the PML forms' rounding differs) is caught."""
+    import numpy
+
+    from oracle import chunks
+
+    name = "conductive_chunks_pml_2d"
+    assert cases.run_case(name) == golden_digests["case_sha256"][name]
+    monkeypatch.setattr(chunks, "conductive_cells",
+                        lambda shape, *a, **k: numpy.ones(tuple(shape), dtype=bool))
+    assert cases.run_case(name) != golden_digests["case_sha256"][name]
