@@ -165,6 +165,13 @@ int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
  * A non-finite sigma is refused. */
 int fdtd_set_conductivity(fdtd_sim *sim, int32_t field, int32_t axis, const float *sigma);
 
+/* The subnormal policy this library was built with (a build-time flag, the reference's
+ * floating-point contract, docs/design/floating-point.md): 0 keep (libmeep_fdtd.so), 1 flush
+ * (libmeep_fdtd_flush.so: float32 and float64 subnormal inputs read as zero and subnormal
+ * results written as zero, sign kept -- x86's FTZ|DAZ, under which MEEP's flush identity set
+ * is minted, meep_identity.py:43-52). No GPU call. */
+int fdtd_subnormal_mode(void);
+
 /* Confine a conductivity set above to the cells whose structure chunk keeps it: cells[q] 0
  * (x fastest, nx*ny*nz bytes) steps the lossless forms there and deposits without the condinv
  * factor, as MEEP does in a chunk whose conductivity samples are all zero (structure.cpp:

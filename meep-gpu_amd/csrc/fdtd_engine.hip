@@ -19,6 +19,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+
+// 1 in the build whose kernels flush float32 and float64 subnormals (denormal modes at flush)
+#ifndef FDTD_FLUSH_SUBNORMALS
+#define FDTD_FLUSH_SUBNORMALS 0
+#endif
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -2198,6 +2203,8 @@ int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *si
     s->dirty = true;
     return FDTD_OK;
 }
+
+int fdtd_subnormal_mode(void) { return FDTD_FLUSH_SUBNORMALS; }
 
 int fdtd_set_conductive_cells(fdtd_sim *s, int32_t field, int32_t c, const uint8_t *cells) {
     if (!s || (field != 0 && field != 1) || c < 0 || c > 2 || !cells)

@@ -9,10 +9,13 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional
+from typing import Dict, Optional
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libmeep_fdtd.so")
+#: The library per subnormal policy: kept (the default) or flushed (float32 and float64).
+LIB_PATHS = {"keep": LIB_PATH, "flush": os.path.join(_HERE, "_lib", "libmeep_fdtd_flush.so")}
+SUBNORMAL_MODES = ("keep", "flush")
 
 _FP = ctypes.POINTER(ctypes.c_float)
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -24,6 +27,7 @@ FDTD_MAX_POLES = 8
 #: Every entry point include/meep_fdtd.h declares, with its ctypes signature.
 SIGNATURES = {
     "fdtd_last_error": (ctypes.c_char_p, []),
+    "fdtd_subnormal_mode": (ctypes.c_int, []),
     "fdtd_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fdtd_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(ctypes.c_uint64)]),
@@ -96,14 +100,17 @@ class EngineError(RuntimeError):
     """A failed engine call, with the engine's own reason."""
 
 
-_lib: Optional[ctypes.CDLL] = None
+_libs: Dict[str, ctypes.CDLL] = {}
 
 
-def load(path: str = LIB_PATH) -> ctypes.CDLL:
-    """Load the engine library and bind every declared symbol (no GPU call is made)."""
-    global _lib
-    if _lib is not None:
-        return _lib
+def load(path: Optional[str] = None, subnormals: str = "keep") -> ctypes.CDLL:
+    """Load the engine library of one subnormal policy ("keep", the default, or "flush") and
+    bind every declared symbol (no GPU call is made)."""
+    if subnormals not in SUBNORMAL_MODES:
+        raise ValueError(f"subnormals must be one of {SUBNORMAL_MODES}, got {subnormals!r}")
+    path = path or LIB_PATHS[subnormals]
+    if path in _libs:
+        return _libs[path]
     if not os.path.exists(path):
         raise EngineError(
             f"the HIP engine library is not built ({path} is missing); run "
@@ -113,13 +120,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
-    _lib = lib
+    want = SUBNORMAL_MODES.index(subnormals)
+    if path == LIB_PATHS[subnormals] and lib.fdtd_subnormal_mode() != want:
+        raise EngineError(f"{path} was not built with subnormals {subnormals!r}")
+    _libs[path] = lib
     return lib
 
 
-def check(rc: int, what: str) -> None:
+def check(rc: int, what: str, lib: Optional[ctypes.CDLL] = None) -> None:
+    """Raise ``EngineError`` with the engine's reason when ``rc`` is not 0; ``lib`` is the
+    library the call went to (the default build when None)."""
     if rc != 0:
-        text = load().fdtd_last_error().decode("utf-8", "replace")
+        text = (lib or load()).fdtd_last_error().decode("utf-8", "replace")
         raise EngineError(f"{what}: {text} (code {rc})")
 
 

@@ -33,10 +33,13 @@ class BatchedRun:
     ``HipSimulation`` as ``.sim`` and its grid as ``.grid`` (a ``capsule.LiftedRun``)."""
 
     def __init__(self, drivers: Sequence[FdtdDriver]):
-        self.lib = _native.load()
         self.drivers = list(drivers)
         if not self.drivers:
             raise ValueError("a batch needs at least one simulation")
+        libs = {id(d.sim.lib) for d in self.drivers if d.sim is not None}
+        if len(libs) > 1:
+            raise ValueError("a batch steps simulations of one subnormal policy")
+        self.lib = self.drivers[0].sim.lib if self.drivers[0].sim is not None else _native.load()
         handles = (ctypes.c_void_p * len(self.drivers))()
         for q, drv in enumerate(self.drivers):
             if drv.sim is None:
@@ -44,7 +47,7 @@ class BatchedRun:
             handles[q] = drv.sim.handle.value
         out = ctypes.c_void_p()
         check(self.lib.fdtd_batch_create(handles, len(self.drivers), ctypes.byref(out)),
-              "fdtd_batch_create")
+              "fdtd_batch_create", self.lib)
         self.handle = out
 
     @property
@@ -52,13 +55,13 @@ class BatchedRun:
         return sum(d.grid.total_cells for d in self.drivers)
 
     def run(self, steps: int):
-        check(self.lib.fdtd_batch_run(self.handle, int(steps)), "fdtd_batch_run")
+        check(self.lib.fdtd_batch_run(self.handle, int(steps)), "fdtd_batch_run", self.lib)
 
     def synchronize(self):
-        check(self.lib.fdtd_batch_synchronize(self.handle), "fdtd_batch_synchronize")
+        check(self.lib.fdtd_batch_synchronize(self.handle), "fdtd_batch_synchronize", self.lib)
 
     def flush(self):
-        check(self.lib.fdtd_batch_flush(self.handle), "fdtd_batch_flush")
+        check(self.lib.fdtd_batch_flush(self.handle), "fdtd_batch_flush", self.lib)
 
     def close(self):
         """End the batch; members keep their state and step count and can be read or stepped."""

@@ -477,14 +477,16 @@ def _source_boxes(src: LiftedSource):
 
 
 class LiftedRun:
-    """A lifted state resident on the GPU: every word pasted, stepped by the engine."""
+    """A lifted state resident on the GPU: every word pasted, stepped by the engine built with
+    the given subnormal policy ("keep" or "flush", the identity set's ``mode``)."""
 
-    def __init__(self, state: LiftedState, currents: Sequence[numpy.ndarray]):
+    def __init__(self, state: LiftedState, currents: Sequence[numpy.ndarray],
+                 subnormals: str = "keep"):
         from .engine import HipSimulation
 
         self.state = state
         grid = state.grid()
-        self.sim = HipSimulation(grid, state.pml)
+        self.sim = HipSimulation(grid, state.pml, subnormals=subnormals)
         try:
             self._configure(state, currents)
         except Exception:
@@ -551,15 +553,18 @@ def compare_step(run, digests: DigestSet, step: int) -> List[str]:
 
 def replay(capsule_dir, max_poles: int = _native.FDTD_MAX_POLES) -> Dict[int, List[str]]:
     """Step a MEEP-identity case (``capsule.capsule``, ``currents.json``, ``digests.json`` in
-    one folder) on the GPU and judge MEEP's stored words at every compared step. Returns the
-    differing components per compared step (empty lists when every word is MEEP's)."""
+    one folder) on the GPU, with the subnormal policy its digests were minted under, and judge
+    MEEP's stored words at every compared step. Returns the differing components per compared
+    step (empty lists when every word is MEEP's)."""
     folder = Path(capsule_dir)
     state = read_capsule(folder / "capsule.capsule", max_poles=max_poles)
     digests = read_digests(folder / "digests.json")
     if any(digests.ghost_cells.values()):
         raise CapsuleRefusal("the digests compare a folded grid's cells")
     currents = read_currents(folder / "currents.json", state, digests.n_steps)
-    run = LiftedRun(state, currents)
+    if digests.mode not in ("keep", "flush"):
+        raise CapsuleRefusal(f"digests minted under subnormal mode {digests.mode!r}")
+    run = LiftedRun(state, currents, subnormals=digests.mode)
     try:
         out = {}
         for step in digests.steps:

@@ -33,8 +33,10 @@ def from_device_layout(buffer: numpy.ndarray) -> numpy.ndarray:
 class HipSimulation:
     """A handle on device-resident fields, layer table, materials, poles, sources, monitors."""
 
-    def __init__(self, grid: Grid, pml: Optional[PML] = None):
-        self.lib = _native.load()
+    def __init__(self, grid: Grid, pml: Optional[PML] = None, subnormals: str = "keep"):
+        """``subnormals``: the build to step with, "keep" (the default) or "flush"."""
+        self.lib = _native.load(subnormals=subnormals)
+        self.subnormals = subnormals
         self.grid = grid
         self.pml = pml if (pml is not None and pml.is_active) else None
         cfg = _native.FdtdConfig()
@@ -45,15 +47,18 @@ class HipSimulation:
         cfg.pml = 1 if self.pml is not None else 0
         cfg.dt = grid.dt
         handle = ctypes.c_void_p()
-        check(self.lib.fdtd_sim_create(ctypes.byref(cfg), ctypes.byref(handle)), "fdtd_sim_create")
+        self._check(self.lib.fdtd_sim_create(ctypes.byref(cfg), ctypes.byref(handle)), "fdtd_sim_create")
         self.handle = handle
         self.poles = []
         self.planes = []
         if self.pml is not None:
             for (half, axis), (kms, sinv, kps) in self.pml.vectors.items():
                 member = self.pml.members[(half, axis)].astype(numpy.uint8)
-                check(self.lib.fdtd_set_layer(self.handle, half, axis, fptr(kms), fptr(sinv),
+                self._check(self.lib.fdtd_set_layer(self.handle, half, axis, fptr(kms), fptr(sinv),
                                               fptr(kps), u8ptr(member)), "fdtd_set_layer")
+
+    def _check(self, rc: int, what: str) -> None:
+        check(rc, what, self.lib)
 
     # --- lifetime ----------------------------------------------------------------------------
     def close(self):
@@ -70,13 +75,13 @@ class HipSimulation:
     @property
     def device_bytes(self) -> int:
         out = ctypes.c_uint64()
-        check(self.lib.fdtd_sim_bytes(self.handle, ctypes.byref(out)), "fdtd_sim_bytes")
+        self._check(self.lib.fdtd_sim_bytes(self.handle, ctypes.byref(out)), "fdtd_sim_bytes")
         return int(out.value)
 
     # --- materials ---------------------------------------------------------------------------
     def set_inverse_epsilon(self, component: str, volume: numpy.ndarray):
         dev = to_device_layout(volume)
-        check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
+        self._check(self.lib.fdtd_set_inverse_epsilon(self.handle, _AXES.index(component[1]),
                                                 fptr(dev)), "fdtd_set_inverse_epsilon")
 
     def set_boundary(self, axis: int, kind: str):
@@ -84,7 +89,7 @@ class HipSimulation:
         kinds = ("periodic", "metallic")
         if kind not in kinds:
             raise ValueError(f"boundary kind must be one of {kinds}, got {kind!r}")
-        check(self.lib.fdtd_set_boundary(self.handle, int(axis), kinds.index(kind)),
+        self._check(self.lib.fdtd_set_boundary(self.handle, int(axis), kinds.index(kind)),
               "fdtd_set_boundary")
 
     def set_conductivity(self, component: str, volume: Optional[numpy.ndarray],
@@ -97,7 +102,7 @@ class HipSimulation:
             raise ValueError(f"a conductivity belongs to Dx..Dz or Bx..Bz, not {component!r}")
         dev = None if volume is None else to_device_layout(
             numpy.broadcast_to(numpy.asarray(volume, dtype=numpy.float32), self.grid.shape))
-        check(self.lib.fdtd_set_conductivity(self.handle, 0 if component[0] == "D" else 1,
+        self._check(self.lib.fdtd_set_conductivity(self.handle, 0 if component[0] == "D" else 1,
                                              _AXES.index(component[1]),
                                              fptr(dev) if dev is not None else None),
               "fdtd_set_conductivity")
@@ -105,7 +110,7 @@ class HipSimulation:
             mask = numpy.ascontiguousarray(numpy.broadcast_to(
                 numpy.asarray(cells, dtype=bool), self.grid.shape).astype(numpy.uint8)
                 .transpose(2, 1, 0))  # x fastest
-            check(self.lib.fdtd_set_conductive_cells(self.handle,
+            self._check(self.lib.fdtd_set_conductive_cells(self.handle,
                                                      0 if component[0] == "D" else 1,
                                                      _AXES.index(component[1]), u8ptr(mask)),
                   "fdtd_set_conductive_cells")
@@ -113,20 +118,20 @@ class HipSimulation:
     def pole_row_fraction(self) -> float:
         """Fraction of the grid's cells inside the per-row ranges where pole work runs."""
         active, rows = ctypes.c_int64(), ctypes.c_int64()
-        check(self.lib.fdtd_pole_rows(self.handle, ctypes.byref(active), ctypes.byref(rows)),
+        self._check(self.lib.fdtd_pole_rows(self.handle, ctypes.byref(active), ctypes.byref(rows)),
               "fdtd_pole_rows")
         return active.value / rows.value if rows.value else 0.0
 
     def inverse_epsilon_bytes(self) -> Tuple[int, int, int]:
         """Bytes per cell the D pair reads for each component's inverse epsilon (1, 2 or 4)."""
         out = numpy.zeros(3, dtype=numpy.int32)
-        check(self.lib.fdtd_inverse_epsilon_bytes(self.handle, iptr(out)),
+        self._check(self.lib.fdtd_inverse_epsilon_bytes(self.handle, iptr(out)),
               "fdtd_inverse_epsilon_bytes")
         return int(out[0]), int(out[1]), int(out[2])
 
     def set_offdiagonal(self, row: str, partner: str, volume: numpy.ndarray):
         dev = to_device_layout(volume)
-        check(self.lib.fdtd_set_offdiagonal(self.handle, _AXES.index(row[1]),
+        self._check(self.lib.fdtd_set_offdiagonal(self.handle, _AXES.index(row[1]),
                                             _AXES.index(partner[1]), fptr(dev)),
               "fdtd_set_offdiagonal")
 
@@ -155,7 +160,7 @@ class HipSimulation:
                 driven[a] = 1 if numpy.any(vol != 0.0) else 0
                 ptrs[a] = fptr(vol)
         index = numpy.zeros(1, dtype=numpy.int32)
-        check(self.lib.fdtd_add_pole(self.handle, fptr(consts), ptrs, fptr(scalars), iptr(driven),
+        self._check(self.lib.fdtd_add_pole(self.handle, fptr(consts), ptrs, fptr(scalars), iptr(driven),
                                      iptr(index)), "fdtd_add_pole")
         self.poles.append(tuple(bool(v) for v in driven))
         return self.poles[-1]
@@ -171,58 +176,58 @@ class HipSimulation:
         cur = numpy.ascontiguousarray(current, dtype=numpy.float64).reshape(-1, 2)
         lo_a = numpy.asarray(lo, dtype=numpy.int32)
         cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
-        check(self.lib.fdtd_add_source(self.handle, 1 if magnetic else 0, int(component),
+        self._check(self.lib.fdtd_add_source(self.handle, 1 if magnetic else 0, int(component),
                                        iptr(lo_a), iptr(cnt_a), dptr(re), dptr(im), dptr(cur),
                                        int(cur.shape[0]), float(dt)), "fdtd_add_source")
 
     # --- state -------------------------------------------------------------------------------
     def upload(self, name: str, array: numpy.ndarray):
         dev = to_device_layout(array)
-        check(self.lib.fdtd_upload(self.handle, name.encode(), fptr(dev), dev.size),
+        self._check(self.lib.fdtd_upload(self.handle, name.encode(), fptr(dev), dev.size),
               f"fdtd_upload({name})")
 
     def download(self, name: str) -> numpy.ndarray:
         nx, ny, nz = self.grid.shape
         buf = numpy.empty((nz, ny, nx), dtype=numpy.float32)
-        check(self.lib.fdtd_download(self.handle, name.encode(), fptr(buf), buf.size),
+        self._check(self.lib.fdtd_download(self.handle, name.encode(), fptr(buf), buf.size),
               f"fdtd_download({name})")
         return from_device_layout(buf)
 
     @property
     def step_count(self) -> int:
         out = numpy.zeros(1, dtype=numpy.int32)
-        check(self.lib.fdtd_get_step(self.handle, iptr(out)), "fdtd_get_step")
+        self._check(self.lib.fdtd_get_step(self.handle, iptr(out)), "fdtd_get_step")
         return int(out[0])
 
     def set_step(self, step: int, h_separated: bool):
-        check(self.lib.fdtd_set_step(self.handle, int(step), 1 if h_separated else 0),
+        self._check(self.lib.fdtd_set_step(self.handle, int(step), 1 if h_separated else 0),
               "fdtd_set_step")
 
     def prime_e(self):
-        check(self.lib.fdtd_prime_e(self.handle), "fdtd_prime_e")
+        self._check(self.lib.fdtd_prime_e(self.handle), "fdtd_prime_e")
 
     # --- stepping ----------------------------------------------------------------------------
     def step_h(self):
-        check(self.lib.fdtd_step_h(self.handle), "fdtd_step_h")
+        self._check(self.lib.fdtd_step_h(self.handle), "fdtd_step_h")
 
     def step_d(self):
-        check(self.lib.fdtd_step_d(self.handle), "fdtd_step_d")
+        self._check(self.lib.fdtd_step_d(self.handle), "fdtd_step_d")
 
     def run(self, steps: int):
-        check(self.lib.fdtd_run(self.handle, int(steps)), "fdtd_run")
+        self._check(self.lib.fdtd_run(self.handle, int(steps)), "fdtd_run")
 
     def synchronize(self):
-        check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
+        self._check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
 
     def run_timed(self, steps: int) -> Tuple[float, float, float]:
         """(total ms, mean H-pair ms, mean D-pair ms) of ``steps`` steps, HIP events per launch."""
         out = numpy.zeros(3, dtype=numpy.float32)
-        check(self.lib.fdtd_run_timed(self.handle, int(steps), fptr(out)), "fdtd_run_timed")
+        self._check(self.lib.fdtd_run_timed(self.handle, int(steps), fptr(out)), "fdtd_run_timed")
         return float(out[0]), float(out[1]), float(out[2])
 
     def time_pairs(self, repeats: int) -> Tuple[float, float]:
         out = numpy.zeros(2, dtype=numpy.float32)
-        check(self.lib.fdtd_time_pairs(self.handle, int(repeats), fptr(out)), "fdtd_time_pairs")
+        self._check(self.lib.fdtd_time_pairs(self.handle, int(repeats), fptr(out)), "fdtd_time_pairs")
         return float(out[0]), float(out[1])
 
     # --- flux --------------------------------------------------------------------------------
@@ -232,7 +237,7 @@ class HipSimulation:
         lo_a = numpy.asarray(lo, dtype=numpy.int32)
         cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
         out = numpy.zeros(1, dtype=numpy.int32)
-        check(self.lib.fdtd_add_flux_plane(self.handle, int(normal), iptr(lo_a), iptr(cnt_a),
+        self._check(self.lib.fdtd_add_flux_plane(self.handle, int(normal), iptr(lo_a), iptr(cnt_a),
                                            int(om.size), dptr(om), float(scale), int(decimation),
                                            iptr(out)), "fdtd_add_flux_plane")
         self.planes.append((int(normal), tuple(int(v) for v in lo), tuple(int(v) for v in cnt),
@@ -252,7 +257,7 @@ class HipSimulation:
         cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
         codes = numpy.asarray([names.index(c) for c in components], dtype=numpy.int32)
         out = numpy.zeros(1, dtype=numpy.int32)
-        check(self.lib.fdtd_add_dft_region(self.handle, iptr(lo_a), iptr(cnt_a), int(codes.size),
+        self._check(self.lib.fdtd_add_dft_region(self.handle, iptr(lo_a), iptr(cnt_a), int(codes.size),
                                            iptr(codes), 1 if yee_grid else 0, int(om.size),
                                            dptr(om), float(scale), int(decimation), iptr(out)),
               "fdtd_add_dft_region")
@@ -266,7 +271,7 @@ class HipSimulation:
         normal, lo, cnt, nf, nslot = self.planes[plane]
         cells = int(numpy.prod(cnt))
         buf = numpy.empty(2 * nslot * nf * cells, dtype=numpy.float64)
-        check(self.lib.fdtd_flux_read(self.handle, int(plane), dptr(buf), buf.size),
+        self._check(self.lib.fdtd_flux_read(self.handle, int(plane), dptr(buf), buf.size),
               "fdtd_flux_read")
         data = buf.view(numpy.complex128).reshape(nslot, nf, cnt[2], cnt[1], cnt[0])
         return numpy.ascontiguousarray(data.transpose(0, 1, 4, 3, 2))
@@ -280,13 +285,13 @@ class HipSimulation:
                              f"{(nslot, nf) + tuple(cnt)}, got {data.shape}")
         buf = numpy.ascontiguousarray(data.astype(numpy.complex128).transpose(0, 1, 4, 3, 2))
         flat = buf.view(numpy.float64).reshape(-1)
-        check(self.lib.fdtd_flux_write(self.handle, int(plane), dptr(flat), flat.size),
+        self._check(self.lib.fdtd_flux_write(self.handle, int(plane), dptr(flat), flat.size),
               "fdtd_flux_write")
 
     def flux_scale(self, plane: int, scale: complex) -> None:
         """Multiply a plane's transforms by ``scale``."""
         z = complex(scale)
-        check(self.lib.fdtd_flux_scale(self.handle, int(plane), z.real, z.imag), "fdtd_flux_scale")
+        self._check(self.lib.fdtd_flux_scale(self.handle, int(plane), z.real, z.imag), "fdtd_flux_scale")
 
 
 def device_info() -> str:

@@ -11,6 +11,7 @@ Volumes are NumPy float32 C-order (nx, ny, nz) arrays, the reference's own layou
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -62,7 +63,20 @@ def lib():
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _FP]
         _lib.oracle_update_p.argtypes = [ctypes.c_void_p, ctypes.c_int, _FP, _FP, _FP, _FP,
                                          ctypes.c_float, _FP, _FP, _FP]
+        _lib.oracle_set_flush.argtypes = [ctypes.c_int]
+        _lib.oracle_flush_state.restype = ctypes.c_int
     return _lib
+
+
+@contextlib.contextmanager
+def flushing(on: bool = True):
+    """Run the restatement (and this thread's NumPy) with x86 FTZ|DAZ set, then clear it."""
+    L = lib()
+    L.oracle_set_flush(1 if on else 0)
+    try:
+        yield
+    finally:
+        L.oracle_set_flush(0)
 
 
 def _fp(array: Optional[numpy.ndarray]):

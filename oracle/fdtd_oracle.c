@@ -36,6 +36,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include <xmmintrin.h>
 
 #define IDX(i, j, k) (((int64_t)(i) * ny + (j)) * nz + (k))
 
@@ -424,3 +425,21 @@ void oracle_dft_accumulate_c128(double *acc, const double *phase, const float *s
         }
     }
 }
+
+/* The subnormal policy of every thread the restatement computes on: on != 0 sets MXCSR's FTZ
+ * (bit 15, a subnormal result is written as zero) and DAZ (bit 6, a subnormal operand is read
+ * as zero), the x86 flush under which the reference mints its flush identity set
+ * (meep_identity.py:43-52); 0 clears both. MXCSR is per thread, so it is written in the calling
+ * thread and in every thread of an OpenMP team. */
+void oracle_set_flush(int on) {
+    const unsigned bits = 0x8040u;
+#pragma omp parallel
+    {
+        const unsigned csr = _mm_getcsr();
+        _mm_setcsr(on ? (csr | bits) : (csr & ~bits));
+    }
+    const unsigned csr = _mm_getcsr();
+    _mm_setcsr(on ? (csr | bits) : (csr & ~bits));
+}
+
+int oracle_flush_state(void) { return (_mm_getcsr() & 0x8040u) == 0x8040u; }

@@ -1,7 +1,7 @@
 """MEEP-identity capsules (tests/golden/meep_identity): the oracle built from a lifted state,
 and the replay loop both tiers judge MEEP's stored words with.
 
-The capsules, currents and digests are the reference's Tier A keep-set fixtures, copied as data
+The capsules, currents and digests are the reference's Tier A keep and flush fixtures, copied as data
 (tests/golden/meep_identity/provenance.json). The product reads them with
 ``meep_gpu.capsule``; the oracle here steps the same pasted state on the CPU.
 """
@@ -16,19 +16,22 @@ import numpy
 from meep_gpu import capsule
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURES = os.path.join(HERE, "golden", "meep_identity", "keep")
+ROOT = os.path.join(HERE, "golden", "meep_identity")
+FIXTURES = os.path.join(ROOT, "keep")
+#: The reference's two subnormal policies: MEEP's words with subnormals kept, and flushed.
+MODES = ("keep", "flush")
 #: The keep-set cases the engine steps (the others need symmetry folding, cylindrical or
 #: complex storage, and are refused by name).
 CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d")
 
 
-def case_dir(case: str) -> str:
-    return os.path.join(FIXTURES, case)
+def case_dir(case: str, mode: str = "keep") -> str:
+    return os.path.join(ROOT, mode, case)
 
 
-def load(case: str, max_poles: int = 8):
-    """(state, digests, currents) of one case."""
-    folder = case_dir(case)
+def load(case: str, max_poles: int = 8, mode: str = "keep"):
+    """(state, digests, currents) of one case in one subnormal policy."""
+    folder = case_dir(case, mode)
     state = capsule.read_capsule(os.path.join(folder, "capsule.capsule"), max_poles=max_poles)
     digests = capsule.read_digests(os.path.join(folder, "digests.json"))
     currents = capsule.read_currents(os.path.join(folder, "currents.json"), state,
@@ -39,9 +42,10 @@ def load(case: str, max_poles: int = 8):
 class OracleRun:
     """A lifted state stepped by the CPU restatement, every word pasted as the engine pastes it."""
 
-    def __init__(self, state: capsule.LiftedState, currents):
+    def __init__(self, state: capsule.LiftedState, currents, subnormals: str = "keep"):
         from oracle.engine import OracleSim, PointDeposit
 
+        self.flush = subnormals == "flush"
         faces = state.pml.faces if state.pml is not None else ((0, 0), (0, 0), (0, 0))
         sim = OracleSim(state.shape, state.dt, state.dx, invariant=state.invariant,
                         pml_faces=faces)
@@ -81,8 +85,11 @@ class OracleRun:
         return self.sim.step_count
 
     def advance(self, steps: int):
-        for _ in range(steps):
-            self.sim.step()
+        from oracle.engine import flushing
+
+        with flushing(self.flush):
+            for _ in range(steps):
+                self.sim.step()
 
     def words(self, component: str) -> numpy.ndarray:
         sim = self.sim

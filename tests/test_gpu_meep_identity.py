@@ -18,12 +18,15 @@ from tests import capsule_cases as cc
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("mode", cc.MODES)
 @pytest.mark.parametrize("case", cc.CASES)
-def test_the_engine_steps_meep_s_words(case):
-    state, digests, currents = cc.load(case)
-    print(f"meep identity {case}: {state.shape}, {len(state.poles)} poles, "
+def test_the_engine_steps_meep_s_words(case, mode):
+    """Subnormals kept (libmeep_fdtd.so) and flushed (libmeep_fdtd_flush.so: the float32 and
+    float64 denormal modes at flush, against the set MEEP minted under x86 FTZ|DAZ)."""
+    state, digests, currents = cc.load(case, mode=mode)
+    print(f"meep identity {case} ({mode}): {state.shape}, {len(state.poles)} poles, "
           f"pml {state.pml is not None}, metallic {state.metallic}", flush=True)
-    run = capsule.LiftedRun(state, currents)
+    run = capsule.LiftedRun(state, currents, subnormals=mode)
     try:
         got = cc.replay_digests(run, digests)
     finally:
@@ -34,8 +37,8 @@ def test_the_engine_steps_meep_s_words(case):
 
 def test_the_replay_entry_point():
     """``capsule.replay`` on a fixture folder: every compared step clean."""
-    print("capsule.replay ex.material-dispersion", flush=True)
-    out = capsule.replay(cc.case_dir("ex.material-dispersion"))
+    print("capsule.replay ex.absorber-1d (flush)", flush=True)
+    out = capsule.replay(cc.case_dir("ex.absorber-1d", "flush"))
     assert out and all(not diff for diff in out.values())
 
 

@@ -22,13 +22,15 @@ from tests import capsule_cases as cc
 
 
 def test_fixture_files_are_the_reference_ledger_s():
-    with open(os.path.join(cc.FIXTURES, "..", "provenance.json")) as fh:
+    with open(os.path.join(cc.ROOT, "provenance.json")) as fh:
         prov = json.load(fh)
-    assert set(prov["cases"]) == set(cc.CASES)
-    for case, files in prov["cases"].items():
-        for name, digest in files.items():
-            with open(os.path.join(cc.case_dir(case), name), "rb") as fh:
-                assert hashlib.sha256(fh.read()).hexdigest() == digest, (case, name)
+    assert set(prov["sets"]) == set(cc.MODES)
+    for mode, cases in prov["sets"].items():
+        assert set(cases) == set(cc.CASES)
+        for case, files in cases.items():
+            for name, digest in files.items():
+                with open(os.path.join(cc.case_dir(case, mode), name), "rb") as fh:
+                    assert hashlib.sha256(fh.read()).hexdigest() == digest, (mode, case, name)
 
 
 @pytest.mark.parametrize("case", cc.CASES)
@@ -162,13 +164,25 @@ def test_the_lifted_ade_words_are_this_package_s(case):
         assert numpy.array_equal(numpy.array(words, dtype=numpy.float32), pole.constants)
 
 
+@pytest.mark.parametrize("mode", cc.MODES)
 @pytest.mark.parametrize("case", cc.CASES)
-def test_the_oracle_steps_meep_s_words(case):
-    """Every compared step's stored words of all twelve components are MEEP's."""
-    state, digests, currents = cc.load(case)
-    got = cc.replay_digests(cc.OracleRun(state, currents), digests)
+def test_the_oracle_steps_meep_s_words(case, mode):
+    """Every compared step's stored words of all twelve components are MEEP's, with subnormals
+    kept, and flushed under x86 FTZ|DAZ (the policy each set was minted under)."""
+    state, digests, currents = cc.load(case, mode=mode)
+    assert digests.mode == mode
+    got = cc.replay_digests(cc.OracleRun(state, currents, subnormals=mode), digests)
     assert set(got) == set(digests.steps)
     assert {step: d for step, d in got.items() if d} == {}
+
+
+def test_the_flush_set_needs_the_flush():
+    """The flush set's words leave the keep policy's once subnormals appear (step 60 of ``a``),
+    so the oracle's FTZ|DAZ is what reproduces them."""
+    state, digests, currents = cc.load("a", mode="flush")
+    got = cc.replay_digests(cc.OracleRun(state, currents, subnormals="keep"), digests,
+                            steps=[200])
+    assert got[200]
 
 
 def test_the_replay_catches_planted_mistakes():

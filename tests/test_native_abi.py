@@ -13,25 +13,27 @@ def _declared():
 
 
 def test_every_declared_symbol_is_exported_and_bound():
+    """Both builds (subnormals kept, flushed) export every declaration and report their
+    policy (fdtd_subnormal_mode is a build-time constant: no GPU call)."""
     from meep_gpu import _native
 
-    lib = _native.load()
     declared = _declared()
     assert declared, "no declarations found"
-    for name in declared:
-        assert hasattr(lib, name), name
     assert set(declared) == set(_native.SIGNATURES), (
         set(declared) ^ set(_native.SIGNATURES))
+    for mode in _native.SUBNORMAL_MODES:
+        lib = _native.load(subnormals=mode)
+        for name in declared:
+            assert hasattr(lib, name), (mode, name)
+        assert lib.fdtd_subnormal_mode() == _native.SUBNORMAL_MODES.index(mode)
+    assert _native.load(subnormals="flush") is not _native.load()
 
 
 def test_missing_library_fails_loudly(tmp_path):
     import pytest
     from meep_gpu import _native
 
-    saved = _native._lib
-    _native._lib = None
-    try:
-        with pytest.raises(_native.EngineError, match="not built"):
-            _native.load(str(tmp_path / "absent.so"))
-    finally:
-        _native._lib = saved
+    with pytest.raises(_native.EngineError, match="not built"):
+        _native.load(str(tmp_path / "absent.so"))
+    with pytest.raises(ValueError, match="subnormals"):
+        _native.load(subnormals="ftz")
