@@ -172,6 +172,12 @@ int fdtd_set_conductivity(fdtd_sim *sim, int32_t field, int32_t axis, const floa
  * is minted, meep_identity.py:43-52). No GPU call. */
 int fdtd_subnormal_mode(void);
 
+/* Host arithmetic under the flush policy: on != 0 sets the calling thread's MXCSR FTZ|DAZ
+ * (x86), 0 clears it. For host-side reductions that must round as MEEP's host does under its
+ * flush policy (the whole-cell get_array of a flush run, reference readback.py under
+ * meep_identity.host_subnormal_mode). No GPU call. */
+int fdtd_host_flush(int32_t on);
+
 /* Confine a conductivity set above to the cells whose structure chunk keeps it: cells[q] 0
  * (x fastest, nx*ny*nz bytes) steps the lossless forms there and deposits without the condinv
  * factor, as MEEP does in a chunk whose conductivity samples are all zero (structure.cpp:

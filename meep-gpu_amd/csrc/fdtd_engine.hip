@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <xmmintrin.h>
 
 // 1 in the build whose kernels flush float32 and float64 subnormals (denormal modes at flush)
 #ifndef FDTD_FLUSH_SUBNORMALS
@@ -2205,6 +2206,13 @@ int fdtd_set_conductivity(fdtd_sim *s, int32_t field, int32_t c, const float *si
 }
 
 int fdtd_subnormal_mode(void) { return FDTD_FLUSH_SUBNORMALS; }
+
+int fdtd_host_flush(int32_t on) {
+    const unsigned bits = 0x8040u;  // MXCSR FTZ (bit 15) | DAZ (bit 6)
+    const unsigned csr = _mm_getcsr();
+    _mm_setcsr(on ? (csr | bits) : (csr & ~bits));
+    return FDTD_OK;
+}
 
 int fdtd_set_conductive_cells(fdtd_sim *s, int32_t field, int32_t c, const uint8_t *cells) {
     if (!s || (field != 0 && field != 1) || c < 0 || c > 2 || !cells)

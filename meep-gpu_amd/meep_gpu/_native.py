@@ -7,6 +7,7 @@ an exception naming the reason, never a CPU substitute.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 from typing import Dict, Optional
@@ -28,6 +29,7 @@ FDTD_MAX_POLES = 8
 SIGNATURES = {
     "fdtd_last_error": (ctypes.c_char_p, []),
     "fdtd_subnormal_mode": (ctypes.c_int, []),
+    "fdtd_host_flush": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fdtd_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.POINTER(ctypes.c_uint64)]),
@@ -125,6 +127,17 @@ def load(path: Optional[str] = None, subnormals: str = "keep") -> ctypes.CDLL:
         raise EngineError(f"{path} was not built with subnormals {subnormals!r}")
     _libs[path] = lib
     return lib
+
+
+@contextlib.contextmanager
+def host_flush(on: bool = True):
+    """Host arithmetic of this thread (NumPy included) under x86 FTZ|DAZ, cleared after."""
+    lib = load()
+    lib.fdtd_host_flush(1 if on else 0)
+    try:
+        yield
+    finally:
+        lib.fdtd_host_flush(0)
 
 
 def check(rc: int, what: str, lib: Optional[ctypes.CDLL] = None) -> None:

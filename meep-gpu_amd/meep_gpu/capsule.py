@@ -394,7 +394,8 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
 
     division = ChunkDivision(
         tuple(Frame(*_axis_frame(frame)) for frame in layout["frames"]),
-        [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]])
+        [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
+        [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
     return LiftedState(
         division=division, shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
         courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),
@@ -452,6 +453,8 @@ class DigestSet:
     steps: List[int]
     stored: Dict[int, Dict[str, str]]
     ghost_cells: Dict[str, int]
+    #: the same steps' digests of MEEP's sim.get_array words (the whole cell)
+    reader: Dict[int, Dict[str, str]] = field(default_factory=dict)
 
 
 def read_digests(path) -> DigestSet:
@@ -460,8 +463,9 @@ def read_digests(path) -> DigestSet:
         raise CapsuleRefusal(f"digests schema {doc.get('schema')!r}; this reader takes "
                              f"{DIGESTS_SCHEMA}")
     stored = {int(step): dict(words) for step, words in doc["stored"].items()}
+    reader = {int(step): dict(words) for step, words in doc.get("reader", {}).items()}
     return DigestSet(doc["case"], doc["mode"], int(doc["n_steps"]),
-                     [int(s) for s in doc["steps"]], stored, dict(doc["ghost_cells"]))
+                     [int(s) for s in doc["steps"]], stored, dict(doc["ghost_cells"]), reader)
 
 
 def word_digest(words: numpy.ndarray) -> str:
@@ -519,8 +523,8 @@ class LiftedRun:
                 continue  # E is formed from D below
             sim.upload(name, vol)
         sim.set_step(state.step, state.h_separated)
-        if not stores_e and not state.conductivity:
-            sim.prime_e()
+        if not stores_e:
+            sim.prime_e()  # E = chi1inv*(D - sum P), as the reference's get_E forms it
 
     @property
     def grid(self) -> Grid:
@@ -541,8 +545,35 @@ class LiftedRun:
             component = "B" + component[1]
         return self.sim.download(component)
 
+    def meep_array(self, component: str) -> numpy.ndarray:
+        """MEEP's whole-cell ``get_array(component)`` of the current words, with the lifted
+        chunk division's H/B alias rule (``readback.meep_array``)."""
+        from .readback import meep_array
+
+        state = self.state
+        words = {}  # downloaded before the host arithmetic changes mode
+
+        def read(name):
+            if name not in words:
+                words[name] = self.words(name)
+            return words[name]
+
+        read(component)
+        if component[0] == "B" and state.pml is not None:
+            read("H" + component[1])
+        with _native.host_flush(self.sim.subnormals == "flush"):
+            return meep_array(read, component, state.grid(),
+                              tuple(not m for m in state.metallic), state.division,
+                              state.pml is not None, self.step_count)
+
     def close(self):
         self.sim.close()
+
+
+def compare_reader(run, digests: DigestSet, step: int) -> List[str]:
+    """The components whose ``get_array`` words differ from MEEP's at one compared step."""
+    want = digests.reader.get(step, {})
+    return [c for c in COMPONENTS if c in want and word_digest(run.meep_array(c)) != want[c]]
 
 
 def compare_step(run, digests: DigestSet, step: int) -> List[str]:

@@ -66,6 +66,8 @@ def grid_frames(grid: Grid, wraps: Sequence[bool] = (True, True, True)) -> Tuple
 class ChunkDivision:
     frames: Tuple[Frame, ...]
     boxes: List[Tuple[Tuple[int, int, int], Tuple[int, int, int]]]
+    #: per chunk and axis: does the chunk carry that axis's PML layer (MEEP's sigsize > 1)
+    flags: List[Tuple[bool, bool, bool]] = None
 
     @classmethod
     def one_rank(cls, grid: Grid, faces, wraps: Sequence[bool] = (True, True, True)):
@@ -74,21 +76,52 @@ class ChunkDivision:
         pieces_by_axis = []
         for axis, frame in enumerate(frames):
             if frame.invariant:
-                pieces_by_axis.append([(frame.origin, frame.origin)])
+                pieces_by_axis.append([(frame.origin, frame.origin, False)])
                 continue
             lo_wall, hi_wall = frame.window
             low, high = faces[axis]
             low_hi = lo_wall + 2 * min(frame.owned, int(low + 1.5)) if low > 0 else lo_wall
             high_lo = hi_wall - 2 * min(frame.owned, int(high + 1.5)) if high > 0 else hi_wall
             if low_hi <= high_lo:
-                pieces = [(lo_wall, low_hi), (low_hi, high_lo), (high_lo, hi_wall)]
+                pieces = [(lo_wall, low_hi, True), (low_hi, high_lo, False),
+                          (high_lo, hi_wall, True)]
             else:  # overlapping effort volumes: the overlap is a chunk of its own
-                pieces = [(lo_wall, high_lo), (high_lo, low_hi), (low_hi, hi_wall)]
+                pieces = [(lo_wall, high_lo, True), (high_lo, low_hi, True),
+                          (low_hi, hi_wall, True)]
             pieces_by_axis.append([p for p in pieces if p[0] < p[1]])
-        boxes = [((px[0], py[0], pz[0]), (px[1], py[1], pz[1]))
-                 for px in pieces_by_axis[0] for py in pieces_by_axis[1]
-                 for pz in pieces_by_axis[2]]
-        return cls(frames, boxes)
+        boxes, flags = [], []
+        for px in pieces_by_axis[0]:
+            for py in pieces_by_axis[1]:
+                for pz in pieces_by_axis[2]:
+                    boxes.append(((px[0], py[0], pz[0]), (px[1], py[1], pz[1])))
+                    flags.append((px[2], py[2], pz[2]))
+        return cls(frames, boxes, flags)
+
+    def chunk_owns(self, chunk: numpy.ndarray, coords) -> numpy.ndarray:
+        """Does chunk ``chunk`` (an index per point, -1 none) own each point of doubled
+        ``coords`` (one broadcastable array per axis)? ``lo < p <= hi`` on resolved axes."""
+        index = numpy.asarray(chunk, dtype=numpy.int64)
+        lows = numpy.asarray([lo for lo, _ in self.boxes], dtype=numpy.int64)
+        highs = numpy.asarray([hi for _, hi in self.boxes], dtype=numpy.int64)
+        safe = numpy.where(index >= 0, index, 0)
+        owned = index >= 0
+        for axis, frame in enumerate(self.frames):
+            if frame.invariant:
+                continue
+            p = numpy.asarray(coords[axis], dtype=numpy.int64)
+            owned = owned & (p > lows[safe, axis]) & (p <= highs[safe, axis])
+        return owned
+
+    def chunk_of(self, coords) -> numpy.ndarray:
+        """The chunk owning each point of doubled ``coords``, or -1 where none does."""
+        arrays = [numpy.asarray(coords[a], dtype=numpy.int64)
+                  for a in range(3) if not self.frames[a].invariant]
+        shape = numpy.broadcast_shapes(*[a.shape for a in arrays]) if arrays else ()
+        out = numpy.full(shape, -1, dtype=numpy.int64)
+        for k in range(len(self.boxes)):
+            hit = self.chunk_owns(numpy.full(shape, k, dtype=numpy.int64), coords)
+            out = numpy.where(hit, k, out)
+        return out
 
     def _resolved(self, axis: int, shift: int) -> numpy.ndarray:
         """Each stored row's doubled coordinate, moved into the window on a wrapping axis; a

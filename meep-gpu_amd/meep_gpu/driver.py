@@ -27,6 +27,7 @@ from .engine import HipSimulation
 from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
 from .grid import Grid
 from .pml import PML
+from .readback import meep_array
 from .sources import DepositBox, current_table
 from .workloads import Workload
 
@@ -358,6 +359,16 @@ class FdtdDriver:
         if component[0] == "H" and sim.pml is None:
             component = "B" + component[1]
         return sim.download(component)
+
+    def get_meep_array(self, component: str) -> numpy.ndarray:
+        """MEEP's ``sim.get_array(component)`` over the whole cell: the centred lattice, word
+        for word (``readback.meep_array``), where ``get_array`` gives the stored words."""
+        sim = self.freeze()
+        pml = sim.pml
+        division = ChunkDivision.one_rank(
+            self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)), self.wraps)
+        return meep_array(self.get_array, component, self.grid, self.wraps, division,
+                          pml is not None, sim.step_count)
 
     def fast_path_report(self) -> Dict[str, object]:
         from .engine import device_info

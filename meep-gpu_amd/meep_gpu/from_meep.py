@@ -122,12 +122,19 @@ class GpuRunResult:
             raise KeyError("not a DFT field monitor of this run")
         return monitor.get_dft_array(component, freq_index)
 
-    def get_array(self, component: str) -> numpy.ndarray:
-        """One stored field component, float32 C-order (nx, ny, nz), a private host copy."""
-        return self.driver.get_array(component)
+    def get_array(self, component: str, center=None, size=None) -> numpy.ndarray:
+        """One field component as MEEP's ``sim.get_array`` returns it over the whole cell, word
+        for word (reference from_meep.py:956-980; ``readback.meep_array``): MEEP's centred
+        lattice, its duplicate plane and its rank. A ``center``/``size`` volume is not built."""
+        if center is not None or size is not None:
+            raise NotImplementedError("get_array over a center/size volume is not built; read "
+                                      "the whole cell and slice it")
+        return self.driver.get_meep_array(component)
 
     def get_field(self, component: str) -> numpy.ndarray:
-        return self.get_array(component)
+        """One stored field component, float32 C-order (nx, ny, nz): the driver's own layout
+        (reference from_meep.py:982)."""
+        return self.driver.get_array(component)
 
     def close(self) -> None:
         self.driver.close()

@@ -99,6 +99,17 @@ class OracleRun:
             return sim.v["B" + component[1]]
         return sim.v[component]
 
+    def meep_array(self, component: str) -> numpy.ndarray:
+        """The product's readback over the oracle's words, under the run's subnormal policy."""
+        from meep_gpu.readback import meep_array
+        from oracle.engine import flushing
+
+        state = self.state
+        with flushing(self.flush):
+            return meep_array(self.words, component, state.grid(),
+                              tuple(not m for m in state.metallic), state.division,
+                              state.pml is not None, self.step_count)
+
     def close(self):
         pass
 
@@ -111,4 +122,13 @@ def replay_digests(run, digests: capsule.DigestSet, steps=None) -> Dict[int, Lis
         run.advance(step - run.step_count)
         print(f"compared step {step}", flush=True)
         out[step] = capsule.compare_step(run, digests, step)
+    return out
+
+
+def replay_reader(run, digests: capsule.DigestSet, steps=None) -> Dict[int, List[str]]:
+    """As ``replay_digests``, judging the ``get_array`` words (MEEP's reader digests)."""
+    out = {}
+    for step in (digests.steps if steps is None else steps):
+        run.advance(step - run.step_count)
+        out[step] = capsule.compare_reader(run, digests, step)
     return out

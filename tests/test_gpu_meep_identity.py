@@ -1,7 +1,7 @@
-"""MEEP identity on the GPU: each lifted capsule (the reference's Tier A keep set), pasted into
-the HIP engine through the C ABI and stepped there, holds MEEP 1.35.0's stored words of all
-twelve components at every compared step (steps 0-20 one at a time, then every tenth of the
-run through replayed graphs).
+"""MEEP identity on the GPU: each lifted capsule (the reference's Tier A keep and flush sets),
+pasted into the HIP engine through the C ABI and stepped there, holds MEEP 1.35.0's stored words
+of all twelve components at every compared step (steps 0-20 one at a time, then every tenth of
+the run through replayed graphs), and MEEP's sim.get_array words of them (readback).
 
 The cases: ``a`` and ``b``, 2-D 200x200 with a 20-cell UPML inside metallic walls (MEEP's
 default boundary) and a Gaussian Ez point source, ``b`` with per-component inverse epsilon;
@@ -28,11 +28,17 @@ def test_the_engine_steps_meep_s_words(case, mode):
           f"pml {state.pml is not None}, metallic {state.metallic}", flush=True)
     run = capsule.LiftedRun(state, currents, subnormals=mode)
     try:
-        got = cc.replay_digests(run, digests)
+        got, read = {}, {}
+        for step in digests.steps:
+            run.advance(step - run.step_count)
+            got[step] = capsule.compare_step(run, digests, step)
+            # MEEP's sim.get_array words of the GPU's words (GpuRunResult.get_array)
+            read[step] = capsule.compare_reader(run, digests, step)
     finally:
         run.close()
     assert set(got) == set(digests.steps)
     assert {step: diff for step, diff in got.items() if diff} == {}
+    assert {step: diff for step, diff in read.items() if diff} == {}
 
 
 def test_the_replay_entry_point():

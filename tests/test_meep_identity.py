@@ -210,3 +210,36 @@ def test_a_planted_pole_word_is_caught():
     state.poles[0].constants = words
     got = cc.replay_digests(cc.OracleRun(state, currents), digests, steps=[40])
     assert "Ez" in got[40]
+
+
+@pytest.mark.parametrize("mode", cc.MODES)
+@pytest.mark.parametrize("case", cc.CASES)
+def test_get_array_is_meep_s_on_the_oracle_s_words(case, mode):
+    """readback.meep_array over the oracle's words gives MEEP's own sim.get_array words (the
+    identity sets' reader digests) at every compared step: the centred lattice, MEEP's
+    collapse of a one-cell axis, and under a PML the chunks' H/B alias rule."""
+    state, digests, currents = cc.load(case, mode=mode)
+    assert digests.reader
+    got = cc.replay_reader(cc.OracleRun(state, currents, subnormals=mode), digests)
+    assert {step: d for step, d in got.items() if d} == {}
+
+
+@pytest.mark.parametrize("planted", ["no_alias_rule", "no_duplicate_plane"])
+def test_get_array_planted_mistakes_are_caught(planted, monkeypatch):
+    """Reading B without the chunks' H/B alias rule (case b under its PML, where the interior
+    chunk's boundary words differ from step 200), or dropping the periodic duplicate plane
+    (the one-cell Lorentz case), leaves MEEP's get_array words."""
+    from meep_gpu import readback
+
+    case = "b" if planted == "no_alias_rule" else "ex.material-dispersion"
+    state, digests, currents = cc.load(case)
+    run = cc.OracleRun(state, currents)
+    if planted == "no_alias_rule":
+        original = readback.centred_sums
+        monkeypatch.setattr(readback, "centred_sums",
+                            lambda read, comp, inv, metal, div, pml, first_step:
+                            original(read, comp, inv, metal, div, False, first_step))
+    else:
+        run.state.metallic = (True, True, False)  # no duplicate plane on x and y
+    got = cc.replay_reader(run, digests, steps=[1, 200])
+    assert any(got.values())
