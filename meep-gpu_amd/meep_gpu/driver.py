@@ -360,15 +360,16 @@ class FdtdDriver:
             component = "B" + component[1]
         return sim.download(component)
 
-    def get_meep_array(self, component: str) -> numpy.ndarray:
-        """MEEP's ``sim.get_array(component)`` over the whole cell: the centred lattice, word
-        for word (``readback.meep_array``), where ``get_array`` gives the stored words."""
+    def get_meep_array(self, component: str, center=None, size=None) -> numpy.ndarray:
+        """MEEP's ``sim.get_array(component)`` over the whole cell or a ``center``/``size``
+        box: the centred lattice, word for word (``readback.meep_array``), where ``get_array``
+        gives the stored words."""
         sim = self.freeze()
         pml = sim.pml
         division = ChunkDivision.one_rank(
             self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)), self.wraps)
         return meep_array(self.get_array, component, self.grid, self.wraps, division,
-                          pml is not None, sim.step_count)
+                          pml is not None, sim.step_count, center=center, size=size)
 
     def fast_path_report(self) -> Dict[str, object]:
         from .engine import device_info

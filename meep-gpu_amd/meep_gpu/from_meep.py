@@ -123,13 +123,11 @@ class GpuRunResult:
         return monitor.get_dft_array(component, freq_index)
 
     def get_array(self, component: str, center=None, size=None) -> numpy.ndarray:
-        """One field component as MEEP's ``sim.get_array`` returns it over the whole cell, word
-        for word (reference from_meep.py:956-980; ``readback.meep_array``): MEEP's centred
-        lattice, its duplicate plane and its rank. A ``center``/``size`` volume is not built."""
-        if center is not None or size is not None:
-            raise NotImplementedError("get_array over a center/size volume is not built; read "
-                                      "the whole cell and slice it")
-        return self.driver.get_meep_array(component)
+        """One field component as MEEP's ``sim.get_array`` returns it, word for word (reference
+        from_meep.py:956-980; ``readback.meep_array``): MEEP's centred lattice, its duplicate
+        plane and its rank, over the whole cell or the ``center``/``size`` box (with MEEP's
+        interpolation weights and collapse on an empty direction)."""
+        return self.driver.get_meep_array(component, center=center, size=size)
 
     def get_field(self, component: str) -> numpy.ndarray:
         """One stored field component, float32 C-order (nx, ny, nz): the driver's own layout

@@ -1,4 +1,4 @@
-"""MEEP's ``sim.get_array(component=c)`` over the whole cell, word for word.
+"""MEEP's ``sim.get_array(component=c)``, word for word: over the whole cell, or a box.
 
 What a user of MEEP (and of the reference's ``GpuRunResult.get_array``) reads back is not the
 stored Yee words but MEEP's centred lattice: at every cell centre the float32 sum of the
@@ -136,9 +136,12 @@ def _vec2diel_ceil(coordinate: float, a: float) -> int:  # loop_in_chunks.cpp:23
 
 def meep_array(read: Callable[[str], numpy.ndarray], component: str, grid,
                wraps: Sequence[bool], division: ChunkDivision, pml_active: bool,
-               step_count: int) -> numpy.ndarray:
-    """MEEP's ``get_array(component)`` over the whole cell (float32, MEEP's rank and axis
-    order). ``read(name)`` returns a stored volume, C order (nx, ny, nz)."""
+               step_count: int, center=None, size=None) -> numpy.ndarray:
+    """MEEP's ``get_array(component)`` (float32, MEEP's rank and axis order): over the whole
+    cell (``fields::total_volume()``), or over the box ``center -/+ size/2`` as ``mp.Volume``
+    forms it. ``read(name)`` returns a stored volume, C order (nx, ny, nz)."""
+    if (center is None) != (size is None):
+        raise ValueError("pass both center and size, or neither (the whole cell)")
     invariant = tuple(grid.is_invariant(a) for a in range(3))
     metallic = tuple(not invariant[a] and not wraps[a] for a in range(3))
     q = centred_sums(read, component, invariant, metallic, division, pml_active,
@@ -157,8 +160,12 @@ def meep_array(read: Callable[[str], numpy.ndarray], component: str, grid,
     collapse = []
     for axis in order:
         frame = division.frames[axis]
-        low = frame.origin * 0.5 * grid.dx  # total_volume: [little_corner, big_corner - 2]
-        high = (frame.origin + 2 * frame.stored - 2) * 0.5 * grid.dx
+        if center is None:
+            low = frame.origin * 0.5 * grid.dx  # total_volume: [little_corner, big_corner - 2]
+            high = (frame.origin + 2 * frame.stored - 2) * 0.5 * grid.dx
+        else:
+            low = float(center[axis]) - float(size[axis]) * 0.5
+            high = float(center[axis]) + float(size[axis]) * 0.5
         lo, hi = min(low, high), max(low, high)
         i_s, i_e = _vec2diel_floor(lo, a), _vec2diel_ceil(hi, a)
         keep = numpy.nonzero((ivec[axis] >= i_s) & (ivec[axis] <= i_e))[0]
@@ -176,6 +183,8 @@ def meep_array(read: Callable[[str], numpy.ndarray], component: str, grid,
             visited = int(ivec[axis][keep[0]])
             w = min(w0, w1) if i_e == i_s else (w0 if visited == i_s else w1)
             weights[axis] = numpy.array([w], dtype=_F64)
+        if keep.size > 2 and lo == hi:
+            raise RuntimeError("an empty dimension spanning more than two centred planes")
     values = q[numpy.ix_(*pick)]
     weight = None
     for axis in order:

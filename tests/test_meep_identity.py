@@ -243,3 +243,36 @@ def test_get_array_planted_mistakes_are_caught(planted, monkeypatch):
         run.state.metallic = (True, True, False)  # no duplicate plane on x and y
     got = cc.replay_reader(run, digests, steps=[1, 200])
     assert any(got.values())
+
+
+def test_get_array_over_a_box_is_a_slice_of_the_whole_cell():
+    """MEEP's get_array over a center/size box. With its faces on integer-lattice coordinates,
+    midway between centred points, it visits the centred points from the one below the low
+    face to the one above the high face, each with weight 1: a slice of the whole cell. A line
+    there (an empty direction) is MEEP's collapse of the two centred points beside it, weights
+    0.5 and 0.5, summed by float32 += from zero. The whole-cell box is the whole cell."""
+    import numpy
+
+    from meep_gpu.readback import meep_array
+
+    state, digests, currents = cc.load("a")
+    run = cc.OracleRun(state, currents)
+    run.advance(40)
+    whole = run.meep_array("Ez")
+    args = (run.words, "Ez", state.grid(), (False, False, True), state.division, True, 40)
+    dx = state.dx
+    assert numpy.any(whole[90:110, 95:105])
+
+    def at(k):  # integer-lattice point k (stored index) as a coordinate: (k - 100) dx
+        return (k - 100) * dx
+
+    box = meep_array(*args, center=(0.5 * (at(92) + at(108)), 0.5 * (at(96) + at(104)), 0.0),
+                     size=(at(108) - at(92), at(104) - at(96), 0.0))
+    assert numpy.array_equal(box, whole[91:109, 95:105])
+    line = meep_array(*args, center=(at(101), 0.5 * (at(96) + at(104)), 0.0),
+                      size=(0.0, at(104) - at(96), 0.0))
+    half = lambda v: (0.5 * v.astype(numpy.float64)).astype(numpy.float32)  # noqa: E731
+    want = (numpy.float32(0.0) + half(whole[100, 95:105])) + half(whole[101, 95:105])
+    assert numpy.array_equal(line, want)
+    assert numpy.array_equal(
+        meep_array(*args, center=(-0.025, -0.025, 0.0), size=(9.95, 9.95, 0.0)), whole)
