@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <xmmintrin.h>
 
 // 1 in the build whose kernels flush float32 and float64 subnormals (denormal modes at flush)
@@ -2104,33 +2105,63 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
 // Host <-> device volume transfers. The device rows are padded to the pitch; the rows are
 // re-pitched on the host and moved as ONE contiguous copy (a pitched 2-D copy from pageable
 // memory goes row by row and costs ~10 us per row).
+// Host transfers go through one pinned staging buffer: the runtime copies pageable memory with
+// blit kernels that wait behind a batch occupying the device, which held an admission (uploads
+// of a new solve) to the batch's pace; from pinned memory the copy engines run beside it. The
+// buffer grows by doubling and is reused (every transfer completes before the call returns).
+static std::mutex g_pin_mu;
+static void *g_pin = nullptr;
+static size_t g_pin_bytes = 0;
+
+static int pinned_buffer(size_t bytes, void **out) {
+    if (bytes > g_pin_bytes) {
+        size_t want = g_pin_bytes ? g_pin_bytes : (size_t)16 << 20;
+        while (want < bytes) want *= 2;
+        void *p = nullptr;
+        HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+        if (g_pin) (void)hipHostFree(g_pin);
+        g_pin = p;
+        g_pin_bytes = want;
+    }
+    *out = g_pin;
+    return FDTD_OK;
+}
+
+// count bytes host -> device (or back) on the simulation's stream, staged in pinned memory
+static int copy_pinned(fdtd_sim *s, void *dev, const void *host, size_t bytes, bool to_device) {
+    std::lock_guard<std::mutex> lock(g_pin_mu);
+    void *pin = nullptr;
+    int rc = pinned_buffer(bytes, &pin);
+    if (rc) return rc;
+    if (to_device) {
+        std::memcpy(pin, host, bytes);
+        HIP_TRY(hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        std::memcpy((void *)host, pin, bytes);
+    }
+    return FDTD_OK;
+}
+
 static int upload_volume(fdtd_sim *s, float *dst, const float *src) {
     const SimDev &h = s->host;
     const size_t rows = (size_t)h.n[1] * h.n[2];
-    if (h.pitch == h.n[0]) {
-        HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float) * rows * h.n[0], hipMemcpyHostToDevice, s->stream));
-    } else {
-        std::vector<float> stage(rows * h.pitch, 0.0f);
-        for (size_t r = 0; r < rows; ++r)
-            std::memcpy(stage.data() + r * h.pitch, src + r * h.n[0], sizeof(float) * h.n[0]);
-        HIP_TRY(hipMemcpyAsync(dst, stage.data(), sizeof(float) * stage.size(), hipMemcpyHostToDevice, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));  // before `stage` goes
-    }
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    return FDTD_OK;
+    if (h.pitch == h.n[0]) return copy_pinned(s, dst, src, sizeof(float) * rows * h.n[0], true);
+    std::vector<float> stage(rows * h.pitch, 0.0f);
+    for (size_t r = 0; r < rows; ++r)
+        std::memcpy(stage.data() + r * h.pitch, src + r * h.n[0], sizeof(float) * h.n[0]);
+    return copy_pinned(s, dst, stage.data(), sizeof(float) * stage.size(), true);
 }
 
 static int download_volume(fdtd_sim *s, float *host, const float *src) {
     const SimDev &h = s->host;
     const size_t rows = (size_t)h.n[1] * h.n[2];
-    if (h.pitch == h.n[0]) {
-        HIP_TRY(hipMemcpyAsync(host, src, sizeof(float) * rows * h.n[0], hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        return FDTD_OK;
-    }
+    if (h.pitch == h.n[0]) return copy_pinned(s, (void *)src, host, sizeof(float) * rows * h.n[0], false);
     std::vector<float> stage(rows * h.pitch);
-    HIP_TRY(hipMemcpyAsync(stage.data(), src, sizeof(float) * stage.size(), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    int rc = copy_pinned(s, (void *)src, stage.data(), sizeof(float) * stage.size(), false);
+    if (rc) return rc;
     for (size_t r = 0; r < rows; ++r)
         std::memcpy(host + r * h.n[0], stage.data() + r * h.pitch, sizeof(float) * h.n[0]);
     return FDTD_OK;
@@ -2243,13 +2274,19 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     if (rc) return rc;
     // the distinct words, by bit pattern (so -0 and NaN payloads stay themselves)
     SimDev &h = s->host;
+    // (neighbouring cells mostly share a word: the last one found skips the hash)
     std::unordered_map<uint32_t, uint32_t> index;
     std::vector<float> table;
     const size_t ncell = (size_t)h.ncell;
+    uint32_t last = 0;
+    bool have_last = false;
     for (size_t q = 0; q < ncell && table.size() <= 65536; ++q) {
         uint32_t w;
         std::memcpy(&w, vol + q, 4);
+        if (have_last && w == last) continue;
         if (index.emplace(w, (uint32_t)table.size()).second) table.push_back(vol[q]);
+        last = w;
+        have_last = true;
     }
     rc = s->pull_desc();
     if (rc) return rc;
@@ -2262,11 +2299,18 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     const bool narrow = table.size() <= 256;
     const size_t rows = (size_t)h.n[1] * h.n[2], esize = narrow ? 1 : 2;
     std::vector<uint8_t> idx((size_t)h.npad * esize, 0);
+    uint32_t last_k = 0;
+    have_last = false;
     for (size_t r = 0; r < rows; ++r)
         for (int i = 0; i < h.n[0]; ++i) {
             uint32_t w;
             std::memcpy(&w, vol + r * h.n[0] + i, 4);
-            const uint32_t k = index[w];
+            if (!have_last || w != last) {
+                last = w;
+                last_k = index[w];
+                have_last = true;
+            }
+            const uint32_t k = last_k;
             const size_t at = r * h.pitch + i;
             if (narrow) idx[at] = (uint8_t)k;
             else std::memcpy(&idx[at * 2], &k, 2);
@@ -2276,7 +2320,8 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     if (rc) return rc;
     rc = s->alloc(&dev_tab, sizeof(float) * table.size());
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(dev_idx, idx.data(), idx.size(), hipMemcpyHostToDevice, s->stream));
+    rc = copy_pinned(s, dev_idx, idx.data(), idx.size(), true);
+    if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(dev_tab, table.data(), sizeof(float) * table.size(), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (narrow) h.inv8[c] = (const uint8_t *)dev_idx;
@@ -2724,9 +2769,7 @@ int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
     if (count != need) return fail(FDTD_EINVAL, "flux read needs %zu doubles, got %zu", need, count);
     int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(out, ph.dev.acc, sizeof(double) * need, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    return FDTD_OK;
+    return copy_pinned(s, ph.dev.acc, out, sizeof(double) * need, false);
 }
 
 // Bring a plane's accumulators up to date (the ring's pending samples transformed in) before

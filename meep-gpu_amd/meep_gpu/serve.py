@@ -16,7 +16,9 @@ frequencies, 1000..8000 steps). ``SolveServer.run`` is continuous batching, not 
   device state and uploads its materials and source tables -- and builds the inputs of the
   requests still waiting, so admission costs the device nothing: the new solves join the
   shared launches at the end of the chunk;
-* a solve that completes is read out (spectra), freed, and its slot and bytes go to the queue.
+* a solve that completes leaves the batch at once, and is read out (spectra) and freed while
+  the device runs the next chunk (or at once when nothing else is resident); its slot and
+  bytes go to the queue when it is freed.
 
 Each solve steps exactly the words it would step alone (a batch member's launches are the
 lone simulation's launches), which the GPU tests check request by request. The report gives
@@ -256,6 +258,17 @@ class SolveServer:
                 nxt += 1
 
         staged: List[_Slot] = []  # admitted while the device ran; they join at the next regroup
+        finishing: List[_Slot] = []  # completed: read out while the next chunk runs
+
+        def finish_done():
+            nonlocal used
+            for s in finishing:
+                spectra, checksum = be.finish(s.handle)
+                results.append(SolveResult(s.request.rid, s.request.cells, s.request.steps,
+                                           s.request.arrival, s.admitted, self.clock() - t0,
+                                           spectra, checksum))
+                used -= s.nbytes  # its bytes stay counted until they are freed
+            finishing.clear()
 
         def admit(into: List[_Slot]):
             """Admit from the head of the queue while a slot and the bytes are there."""
@@ -280,13 +293,14 @@ class SolveServer:
             peak_n = max(peak_n, len(running) + len(staged))
             peak_b = max(peak_b, used)
 
-        while nxt < len(reqs) or queue or running or staged:
+        while nxt < len(reqs) or queue or running or staged or finishing:
             arrivals(self.clock() - t0)
             if staged:
                 running.extend(staged)
                 staged.clear()
                 regroup = True
             if not running:
+                finish_done()  # nothing on the device to overlap the read-out with
                 admit(running)
                 regroup = True
                 if not running:
@@ -301,8 +315,10 @@ class SolveServer:
                 regroupings += 1
             chunk = min(self.quantum, min(s.remaining for s in running))
             be.advance(batch, chunk)
-            # overlapped with the device: admit (allocate and upload) what has arrived and fits,
-            # and build the host inputs of the requests still waiting
+            # overlapped with the device: read out and free the solves that completed, admit
+            # (allocate and upload) what has arrived and fits, and build the host inputs of the
+            # requests still waiting
+            finish_done()
             arrivals(self.clock() - t0)
             admit(staged)
             for r in list(queue)[: self.max_slots]:
@@ -316,12 +332,7 @@ class SolveServer:
                 be.ungroup(batch)
                 batch = None
                 regroup = True
-                for s in done:
-                    spectra, checksum = be.finish(s.handle)
-                    results.append(SolveResult(s.request.rid, s.request.cells, s.request.steps,
-                                               s.request.arrival, s.admitted,
-                                               self.clock() - t0, spectra, checksum))
-                    used -= s.nbytes
+                finishing.extend(done)
                 running = [s for s in running if s.remaining > 0]
         if batch is not None:
             be.ungroup(batch)

@@ -80,12 +80,23 @@ def _sphere_inverse_epsilon(grid: Grid, radius: float, eps_in: float, eps_out: f
     """Sub-pixel chi1inv of a sphere: the fill fraction of each Yee cell from the signed
     distance to the surface, f = clip(1/2 - d/dx, 0, 1) (a locally planar interface through
     the cell), then chi1inv = f/eps_in + (1 - f)/eps_out."""
+    # f is exactly 1 (0) more than half a cell inside (outside) the surface, so only a band of
+    # +-1 cell around it takes the formula; the rest are the two constants the formula gives
+    # there, element for element the same words (the admission of a served solve waits on this)
+    inside = numpy.float32(1.0 / eps_in + (1.0 - 1.0) / eps_out)
+    outside = numpy.float32(0.0 / eps_in + (1.0 - 0.0) / eps_out)
+    near2 = (radius - grid.dx) ** 2 if radius > grid.dx else -1.0
+    far2 = (radius + grid.dx) ** 2
     out = {}
     for c in _E:
         x, y, z = _yee_coordinates(grid, c)
-        r = numpy.sqrt(x[:, None, None] ** 2 + y[None, :, None] ** 2 + z[None, None, :] ** 2)
+        r2 = (x[:, None, None] ** 2 + y[None, :, None] ** 2) + z[None, None, :] ** 2
+        vol = numpy.where(r2 <= near2, inside, outside).astype(numpy.float32)
+        band = (r2 > near2) & (r2 < far2)
+        r = numpy.sqrt(r2[band])
         f = numpy.clip(0.5 - (r - radius) / grid.dx, 0.0, 1.0)
-        out[c] = (f / eps_in + (1.0 - f) / eps_out).astype(numpy.float32)
+        vol[band] = (f / eps_in + (1.0 - f) / eps_out).astype(numpy.float32)
+        out[c] = vol
     return out
 
 
