@@ -2108,40 +2108,36 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
 // Host transfers go through one pinned staging buffer: the runtime copies pageable memory with
 // blit kernels that wait behind a batch occupying the device, which held an admission (uploads
 // of a new solve) to the batch's pace; from pinned memory the copy engines run beside it. The
-// buffer grows by doubling and is reused (every transfer completes before the call returns).
+// buffer is two fixed halves of kPinPiece bytes, allocated once per process, and a transfer of
+// any size streams through them piece by piece: an upload's memcpy of one piece into one half
+// overlaps the copy engine's read of the previous piece from the other. Every transfer has
+// completed when the call returns.
+static constexpr size_t kPinPiece = (size_t)16 << 20;
 static std::mutex g_pin_mu;
-static void *g_pin = nullptr;
-static size_t g_pin_bytes = 0;
+static char *g_pin = nullptr;  // 2 * kPinPiece bytes
 
-static int pinned_buffer(size_t bytes, void **out) {
-    if (bytes > g_pin_bytes) {
-        size_t want = g_pin_bytes ? g_pin_bytes : (size_t)16 << 20;
-        while (want < bytes) want *= 2;
-        void *p = nullptr;
-        HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
-        if (g_pin) (void)hipHostFree(g_pin);
-        g_pin = p;
-        g_pin_bytes = want;
-    }
-    *out = g_pin;
-    return FDTD_OK;
-}
-
-// count bytes host -> device (or back) on the simulation's stream, staged in pinned memory
 static int copy_pinned(fdtd_sim *s, void *dev, const void *host, size_t bytes, bool to_device) {
     std::lock_guard<std::mutex> lock(g_pin_mu);
-    void *pin = nullptr;
-    int rc = pinned_buffer(bytes, &pin);
-    if (rc) return rc;
-    if (to_device) {
-        std::memcpy(pin, host, bytes);
-        HIP_TRY(hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-    } else {
-        HIP_TRY(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        std::memcpy((void *)host, pin, bytes);
+    if (!g_pin) HIP_TRY(hipHostMalloc((void **)&g_pin, 2 * kPinPiece, hipHostMallocDefault));
+    char *d = (char *)dev;
+    const char *hin = (const char *)host;
+    char *hout = (char *)host;
+    size_t k = 0;
+    for (size_t off = 0; off < bytes; off += kPinPiece, ++k) {
+        const size_t piece = std::min(kPinPiece, bytes - off);
+        char *half = g_pin + (k & 1) * kPinPiece;
+        if (to_device) {
+            // this half was last read by piece k - 2, which the previous iteration's wait ended
+            std::memcpy(half, hin + off, piece);
+            HIP_TRY(hipStreamSynchronize(s->stream));  // piece k - 1 leaves the other half
+            HIP_TRY(hipMemcpyAsync(d + off, half, piece, hipMemcpyHostToDevice, s->stream));
+        } else {
+            HIP_TRY(hipMemcpyAsync(half, d + off, piece, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            std::memcpy(hout + off, half, piece);
+        }
     }
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return FDTD_OK;
 }
 
