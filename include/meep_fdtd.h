@@ -109,7 +109,9 @@ int fdtd_add_source(fdtd_sim *sim, int32_t field, int32_t component, const int32
 
 /* Upload / download one volume by name: "Dx".."Bz", "Ex".., "Hx".., "fu_Bx".., "fu_Dx"..,
  * "f_w_Ex".., "f_w_Hx".., "P[k].Ex", "P_prev[k].Ex" (the reference's digest names,
- * array_path_oracle.py:709). count = nx*ny*nz floats, x fastest. */
+ * array_path_oracle.py:709). count = nx*ny*nz floats, x fastest. "Hx".. exist only with a
+ * PML (without one H is B's words: pass "Bx"..). `host` may be pageable: the copy is staged
+ * through the library's pinned buffer and has completed when the call returns. */
 int fdtd_upload(fdtd_sim *sim, const char *name, const float *host, size_t count);
 int fdtd_download(fdtd_sim *sim, const char *name, float *host, size_t count);
 
