@@ -121,6 +121,43 @@ def test_chunks_end_where_a_resident_solve_completes_and_regroup():
     assert rep.results[0].finished < rep.results[1].finished
 
 
+def test_a_completed_solve_is_read_out_while_the_next_chunk_runs():
+    """The read-out of a completed solve follows the launch of the next chunk of the solves
+    still resident; with nothing else resident it is read out at once. Its bytes stay counted
+    until it is freed, so a request needing them is admitted only after the read-out."""
+    log = []
+
+    class Logged(FakeBackend):
+        def admit(self, r, inputs):
+            log.append(("admit", r.rid))
+            return super().admit(r, inputs)
+
+        def advance(self, batch, steps):
+            log.append(("advance", tuple(r.rid for r in batch), steps))
+            super().advance(batch, steps)
+
+        def finish(self, r):
+            log.append(("finish", r.rid))
+            return super().finish(r)
+
+    clock = FakeClock()
+    be = Logged(clock)
+    fp = lambda r: r.cells * 100  # noqa: E731
+    reqs = [SolveRequest(rid=0, arrival=0.0, edge=32, steps=300),
+            SolveRequest(rid=1, arrival=0.0, edge=32, steps=1000),
+            SolveRequest(rid=2, arrival=0.0, edge=32, steps=10)]
+    srv = SolveServer(budget_bytes=2 * 32 ** 3 * 100, max_slots=64, quantum=256,
+                      max_batch_cells=None, backend=be, footprint=fp, clock=clock,
+                      sleep=clock.sleep)
+    srv.run(reqs)
+    assert log[:4] == [("admit", 0), ("admit", 1), ("advance", (0, 1), 256),
+                       ("advance", (0, 1), 44)]
+    # rid 0 completed: rid 1 goes on, then rid 0 is read out, then its bytes admit rid 2
+    assert log[4:7] == [("advance", (1,), 256), ("finish", 0), ("admit", 2)]
+    assert log.index(("finish", 1)) == len(log) - 1  # the last solve: read out at once
+    assert [e for e in log if e[0] == "finish"] == [("finish", 0), ("finish", 2), ("finish", 1)]
+
+
 def test_requests_are_prepared_while_the_device_runs():
     reqs = [SolveRequest(rid=0, arrival=0.0, edge=64, steps=4000),
             SolveRequest(rid=1, arrival=0.05, edge=32, steps=100)]
