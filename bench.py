@@ -148,6 +148,11 @@ def config_line(name: str, steps: int) -> dict:
             "spectra_finite": bool(finite)}
 
 
+#: False under --no-graphs: every leg, the one-at-a-time baselines' "graphed" runs included,
+#: launches eagerly (kernel tracing then sees no graph replay)
+GRAPHS = True
+
+
 def batched_line(count: int, steps: int, seed: int) -> dict:
     """The batched form: ``count`` uneven synthetic simulations stepped by shared launches,
     against the same simulations stepped one at a time."""
@@ -180,7 +185,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
 
     alone = {}
     for graphs in (False, True):
-        set_graphs(graphs)
+        set_graphs(graphs and GRAPHS)
         alone_s = 0.0
         for q, d in enumerate(drivers):
             if q % 16 == 0:
@@ -194,7 +199,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             d.sim.synchronize()
             alone_s += time.perf_counter() - t0
         alone[graphs] = cells * steps / alone_s / 1e6
-    set_graphs(True)
+    set_graphs(GRAPHS)
     finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum()))
                  for d in drivers for m in d.flux_monitors)
     for d in drivers:
@@ -214,7 +219,7 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "note": "algorithmic bytes of both fused pairs per step, all members"},
-            "launch_shape": {"h_pair": list(shape[0]), "d_pair": list(shape[1])}, "graphs": True, "spectra_finite": bool(finite),
+            "launch_shape": {"h_pair": list(shape[0]), "d_pair": list(shape[1])}, "graphs": GRAPHS, "spectra_finite": bool(finite),
             "host_build_s": round(build_s, 2)}
 
 
@@ -229,9 +234,9 @@ def served_line(count: int, rate: float, seed: int) -> dict:
     rep = SolveServer(max_slots=64, quantum=256).run(reqs)
     one = {}
     for graphs in (False, True):
-        set_graphs(graphs)
+        set_graphs(graphs and GRAPHS)
         one[graphs] = SolveServer(max_slots=1, quantum=256).run(reqs).summary()
-    set_graphs(True)
+    set_graphs(GRAPHS)
     s, o, og = rep.summary(), one[False], one[True]
     return {"workload": f"{count} Poisson arrivals at {rate}/s (seed {seed}): sphere solves, "
                         f"edge 32-160 log-uniform, 30% Lorentz, 1-8 planes x 16-100 freqs, "
@@ -267,7 +272,7 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-saturated-rate", type=float, default=32.0,
                     help="a second stream whose arrivals outpace the device (capacity)")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="launch every step eagerly (profiling the batched form)")
+                    help="launch every step eagerly, in every leg (for kernel tracing)")
     ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[/V,BPC,REMAP,NT]",
                     help="fixed launch shapes (H pair[/D pair]) instead of the autotuned ones "
                          "(profiling runs)")
@@ -297,6 +302,8 @@ def main(argv=None) -> int:
     if args.no_graphs:
         from meep_gpu.engine import set_graphs
 
+        global GRAPHS
+        GRAPHS = False
         set_graphs(False)
     wl = workloads.config(args.workload)
     if args.launch:
