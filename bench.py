@@ -3,13 +3,17 @@
 Workload (default): the 192^3 dielectric sphere (7,077,888 cells, 24 px/unit, 8-cell UPML on
 all faces, sphere radius 38.4 cells, eps 2.25 with a sub-pixel diagonal chi1inv, Ez Gaussian
 point source fcen 0.15 / df 0.1), 8 flux planes x 100 frequencies transformed in float64.
-A step is one whole time step: the fused H pair, the fused D pair and the flux sampling (and,
-every 32 steps, the float64 transform pass). Inputs are resident in HBM before timing starts.
+A step is one whole time step: the fused H pair, the fused D pair and the flux sampling; every
+128 samples a plane's ring is transformed in float64, and the timed window ends with the pass
+over the samples still buffered, so every sample taken in the window is transformed inside it.
+Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line. ``value`` is Mcell-steps/s = cells * steps / elapsed; ``roofline`` is
-for the D pair (the heavier fused launch): algorithmic bytes per launch over its mean launch
-time from HIP events around every launch in the timed region; ``cpu_baseline`` times the C
-restatement (OpenMP on the host cores) on a bounded sample of the same workload.
+for the D pair (the heavier fused launch): the stencil's compulsory bytes per launch (every
+array once, neighbours free, auxiliaries on their layer rows, inverse epsilon as float32) over its mean launch time from HIP events around every launch in the timed region,
+with the engine's compacted layout's own byte count beside it; ``cpu_baseline`` times the C
+restatement (OpenMP on the host cores) on a bounded sample of the same workload, and the same
+leg states each spectrum's error against that float64 oracle run (``precision``).
 """
 
 from __future__ import annotations
@@ -53,25 +57,54 @@ def algorithmic_bytes(sim) -> dict:
     return {"h": h, "d": d}
 
 
-def cpu_baseline(wl, seconds: float) -> dict:
-    """Time the C restatement (oracle) on the same workload for about ``seconds``."""
+def _oracle_inputs(wl, steps: int):
     from meep_gpu.dft import FluxMonitor
-    from meep_gpu.pml import PML, normalize_faces
+    from meep_gpu.pml import normalize_faces
     from meep_gpu.sources import current_table
-    from oracle import workload as ow
 
     grid = wl.grid
     faces = normalize_faces(wl.pml_faces, grid) if wl.pml_faces else ((0, 0),) * 3
-    steps_cap = 2000
     deposits = []
     for box, fn in wl.sources:
         deposits.append((box.component, box.lo, box.cnt, box.amp_re, box.amp_im,
-                         current_table(fn, grid.dt, steps_cap, box.magnetic)))
+                         current_table(fn, grid.dt, steps, box.magnetic)))
     planes = []
     for p in wl.planes:
         mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
         planes.append((p.normal, p.lo, p.cnt, mon.omegas, mon.scale))
     poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in wl.poles]
+    return faces, deposits, planes, poles
+
+
+def spectrum_errors(wl, steps: int, oplanes) -> dict:
+    """The engine's spectra after ``steps`` steps of ``wl`` against the float64 oracle run of
+    the same steps (``oplanes``): per spectrum, max |engine - oracle| over the oracle's peak."""
+    from meep_gpu.driver import FdtdDriver
+
+    drv = FdtdDriver.from_workload(wl)
+    drv.freeze(planned_steps=steps + 1)
+    drv.run(steps=steps)
+    drv.synchronize()
+    errs = []
+    for mon, plane in zip(drv.flux_monitors, oplanes):
+        got = mon.get_flux_spectrum()
+        ref = plane.flux(wl.grid.dx, "c128")
+        peak = float(numpy.max(numpy.abs(ref)))
+        errs.append(float(numpy.max(numpy.abs(got - ref)) / peak) if peak > 0 else None)
+    drv.close()
+    return {"steps": steps, "spectra": len(errs), "frequencies": int(wl.planes[0].frequencies.size),
+            "max_rel_error_vs_fp64_oracle": [None if e is None else float(f"{e:.3e}") for e in errs],
+            "tolerance": 1e-9}
+
+
+def cpu_baseline(wl, seconds: float):
+    """Time the C restatement (oracle) on the same workload for about ``seconds``; then step
+    the engine for the same steps and state each spectrum's error against that float64 run."""
+    from oracle import workload as ow
+
+    grid = wl.grid
+    steps_cap = 2000
+    faces, deposits, planes, poles = _oracle_inputs(wl, steps_cap)
     sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, wl.inv, poles, deposits, planes,
                             grid.invariant)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -82,10 +115,25 @@ def cpu_baseline(wl, seconds: float) -> dict:
         ow.run(sim, oplanes, 1, reference_precision=False)
         steps += 1
     elapsed = time.perf_counter() - t0
-    return {"value": round(grid.total_cells * steps / elapsed / 1e6, 2), "unit": "Mcell-steps/s",
+    base = {"value": round(grid.total_cells * steps / elapsed / 1e6, 2), "unit": "Mcell-steps/s",
             "cores": threads, "kind": "port",
             "sample": f"{wl.name}: {steps} steps of the C restatement (OpenMP, {threads} threads) "
                       f"incl. the {len(wl.planes)} flux planes in float64, {elapsed:.1f} s"}
+    return base, spectrum_errors(wl, steps + 1, oplanes)
+
+
+def oracle_precision(name: str, steps: int) -> dict:
+    """Each spectrum's error against a float64 oracle run of ``steps`` steps of config ``name``."""
+    from meep_gpu import workloads
+    from oracle import workload as ow
+
+    wl = workloads.config(name)
+    grid = wl.grid
+    faces, deposits, planes, poles = _oracle_inputs(wl, steps + 1)
+    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, wl.inv, poles, deposits, planes,
+                            grid.invariant)
+    ow.run(sim, oplanes, steps, reference_precision=False)
+    return spectrum_errors(wl, steps, oplanes)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_pairs_sphere192.txt")
@@ -103,6 +151,30 @@ def measured_traffic(kernel: str):
     except OSError:
         pass
     return None
+
+
+def compulsory_bytes(sim) -> dict:
+    """Algorithmic HBM bytes per launch of each fused pair, the compulsory traffic of the
+    stencil with every array touched once per launch and neighbour reads free, independent of
+    this engine's storage choices: H pair E r 12, B r/w 24, H w 12 and, per component on its
+    layer rows, fu_B r/w 8,
+    H r 4, f_w_H r/w 8; D pair H r 12, D r/w 24, inverse epsilon r 12 (float32), E w 12 and,
+    per component on its layer rows, fu_D r/w 8, E r 4, f_w_E r/w 8; plus P r, P_prev r, P' w
+    and sigma r per driven pole component on the rows where a pole acts."""
+    grid = sim.grid
+    n = grid.total_cells
+    pml = sim.pml
+    frac = (lambda axis, half: pml.member_fraction(axis, half)) if pml else (lambda a, h: 0.0)
+    h = n * 48.0
+    d = n * 60.0
+    for c in range(3):
+        dsigu = (c + 2) % 3
+        h += n * (8 * frac(dsigu, 1) + 12 * frac(c, 0))
+        d += n * (8 * frac(dsigu, 0) + 12 * frac(c, 1))
+    rows = sim.pole_row_fraction()
+    for driven in sim.poles:
+        d += n * 16 * sum(driven) * rows
+    return {"h": h, "d": d}
 
 
 def step_bytes(sim) -> float:
@@ -131,11 +203,12 @@ def config_line(name: str, steps: int) -> dict:
     shapes = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
                            wl.poles, use_cache=False, inv=wl.inv)
     drv = FdtdDriver.from_workload(wl)
-    sim = drv.freeze(planned_steps=steps + 101)
-    sim.run(100)
+    sim = drv.freeze(planned_steps=steps + WARM + 1)
+    sim.run(WARM)  # the first step graph is captured (and replayed once) here
     sim.synchronize()
     t0 = time.perf_counter()
     sim.run(steps)
+    sim.flush()  # the samples the window took and has not transformed yet
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
@@ -147,6 +220,10 @@ def config_line(name: str, steps: int) -> dict:
             "launch_shape": {"h_pair": list(shapes[0]), "d_pair": list(shapes[1])},
             "spectra_finite": bool(finite)}
 
+
+#: warm-up steps of the configuration, batched and one-at-a-time legs: one ring period, so the
+#: first 128-step graph is captured before any timed window and the rings start it empty
+WARM = 128
 
 #: False under --no-graphs: every leg, the one-at-a-time baselines' "graphed" runs included,
 #: launches eagerly (kernel tracing then sees no graph replay)
@@ -165,16 +242,18 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     drivers = []
     for wl in wls:
         drv = FdtdDriver.from_workload(wl)
-        drv.freeze(planned_steps=3 * steps + 3 * 64 + 4 * 16 * len(autotune.CANDIDATES) + 400)
+        drv.freeze(planned_steps=3 * steps + 3 * WARM + 4 * 16 * len(autotune.CANDIDATES) + 400)
         drivers.append(drv)
     build_s = time.perf_counter() - t_build
     cells = sum(d.grid.total_cells for d in drivers)
     batch = BatchedRun(drivers)
     shape = autotune.tune_batch(batch, steps=16)
-    batch.run(64)  # warm: first capture of the step graph
+    batch.flush()
+    batch.run(WARM)  # warm: the first capture of the step graph, rings empty after it
     batch.synchronize()
     t0 = time.perf_counter()
     batch.run(steps)
+    batch.flush()
     batch.synchronize()
     batched_s = time.perf_counter() - t0
     batch.close()
@@ -192,10 +271,12 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
                 print(f"bench: one at a time (graphs {graphs}), simulation {q}/{len(drivers)}",
                       file=sys.stderr, flush=True)
             autotune.tune(d.grid, d.sim.pml, [], repeats=3)
-            d.sim.run(64)
+            d.sim.flush()
+            d.sim.run(WARM)
             d.sim.synchronize()
             t0 = time.perf_counter()
             d.sim.run(steps)
+            d.sim.flush()
             d.sim.synchronize()
             alone_s += time.perf_counter() - t0
         alone[graphs] = cells * steps / alone_s / 1e6
@@ -225,32 +306,26 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
 
 def served_line(count: int, rate: float, seed: int) -> dict:
     """The served form: a Poisson stream of solves under continuous batching, beside the same
-    stream served one solve at a time (one slot)."""
+    stream served one solve at a time (one slot, each solve with its step graphs)."""
     from meep_gpu.serve import SolveServer, poisson_requests
-
-    from meep_gpu.engine import set_graphs
 
     reqs = poisson_requests(count=count, rate=rate, seed=seed)
     rep = SolveServer(max_slots=64, quantum=256).run(reqs)
-    one = {}
-    for graphs in (False, True):
-        set_graphs(graphs and GRAPHS)
-        one[graphs] = SolveServer(max_slots=1, quantum=256).run(reqs).summary()
-    set_graphs(GRAPHS)
-    s, o, og = rep.summary(), one[False], one[True]
+    print("bench: served form, one solve at a time", file=sys.stderr, flush=True)
+    one = SolveServer(max_slots=1, quantum=256).run(reqs).summary()
+    s = rep.summary()
     return {"workload": f"{count} Poisson arrivals at {rate}/s (seed {seed}): sphere solves, "
                         f"edge 32-160 log-uniform, 30% Lorentz, 1-8 planes x 16-100 freqs, "
                         f"1000-8000 steps",
             "cell_steps": rep.cell_steps, **s,
-            "one_at_a_time": {k: o[k] for k in ("p50_latency_s", "p99_latency_s",
-                                                 "aggregate_mcell_steps_per_s", "makespan_s")},
+            "one_at_a_time": {k: one[k] for k in ("p50_latency_s", "p99_latency_s",
+                                                   "aggregate_mcell_steps_per_s", "makespan_s")},
+            "one_at_a_time_note": ("the same stream through one slot: each solve alone, with "
+                                   "its 128-step graphs" if GRAPHS else
+                                   "the same stream through one slot: each solve alone"),
             "speedup_vs_one_at_a_time": round(s["aggregate_mcell_steps_per_s"]
-                                              / max(o["aggregate_mcell_steps_per_s"], 1e-9), 2),
-            "one_at_a_time_graphed": {k: og[k] for k in ("p50_latency_s", "p99_latency_s",
-                                                         "aggregate_mcell_steps_per_s",
-                                                         "makespan_s")},
-            "speedup_vs_one_at_a_time_graphed": round(
-                s["aggregate_mcell_steps_per_s"] / max(og["aggregate_mcell_steps_per_s"], 1e-9), 2)}
+                                              / max(one["aggregate_mcell_steps_per_s"], 1e-9),
+                                              2)}
 
 
 def main(argv=None) -> int:
@@ -260,17 +335,20 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="sphere192")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU restatement and the float64 oracle comparison")
+    ap.add_argument("--precision-steps", type=int, default=160,
+                    help="steps of the Lorentz slab's float64 oracle comparison")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the sphere80 and slab160 lines")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--serve-count", type=int, default=32)
+    ap.add_argument("--serve-count", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-saturated-rate", type=float, default=32.0,
-                    help="a second stream whose arrivals outpace the device (capacity)")
+    ap.add_argument("--serve-saturated-rate", type=float, default=0.0,
+                    help="a second stream whose arrivals outpace the device (capacity); 0: none")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step eagerly, in every leg (for kernel tracing)")
     ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[/V,BPC,REMAP,NT]",
@@ -288,7 +366,7 @@ def main(argv=None) -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not run here (the engine is a "
-              f"single-GPU engine; how a run would shard is described in DESIGN.md)",
+              f"single-GPU engine)",
               file=sys.stderr)
         return 2
 
@@ -321,15 +399,18 @@ def main(argv=None) -> int:
     sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
     print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
     sim.run(args.warmup)
+    sim.flush()  # the window starts with empty rings: it transforms exactly its own samples
     sim.synchronize()
     t0 = time.perf_counter()
-    total_ms, h_ms, d_ms = sim.run_timed(args.steps)
+    total_ms, h_ms, d_ms = sim.run_timed(args.steps)  # ends with the pass over what is buffered
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     cells = wl.grid.total_cells
     value = cells * args.steps / elapsed / 1e6
-    nbytes = algorithmic_bytes(sim)
+    nbytes = compulsory_bytes(sim)
+    compact = algorithmic_bytes(sim)
     achieved = nbytes["d"] / (d_ms * 1e-3) / 1e9
+    compact_achieved = compact["d"] / (d_ms * 1e-3) / 1e9
     dshape = shape[1]
     widths = set(sim.inverse_epsilon_bytes())
     inv_mode = {1: 1, 2: 2, 4: 0}[widths.pop()] if len(widths) == 1 else 3
@@ -363,10 +444,21 @@ def main(argv=None) -> int:
                                         if traffic else f"no profile line for {dkernel}"),
                      "kernel": f"{dkernel} (step_D+deposit+update_E+update_P)",
                      "bytes_per_launch": int(nbytes["d"]), "mean_launch_ms": round(d_ms, 5),
+                     "bytes_model": ("every array once per launch, neighbours free, layer "
+                                     "auxiliaries on their rows, inverse epsilon float32"),
+                     "compact_layout": {"bytes_per_launch": int(compact["d"]),
+                                        "achieved": round(compact_achieved, 1),
+                                        "frac": round(compact_achieved / HBM_PEAK_GBS, 4),
+                                        "note": "the bytes this engine's layout must move: "
+                                                "inverse epsilon as 1-byte table indices, H "
+                                                "stored only on its layer rows"},
                      "h_pair": {"bytes_per_launch": int(nbytes["h"]),
                                 "mean_launch_ms": round(h_ms, 5),
-                                "achieved": round(nbytes["h"] / (h_ms * 1e-3) / 1e9, 1)},
-                     "step_model_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2)},
+                                "achieved": round(nbytes["h"] / (h_ms * 1e-3) / 1e9, 1),
+                                "compact_bytes_per_launch": int(compact["h"])},
+                     "step_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2),
+                     "compact_step_bytes_per_cell": round((compact["h"] + compact["d"]) / cells,
+                                                          2)},
         "spectra_finite": bool(spectra_ok),
         "launch_shape": {name: {"cells_per_thread": sh[0], "blocks_per_cu": sh[1],
                                 "xcd_remap": int(sh[2]), "nt_stores": bool(sh[3])}
@@ -389,7 +481,14 @@ def main(argv=None) -> int:
             line["served_saturated"] = served_line(args.serve_count, args.serve_saturated_rate,
                                                    20261203)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+        print("bench: CPU restatement and the float64 oracle comparison", file=sys.stderr,
+              flush=True)
+        line["cpu_baseline"], own = cpu_baseline(wl, args.cpu_seconds)
+        line["precision"] = {wl.name: own}
+        if not args.no_configs:
+            line["precision"]["slab_160"] = oracle_precision("slab160", args.precision_steps)
+        line["precision"]["note"] = ("fields float32 word for word with the oracle; phase and "
+                                     "spectral sums float64 on both sides")
     print(json.dumps(line), flush=True)
     return 0
 

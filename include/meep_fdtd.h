@@ -60,6 +60,8 @@ int fdtd_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
  * Replaces Fields allocation (fields.py:605-760) and enable_pml_storage (fields.py:708).
  * All volumes start at +0; inverse epsilon starts at 1. */
 int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out);
+/* Release a simulation (waits for its stream). A member of a live batch is not freed at once:
+ * its release is deferred to fdtd_batch_destroy (fdtd_last_error says so). */
 void fdtd_sim_destroy(fdtd_sim *sim);
 
 /* Bytes of device memory the simulation holds. */
@@ -137,6 +139,12 @@ int fdtd_run(fdtd_sim *sim, int32_t nsteps);
 
 /* Wait for the simulation's stream. */
 int fdtd_synchronize(fdtd_sim *sim);
+
+/* Run the float64 transform pass over every plane's buffered samples now, on the simulation's
+ * stream (reads do this for their plane; a timed run calls it so that all the samples of its
+ * window are transformed inside it). The counterpart of fdtd_batch_flush for a lone
+ * simulation; the reference transforms at every step (dft.py:3031), so it has no call for it. */
+int fdtd_flush(fdtd_sim *sim);
 
 /* A planar flux monitor over the box [lo, lo+cnt) (one count is 1 along normal_axis):
  * nf angular frequencies omega, scale = dt*decimation/sqrt(2*pi). The four tangential
@@ -261,8 +269,9 @@ int fdtd_time_pairs(fdtd_sim *sim, int32_t repeats, float out_ms[2]);
 /* nsteps whole steps launched eagerly in chunks of 128, as fdtd_run launches the steps it does
  * not replay from a graph (each launch reads the step counters at its step's offset in the
  * chunk; the counters advance once per chunk), with HIP events around every fused-pair launch
- * on the simulation's stream, read after each chunk. out_ms[0] = total elapsed, [1] = mean H
- * pair, [2] = mean D pair (per step). */
+ * on the simulation's stream, read after each chunk, then the transform pass over every
+ * sample still buffered (fdtd_flush), inside the window. out_ms[0] = total elapsed, [1] = mean
+ * H pair, [2] = mean D pair (per step). */
 int fdtd_run_timed(fdtd_sim *sim, int32_t nsteps, float out_ms[3]);
 
 /* Diagnostic: mean duration of a device copy of `bytes` (the reference point that scales the

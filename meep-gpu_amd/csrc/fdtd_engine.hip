@@ -1073,6 +1073,9 @@ struct PlaneDev {
     int nf;
     int dec;
     int first_step;     // sampling counts steps after this one
+    int base;           // ring cursor: sample n (0-based since first_step) goes to slot
+                        // (n - base) mod kRing, so that the transform pass after an early
+                        // read (which moves base by the samples it took) starts at slot 0
     double scale;
     const double *omega;
     float *ring;        // [kRing][nslot][row]
@@ -1153,7 +1156,8 @@ __global__ void flux_sample_kernel(const PlaneDev *planes, int nplanes, int adv)
     const int step = s.step + adv;
     const int since = step - p.first_step;
     if (since <= 0 || since % p.dec) return;
-    const int slot = (since / p.dec - 1) % kRing;
+    int slot = (since / p.dec - 1 - p.base) % kRing;
+    if (slot < 0) slot += kRing;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) gp(p.times)[slot] = (double)step * s.dt;
     if (q >= p.cells) return;
@@ -1337,6 +1341,12 @@ __global__ void flux_scale_kernel(double *acc, size_t n, double re, double im) {
     i = i + y * re;
     a[0] = r;
     a[1] = i;
+}
+
+// One int of a device structure set in stream order (a plane's ring cursor after an early
+// transform pass), with the value as a kernel argument: no host buffer has to outlive the call.
+__global__ void set_int_kernel(int *p, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
 __global__ void fill_kernel(float *p, size_t n, float v) {
@@ -1570,19 +1580,35 @@ int g_graphs = 1;  // replay captured step sequences where eligible
 int g_flux_mfma = 1;  // the flux transform on the matrix cores with this many 16-cell tiles per
                       // wave (1, 2 or 4), or 0: the vector FMA kernel
 
+// The kernel instantiations a group's launches select (its general/lossless forms, inverse
+// epsilon storage and off-diagonal path): a captured sequence is valid only for the same ones.
+int group_signature(const Group &g) {
+    return (g.general ? 1 : 0) | (g.inv_mode << 1) | (g.any_offdiag ? 8 : 0) |
+           (g.all_offdiag ? 16 : 0);
+}
+
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
+    hipStream_t st = nullptr;  // the stream its replays are queued on
     std::vector<int> phase;
     LaunchShape shape[2];
     int flux_mode = -1;  // the transform kernel the sequence was captured with
+    int signature = -1;  // group_signature at capture
+    // Replays of the exec may still be queued on its stream: they finish before it is destroyed
+    // (and before any buffer its launches read is freed, which callers do after a reset).
     void reset() {
-        if (exec) (void)hipGraphExecDestroy(exec);
+        if (exec) {
+            if (st) (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(exec);
+        }
         exec = nullptr;
+        st = nullptr;
         phase.clear();
     }
-    bool matches(const std::vector<int> &ph) const {
+    bool matches(const std::vector<int> &ph, const Group &g) const {
         return exec && ph == phase && same_shape(shape[0], g_shape[0]) &&
-               same_shape(shape[1], g_shape[1]) && flux_mode == g_flux_mfma;
+               same_shape(shape[1], g_shape[1]) && flux_mode == g_flux_mfma &&
+               signature == group_signature(g);
     }
     ~StepGraph() { reset(); }
 };
@@ -1603,6 +1629,7 @@ struct fdtd_sim {
     uint64_t bytes = 0;
     bool dirty = true;
     bool in_batch = false;
+    bool destroy_pending = false;  // fdtd_sim_destroy was called while in a batch
     StepGraph graph;
     std::vector<hipEvent_t> timed_ev;  // fdtd_run_timed: around each fused pair of a chunk
 
@@ -1698,6 +1725,13 @@ int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
         flux_accumulate_kernel<<<dim3((p.cells + per_block - 1) / per_block, p.nslot, (p.nf + kFZ - 1) / kFZ),
                                  256, 0, st>>>(p.ring, p.phase, p.acc, p.cells, p.row, p.nf, ph.filled,
                                                p.nslot, p.hmask);
+    }
+    // the next sample lands in slot 0: move the cursor by what this pass took (a full ring
+    // leaves it where it was, so a captured period's passes never write it)
+    const int base = (p.base + ph.filled) % kRing;
+    if (base != p.base) {
+        ph.dev.base = base;
+        set_int_kernel<<<1, 64, 0, st>>>(&dev_plane->base, base);
     }
     HIP_TRY(hipGetLastError());
     ph.filled = 0;
@@ -1864,10 +1898,12 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
         return rc;
     }
     k.graph->exec = exec;
+    k.graph->st = k.st;
     k.graph->phase = phase;
     k.graph->shape[0] = g_shape[0];
     k.graph->shape[1] = g_shape[1];
     k.graph->flux_mode = g_flux_mfma;
+    k.graph->signature = group_signature(k.g);
     HIP_TRY(hipGraphLaunch(exec, k.st));
     return FDTD_OK;
 }
@@ -1884,7 +1920,7 @@ int run_steps(const Stepper &k, int nsteps) {
             phase.push_back(ph.filled);
         }
     if (eligible) {
-        if (!k.graph->matches(phase)) {
+        if (!k.graph->matches(phase, k.g)) {
             int rc = capture_period(k, phase);
             if (rc) return rc;
             done = kRing;
@@ -2058,7 +2094,15 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
 
 void fdtd_sim_destroy(fdtd_sim *s) {
     if (!s) return;
+    if (s->in_batch) {
+        // the batch's descriptors and plane copies point into this simulation's memory: its
+        // release waits for fdtd_batch_destroy, which hands the counters back and then frees it
+        s->destroy_pending = true;
+        (void)fail(FDTD_ESTATE, "simulation is stepped by a batch: released when the batch is destroyed");
+        return;
+    }
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->graph.reset();  // before the memory its launches read is freed
     for (hipEvent_t e : s->timed_ev) (void)hipEventDestroy(e);
     for (void *p : s->allocs) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -2437,6 +2481,7 @@ int fdtd_add_pole(fdtd_sim *s, const float consts[4], const float *const sigma[3
     }
     h.npol = q + 1;
     s->dirty = true;
+    s->graph.reset();  // a captured sequence steps the poles (and instantiation) it had
     const bool exact_skip = nonnegative(consts[0]) && nonnegative(consts[1]) && nonnegative(consts[3]);
     for (int c = 0; c < 3; ++c) {
         if (!driven[c]) continue;
@@ -2648,6 +2693,21 @@ int fdtd_step_d(fdtd_sim *s) {
     return flux_after_step(lone(s), true);
 }
 
+// The transform pass over every plane's buffered samples (its ring then starts empty).
+static int flush_all(fdtd_sim *s) {
+    for (size_t r = 0; r < s->planes.size(); ++r) {
+        int rc = flux_flush_plane(s->dev_planes + r, s->planes[r], s->stream);
+        if (rc) return rc;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_flush(fdtd_sim *s) {
+    if (!s) return fail(FDTD_EINVAL, "null argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    return flush_all(s);
+}
+
 int fdtd_run(fdtd_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
@@ -2703,6 +2763,7 @@ static int add_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], in
     p.nf = nf;
     p.dec = dec;
     p.first_step = s->host.step;
+    p.base = 0;
     p.scale = scale;
     void *m;
     rc = s->alloc(&m, sizeof(double) * nf);
@@ -2855,6 +2916,9 @@ int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
     hipEvent_t t0 = s->timed_ev[3 * kRing], t1 = s->timed_ev[3 * kRing + 1];
     HIP_TRY(hipEventRecord(t0, s->stream));
     rc = run_steps(k, nsteps);
+    if (rc) return rc;
+    // every sample taken in the window is transformed in it: the passes still due run here
+    rc = flush_all(s);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(t1, s->stream));
     HIP_TRY(hipEventSynchronize(t1));
@@ -3016,7 +3080,10 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
 void fdtd_batch_destroy(fdtd_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    // hand the advanced counters back to each simulation's own descriptor
+    b->graph.reset();  // before the tile lists and plane copies its launches read are freed
+    // hand the advanced counters back to each simulation's own descriptor, and the planes'
+    // ring cursors (moved by transform passes over the batch's copies) to its own plane copies
+    std::vector<fdtd_sim *> release;
     if (b->group.dev && !b->sims.empty()) {
         std::vector<SimDev> descs(b->sims.size());
         if (hipMemcpy(descs.data(), b->group.dev, sizeof(SimDev) * descs.size(), hipMemcpyDeviceToHost) == hipSuccess)
@@ -3026,11 +3093,19 @@ void fdtd_batch_destroy(fdtd_batch *b) {
                 b->sims[q]->host.ppar = descs[q].ppar;
                 b->sims[q]->dirty = true;
             }
-        for (auto *s : b->sims) s->in_batch = false;
+        for (auto *s : b->sims) {
+            s->in_batch = false;
+            if (s->destroy_pending) {
+                release.push_back(s);
+                continue;
+            }
+            if (s->push_planes() == FDTD_OK) (void)hipStreamSynchronize(s->stream);
+        }
     }
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    for (auto *s : release) fdtd_sim_destroy(s);
 }
 
 int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {

@@ -56,6 +56,7 @@ SIGNATURES = {
     "fdtd_step_d": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fdtd_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_flush": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_add_flux_plane": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _IP, _IP,
                                            ctypes.c_int32, _DP, ctypes.c_double, ctypes.c_int32,
                                            _IP]),

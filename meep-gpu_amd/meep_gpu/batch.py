@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 from typing import List, Optional, Sequence
 
 import numpy
@@ -49,6 +50,8 @@ class BatchedRun:
         check(self.lib.fdtd_batch_create(handles, len(self.drivers), ctypes.byref(out)),
               "fdtd_batch_create", self.lib)
         self.handle = out
+        for drv in self.drivers:
+            drv.sim._batch = weakref.ref(self)
 
     @property
     def cells(self) -> int:
@@ -68,6 +71,9 @@ class BatchedRun:
         if self.handle is not None and self.handle.value:
             self.lib.fdtd_batch_destroy(self.handle)
         self.handle = None
+        for drv in self.drivers:
+            if drv.sim is not None:
+                drv.sim._batch = None
 
     def __del__(self):
         try:

@@ -61,14 +61,32 @@ class HipSimulation:
         check(rc, what, self.lib)
 
     # --- lifetime ----------------------------------------------------------------------------
+    #: the live ``BatchedRun`` stepping this simulation (a weak reference), or None
+    _batch = None
+
+    def batch(self):
+        """The live batch this simulation is a member of, or None."""
+        ref = self._batch
+        b = ref() if ref is not None else None
+        return b if b is not None and getattr(b, "handle", None) is not None else None
+
     def close(self):
+        """Free the device state. Refused while a live batch steps the simulation: its
+        descriptors point into this memory, so close the ``BatchedRun`` first."""
+        if self.batch() is not None:
+            raise RuntimeError("this simulation is a member of a live BatchedRun: close the "
+                               "batch before the simulation")
+        self._release()
+
+    def _release(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
+            # a member of a batch the engine still holds is released with that batch
             self.lib.fdtd_sim_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
         try:
-            self.close()
+            self._release()
         except Exception:
             pass
 
@@ -218,6 +236,10 @@ class HipSimulation:
 
     def synchronize(self):
         self._check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
+
+    def flush(self):
+        """Transform every plane's buffered samples now (``fdtd_flush``), in stream order."""
+        self._check(self.lib.fdtd_flush(self.handle), "fdtd_flush")
 
     def run_timed(self, steps: int) -> Tuple[float, float, float]:
         """(total ms, mean H-pair ms, mean D-pair ms) of ``steps`` steps, HIP events per launch."""

@@ -1,9 +1,9 @@
 """Sphere and Lorentz-slab workloads with sources and flux planes: HIP engine vs the oracle (GPU).
 
-Fields are compared word for word. Spectra: the engine transforms in float64 (blocked over 32
+Fields are compared word for word. Spectra: the engine transforms in float64 (blocked over 128
 steps), the oracle in float64 per step (the sums differ only in association, so 1e-9 relative
 to the spectrum's peak is the stated tolerance), and the reference's own complex64
-accumulation lies within 1e-4 of both (the north-star flux tolerance).
+accumulation lies within 1e-4 of both (the stated flux tolerance).
 """
 
 import numpy
