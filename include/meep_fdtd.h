@@ -220,6 +220,16 @@ int fdtd_flux_write(fdtd_sim *sim, int32_t plane, const double *in, size_t count
  * -1 after fdtd_flux_write is MEEP's load_minus_flux_data (dft.py:3297-3316). */
 int fdtd_flux_scale(fdtd_sim *sim, int32_t plane, double re, double im);
 
+/* The flux spectrum of one flux plane, reduced on the device from its float64 transforms
+ * (the buffered samples are transformed in first): out[f] = sum over the plane's cells of
+ * w * Re(E1 conj(H2) - E2 conj(H1)) for each of its count = nf frequencies, w = weights[cell]
+ * (host, per box cell, x fastest: the layout fdtd_flux_read writes) or 1 when weights is NULL.
+ * The caller scales by the area element and the plane's weight. Replaces the summation of
+ * FluxMonitor._integrate_flux (reference dft.py:3107-3130) over downloaded transforms: nf
+ * doubles cross to the host instead of 8*nf*cells. Not for DFT field regions. */
+int fdtd_flux_spectrum(fdtd_sim *sim, int32_t plane, const double *weights, double *out,
+                       size_t count);
+
 /* Batched form: many simulations advanced together, one fused launch per half-step over a
  * work list of (simulation, tile) entries, and one sampling launch for every flux plane of
  * every member. The batch does not own the simulations; while one exists its members are
@@ -277,7 +287,11 @@ int fdtd_run_timed(fdtd_sim *sim, int32_t nsteps, float out_ms[3]);
 /* Diagnostic: mean duration of a device copy of `bytes` (the reference point that scales the
  * HBM byte counters of a profile): mode 0 one float4 per thread per pass, 1 four loads in
  * flight per thread, 2 the same with non-temporal stores, 3 read only, 4 write only (the time
- * is then for `bytes` moved once, not twice). No counterpart in the reference. */
+ * is then for `bytes` moved once, not twice); one-pass copies with every load issued before the
+ * first store: 5 eight float4 loads per thread, non-temporal loads and stores, 6 the same with
+ * plain loads and stores, 7 four per thread non-temporal, 8 four per thread with plain loads
+ * and non-temporal stores (bytes rounded down to whole 32 KiB groups). No counterpart in the
+ * reference. */
 int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms);
 
 /* The fused pairs' access pattern without their arithmetic: `nread` read streams and `nwrite`

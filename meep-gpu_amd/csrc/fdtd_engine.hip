@@ -1343,6 +1343,58 @@ __global__ void flux_scale_kernel(double *acc, size_t n, double re, double im) {
     a[1] = i;
 }
 
+// The flux spectrum of one plane reduced on the device (dft.py:3107-3130 as the host restates
+// it): per frequency f, sum over the plane's cells of w * Re(E1 conj(H2) - E2 conj(H1)) from the
+// float64 accumulators [E1, E2, H1, H2][f][cell], w per cell or 1. Stage 1: workgroup (b, f)
+// sums the cells b, b + G, ... (G = gridDim.x workgroups of 256 threads per frequency) into
+// partial[f][b]: each lane adds its cells in order, the wave folds its 64 lanes with cross-lane
+// shuffles and the four waves meet in LDS. Stage 2 sums partial[f][0..G) in order. Both are
+// float64; the host's own sum associates differently and agrees to rounding.
+constexpr int kSpecBlocks = 128;  // stage-1 workgroups per frequency (at most)
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) flux_spectrum_partial_kernel(
+    const double *__restrict__ acc, const double *__restrict__ w, int cells, int nf,
+    double *__restrict__ partial) {
+    __shared__ double part[4];
+    const int f = blockIdx.y, G = gridDim.x;
+    const size_t comp = (size_t)nf * cells * 2;  // doubles per component
+    const GMEM double *e1 = gp(acc) + (size_t)f * cells * 2;
+    const GMEM double *e2 = e1 + comp;
+    const GMEM double *h1 = e1 + 2 * comp;
+    const GMEM double *h2 = e1 + 3 * comp;
+    double s = 0.0;
+    for (int c = blockIdx.x * 256 + (int)threadIdx.x; c < cells; c += G * 256) {
+        double a = e1[2 * c] * h2[2 * c];
+        a = a + e1[2 * c + 1] * h2[2 * c + 1];
+        double b = e2[2 * c] * h1[2 * c];
+        b = b + e2[2 * c + 1] * h1[2 * c + 1];
+        double t = a - b;
+        if (w) t = t * gp(w)[c];
+        s = s + t;
+    }
+    s = wave_sum(s);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        gp(partial)[(size_t)f * G + blockIdx.x] = ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+__global__ void flux_spectrum_final_kernel(const double *__restrict__ partial, int G, int nf,
+                                           double *__restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    double s = 0.0;
+    for (int b = 0; b < G; ++b) s = s + gp(partial)[(size_t)f * G + b];
+    gp(out)[f] = s;
+}
+
 // One int of a device structure set in stream order (a plane's ring cursor after an early
 // transform pass), with the value as a kernel argument: no host buffer has to outlive the call.
 __global__ void set_int_kernel(int *p, int v) {
@@ -1422,6 +1474,27 @@ __global__ void copy_probe4_kernel(const float4 *__restrict__ src, float4 *__res
     }
 }
 
+// A one-pass copy with U float4 loads in flight per thread, all issued before the first store:
+// workgroup b copies the U consecutive 256-float4 slabs starting at slab b*U (each slab one
+// coalesced dwordx4 sweep of the 256 threads). NTL / NTS: non-temporal loads / stores. The grid
+// covers n exactly (n a multiple of 256*U), so every access is in bounds.
+template <int U, bool NTL, bool NTS>
+__global__ void __launch_bounds__(256) copy_wide_kernel(const f4v *__restrict__ src,
+                                                        f4v *__restrict__ dst) {
+    const size_t base = (size_t)blockIdx.x * U * 256 + threadIdx.x;
+    const GMEM f4v *s = gp(src);
+    GMEM f4v *d = gp(dst);
+    f4v v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        v[u] = NTL ? __builtin_nontemporal_load(s + base + (size_t)u * 256) : s[base + (size_t)u * 256];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (NTS) __builtin_nontemporal_store(v[u], d + base + (size_t)u * 256);
+        else d[base + (size_t)u * 256] = v[u];
+    }
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -1463,6 +1536,9 @@ bool same_shape(const LaunchShape &a, const LaunchShape &b) {
 struct PlaneHost {
     PlaneDev dev;
     int filled = 0;  // samples in the ring since the last transform pass
+    // the device flux-spectrum reduction's buffers (fdtd_flux_spectrum), made on first use:
+    // cell weights, stage-1 partial sums [nf][kSpecBlocks], the nf results
+    double *spec_w = nullptr, *spec_partial = nullptr, *spec_out = nullptr;
 };
 
 // A set of simulations stepped by the same launches: one, or a batch.
@@ -2861,6 +2937,54 @@ int fdtd_flux_scale(fdtd_sim *s, int32_t plane, double re, double im) {
     return FDTD_OK;
 }
 
+// A flux plane (the four tangential components, cell-centred), as opposed to a DFT region.
+static bool is_flux_plane(const PlaneDev &p) {
+    return p.nslot == 4 && !p.yee && p.comp[0] < 3 && p.comp[1] == (p.comp[0] + 1) % 3 &&
+           p.comp[2] == p.comp[0] + 3 && p.comp[3] == p.comp[1] + 3;
+}
+
+// The two reduction launches of a plane's spectrum into out_dev[nf] (see the kernels).
+static int launch_spectrum(const PlaneDev &p, const double *w_dev, double *partial,
+                           double *out_dev, hipStream_t st) {
+    const int G = std::max(1, std::min(kSpecBlocks, (p.cells + 255) / 256));
+    flux_spectrum_partial_kernel<<<dim3(G, p.nf), 256, 0, st>>>(p.acc, w_dev, p.cells, p.nf,
+                                                                 partial);
+    flux_spectrum_final_kernel<<<(p.nf + 63) / 64, 64, 0, st>>>(partial, G, p.nf, out_dev);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int fdtd_flux_spectrum(fdtd_sim *s, int32_t plane, const double *weights, double *out,
+                       size_t count) {
+    if (!s || plane < 0 || plane >= (int)s->planes.size() || !out)
+        return fail(FDTD_EINVAL, "fdtd_flux_spectrum: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    PlaneHost &ph = s->planes[plane];
+    const PlaneDev &p = ph.dev;
+    if (!is_flux_plane(p)) return fail(FDTD_EINVAL, "region %d is not a flux plane", plane);
+    if (count != (size_t)p.nf) return fail(FDTD_EINVAL, "the plane has %d frequencies, got %zu", p.nf, count);
+    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    if (rc) return rc;
+    if (!ph.spec_partial) {
+        rc = s->alloc((void **)&ph.spec_partial, sizeof(double) * (size_t)p.nf * kSpecBlocks);
+        if (rc == FDTD_OK) rc = s->alloc((void **)&ph.spec_out, sizeof(double) * (size_t)p.nf);
+        if (rc) return rc;
+    }
+    if (weights && !ph.spec_w) {
+        rc = s->alloc((void **)&ph.spec_w, sizeof(double) * (size_t)p.cells);
+        if (rc) return rc;
+    }
+    if (weights)
+        HIP_TRY(hipMemcpyAsync(ph.spec_w, weights, sizeof(double) * (size_t)p.cells,
+                               hipMemcpyHostToDevice, s->stream));
+    rc = launch_spectrum(p, weights ? ph.spec_w : nullptr, ph.spec_partial, ph.spec_out, s->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, ph.spec_out, sizeof(double) * (size_t)p.nf, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
 int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
     if (!s || repeats < 1 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
@@ -2931,8 +3055,10 @@ int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
 }
 
 int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms) {
-    if (bytes < 1024 || repeats < 1 || !ms || mode < 0 || mode > 4) return fail(FDTD_EINVAL, "bad argument");
-    const size_t n = (size_t)(bytes / 16);
+    if (bytes < 1024 || repeats < 1 || !ms || mode < 0 || mode > 8) return fail(FDTD_EINVAL, "bad argument");
+    // the one-pass wide copies (modes 5-8) take whole 256*8-float4 groups
+    const size_t n = mode >= 5 ? (size_t)(bytes / (16 * 256 * 8)) * 256 * 8 : (size_t)(bytes / 16);
+    if (n == 0) return fail(FDTD_EINVAL, "copy probe: %llu bytes is too small", (unsigned long long)bytes);
     float4 *a = nullptr, *b = nullptr;
     HIP_TRY(hipMalloc((void **)&a, n * 16));
     if (hipMalloc((void **)&b, n * 16) != hipSuccess) { (void)hipFree(a); return fail(FDTD_ENOMEM, "copy probe"); }
@@ -2946,7 +3072,11 @@ int fdtd_copy_probe(uint64_t bytes, int32_t repeats, int32_t mode, float *ms) {
         else if (mode == 1) copy_probe4_kernel<false><<<grid, 256>>>(a, b, n);
         else if (mode == 2) copy_probe4_kernel<true><<<grid, 256>>>(a, b, n);
         else if (mode == 3) read_probe_kernel<<<grid, 256>>>(a, (float *)b, n);
-        else write_probe_kernel<<<grid, 256>>>(b, n);
+        else if (mode == 4) write_probe_kernel<<<grid, 256>>>(b, n);
+        else if (mode == 5) copy_wide_kernel<8, true, true><<<(unsigned)(n / 2048), 256>>>((const f4v *)a, (f4v *)b);
+        else if (mode == 6) copy_wide_kernel<8, false, false><<<(unsigned)(n / 2048), 256>>>((const f4v *)a, (f4v *)b);
+        else if (mode == 7) copy_wide_kernel<4, true, true><<<(unsigned)(n / 1024), 256>>>((const f4v *)a, (f4v *)b);
+        else copy_wide_kernel<4, false, true><<<(unsigned)(n / 1024), 256>>>((const f4v *)a, (f4v *)b);
     };
     go();
     HIP_TRY(hipEventRecord(e0, 0));

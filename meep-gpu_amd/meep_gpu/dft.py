@@ -307,6 +307,19 @@ class FluxMonitor:
         self.scale_dfts(-1.0)
 
     def get_flux_spectrum(self) -> numpy.ndarray:
+        """The flux per frequency (reference ``get_flux_spectrum``, dft.py:3090): each sample
+        plane's sum reduced on the device, scaled by the area element and its weight."""
+        if self._sim is None:
+            raise RuntimeError("the monitor is not attached to a running simulation")
+        total = None
+        for pid, (_, _, w) in zip(self.plane_ids, self.plane_boxes()):
+            part = self._sim.flux_spectrum(pid, self.cell_weights) * self.area_element * (
+                self.weight * w)
+            total = part if total is None else total + part
+        return total
+
+    def get_flux_spectrum_host(self) -> numpy.ndarray:
+        """The same spectrum summed on the host over the downloaded transforms."""
         if self._sim is None:
             raise RuntimeError("the monitor is not attached to a running simulation")
         total = None

@@ -298,6 +298,21 @@ class HipSimulation:
         data = buf.view(numpy.complex128).reshape(nslot, nf, cnt[2], cnt[1], cnt[0])
         return numpy.ascontiguousarray(data.transpose(0, 1, 4, 3, 2))
 
+    def flux_spectrum(self, plane: int, cell_weights: Optional[numpy.ndarray] = None
+                      ) -> numpy.ndarray:
+        """sum over the plane of w * Re(E1 conj(H2) - E2 conj(H1)) per frequency, reduced on the
+        device (``fdtd_flux_spectrum``); ``cell_weights`` C-order over the box, or None for 1."""
+        normal, lo, cnt, nf, nslot = self.planes[plane]
+        out = numpy.empty(nf, dtype=numpy.float64)
+        w = None
+        if cell_weights is not None:
+            w = numpy.ascontiguousarray(numpy.broadcast_to(
+                numpy.asarray(cell_weights, dtype=numpy.float64), tuple(cnt)).transpose(2, 1, 0))
+        self._check(self.lib.fdtd_flux_spectrum(self.handle, int(plane),
+                                                dptr(w) if w is not None else None, dptr(out),
+                                                out.size), "fdtd_flux_spectrum")
+        return out
+
     def flux_write(self, plane: int, transforms: numpy.ndarray) -> None:
         """Replace a plane's transforms: complex [4][nf][cnt...] as ``flux_read`` returns them."""
         normal, lo, cnt, nf, nslot = self.planes[plane]
