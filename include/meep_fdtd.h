@@ -230,6 +230,17 @@ int fdtd_flux_scale(fdtd_sim *sim, int32_t plane, double re, double im);
 int fdtd_flux_spectrum(fdtd_sim *sim, int32_t plane, const double *weights, double *out,
                        size_t count);
 
+/* The same spectrum with nothing allocated: for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops). No counterpart in the reference. `weights` (or NULL for
+ * 1) and `out` (count = nf doubles) are DEVICE pointers the caller owns; the result is written
+ * in the simulation's stream order (fdtd_synchronize before reading it on the host). The partial
+ * sums live in the library's per-GPU static scratch of 1 MiB, so nf is at most 1024 (larger is
+ * refused with FDTD_EINVAL); calls that overlap on one GPU (simulations on different streams)
+ * race on that scratch and must be ordered by the caller. Stage 1 writes every partial sum
+ * stage 2 reads, so no call depends on what an earlier one left there. */
+int fdtd_flux_spectrum_noalloc(fdtd_sim *sim, int32_t plane, const double *weights, double *out,
+                               size_t count);
+
 /* Batched form: many simulations advanced together, one fused launch per half-step over a
  * work list of (simulation, tile) entries, and one sampling launch for every flux plane of
  * every member. The batch does not own the simulations; while one exists its members are

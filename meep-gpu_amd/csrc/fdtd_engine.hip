@@ -1497,6 +1497,12 @@ __global__ void __launch_bounds__(256) copy_wide_kernel(const f4v *__restrict__ 
 
 }  // namespace
 
+// The device scratch of fdtd_flux_spectrum_noalloc: the stage-1 partial sums of one spectrum,
+// [nf][kSpecBlocks] doubles for up to kScratchFreqs frequencies (1 MiB). Each GPU holds its own
+// copy for the life of the process; every call on a GPU shares it.
+constexpr int kScratchFreqs = 1024;
+static __device__ __align__(256) double scratch[kSpecBlocks * kScratchFreqs];
+
 // ==========================================================================================
 // Host side
 // ==========================================================================================
@@ -2983,6 +2989,48 @@ int fdtd_flux_spectrum(fdtd_sim *s, int32_t plane, const double *weights, double
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return FDTD_OK;
+}
+
+// The device address of `scratch` on the current GPU, looked up once per device with that
+// device current (host code never passes the array's own name, which is a host-side stand-in).
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64] = {};
+
+static int scratch_address(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(FDTD_EINVAL, "device ordinal %d", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FDTD_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FDTD_OK;
+}
+
+int fdtd_flux_spectrum_noalloc(fdtd_sim *s, int32_t plane, const double *weights, double *out,
+                               size_t count) {
+    if (!s || plane < 0 || plane >= (int)s->planes.size() || !out)
+        return fail(FDTD_EINVAL, "fdtd_flux_spectrum_noalloc: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    PlaneHost &ph = s->planes[plane];
+    const PlaneDev &p = ph.dev;
+    if (!is_flux_plane(p)) return fail(FDTD_EINVAL, "region %d is not a flux plane", plane);
+    if (count != (size_t)p.nf) return fail(FDTD_EINVAL, "the plane has %d frequencies, got %zu", p.nf, count);
+    if (p.nf > kScratchFreqs)
+        return fail(FDTD_EINVAL, "fdtd_flux_spectrum_noalloc takes at most %d frequencies, the plane has %d",
+                    kScratchFreqs, p.nf);
+    double *partial = nullptr;
+    int rc = scratch_address(&partial);
+    if (rc) return rc;
+    rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    if (rc) return rc;
+    // stage 1 writes every partial[f][b] stage 2 reads (b < the launch's G): nothing to clear
+    return launch_spectrum(p, weights, partial, out, s->stream);
 }
 
 int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {

@@ -63,6 +63,9 @@ SIGNATURES = {
     "fdtd_flux_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _DP, ctypes.c_size_t]),
     "fdtd_flux_spectrum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _DP, _DP,
                                           ctypes.c_size_t]),
+    "fdtd_flux_spectrum_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_size_t]),
     "fdtd_flux_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _DP, ctypes.c_size_t]),
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_conductivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
