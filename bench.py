@@ -139,18 +139,35 @@ def oracle_precision(name: str, steps: int) -> dict:
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_pairs_sphere192.txt")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (scaled by the copy calibration it states), or None when it has no line for it."""
+def measured_traffic(pair: str, kernels: str):
+    """HBM bytes per step of one fused pair (``pair`` "h_pair" or "d_pair", its launches named by
+    ``kernels``) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy
+    calibration it states), or None when it has no line for that pair and those kernels."""
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
-                    return int(parts[2])
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1] == pair \
+                        and parts[2] == kernels:
+                    return int(parts[3])
     except OSError:
         pass
     return None
+
+
+def pair_kernels(sim, shape) -> dict:
+    """The kernels each fused pair launches for this workload under ``shape`` ((H, D) of
+    (cells per thread, workgroups per CU, tile order, non-temporal stores, shell cells per
+    thread)): the box's rows and the shell's rows as two lean kernels where the simulation runs
+    the lossless forms, which the benchmark workloads do."""
+    (hv, _, _, hnt, hsv), (dv, _, _, dnt, dsv) = shape
+    widths = set(sim.inverse_epsilon_bytes())
+    inv = 1 if widths == {1} else 0
+    pole = "true" if len(sim.poles) == 1 else "false"
+    b = lambda x: "true" if x else "false"
+    return {"h_pair": f"hshell_kernel<{hv},{b(hnt)},true>+hshell_kernel<{hsv},{b(hnt)},false>",
+            "d_pair": (f"dshell_kernel<{dv},{b(dnt)},{inv},{pole},true>"
+                       f"+dshell_kernel<{dsv},{b(dnt)},{inv},{pole},false>")}
 
 
 def compulsory_bytes(sim) -> dict:
@@ -351,9 +368,9 @@ def main(argv=None) -> int:
                     help="a second stream whose arrivals outpace the device (capacity); 0: none")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step eagerly, in every leg (for kernel tracing)")
-    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[/V,BPC,REMAP,NT]",
-                    help="fixed launch shapes (H pair[/D pair]) instead of the autotuned ones "
-                         "(profiling runs)")
+    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[,SV][/V,BPC,REMAP,NT[,SV]]",
+                    help="fixed launch shapes (H pair[/D pair], SV the shell's cells per "
+                         "thread) instead of the autotuned ones (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="time a plain device copy of BYTES (profile counter calibration)")
     args = ap.parse_args(argv)
@@ -384,17 +401,22 @@ def main(argv=None) -> int:
         GRAPHS = False
         set_graphs(False)
     wl = workloads.config(args.workload)
+    from meep_gpu.engine import get_launch_shell, set_launch_shell
+
     if args.launch:
         parts = args.launch.split("/")
         shapes = []
-        for part in (parts[0], parts[-1]):
-            v, bpc, remap, nt = (int(x) for x in part.split(","))
+        for pair, part in enumerate((parts[0], parts[-1])):
+            fields = [int(x) for x in part.split(",")]
+            v, bpc, remap, nt = fields[:4]
+            set_launch_shell(pair, fields[4] if len(fields) > 4 else 1)
             shapes.append((v, bpc, int(remap), bool(nt)))
         shape = tuple(shapes)
         autotune.install(shape)
     else:
         shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
                               wl.poles, use_cache=False, inv=wl.inv)
+    shape = tuple(tuple(sh) + (get_launch_shell(pair),) for pair, sh in enumerate(shape))
     drv = FdtdDriver.from_workload(wl)
     sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
     print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
@@ -411,12 +433,9 @@ def main(argv=None) -> int:
     compact = algorithmic_bytes(sim)
     achieved = nbytes["d"] / (d_ms * 1e-3) / 1e9
     compact_achieved = compact["d"] / (d_ms * 1e-3) / 1e9
-    dshape = shape[1]
-    widths = set(sim.inverse_epsilon_bytes())
-    inv_mode = {1: 1, 2: 2, 4: 0}[widths.pop()] if len(widths) == 1 else 3
-    # the lossless periodic instantiation (the workload has no conductivity and no metallic wall)
-    dkernel = f"dpair_kernel<{dshape[0]},{'true' if dshape[3] else 'false'},{inv_mode},false>"
-    traffic = measured_traffic(dkernel)
+    kernels = pair_kernels(sim, shape)
+    dkernel = kernels["d_pair"]
+    traffic = measured_traffic("d_pair", dkernel)
     spectra_ok = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
     line = {
         "metric": "Mcell-steps/s (plain single-simulation FDTD step, fused HIP kernels)",
@@ -439,10 +458,12 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "traffic_source": (f"profiles/traffic_pairs_sphere192.txt ({dkernel}: "
-                                        "FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
-                                        if traffic else f"no profile line for {dkernel}"),
-                     "kernel": f"{dkernel} (step_D+deposit+update_E+update_P)",
+                     "traffic_source": ("profiles/traffic_pairs_sphere192.txt (FETCH_SIZE and "
+                                        "WRITE_SIZE per step over both launches, "
+                                        "copy-calibrated)" if traffic else
+                                        f"no profile line for {dkernel}"),
+                     "kernel": (f"{dkernel} (step_D+deposit+update_E+update_P: the box's rows "
+                                f"and the shell's rows)"),
                      "bytes_per_launch": int(nbytes["d"]), "mean_launch_ms": round(d_ms, 5),
                      "bytes_model": ("every array once per launch, neighbours free, layer "
                                      "auxiliaries on their rows, inverse epsilon float32"),
@@ -452,16 +473,21 @@ def main(argv=None) -> int:
                                         "note": "the bytes this engine's layout must move: "
                                                 "inverse epsilon as 1-byte table indices, H "
                                                 "stored only on its layer rows"},
-                     "h_pair": {"bytes_per_launch": int(nbytes["h"]),
+                     "h_pair": {"kernel": kernels["h_pair"],
+                                "bytes_per_launch": int(nbytes["h"]),
                                 "mean_launch_ms": round(h_ms, 5),
                                 "achieved": round(nbytes["h"] / (h_ms * 1e-3) / 1e9, 1),
-                                "compact_bytes_per_launch": int(compact["h"])},
+                                "frac": round(nbytes["h"] / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                              4),
+                                "compact_bytes_per_launch": int(compact["h"]),
+                                "traffic": measured_traffic("h_pair", kernels["h_pair"])},
                      "step_bytes_per_cell": round((nbytes["h"] + nbytes["d"]) / cells, 2),
                      "compact_step_bytes_per_cell": round((compact["h"] + compact["d"]) / cells,
                                                           2)},
         "spectra_finite": bool(spectra_ok),
         "launch_shape": {name: {"cells_per_thread": sh[0], "blocks_per_cu": sh[1],
-                                "xcd_remap": int(sh[2]), "nt_stores": bool(sh[3])}
+                                "xcd_remap": int(sh[2]), "nt_stores": bool(sh[3]),
+                                "shell_cells_per_thread": sh[4]}
                          for name, sh in (("h_pair", shape[0]), ("d_pair", shape[1]))},
     }
     drv.close()

@@ -90,6 +90,13 @@ int fdtd_inverse_epsilon_bytes(fdtd_sim *sim, int32_t out[3]);
  * whole grid, dispersion.py:689). */
 int fdtd_pole_rows(fdtd_sim *sim, int64_t *active, int64_t *total);
 
+/* How the fused launches split the grid for runs of `vec` (1, 2, 4) cells: out[0..5] = the
+ * box x0, x1, y0, y1, z0, z1 (whole padded rows, x0 = 0 and x1 = the row pitch, whose y and z are
+ * on no layer line: only the x line's forms hold there), out[6] = its runs, out[7] = the runs of
+ * the shell (the rows on a y or z layer line, each run holding a real cell). No counterpart in
+ * the reference (its region plan launches one kernel per region, chunk_map.py:1093). */
+int fdtd_work_split(fdtd_sim *sim, int32_t vec, int32_t out[8]);
+
 /* Off-diagonal chi1inv[row][partner] volume (fields.py:1396, stepping.py:1594). A row with
  * any partner switches update_E to the double-precision row form. */
 int fdtd_set_offdiagonal(fdtd_sim *sim, int32_t row, int32_t partner, const float *volume);
@@ -268,6 +275,13 @@ int fdtd_get_launch(int32_t out[4]);  /* the H pair's shape */
 int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32_t xcd_remap,
                          int32_t nt_stores);
 int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
+
+/* Cells per thread (1, 2 or 4; default 1) of a pair's shell kernel: the rows on a y or z
+ * layer line, stepped with every layer form where no simulation runs the general forms (the box
+ * of the other rows takes only the x line's forms, with the pair's own width above). The
+ * shell's forms hold more operands per cell, so a narrower run keeps more waves resident. */
+int fdtd_set_launch_shell(int32_t pair, int32_t vec);
+int fdtd_get_launch_shell(int32_t pair, int32_t *vec);
 
 /* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 128 whole
  * steps (both fused pairs, flux sampling and the transform passes that fall due, each launch

@@ -9,12 +9,15 @@
 // Device layout: every volume x fastest, element (i,j,k) at (k*ny+j)*pitch+i with the row
 // pitch nx rounded up to a multiple of 4 floats, every axis periodic (an absorber is a
 // material, reference stepping.py:2984). One `SimDev` descriptor per simulation lives in device
-// memory. Kernels walk a list of tiles of kThreads*V padded cells (each thread owns V
-// x-adjacent cells of one row), so a single run and a batch of uneven grids run the same
-// kernels. The launch is one workgroup per tile or a persistent grid; with the XCD remap,
-// workgroup b (which the dispatcher places on XCD b%8) takes tiles from the b%8-th contiguous
-// eighth of the list, so the neighbour rows and planes a tile reads were fetched into that
-// XCD's own L2 by the tiles just before it. Step counters live on the device and one wave
+// memory. Kernels walk a list of work items of kThreads runs of V x-adjacent cells (each thread
+// owns one run of one row), so a single run and a batch of uneven grids run the same kernels.
+// A grid's runs split into its box -- the whole rows whose y and z are on no absorber layer
+// line, where only the x line's forms can hold (off every line the layer forms are exactly the
+// lossless ones) -- and a shell of the rows on a y or z line; where no simulation runs the
+// general forms the two are stepped by two kernels, each with the registers its forms need. The launch is one workgroup per item or a persistent
+// grid; with the XCD remap, workgroup b (which the dispatcher places on XCD b%8) takes items
+// from the b%8-th contiguous eighth of the list, so the neighbour rows and planes an item reads
+// were fetched into that XCD's own L2 by the items just before it. Step counters live on the device and one wave
 // advances them after each D pair, so a step sequence carries no host state.
 #include <hip/hip_runtime.h>
 
@@ -35,6 +38,7 @@
 #include <new>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/meep_fdtd.h"
@@ -102,6 +106,7 @@ struct SimDev {
     const uint8_t *inv8[3];
     const uint16_t *inv16[3];
     const float *invtab[3];
+    int invn[3];  // words in invtab[c]
     const float *u[3][3];
     const float4 *line[2][3];  // (kms, sinv, kps, member) per row, [half][axis]
     int mr[2][3][2 * kMaxRuns];  // the membership lines as up to kMaxRuns [lo, hi) runs
@@ -120,6 +125,12 @@ struct SimDev {
     float *fcond[2][3];
     int cond;      // any conductivity
     int metal[3];  // metallic (PEC) axes
+    // The work split of every fused launch: the interior box [x0,x1) x [j0,j1) x [k0,k1) holds
+    // cells on no layer line of either lattice (x0, x1 multiples of 4), where the layer forms
+    // are exactly the lossless ones and every H read is B's word; every other run of V cells
+    // with a real cell is listed (its padded index) in shell[V = 1, 2, 4 -> 0, 1, 2].
+    int ibox[6];
+    const int *shell[3];
 };
 
 // ------------------------------------------------------------------------------------------
@@ -217,6 +228,23 @@ __device__ inline Run locate(const SimDev &s, int at) {
     return q;
 }
 
+// A run given by its coordinates (i0 a multiple of V inside the stored row).
+__device__ inline Run locate_ijk(const SimDev &s, int i0, int j, int k) {
+    Run q;
+    const int pitch = s.pitch, ny = s.n[1], nz = s.n[2];
+    q.i0 = i0;
+    q.j = j;
+    q.k = k;
+    q.row = (k * ny + j) * pitch;
+    q.at = q.row + i0;
+    const int plane = pitch * ny;
+    q.yp = (j + 1 < ny) ? pitch : -(ny - 1) * pitch;
+    q.ym = (j > 0) ? -pitch : (ny - 1) * pitch;
+    q.zp = (k + 1 < nz) ? plane : -(nz - 1) * plane;
+    q.zm = (k > 0) ? -plane : (nz - 1) * plane;
+    return q;
+}
+
 template <int V>
 struct VF {
     float v[V];
@@ -230,6 +258,22 @@ template <class T>
 __device__ __forceinline__ GMEM T *gp(T *p) { return (GMEM T *)p; }
 template <class T>
 __device__ __forceinline__ const GMEM T *gp(const T *p) { return (const GMEM T *)p; }
+
+// One work item of a fused launch: (simulation, kind, first, count) -- kind 0 the interior
+// box's runs first .. first+count-1 (numbered x fastest, then y, then z), kind 1 the shell
+// list's entries first .. first+count-1. Thread t of the workgroup takes entry first + t.
+template <int V>
+__device__ inline Run work_run(const SimDev &s, const int4 &w, int t) {
+    if (w.y == 0) {
+        const int q = w.z + t;
+        const int nqx = (s.ibox[1] - s.ibox[0]) / V, nj = s.ibox[3] - s.ibox[2];
+        const int r = q / nqx;
+        const int kk = r / nj;
+        return locate_ijk(s, s.ibox[0] + (q - r * nqx) * V, s.ibox[2] + (r - kk * nj),
+                          s.ibox[4] + kk);
+    }
+    return locate(s, gp(s.shell[V == 4 ? 2 : V - 1])[w.z + t]);
+}
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
@@ -602,17 +646,16 @@ __device__ inline void hpair_run(const SimDev &s, const Clock &ck, const Run &q)
 
 template <int V, bool NT, bool GEN>
 __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restrict__ sims,
-                                                         const int2 *__restrict__ tiles,
+                                                         const int4 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
         if (t >= ntiles) continue;
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
+        const int4 tile = tiles[t];
+        if ((int)threadIdx.x >= tile.w) continue;
         const SimDev &s = sims[tile.x];
-        const int at = tile.y + V * (int)threadIdx.x;
-        if (at >= s.npad) continue;
-        const Run q = locate(s, at);
+        const Run q = work_run<V>(s, tile, threadIdx.x);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
             hpair_run<V, true, NT, GEN>(s, ck, q);
@@ -888,20 +931,384 @@ __device__ inline void dpair_run(const SimDev &s, const Clock &ck, const Run &q)
     dpair_component<V, 2, PML, NT, GEN>(s, ck, q, h, hn, d[2], iv[2]);
 }
 
+// ------------------------------------------------------------------------------------------
+// The shell's layer forms, loads first (lossless simulations with at most one pole). Each
+// component's three layer roles -- the dsig and dsigu lines of its curl and its own axis's line
+// of the constitutive update -- lie on the three different axes, so exactly one of them varies
+// along the run (x) and the other two hold for all V cells. Every operand of the three
+// components (the layer words each form uses, the split fields, the f_w histories and old E or
+// H words, the pole's state) is loaded before the first store: on CDNA one in-order counter
+// covers loads and stores, and loads issued after a store would each wait for it. The
+// arithmetic is curl_apply_pml's and e_and_p's, in the same order, so the words are the same.
+// ------------------------------------------------------------------------------------------
+template <int V, int AXIS>
+struct Role {  // a y or z line: one membership and one word pair for the whole run
+    bool in_, any;
+    float2 kw_;
+    __device__ bool in(int) const { return in_; }
+    __device__ float2 kw(int) const { return kw_; }
+};
+
+template <int V>
+struct Role<V, 0> {  // the x line: per cell
+    bool in_[V], any;
+    float2 kw_[V];
+    __device__ bool in(int m) const { return in_[m]; }
+    __device__ float2 kw(int m) const { return kw_[m]; }
+};
+
+// Line (half, AXIS)'s role over the run: the words its form multiplies by, (kms, sinv), or with
+// `use_kps` (kms, kps); the identity's off the line. XO: the run is in the box, whose y and z
+// are on no line, so only the x line can hold.
+template <int V, int AXIS, bool XO>
+__device__ inline Role<V, AXIS> load_role(const SimDev &s, const Run &q, int half, bool use_kps) {
+    Role<V, AXIS> r;
+    if constexpr (XO && AXIS != 0) {
+        r.in_ = false;
+        r.any = false;
+        r.kw_ = make_float2(1.0f, 1.0f);
+    } else if constexpr (AXIS == 0) {
+        r.any = false;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            r.in_[m] = member(s, half, 0, q.i0 + m);
+            const float4 c = coeff(s, half, 0, q.i0 + m, r.in_[m]);
+            r.kw_[m] = make_float2(c.x, use_kps ? c.z : c.y);
+            r.any |= r.in_[m];
+        }
+    } else {
+        const int cc = AXIS == 1 ? q.j : q.k;
+        r.in_ = member(s, half, AXIS, cc);
+        const float4 c = coeff(s, half, AXIS, cc, r.in_);
+        r.kw_ = make_float2(c.x, use_kps ? c.z : c.y);
+        r.any = r.in_;
+    }
+    return r;
+}
+
+// One D-pair component's layer operands.
+template <int V, int C>
+struct DLayer {
+    Role<V, (C + 1) % 3> rs;  // dsig of the curl (integer lattice)
+    Role<V, (C + 2) % 3> ru;  // dsigu of the curl
+    Role<V, C> rw;            // own axis of update_E (half-integer lattice)
+    VF<V> fu, fw, e;          // split field, f_w_E and E where a cell of the run needs them
+    VF<V> p, pp, sg;          // the pole's P^n, P^{n-1} and sigma (POLE)
+};
+
+template <int V, int C, bool POLE, bool XO>
+__device__ inline void load_dlayer(const SimDev &s, const Clock &ck, const Run &q, bool active,
+                                   DLayer<V, C> &o) {
+    o.rs = load_role<V, (C + 1) % 3, XO>(s, q, 0, false);
+    o.ru = load_role<V, (C + 2) % 3, XO>(s, q, 0, false);
+    o.rw = load_role<V, C, XO>(s, q, 1, true);
+    if (o.ru.any) o.fu = ldv<V>(s.fuD[C], q.at);
+    if (o.rw.any) {
+        o.fw = ldv<V>(s.fwE[C], q.at);
+        o.e = ldv<V>(s.E[C], q.at);
+    }
+    if constexpr (POLE) {
+        if (active && s.pdriven[0][C]) {
+            o.p = ldv<V>(s.pbuf[0][C][ck.ppar], q.at);
+            o.pp = ldv<V>(s.pbuf[0][C][ck.ppar ^ 1], q.at);
+            if (s.psig[0][C]) o.sg = ldv<V>(s.psig[0][C], q.at);
+            else {
+#pragma unroll
+                for (int m = 0; m < V; ++m) o.sg.v[m] = s.psig_s[0][C];
+            }
+        }
+    }
+}
+
+// The component's new words, then its stores (after every component's loads).
+template <int V, int C, bool NT, bool POLE>
+__device__ inline void dlayer_component(const SimDev &s, const Clock &ck, const Run &q,
+                                        const VF<V> (&h)[3], const VF<V> (&hn)[3][3],
+                                        const VF<V> &d0, const VF<V> &iv, bool active,
+                                        const DLayer<V, C> &o) {
+    constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
+    constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
+    VF<V> d = d0, fu = o.fu;
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
+                                   hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
+        const float2 ls = o.rs.kw(m), lu = o.ru.kw(m);
+        if (o.ru.in(m)) {  // curl_apply_pml's split-field recurrence
+            const float prev = o.fu.v[m];
+            float u = ls.x * prev;
+            u = u - curl;
+            u = ls.y * u;
+            fu.v[m] = u;
+            float v = lu.x * d.v[m];
+            v = v + u;
+            v = v - prev;
+            d.v[m] = lu.y * v;
+        } else if (o.rs.in(m)) {  // the dsig-only form
+            float v = ls.x * d.v[m];
+            v = v - curl;
+            d.v[m] = ls.y * v;
+        } else {
+            d.v[m] = d.v[m] - curl;
+        }
+    }
+    if (s.nsrc) deposit_run<V, false>(s, ck.step, 0, C, q, d);
+    const bool pole = POLE && active && s.pdriven[0][C];
+    VF<V> dmp = d;
+    if (pole) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) dmp.v[m] = dmp.v[m] - o.p.v[m];
+    }
+    VF<V> cst, e, fw = o.fw;
+#pragma unroll
+    for (int m = 0; m < V; ++m) cst.v[m] = iv.v[m] * dmp.v[m];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        float ev = cst.v[m];
+        if (o.rw.in(m)) {  // update_E on the own-axis layer (e_and_p)
+            const float2 lw = o.rw.kw(m);  // (kms, kps)
+            const float prev = o.fw.v[m];
+            fw.v[m] = cst.v[m];
+            const float a = lw.y * cst.v[m];
+            const float mm = lw.x * prev;
+            ev = o.e.v[m] + (a - mm);
+        }
+        e.v[m] = ev;
+    }
+    stv<V, NT>(s.D[C], q.at, d);
+    if (o.ru.any) stv<V>(s.fuD[C], q.at, fu);  // cells off the line keep their word
+    if (o.rw.any) stv<V>(s.fwE[C], q.at, fw);
+    stv<V, NT>(s.E[C], q.at, e);
+    if (pole) {
+        VF<V> r;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            float t = s.pc[0][0] * o.p.v[m];
+            t = t - s.pc[0][1] * o.pp.v[m];
+            float dr = o.sg.v[m] * cst.v[m];
+            dr = s.pc[0][2] * dr;
+            t = t + dr;
+            r.v[m] = s.pc[0][3] * t;
+        }
+        stv<V>(s.pbuf[0][C][ck.ppar ^ 1], q.at, r);
+    }
+}
+
+// The box's H operands: H is stored only on its own-axis layer rows, and in the box y and z
+// are on no line, so Hy and Hz are B's words and only Hx may be H's (at the x layer's cells).
+template <int V>
+__device__ inline void load_h_box(const SimDev &s, const Run &q, VF<V> (&h)[3], VF<V> (&hn)[3][3]) {
+    h[0] = ld_hb<V>(s, q, 0, 0);
+    h[1] = ldv<V>(s.B[1], q.at);
+    h[2] = ldv<V>(s.B[2], q.at);
+    hn[0][1] = ld_hb<V>(s, q, 0, q.ym);
+    hn[0][2] = ld_hb<V>(s, q, 0, q.zm);
+    hn[1][2] = ldv<V>(s.B[1], q.at + q.zm);
+    hn[1][0] = x_down<V>(s, q, s.B[1], h[1]);
+    hn[2][0] = x_down<V>(s, q, s.B[2], h[2]);
+    hn[2][1] = ldv<V>(s.B[2], q.at + q.ym);
+}
+
+template <int V, bool NT, int INV, bool POLE, bool XO>
+__device__ inline void dshell_run(const SimDev &s, const Clock &ck, const Run &q) {
+    InvWords iw[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<V, INV, NT>(s, c, q.at);
+    VF<V> h[3], hn[3][3];
+    if constexpr (XO) load_h_box<V>(s, q, h, hn);
+    else load_h<V, true>(s, q, h, hn);
+    VF<V> d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<V>(s.D[c], q.at) : ldv<V>(s.D[c], q.at);
+    const bool active = POLE && pole_run_active(s, q, V);
+    DLayer<V, 0> l0;
+    DLayer<V, 1> l1;
+    DLayer<V, 2> l2;
+    load_dlayer<V, 0, POLE, XO>(s, ck, q, active, l0);
+    load_dlayer<V, 1, POLE, XO>(s, ck, q, active, l1);
+    load_dlayer<V, 2, POLE, XO>(s, ck, q, active, l2);
+    VF<V> iv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iv[c] = inv_values<V, INV>(s, c, q.at, iw[c]);
+    dlayer_component<V, 0, NT, POLE>(s, ck, q, h, hn, d[0], iv[0], active, l0);
+    dlayer_component<V, 1, NT, POLE>(s, ck, q, h, hn, d[1], iv[1], active, l1);
+    dlayer_component<V, 2, NT, POLE>(s, ck, q, h, hn, d[2], iv[2], active, l2);
+}
+
+// One H-pair component's layer operands: dsig and dsigu of the curl on the half-integer lattice,
+// the own-axis integer-lattice line of update_H (H and f_w_H where a cell needs them).
+template <int V, int C>
+struct HLayer {
+    Role<V, (C + 1) % 3> rs;
+    Role<V, (C + 2) % 3> ru;
+    Role<V, C> rw;
+    VF<V> fu, hw, fw;
+};
+
+template <int V, int C, bool XO>
+__device__ inline void load_hlayer(const SimDev &s, const Clock &ck, const Run &q, HLayer<V, C> &o) {
+    o.rs = load_role<V, (C + 1) % 3, XO>(s, q, 1, false);
+    o.ru = load_role<V, (C + 2) % 3, XO>(s, q, 1, false);
+    o.rw = load_role<V, C, XO>(s, q, 0, true);
+    if (o.ru.any) o.fu = ldv<V>(s.fuB[C], q.at);
+    if (o.rw.any && ck.hsep) {
+        o.hw = ldv<V>(s.H[C], q.at);
+        o.fw = ldv<V>(s.fwH[C], q.at);
+    }
+}
+
+template <int V, int C, bool NT>
+__device__ inline void hlayer_component(const SimDev &s, const Clock &ck, const Run &q,
+                                        const VF<V> (&e)[3], const VF<V> (&en)[3][3],
+                                        const VF<V> &b0, const HLayer<V, C> &o) {
+    constexpr int first = (C + 2) % 3, fa = (C + 1) % 3;
+    constexpr int second = (C + 1) % 3, sa = (C + 2) % 3;
+    VF<V> b = b0, fu = o.fu;
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        const float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
+                                   en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
+        const float2 ls = o.rs.kw(m), lu = o.ru.kw(m);
+        if (o.ru.in(m)) {
+            const float prev = o.fu.v[m];
+            float u = ls.x * prev;
+            u = u - curl;
+            u = ls.y * u;
+            fu.v[m] = u;
+            float v = lu.x * b.v[m];
+            v = v + u;
+            v = v - prev;
+            b.v[m] = lu.y * v;
+        } else if (o.rs.in(m)) {
+            float v = ls.x * b.v[m];
+            v = v - curl;
+            b.v[m] = ls.y * v;
+        } else {
+            b.v[m] = b.v[m] - curl;
+        }
+    }
+    if (s.nsrc) deposit_run<V, false>(s, ck.step, 1, C, q, b);
+    stv<V, NT>(s.B[C], q.at, b);
+    if (o.ru.any) stv<V>(s.fuB[C], q.at, fu);
+    if (!o.rw.any) return;
+    // update_H on the own-axis layer: H accumulates there; elsewhere H is B's word and H is
+    // not stored, so the words go out per cell
+    GMEM float *gh = gp(s.H[C]), *gf = gp(s.fwH[C]);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+        if (!o.rw.in(m)) continue;
+        const float2 lw = o.rw.kw(m);  // (kms, kps)
+        const float hold = ck.hsep ? o.hw.v[m] : b.v[m];
+        const float prev = ck.hsep ? o.fw.v[m] : b.v[m];
+        gf[q.at + m] = b.v[m];
+        const float a = lw.y * b.v[m];
+        const float mm = lw.x * prev;
+        gh[q.at + m] = hold + (a - mm);
+    }
+}
+
+template <int V, bool NT, bool XO>
+__device__ inline void hshell_run(const SimDev &s, const Clock &ck, const Run &q) {
+    VF<V> e[3], en[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[c] = ldv<V>(s.E[c], q.at);
+    en[0][1] = ldv<V>(s.E[0], q.at + q.yp);
+    en[0][2] = ldv<V>(s.E[0], q.at + q.zp);
+    en[1][2] = ldv<V>(s.E[1], q.at + q.zp);
+    en[1][0] = x_up<V>(s, q, s.E[1], e[1]);
+    en[2][0] = x_up<V>(s, q, s.E[2], e[2]);
+    en[2][1] = ldv<V>(s.E[2], q.at + q.yp);
+    VF<V> b[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[c] = NT ? ldv_once<V>(s.B[c], q.at) : ldv<V>(s.B[c], q.at);
+    HLayer<V, 0> l0;
+    HLayer<V, 1> l1;
+    HLayer<V, 2> l2;
+    load_hlayer<V, 0, XO>(s, ck, q, l0);
+    load_hlayer<V, 1, XO>(s, ck, q, l1);
+    load_hlayer<V, 2, XO>(s, ck, q, l2);
+    hlayer_component<V, 0, NT>(s, ck, q, e, en, b[0], l0);
+    hlayer_component<V, 1, NT>(s, ck, q, e, en, b[1], l1);
+    hlayer_component<V, 2, NT>(s, ck, q, e, en, b[2], l2);
+}
+
+// The lossless groups' launches: the box's rows and the shell's rows as two kernels, each
+// with the registers its own forms need -- the box's with the x line's layer forms only (or,
+// where no member has an absorber, the lossless forms alone: dint, hint), the shell's with every
+// role.
+template <int V, bool NT, int INV>
+__global__ void __launch_bounds__(kThreads) dint_kernel(const SimDev *__restrict__ sims,
+                                                        const int4 *__restrict__ items,
+                                                        int nitems, int xcd_remap, int adv) {
+    const TileWalk w = tile_walk(nitems, xcd_remap);
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= nitems) continue;
+        const int4 it = items[t];
+        if ((int)threadIdx.x >= it.w) continue;
+        const SimDev &s = sims[it.x];
+        dpair_run<V, false, NT, INV, false>(s, clock_of(s, adv), work_run<V>(s, it, threadIdx.x));
+    }
+}
+
+template <int V, bool NT, int INV, bool POLE, bool XO>
+__global__ void __launch_bounds__(kThreads) dshell_kernel(const SimDev *__restrict__ sims,
+                                                          const int4 *__restrict__ items,
+                                                          int nitems, int xcd_remap, int adv) {
+    const TileWalk w = tile_walk(nitems, xcd_remap);
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= nitems) continue;
+        const int4 it = items[t];
+        if ((int)threadIdx.x >= it.w) continue;
+        const SimDev &s = sims[it.x];
+        dshell_run<V, NT, INV, POLE, XO>(s, clock_of(s, adv), work_run<V>(s, it, threadIdx.x));
+    }
+}
+
+template <int V, bool NT>
+__global__ void __launch_bounds__(kThreads) hint_kernel(const SimDev *__restrict__ sims,
+                                                        const int4 *__restrict__ items,
+                                                        int nitems, int xcd_remap, int adv) {
+    const TileWalk w = tile_walk(nitems, xcd_remap);
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= nitems) continue;
+        const int4 it = items[t];
+        if ((int)threadIdx.x >= it.w) continue;
+        const SimDev &s = sims[it.x];
+        hpair_run<V, false, NT, false>(s, clock_of(s, adv), work_run<V>(s, it, threadIdx.x));
+    }
+}
+
+template <int V, bool NT, bool XO>
+__global__ void __launch_bounds__(kThreads) hshell_kernel(const SimDev *__restrict__ sims,
+                                                          const int4 *__restrict__ items,
+                                                          int nitems, int xcd_remap, int adv) {
+    const TileWalk w = tile_walk(nitems, xcd_remap);
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= nitems) continue;
+        const int4 it = items[t];
+        if ((int)threadIdx.x >= it.w) continue;
+        const SimDev &s = sims[it.x];
+        hshell_run<V, NT, XO>(s, clock_of(s, adv), work_run<V>(s, it, threadIdx.x));
+    }
+}
+
 template <int V, bool NT, int INV, bool GEN>
 __global__ void __launch_bounds__(kThreads) dpair_kernel(const SimDev *__restrict__ sims,
-                                                         const int2 *__restrict__ tiles,
+                                                         const int4 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
         if (t >= ntiles) continue;
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
+        const int4 tile = tiles[t];
+        if ((int)threadIdx.x >= tile.w) continue;
         const SimDev &s = sims[tile.x];
         if (s.offdiag) continue;  // stepped by the two-kernel row path below
-        const int at = tile.y + V * (int)threadIdx.x;
-        if (at >= s.npad) continue;
-        const Run q = locate(s, at);
+        const Run q = work_run<V>(s, tile, threadIdx.x);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
             dpair_run<V, true, NT, INV, GEN>(s, ck, q);
@@ -927,18 +1334,17 @@ __device__ inline void dcurl_run(const SimDev &s, const Clock &ck, const Run &q)
 
 template <int V>
 __global__ void __launch_bounds__(kThreads) dcurl_kernel(const SimDev *__restrict__ sims,
-                                                         const int2 *__restrict__ tiles,
+                                                         const int4 *__restrict__ tiles,
                                                          int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
         if (t >= ntiles) continue;
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
+        const int4 tile = tiles[t];
+        if ((int)threadIdx.x >= tile.w) continue;
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
-        const int at = tile.y + V * (int)threadIdx.x;
-        if (at >= s.npad) continue;
-        const Run q = locate(s, at);
+        const Run q = work_run<V>(s, tile, threadIdx.x);
         const Clock ck = clock_of(s, adv);
         if (s.pml)
             dcurl_run<V, true>(s, ck, q);
@@ -1022,18 +1428,17 @@ __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Ru
 
 template <int V>
 __global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict__ sims,
-                                                        const int2 *__restrict__ tiles,
+                                                        const int4 *__restrict__ tiles,
                                                         int ntiles, int xcd_remap, int adv) {
     const TileWalk w = tile_walk(ntiles, xcd_remap);
     for (int g = w.next; g < w.end; g += w.stride) {
         const int t = tile_at(w, g);
         if (t >= ntiles) continue;
-        const int2 tile = tiles ? tiles[t] : make_int2(0, t * kThreads * V);
+        const int4 tile = tiles[t];
+        if ((int)threadIdx.x >= tile.w) continue;
         const SimDev &s = sims[tile.x];
         if (!s.offdiag) continue;
-        const int at = tile.y + V * (int)threadIdx.x;
-        if (at >= s.npad) continue;
-        const Run q = locate(s, at);
+        const Run q = work_run<V>(s, tile, threadIdx.x);
         const Clock ck = clock_of(s, adv);
         if (s.pml) {
             erow_component<V, 0, true>(s, ck, q);
@@ -1512,6 +1917,8 @@ struct LaunchShape {
     int blocks_per_cu = 8;  // 0: one workgroup per tile; else a persistent grid
     int xcd_remap = 0;
     int nt = 0;             // non-temporal stores of the fused pairs' outputs
+    int svec = 1;           // cells per thread of the shell's lean kernel (its layer forms hold
+                            // more operands, so fewer cells a thread keep more waves resident)
 };
 LaunchShape g_shape[2];  // [0]: the H pair's launches, [1]: the D pair's (and its row path)
 int g_num_cu = 0;
@@ -1536,7 +1943,7 @@ int grid_for(const LaunchShape &sh, int ntiles) {
 
 bool same_shape(const LaunchShape &a, const LaunchShape &b) {
     return a.vec == b.vec && a.blocks_per_cu == b.blocks_per_cu && a.xcd_remap == b.xcd_remap &&
-           a.nt == b.nt;
+           a.nt == b.nt && a.svec == b.svec;
 }
 
 struct PlaneHost {
@@ -1551,8 +1958,12 @@ struct PlaneHost {
 struct Group {
     SimDev *dev = nullptr;          // descriptors, contiguous
     int nsims = 0;
-    std::map<int, int2 *> tiles;    // per vector width; absent for a lone simulation
+    std::map<int, const int4 *> tiles;  // work items per vector width (interior and shell)
     std::map<int, int> ntiles;
+    std::map<int, const int4 *> shell_tiles, int_tiles;  // the shell's items, the box's items
+    std::map<int, int> nshell_tiles, nint_tiles;
+    int max_npol = 0;     // the most poles a member has
+    bool any_pml = false;  // some member has an absorber (its box rows hold x-layer cells)
     bool any_offdiag = false, all_offdiag = false;
     bool general = false;  // some member has a conductivity, a metallic wall or more than
                            // kLosslessPoles poles: the pairs' general instantiation (the others
@@ -1572,7 +1983,7 @@ int inv_mode_of(const SimDev &h) {
 }
 
 template <int V, bool NT, bool GEN>
-void launch_dpair_c(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
+void launch_dpair_c(const Group &g, int G, const int4 *tl, int nt, int rm, int adv, hipStream_t st) {
     switch (g.inv_mode) {
         case 0: dpair_kernel<V, NT, 0, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
         case 1: dpair_kernel<V, NT, 1, GEN><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv); break;
@@ -1584,26 +1995,59 @@ void launch_dpair_c(const Group &g, int G, const int2 *tl, int nt, int rm, int a
 // The general instantiations run only where a member has a conductivity, a metallic wall or
 // more than kLosslessPoles poles: the lossless periodic pairs' code stays what it is.
 template <int V, bool NT>
-void launch_dpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
+void launch_dpair(const Group &g, int G, const int4 *tl, int nt, int rm, int adv, hipStream_t st) {
     if (g.general) launch_dpair_c<V, NT, true>(g, G, tl, nt, rm, adv, st);
     else launch_dpair_c<V, NT, false>(g, G, tl, nt, rm, adv, st);
 }
 
 template <int V, bool NT>
-void launch_hpair(const Group &g, int G, const int2 *tl, int nt, int rm, int adv, hipStream_t st) {
+void launch_hpair(const Group &g, int G, const int4 *tl, int nt, int rm, int adv, hipStream_t st) {
     if (g.general) hpair_kernel<V, NT, true><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
     else hpair_kernel<V, NT, false><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
 }
 
+// The two-kernel form: where no member runs the general forms, the box's rows and the shell's
+// are two lean kernels (the D pair's also needs no off-diagonal row, float or byte-indexed
+// inverse epsilon and at most one pole).
+bool lean_d(const Group &g) {
+    return !g.general && !g.any_offdiag && (g.inv_mode == 0 || g.inv_mode == 1) && g.max_npol <= 1;
+}
+
 // The fused pairs' launches `adv` steps after the descriptors' last counter advance (0 for an
 // eager step; a captured step's offset within its sequence).
+template <int V, bool NT>
+void launch_hbox(const Group &g, int G, const int4 *l, int n, int rm, int adv, hipStream_t st) {
+    if (g.any_pml) hshell_kernel<V, NT, true><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv);
+    else hint_kernel<V, NT><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv);
+}
+
 int launch_h(const Group &g, hipStream_t st, int adv = 0) {
     const LaunchShape &sh = g_shape[0];
-    const int V = sh.vec;
-    const int nt = g.ntiles.at(V);
-    const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
-    const int G = grid_for(sh, nt);
+    const int V = sh.vec, SV = sh.svec;
     const int rm = sh.xcd_remap;
+    if (!g.general) {  // the box's rows, then the shell's: two lean kernels
+        const int ni = g.nint_tiles.at(V);
+        const int4 *il = g.int_tiles.at(V);
+        const int GI = grid_for(sh, ni);
+        if (ni) {
+            if (V == 4) { if (sh.nt) launch_hbox<4, true>(g, GI, il, ni, rm, adv, st); else launch_hbox<4, false>(g, GI, il, ni, rm, adv, st); }
+            else if (V == 2) { if (sh.nt) launch_hbox<2, true>(g, GI, il, ni, rm, adv, st); else launch_hbox<2, false>(g, GI, il, ni, rm, adv, st); }
+            else { if (sh.nt) launch_hbox<1, true>(g, GI, il, ni, rm, adv, st); else launch_hbox<1, false>(g, GI, il, ni, rm, adv, st); }
+        }
+        const int ns = g.nshell_tiles.at(SV);
+        const int4 *sl = g.shell_tiles.at(SV);
+        const int GS = grid_for(sh, ns);
+        if (ns) {
+            if (SV == 4) { if (sh.nt) hshell_kernel<4, true, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); else hshell_kernel<4, false, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); }
+            else if (SV == 2) { if (sh.nt) hshell_kernel<2, true, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); else hshell_kernel<2, false, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); }
+            else { if (sh.nt) hshell_kernel<1, true, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); else hshell_kernel<1, false, false><<<GS, kThreads, 0, st>>>(g.dev, sl, ns, rm, adv); }
+        }
+        HIP_TRY(hipGetLastError());
+        return FDTD_OK;
+    }
+    const int nt = g.ntiles.at(V);
+    const int4 *tl = g.tiles.at(V);
+    const int G = grid_for(sh, nt);
     if (sh.nt) {
         if (V == 4) launch_hpair<4, true>(g, G, tl, nt, rm, adv, st);
         else if (V == 2) launch_hpair<2, true>(g, G, tl, nt, rm, adv, st);
@@ -1617,16 +2061,48 @@ int launch_h(const Group &g, hipStream_t st, int adv = 0) {
     return FDTD_OK;
 }
 
+// The lean D kernels over a list: the box's rows (BOX: the x layer's forms where a member has
+// an absorber, else the lossless forms) or the shell's (every layer role).
+template <int V, bool NT, int INV, bool BOX>
+void launch_dlean_inv(const Group &g, int G, const int4 *l, int n, int rm, int adv, hipStream_t st) {
+    if (BOX && !g.any_pml) dint_kernel<V, NT, INV><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv);
+    else if (g.max_npol == 1) dshell_kernel<V, NT, INV, true, BOX><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv);
+    else dshell_kernel<V, NT, INV, false, BOX><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv);
+}
+
+template <int V, bool NT, bool BOX>
+void launch_dlean(const Group &g, int G, const int4 *l, int n, int rm, int adv, hipStream_t st) {
+    if (g.inv_mode == 1) launch_dlean_inv<V, NT, 1, BOX>(g, G, l, n, rm, adv, st);
+    else launch_dlean_inv<V, NT, 0, BOX>(g, G, l, n, rm, adv, st);
+}
+
 // ... then the counter advance by `advance_by` steps (0 inside a captured sequence, which
 // advances once at its end).
 int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
     const LaunchShape &sh = g_shape[1];
-    const int V = sh.vec;
-    const int nt = g.ntiles.at(V);
-    const int2 *tl = g.tiles.count(V) ? g.tiles.at(V) : nullptr;
-    const int G = grid_for(sh, nt);
+    const int V = sh.vec, SV = sh.svec;
     const int rm = sh.xcd_remap;
-    if (!g.all_offdiag) {
+    const int nt = g.ntiles.at(V);
+    const int4 *tl = g.tiles.at(V);
+    const int G = grid_for(sh, nt);
+    if (lean_d(g)) {  // the box's rows, then the shell's
+        const int ni = g.nint_tiles.at(V);
+        const int4 *il = g.int_tiles.at(V);
+        const int GI = grid_for(sh, ni);
+        if (ni) {
+            if (V == 4) { if (sh.nt) launch_dlean<4, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<4, false, true>(g, GI, il, ni, rm, adv, st); }
+            else if (V == 2) { if (sh.nt) launch_dlean<2, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<2, false, true>(g, GI, il, ni, rm, adv, st); }
+            else { if (sh.nt) launch_dlean<1, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<1, false, true>(g, GI, il, ni, rm, adv, st); }
+        }
+        const int ns = g.nshell_tiles.at(SV);
+        const int4 *sl = g.shell_tiles.at(SV);
+        const int GS = grid_for(sh, ns);
+        if (ns) {
+            if (SV == 4) { if (sh.nt) launch_dlean<4, true, false>(g, GS, sl, ns, rm, adv, st); else launch_dlean<4, false, false>(g, GS, sl, ns, rm, adv, st); }
+            else if (SV == 2) { if (sh.nt) launch_dlean<2, true, false>(g, GS, sl, ns, rm, adv, st); else launch_dlean<2, false, false>(g, GS, sl, ns, rm, adv, st); }
+            else { if (sh.nt) launch_dlean<1, true, false>(g, GS, sl, ns, rm, adv, st); else launch_dlean<1, false, false>(g, GS, sl, ns, rm, adv, st); }
+        }
+    } else if (!g.all_offdiag) {
         if (sh.nt) {
             if (V == 4) launch_dpair<4, true>(g, G, tl, nt, rm, adv, st);
             else if (V == 2) launch_dpair<2, true>(g, G, tl, nt, rm, adv, st);
@@ -1714,6 +2190,17 @@ struct fdtd_sim {
     bool destroy_pending = false;  // fdtd_sim_destroy was called while in a batch
     StepGraph graph;
     std::vector<hipEvent_t> timed_ev;  // fdtd_run_timed: around each fused pair of a chunk
+    // the work split (see SimDev::ibox): per V (1, 2, 4 -> 0, 1, 2) the shell list's length,
+    // the interior's run count and the lone simulation's work items
+    int nshell[3] = {0, 0, 0}, ninterior[3] = {0, 0, 0};
+    const int4 *work[3] = {nullptr, nullptr, nullptr};
+    int nwork[3] = {0, 0, 0};
+    const int4 *shellw[3] = {nullptr, nullptr, nullptr};  // the shell's items alone
+    int nshellw[3] = {0, 0, 0};
+    const int4 *intw[3] = {nullptr, nullptr, nullptr};    // the interior box's items alone
+    int nintw[3] = {0, 0, 0};
+    std::vector<void *> work_allocs;  // the device lists above (replaced when the lines change)
+    bool work_dirty = true;           // the layer lines changed since the split was built
 
     int alloc(void **p, size_t bytes_) {
         hipError_t e = hipMalloc(p, bytes_);
@@ -1722,6 +2209,14 @@ struct fdtd_sim {
         bytes += bytes_;
         return FDTD_OK;
     }
+    int work_alloc(void **p, size_t bytes_) {
+        hipError_t e = hipMalloc(p, bytes_);
+        if (e != hipSuccess) return fail(FDTD_ENOMEM, "hipMalloc(%zu) failed: %s", bytes_, hipGetErrorString(e));
+        work_allocs.push_back(*p);
+        work_bytes += bytes_;
+        return FDTD_OK;
+    }
+    uint64_t work_bytes = 0;
     int volume(float **p, float fill = 0.0f) {
         const size_t n = (size_t)host.npad;
         int rc = alloc((void **)p, n * sizeof(float));
@@ -1733,6 +2228,15 @@ struct fdtd_sim {
             HIP_TRY(hipGetLastError());
         }
         return FDTD_OK;
+    }
+    int prepare_work();
+    // before a launch: the work split is current, then the descriptor is on the device
+    int ready() {
+        if (work_dirty) {
+            int rc = prepare_work();
+            if (rc) return rc;
+        }
+        return sync_desc();
     }
     int sync_desc() {
         if (!dirty) return FDTD_OK;
@@ -1752,10 +2256,16 @@ struct fdtd_sim {
         Group g;
         g.dev = dev;
         g.nsims = 1;
-        for (const LaunchShape &sh : g_shape) {
-            const int64_t tc = (int64_t)kThreads * sh.vec;
-            g.ntiles[sh.vec] = (int)(((int64_t)host.npad + tc - 1) / tc);
+        for (int vi = 0; vi < 3; ++vi) {
+            g.tiles[1 << vi] = work[vi];
+            g.ntiles[1 << vi] = nwork[vi];
+            g.shell_tiles[1 << vi] = shellw[vi];
+            g.nshell_tiles[1 << vi] = nshellw[vi];
+            g.int_tiles[1 << vi] = intw[vi];
+            g.nint_tiles[1 << vi] = nintw[vi];
         }
+        g.max_npol = host.npol;
+        g.any_pml = host.pml != 0;
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
         g.general = host.cond != 0 || host.metal[0] || host.metal[1] || host.metal[2] ||
                     host.npol > kLosslessPoles;
@@ -1771,6 +2281,122 @@ struct fdtd_sim {
         return FDTD_OK;
     }
 };
+
+// The box of a simulation's grid: its whole rows [0, pitch) x [j0, j1) x [k0, k1) where j and
+// k are on no layer line of either lattice (per axis the longest range no membership run
+// covers), so that only the x line's forms can hold in it. Without an absorber, the whole
+// padded grid.
+static void interior_box(const SimDev &h, int box[6]) {
+    if (!h.pml) {
+        const int whole[6] = {0, h.pitch, 0, h.n[1], 0, h.n[2]};
+        for (int t = 0; t < 6; ++t) box[t] = whole[t];
+        return;
+    }
+    box[0] = 0;
+    box[1] = h.pitch;
+    for (int a = 1; a < 3; ++a) {
+        const int n = h.n[a];
+        std::vector<uint8_t> covered(n, 0);
+        for (int half = 0; half < 2; ++half)
+            for (int r = 0; r < kMaxRuns; ++r)
+                for (int c = std::max(0, h.mr[half][a][2 * r]); c < std::min(n, h.mr[half][a][2 * r + 1]); ++c)
+                    covered[c] = 1;
+        int best_lo = 0, best_hi = 0;
+        for (int c = 0; c < n;) {
+            if (covered[c]) { ++c; continue; }
+            int e = c;
+            while (e < n && !covered[e]) ++e;
+            if (e - c > best_hi - best_lo) { best_lo = c; best_hi = e; }
+            c = e;
+        }
+        if (best_hi < best_lo) best_hi = best_lo;
+        box[2 * a] = best_lo;
+        box[2 * a + 1] = best_hi;
+    }
+}
+
+// Build the work split: the interior box, the shell lists for V = 1, 2, 4 (every run of V
+// cells holding a real cell that is not wholly inside the box, in address order) and the lone
+// simulation's work items (kThreads runs each), interior and shell items merged by address.
+int fdtd_sim::prepare_work() {
+    graph.reset();  // a captured sequence launched the old lists (its replays finish first)
+    if (stream) HIP_TRY(hipStreamSynchronize(stream));
+    for (void *p : work_allocs) (void)hipFree(p);
+    work_allocs.clear();
+    work_bytes = 0;
+    SimDev &h = host;
+    int box[6];
+    interior_box(h, box);
+    for (int t = 0; t < 6; ++t) h.ibox[t] = box[t];
+    const int nx = h.n[0], ny = h.n[1], nz = h.n[2];
+    const bool empty = box[0] >= box[1] || box[2] >= box[3] || box[4] >= box[5];
+    for (int vi = 0; vi < 3; ++vi) {
+        const int V = 1 << vi;
+        std::vector<int> list;
+        if (h.pml) {
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) {
+                    const bool row_in = !empty && j >= box[2] && j < box[3] && k >= box[4] && k < box[5];
+                    for (int x = 0; x < nx; x += V) {
+                        if (row_in && x >= box[0] && x + V <= box[1]) continue;
+                        list.push_back((k * ny + j) * h.pitch + x);
+                    }
+                }
+        }
+        const int64_t nint = empty ? 0 : (int64_t)((box[1] - box[0]) / V) * (box[3] - box[2]) * (box[5] - box[4]);
+        if (nint + (int64_t)list.size() >= ((int64_t)1 << 31))
+            return fail(FDTD_EINVAL, "grid too large for the work lists");
+        ninterior[vi] = (int)nint;
+        nshell[vi] = (int)list.size();
+        // interior runs numbered x fastest: the address of run q
+        const int nqx = empty ? 1 : (box[1] - box[0]) / V, nj = empty ? 1 : box[3] - box[2];
+        auto interior_at = [&](int q) {
+            const int r = q / nqx, kk = r / nj;
+            return ((box[4] + kk) * ny + box[2] + (r - kk * nj)) * h.pitch + box[0] + (q - r * nqx) * V;
+        };
+        std::vector<int4> items;
+        int qi = 0, qs = 0;
+        while (qi < nint || qs < (int)list.size()) {
+            const bool take_interior =
+                qs >= (int)list.size() || (qi < nint && interior_at(qi) < list[qs]);
+            if (take_interior) {
+                const int c = (int)std::min<int64_t>(kThreads, nint - qi);
+                items.push_back(make_int4(0, 0, qi, c));
+                qi += c;
+            } else {
+                const int c = std::min(kThreads, (int)list.size() - qs);
+                items.push_back(make_int4(0, 1, qs, c));
+                qs += c;
+            }
+        }
+        void *p = nullptr;
+        int rc = work_alloc(&p, sizeof(int) * std::max<size_t>(1, list.size()));
+        if (rc) return rc;
+        if (!list.empty())
+            HIP_TRY(hipMemcpy(p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+        h.shell[vi] = (const int *)p;
+        rc = work_alloc(&p, sizeof(int4) * std::max<size_t>(1, items.size()));
+        if (rc) return rc;
+        if (!items.empty())
+            HIP_TRY(hipMemcpy(p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+        work[vi] = (const int4 *)p;
+        nwork[vi] = (int)items.size();
+        // the shell's items alone and the box's alone, for the two-kernel form
+        std::vector<int4> shell_items, int_items;
+        for (const int4 &w : items) (w.y == 1 ? shell_items : int_items).push_back(w);
+        for (int which = 0; which < 2; ++which) {
+            const std::vector<int4> &v = which ? int_items : shell_items;
+            rc = work_alloc(&p, sizeof(int4) * std::max<size_t>(1, v.size()));
+            if (rc) return rc;
+            if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice));
+            if (which) { intw[vi] = (const int4 *)p; nintw[vi] = (int)v.size(); }
+            else { shellw[vi] = (const int4 *)p; nshellw[vi] = (int)v.size(); }
+        }
+    }
+    work_dirty = false;
+    dirty = true;
+    return FDTD_OK;
+}
 
 struct fdtd_batch {
     std::vector<fdtd_sim *> sims;
@@ -2038,7 +2664,7 @@ Stepper lone(fdtd_sim *&s) {
 }
 
 int sim_step(fdtd_sim *s) {
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     return step_once(lone(s), true);
 }
@@ -2091,6 +2717,19 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
     return FDTD_OK;
 }
 
+int fdtd_set_launch_shell(int32_t pair, int32_t vec) {
+    if (pair != 0 && pair != 1) return fail(FDTD_EINVAL, "pair must be 0 (H) or 1 (D), got %d", pair);
+    if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
+    g_shape[pair].svec = vec;
+    return FDTD_OK;
+}
+
+int fdtd_get_launch_shell(int32_t pair, int32_t *vec) {
+    if ((pair != 0 && pair != 1) || !vec) return fail(FDTD_EINVAL, "bad argument");
+    *vec = g_shape[pair].svec;
+    return FDTD_OK;
+}
+
 int fdtd_get_launch_pair(int32_t pair, int32_t out[4]) {
     if (!out || (pair != 0 && pair != 1)) return fail(FDTD_EINVAL, "bad argument");
     out[0] = g_shape[pair].vec;
@@ -2119,6 +2758,7 @@ int fdtd_sim_create(const fdtd_config *cfg, fdtd_sim **out) {
     const int64_t npad = pitch * cfg->n[1] * cfg->n[2];
     if (npad >= ((int64_t)1 << 31) - 4 * kThreads * 4)
         return fail(FDTD_EINVAL, "grid of %lld padded cells exceeds the 2^31 index range", (long long)npad);
+    if (cfg->n[2] >= 32768) return fail(FDTD_EINVAL, "at most 32767 cells along z, got %d", cfg->n[2]);
     if (!(cfg->dt > 0.0)) return fail(FDTD_EINVAL, "dt must be positive");
     fdtd_sim *s = new (std::nothrow) fdtd_sim();
     if (!s) return fail(FDTD_ENOMEM, "host allocation");
@@ -2187,13 +2827,14 @@ void fdtd_sim_destroy(fdtd_sim *s) {
     s->graph.reset();  // before the memory its launches read is freed
     for (hipEvent_t e : s->timed_ev) (void)hipEventDestroy(e);
     for (void *p : s->allocs) (void)hipFree(p);
+    for (void *p : s->work_allocs) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
 int fdtd_sim_bytes(const fdtd_sim *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(FDTD_EINVAL, "null argument");
-    *bytes = s->bytes;
+    *bytes = s->bytes + s->work_bytes;
     return FDTD_OK;
 }
 
@@ -2201,6 +2842,7 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
                    const float *kps, const uint8_t *member) {
     if (!s || half < 0 || half > 1 || axis < 0 || axis > 2 || !kms || !sinv || !kps || !member)
         return fail(FDTD_EINVAL, "fdtd_set_layer: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     const int n = s->host.n[axis];
     std::vector<float4> v(n);
     for (int q = 0; q < n; ++q) v[q] = make_float4(kms[q], sinv[q], kps[q], member[q] ? 1.f : 0.f);
@@ -2222,6 +2864,7 @@ int fdtd_set_layer(fdtd_sim *s, int32_t half, int32_t axis, const float *kms, co
     if (rc) return rc;
     for (int t = 0; t < 2 * kMaxRuns; ++t) s->host.mr[half][axis][t] = runs[t];
     s->dirty = true;
+    s->work_dirty = true;  // the interior box and shell lists follow the lines
     HIP_TRY(hipMemcpyAsync((void *)s->host.line[half][axis], v.data(), sizeof(float4) * n,
                            hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -2415,6 +3058,7 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     h.inv8[c] = nullptr;
     h.inv16[c] = nullptr;
     h.invtab[c] = nullptr;
+    h.invn[c] = 0;
     s->dirty = true;
     s->graph.reset();  // a captured sequence launched the kernels of the old storage
     if (table.size() > 65536) return FDTD_OK;  // the float volume serves
@@ -2449,6 +3093,20 @@ int fdtd_set_inverse_epsilon(fdtd_sim *s, int32_t c, const float *vol) {
     if (narrow) h.inv8[c] = (const uint8_t *)dev_idx;
     else h.inv16[c] = (const uint16_t *)dev_idx;
     h.invtab[c] = (const float *)dev_tab;
+    h.invn[c] = (int)table.size();
+    return FDTD_OK;
+}
+
+int fdtd_work_split(fdtd_sim *s, int32_t vec, int32_t out[8]) {
+    if (!s || !out || (vec != 1 && vec != 2 && vec != 4)) return fail(FDTD_EINVAL, "fdtd_work_split: bad argument");
+    if (s->work_dirty) {
+        int rc = s->prepare_work();
+        if (rc) return rc;
+    }
+    const int vi = vec == 4 ? 2 : vec - 1;
+    for (int t = 0; t < 6; ++t) out[t] = s->host.ibox[t];
+    out[6] = s->ninterior[vi];
+    out[7] = s->nshell[vi];
     return FDTD_OK;
 }
 
@@ -2759,7 +3417,7 @@ int fdtd_prime_e(fdtd_sim *s) {
 int fdtd_step_h(fdtd_sim *s) {
     if (!s) return fail(FDTD_EINVAL, "null argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     return launch_h(s->group(), s->stream);
 }
@@ -2767,7 +3425,7 @@ int fdtd_step_h(fdtd_sim *s) {
 int fdtd_step_d(fdtd_sim *s) {
     if (!s) return fail(FDTD_EINVAL, "null argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     rc = launch_d(s->group(), s->stream);
     if (rc) return rc;
@@ -2793,7 +3451,7 @@ int fdtd_flush(fdtd_sim *s) {
 int fdtd_run(fdtd_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     return run_steps(lone(s), nsteps);
 }
@@ -3037,7 +3695,7 @@ int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
     if (!s || repeats < 1 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     if (!s->planes.empty()) return fail(FDTD_ESTATE, "fdtd_time_pairs steps a simulation without flux planes");
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     const Group g = s->group();
     hipEvent_t a, b, c;
@@ -3072,7 +3730,7 @@ int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
 int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
     if (!s || nsteps < 0 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    int rc = s->sync_desc();
+    int rc = s->ready();
     if (rc) return rc;
     if (s->timed_ev.empty()) {
         s->timed_ev.resize(3 * kRing + 2);
@@ -3176,20 +3834,54 @@ int fdtd_stream_probe(int32_t nread, int32_t nwrite, int32_t n, uint64_t stagger
 // ------------------------------------------------------------------------------------------
 // Batch
 // ------------------------------------------------------------------------------------------
+// The batch's work items for vector width V: each member's own items (interior and shell,
+// merged by address), re-numbered to the member's descriptor.
+// One of every member's device lists for width index vi -- which 0: all items, 1: the
+// shell's, 2: the box's -- concatenated.
+static int batch_list(fdtd_batch *b, int vi, int which, const int4 **out, int *nout) {
+    std::vector<int4> items;
+    for (int q = 0; q < (int)b->sims.size(); ++q) {
+        fdtd_sim *s = b->sims[q];
+        const int4 *list = which == 0 ? s->work[vi] : which == 1 ? s->shellw[vi] : s->intw[vi];
+        const int n = which == 0 ? s->nwork[vi] : which == 1 ? s->nshellw[vi] : s->nintw[vi];
+        std::vector<int4> own(std::max(1, n));
+        if (n) HIP_TRY(hipMemcpy(own.data(), list, sizeof(int4) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t) {
+            int4 w = own[t];
+            w.x = q;
+            items.push_back(w);
+        }
+    }
+    int4 *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(int4) * std::max<size_t>(1, items.size())));
+    b->allocs.push_back(d);
+    if (!items.empty())
+        HIP_TRY(hipMemcpy(d, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+    *out = d;
+    *nout = (int)items.size();
+    return FDTD_OK;
+}
+
+// The batch's work items for vector width V: each member's own lists (all items, the shell's,
+// the box's), re-numbered to the member's descriptor.
 static int batch_tiles(fdtd_batch *b, int V) {
     if (b->group.tiles.count(V)) return FDTD_OK;
-    std::vector<int2> tiles;
-    const int tc = kThreads * V;
-    for (int q = 0; q < (int)b->sims.size(); ++q) {
-        const int nt = (b->sims[q]->host.npad + tc - 1) / tc;
-        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(q, t * tc));
-    }
-    int2 *d = nullptr;
-    HIP_TRY(hipMalloc((void **)&d, sizeof(int2) * tiles.size()));
-    b->allocs.push_back(d);
-    HIP_TRY(hipMemcpy(d, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
-    b->group.tiles[V] = d;
-    b->group.ntiles[V] = (int)tiles.size();
+    const int vi = V == 4 ? 2 : V - 1;
+    Group &g = b->group;
+    const int4 *p = nullptr;
+    int n = 0;
+    int rc = batch_list(b, vi, 0, &p, &n);
+    if (rc) return rc;
+    g.tiles[V] = p;
+    g.ntiles[V] = n;
+    rc = batch_list(b, vi, 1, &p, &n);
+    if (rc) return rc;
+    g.shell_tiles[V] = p;
+    g.nshell_tiles[V] = n;
+    rc = batch_list(b, vi, 2, &p, &n);
+    if (rc) return rc;
+    g.int_tiles[V] = p;
+    g.nint_tiles[V] = n;
     return FDTD_OK;
 }
 
@@ -3206,7 +3898,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     std::vector<SimDev> descs;
     for (int q = 0; q < count; ++q) {
         fdtd_sim *s = sims[q];
-        int rc = s->pull_desc();
+        int rc = s->ready();  // the member's work split is what the batch's lists copy
+        if (rc == FDTD_OK) rc = s->pull_desc();
         if (rc) { delete b; return rc; }
         HIP_TRY(hipStreamSynchronize(s->stream));
         b->sims.push_back(s);
@@ -3214,6 +3907,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         b->group.any_offdiag |= s->host.offdiag != 0;
         b->group.general |= s->host.cond != 0 || s->host.metal[0] || s->host.metal[1] ||
                             s->host.metal[2] || s->host.npol > kLosslessPoles;
+        b->group.max_npol = std::max(b->group.max_npol, s->host.npol);
+        b->group.any_pml |= s->host.pml != 0;
     }
     b->group.all_offdiag = true;
     for (auto *s : b->sims) b->group.all_offdiag &= s->host.offdiag != 0;
@@ -3247,8 +3942,11 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         if (hipMemcpy(b->planes, planes.data(), sizeof(PlaneDev) * planes.size(), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(FDTD_EHIP, "batch plane upload");
     }
-    int rc = batch_tiles(b, g_shape[0].vec);
-    if (rc == FDTD_OK) rc = batch_tiles(b, g_shape[1].vec);
+    int rc = FDTD_OK;
+    for (const LaunchShape &sh : g_shape) {  // the widths the launches take, boxes and shells
+        if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
+        if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
+    }
     if (rc) { fdtd_batch_destroy(b); return rc; }
     for (auto *s : b->sims) s->in_batch = true;
     *out = b;
@@ -3288,8 +3986,11 @@ void fdtd_batch_destroy(fdtd_batch *b) {
 
 int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
-    int rc = batch_tiles(b, g_shape[0].vec);
-    if (rc == FDTD_OK) rc = batch_tiles(b, g_shape[1].vec);
+    int rc = FDTD_OK;
+    for (const LaunchShape &sh : g_shape) {  // the widths the launches take, boxes and shells
+        if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
+        if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
+    }
     if (rc) return rc;
     Stepper k;
     k.g = b->group;

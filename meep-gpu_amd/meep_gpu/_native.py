@@ -42,6 +42,7 @@ SIGNATURES = {
     "fdtd_pole_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]),
     "fdtd_inverse_epsilon_bytes": (ctypes.c_int, [ctypes.c_void_p, _IP]),
+    "fdtd_work_split": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _IP]),
     "fdtd_set_offdiagonal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                             _FP]),
     "fdtd_add_pole": (ctypes.c_int, [ctypes.c_void_p, _FP, ctypes.POINTER(_FP), _FP, _IP, _IP]),
@@ -88,6 +89,8 @@ SIGNATURES = {
     "fdtd_set_launch_pair": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_int32]),
     "fdtd_get_launch_pair": (ctypes.c_int, [ctypes.c_int32, _IP]),
+    "fdtd_set_launch_shell": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
+    "fdtd_get_launch_shell": (ctypes.c_int, [ctypes.c_int32, _IP]),
     "fdtd_set_graphs": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_set_flux_transform": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_batch_flush": (ctypes.c_int, [ctypes.c_void_p]),

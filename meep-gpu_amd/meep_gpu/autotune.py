@@ -20,8 +20,8 @@ from typing import Dict, Iterable, List, Optional, Tuple
 
 import numpy
 
-from .engine import (HipSimulation, device_info, get_launch_pair, set_launch,
-                     set_launch_pair)
+from .engine import (HipSimulation, device_info, get_launch_pair, get_launch_shell,
+                     set_launch, set_launch_pair, set_launch_shell)
 from .grid import Grid
 from .pml import PML
 
@@ -106,6 +106,36 @@ def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
     return out
 
 
+#: widths tried for the shell kernels (the rows on a y or z layer line)
+SHELL_WIDTHS = (1, 2)
+
+
+def tune_shell(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
+               inv: Optional[dict] = None) -> Tuple[int, int]:
+    """With the pairs' shapes installed, the fastest shell width of each pair (installed)."""
+    if pml is None:
+        return get_launch_shell(0), get_launch_shell(1)
+    sim = HipSimulation(grid, pml)
+    for c, vol in (inv or {}).items():
+        sim.set_inverse_epsilon(c, vol)
+    for term, sigma in poles:
+        sim.add_pole(term, sigma)
+    times = []
+    try:
+        for sv in SHELL_WIDTHS:
+            set_launch_shell(0, sv)
+            set_launch_shell(1, sv)
+            sim.time_pairs(2)
+            h, d = sim.time_pairs(repeats)
+            times.append((sv, h, d))
+    finally:
+        sim.close()
+    best = (min(times, key=lambda r: r[1])[0], min(times, key=lambda r: r[2])[0])
+    set_launch_shell(0, best[0])
+    set_launch_shell(1, best[1])
+    return best
+
+
 def install(shapes: PairShapes) -> None:
     set_launch_pair(0, *shapes[0])
     set_launch_pair(1, *shapes[1])
@@ -131,6 +161,7 @@ def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10
         return shapes  # type: ignore[return-value]
     shapes = best_pair_shapes(measure(grid, pml, poles, repeats=repeats, inv=inv))
     install(shapes)
+    tune_shell(grid, pml, poles, repeats=repeats, inv=inv)
     if use_cache:
         cache[key] = [list(shapes[0]), list(shapes[1])]
         _store_cache(cache)

@@ -140,6 +140,14 @@ class HipSimulation:
               "fdtd_pole_rows")
         return active.value / rows.value if rows.value else 0.0
 
+    def work_split(self, vec: int = 4) -> Dict[str, object]:
+        """The fused launches' split for runs of ``vec`` cells: the interior box (stepped by the
+        lossless forms) and the run counts of the interior and of the shell around it."""
+        out = numpy.zeros(8, dtype=numpy.int32)
+        self._check(self.lib.fdtd_work_split(self.handle, int(vec), iptr(out)), "fdtd_work_split")
+        return {"box": tuple(int(v) for v in out[:6]), "interior_runs": int(out[6]),
+                "shell_runs": int(out[7])}
+
     def inverse_epsilon_bytes(self) -> Tuple[int, int, int]:
         """Bytes per cell the D pair reads for each component's inverse epsilon (1, 2 or 4)."""
         out = numpy.zeros(3, dtype=numpy.int32)
@@ -367,6 +375,17 @@ def set_launch_pair(pair: int, vec: int, blocks_per_cu: int, xcd_remap: int,
           "fdtd_set_launch_pair")
 
 
+def set_launch_shell(pair: int, vec: int):
+    """Cells per thread of one pair's shell kernel (the layer forms outside the interior box)."""
+    check(_native.load().fdtd_set_launch_shell(int(pair), int(vec)), "fdtd_set_launch_shell")
+
+
+def get_launch_shell(pair: int) -> int:
+    out = numpy.zeros(1, dtype=numpy.int32)
+    check(_native.load().fdtd_get_launch_shell(int(pair), iptr(out)), "fdtd_get_launch_shell")
+    return int(out[0])
+
+
 def get_launch_pair(pair: int) -> Tuple[int, int, int, bool]:
     out = numpy.zeros(4, dtype=numpy.int32)
     check(_native.load().fdtd_get_launch_pair(int(pair), iptr(out)), "fdtd_get_launch_pair")
@@ -406,7 +425,8 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
     ``driven_poles``: per pole (driven components, sigma volumes); ``planes``: per flux plane
     (cells, frequencies). Mirrors the engine's allocations: 12 float32 volumes (D, B, E and
     inverse epsilon), 15 more under an absorber (H, fu_B, fu_D, f_w_E, f_w_H), two per driven
-    pole component plus its sigma volume, the ring, phase and float64 accumulators per plane.
+    pole component plus its sigma volume, the ring, phase and float64 accumulators per plane,
+    and the fused launches' work lists.
     """
     nx, ny, nz = (int(v) for v in shape)
     npad = ((nx + 3) // 4) * 4 * ny * nz
@@ -427,6 +447,9 @@ def footprint_bytes(shape: Sequence[int], pml: bool, driven_poles: Sequence[Tupl
         total += 8 * nf + 4 * RING * 4 * row + 8 * RING + 8 * RING * 4 * nf + 64 * nf * cells
         total += 512 * (len(planes) + 1)
     total += 16 * source_points + 16 * (source_steps + 1) + 256
+    # the work split: shell lists for runs of 1, 2 and 4 cells (at most every run) and the
+    # work items of kThreads runs each, per width
+    total += 4 * (npad + npad // 2 + npad // 4) + 16 * 3 * (npad // 256 + 2) + 64
     return int(total)
 
 

@@ -77,3 +77,30 @@ def test_matrix_core_and_vector_flux_transforms_agree():
             scale = numpy.abs(b).max()
             assert scale > 0
             assert numpy.abs(a - b).max() <= 1e-12 * scale
+
+
+def test_the_work_split_covers_the_grid():
+    """The box is whole rows whose y and z lie on no layer line (so only x-layer forms hold in
+    it), the shell every other row: together they are every run of real cells (the parity tests
+    above check the words)."""
+    from meep_gpu.driver import FdtdDriver
+    from tests.workload_runs import small
+
+    drv = FdtdDriver.from_workload(small("sphere"))
+    drv.freeze(planned_steps=2)
+    nx, ny, nz = drv.grid.shape
+    pitch = -(-nx // 4) * 4
+    pml = drv.sim.pml
+    for vec in (1, 2, 4):
+        split = drv.sim.work_split(vec)
+        x0, x1, y0, y1, z0, z1 = split["box"]
+        assert (x0, x1) == (0, pitch) and y0 < y1 and z0 < z1
+        for axis, (lo, hi) in ((1, (y0, y1)), (2, (z0, z1))):
+            for half in (0, 1):
+                assert not numpy.any(pml.members[(half, axis)][lo:hi])
+                # the box is the longest such range: the rows just past it are on a line
+                assert pml.members[(half, axis)][lo - 1] or pml.members[(1 - half, axis)][lo - 1]
+        inside_rows = (y1 - y0) * (z1 - z0)
+        assert split["interior_runs"] == (pitch // vec) * inside_rows
+        assert split["shell_runs"] == (ny * nz - inside_rows) * -(-nx // vec)
+    drv.close()
