@@ -364,7 +364,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-count", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-saturated-rate", type=float, default=0.0,
+    ap.add_argument("--serve-saturated-rate", type=float, default=64.0,
                     help="a second stream whose arrivals outpace the device (capacity); 0: none")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step eagerly, in every leg (for kernel tracing)")

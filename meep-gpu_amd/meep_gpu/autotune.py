@@ -33,23 +33,20 @@ CANDIDATES: Tuple[Shape, ...] = tuple(
     for nt in (False, True))
 
 
-def plane_superblock(grid: Grid, vec: int) -> int:
-    """Super-block size that keeps a tile's j-1 rows and k-1 plane on its own XCD: an eighth of
-    the tiles per z plane when that is a whole number (else 0)."""
-    nx, ny, _ = grid.shape
-    plane = ((nx + 3) // 4) * 4 * ny
-    tile = 256 * vec
-    if plane % tile:
-        return 0
-    per_plane = plane // tile
-    return per_plane // 8 if per_plane % 8 == 0 and per_plane // 8 >= 2 else 0
+def plane_superblock(sim, vec: int) -> int:
+    """Super-block size (work items) of about one eighth of a plane of the box, for the tile
+    order that deals super-blocks round robin to the 8 XCDs (0 when under 2 items)."""
+    x0, x1, y0, y1, z0, z1 = sim.work_split(vec)["box"]
+    items = -(-((x1 - x0) // vec) * (y1 - y0) // 256)
+    sb = -(-items // 8)
+    return sb if sb >= 2 else 0
 
 
-def candidates_for(grid: Grid) -> Tuple[Shape, ...]:
-    """The generic candidates plus, per vector width, the plane-aligned super-block order."""
+def candidates_for(sim) -> Tuple[Shape, ...]:
+    """The generic candidates plus, per vector width, the plane-eighth super-block order."""
     extra = []
     for vec in (1, 2, 4):
-        sb = plane_superblock(grid, vec)
+        sb = plane_superblock(sim, vec)
         if sb:
             extra += [(vec, bpc, sb, nt) for bpc in (0, 4, 8) for nt in (False, True)]
     return CANDIDATES + tuple(extra)
@@ -85,13 +82,13 @@ def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
             inv: Optional[dict] = None) -> List[Tuple[Shape, float, float]]:
     """(shape, mean H-pair ms, mean D-pair ms) for every candidate, on a scratch simulation
     (with the run's inverse epsilon when given: its storage form changes the D pair)."""
-    if candidates is None:
-        candidates = candidates_for(grid)
     sim = HipSimulation(grid, pml)
     for c, vol in (inv or {}).items():
         sim.set_inverse_epsilon(c, vol)
     for term, sigma in poles:
         sim.add_pole(term, sigma)
+    if candidates is None:
+        candidates = candidates_for(sim)
     saved = (get_launch_pair(0), get_launch_pair(1))
     out = []
     try:
