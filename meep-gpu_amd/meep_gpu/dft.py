@@ -37,20 +37,26 @@ def meep_frequency_span(fcen: float, df: float, nfreq: int) -> numpy.ndarray:
     return numpy.linspace(fcen - 0.5 * df, fcen + 0.5 * df, int(nfreq))
 
 
-def boundary_weight_ladder(where_min: float, where_max: float, a: float
+def boundary_weight_ladder(where_min: float, where_max: float, a: float, iyee_c: int = 0
                            ) -> Tuple[int, int, Tuple[float, float, float, float]]:
-    """MEEP's integration ladder for one direction of a monitor volume on the centred grid.
+    """MEEP's integration ladder for one direction of a volume on the centred grid.
 
     ``vec2diel_floor`` / ``vec2diel_ceil`` bracket the bounds on the odd doubled lattice (the
     cell centres sit at doubled coordinates 2k+1, i.e. at (k + 1/2)/a from the origin) and
     ``compute_boundary_weights`` picks the four end weights by the same case ladder
     (MEEP loop_in_chunks.cpp:233-297 as reference dft.py:604-690 states it, for parity 1).
-    ``w0``/``w1`` are the overlaps of the first and last cell with the request, so an extended
-    direction integrates to exactly ``(where_max - where_min) * a`` cells and a flat one is a
-    linear interpolation (the weights sum to 1). Returns (is, ie, (s0, s1, e0, e1)).
+    ``iyee_c`` is the Centered shift minus a component's Yee shift along the direction (0
+    for a monitor's cell centres, 1 for a component on the integer lattice,
+    loop_in_chunks.cpp:352-359, reference sources.py:870-885): the bounds move by
+    ``iyee_c/2`` cells onto that component's samples and the weights stay the overlaps of
+    the original request. ``w0``/``w1`` are the overlaps of the first and last sample with
+    the request, so an extended direction integrates to exactly ``(where_max - where_min) *
+    a`` cells and a flat one is a linear interpolation (the weights sum to 1).
+    Returns (is, ie, (s0, s1, e0, e1)).
     """
-    is_d = 1 + 2 * math.floor(where_min * a - 0.5)
-    ie_d = 1 + 2 * math.ceil(where_max * a - 0.5)
+    yee_c = iyee_c * 0.5 * (1.0 / a)
+    is_d = 1 + 2 * math.floor((where_min + yee_c) * a - 0.5) - iyee_c
+    ie_d = 1 + 2 * math.ceil((where_max + yee_c) * a - 0.5) - iyee_c
     w0 = 1.0 - where_min * a + 0.5 * is_d
     w1 = 1.0 + where_max * a - 0.5 * ie_d
     span = ie_d - is_d

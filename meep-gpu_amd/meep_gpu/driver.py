@@ -28,7 +28,7 @@ from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
 from .grid import Grid
 from .pml import PML
 from .readback import meep_array
-from .sources import DepositBox, current_table
+from .sources import DepositBox, current_table, meep_volume_source
 from .workloads import Workload
 
 _E = ("Ex", "Ey", "Ez")
@@ -174,9 +174,21 @@ class FdtdDriver:
         self._pml_spec = thickness_cells
         self._pml_kwargs = dict(order=order, R_asymptotic=R_asymptotic)
 
-    def add_source(self, box: DepositBox, time_function):
+    def add_source(self, box, time_function):
+        """A deposit box (or a list of them, deposited in order) driven by ``time_function``."""
         self._require_unfrozen("add_source")
-        self._sources.append((box, time_function))
+        for b in ([box] if isinstance(box, DepositBox) else list(box)):
+            self._sources.append((b, time_function))
+
+    def add_volume_source(self, component: str, center: Sequence[float], size: Sequence[float],
+                          time_function, amplitude: complex = 1.0) -> List[DepositBox]:
+        """``mp.Source(src, component, center, size, amplitude)``: placed as MEEP places it
+        on this grid and its boundaries (``meep_volume_source``); returns the deposit boxes."""
+        self._require_unfrozen("add_volume_source")
+        boxes = meep_volume_source(self.grid, component, center, size, amplitude,
+                                   metallic=tuple(not w for w in self.wraps))
+        self.add_source(boxes, time_function)
+        return boxes
 
     def add_flux_monitor(self, normal: int, lo: Sequence[int], cnt: Sequence[int],
                          fcen: float = None, df: float = None, nfreq: int = None,

@@ -22,9 +22,11 @@ from __future__ import annotations
 import cmath
 import math
 from dataclasses import dataclass, field
-from typing import Callable, Optional, Sequence, Tuple
+from typing import Callable, List, Optional, Sequence, Tuple
 
 import numpy
+
+from .dft import boundary_weight_ladder, ladder_weight
 
 
 def amp_factor(frequency: float) -> complex:
@@ -139,56 +141,143 @@ _IYEE = {"Ex": (1, 0, 0), "Ey": (0, 1, 0), "Ez": (0, 0, 1),
          "Hx": (0, 1, 1), "Hy": (1, 0, 1), "Hz": (1, 1, 0)}
 
 
-def _zero_extent_weights(coord: float, n: int, dx: float, shift: int) -> Tuple[int, Tuple[float, ...]]:
-    """The two samples a zero-size source direction lands between and their linear weights.
+def _cmul(p: Tuple[float, float], q: Tuple[float, float]) -> Tuple[float, float]:
+    """C++ ``complex<double>`` multiply with every product rounded: (ac - bd, ad + bc)."""
+    return (p[0] * q[0] - p[1] * q[1], p[0] * q[1] + p[1] * q[0])
 
-    MEEP's boundary weights for a zero-extent direction (``where_min == where_max``,
-    loop_in_chunks.cpp:257-297, reference sources.py:657-696): the point at ``coord`` (origin
-    at the cell centre) splits between the component's samples below and above it as
-    ``(1 - f, f)``. A point exactly on a sample keeps both, the upper with weight 0.
+
+def _end_weights(start: int, end: int, is_d: int, ie_d: int, ends):
+    """The end weights of a run [start, end] of doubled coordinates clipped out of [is, ie]
+    (reference sources.py:1097-1121): a run that lost its first (last) point takes the next
+    end weight, and a run of one, two or three points keeps the smaller of the weights that
+    meet on a point."""
+    s0, s1, e0, e1 = ends
+    ws0 = ws1 = we0 = we1 = 1.0
+    if start == is_d:
+        ws0, ws1 = s0, s1
+    elif start == is_d + 2:
+        ws0 = s1
+    if end == ie_d:
+        we0, we1 = e0, e1
+    elif end == ie_d - 2:
+        we0 = e1
+    if end == start:
+        ws0 = we0 = ws1 = we1 = min(ws0, we0)
+    elif end == start + 2:
+        ws0 = we1 = min(ws0, we1)
+        ws1 = we0 = min(ws1, we0)
+    elif end == start + 4:
+        ws1 = we1 = min(ws1, we1)
+    return ws0, ws1, we0, we1
+
+
+def _axis_passes(n: int, a: float, iyee: int, where_min: float, where_max: float, wraps: bool):
+    """One resolved direction of a source request: for each lattice shift along it, the
+    stored rows it deposits into and their weights, as contiguous (first row, weights) runs.
+
+    MEEP loop_in_chunks.cpp:349-418 on one chunk: the bounds and boundary weights on the
+    component's lattice, the owned window ``little_corner + 2 - iyee .. big_corner - iyee``
+    with the axis starting at doubled ``io = -(n - n % 2)`` (the centred origin of an even
+    or odd count), and on a periodic axis every shift of one lattice vector that meets the
+    window. A row that the window maps past the last stored row is the periodic image of row
+    0, or on a metallic high wall a plane zeroed before anything reads it (no deposit).
+    Returns {shift: [(first_row, weights), ...]}, shifts in increasing order.
     """
-    pos = (coord + 0.5 * n * dx) / dx - 0.5 * shift  # in sample units of this component
-    i0 = int(math.floor(pos + 1e-12))
-    f = pos - i0
-    if abs(f) < 1e-12:
-        f = 0.0
-    if not 0 <= i0 < n - 1:
-        raise ValueError(f"a source at {coord} is not between two interior samples")
-    return i0, (1.0 - f, f)
+    is_d, ie_d, ends = boundary_weight_ladder(where_min, where_max, a, 1 - iyee)
+    io = -(n - n % 2)
+    low, high = io + 2 - iyee, io + 2 * n - iyee
+    shifts = [0]
+    if wraps:
+        half = 0.5 / a
+        lattice = 2 * n * half
+        shifts = range(int(math.floor((where_min - (io + 2 * n) * half) / lattice)),
+                       int(math.ceil((where_max - io * half) / lattice)) + 1)
+    out = {}
+    for k in shifts:
+        sh = 2 * n * k
+        start, end = max(is_d - sh, low), min(ie_d - sh, high)
+        if start > end:
+            continue
+        ws = _end_weights(start + sh, end + sh, is_d, ie_d, ends)
+        count = (end - start) // 2 + 1
+        runs, rows, weights = [], [], []
+        for q in range(count):
+            row = (start + 2 * q - io) // 2
+            if row >= n:
+                if not wraps:
+                    continue
+                row -= n
+            if rows and row != rows[-1] + 1:
+                runs.append((rows[0], numpy.array(weights)))
+                rows, weights = [], []
+            rows.append(row)
+            weights.append(ladder_weight(q, count, *ws))
+        if rows:
+            runs.append((rows[0], numpy.array(weights)))
+        if runs:
+            out[k] = runs
+    return out
 
 
 def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequence[float],
-                       amplitude: complex = 1.0) -> DepositBox:
-    """A MEEP ``mp.Source`` whose size is zero or the whole cell along each axis.
+                       amplitude: complex = 1.0, metallic: Sequence[bool] = (False, False, False)
+                       ) -> List[DepositBox]:
+    """A MEEP ``mp.Source`` with a constant amplitude, placed as MEEP places it on one chunk.
 
-    Zero-size directions interpolate linearly between the two nearest samples of the
-    component and scale the amplitude by ``a`` (the resolution, MEEP sources.cpp:481-484,
-    reference sources.py:860-866): a point source in 3-D deposits ``amp * a^3 * w`` -- the
-    discrete delta of a current density. A direction the source spans whole takes every
-    sample with weight 1 (a periodic axis has no boundary sample to halve).
+    MEEP's ``add_volume_source`` (sources.cpp:456-489, reference sources.py:787-1140): the
+    amplitude is scaled by ``a`` (the resolution) once per zero-size resolved direction -- a
+    point source in 3-D deposits ``amp * a^3 * w``, the discrete delta of a current density --
+    and each resolved direction integrates the request over the component's samples with
+    MEEP's boundary weights: a zero-size direction interpolates linearly between the two
+    samples around the point (one sample with weight 1 when the point is on it), an extended
+    one takes the ladder's end weights at its two ends. A direction the dimension does not
+    carry is one sample of weight 1.
+
+    The result is one box per lattice-shift pass (MEEP's ``src_vol`` of each shift, deposited
+    in that order), split where the owned window wraps: a request that reaches a periodic
+    face deposits the two images that meet on a row separately, rounding twice as MEEP does.
+    The point weight is ``w_z * (w_y * w_x)`` (vec.hpp:381-383, exact in any grouping of the
+    two-factor products) and the amplitude meets it, and the shift phase 1, as complex
+    products.
     """
     iyee = _IYEE[component]
-    a = grid.resolution
-    lo, cnt, weights = [], [], []
-    amp = complex(amplitude)
+    a = float(grid.resolution)
+    inva = 1.0 / a
+    amp = (float(complex(amplitude).real), float(complex(amplitude).imag))
+    lengths = tuple(grid.shape[d] * grid.dx for d in range(3))
+    per_axis = []
     for d in range(3):
         n = grid.shape[d]
         if grid.is_invariant(d):
-            lo.append(0)
-            cnt.append(1)
-            weights.append(numpy.ones(1))
+            per_axis.append({0: [(0, numpy.ones(1))]})
             continue
-        if size[d] == 0.0:
-            i0, w = _zero_extent_weights(float(center[d]), n, grid.dx, iyee[d])
-            lo.append(i0)
-            cnt.append(2)
-            weights.append(numpy.array(w))
-            amp = amp * a
-        elif abs(size[d] - n * grid.dx) < 1e-9 * n * grid.dx:
-            lo.append(0)
-            cnt.append(n)
-            weights.append(numpy.ones(n))
-        else:
-            raise ValueError("a source extent must be zero or the whole cell along each axis")
-    w = weights[0][:, None, None] * weights[1][None, :, None] * weights[2][None, None, :]
-    return DepositBox(component, tuple(lo), tuple(cnt), w * amp.real, w * amp.imag)
+        lo_w = float(center[d]) - float(size[d]) / 2
+        hi_w = float(center[d]) + float(size[d]) / 2
+        span = hi_w - lo_w
+        if span < 0.0:
+            raise ValueError("a source size must be non-negative")
+        if span > lengths[d] + inva:
+            raise ValueError(f"source size {size[d]} exceeds the cell length {lengths[d]} by "
+                             f"more than one cell (MEEP's add_volume_source aborts)")
+        if span > lengths[d]:  # MEEP snaps a request wider by under a pixel to one cell
+            lo_w = lo_w - (span - lengths[d]) * 0.5
+            hi_w = lo_w + lengths[d]
+        if hi_w - lo_w == 0.0:
+            amp = (amp[0] * a, amp[1] * a)
+        per_axis.append(_axis_passes(n, a, iyee[d], lo_w, hi_w, not metallic[d]))
+    pass_amp = _cmul(amp, (1.0, -0.0))
+    boxes: List[DepositBox] = []
+    # MEEP's shift odometer runs its first direction fastest
+    for kz in sorted(per_axis[2]):
+        for ky in sorted(per_axis[1]):
+            for kx in sorted(per_axis[0]):
+                for x0, wx in per_axis[0][kx]:
+                    for y0, wy in per_axis[1][ky]:
+                        for z0, wz in per_axis[2][kz]:
+                            w = (wx[:, None, None] * wy[None, :, None]) * wz[None, None, :]
+                            re = w * pass_amp[0]
+                            im = w * pass_amp[1]
+                            # times the amplitude function's value, one: (re*1 - im*0, re*0 + im*1)
+                            boxes.append(DepositBox(component, (x0, y0, z0), w.shape,
+                                                    re * 1.0 - im * 0.0, re * 0.0 + im * 1.0))
+    return boxes

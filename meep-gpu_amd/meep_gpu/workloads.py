@@ -100,7 +100,7 @@ def _sphere_inverse_epsilon(grid: Grid, radius: float, eps_in: float, eps_out: f
     return out
 
 
-def _ez_point(grid: Grid, x: float, y: float, z: float) -> DepositBox:
+def _ez_point(grid: Grid, x: float, y: float, z: float) -> List[DepositBox]:
     """An Ez point source at (x, y, z), deposited as MEEP deposits a zero-size source."""
     return meep_volume_source(grid, "Ez", (x, y, z), (0.0, 0.0, 0.0))
 
@@ -144,7 +144,7 @@ def sphere(edge: int = 80, resolution: float = 10.0, pml_cells: int = 8,
     freqs = meep_frequency_span(fcen, df, nfreq)
     src = GaussianSource(fcen, df)
     wl = Workload(f"sphere_{edge}", grid, pml_cells, inv, steps=steps)
-    wl.sources.append((_ez_point(grid, source_x, 0.0, 0.0), src))
+    wl.sources.extend((box, src) for box in _ez_point(grid, source_x, 0.0, 0.0))
     c = edge // 2
     if planes == "one":
         half = edge // 4
@@ -187,8 +187,9 @@ def slab(edge: int = 160, resolution: float = 20.0, pml_cells: int = 10, thickne
     wl.poles.append((Susceptibility(f0, gamma, LORENTZIAN), {c: sig for c in _E}))
     src_x = pml_cells + 5
     x = -0.5 * L + (src_x + 0.25) * grid.dx  # a quarter cell past the sample: both weights live
-    wl.sources.append((meep_volume_source(grid, "Ez", (x, 0.0, 0.0), (0.0, L, L)),
-                       GaussianSource(fcen, df)))
+    pulse = GaussianSource(fcen, df)
+    wl.sources.extend((box, pulse)
+                      for box in meep_volume_source(grid, "Ez", (x, 0.0, 0.0), (0.0, L, L)))
     freqs = meep_frequency_span(fcen, df, nfreq)
     inner = edge - 2 * pml_cells
     for pos in (src_x + 5, lo_x + thickness + 10):

@@ -27,12 +27,22 @@ from typing import Any, Callable, Dict, List, Optional, Tuple
 import numpy
 
 from . import chunks, fixture, layer
+from . import sources as placement
 from .engine import OracleSim, PointDeposit
 
 _E = ("Ex", "Ey", "Ez")
 #: Yee shifts of the conductive components (D like E, B like H; MEEP's iyee_shift)
 _IYEE = {"Dx": (1, 0, 0), "Dy": (0, 1, 0), "Dz": (0, 0, 1),
          "Bx": (0, 1, 1), "By": (1, 0, 1), "Bz": (1, 1, 0)}
+
+
+@dataclass(frozen=True)
+class Source:
+    """One stored-current source of a case (array_path_oracle.py:393-399): a custom waveform
+    that returns the case's stored current word for each step."""
+    component: str
+    center: Tuple[float, float, float]
+    size: Tuple[float, float, float] = (0.0, 0.0, 0.0)
 
 
 @dataclass(frozen=True)
@@ -49,6 +59,11 @@ class Case:
     #: A conductivity kept only inside this block of fractions of each axis
     #: (array_path_oracle.py:626-633).
     conductivity_block: Any = None
+    sources: Tuple[Source, ...] = ()
+    #: The fused route's step order (array_path_oracle.py:1061-1109): the curl and the
+    #: constitutive update run before the deposit and the deposit points' constitutive words
+    #: are recomputed afterwards, which leaves the words of the plain order.
+    repair: bool = False
 
 
 #: The fixture cases this engine steps: real storage, no symmetry; periodic or metallic axes;
@@ -72,6 +87,10 @@ CASES: Tuple[Case, ...] = (
     Case("conductive_chunks_pml_2d", (2.0, 2.0, 0.0), dimensions=2, pml={"x": 2, "y": 2},
          d_conductivity=True, b_conductivity=True, metallic=(True, True, False),
          conductivity_block=((0.5, 1.0), (0.5, 1.0), (0.0, 1.0))),
+    # an electric point between four samples and a magnetic one on a sample row in x
+    Case("repair_pml_2d", (1.5, 1.5, 0.0), dimensions=2, pml={"x": 2, "y": 2},
+         sources=(Source("Ez", (0.0625, 0.0625, 0.0)), Source("Hz", (-0.1875, 0.125, 0.0))),
+         repair=True),
 )
 
 #: Cases of this build's own, in the fixture cases' form (seeded the same way, by name) but with
@@ -102,6 +121,8 @@ class CaseState:
     volumes: Dict[str, numpy.ndarray]    # the seeded stored state, by digest name
     stores_E: bool
     pml: bool
+    #: {component, index (C-order flat), amp_re, amp_im, table (stored current per step)}
+    sources: List[Dict[str, Any]] = field(default_factory=list)
 
 
 def _grid_shape(case: Case) -> Tuple[Tuple[int, int, int], Tuple[bool, bool, bool]]:
@@ -172,8 +193,16 @@ def build_state(name: str) -> CaseState:
             if numpy.any(pole["sigma"][c] != 0):
                 names += [f"P[{k}].{c}", f"P_prev[{k}].{c}"]
     volumes = {n: fixture.state_words(fixture.seed(name, "state", n), shape) for n in names}
+    placed = []
+    for index, src in enumerate(case.sources):  # array_path_oracle.py:692-703
+        cells, amp_re, amp_im = placement.deposit_points(
+            shape, invariant, fixture.RESOLUTION, src.component, src.center, src.size,
+            wraps=tuple(not m for m in case.metallic))
+        placed.append({"component": src.component, "index": cells, "amp_re": amp_re,
+                       "amp_im": amp_im,
+                       "table": fixture.stored_currents(fixture.seed(name, "current", str(index)))})
     return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, conductivity,
-                     volumes, stores_E, pml)
+                     volumes, stores_E, pml, placed)
 
 
 def oracle_sim(state: CaseState) -> OracleSim:
@@ -193,6 +222,11 @@ def oracle_sim(state: CaseState) -> OracleSim:
     for n, vol in state.volumes.items():
         target = sim.stored_volumes()[n]
         target[...] = vol
+    for src in state.sources:
+        table = src["table"]
+        sim.add_deposit(PointDeposit(
+            src["component"], src["index"], src["amp_re"], src["amp_im"],
+            lambda t, table=table: fixture.stored_current_at(table, t, state.dt)))
     return sim
 
 

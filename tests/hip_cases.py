@@ -17,6 +17,7 @@ from meep_gpu.dispersion import DRUDE, LORENTZIAN, Susceptibility
 from meep_gpu.engine import HipSimulation
 from meep_gpu.grid import Grid
 from meep_gpu.pml import PML
+from meep_gpu.sources import meep_volume_source
 from oracle import cases, fixture
 
 
@@ -44,6 +45,13 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
     for axis, metal in enumerate(case.metallic):
         if metal:
             sim.set_boundary(axis, "metallic")
+    for spec, placed in zip(case.sources, state.sources):
+        # the product's placement; the stored current word of step n serves both slots
+        current = numpy.array([[w.real, w.imag] for w in placed["table"]], dtype=numpy.float64)
+        for box in meep_volume_source(grid, spec.component, spec.center, spec.size,
+                                      metallic=case.metallic):
+            sim.add_source(box.magnetic, "xyz".index(box.component[1]), box.lo, box.cnt,
+                           box.amp_re, box.amp_im, current, grid.dt)
     for n, vol in state.volumes.items():
         sim.upload(n, vol)
     if not state.stores_E:

@@ -36,7 +36,7 @@ class Example:
     grid: Grid
     inv: Dict[str, numpy.ndarray]
     poles: list
-    box: DepositBox
+    boxes: List[DepositBox]
     source: GaussianSource
 
 
@@ -68,8 +68,9 @@ def build(name: str) -> Example:
         lz = slab["lorentz"]
         poles.append((Susceptibility(lz["frequency"], lz["gamma"], LORENTZIAN), sig))
     src = spec["source"]
-    box = meep_volume_source(grid, src["component"], tuple(src["center"]), (0.0, 0.0, 0.0))
-    return Example(name, spec, grid, inv, poles, box, GaussianSource(src["frequency"], src["fwidth"]))
+    boxes = meep_volume_source(grid, src["component"], tuple(src["center"]), (0.0, 0.0, 0.0))
+    return Example(name, spec, grid, inv, poles, boxes,
+                   GaussianSource(src["frequency"], src["fwidth"]))
 
 
 def centred_ez_norms(ez: numpy.ndarray):

@@ -20,7 +20,7 @@ def run_engine(name):
     for term, sigma in ex.poles:
         drv.add_susceptibility(term, sigma)
     drv.setup_pml(int(round(spec["pml"] * spec["resolution"])))
-    drv.add_source(ex.box, ex.source)
+    drv.add_source(ex.boxes, ex.source)
     fl = spec["flux"]
     mons = [drv.add_flux_region(r["center"], r["size"], fl["fcen"], fl["df"], fl["nfreq"],
                                 decimation=spec["decimation"]) for r in fl["regions"]]

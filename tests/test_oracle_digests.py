@@ -70,3 +70,34 @@ def test_keeping_the_conductivity_in_every_chunk_moves_the_block_digest(golden_d
     monkeypatch.setattr(chunks, "conductive_cells",
                         lambda shape, *a, **k: numpy.ones(tuple(shape), dtype=bool))
     assert cases.run_case(name) != golden_digests["case_sha256"][name]
+
+
+@pytest.mark.parametrize("mistake", ["magnetic_after_update_H", "magnetic_a_step_late"])
+def test_a_planted_deposit_mistake_moves_the_repair_digest(golden_digests, monkeypatch, mistake):
+    """repair_pml_2d pins the magnetic deposit: into B before H is formed from it, with the
+    stored current word of its own step. (Its points are pinned by the reference's recorded
+    point counts: a zero-weight neighbour would not move this digest.)"""
+    from oracle import engine
+
+    name = "repair_pml_2d"
+    if mistake == "magnetic_after_update_H":
+        def step(self):
+            self.step_B()
+            self.zero_metal("B")
+            self.update_H()
+            self.deposit(True, self.time)
+            self.step_D()
+            self.deposit(False, self.time + 0.5 * self.dt)
+            self.zero_metal("D")
+            self.update_E()
+            self.update_P()
+            self.step_count += 1
+            self.time = self.step_count * self.dt
+        monkeypatch.setattr(engine.OracleSim, "step", step)
+    elif mistake == "magnetic_a_step_late":
+        original = engine.OracleSim.deposit
+
+        def late(self, magnetic, time_value):
+            original(self, magnetic, time_value + (self.dt if magnetic else 0.0))
+        monkeypatch.setattr(engine.OracleSim, "deposit", late)
+    assert cases.run_case(name) != golden_digests["case_sha256"][name]

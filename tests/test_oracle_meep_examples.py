@@ -38,8 +38,8 @@ def run_oracle(name, decimation=None):
             planes.append((0, lo, cnt, mon.omegas, mon.scale, dec))
     poles = [(t.frequency, t.gamma, t.is_drude, s) for t, s in ex.poles]
     table = current_table(ex.source, grid.dt, steps + 1, False)
-    deposit = (ex.box.component, ex.box.lo, ex.box.cnt, ex.box.amp_re, ex.box.amp_im, table)
-    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, ex.inv, poles, [deposit],
+    deposits = [(b.component, b.lo, b.cnt, b.amp_re, b.amp_im, table) for b in ex.boxes]
+    sim, oplanes = ow.build(grid.shape, grid.dt, grid.dx, faces, ex.inv, poles, deposits,
                             planes, grid.invariant)
     ow.run(sim, oplanes, steps, reference_precision=False)
     spectra = []

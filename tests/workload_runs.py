@@ -8,7 +8,7 @@ from meep_gpu import workloads
 from meep_gpu.dft import FluxMonitor
 from meep_gpu.driver import FdtdDriver
 from meep_gpu.pml import normalize_faces
-from meep_gpu.sources import current_table
+from meep_gpu.sources import GaussianSource, current_table, meep_volume_source
 from oracle import workload as ow
 
 
@@ -20,6 +20,15 @@ def small(name: str):
     if name == "slab":
         return workloads.slab(edge=40, resolution=8.0, pml_cells=6, thickness=6, nfreq=9,
                               steps=160)
+    if name == "sphere_sources":
+        # the sphere with a magnetic point source off the lattice and a finite electric sheet
+        wl = small("sphere")
+        grid = wl.grid
+        for box in meep_volume_source(grid, "Hz", (0.33, -0.71, 0.52), (0.0, 0.0, 0.0)):
+            wl.sources.append((box, GaussianSource(0.12, 0.08)))
+        for box in meep_volume_source(grid, "Ex", (-0.4, 0.9, 0.1), (0.0, 1.3, 0.7)):
+            wl.sources.append((box, GaussianSource(0.18, 0.1)))
+        return wl
     if name == "sphere_box":
         return workloads.sphere(edge=40, resolution=5.0, pml_cells=6, radius_cells=8.0,
                                 nfreq=6, planes="box8", steps=120)
