@@ -111,7 +111,13 @@ int fdtd_add_pole(fdtd_sim *sim, const float consts[4], const float *const sigma
  * fastest), and the complex current J for steps 0..nsteps-1 as (re, im) pairs: the electric
  * table at (n+1/2)*dt, the magnetic at n*dt. field 0 deposits into D (after step_D), 1 into
  * B (after step_B). Replaces sources._deposit (sources.py:1357) + host_writes.subtract_double
- * (host_writes.py:140): f = float(double(f) - (amp_re*J_re - amp_im*J_im)*dt). */
+ * (host_writes.py:140): f = float(double(f) - (amp_re*J_re - amp_im*J_im)*dt), times the
+ * cell's condinv word on a conductive component.
+ * field 2 is an electric polarization (MEEP is_integrated) source, which replaces
+ * sources.f_minus_p_begin/_end (sources.py:1697-1760): nothing is deposited into D; the table
+ * is the dipole P at (n+1)*dt, and update_E reads float(double(D) - (amp_re*P_re -
+ * amp_im*P_im)) at the box's cells (no dt, no condinv) while D keeps its word. field 3 (a
+ * magnetic polarization source) and field 2 with an off-diagonal chi1inv return FDTD_EINVAL. */
 int fdtd_add_source(fdtd_sim *sim, int32_t field, int32_t component, const int32_t lo[3],
                     const int32_t cnt[3], const double *amp_re, const double *amp_im,
                     const double *current, int32_t nsteps, double dt);

@@ -73,7 +73,8 @@ int fail(int code, const char *fmt, ...) {
                         __FILE__, __LINE__);                                                 \
     } while (0)
 
-// One source box: deposit into D (field 0) or B (field 1) component `comp`.
+// One source box: deposit into D (field 0) or B (field 1) component `comp`, or (field 2) an
+// electric polarization source, whose dipole the constitutive update subtracts from D.
 struct SrcDev {
     int field, comp;
     int lo[3], cnt[3];
@@ -92,6 +93,7 @@ struct SimDev {
     int offdiag;
     int npol;
     int nsrc;
+    int npsrc;      // electric polarization (integrated) sources among them
     int step;       // completed steps (driver.py:3592)
     int hsep;       // MEEP's one-time H start done (stepping.py:2789)
     int ppar;       // which P buffer holds P^n
@@ -118,7 +120,7 @@ struct SimDev {
     const float *psig[kMaxPoles][3];
     float psig_s[kMaxPoles][3];
     const SrcDev *src;
-    int src_lo[2][3], src_hi[2][3];  // per field (D, B): union of the source boxes, [lo, hi)
+    int src_lo[3][3], src_hi[3][3];  // per field (D, B, polarization): union of the boxes, [lo, hi)
     // per field (0 = D, 1 = B) and component: a conductivity's condfac and condinv words and,
     // under an absorber, its f_cond history; null where the component is lossless
     const float *cf[2][3], *ci[2][3];
@@ -492,6 +494,39 @@ __device__ inline void deposit_run(const SimDev &s, int step, int field, int com
     }
 }
 
+// The electric polarization (integrated) sources of component `comp` at the run: MEEP's
+// f_minus_p (update_eh.cpp:66-140, sources.py:1697-1747). The dipole P, the step's table word
+// (at (n+1)*dt), is subtracted from the copy of D that the constitutive update reads:
+// x = float(double(x) - (amp_re*P_re - amp_im*P_im)), once per source in list order, with no
+// dt and no conductivity; D itself keeps its word.
+template <int V>
+__device__ inline void polarize_run(const SimDev &s, int step, int comp, const Run &q, VF<V> &x) {
+    if (q.j < s.src_lo[2][1] || q.j >= s.src_hi[2][1] || q.k < s.src_lo[2][2] ||
+        q.k >= s.src_hi[2][2] || q.i0 + V <= s.src_lo[2][0] || q.i0 >= s.src_hi[2][0])
+        return;
+    for (int n = 0; n < s.nsrc; ++n) {
+        const GMEM SrcDev &d = gp(s.src)[n];
+        if (d.field != 2 || d.comp != comp) continue;
+        const int b = q.j - d.lo[1], e = q.k - d.lo[2];
+        if (b < 0 || e < 0 || b >= d.cnt[1] || e >= d.cnt[2]) continue;
+        if (q.i0 + V <= d.lo[0] || q.i0 >= d.lo[0] + d.cnt[0]) continue;
+        if (step >= d.ncur) continue;
+        const double pre = gp(d.cur)[2 * step], pim = gp(d.cur)[2 * step + 1];
+        if (plus_zero(pre) && plus_zero(pim)) continue;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            const int a = q.i0 + m - d.lo[0];
+            if (a < 0 || a >= d.cnt[0]) continue;
+            const int p = (e * d.cnt[1] + b) * d.cnt[0] + a;
+            double r = gp(d.amp_re)[p] * pre;
+            r = r - gp(d.amp_im)[p] * pim;
+            double w = (double)x.v[m];
+            w = w - r;
+            x.v[m] = (float)w;
+        }
+    }
+}
+
 __device__ inline int coord_of(const Run &q, int axis, int m) {
     return axis == 0 ? q.i0 + m : (axis == 1 ? q.j : q.k);
 }
@@ -855,8 +890,12 @@ template <int V, int C, bool PML, bool NT, bool GEN>
 __device__ inline void dpair_component(const SimDev &s, const Clock &ck, const Run &q,
                                        const VF<V> (&h)[3], const VF<V> (&hn)[3][3],
                                        const VF<V> &d0, const VF<V> &iv) {
-    const VF<V> d = dcurl_component<V, C, PML, GEN>(s, ck, q, h, hn, d0);
+    VF<V> d = dcurl_component<V, C, PML, GEN>(s, ck, q, h, hn, d0);
     stv<V, NT>(s.D[C], q.at, d);
+    // the constitutive update reads D less the polarization sources' dipole (general forms)
+    if constexpr (GEN) {
+        if (s.npsrc) polarize_run<V>(s, ck.step, C, q, d);
+    }
     constexpr int NP = GEN ? kMaxPoles : kLosslessPoles;
     VF<V> p[NP];
     // a cell where every pole's sigma and stored P are zero keeps P = +0 for good (its update
@@ -1971,6 +2010,13 @@ struct Group {
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
 };
 
+// Does a descriptor need the pairs' general instantiation: a conductivity, a metallic wall,
+// more than kLosslessPoles poles or a polarization source?
+bool needs_general(const SimDev &h) {
+    return h.cond != 0 || h.metal[0] || h.metal[1] || h.metal[2] || h.npol > kLosslessPoles ||
+           h.npsrc > 0;
+}
+
 // The one inverse-epsilon storage all three components of a descriptor use, or 3 (mixed).
 int inv_mode_of(const SimDev &h) {
     int m = -1;
@@ -2267,8 +2313,7 @@ struct fdtd_sim {
         g.max_npol = host.npol;
         g.any_pml = host.pml != 0;
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
-        g.general = host.cond != 0 || host.metal[0] || host.metal[1] || host.metal[2] ||
-                    host.npol > kLosslessPoles;
+        g.general = needs_general(host);
         g.inv_mode = inv_mode_of(host);
         return g;
     }
@@ -3131,6 +3176,9 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
     if (!s || row < 0 || row > 2 || partner < 0 || partner > 2 || row == partner || !vol)
         return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (s->host.npsrc)
+        return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: an off-diagonal chi1inv with a "
+                                 "polarization (integrated) source is not supported");
     s->graph.reset();  // the off-diagonal path launches other kernels
     int rc = s->pull_desc();
     if (rc) return rc;
@@ -3237,10 +3285,18 @@ int fdtd_add_pole(fdtd_sim *s, const float consts[4], const float *const sigma[3
 int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3], const int32_t cnt[3],
                     const double *amp_re, const double *amp_im, const double *current, int32_t nsteps,
                     double dt) {
-    if (!s || field < 0 || field > 1 || comp < 0 || comp > 2 || !lo || !cnt || !amp_re || !amp_im ||
+    if (!s || field < 0 || field > 3 || comp < 0 || comp > 2 || !lo || !cnt || !amp_re || !amp_im ||
         !current || nsteps < 0)
         return fail(FDTD_EINVAL, "fdtd_add_source: bad argument");
+    if (field == 3)
+        return fail(FDTD_EINVAL, "fdtd_add_source: a magnetic polarization (integrated) source is "
+                                 "not supported: H is kept apart from B only on the layer rows, "
+                                 "and such a source makes H differ from B at its points");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (field == 2 && s->host.offdiag)
+        return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source with an "
+                                 "off-diagonal chi1inv is not supported: the off-diagonal rows "
+                                 "read D at neighbouring cells, where MEEP reads f_minus_p");
     int64_t npts = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] + cnt[a] > s->host.n[a])
@@ -3275,7 +3331,8 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->host.src = (const SrcDev *)p;
     s->host.nsrc = (int)s->srcs.size();
-    for (int f = 0; f < 2; ++f)
+    s->host.npsrc += field == 2;
+    for (int f = 0; f < 3; ++f)
         for (int a = 0; a < 3; ++a) {
             s->host.src_lo[f][a] = 1 << 30;
             s->host.src_hi[f][a] = -1;
@@ -3905,8 +3962,7 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         b->sims.push_back(s);
         descs.push_back(s->host);
         b->group.any_offdiag |= s->host.offdiag != 0;
-        b->group.general |= s->host.cond != 0 || s->host.metal[0] || s->host.metal[1] ||
-                            s->host.metal[2] || s->host.npol > kLosslessPoles;
+        b->group.general |= needs_general(s->host);
         b->group.max_npol = std::max(b->group.max_npol, s->host.npol);
         b->group.any_pml |= s->host.pml != 0;
     }

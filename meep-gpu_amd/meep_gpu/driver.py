@@ -181,12 +181,14 @@ class FdtdDriver:
             self._sources.append((b, time_function))
 
     def add_volume_source(self, component: str, center: Sequence[float], size: Sequence[float],
-                          time_function, amplitude: complex = 1.0) -> List[DepositBox]:
-        """``mp.Source(src, component, center, size, amplitude)``: placed as MEEP places it
-        on this grid and its boundaries (``meep_volume_source``); returns the deposit boxes."""
+                          time_function, amplitude: complex = 1.0,
+                          is_integrated: bool = False) -> List[DepositBox]:
+        """``mp.Source(src, component, center, size, amplitude, is_integrated)``: placed as MEEP
+        places it on this grid and its boundaries (``meep_volume_source``); returns the boxes."""
         self._require_unfrozen("add_volume_source")
         boxes = meep_volume_source(self.grid, component, center, size, amplitude,
-                                   metallic=tuple(not w for w in self.wraps))
+                                   metallic=tuple(not w for w in self.wraps),
+                                   integrated=is_integrated)
         self.add_source(boxes, time_function)
         return boxes
 
@@ -316,10 +318,10 @@ class FdtdDriver:
             # MEEP keeps a chunk's conductivity only where it samples a nonzero sigma
             sim.set_conductivity(name, vol, division.conductive_mask(vol, IYEE[name]))
         for box, fn in self._sources:
-            table = current_table(fn, self.grid.dt, steps, box.magnetic)
+            table = current_table(fn, self.grid.dt, steps, box.magnetic, box.integrated)
             axis = "xyz".index(box.component[1])
             sim.add_source(box.magnetic, axis, box.lo, box.cnt, box.amp_re, box.amp_im, table,
-                           self.grid.dt)
+                           self.grid.dt, box.integrated)
         for mon in self.flux_monitors:
             mon.attach(sim)
         self._place_default_dft_regions(pml)

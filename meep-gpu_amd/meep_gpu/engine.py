@@ -193,8 +193,9 @@ class HipSimulation:
 
     def add_source(self, magnetic: bool, component: int, lo: Sequence[int], cnt: Sequence[int],
                    amp_re: numpy.ndarray, amp_im: numpy.ndarray, current: numpy.ndarray,
-                   dt: float):
-        """A deposit over box [lo, lo+cnt); amplitudes C-order (cnt), current (nsteps, 2) f64."""
+                   dt: float, integrated: bool = False):
+        """A deposit over box [lo, lo+cnt); amplitudes C-order (cnt), current (nsteps, 2) f64.
+        ``integrated``: an electric polarization source, ``current`` its dipole table."""
         re = numpy.ascontiguousarray(numpy.asarray(amp_re, dtype=numpy.float64)
                                      .reshape(tuple(cnt)).transpose(2, 1, 0))
         im = numpy.ascontiguousarray(numpy.asarray(amp_im, dtype=numpy.float64)
@@ -202,7 +203,8 @@ class HipSimulation:
         cur = numpy.ascontiguousarray(current, dtype=numpy.float64).reshape(-1, 2)
         lo_a = numpy.asarray(lo, dtype=numpy.int32)
         cnt_a = numpy.asarray(cnt, dtype=numpy.int32)
-        self._check(self.lib.fdtd_add_source(self.handle, 1 if magnetic else 0, int(component),
+        field = (3 if magnetic else 2) if integrated else (1 if magnetic else 0)
+        self._check(self.lib.fdtd_add_source(self.handle, field, int(component),
                                        iptr(lo_a), iptr(cnt_a), dptr(re), dptr(im), dptr(cur),
                                        int(cur.shape[0]), float(dt)), "fdtd_add_source")
 

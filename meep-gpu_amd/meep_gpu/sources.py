@@ -93,12 +93,21 @@ class ContinuousSource:
         return complex((late.real - early.real) / dt, (late.imag - early.imag) / dt)
 
 
-def current_table(time_function, dt: float, steps: int, magnetic: bool) -> numpy.ndarray:
-    """(steps, 2) float64 current words, one per step, at the source's slot time."""
+def current_table(time_function, dt: float, steps: int, magnetic: bool,
+                  integrated: bool = False) -> numpy.ndarray:
+    """(steps, 2) float64 words, one per step, at the source's slot time: the current, for a
+    deposit (electric at (n+1/2)*dt, magnetic at n*dt), or for an integrated source the dipole
+    the constitutive update subtracts (electric at (n+1)*dt, step.cpp:105-109; magnetic at
+    (n+1/2)*dt)."""
     out = numpy.zeros((steps, 2), dtype=numpy.float64)
-    offset = 0.0 if magnetic else 0.5 * dt
+    if integrated:
+        offset = 0.5 * dt if magnetic else dt
+        value_at = lambda t: time_function.dipole(t)
+    else:
+        offset = 0.0 if magnetic else 0.5 * dt
+        value_at = lambda t: time_function.current(t, dt)
     for n in range(steps):
-        value = complex(time_function.current(n * dt + offset, dt))
+        value = complex(value_at(n * dt + offset))
         out[n, 0] = value.real
         out[n, 1] = value.imag
     return out
@@ -112,6 +121,9 @@ class DepositBox:
     cnt: Tuple[int, int, int]
     amp_re: numpy.ndarray                # float64, shape cnt (C order)
     amp_im: numpy.ndarray
+    #: MEEP's ``is_integrated`` (a polarization source): nothing is deposited into D; the
+    #: constitutive update reads D less ``amp * dipole`` at these cells (update_eh.cpp:66-140)
+    integrated: bool = False
 
     @property
     def magnetic(self) -> bool:
@@ -220,8 +232,8 @@ def _axis_passes(n: int, a: float, iyee: int, where_min: float, where_max: float
 
 
 def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequence[float],
-                       amplitude: complex = 1.0, metallic: Sequence[bool] = (False, False, False)
-                       ) -> List[DepositBox]:
+                       amplitude: complex = 1.0, metallic: Sequence[bool] = (False, False, False),
+                       integrated: bool = False) -> List[DepositBox]:
     """A MEEP ``mp.Source`` with a constant amplitude, placed as MEEP places it on one chunk.
 
     MEEP's ``add_volume_source`` (sources.cpp:456-489, reference sources.py:787-1140): the
@@ -238,8 +250,11 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
     face deposits the two images that meet on a row separately, rounding twice as MEEP does.
     The point weight is ``w_z * (w_y * w_x)`` (vec.hpp:381-383, exact in any grouping of the
     two-factor products) and the amplitude meets it, and the shift phase 1, as complex
-    products.
+    products. ``integrated`` marks the boxes as MEEP's polarization source (electric only).
     """
+    if integrated and component[0] in "HB":
+        raise ValueError("a magnetic integrated source is not supported: the engine keeps H "
+                         "apart from B only on the layer rows")
     iyee = _IYEE[component]
     a = float(grid.resolution)
     inva = 1.0 / a
@@ -279,5 +294,6 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
                             im = w * pass_amp[1]
                             # times the amplitude function's value, one: (re*1 - im*0, re*0 + im*1)
                             boxes.append(DepositBox(component, (x0, y0, z0), w.shape,
-                                                    re * 1.0 - im * 0.0, re * 0.0 + im * 1.0))
+                                                    re * 1.0 - im * 0.0, re * 0.0 + im * 1.0,
+                                                    integrated))
     return boxes

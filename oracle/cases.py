@@ -43,6 +43,9 @@ class Source:
     component: str
     center: Tuple[float, float, float]
     size: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    #: MEEP's ``is_integrated``: the stored word is the dipole, which the constitutive update
+    #: subtracts from D (sources.py:1697-1760) instead of a current deposited into it
+    integrated: bool = False
 
 
 @dataclass(frozen=True)
@@ -91,6 +94,18 @@ CASES: Tuple[Case, ...] = (
     Case("repair_pml_2d", (1.5, 1.5, 0.0), dimensions=2, pml={"x": 2, "y": 2},
          sources=(Source("Ez", (0.0625, 0.0625, 0.0)), Source("Hz", (-0.1875, 0.125, 0.0))),
          repair=True),
+    # points, a finite sheet and an integrated (polarization) source under a 3-cell layer
+    Case("sources_pml_2d", (2.0, 2.0, 0.0), dimensions=2, pml={"x": 3, "y": 3},
+         sources=(Source("Ez", (-0.8125, 0.0625, 0.0)),
+                  Source("Ex", (0.0, 0.25, 0.0), size=(0.75, 0.0, 0.0)),
+                  Source("Hz", (0.1875, -0.1875, 0.0)),
+                  Source("Ey", (0.125, 0.0, 0.0), integrated=True))),
+    # the same kinds into conductive D components: the current scaled by condinv, the
+    # integrated dipole not
+    Case("conductive_source_2d", (1.5, 1.25, 0.0), dimensions=2, d_conductivity=True,
+         sources=(Source("Ez", (0.0625, -0.0625, 0.0)),
+                  Source("Ex", (0.0, 0.125, 0.0), size=(0.5, 0.0, 0.0)),
+                  Source("Ey", (-0.125, 0.0, 0.0), integrated=True))),
 )
 
 #: Cases of this build's own, in the fixture cases' form (seeded the same way, by name) but with
@@ -121,7 +136,8 @@ class CaseState:
     volumes: Dict[str, numpy.ndarray]    # the seeded stored state, by digest name
     stores_E: bool
     pml: bool
-    #: {component, index (C-order flat), amp_re, amp_im, table (stored current per step)}
+    #: {component, index (C-order flat), amp_re, amp_im, integrated, table (the stored word per
+    #: step: the current, or an integrated source's dipole)}
     sources: List[Dict[str, Any]] = field(default_factory=list)
 
 
@@ -179,7 +195,9 @@ def build_state(name: str) -> CaseState:
                     conductivity[side + c] = vol
     faces = layer.pml_faces(case.pml, case.dimensions)
     pml = any(lo > 0 or hi > 0 for lo, hi in faces)
-    stores_E = pml or bool(offdiag) or bool(poles)
+    # an electric integrated source makes E a stored array (driver.py:2718-2733)
+    stores_E = (pml or bool(offdiag) or bool(poles)
+                or any(src.integrated and src.component[0] == "E" for src in case.sources))
     names = ["D" + c for c in "xyz"] + ["B" + c for c in "xyz"]
     if stores_E:
         names += ["E" + c for c in "xyz"]
@@ -199,7 +217,7 @@ def build_state(name: str) -> CaseState:
             shape, invariant, fixture.RESOLUTION, src.component, src.center, src.size,
             wraps=tuple(not m for m in case.metallic))
         placed.append({"component": src.component, "index": cells, "amp_re": amp_re,
-                       "amp_im": amp_im,
+                       "amp_im": amp_im, "integrated": src.integrated,
                        "table": fixture.stored_currents(fixture.seed(name, "current", str(index)))})
     return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, conductivity,
                      volumes, stores_E, pml, placed)
@@ -226,7 +244,8 @@ def oracle_sim(state: CaseState) -> OracleSim:
         table = src["table"]
         sim.add_deposit(PointDeposit(
             src["component"], src["index"], src["amp_re"], src["amp_im"],
-            lambda t, table=table: fixture.stored_current_at(table, t, state.dt)))
+            lambda t, table=table: fixture.stored_current_at(table, t, state.dt),
+            integrated=src["integrated"]))
     return sim
 
 

@@ -108,7 +108,10 @@ class PointDeposit:
     index: numpy.ndarray              # int64 flat indices
     amp_re: numpy.ndarray
     amp_im: numpy.ndarray
-    current: object                   # callable t -> complex
+    current: object                   # callable t -> complex (the dipole, when integrated)
+    #: MEEP's integrated (polarization) source: never deposited; its dipole is subtracted
+    #: from the copy of D (B) the constitutive update reads (reference sources.py:1697-1760)
+    integrated: bool = False
 
     @property
     def magnetic(self) -> bool:
@@ -380,9 +383,11 @@ class OracleSim:
                                       _fp(self.v.get("f_w_" + name)), _fp(pp))
                 pole.P[name], pole.P_prev[name] = pp, p
 
-    def deposit(self, magnetic: bool, time_value: float):
+    def deposit(self, magnetic: bool, time_value: float, integrated: bool = False):
+        """The sources' deposit into D (B); with ``integrated``, the integrated sources' dipole
+        subtracted instead, with no dt and no conductivity (``f_minus_p``)."""
         for dep in self.deposits:
-            if dep.magnetic != magnetic:
+            if dep.magnetic != magnetic or dep.integrated != integrated:
                 continue
             cur = complex(dep.current(time_value))
             if (numpy.copysign(1.0, cur.real) > 0 and cur.real == 0.0
@@ -400,17 +405,53 @@ class OracleSim:
                                  .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                  numpy.ascontiguousarray(dep.amp_im, dtype=numpy.float64)
                                  .ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                 int(idx.size), cur.real, cur.imag, self.dt, _fp(condinv))
+                                 int(idx.size), cur.real, cur.imag,
+                                 1.0 if integrated else self.dt,
+                                 None if integrated else _fp(condinv))
+
+    def f_minus_p_begin(self, magnetic: bool, time_value: float):
+        """MEEP's ``f_minus_p`` in place (reference sources.py:1697-1747, update_eh.cpp:66-140):
+        the words at the integrated sources' cells are saved, the dipole subtracted from them
+        and the wall clear run again; returns what ``f_minus_p_end`` restores, or None."""
+        deps = [d for d in self.deposits if d.integrated and d.magnetic == magnetic]
+        if not deps:
+            return None
+        name = "B" if magnetic else "D"
+        saved = []
+        for c in COMPONENTS:
+            cells = [numpy.asarray(d.index, dtype=numpy.int64) for d in deps
+                     if d.component[1] == c]
+            if cells:
+                flat = numpy.unique(numpy.concatenate(cells))
+                saved.append((name + c, flat, self.v[name + c].reshape(-1)[flat].copy()))
+        self.deposit(magnetic, time_value, integrated=True)
+        self.zero_metal(name)
+        return saved
+
+    def f_minus_p_end(self, saved):
+        if saved is None:
+            return
+        for name, flat, words in saved:
+            self.v[name].reshape(-1)[flat] = words
+        self.zero_metal(name[0])
 
     def step(self):
+        """The reference driver's step (driver.py:3524-3590)."""
         self.step_B()
         self.deposit(True, self.time)
         self.zero_metal("B")
+        held = self.f_minus_p_begin(True, self.time + 0.5 * self.dt)
         self.update_H()
+        if self.pml:  # a stored H: B takes its words back at once
+            self.f_minus_p_end(held)
+            held = None
         self.step_D()
+        self.f_minus_p_end(held)  # without a stored H the D curl read f_minus_p as H
         self.deposit(False, self.time + 0.5 * self.dt)
         self.zero_metal("D")
+        held = self.f_minus_p_begin(False, self.time + self.dt)
         self.update_E()
+        self.f_minus_p_end(held)
         self.update_P()
         self.step_count += 1
         self.time = self.step_count * self.dt

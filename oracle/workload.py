@@ -167,27 +167,33 @@ def region_flux(oracle_planes, weights, normal_weights, area):
 def build(shape, dt, dx, faces, inv: Dict[str, numpy.ndarray], poles: Sequence, deposits,
           planes, invariant=(False, False, False), ln_reflection=None):
     """An ``OracleSim`` plus its planes. ``poles``: (frequency, gamma, drude, sigma dict);
-    ``deposits``: (component, lo, cnt, amp_re, amp_im, current (steps, 2)); ``planes``:
-    (normal, lo, cnt, omegas, scale)."""
+    ``deposits``: (component, lo, cnt, amp_re, amp_im, current[, integrated]) with ``current``
+    a (steps, 2) table whose row n is the word at n*dt or (n+1/2)*dt (the row
+    ``floor(t/dt + 1/4)``), or a callable of the time; ``planes``: (normal, lo, cnt, omegas,
+    scale)."""
     sim = oe.OracleSim(shape, dt, dx, invariant, faces, ln_reflection=ln_reflection)
     sim.set_inverse_epsilon(inv)
     for frequency, gamma, drude, sigma in poles:
         sim.add_pole(frequency, gamma, drude, sigma)
     nx, ny, nz = shape
-    for component, lo, cnt, amp_re, amp_im, current in deposits:
+    for component, lo, cnt, amp_re, amp_im, current, *rest in deposits:
+        integrated = bool(rest[0]) if rest else False
         ii, jj, kk = numpy.meshgrid(*[numpy.arange(l, l + c) for l, c in zip(lo, cnt)],
                                     indexing="ij")
         flat = ((ii * ny + jj) * nz + kk).reshape(-1).astype(numpy.int64)
-        table = numpy.asarray(current, dtype=numpy.float64).reshape(-1, 2)
+        if callable(current):
+            cur = current
+        else:
+            table = numpy.asarray(current, dtype=numpy.float64).reshape(-1, 2)
 
-        def cur(t, table=table, magnetic=component[0] in "HB"):
-            n = int(math.floor(t / dt + 0.25))
-            return complex(table[n, 0], table[n, 1]) if 0 <= n < len(table) else 0j
+            def cur(t, table=table):
+                n = int(math.floor(t / dt + 0.25))
+                return complex(table[n, 0], table[n, 1]) if 0 <= n < len(table) else 0j
 
         sim.add_deposit(oe.PointDeposit(component, flat,
                                         numpy.asarray(amp_re, dtype=numpy.float64).reshape(-1),
                                         numpy.asarray(amp_im, dtype=numpy.float64).reshape(-1),
-                                        cur))
+                                        cur, integrated))
     out_planes = [OraclePlane(*p) for p in planes]
     return sim, out_planes
 

@@ -46,12 +46,14 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
         if metal:
             sim.set_boundary(axis, "metallic")
     for spec, placed in zip(case.sources, state.sources):
-        # the product's placement; the stored current word of step n serves both slots
-        current = numpy.array([[w.real, w.imag] for w in placed["table"]], dtype=numpy.float64)
+        # the product's placement; the stored word of step n serves a current's two slots, and
+        # an integrated source's dipole at (n+1)*dt is the word of step n+1
+        words = placed["table"][1:] if spec.integrated else placed["table"]
+        current = numpy.array([[w.real, w.imag] for w in words], dtype=numpy.float64)
         for box in meep_volume_source(grid, spec.component, spec.center, spec.size,
-                                      metallic=case.metallic):
+                                      metallic=case.metallic, integrated=spec.integrated):
             sim.add_source(box.magnetic, "xyz".index(box.component[1]), box.lo, box.cnt,
-                           box.amp_re, box.amp_im, current, grid.dt)
+                           box.amp_re, box.amp_im, current, grid.dt, box.integrated)
     for n, vol in state.volumes.items():
         sim.upload(n, vol)
     if not state.stores_E:

@@ -14,7 +14,8 @@ from tests import workload_runs as wr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["sphere", "slab", "sphere_box", "sphere_sources"])
+@pytest.mark.parametrize("name", ["sphere", "slab", "sphere_box", "sphere_sources",
+                                  "sphere_integrated"])
 def test_workload_fields_and_spectra_match_the_oracle(name):
     print(f"workload {name}", flush=True)
     wl = wr.small(name)

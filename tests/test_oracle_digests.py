@@ -101,3 +101,46 @@ def test_a_planted_deposit_mistake_moves_the_repair_digest(golden_digests, monke
             original(self, magnetic, time_value + (self.dt if magnetic else 0.0))
         monkeypatch.setattr(engine.OracleSim, "deposit", late)
     assert cases.run_case(name) != golden_digests["case_sha256"][name]
+
+
+@pytest.mark.parametrize("mistake", ["dipole_deposited_as_a_current", "dipole_times_dt",
+                                     "dipole_a_half_step_early"])
+def test_a_planted_integrated_source_mistake_moves_the_digests(golden_digests, monkeypatch,
+                                                               mistake):
+    """sources_pml_2d and conductive_source_2d pin MEEP's polarization source: never deposited
+    into D, its dipole at (n+1)*dt subtracted from the copy of D the constitutive update reads,
+    with no dt (and, in the conductive case, no condinv)."""
+    from oracle import engine
+
+    if mistake == "dipole_deposited_as_a_current":
+        original = engine.OracleSim.deposit
+
+        def both(self, magnetic, time_value, integrated=False):
+            if integrated:
+                return
+            for dep in self.deposits:
+                dep.integrated = False
+            original(self, magnetic, time_value)
+        monkeypatch.setattr(engine.OracleSim, "deposit", both)
+    elif mistake == "dipole_times_dt":
+        original = engine.OracleSim.deposit
+
+        def scaled(self, magnetic, time_value, integrated=False):
+            if not integrated:
+                return original(self, magnetic, time_value)
+            for dep in self.deposits:
+                if dep.integrated:
+                    dep.amp_re = dep.amp_re * self.dt
+            original(self, magnetic, time_value, integrated=True)
+            for dep in self.deposits:
+                if dep.integrated:
+                    dep.amp_re = dep.amp_re / self.dt
+        monkeypatch.setattr(engine.OracleSim, "deposit", scaled)
+    else:
+        original = engine.OracleSim.f_minus_p_begin
+
+        def early(self, magnetic, time_value):
+            return original(self, magnetic, time_value - 0.5 * self.dt)
+        monkeypatch.setattr(engine.OracleSim, "f_minus_p_begin", early)
+    for name in ("sources_pml_2d", "conductive_source_2d"):
+        assert cases.run_case(name) != golden_digests["case_sha256"][name], (mistake, name)

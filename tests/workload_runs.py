@@ -29,6 +29,13 @@ def small(name: str):
         for box in meep_volume_source(grid, "Ex", (-0.4, 0.9, 0.1), (0.0, 1.3, 0.7)):
             wl.sources.append((box, GaussianSource(0.18, 0.1)))
         return wl
+    if name == "sphere_integrated":
+        # the sphere driven by an electric polarization (integrated) source as well
+        wl = small("sphere")
+        for box in meep_volume_source(wl.grid, "Ey", (0.41, 0.23, -0.37), (0.0, 0.0, 0.0),
+                                      integrated=True):
+            wl.sources.append((box, GaussianSource(0.15, 0.1)))
+        return wl
     if name == "sphere_box":
         return workloads.sphere(edge=40, resolution=5.0, pml_cells=6, radius_cells=8.0,
                                 nfreq=6, planes="box8", steps=120)
@@ -40,8 +47,13 @@ def oracle_run(wl, steps: int, reference_precision: bool = True, dft_monitors=()
     monitor in ``dft_monitors`` (after them in the returned list)."""
     grid = wl.grid
     faces = normalize_faces(wl.pml_faces, grid) if wl.pml_faces else ((0, 0),) * 3
-    deposits = [(b.component, b.lo, b.cnt, b.amp_re, b.amp_im,
-                 current_table(fn, grid.dt, steps + 1, b.magnetic)) for b, fn in wl.sources]
+    deposits = []
+    for b, fn in wl.sources:
+        if b.integrated:  # the dipole, evaluated at the oracle's own time
+            table = fn.dipole
+        else:
+            table = current_table(fn, grid.dt, steps + 1, b.magnetic)
+        deposits.append((b.component, b.lo, b.cnt, b.amp_re, b.amp_im, table, b.integrated))
     planes = []
     for p in wl.planes:
         mon = FluxMonitor(p.normal, p.lo, p.cnt, p.frequencies, grid.dx, grid.dt)
