@@ -31,9 +31,9 @@ from typing import Dict, Optional, Sequence, Tuple
 
 import numpy
 
-DEFAULT_R_ASYMPTOTIC = 1e-15
-QUADRATIC_PROFILE_INTEGRAL = 1.0 / 3.0
-CONDUCTIVITY_COMPONENTS: Tuple[str, ...] = ("Dx", "Dy", "Dz", "Bx", "By", "Bz")
+REFLECTION_DEFAULT = 1e-15
+_U_SQUARED_MEAN = 1.0 / 3.0
+GRADED_COMPONENTS: Tuple[str, ...] = ("Dx", "Dy", "Dz", "Bx", "By", "Bz")
 
 # Yee shift (1 = half-integer position) of each component along x, y, z
 _SHIFTS = {"Dx": (1, 0, 0), "Dy": (0, 1, 0), "Dz": (0, 0, 1),
@@ -41,54 +41,56 @@ _SHIFTS = {"Dx": (1, 0, 0), "Dy": (0, 1, 0), "Dz": (0, 0, 1),
 
 
 @dataclass(frozen=True)
-class AbsorberLayer:
+class AbsorbingFace:
     """One absorbing face: ``thickness`` in length units (MEEP's L, not a cell count) on side
     ``"low"`` or ``"high"`` of axis 0/1/2."""
     thickness: float
     axis: int
     side: str
-    R_asymptotic: float = DEFAULT_R_ASYMPTOTIC
+    R_asymptotic: float = REFLECTION_DEFAULT
 
     def __post_init__(self):
         if self.axis not in (0, 1, 2):
-            raise ValueError(f"absorber axis must be 0, 1 or 2, got {self.axis!r}")
+            raise ValueError(f"an absorbing face needs axis 0, 1 or 2 (got {self.axis!r})")
         if self.side not in ("low", "high"):
-            raise ValueError(f"absorber side must be 'low' or 'high', got {self.side!r}")
+            raise ValueError(f"an absorbing face is on side 'low' or 'high' (got {self.side!r})")
         if not math.isfinite(self.thickness) or self.thickness <= 0.0:
-            raise ValueError(f"absorber thickness must be finite and > 0, got {self.thickness!r}")
+            raise ValueError(f"an absorbing face needs a positive finite thickness "
+                             f"(got {self.thickness!r})")
         if not math.isfinite(self.R_asymptotic) or not 0.0 < self.R_asymptotic < 1.0:
-            raise ValueError(f"absorber R_asymptotic must lie in (0, 1), got {self.R_asymptotic!r}")
+            raise ValueError(f"the asymptotic reflection is a fraction strictly between 0 and 1 "
+                             f"(got {self.R_asymptotic!r})")
 
 
-def layers_for(thickness: float, direction: Optional[int] = None, side: Optional[str] = None,
-               R_asymptotic: float = DEFAULT_R_ASYMPTOTIC,
+def faces_for(thickness: float, direction: Optional[int] = None, side: Optional[str] = None,
+               R_asymptotic: float = REFLECTION_DEFAULT,
                resolved: Sequence[bool] = (True, True, True)):
     """``mp.Absorber(thickness, direction, side, R_asymptotic)`` as layers: every resolved axis
     when ``direction`` is None, both sides when ``side`` is None."""
     axes = [a for a in range(3) if resolved[a]] if direction is None else [int(direction)]
     sides = ("low", "high") if side is None else (side,)
-    return [AbsorberLayer(float(thickness), a, s, float(R_asymptotic)) for a in axes for s in sides]
+    return [AbsorbingFace(float(thickness), a, s, float(R_asymptotic)) for a in axes for s in sides]
 
 
-def absorber_profile(thickness: float, resolution: float,
-                     R_asymptotic: float = DEFAULT_R_ASYMPTOTIC) -> Tuple[int, numpy.ndarray]:
+def profile_table(thickness: float, resolution: float,
+                     R_asymptotic: float = REFLECTION_DEFAULT) -> Tuple[int, numpy.ndarray]:
     """(N, prof[0..N]) of one layer: the table sampled at half pixels."""
     if resolution <= 0.0 or not math.isfinite(resolution):
-        raise ValueError(f"resolution must be finite and > 0, got {resolution!r}")
+        raise ValueError(f"the resolution must be positive and finite (got {resolution!r})")
     half_pixel = 0.5 / resolution
     count = int(thickness / half_pixel + 0.5)
     if count < 1:
-        raise ValueError(f"absorber thickness {thickness!r} tabulates as N = {count} at "
-                         f"resolution {resolution!r}; the lookup divides by N")
-    prefac = (-math.log(R_asymptotic)) / (4.0 * thickness * QUADRATIC_PROFILE_INTEGRAL)
+        raise ValueError(f"a face {thickness!r} thick holds {count} half pixels at resolution "
+                         f"{resolution!r}: the profile table needs at least one")
+    prefac = (-math.log(R_asymptotic)) / (4.0 * thickness * _U_SQUARED_MEAN)
     index = numpy.arange(count + 1, dtype=numpy.float64)
     return count, prefac * (index / count) ** 2
 
 
-def sigma_along_axis(coordinates: numpy.ndarray, layer: AbsorberLayer, half_length: float,
+def sigma_on_axis(coordinates: numpy.ndarray, layer: AbsorbingFace, half_length: float,
                      resolution: float) -> numpy.ndarray:
     """One layer's float64 sigma at the given coordinates (zero outside the layer)."""
-    count, profile = absorber_profile(layer.thickness, resolution, layer.R_asymptotic)
+    count, profile = profile_table(layer.thickness, resolution, layer.R_asymptotic)
     L = float(layer.thickness)
     x = numpy.asarray(coordinates, dtype=numpy.float64)
     if layer.side == "high":
@@ -108,7 +110,7 @@ def sigma_along_axis(coordinates: numpy.ndarray, layer: AbsorberLayer, half_leng
     return numpy.where(inside, numpy.where(saturated, profile[count], interp), 0.0)
 
 
-def component_coordinates(n: int, dx: float, shift: int, wraps: bool = True) -> numpy.ndarray:
+def sample_coordinates(n: int, dx: float, shift: int, wraps: bool = True) -> numpy.ndarray:
     """Coordinates of one Yee sub-lattice along an axis of n cells centred on 0: the doubled
     integer -n + 2 i + shift times dx/2, with index 0 of the integer lattice (shift 0) taking
     the seam copy's index n on a wrapping axis."""
@@ -118,7 +120,7 @@ def component_coordinates(n: int, dx: float, shift: int, wraps: bool = True) -> 
     return (-n + 2 * index + shift) * (0.5 * float(dx))
 
 
-def absorber_conductivity(grid, layers: Sequence[AbsorberLayer],
+def graded_sigma(grid, layers: Sequence[AbsorbingFace],
                           wraps: Sequence[bool] = (True, True, True)) -> Dict[str, numpy.ndarray]:
     """The six float64 sigma volumes ("Dx".."Bz", C order (nx, ny, nz)) of a set of layers;
     empty for none. Sigma is a sum over the layers of a function of one axis's coordinate, so
@@ -134,14 +136,14 @@ def absorber_conductivity(grid, layers: Sequence[AbsorberLayer],
         n = grid.shape[layer.axis]
         half_length = 0.5 * float(grid.cell_size[layer.axis])
         for shift in (0, 1):
-            line = sigma_along_axis(component_coordinates(n, grid.dx, shift,
+            line = sigma_on_axis(sample_coordinates(n, grid.dx, shift,
                                                           bool(wraps[layer.axis])),
                                     layer, half_length, resolution)
             key = (layer.axis, shift)
             lines[key] = line if key not in lines else lines[key] + line
     shape = tuple(grid.shape)
     out = {}
-    for comp in CONDUCTIVITY_COMPONENTS:
+    for comp in GRADED_COMPONENTS:
         total = numpy.zeros(shape, dtype=numpy.float64)
         for axis in range(3):
             line = lines.get((axis, _SHIFTS[comp][axis]))

@@ -42,7 +42,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy
 
 from . import _native
-from .chunks import IYEE, ChunkDivision, Frame
+from .chunks import IYEE, ChunkDivision, from_boxes
 from .grid import Grid
 
 CAPSULE_SCHEMA = ("meep_gpu.capsule", 1)
@@ -392,10 +392,13 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         for c, value in state["P_prev"].items():
             volumes[f"P_prev[{k}].{c}"] = _volume(value)
 
-    division = ChunkDivision(
-        tuple(Frame(*_axis_frame(frame)) for frame in layout["frames"]),
-        [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
-        [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
+    try:
+        division = from_boxes(
+            [_axis_frame(frame) for frame in layout["frames"]],
+            [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
+            [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
+    except ValueError as err:
+        raise CapsuleRefusal(f"the lifted chunk division: {err}") from None
     return LiftedState(
         division=division, shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
         courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),

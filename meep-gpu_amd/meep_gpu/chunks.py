@@ -1,22 +1,28 @@
-"""The one-rank chunk division and the per-chunk facts a component steps with.
+"""MEEP's one-rank chunk division as per-axis cuts, and the per-chunk facts it decides.
 
-MEEP divides a one-process simulation into structure chunks along the PML's effort volumes
-(structure.cpp:509-524): per axis a low layer piece, the interior and a high layer piece, each
-layer piece spanning ``int(cells + 1.5)`` cells from its wall, and every product of pieces is a
-chunk. Two facts are per chunk, not per cell:
+MEEP splits a one-process run into structure chunks at the PML's effort volumes
+(structure.cpp:509-524): along each axis a piece at each face that carries a layer, spanning
+``int(thickness + 1.5)`` cells from the wall, and the rest; the chunks are the products of
+the pieces. Everything this engine needs from the division is per axis, so it is held as
+per-axis cut rows rather than as boxes:
 
-* whether the chunk keeps a conductivity array for a component: ``set_conductivity`` samples
-  the chunk's owned samples plus one and deletes the array when all are exactly zero
-  (structure.cpp:883-900), and a chunk without one steps the lossless forms (MEEP
-  step_generic.cpp), which under a PML round differently from the conductive forms with
-  sigma 0;
-* which chunk owns each stored cell (MEEP's ``owns``: ``lo < p <= hi`` in doubled
-  coordinates, vec.cpp:445-462).
+* **Ownership** (MEEP's ``owns``, ``lo < p <= hi`` in doubled coordinates, vec.cpp:445-462).
+  For a piece ``[a, b)`` in cells, a sample on the half-integer lattice (Yee shift 1) at row
+  ``i`` sits at doubled ``2 i + 1`` from the axis origin, so the piece owns rows ``a .. b-1``;
+  one on the integer lattice sits at ``2 i`` and the piece owns rows ``a+1 .. b``: row ``b``
+  of the last piece is row 0's periodic image, and on a metallic axis row 0 is the wall,
+  which no piece owns (it is counted with its neighbour, row 1).
+* **The conductivity keep test** (structure.cpp:883-900): a chunk keeps a component's array
+  when a sample its loop visits is nonzero, and the loop runs over the owned samples plus
+  the one on the other side of the chunk's low face: rows ``a .. b`` inclusive on either
+  lattice (the periodic image past the top, the last row past a wall). A chunk without the
+  array steps the lossless forms (MEEP step_generic.cpp), which under a PML round
+  differently from the conductive forms with sigma 0.
+* **The first row of a piece**, where a centred read of an integer-lattice component takes a
+  word the reading chunk does not own (``readback``).
 
-Restated from the reference's ``ChunkLayout.one_rank`` (chunk_map.py:384-428),
-``_resolve``/``owned`` (:497-523), ``_resolved_rows`` (:613-647), ``owner_volume`` (:649-673),
-``loop_rows`` (:675-718) and ``conductive_chunks``/``conductive_mask`` (:720-755), for the
-periodic and metallic Cartesian grids this engine steps (no folded or cylindrical axis).
+The grid is centred (an axis of ``n`` cells starts at doubled ``-(n - n % 2)``, MEEP's
+``center_origin``); periodic and metallic Cartesian axes only (no folded or cylindrical axis).
 """
 
 from __future__ import annotations
@@ -36,145 +42,149 @@ IYEE = {"Ex": (1, 0, 0), "Ey": (0, 1, 0), "Ez": (0, 0, 1),
 
 
 @dataclass(frozen=True)
-class Frame:
-    """One axis on MEEP's doubled lattice: stored index i at Yee shift s sits at doubled
-    ``origin + 2 i + s``; the window is ``(origin, origin + 2 owned]``."""
-    origin: int
-    stored: int
-    owned: int
-    wraps: bool
-    invariant: bool
+class AxisCuts:
+    """One axis: ``n`` stored rows cut into pieces ``[cuts[k], cuts[k+1])`` of cells, each
+    carrying the axis's layer or not."""
+    n: int
+    cuts: Tuple[int, ...]
+    layer: Tuple[bool, ...]
+    wraps: bool = True
+    invariant: bool = False
 
     @property
-    def window(self) -> Tuple[int, int]:
-        return self.origin, self.origin + 2 * self.owned
+    def pieces(self) -> int:
+        return len(self.layer)
+
+    @property
+    def origin(self) -> int:
+        """The doubled coordinate of the axis's low face (the centred grid's)."""
+        return 0 if self.invariant else -(self.n - self.n % 2)
+
+    @classmethod
+    def flat(cls) -> "AxisCuts":
+        """An axis the dimension does not carry: one row, one piece."""
+        return cls(1, (0, 1), (False,), True, True)
+
+    @classmethod
+    def effort(cls, n: int, low: float, high: float, wraps: bool) -> "AxisCuts":
+        """MEEP's effort-volume cut of one axis with layers ``low``/``high`` cells thick."""
+        lo = min(n, int(low + 1.5)) if low > 0 else 0
+        hi = n - min(n, int(high + 1.5)) if high > 0 else n
+        if lo <= hi:
+            cuts, layer = [0, lo, hi, n], [True, False, True]
+        else:  # the two layer pieces overlap: the overlap is a piece of its own, all layer
+            cuts, layer = [0, hi, lo, n], [True, True, True]
+        keep = [k for k in range(3) if cuts[k] < cuts[k + 1]]
+        return cls(n, tuple([cuts[keep[0]]] + [cuts[k + 1] for k in keep]),
+                   tuple(layer[k] for k in keep), bool(wraps))
+
+    def owner(self, shift: int) -> numpy.ndarray:
+        """Per stored row, the piece owning the sample with Yee shift ``shift`` there."""
+        if self.invariant:
+            return numpy.zeros(self.n, dtype=numpy.int64)
+        rows = numpy.arange(self.n)
+        edges = numpy.asarray(self.cuts[1:-1], dtype=numpy.int64)
+        if shift:
+            return numpy.searchsorted(edges, rows, side="right").astype(numpy.int64)
+        # rows a+1 .. b belong to piece [a, b): row r to the piece whose interval holds r - 1
+        out = numpy.searchsorted(edges, rows - 1, side="right").astype(numpy.int64)
+        out[0] = self.pieces - 1 if self.wraps else 0  # the top's image, or the wall
+        return out
+
+    def visited(self, piece: int) -> numpy.ndarray:
+        """The stored rows one piece's conductivity loop visits: ``a .. b`` inclusive."""
+        if self.invariant:
+            return numpy.zeros(1, dtype=numpy.int64)
+        rows = numpy.arange(self.cuts[piece], self.cuts[piece + 1] + 1, dtype=numpy.int64)
+        return rows % self.n if self.wraps else numpy.minimum(rows, self.n - 1)
+
+    def first_rows(self) -> numpy.ndarray:
+        """Per stored row: is it the first row of its piece (on the half-integer lattice)?"""
+        out = numpy.zeros(self.n, dtype=bool)
+        if not self.invariant:
+            out[numpy.asarray(self.cuts[:-1], dtype=numpy.int64)] = True
+        return out
+
+    def layer_rows(self) -> numpy.ndarray:
+        """Per stored row (half-integer lattice): does its piece carry the layer?"""
+        return numpy.asarray(self.layer, dtype=bool)[self.owner(1)]
 
 
-def grid_frames(grid: Grid, wraps: Sequence[bool] = (True, True, True)) -> Tuple[Frame, ...]:
-    """A centred axis starts at ``-(n - n % 2)`` (MEEP's ``icenter``, grid.py:1095-1130)."""
-    frames = []
-    for axis in range(3):
-        n = grid.shape[axis]
-        if grid.is_invariant(axis):
-            frames.append(Frame(0, 1, 1, True, True))
-        else:
-            frames.append(Frame(-(n - n % 2), n, n, bool(wraps[axis]), False))
-    return tuple(frames)
-
-
-@dataclass
+@dataclass(frozen=True)
 class ChunkDivision:
-    frames: Tuple[Frame, ...]
-    boxes: List[Tuple[Tuple[int, int, int], Tuple[int, int, int]]]
-    #: per chunk and axis: does the chunk carry that axis's PML layer (MEEP's sigsize > 1)
-    flags: List[Tuple[bool, bool, bool]] = None
+    """The product of three axes' pieces; chunk ``(px, py, pz)`` is number
+    ``(px * Py + py) * Pz + pz``."""
+    axes: Tuple[AxisCuts, AxisCuts, AxisCuts]
 
     @classmethod
     def one_rank(cls, grid: Grid, faces, wraps: Sequence[bool] = (True, True, True)):
-        """The {low, interior, high} product per axis; ``faces`` per axis (low, high) cells."""
-        frames = grid_frames(grid, wraps)
-        pieces_by_axis = []
-        for axis, frame in enumerate(frames):
-            if frame.invariant:
-                pieces_by_axis.append([(frame.origin, frame.origin, False)])
-                continue
-            lo_wall, hi_wall = frame.window
-            low, high = faces[axis]
-            low_hi = lo_wall + 2 * min(frame.owned, int(low + 1.5)) if low > 0 else lo_wall
-            high_lo = hi_wall - 2 * min(frame.owned, int(high + 1.5)) if high > 0 else hi_wall
-            if low_hi <= high_lo:
-                pieces = [(lo_wall, low_hi, True), (low_hi, high_lo, False),
-                          (high_lo, hi_wall, True)]
-            else:  # overlapping effort volumes: the overlap is a chunk of its own
-                pieces = [(lo_wall, high_lo, True), (high_lo, low_hi, True),
-                          (low_hi, hi_wall, True)]
-            pieces_by_axis.append([p for p in pieces if p[0] < p[1]])
-        boxes, flags = [], []
-        for px in pieces_by_axis[0]:
-            for py in pieces_by_axis[1]:
-                for pz in pieces_by_axis[2]:
-                    boxes.append(((px[0], py[0], pz[0]), (px[1], py[1], pz[1])))
-                    flags.append((px[2], py[2], pz[2]))
-        return cls(frames, boxes, flags)
+        """``faces`` per axis: (low, high) layer thickness in cells."""
+        return cls(tuple(AxisCuts.flat() if grid.is_invariant(a) else
+                         AxisCuts.effort(grid.shape[a], *faces[a], wraps[a])
+                         for a in range(3)))
 
-    def chunk_owns(self, chunk: numpy.ndarray, coords) -> numpy.ndarray:
-        """Does chunk ``chunk`` (an index per point, -1 none) own each point of doubled
-        ``coords`` (one broadcastable array per axis)? ``lo < p <= hi`` on resolved axes."""
-        index = numpy.asarray(chunk, dtype=numpy.int64)
-        lows = numpy.asarray([lo for lo, _ in self.boxes], dtype=numpy.int64)
-        highs = numpy.asarray([hi for _, hi in self.boxes], dtype=numpy.int64)
-        safe = numpy.where(index >= 0, index, 0)
-        owned = index >= 0
-        for axis, frame in enumerate(self.frames):
-            if frame.invariant:
-                continue
-            p = numpy.asarray(coords[axis], dtype=numpy.int64)
-            owned = owned & (p > lows[safe, axis]) & (p <= highs[safe, axis])
-        return owned
+    @property
+    def count(self) -> int:
+        return self.axes[0].pieces * self.axes[1].pieces * self.axes[2].pieces
 
-    def chunk_of(self, coords) -> numpy.ndarray:
-        """The chunk owning each point of doubled ``coords``, or -1 where none does."""
-        arrays = [numpy.asarray(coords[a], dtype=numpy.int64)
-                  for a in range(3) if not self.frames[a].invariant]
-        shape = numpy.broadcast_shapes(*[a.shape for a in arrays]) if arrays else ()
-        out = numpy.full(shape, -1, dtype=numpy.int64)
-        for k in range(len(self.boxes)):
-            hit = self.chunk_owns(numpy.full(shape, k, dtype=numpy.int64), coords)
-            out = numpy.where(hit, k, out)
-        return out
-
-    def _resolved(self, axis: int, shift: int) -> numpy.ndarray:
-        """Each stored row's doubled coordinate, moved into the window on a wrapping axis; a
-        row MEEP does not own (a wall) takes the nearest owned row's."""
-        frame = self.frames[axis]
-        if frame.invariant:
-            return numpy.zeros(frame.stored, dtype=numpy.int64)
-        points = frame.origin + 2 * numpy.arange(frame.stored, dtype=numpy.int64) + shift
-        lo, hi = frame.window
-        if frame.wraps:
-            points = numpy.where(points <= lo, points + (hi - lo), points)
-            points = numpy.where(points > hi, points - (hi - lo), points)
-        owned = (points > lo) & (points <= hi)
-        owned_at = numpy.flatnonzero(owned)
-        for row in numpy.flatnonzero(~owned):
-            points[row] = points[owned_at[numpy.argmin(numpy.abs(owned_at - row))]]
-        return points
+    def _number(self, px, py, pz):
+        return (px * self.axes[1].pieces + py) * self.axes[2].pieces + pz
 
     def owner_volume(self, iyee: Sequence[int]) -> numpy.ndarray:
-        """Per stored cell of one component lattice, the index of the chunk owning it."""
-        coords = [self._resolved(axis, int(iyee[axis])) for axis in range(3)]
-        grids = numpy.meshgrid(*coords, indexing="ij")
-        owner = numpy.full(grids[0].shape, -1, dtype=numpy.int64)
-        for k, (lo, hi) in enumerate(self.boxes):
-            hit = owner < 0
-            for axis, frame in enumerate(self.frames):
-                if not frame.invariant:
-                    hit &= (grids[axis] > lo[axis]) & (grids[axis] <= hi[axis])
-            owner[hit] = k
-        if numpy.any(owner < 0):
-            raise ValueError("a cell's coordinate is owned by no chunk")
-        return owner
+        """Per stored cell of one component lattice, the number of the chunk owning it."""
+        px, py, pz = (self.axes[a].owner(int(iyee[a])) for a in range(3))
+        return self._number(px[:, None, None], py[None, :, None], pz[None, None, :])
 
-    def loop_rows(self, axis: int, shift: int, chunk: int) -> numpy.ndarray:
-        """The stored rows of the samples MEEP's LOOP_OVER_VOL visits in one chunk: its owned
-        samples plus one (below on the integer lattice, above on the half one); a sample past
-        a wrapping window's top reads its periodic image, past a wall the last stored row."""
-        frame = self.frames[axis]
-        if frame.invariant:
-            return numpy.arange(frame.stored, dtype=numpy.int64)
-        lo, hi = self.boxes[chunk][0][axis], self.boxes[chunk][1][axis]
-        rows = (numpy.arange(lo + shift, hi + shift + 1, 2, dtype=numpy.int64)
-                - frame.origin - shift) // 2
-        if frame.wraps:
-            return rows % frame.owned
-        return numpy.clip(rows, 0, frame.stored - 1)
+    def keeps(self, sigma: numpy.ndarray, iyee: Sequence[int]) -> numpy.ndarray:
+        """Per chunk: does it keep this component's conductivity (a visited sample nonzero)?"""
+        sigma = numpy.asarray(sigma)
+        out = numpy.zeros(self.count, dtype=bool)
+        for px in range(self.axes[0].pieces):
+            rx = self.axes[0].visited(px)
+            for py in range(self.axes[1].pieces):
+                ry = self.axes[1].visited(py)
+                for pz in range(self.axes[2].pieces):
+                    rz = self.axes[2].visited(pz)
+                    out[self._number(px, py, pz)] = bool(
+                        numpy.any(sigma[numpy.ix_(rx, ry, rz)] != 0))
+        return out
 
     def conductive_mask(self, sigma: numpy.ndarray, iyee: Sequence[int]) -> numpy.ndarray:
-        """Per stored cell: does the chunk owning it keep this component's conductivity (some
-        sample it loops over nonzero)?"""
-        sigma = numpy.asarray(sigma)
-        present = numpy.array([
-            bool(numpy.any(sigma[numpy.ix_(*[self.loop_rows(a, int(iyee[a]), k)
-                                              for a in range(3)])] != 0))
-            for k in range(len(self.boxes))], dtype=bool)
-        return present[self.owner_volume(iyee)]
+        """Per stored cell: does the chunk owning it keep this component's conductivity?"""
+        return self.keeps(sigma, iyee)[self.owner_volume(iyee)]
+
+
+def from_boxes(frames: Sequence[Tuple[int, int, int, bool, bool]], boxes, layer_flags
+               ) -> ChunkDivision:
+    """A lifted division: MEEP's chunk boxes ``(lo, hi]`` in doubled coordinates, with per
+    box and axis the layer flag, over axes ``(origin, stored, owned, wraps, invariant)``.
+    Only a product of per-axis pieces over unfolded axes (``owned == stored``) is taken."""
+    axes: List[AxisCuts] = []
+    for a, (origin, stored, owned, wraps, invariant) in enumerate(frames):
+        if invariant:
+            axes.append(AxisCuts.flat())
+            continue
+        if owned != stored:
+            raise ValueError("a folded axis (owned rows differ from stored rows) is not built")
+        if origin != -(stored - stored % 2):
+            raise ValueError("an axis whose origin is not the centred grid's")
+        spans = sorted({(int(lo[a]), int(hi[a])) for lo, hi in boxes})
+        cuts = [(spans[0][0] - origin) // 2]
+        for (lo, hi), nxt in zip(spans, spans[1:] + [None]):
+            if nxt is not None and nxt[0] != hi:
+                raise ValueError("the lifted chunk boxes do not tile the axis")
+            cuts.append((hi - origin) // 2)
+        if cuts[0] != 0 or cuts[-1] != stored:
+            raise ValueError("the lifted chunk boxes do not span the axis")
+        layer = []
+        for lo, hi in spans:
+            flags = {bool(f[a]) for (blo, bhi), f in zip(boxes, layer_flags)
+                     if (int(blo[a]), int(bhi[a])) == (lo, hi)}
+            if len(flags) != 1:
+                raise ValueError("the lifted chunks disagree on a piece's layer flag")
+            layer.append(flags.pop())
+        axes.append(AxisCuts(int(stored), tuple(cuts), tuple(layer), bool(wraps)))
+    division = ChunkDivision(tuple(axes))
+    if division.count != len(boxes):
+        raise ValueError("the lifted chunk boxes are not a product of per-axis pieces")
+    return division

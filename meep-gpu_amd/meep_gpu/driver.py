@@ -19,7 +19,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy
 
-from .absorber import AbsorberLayer, absorber_conductivity, layers_for
+from .absorber import AbsorbingFace, graded_sigma, faces_for
 from .chunks import IYEE, ChunkDivision
 from .dft import DftFieldsMonitor, FluxMonitor, meep_frequency_span
 from .dispersion import Susceptibility
@@ -45,7 +45,7 @@ class FdtdDriver:
         self._inv = {c: numpy.ones(shape, dtype=numpy.float32) for c in _E}
         self._offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
         self._conductivity: Dict[str, numpy.ndarray] = {}
-        self._absorber_layers: List[AbsorberLayer] = []
+        self._absorber_layers: List[AbsorbingFace] = []
         self._boundaries: Dict[int, str] = {}
         self._poles: List[Tuple[Susceptibility, Dict[str, object]]] = []
         self._pml_spec = None
@@ -141,14 +141,14 @@ class FdtdDriver:
         :1596-1625), added to any material conductivity in double and rounded once. Every
         resolved axis when ``direction`` is None, both sides when ``side`` is None."""
         self._require_unfrozen("set_absorber")
-        self._absorber_layers += layers_for(thickness, direction, side, R_asymptotic,
+        self._absorber_layers += faces_for(thickness, direction, side, R_asymptotic,
                                             [not self.grid.is_invariant(a) for a in range(3)])
 
     def conductivity_volumes(self) -> Dict[str, numpy.ndarray]:
         """The float32 sigma each conductive component is stepped with: the material's plus
         the absorber's, summed in double and rounded once."""
         total = {n: v.copy() for n, v in self._conductivity.items()}
-        for name, vol in absorber_conductivity(self.grid, self._absorber_layers,
+        for name, vol in graded_sigma(self.grid, self._absorber_layers,
                                                self.wraps).items():
             total[name] = total[name] + vol if name in total else vol
         return {n: v.astype(numpy.float32) for n, v in total.items() if numpy.any(v)}

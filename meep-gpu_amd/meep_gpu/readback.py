@@ -11,10 +11,11 @@ Cartesian grids without mirror folding or a Bloch phase:
 * ``_up`` (:455-488) and ``_four_words`` (:580-596): the next word up along each axis where the
   component's Yee shift is 0 -- the owned wrap on a periodic axis, +0 past a metallic wall (and
   the corner past one);
-* the B reads under a PML (``HAliasRule.masks`` :491-546, ``h_alias_rule`` :549-570): in a
-  chunk that holds H as B's alias (no layer on the component's own axis) a word the chunk does
-  not own is the owner's H word, and after the first step a chunk with H of its own reads +0
-  there;
+* the B reads under a PML (the rule the reference states as ``HAliasRule.masks`` :491-546
+  and ``h_alias_rule`` :549-570, here from the division's per-axis pieces): in a chunk that
+  holds H as B's alias (no layer on the component's own axis) a word the chunk does not own is
+  the owner's H word, and a chunk with H of its own reads +0 there when the run has taken
+  exactly one step (``first_step``);
 * ``centred_sums`` (:604-635), ``CentredLattice`` (:751-819: the periodic duplicate plane below
   the first on a non-metallic axis), ``total_volume`` (:829-853) and ``get_array`` (:856-972:
   the lattice points inside the volume, MEEP's interpolation weights and float ``+=`` collapse
@@ -76,27 +77,35 @@ def _chunk_words(stored: numpy.ndarray, component: str, invariant, metallic) -> 
 
 def _alias_masks(division: ChunkDivision, component: str, axes: Sequence[int], shape,
                  first_step: bool):
-    """Per word of the centred sum: (take the H word, take a zero) (HAliasRule.masks)."""
-    centre = []
-    for axis in range(3):
+    """Per word of the centred sum: (take the H word, take a zero).
+
+    The chunk that reads a cell centre is the one owning the centre, row by row the piece
+    holding the centre's row on each axis. Along an integer-lattice axis the word above the
+    centre is always that chunk's own; the word below is another chunk's exactly at the first
+    row of the reader's piece (its low face). A word the reader does not own is what MEEP's
+    boundary exchange left in the reader's array there: where the reader holds H as B's alias
+    (no layer on the component's own axis) that is the owner's H word, and where the reader
+    keeps an H of its own it is +0 when the run has taken exactly one step (``first_step``).
+    MEEP array_slice.cpp over the chunk arrays; the reference states the same rule as
+    HAliasRule.masks (readback.py:491-546).
+    """
+    own = "xyz".index(component[1])
+
+    def along(axis, rows):
         view = [1, 1, 1]
         view[axis] = shape[axis]
-        frame = division.frames[axis]
-        centre.append((frame.origin + 2 * numpy.arange(shape[axis], dtype=numpy.int64) + 1)
-                      .reshape(view))
-    reader = division.chunk_of(centre)
-    own_axis = "xyz".index(component[1])
-    flags = numpy.array([not f[own_axis] for f in division.flags], dtype=bool)
-    aliasing = numpy.where(reader >= 0, flags[numpy.where(reader >= 0, reader, 0)], False)
-    offsets = [(-1,), (+1,)] if len(axes) == 1 else [(-1, -1), (+1, -1), (-1, +1), (+1, +1)]
+        return numpy.asarray(rows).reshape(view)
+
+    aliasing = ~along(own, division.axes[own].layer_rows())
+    below = [along(axis, division.axes[axis].first_rows()) for axis in axes]
     take_h, take_zero = [], []
-    for offs in offsets:
-        coords = list(centre)
-        for axis, off in zip(axes, offs):
-            coords[axis] = centre[axis] + off
-        notowned = ~division.chunk_owns(reader, coords) & (reader >= 0)
+    for corner in range(1 << len(axes)):  # bit q set: the word above the centre along axes[q]
+        notowned = numpy.zeros((1, 1, 1), dtype=bool)
+        for q in range(len(axes)):
+            if not corner >> q & 1:
+                notowned = notowned | below[q]
         take_h.append(numpy.broadcast_to(aliasing & notowned, shape))
-        stale = notowned & ~aliasing if first_step else numpy.zeros_like(notowned)
+        stale = notowned & ~aliasing if first_step else numpy.zeros((1, 1, 1), dtype=bool)
         take_zero.append(numpy.broadcast_to(stale, shape))
     return take_h, take_zero
 
@@ -149,8 +158,7 @@ def meep_array(read: Callable[[str], numpy.ndarray], component: str, grid,
     order = [a for a in range(3) if not invariant[a]]
     ivec = {}
     for axis in order:
-        frame = division.frames[axis]
-        ivec[axis] = frame.origin + 2 * numpy.arange(q.shape[axis]) + 1
+        ivec[axis] = division.axes[axis].origin + 2 * numpy.arange(q.shape[axis]) + 1
         if not metallic[axis]:  # the periodic duplicate plane below the first
             q = numpy.concatenate((q[_slab(axis, slice(-1, None))], q), axis=axis)
             ivec[axis] = numpy.concatenate(([ivec[axis][0] - 2], ivec[axis]))
@@ -159,10 +167,10 @@ def meep_array(read: Callable[[str], numpy.ndarray], component: str, grid,
     weights = {}
     collapse = []
     for axis in order:
-        frame = division.frames[axis]
+        cut = division.axes[axis]
         if center is None:
-            low = frame.origin * 0.5 * grid.dx  # total_volume: [little_corner, big_corner - 2]
-            high = (frame.origin + 2 * frame.stored - 2) * 0.5 * grid.dx
+            low = cut.origin * 0.5 * grid.dx  # total_volume: [little_corner, big_corner - 2]
+            high = (cut.origin + 2 * cut.n - 2) * 0.5 * grid.dx
         else:
             low = float(center[axis]) - float(size[axis]) * 0.5
             high = float(center[axis]) + float(size[axis]) * 0.5

@@ -9,23 +9,23 @@ import math
 import numpy
 import pytest
 
-from meep_gpu.absorber import (CONDUCTIVITY_COMPONENTS, AbsorberLayer, absorber_conductivity,
-                               absorber_profile, component_coordinates, layers_for)
+from meep_gpu.absorber import (GRADED_COMPONENTS, AbsorbingFace, graded_sigma,
+                               profile_table, sample_coordinates, faces_for)
 from meep_gpu.grid import Grid
 
 
 def test_profile_table_is_meeps():
-    count, prof = absorber_profile(1.0, 20.0)
+    count, prof = profile_table(1.0, 20.0)
     assert count == 40
     assert prof[-1] == pytest.approx(25.904082296183013, rel=1e-14)
-    count, prof = absorber_profile(0.7, 13.0)
+    count, prof = profile_table(0.7, 13.0)
     assert count == 18
     assert prof[-1] == pytest.approx(37.005831851690026, rel=1e-14)
-    assert absorber_profile(2.0, 10.0)[0] == 40  # half pixels, not pixels
-    a, b = absorber_profile(0.70, 13.0), absorber_profile(0.71, 13.0)
+    assert profile_table(2.0, 10.0)[0] == 40  # half pixels, not pixels
+    a, b = profile_table(0.70, 13.0), profile_table(0.71, 13.0)
     assert a[0] == b[0] == 18 and a[1][-1] != b[1][-1]  # the thickness is not rounded
     with pytest.raises(ValueError):
-        absorber_profile(0.01, 10.0)
+        profile_table(0.01, 10.0)
 
 
 def _scalar_sigma(x, L, half, resolution, side, R=1e-15):
@@ -52,9 +52,9 @@ def _scalar_sigma(x, L, half, resolution, side, R=1e-15):
 
 def test_lookup_matches_a_scalar_reading_at_every_yee_position():
     grid = Grid(resolution=13.0, cell_size=(8.0, 8.0, 0.0), dimensions=2)
-    layers = layers_for(0.7, direction=0, resolved=(True, True, False))
-    vols = absorber_conductivity(grid, layers)
-    assert set(vols) == set(CONDUCTIVITY_COMPONENTS)
+    layers = faces_for(0.7, direction=0, resolved=(True, True, False))
+    vols = graded_sigma(grid, layers)
+    assert set(vols) == set(GRADED_COMPONENTS)
     n = grid.shape[0]
     for comp, shift in (("Dz", 0), ("Dx", 1), ("By", 1), ("Bx", 0)):
         sigma = vols[comp][:, 0, 0]
@@ -70,26 +70,26 @@ def test_the_seam_cell_takes_the_far_faces_sigma():
     """A one-sided high layer: index 0 of the integer lattice is MEEP's copy of index n, at
     +L/2, deep in the layer; the half-integer lattice's index 0 sits at -L/2 + dx/2, outside."""
     grid = Grid(resolution=10.0, cell_size=(6.0, 1.0, 0.0), dimensions=2)
-    vols = absorber_conductivity(grid, [AbsorberLayer(1.0, 0, "high")])
-    count, prof = absorber_profile(1.0, 10.0)
+    vols = graded_sigma(grid, [AbsorbingFace(1.0, 0, "high")])
+    count, prof = profile_table(1.0, 10.0)
     assert vols["Dz"][0, 0, 0] == prof[count]       # shift 0 along x
     assert vols["Dx"][0, 0, 0] == 0.0               # shift 1 along x
-    x = component_coordinates(grid.shape[0], grid.dx, 0)
+    x = sample_coordinates(grid.shape[0], grid.dx, 0)
     assert x[0] == pytest.approx(3.0) and x[1] == pytest.approx(-2.9)
 
 
 def test_corners_add_and_the_interior_is_zero():
     grid = Grid(resolution=10.0, cell_size=(4.0, 3.0, 2.0))
-    layers = layers_for(0.5)
+    layers = faces_for(0.5)
     assert len(layers) == 6
-    vols = absorber_conductivity(grid, layers)
+    vols = graded_sigma(grid, layers)
     for comp, v in vols.items():
         assert numpy.all(v[6:-6, 6:-6, 6:-6] == 0.0), comp
         assert v.dtype == numpy.float64
-    x_only = absorber_conductivity(grid, layers_for(0.5, direction=0))
-    y_only = absorber_conductivity(grid, layers_for(0.5, direction=1))
-    assert numpy.array_equal(absorber_conductivity(grid, layers_for(0.5, direction=0)
-                                                   + layers_for(0.5, direction=1))["Dz"],
+    x_only = graded_sigma(grid, faces_for(0.5, direction=0))
+    y_only = graded_sigma(grid, faces_for(0.5, direction=1))
+    assert numpy.array_equal(graded_sigma(grid, faces_for(0.5, direction=0)
+                                                   + faces_for(0.5, direction=1))["Dz"],
                              x_only["Dz"] + y_only["Dz"])
 
 
@@ -102,8 +102,8 @@ def test_the_driver_adds_a_material_sigma_in_double_and_rounds_once():
     drv.set_conductivity({"Dz": material})
     drv.set_absorber(0.6, direction=0)
     vols = drv.conductivity_volumes()
-    ab = absorber_conductivity(grid, layers_for(0.6, direction=0, resolved=(True, True, False)))
-    assert set(vols) == set(CONDUCTIVITY_COMPONENTS)
+    ab = graded_sigma(grid, faces_for(0.6, direction=0, resolved=(True, True, False)))
+    assert set(vols) == set(GRADED_COMPONENTS)
     assert numpy.array_equal(vols["Dz"], (material + ab["Dz"]).astype(numpy.float32))
     assert vols["Dz"].dtype == numpy.float32
     with pytest.raises(ValueError):

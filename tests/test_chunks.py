@@ -41,10 +41,32 @@ def test_the_block_case_keeps_its_conductivity_in_four_chunks():
     grid = Grid(resolution=8, cell_size=(2.0, 2.0, 0.0), dimensions=2)
     faces = ((2, 2), (2, 2), (0, 0))
     division = ChunkDivision.one_rank(grid, faces, (False, False, True))
-    assert len(division.boxes) == 9
+    assert division.count == 9
     sigma = _block(grid.shape, (8, 8, 0), (16, 16, 1), 3)
     owner = division.owner_volume(IYEE["Dz"])
     mask = division.conductive_mask(sigma, IYEE["Dz"])
     kept = sorted({int(k) for k in numpy.unique(owner[mask])})
     assert len(kept) == 4
     assert not mask[:4, :4].any() and mask[-1, -1].all()
+
+
+def test_product_and_oracle_agree_on_random_divisions():
+    """Random cells, layer thicknesses (overlapping ones included), walls and sparse sigmas:
+    every component lattice keeps its conductivity in the same cells under both rules."""
+    rng = numpy.random.default_rng(20261204)
+    for trial in range(40):
+        dims = 2 if trial % 3 == 0 else 3
+        n = [int(rng.integers(4, 15)) for _ in range(3)]
+        cell = (n[0] / 8, n[1] / 8, n[2] / 8 if dims == 3 else 0.0)
+        grid = Grid(resolution=8, cell_size=cell, dimensions=dims)
+        faces = tuple((float(rng.choice([0, 1, 2, 2.5, 3, 6])), float(rng.choice([0, 1, 2, 3, 6])))
+                      if not grid.is_invariant(a) else (0.0, 0.0) for a in range(3))
+        metallic = tuple(bool(rng.random() < 0.4) and not grid.is_invariant(a) for a in range(3))
+        division = ChunkDivision.one_rank(grid, faces, tuple(not m for m in metallic))
+        sigma = numpy.where(rng.random(grid.shape) < 0.03,
+                            rng.uniform(0.1, 2.0, grid.shape), 0.0).astype(numpy.float32)
+        for name in ("Dx", "Dy", "Dz", "Bx", "By", "Bz"):
+            mine = division.conductive_mask(sigma, IYEE[name])
+            theirs = ochunks.conductive_cells(grid.shape, grid.invariant, metallic, faces, sigma,
+                                              IYEE[name])
+            assert numpy.array_equal(mine, theirs), (trial, name, faces, metallic)

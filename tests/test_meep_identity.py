@@ -14,7 +14,7 @@ import numpy
 import pytest
 
 from meep_gpu import capsule
-from meep_gpu.absorber import AbsorberLayer, absorber_conductivity
+from meep_gpu.absorber import AbsorbingFace, graded_sigma
 from meep_gpu.dispersion import Susceptibility
 from meep_gpu.pml import PML
 from tests import capsule_cases as cc
@@ -124,20 +124,20 @@ def test_a_metallic_axis_grades_row_zero_from_the_low_wall():
 
 def test_the_lifted_absorber_is_this_package_s_on_a_metallic_axis():
     """mp.Absorber on a metallic z: the float64 sigma MEEP's lift stored equals
-    absorber.absorber_conductivity with no seam wrap, bit for bit, and so does its float32."""
+    absorber.graded_sigma with no seam wrap, bit for bit, and so does its float32."""
     state, _, _ = cc.load("ex.absorber-1d")
     assert state.metallic == (False, False, True) and state.dimensions == 1
-    layers = [AbsorberLayer(t, axis, side, r) for t, axis, side, r in state.absorber_layers]
-    mine = absorber_conductivity(state.grid(), layers, wraps=(True, True, False))
+    layers = [AbsorbingFace(t, axis, side, r) for t, axis, side, r in state.absorber_layers]
+    mine = graded_sigma(state.grid(), layers, wraps=(True, True, False))
     assert set(mine) == set(state.absorber_sigma)
     for name, lifted in state.absorber_sigma.items():
         assert numpy.array_equal(mine[name].view(numpy.uint64), lifted.view(numpy.uint64)), name
         assert numpy.array_equal(mine[name].astype(numpy.float32), state.conductivity[name])
     # a one-sided layer tells the seam rule apart: integer row 0 is the far wall's image on a
     # wrapping axis (inside a high layer), the near wall on a metallic one (outside it)
-    high = [AbsorberLayer(1.0, 2, "high")]
-    walls = absorber_conductivity(state.grid(), high, wraps=(True, True, False))["Dx"]
-    periodic = absorber_conductivity(state.grid(), high)["Dx"]
+    high = [AbsorbingFace(1.0, 2, "high")]
+    walls = graded_sigma(state.grid(), high, wraps=(True, True, False))["Dx"]
+    periodic = graded_sigma(state.grid(), high)["Dx"]
     assert walls[0, 0, 0] == 0.0 and periodic[0, 0, 0] > 0.0
     assert numpy.array_equal(walls[0, 0, 1:], periodic[0, 0, 1:])
 
