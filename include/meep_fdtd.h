@@ -178,6 +178,20 @@ int fdtd_add_flux_plane(fdtd_sim *sim, int32_t normal_axis, const int32_t lo[3],
  * invariant axis. */
 int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
 
+/* A mirror fold of one axis (the grid then stores MEEP's halved count N - N/2 + 1, plus one
+ * over periodic boundaries): kind 0 none, 1 over metallic boundaries, 2 over periodic ones;
+ * phase +1 (even mirror) or -1 (odd); far_row (kind 2) the stored row the last slot images,
+ * N - stored + 2. Cell 0 of a component whose Yee shift on the axis is 0 and, for kind 2, the
+ * last slot of a shift-1 one are ghosts: their curl is dropped and, after the sub-step's
+ * sources, each takes its image times MEEP's parity (vector components odd about the plane
+ * normal to their own axis, pseudovectors even; times the phase), H updated from the new B on
+ * the layer rows. Past the top of a kind-1 axis a curl operand reads zero. Replaces the
+ * reference's MIRROR ghost rules, _mask_non_owned_cells and the fill passes
+ * fill_symmetry_bc_D/_B and fill_folded_far_ghosts_D/_B (stepping.py:2004-2160, :2361-2626;
+ * grid.py:404-575). Refused on an invariant axis, with flux or DFT regions and with
+ * polarization (integrated) sources. */
+int fdtd_set_fold(fdtd_sim *sim, int32_t axis, int32_t kind, int32_t phase, int32_t far_row);
+
 /* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of
  * D (field 0, MEEP's D_conductivity) or B (field 1, B_conductivity); null or all zero removes it.
  * Stores condfac = 1 - sigma*dt2 (float32, dt2 = float32(dt)/2) and condinv = 1/(1 + sigma*dt/2)

@@ -52,6 +52,7 @@ constexpr int kLosslessPoles = 4;  // poles the lossless periodic instantiations
                                    // simulation with more runs the general ones
 constexpr int kRing = 128;      // flux samples buffered before one float64 transform pass
 constexpr int kMaxRuns = 2;     // runs of a layer membership line held in the descriptor
+constexpr int kFoldItem = 1024;  // ghost-plane cells per fold_kernel workgroup
 
 thread_local std::string g_error;
 
@@ -127,6 +128,10 @@ struct SimDev {
     float *fcond[2][3];
     int cond;      // any conductivity
     int metal[3];  // metallic (PEC) axes
+    // mirror folds (grid.py:404-575): 0 none, 1 over metallic boundaries (no stored far wall:
+    // a zero ghost past the top), 2 over periodic ones (one stored slot past the window); the
+    // plane's phase (+1 even, -1 odd) and, for 2, the row the far ghost images
+    int fold[3], fphase[3], frow[3];
     // The work split of every fused launch: the interior box [x0,x1) x [j0,j1) x [k0,k1) holds
     // cells on no layer line of either lattice (x0, x1 multiples of 4), where the layer forms
     // are exactly the lossless ones and every H read is B's word; every other run of V cells
@@ -558,6 +563,28 @@ __device__ inline bool on_wall(const SimDev &s, const Run &q, int axis, int m) {
     return s.metal[axis] && coord_of(q, axis, m) == 0;
 }
 
+// The ghosts of a mirror-folded axis (stepping.py:2589-2626): stored cell 0 of a component
+// whose Yee shift on the axis is 0 and, over periodic boundaries, the last slot of a shift-1
+// one. Their curl is dropped and the fold kernel writes them from their images. family 0: D
+// (shift 1 on its own axis), 1: B (shift 0 on its own axis).
+__device__ inline bool ghost_at(const SimDev &s, const Run &q, int family, int comp, int m) {
+    bool g = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!s.fold[a]) continue;
+        const bool shift0 = family ? (a == comp) : (a != comp);
+        const int c = coord_of(q, a, m);
+        g |= shift0 && c == 0;
+        g |= s.fold[a] == 2 && !shift0 && c == s.n[a] - 1;
+    }
+    return g;
+}
+
+// Past the top of a metallic axis, walled or folded, a +1 curl operand reads zero.
+__device__ inline bool zero_top(const SimDev &s, int axis) {
+    return s.metal[axis] || s.fold[axis] == 1;
+}
+
 template <int V>
 __device__ inline void zero_vf(VF<V> &x) {
 #pragma unroll
@@ -567,9 +594,9 @@ __device__ inline void zero_vf(VF<V> &x) {
 // The H pair's +1 operands past the high wall.
 template <int V>
 __device__ inline void metal_up(const SimDev &s, const Run &q, VF<V> (&en)[3][3]) {
-    if (s.metal[1] && q.j == s.n[1] - 1) { zero_vf(en[0][1]); zero_vf(en[2][1]); }
-    if (s.metal[2] && q.k == s.n[2] - 1) { zero_vf(en[0][2]); zero_vf(en[1][2]); }
-    if (s.metal[0]) {
+    if (zero_top(s, 1) && q.j == s.n[1] - 1) { zero_vf(en[0][1]); zero_vf(en[2][1]); }
+    if (zero_top(s, 2) && q.k == s.n[2] - 1) { zero_vf(en[0][2]); zero_vf(en[1][2]); }
+    if (zero_top(s, 0)) {
 #pragma unroll
         for (int m = 0; m < V; ++m)
             if (q.i0 + m == s.n[0] - 1) en[1][0].v[m] = en[2][0].v[m] = 0.0f;
@@ -603,7 +630,7 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
         float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
                              en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
         if constexpr (GEN) {
-            if (on_wall(s, q, C, m)) curl = 0.0f;
+            if (on_wall(s, q, C, m) || ghost_at(s, q, 1, C, m)) curl = 0.0f;
             // a condinv word of 0 marks a cell whose chunk keeps no conductivity array: it
             // steps the lossless forms (structure.cpp:883-900)
             const float civ = s.cf[1][C] ? gp(s.ci[1][C])[q.at + m] : 0.0f;
@@ -641,7 +668,8 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
 #pragma unroll
         for (int m = 0; m < V; ++m) {
             const int cw = coord_of(q, C, m);
-            if (member(s, 0, C, cw)) {
+            // a fold's ghost takes its B from its image first: the fold kernel updates its H
+            if (member(s, 0, C, cw) && !(GEN && ghost_at(s, q, 1, C, m))) {
                 const float4 lw = coeff(s, 0, C, cw, true);
                 float hold = b.v[m], prev = b.v[m];
                 if (ck.hsep) {
@@ -713,7 +741,9 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
         float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
                              hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
         if constexpr (GEN) {
-            if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m)) curl = 0.0f;
+            if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m) ||
+                ghost_at(s, q, 0, C, m))
+                curl = 0.0f;
             const float civ = s.cf[0][C] ? gp(s.ci[0][C])[q.at + m] : 0.0f;
             if (civ != 0.0f) {  // 0: the cell's chunk keeps no conductivity (lossless forms)
                 const int cs = coord_of(q, dsig, m), cu = coord_of(q, dsigu, m);
@@ -1491,6 +1521,92 @@ __global__ void __launch_bounds__(kThreads) erow_kernel(const SimDev *__restrict
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Mirror folds: the ghost planes from their images, after the sub-step's sources
+// (fill_symmetry_bc_* stepping.py:2004-2063, zero_metal_* :3094-3183, then
+// fill_folded_far_ghosts_* :2083-2145). A ghost's image is the stored cell that every ghost
+// coordinate maps to -- row 2 for the near plane, frow for the far one -- which no fold writes
+// and which the pair left final (its sources deposited). The reference's plane-by-plane passes
+// leave the image times the near planes' parities, then +0 if the cell lies on a wall (the
+// wall is cleared after the near fill), then times the far planes' parities (the far fill
+// comes last): the word written here, signs of zero included. An item is (simulation, plane
+// 2*axis + far, first cell, cells) of one folded plane; a cell that a component has as a ghost
+// on two planes is written from the first of them (x before y before z, near before far).
+// Family 1 (B) then updates H at the ghosts on the own-axis layer's rows, as the H pair does
+// everywhere else; family 0 (D) leaves E to the row kernel that follows it.
+// ------------------------------------------------------------------------------------------
+__device__ inline float mirror_parity(int family, int comp, int axis, int phase) {
+    // MEEP's phase_shift (reference fields.py:121-160): a vector (D) is odd about the plane
+    // normal to its own axis and even otherwise, a pseudovector (B) the opposite; both times
+    // the plane's declared phase
+    const bool own = comp == axis;
+    const int sign = family ? (own ? 1 : -1) : (own ? -1 : 1);
+    return (float)(sign * phase);
+}
+
+__global__ void __launch_bounds__(256) fold_kernel(const SimDev *__restrict__ sims,
+                                                   const int4 *__restrict__ items, int family,
+                                                   int adv) {
+    const int4 it = items[blockIdx.x];
+    const SimDev &s = sims[it.x];
+    const int a = it.y >> 1, far = it.y & 1;
+    const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+    const Clock ck = clock_of(s, adv);
+    for (int t = threadIdx.x; t < it.w; t += blockDim.x) {
+        const int p = it.z + t;
+        int c3[3];
+        c3[a] = far ? s.n[a] - 1 : 0;
+        c3[b1] = p % s.n[b1];
+        c3[b2] = p / s.n[b1];
+        for (int comp = 0; comp < 3; ++comp) {
+            int src[3] = {c3[0], c3[1], c3[2]};
+            float near_par = 1.0f, far_par = 1.0f;
+            int first = -1;  // the first plane (2*axis + far) on which the cell is a ghost
+            bool wall = false;  // on a wall of a (walled, unfolded) metallic axis
+            for (int x = 0; x < 3; ++x) {
+                const bool shift0 = family ? (x == comp) : (x != comp);
+                wall |= s.metal[x] && shift0 && c3[x] == 0;
+                if (!s.fold[x]) continue;
+                int code = -1;
+                if (shift0 && c3[x] == 0) {
+                    src[x] = 2;
+                    code = 2 * x;
+                    near_par *= mirror_parity(family, comp, x, s.fphase[x]);
+                } else if (s.fold[x] == 2 && !shift0 && c3[x] == s.n[x] - 1) {
+                    src[x] = s.frow[x];
+                    code = 2 * x + 1;
+                    far_par *= mirror_parity(family, comp, x, s.fphase[x]);
+                }
+                if (code >= 0 && first < 0) first = code;
+            }
+            if (first != it.y) continue;
+            const int at = (c3[2] * s.n[1] + c3[1]) * s.pitch + c3[0];
+            const int sat = (src[2] * s.n[1] + src[1]) * s.pitch + src[0];
+            float *F = family ? s.B[comp] : s.D[comp];
+            float w = 0.0f;
+            if (!wall) {
+                const float v = gp(F)[sat];
+                w = near_par < 0.0f ? -v : v;
+            }
+            if (far_par < 0.0f) w = -w;
+            gp(F)[at] = w;
+            if (family && s.pml && member(s, 0, comp, c3[comp])) {
+                // update_H at the ghost: the H pair's own-axis layer form
+                const float4 lw = coeff(s, 0, comp, c3[comp], true);
+                float hold = w, prev = w;
+                if (ck.hsep) {
+                    hold = gp(s.H[comp])[at];
+                    prev = gp(s.fwH[comp])[at];
+                }
+                gp(s.fwH[comp])[at] = w;
+                const float aa = lw.z * w;
+                const float mm = lw.x * prev;
+                gp(s.H[comp])[at] = hold + (aa - mm);
+            }
+        }
+    }
+}
+
 // E = chi1inv * (D - sum P): the reference's on-demand get_E for runs that do not store E.
 __global__ void prime_e_kernel(SimDev *sims, int count) {
     const SimDev &s = sims[0];
@@ -2008,13 +2124,15 @@ struct Group {
                            // kLosslessPoles poles: the pairs' general instantiation (the others
                            // keep the lossless periodic code)
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
+    const int4 *fold_items = nullptr;  // the folded members' ghost planes (fold_kernel)
+    int nfold = 0;
 };
 
 // Does a descriptor need the pairs' general instantiation: a conductivity, a metallic wall,
 // more than kLosslessPoles poles or a polarization source?
 bool needs_general(const SimDev &h) {
     return h.cond != 0 || h.metal[0] || h.metal[1] || h.metal[2] || h.npol > kLosslessPoles ||
-           h.npsrc > 0;
+           h.npsrc > 0 || h.fold[0] || h.fold[1] || h.fold[2];
 }
 
 // The one inverse-epsilon storage all three components of a descriptor use, or 3 (mixed).
@@ -2103,6 +2221,7 @@ int launch_h(const Group &g, hipStream_t st, int adv = 0) {
         else if (V == 2) launch_hpair<2, false>(g, G, tl, nt, rm, adv, st);
         else launch_hpair<1, false>(g, G, tl, nt, rm, adv, st);
     }
+    if (g.nfold) fold_kernel<<<g.nfold, 256, 0, st>>>(g.dev, g.fold_items, 1, adv);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
@@ -2160,16 +2279,14 @@ int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
         }
     }
     if (g.any_offdiag) {
-        if (V == 4) {
-            dcurl_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-            erow_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        } else if (V == 2) {
-            dcurl_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-            erow_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        } else {
-            dcurl_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-            erow_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
-        }
+        // (a folded member steps this path too: its ghost planes are filled between the two)
+        if (V == 4) dcurl_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else if (V == 2) dcurl_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else dcurl_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        if (g.nfold) fold_kernel<<<g.nfold, 256, 0, st>>>(g.dev, g.fold_items, 0, adv);
+        if (V == 4) erow_kernel<4><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else if (V == 2) erow_kernel<2><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
+        else erow_kernel<1><<<G, kThreads, 0, st>>>(g.dev, tl, nt, rm, adv);
     }
     if (advance_by > 0) advance_kernel<<<(g.nsims + 63) / 64, 64, 0, st>>>(g.dev, g.nsims, advance_by);
     HIP_TRY(hipGetLastError());
@@ -2245,6 +2362,9 @@ struct fdtd_sim {
     int nshellw[3] = {0, 0, 0};
     const int4 *intw[3] = {nullptr, nullptr, nullptr};    // the interior box's items alone
     int nintw[3] = {0, 0, 0};
+    const int4 *foldw = nullptr;  // the ghost planes of the folded axes (fold_kernel items)
+    int nfoldw = 0;
+    std::vector<int4> fold_host;      // the same items, for a batch's concatenation
     std::vector<void *> work_allocs;  // the device lists above (replaced when the lines change)
     bool work_dirty = true;           // the layer lines changed since the split was built
 
@@ -2315,6 +2435,8 @@ struct fdtd_sim {
         g.any_offdiag = g.all_offdiag = host.offdiag != 0;
         g.general = needs_general(host);
         g.inv_mode = inv_mode_of(host);
+        g.fold_items = foldw;
+        g.nfold = nfoldw;
         return g;
     }
     int push_planes() {
@@ -2437,6 +2559,24 @@ int fdtd_sim::prepare_work() {
             if (which) { intw[vi] = (const int4 *)p; nintw[vi] = (int)v.size(); }
             else { shellw[vi] = (const int4 *)p; nshellw[vi] = (int)v.size(); }
         }
+    }
+    // the folded axes' ghost planes, in pieces of up to kFoldItem cells
+    fold_host.clear();
+    for (int a = 0; a < 3; ++a) {
+        if (!h.fold[a]) continue;
+        const int cells = h.n[(a + 1) % 3] * h.n[(a + 2) % 3];
+        for (int far = 0; far < (h.fold[a] == 2 ? 2 : 1); ++far)
+            for (int c0 = 0; c0 < cells; c0 += kFoldItem)
+                fold_host.push_back(make_int4(0, 2 * a + far, c0, std::min(kFoldItem, cells - c0)));
+    }
+    foldw = nullptr;
+    nfoldw = (int)fold_host.size();
+    if (nfoldw) {
+        void *p = nullptr;
+        int rc = work_alloc(&p, sizeof(int4) * fold_host.size());
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(p, fold_host.data(), sizeof(int4) * fold_host.size(), hipMemcpyHostToDevice));
+        foldw = (const int4 *)p;
     }
     work_dirty = false;
     dirty = true;
@@ -2986,9 +3126,52 @@ int fdtd_set_boundary(fdtd_sim *s, int32_t axis, int32_t kind) {
     SimDev &h = s->host;
     if (kind == 1) {
         if (!h.present[axis]) return fail(FDTD_EINVAL, "axis %d is invariant: it has no wall", axis);
+        if (h.fold[axis])
+            return fail(FDTD_EINVAL, "axis %d is folded: its metallic boundary is the fold's kind", axis);
     }
     h.metal[axis] = kind;
     s->graph.reset();
+    s->dirty = true;
+    return FDTD_OK;
+}
+
+// Whether a descriptor takes the two-kernel D path (dcurl, then the row kernel): an
+// off-diagonal chi1inv row, or a mirror fold (whose ghost planes are filled between the two).
+static int split_d(const SimDev &h) {
+    for (int r = 0; r < 3; ++r)
+        for (int p = 0; p < 3; ++p)
+            if (h.u[r][p]) return 1;
+    return (h.fold[0] || h.fold[1] || h.fold[2]) ? 1 : 0;
+}
+
+int fdtd_set_fold(fdtd_sim *s, int32_t axis, int32_t kind, int32_t phase, int32_t far_row) {
+    if (!s || axis < 0 || axis > 2 || kind < 0 || kind > 2)
+        return fail(FDTD_EINVAL, "fdtd_set_fold: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    int rc = s->pull_desc();
+    if (rc) return rc;
+    SimDev &h = s->host;
+    if (kind) {
+        if (!h.present[axis]) return fail(FDTD_EINVAL, "axis %d is invariant: it has no mirror plane", axis);
+        if (phase != 1 && phase != -1)
+            return fail(FDTD_EINVAL, "a mirror's phase is +1 or -1, got %d", (int)phase);
+        if (h.n[axis] < 4) return fail(FDTD_EINVAL, "a folded axis needs 4 stored cells, axis %d has %d", axis, h.n[axis]);
+        if (kind == 2 && (far_row < 3 || far_row >= h.n[axis] - 1))
+            return fail(FDTD_EINVAL, "far ghost row %d is not an interior row of axis %d", (int)far_row, axis);
+        if (h.npsrc)
+            return fail(FDTD_EINVAL, "fdtd_set_fold: a polarization (integrated) source on a "
+                                     "folded grid is not supported");
+        if (!s->planes.empty())
+            return fail(FDTD_EINVAL, "fdtd_set_fold: flux and DFT regions on a folded grid are "
+                                     "not supported (fold before adding none)");
+    }
+    h.fold[axis] = kind;
+    h.fphase[axis] = kind ? phase : 0;
+    h.frow[axis] = kind == 2 ? far_row : 0;
+    if (kind) h.metal[axis] = 0;  // a fold's ghost row is no wall
+    h.offdiag = split_d(h);
+    s->graph.reset();
+    s->work_dirty = true;  // the ghost planes are work items
     s->dirty = true;
     return FDTD_OK;
 }
@@ -3293,6 +3476,9 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
                                  "not supported: H is kept apart from B only on the layer rows, "
                                  "and such a source makes H differ from B at its points");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (field == 2 && (s->host.fold[0] || s->host.fold[1] || s->host.fold[2]))
+        return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source on a "
+                                 "folded grid is not supported");
     if (field == 2 && s->host.offdiag)
         return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source with an "
                                  "off-diagonal chi1inv is not supported: the off-diagonal rows "
@@ -3536,6 +3722,9 @@ int fdtd_synchronize(fdtd_sim *s) {
 static int add_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], int nslot,
                       const int32_t *comps, int yee, int32_t nf, const double *omega,
                       double scale, int32_t dec, int32_t *plane) {
+    if (s->host.fold[0] || s->host.fold[1] || s->host.fold[2])
+        return fail(FDTD_EINVAL, "flux and DFT regions on a folded grid are not supported: the "
+                                 "reference's read-out weighs a folded plane by its images");
     s->graph.reset();  // a captured sequence samples the regions it was captured with
     int cells = 1;
     for (int a = 0; a < 3; ++a) {
@@ -4004,6 +4193,22 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
     }
     if (rc) { fdtd_batch_destroy(b); return rc; }
+    // the folded members' ghost planes, re-numbered to their descriptors
+    std::vector<int4> folds;
+    for (int q = 0; q < count; ++q)
+        for (int4 w : b->sims[q]->fold_host) {
+            w.x = q;
+            folds.push_back(w);
+        }
+    if (!folds.empty()) {
+        int4 *d = nullptr;
+        if (hipMalloc((void **)&d, sizeof(int4) * folds.size()) != hipSuccess) return cleanup(FDTD_ENOMEM, "batch fold items");
+        b->allocs.push_back(d);
+        if (hipMemcpy(d, folds.data(), sizeof(int4) * folds.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(FDTD_EHIP, "batch fold item upload");
+        b->group.fold_items = d;
+        b->group.nfold = (int)folds.size();
+    }
     for (auto *s : b->sims) s->in_batch = true;
     *out = b;
     return FDTD_OK;

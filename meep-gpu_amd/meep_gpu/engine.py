@@ -51,6 +51,10 @@ class HipSimulation:
         self.handle = handle
         self.poles = []
         self.planes = []
+        for axis, fold in enumerate(getattr(grid, "folds", (None, None, None))):
+            if fold is not None:
+                phase, kind, far_row = fold
+                self.set_fold(axis, kind, phase, far_row or 0)
         if self.pml is not None:
             for (half, axis), (kms, sinv, kps) in self.pml.vectors.items():
                 member = self.pml.members[(half, axis)].astype(numpy.uint8)
@@ -109,6 +113,12 @@ class HipSimulation:
             raise ValueError(f"boundary kind must be one of {kinds}, got {kind!r}")
         self._check(self.lib.fdtd_set_boundary(self.handle, int(axis), kinds.index(kind)),
               "fdtd_set_boundary")
+
+    def set_fold(self, axis: int, kind: int, phase: int = 1, far_row: int = 0):
+        """A mirror fold of one axis (``fdtd_set_fold``): kind 1 over metallic boundaries, 2
+        over periodic ones (``far_row``: the row the far ghost images), 0 none."""
+        self._check(self.lib.fdtd_set_fold(self.handle, int(axis), int(kind), int(phase),
+                                           int(far_row)), "fdtd_set_fold")
 
     def set_conductivity(self, component: str, volume: Optional[numpy.ndarray],
                          cells: Optional[numpy.ndarray] = None):

@@ -4,12 +4,18 @@ Mirrors reference ``meep_gpu.grid.Grid`` (grid.py:293) for the configurations th
 steps: a Cartesian cell in 3-D, in MEEP's 2-D (z translationally invariant, one stored cell,
 reference grid.py:66-76) or in its 1-D (x and y invariant, the cell along z; grid.py:1217-1223).
 Every axis wraps unless the driver marks it metallic.
+
+Mirror symmetry (reference grid.py:404-575, MEEP's grid_volume::halve in vec.cpp): a folded
+axis of N full cells stores MEEP's halved ``N - N//2 + 1`` cells with cell 0 one cell below the
+mirror plane, and one more over periodic boundaries -- the slot past the window top that images
+row ``N - stored + 2`` about the second mirror a periodic fold implies (stepping.py:2361-2390).
+Over metallic boundaries the far wall is not stored.
 """
 
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Tuple
+from typing import Optional, Sequence, Tuple
 
 
 def meep_cell_count(length: float, resolution: float) -> int:
@@ -23,7 +29,16 @@ class Grid:
     cell_size: Tuple[float, float, float]
     courant: float = 0.5
     dimensions: int = 3
+    #: mirror planes: ``(axis, phase)`` with axis "X"/"Y"/"Z" (or 0..2) and phase +1 (even,
+    #: MEEP's default) or -1 (odd); a bare axis name is an even mirror
+    symmetry: Sequence = ()
+    #: per axis: a metallic boundary (only a fold's storage depends on it here)
+    metallic: Tuple[bool, bool, bool] = (False, False, False)
     shape: Tuple[int, int, int] = field(init=False)
+    shape_full: Tuple[int, int, int] = field(init=False)
+    #: per axis None or (phase, kind, far row): kind 1 over metallic boundaries, 2 over
+    #: periodic ones, whose last slot images stored row `far row`
+    folds: Tuple[Optional[Tuple[int, int, Optional[int]]], ...] = field(init=False)
     dx: float = field(init=False)
     dt: float = field(init=False)
 
@@ -45,7 +60,35 @@ class Grid:
                 counts.append(meep_cell_count(self.cell_size[axis], self.resolution))
         if min(counts) < 1:
             raise ValueError(f"cell {self.cell_size!r} at resolution {self.resolution} is empty")
+        self.shape_full = tuple(counts)
+        self.metallic = tuple(bool(m) for m in self.metallic)
+        folds = [None, None, None]
+        for entry in self.symmetry:
+            axis, phase = (entry, 1) if isinstance(entry, (str, int)) else entry
+            a = "XYZ".index(axis.upper()) if isinstance(axis, str) else int(axis)
+            if a not in (0, 1, 2):
+                raise ValueError(f"a mirror plane is normal to X, Y or Z, got {axis!r}")
+            if phase not in (1, -1):
+                raise ValueError(f"a mirror's phase is +1 (even) or -1 (odd), got {phase!r}")
+            if folds[a] is not None:
+                raise ValueError(f"two mirror planes normal to {'XYZ'[a]}")
+            if self.is_invariant(a):
+                raise ValueError(f"axis {'xyz'[a]} is invariant: it has no mirror plane")
+            full = counts[a]
+            periodic = not self.metallic[a]
+            stored = full - full // 2 + 1 + (1 if periodic else 0)
+            if stored < 4:
+                raise ValueError(f"a folded axis needs 4 stored cells, {'xyz'[a]} has {stored}")
+            counts[a] = stored
+            folds[a] = (int(phase), 2 if periodic else 1, full - stored + 2 if periodic else None)
+        self.folds = tuple(folds)
         self.shape = tuple(counts)
+
+    def owned(self, axis: int) -> int:
+        """MEEP's window on the axis in cells: the stored count, less the far ghost slot of a
+        periodic fold (reference Grid.owned_cells)."""
+        fold = self.folds[axis]
+        return self.shape[axis] - (1 if fold is not None and fold[1] == 2 else 0)
 
     def is_invariant(self, axis: int) -> bool:
         """MEEP's assignment: z at 2-D, x and y at 1-D."""

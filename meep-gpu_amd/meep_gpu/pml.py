@@ -58,8 +58,15 @@ def normalize_faces(spec, grid: Grid) -> Faces:
         faces = [pair(v) for v in spec]
     else:
         faces = [pair(spec) if not grid.is_invariant(a) else (0, 0) for a in range(3)]
+        # the scalar means every face that is a boundary: a mirror plane is not (pml.py:387-415)
+        for a in range(3):
+            if grid.folds[a] is not None:
+                faces[a] = (0, faces[a][1])
     for a in range(3):
         lo, hi = faces[a]
+        if grid.folds[a] is not None and lo:
+            raise ValueError(f"the {_AXES[a]} axis is mirror-symmetric: its low face is the "
+                             f"mirror plane, which carries no absorber")
         if lo < 0 or hi < 0 or not (math.isfinite(lo) and math.isfinite(hi)):
             raise ValueError(f"PML thickness on {_AXES[a]} must be finite and >= 0")
         if grid.is_invariant(a) and (lo or hi):
@@ -98,12 +105,16 @@ class PML:
         self.wraps = tuple(bool(w) for w in self.wraps)
         if len(self.wraps) != 3:
             raise ValueError("wraps needs one flag per axis")
+        # a folded axis does not wrap; its upper wall is MEEP's window top, 2 * owned half
+        # cells, one cell below the stored end on a periodic fold (pml.py:600-611)
+        self.wraps = tuple(w and self.grid.folds[a] is None for a, w in enumerate(self.wraps))
         self.vectors = {}
         self.members = {}
         for half in (0, 1):
             for axis in range(3):
                 n = self.grid.shape[axis]
-                sig = self._sigma(n, self.faces[axis], half, self.wraps[axis])
+                sig = self._sigma(n, self.faces[axis], half, self.wraps[axis],
+                                  2 * self.grid.owned(axis))
                 one = numpy.ones(n, dtype=numpy.float32)
                 total = one + sig
                 self.vectors[(half, axis)] = ((one - sig).astype(numpy.float32),
@@ -125,14 +136,14 @@ class PML:
         return float(ln_r / (4.0 * depth * (1.0 / (self.order + 1))))
 
     def _sigma(self, n: int, faces: Tuple[float, float], half: int,
-               wraps: bool = True) -> numpy.ndarray:
+               wraps: bool = True, top: Optional[int] = None) -> numpy.ndarray:
         out = numpy.zeros(n, dtype=numpy.float32)
         # half-cell index of every stored row; on a wrapping axis index 0 of the integer
         # lattice is the image of 2n
         pos = 2 * numpy.arange(n) + half
         if wraps:
             pos = (pos - 1) % (2 * n) + 1
-        for wall, cells in ((0, faces[0]), (2 * n, faces[1])):
+        for wall, cells in ((0, faces[0]), (2 * n if top is None else top, faces[1])):
             if cells <= 0:
                 continue
             reach = half_cell_extent(cells)

@@ -22,7 +22,7 @@ from __future__ import annotations
 import hashlib
 import math
 from dataclasses import dataclass, field
-from typing import Any, Callable, Dict, List, Optional, Tuple
+from typing import Any, Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy
 
@@ -67,6 +67,8 @@ class Case:
     #: constitutive update run before the deposit and the deposit points' constitutive words
     #: are recomputed afterwards, which leaves the words of the plain order.
     repair: bool = False
+    #: mirror planes as (axis, phase) (array_path_oracle.py:409, grid.Mirror)
+    symmetry: Tuple[Tuple[str, int], ...] = ()
 
 
 #: The fixture cases this engine steps: real storage, no symmetry; periodic or metallic axes;
@@ -85,6 +87,10 @@ CASES: Tuple[Case, ...] = (
          b_conductivity=True),
     Case("metallic_2d", (1.5, 1.25, 0.0), dimensions=2, pml={"y": 2},
          metallic=(True, False, False)),
+    # an even mirror on x and an odd one on y over periodic boundaries: both axes store the
+    # far ghost slot; a layer on the high faces of both and on z
+    Case("folded_3d", (1.5, 1.25, 1.0), pml={"x": (0, 2), "y": (0, 2), "z": 2},
+         symmetry=(("X", 1), ("Y", -1))),
     # the block conductivity reaches the interior chunk, the x-high and y-high edge chunks
     # and their corner; the other five chunks keep none and step the lossless PML forms
     Case("conductive_chunks_pml_2d", (2.0, 2.0, 0.0), dimensions=2, pml={"x": 2, "y": 2},
@@ -115,6 +121,15 @@ CASES: Tuple[Case, ...] = (
 VARIANTS: Tuple[Case, ...] = (
     Case("offdiag_pml_metallic_3d", (1.5, 1.25, 1.25), pml=2, offdiagonal=True,
          metallic=(True, False, True)),
+    # folds over metallic boundaries (no stored far wall), an odd full count on z (9 -> 6 and
+    # 11 -> 7 stored), walls on x, a layer on every boundary face
+    Case("folded_metallic_3d", (1.5, 1.125, 1.375), pml={"x": 2, "y": (0, 2), "z": (0, 2)},
+         symmetry=(("Y", -1), ("Z", 1)), metallic=(True, True, True)),
+    # an odd periodic fold on x (11 -> 8 stored, far image row 5) and a metallic one on z,
+    # with D and B conductivities, a Lorentz pole and a layer on y and the high faces
+    Case("folded_conductive_pole_3d", (1.375, 1.0, 1.25), pml={"x": (0, 2), "y": 2, "z": (0, 2)},
+         symmetry=(("X", -1), ("Z", 1)), metallic=(False, False, True),
+         d_conductivity=True, b_conductivity=True, susceptibilities=("lorentz",)),
 )
 
 CASE_BY_NAME = {case.name: case for case in CASES + VARIANTS}
@@ -139,13 +154,32 @@ class CaseState:
     #: {component, index (C-order flat), amp_re, amp_im, integrated, table (the stored word per
     #: step: the current, or an integrated source's dipole)}
     sources: List[Dict[str, Any]] = field(default_factory=list)
+    #: per axis None or a mirror fold (phase, kind, far row): kind 1 over metallic boundaries,
+    #: 2 over periodic ones (OracleSim.set_folding)
+    folds: Tuple[Any, Any, Any] = (None, None, None)
+
+
+def folded_axes(case: Case, full: Sequence[int]) -> Tuple[Tuple[int, ...], Tuple[Any, ...]]:
+    """Stored counts and folds of the case's mirror planes (grid.py:568-575): a folded axis
+    keeps MEEP's halved ``N - N//2 + 1`` cells, one more over periodic boundaries (the far
+    ghost slot), whose image row is ``N - stored + 2`` (stepping._far_reflect_rows :2361)."""
+    counts = list(full)
+    folds: List[Any] = [None, None, None]
+    for axis_name, phase in case.symmetry:
+        a = "XYZ".index(axis_name)
+        periodic = not case.metallic[a]
+        stored = full[a] - full[a] // 2 + 1 + (1 if periodic else 0)
+        counts[a] = stored
+        folds[a] = (int(phase), 2 if periodic else 1,
+                    full[a] - stored + 2 if periodic else None)
+    return tuple(counts), tuple(folds)
 
 
 def _grid_shape(case: Case) -> Tuple[Tuple[int, int, int], Tuple[bool, bool, bool]]:
     invariant = tuple(case.dimensions == 2 and axis == 2 for axis in range(3))
     shape = tuple(1 if invariant[a] else layer.cell_count(case.cell[a], fixture.RESOLUTION)
                   for a in range(3))
-    return shape, invariant
+    return folded_axes(case, shape)[0], invariant
 
 
 def _within(case: Case, values: numpy.ndarray) -> numpy.ndarray:
@@ -219,8 +253,10 @@ def build_state(name: str) -> CaseState:
         placed.append({"component": src.component, "index": cells, "amp_re": amp_re,
                        "amp_im": amp_im, "integrated": src.integrated,
                        "table": fixture.stored_currents(fixture.seed(name, "current", str(index)))})
+    full = tuple(1 if invariant[a] else layer.cell_count(case.cell[a], fixture.RESOLUTION)
+                 for a in range(3))
     return CaseState(name, shape, dt, dx, invariant, faces, inv, offdiag, poles, conductivity,
-                     volumes, stores_E, pml, placed)
+                     volumes, stores_E, pml, placed, folded_axes(case, full)[1])
 
 
 def oracle_sim(state: CaseState) -> OracleSim:
@@ -234,7 +270,9 @@ def oracle_sim(state: CaseState) -> OracleSim:
                                         state.faces, vol, _IYEE[n])
              for n, vol in state.conductivity.items()}
     sim.set_conductivity(state.conductivity, cells)
-    sim.set_metallic(case.metallic)
+    # a folded metallic axis stores no wall (stepping._zero_metal :3145-3160): it is a fold
+    sim.set_metallic(tuple(m and f is None for m, f in zip(case.metallic, state.folds)))
+    sim.set_folding(state.folds)
     if state.stores_E:
         sim.enable_field_storage()
     for n, vol in state.volumes.items():

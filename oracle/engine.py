@@ -34,7 +34,8 @@ class _OGrid(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 3), ("present", ctypes.c_int * 3), ("dtdx", ctypes.c_float),
                 ("pml", ctypes.c_int),
                 ("kms", (_FP * 3) * 2), ("sinv", (_FP * 3) * 2), ("kps", (_FP * 3) * 2),
-                ("mem", (_U8P * 3) * 2), ("metal", ctypes.c_int * 3)]
+                ("mem", (_U8P * 3) * 2), ("metal", ctypes.c_int * 3),
+                ("fold", ctypes.c_int * 3)]
 
 
 MAX_POLES = 8  # ORACLE_MAX_POLES of the C restatement
@@ -146,6 +147,9 @@ class OracleSim:
         self.deposits: List[PointDeposit] = []
         self.stores_E = False
         self.metal = (False, False, False)  # metallic (PEC) axes
+        # per axis: None, or a mirror fold (phase, kind, far row): kind 1 over metallic
+        # boundaries, 2 over periodic ones with the far ghost imaging stored row `far row`
+        self.folds: Tuple[Optional[Tuple[int, int, Optional[int]]], ...] = (None, None, None)
         self.condfac: Dict[str, numpy.ndarray] = {}  # "Dx".."Bz": MEEP's conductivity words
         self.condinv: Dict[str, numpy.ndarray] = {}
         # coefficient vectors and membership lines [half][axis]
@@ -242,6 +246,55 @@ class OracleSim:
                 mem = layer.membership_line(n, self.faces[axis], bool(half), wraps=False)
                 self._vec[(half, axis)] = (kms, sinv, kps, mem.astype(numpy.uint8))
 
+    def set_folding(self, folds):
+        """Mirror folds (grid.py:404-575): per axis None or ``(phase, kind, far_row)``, kind 1
+        over metallic boundaries and 2 over periodic ones. A folded axis does not wrap, so its
+        layer is graded from the window top (``2 * owned`` half cells: pml.py:600-611) without
+        a wrap of row 0, and the membership line is ``one_rank_line`` on the stored count
+        (chunk_map.py:180-233)."""
+        self.folds = tuple(folds)
+        order, ln_r = self._layer_args
+        for axis, fold in enumerate(self.folds):
+            if fold is None or self.invariant[axis]:
+                continue
+            n = self.shape[axis]
+            owned = n - 1 if fold[1] == 2 else n
+            for half in (0, 1):
+                kms, sinv, kps = layer.axis_coefficients(n, self.faces[axis], 0.5 * half,
+                                                         self.dt, self.dx, order, ln_r,
+                                                         wraps=False, wall_high=2 * owned)
+                mem = layer.membership_line(n, self.faces[axis], bool(half), wraps=False)
+                self._vec[(half, axis)] = (kms, sinv, kps, mem.astype(numpy.uint8))
+
+    def fill_folds(self, field: str, far: bool):
+        """The ghost planes of the folded axes, after the sub-step's sources.
+
+        Near (``far`` False; fill_symmetry_bc_D/_B, stepping.py:2004-2063): stored cell 0 of
+        every component with Yee shift 0 on a folded axis takes ``parity * cell 2``, the axes
+        in x, y, z order. Far (fill_folded_far_ghosts_*, :2083-2145): on a folded periodic
+        axis the last slot of every shift-1 component takes ``parity * cell far_row``.
+        The parity is MEEP's phase_shift (fields.mirror_parity, fields.py:121): a vector
+        (E, D) is odd about the plane its own axis is normal to and even otherwise, a
+        pseudovector (H, B) the opposite, both times the plane's declared phase."""
+        for axis, fold in enumerate(self.folds):
+            if fold is None or self.invariant[axis]:
+                continue
+            phase, kind, far_row = fold
+            if far and kind != 2:
+                continue
+            for c, name in enumerate(COMPONENTS):
+                shift0 = (axis == c) if field == "B" else (axis != c)
+                if shift0 == far:
+                    continue
+                own = axis == c
+                parity = phase * ((1 if own else -1) if field == "B" else (-1 if own else 1))
+                vol = self.v[field + name]
+                dst = [slice(None)] * 3
+                src = [slice(None)] * 3
+                dst[axis] = -1 if far else 0
+                src[axis] = far_row if far else 2
+                vol[tuple(dst)] = vol[tuple(src)] * numpy.float32(parity)
+
     def set_layer_tables(self, vectors, members):
         """Paste a layer table as lifted (``{(half, axis): (kms, sinv, kps)}`` float32 and
         ``{(half, axis): bool line}``) in place of the one this run graded itself."""
@@ -300,6 +353,7 @@ class OracleSim:
         g.pml = 1 if self.pml else 0
         for a in range(3):
             g.metal[a] = 1 if self.metal[a] else 0
+            g.fold[a] = 0 if self.folds[a] is None else self.folds[a][1]
         for half in (0, 1):
             for a in range(3):
                 kms, sinv, kps, mem = self._vec[(half, a)]
@@ -439,7 +493,9 @@ class OracleSim:
         """The reference driver's step (driver.py:3524-3590)."""
         self.step_B()
         self.deposit(True, self.time)
+        self.fill_folds("B", far=False)
         self.zero_metal("B")
+        self.fill_folds("B", far=True)
         held = self.f_minus_p_begin(True, self.time + 0.5 * self.dt)
         self.update_H()
         if self.pml:  # a stored H: B takes its words back at once
@@ -448,7 +504,9 @@ class OracleSim:
         self.step_D()
         self.f_minus_p_end(held)  # without a stored H the D curl read f_minus_p as H
         self.deposit(False, self.time + 0.5 * self.dt)
+        self.fill_folds("D", far=False)
         self.zero_metal("D")
+        self.fill_folds("D", far=True)
         held = self.f_minus_p_begin(False, self.time + self.dt)
         self.update_E()
         self.f_minus_p_end(held)

@@ -51,6 +51,10 @@ typedef struct {
     const float *kps[2][3];
     const uint8_t *mem[2][3];  /* one-rank chunk membership lines */
     int metal[3];          /* 1: a metallic (PEC) axis -- stepping.py:140-146, :2429-2530 */
+    /* a mirror-folded axis (grid.py:404-575, stepping.py:2004-2160, :2423-2626): 1 over
+     * metallic boundaries (no stored far wall, a zero ghost past the top), 2 over periodic
+     * ones (one stored slot past the window, the far ghost) */
+    int fold[3];
 } ogrid;
 
 static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int axis, int step) {
@@ -68,6 +72,9 @@ static inline float rd(const ogrid *g, const float *G, int64_t i, int64_t j, int
     const int64_t c[3] = {i, j, k};
     const int64_t m = c[axis] + step;
     if (g->metal[axis] && (m < 0 || m >= g->n[axis])) return 0.0f;
+    /* past the top of a folded metallic axis: the unstored wall plane, zero (_shift_up). The
+     * other reads across a folded axis's faces feed a curl that the ghost mask drops. */
+    if (g->fold[axis] == 1 && m >= g->n[axis]) return 0.0f;
     return G[nb(g, i, j, k, axis, step)];
 }
 
@@ -119,6 +126,11 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
                     for (int a = 0; a < 3; ++a) {
                         const int shift0 = family == 0 ? (a == c) : (a != c);
                         if (g->metal[a] && shift0 && ic[a] == 0) curl = 0.0f;
+                        /* the ghosts of a folded axis are not owned: cell 0 of a shift-0
+                         * component and, over periodic boundaries, the last slot of a shift-1
+                         * one (_mask_non_owned_cells :2589-2626); the fills write them */
+                        if (g->fold[a] && shift0 && ic[a] == 0) curl = 0.0f;
+                        if (g->fold[a] == 2 && !shift0 && ic[a] == g->n[a] - 1) curl = 0.0f;
                     }
                     /* a condinv word of 0 (no finite sigma gives it) marks a cell whose chunk
                      * keeps no conductivity array: it steps the lossless forms below

@@ -47,13 +47,18 @@ def prefac(cells: float, dx: float, order: int = 2, ln_reflection: Optional[floa
 
 def graded_sigma(n_cells: int, faces: Tuple[float, float], offset: float, dt: float, dx: float,
                  order: int = 2, ln_reflection: Optional[float] = None,
-                 wraps: bool = True) -> numpy.ndarray:
-    """float32 ``0.5*dt*prefac*u**order`` on the integer (offset 0) or half (0.5) lattice."""
+                 wraps: bool = True, wall_high: Optional[int] = None) -> numpy.ndarray:
+    """float32 ``0.5*dt*prefac*u**order`` on the integer (offset 0) or half (0.5) lattice.
+
+    ``wall_high`` is the upper wall's half-cell index, ``2 * n_cells`` unless given: a folded
+    periodic axis stores one row past its window, and its wall is ``2 * owned`` (pml.py:600-611).
+    """
     sig = numpy.zeros(n_cells, dtype=numpy.float32)
     half_index = 2 * numpy.arange(n_cells) + int(2 * offset)
     if wraps:
         half_index = (half_index - 1) % (2 * n_cells) + 1
-    wall_high = 2 * n_cells
+    if wall_high is None:
+        wall_high = 2 * n_cells
     for wall, n_pml in ((0, faces[0]), (wall_high, faces[1])):
         if n_pml <= 0:
             continue
@@ -69,9 +74,10 @@ def graded_sigma(n_cells: int, faces: Tuple[float, float], offset: float, dt: fl
 
 def axis_coefficients(n_cells: int, faces: Tuple[float, float], offset: float, dt: float,
                       dx: float, order: int = 2, ln_reflection: Optional[float] = None,
-                      wraps: bool = True) -> Tuple[numpy.ndarray, numpy.ndarray, numpy.ndarray]:
+                      wraps: bool = True, wall_high: Optional[int] = None
+                      ) -> Tuple[numpy.ndarray, numpy.ndarray, numpy.ndarray]:
     """``(kms, sinv, kps)`` float32 vectors: ``1-sig``, ``1/(1+sig)``, ``1+sig`` with kappa 1."""
-    sig = graded_sigma(n_cells, faces, offset, dt, dx, order, ln_reflection, wraps)
+    sig = graded_sigma(n_cells, faces, offset, dt, dx, order, ln_reflection, wraps, wall_high)
     kap = numpy.ones(n_cells, dtype=numpy.float32)
     sinv = (f32(1.0) / (kap + sig)).astype(numpy.float32)
     return (kap - sig).astype(numpy.float32), sinv, (kap + sig).astype(numpy.float32)

@@ -24,7 +24,7 @@ from oracle import cases, fixture
 def hip_simulation(state: cases.CaseState) -> HipSimulation:
     case = cases.CASE_BY_NAME[state.name]
     grid = Grid(resolution=fixture.RESOLUTION, cell_size=case.cell, courant=fixture.COURANT,
-                dimensions=case.dimensions)
+                dimensions=case.dimensions, symmetry=case.symmetry, metallic=case.metallic)
     assert grid.shape == state.shape, (grid.shape, state.shape)
     pml = (PML(grid, case.pml, ln_reflection=fixture.PML_LN_REFLECTION,
                wraps=tuple(not m for m in case.metallic)) if case.pml else None)
@@ -43,7 +43,7 @@ def hip_simulation(state: cases.CaseState) -> HipSimulation:
     for comp, vol in state.conductivity.items():
         sim.set_conductivity(comp, vol, division.conductive_mask(vol, IYEE[comp]))
     for axis, metal in enumerate(case.metallic):
-        if metal:
+        if metal and grid.folds[axis] is None:  # a folded metallic axis is the fold's kind
             sim.set_boundary(axis, "metallic")
     for spec, placed in zip(case.sources, state.sources):
         # the product's placement; the stored word of step n serves a current's two slots, and
