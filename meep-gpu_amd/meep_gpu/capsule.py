@@ -21,11 +21,14 @@ What is read, and how:
   conductivity and the sources' point amplitudes go to the engine as the capsule holds them,
   and the PML membership lines come from the lifted chunk division (chunk_map.py:497-570).
 
-What is refused, by name (``CapsuleRefusal``): symmetry folding, cylindrical and complex
-(Bloch) storage, chi2/chi3, off-diagonal chi1inv, monitors, a chunk division no membership
-line describes, more poles than the engine holds, a source without point tables, and a
-currents table whose times are not the driver's (electric sources at ``n*dt + 0.5*dt``,
-magnetic at ``n*dt``; driver.py:3536, :3565).
+What is refused, by name (``CapsuleRefusal``): cylindrical and complex (Bloch) storage,
+chi2/chi3, off-diagonal chi1inv, monitors, a chunk division no membership line describes,
+more poles than the engine holds, a source without point tables, a currents table whose
+times are not the driver's (electric sources at ``n*dt + 0.5*dt``, magnetic at ``n*dt``;
+driver.py:3536, :3565), and on a mirror-folded grid a conductivity and MEEP's ``get_array``
+read-out (which unfolds the images). A folded grid steps with the engine's folds; its digests
+hash every stored cell but the mirror ghosts (meep_identity.py:18-24, ``cell_classes``
+:146-174), which MEEP does not store.
 """
 
 from __future__ import annotations
@@ -217,12 +220,16 @@ class LiftedState:
     h_separated: bool
     absent: Tuple[str, ...]
     division: ChunkDivision = None  # the lifted chunk division (per-chunk conductivity)
+    #: mirror planes ((axis 0..2, phase +-1), ...) and the full (unfolded) cell counts
+    symmetry: Tuple[Tuple[int, int], ...] = ()
+    shape_full: Tuple[int, int, int] = None
 
     def grid(self) -> Grid:
         """The engine grid; refuses when its dx, dt or cell counts are not the lifted words."""
-        size = [0.0 if self.invariant[a] else self.shape[a] / self.resolution for a in range(3)]
+        full = self.shape_full or self.shape
+        size = [0.0 if self.invariant[a] else full[a] / self.resolution for a in range(3)]
         grid = Grid(resolution=self.resolution, cell_size=tuple(size), courant=self.courant,
-                    dimensions=self.dimensions)
+                    dimensions=self.dimensions, symmetry=self.symmetry, metallic=self.metallic)
         if tuple(grid.shape) != tuple(self.shape) or grid.invariant != self.invariant:
             raise CapsuleRefusal(f"grid {grid.shape} {grid.invariant} is not the lifted "
                                  f"{self.shape} {self.invariant}")
@@ -233,11 +240,12 @@ class LiftedState:
 
 
 def _axis_frame(frame: Node) -> Tuple[int, int, int, bool, bool]:
-    for ghost in ("near_ghost_row", "far_ghost_row"):
-        if frame.get(ghost) is not None:
-            raise CapsuleRefusal("a folded axis (symmetry ghost rows) is not built")
     return (int(frame["origin"]), int(frame["stored"]), int(frame["owned"]),
             bool(frame["wraps"]), bool(frame["invariant"]))
+
+
+def _folded_frame(frame: Node) -> bool:
+    return frame.get("near_ghost_row") is not None or frame.get("far_ghost_row") is not None
 
 
 def layout_membership(layout: Node, axis: int, half: bool) -> numpy.ndarray:
@@ -305,8 +313,8 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         raise CapsuleRefusal(f"the capsule holds a {driver.cls}, not an FdtdDriver")
     fields = driver["fields"]
     grid = fields["grid"]
-    if grid["symmetry"] or any(m is not None for m in grid["mirrors_by_axis"]):
-        raise CapsuleRefusal("symmetry folding is not built")
+    symmetry = tuple((axis, int(m["phase"])) for axis, m in enumerate(grid["mirrors_by_axis"])
+                     if m is not None)
     if grid["cylindrical"]:
         raise CapsuleRefusal("cylindrical coordinates are not built")
     if grid["has_bloch"] or fields["force_complex_fields"] or driver.get("force_complex_fields"):
@@ -392,13 +400,17 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         for c, value in state["P_prev"].items():
             volumes[f"P_prev[{k}].{c}"] = _volume(value)
 
-    try:
-        division = from_boxes(
-            [_axis_frame(frame) for frame in layout["frames"]],
-            [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
-            [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
-    except ValueError as err:
-        raise CapsuleRefusal(f"the lifted chunk division: {err}") from None
+    division = None  # the per-chunk facts of a folded division are not built (no
+    if not any(_folded_frame(frame) for frame in layout["frames"]):  # conductivity there)
+        try:
+            division = from_boxes(
+                [_axis_frame(frame) for frame in layout["frames"]],
+                [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
+                [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
+        except ValueError as err:
+            raise CapsuleRefusal(f"the lifted chunk division: {err}") from None
+    elif conductivity:
+        raise CapsuleRefusal("a conductivity on a mirror-folded grid is not built")
     return LiftedState(
         division=division, shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
         courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),
@@ -406,7 +418,8 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         conductivity=conductivity, condfac=side(fields["_condfac"]),
         condinv=side(fields["_condinv"]), absorber_sigma=absorber_sigma, absorber_layers=layers,
         poles=poles, sources=sources, volumes=volumes, step=int(driver["step_count"]),
-        h_separated=bool(fields["h_separated"]), absent=tuple(fields["absent_components"]))
+        h_separated=bool(fields["h_separated"]), absent=tuple(fields.get("absent_components") or ()),
+        symmetry=symmetry, shape_full=tuple(int(n) for n in grid["shape_full"]))
 
 
 def read_capsule(path, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedState:
@@ -503,7 +516,7 @@ class LiftedRun:
     def _configure(self, state: LiftedState, currents):
         sim = self.sim
         for axis in range(3):
-            if state.metallic[axis]:
+            if state.metallic[axis] and sim.grid.folds[axis] is None:  # a fold carries its own
                 sim.set_boundary(axis, "metallic")
         for c, vol in state.inv_eps.items():
             sim.set_inverse_epsilon(c, vol)
@@ -554,6 +567,9 @@ class LiftedRun:
         from .readback import meep_array
 
         state = self.state
+        if state.symmetry:
+            raise CapsuleRefusal("MEEP's get_array on a mirror-folded grid (the images "
+                                 "unfolded) is not built")
         words = {}  # downloaded before the host arithmetic changes mode
 
         def read(name):
@@ -579,10 +595,40 @@ def compare_reader(run, digests: DigestSet, step: int) -> List[str]:
     return [c for c in COMPONENTS if c in want and word_digest(run.meep_array(c)) != want[c]]
 
 
+def ghost_cells(grid: Grid, component: str) -> numpy.ndarray:
+    """The mirror ghosts of one component's stored array (reference meep_identity.py:146-174):
+    on a folded axis stored cell 0 where the component's Yee shift is 0, and over periodic
+    boundaries the last slot where it is 1. MEEP stores neither."""
+    ghost = numpy.zeros(grid.shape, dtype=bool)
+    for axis, fold in enumerate(grid.folds):
+        if fold is None:
+            continue
+        face = [slice(None)] * 3
+        if IYEE[component][axis] == 0:
+            face[axis] = 0
+        elif fold[1] == 2:
+            face[axis] = -1
+        else:
+            continue
+        ghost[tuple(face)] = True
+    return ghost
+
+
+def compared_words(run, component: str) -> numpy.ndarray:
+    """One component's words on the cells MEEP stores: every stored cell but the mirror
+    ghosts, in C order (x, y, z)."""
+    words = run.words(component)
+    grid = run.grid
+    if not any(f is not None for f in grid.folds):
+        return words
+    return words[~ghost_cells(grid, component)]
+
+
 def compare_step(run, digests: DigestSet, step: int) -> List[str]:
     """The components whose words differ from MEEP's at one compared step."""
     want = digests.stored[step]
-    return [c for c in COMPONENTS if c in want and word_digest(run.words(c)) != want[c]]
+    return [c for c in COMPONENTS
+            if c in want and word_digest(compared_words(run, c)) != want[c]]
 
 
 def replay(capsule_dir, max_poles: int = _native.FDTD_MAX_POLES) -> Dict[int, List[str]]:
@@ -593,8 +639,6 @@ def replay(capsule_dir, max_poles: int = _native.FDTD_MAX_POLES) -> Dict[int, Li
     folder = Path(capsule_dir)
     state = read_capsule(folder / "capsule.capsule", max_poles=max_poles)
     digests = read_digests(folder / "digests.json")
-    if any(digests.ghost_cells.values()):
-        raise CapsuleRefusal("the digests compare a folded grid's cells")
     currents = read_currents(folder / "currents.json", state, digests.n_steps)
     if digests.mode not in ("keep", "flush"):
         raise CapsuleRefusal(f"digests minted under subnormal mode {digests.mode!r}")

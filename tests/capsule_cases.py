@@ -20,9 +20,12 @@ ROOT = os.path.join(HERE, "golden", "meep_identity")
 FIXTURES = os.path.join(ROOT, "keep")
 #: The reference's two subnormal policies: MEEP's words with subnormals kept, and flushed.
 MODES = ("keep", "flush")
-#: The keep-set cases the engine steps (the others need symmetry folding, cylindrical or
-#: complex storage, and are refused by name).
-CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d")
+#: The cases the engine steps (the holey waveguide needs complex storage and the ring
+#: cylindrical storage; both are refused by name).
+CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d", "c", "c_prime")
+#: Those on a mirror-folded grid (Y and Z odd mirrors over metallic boundaries, an Hx point
+#: source on both planes): their get_array read-out is not built.
+FOLDED = ("c", "c_prime")
 
 
 def case_dir(case: str, mode: str = "keep") -> str:
@@ -49,7 +52,9 @@ class OracleRun:
         faces = state.pml.faces if state.pml is not None else ((0, 0), (0, 0), (0, 0))
         sim = OracleSim(state.shape, state.dt, state.dx, invariant=state.invariant,
                         pml_faces=faces)
-        sim.set_metallic(state.metallic)
+        folds = state.grid().folds
+        sim.set_metallic(tuple(m and f is None for m, f in zip(state.metallic, folds)))
+        sim.set_folding(folds)
         if state.pml is not None:
             sim.set_layer_tables(state.pml.vectors, state.pml.members)
         sim.set_inverse_epsilon(dict(state.inv_eps))
@@ -83,6 +88,10 @@ class OracleRun:
     @property
     def step_count(self) -> int:
         return self.sim.step_count
+
+    @property
+    def grid(self):
+        return self.state.grid()
 
     def advance(self, steps: int):
         from oracle.engine import flushing

@@ -7,7 +7,10 @@ The cases: ``a`` and ``b``, 2-D 200x200 with a 20-cell UPML inside metallic wall
 default boundary) and a Gaussian Ez point source, ``b`` with per-component inverse epsilon;
 ``ex.material-dispersion``, one periodic cell with two Lorentz poles; ``ex.absorber-1d``,
 400 cells along z between metallic walls, an mp.Absorber (conductivity on D and B) and five
-Drude/Lorentz poles, which runs the general instantiation.
+Drude/Lorentz poles, which runs the general instantiation; ``c`` and ``c_prime``, 3-D
+39x18x19 stored cells of a 39x33x35 cell folded by odd Y and Z mirrors over metallic
+boundaries, a PML on both x faces and the high y and z faces, an Hx point source on both
+mirror planes (their get_array read-out, which unfolds the images, is not built).
 """
 
 import pytest
@@ -33,7 +36,8 @@ def test_the_engine_steps_meep_s_words(case, mode):
             run.advance(step - run.step_count)
             got[step] = capsule.compare_step(run, digests, step)
             # MEEP's sim.get_array words of the GPU's words (GpuRunResult.get_array)
-            read[step] = capsule.compare_reader(run, digests, step)
+            if case not in cc.FOLDED:
+                read[step] = capsule.compare_reader(run, digests, step)
     finally:
         run.close()
     assert set(got) == set(digests.steps)

@@ -51,7 +51,6 @@ def _decoded(case):
 
 
 @pytest.mark.parametrize("change, reason", [
-    (lambda d: d["fields"]["grid"].attrs.__setitem__("symmetry", ("mirror",)), "symmetry"),
     (lambda d: d["fields"]["grid"].attrs.__setitem__("cylindrical", True), "cylindrical"),
     (lambda d: d["fields"].attrs.__setitem__("force_complex_fields", True), "complex"),
     (lambda d: d.attrs.__setitem__("_chi3", 1.0), "chi2/chi3"),
@@ -212,8 +211,48 @@ def test_a_planted_pole_word_is_caught():
     assert "Ez" in got[40]
 
 
+def test_a_folded_capsule_s_conductivity_and_read_out_are_refused_by_name():
+    driver = capsule.read_capsule_document(os.path.join(cc.case_dir("c"), "capsule.capsule"))[1]
+    sigma = numpy.ones(tuple(driver["fields"]["grid"]["shape"]), dtype=numpy.float32)
+    driver["fields"].attrs["_conductivity"] = {"Dx": sigma}
+    with pytest.raises(capsule.CapsuleRefusal, match="mirror-folded"):
+        capsule.lift_state(driver)
+    state, digests, currents = cc.load("c")
+    run = cc.OracleRun(state, currents)
+    with pytest.raises(capsule.CapsuleRefusal, match="mirror-folded"):
+        capsule.LiftedRun.meep_array(run, "Ez")
+
+
+@pytest.mark.parametrize("planted", ["even_parity", "no_fill", "no_zero_top"])
+def test_the_folded_replay_catches_planted_mistakes(planted, monkeypatch):
+    """MEEP's words of ``c`` (odd Y and Z mirrors over metallic boundaries) catch a fold taken
+    even, the ghost planes left unfilled, and the zero past a folded axis's top read as a
+    periodic wrap (the oracle's C grid told the axes are not folded)."""
+    from oracle import engine
+
+    state, digests, currents = cc.load("c")
+    run = cc.OracleRun(state, currents)
+    sim = run.sim
+    if planted == "even_parity":
+        sim.folds = tuple(None if f is None else (1, f[1], f[2]) for f in sim.folds)
+    elif planted == "no_fill":
+        monkeypatch.setattr(engine.OracleSim, "fill_folds", lambda self, field, far: None)
+    else:
+        real = engine.OracleSim._grid
+
+        def grid(self):
+            g = real(self)
+            for a in range(3):
+                g.fold[a] = 0
+            return g
+
+        monkeypatch.setattr(engine.OracleSim, "_grid", grid)
+    got = cc.replay_digests(run, digests, steps=[20])
+    assert got[20]
+
+
 @pytest.mark.parametrize("mode", cc.MODES)
-@pytest.mark.parametrize("case", cc.CASES)
+@pytest.mark.parametrize("case", [c for c in cc.CASES if c not in cc.FOLDED])
 def test_get_array_is_meep_s_on_the_oracle_s_words(case, mode):
     """readback.meep_array over the oracle's words gives MEEP's own sim.get_array words (the
     identity sets' reader digests) at every compared step: the centred lattice, MEEP's
