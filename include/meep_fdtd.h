@@ -122,6 +122,13 @@ int fdtd_add_source(fdtd_sim *sim, int32_t field, int32_t component, const int32
                     const int32_t cnt[3], const double *amp_re, const double *amp_im,
                     const double *current, int32_t nsteps, double dt);
 
+/* Replaces source `index`'s current table (the order fdtd_add_source added them) with `nsteps`
+ * (re, im) float64 pairs from step 0: a run longer than the table first planned extends it
+ * here instead of re-creating the simulation. The reference evaluates src_time::current on the
+ * host at every step (sources.py:1844-1936, inject :2168-2173); this engine holds the values
+ * for the steps ahead. Refused while the simulation is stepped by a batch. */
+int fdtd_set_source_currents(fdtd_sim *sim, int32_t index, const double *current, int32_t nsteps);
+
 /* Upload / download one volume by name: "Dx".."Bz", "Ex".., "Hx".., "fu_Bx".., "fu_Dx"..,
  * "f_w_Ex".., "f_w_Hx".., "P[k].Ex", "P_prev[k].Ex" (the reference's digest names,
  * array_path_oracle.py:709). count = nx*ny*nz floats, x fastest. "Hx".. exist only with a

@@ -3532,6 +3532,30 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
     return FDTD_OK;
 }
 
+int fdtd_set_source_currents(fdtd_sim *s, int32_t index, const double *current, int32_t nsteps) {
+    if (!s || !current || nsteps < 0) return fail(FDTD_EINVAL, "fdtd_set_source_currents: bad argument");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    if (index < 0 || index >= (int)s->srcs.size())
+        return fail(FDTD_EINVAL, "source %d of %d", (int)index, (int)s->srcs.size());
+    int rc = s->pull_desc();
+    if (rc) return rc;
+    // a new table (the old one stays allocated until the simulation is destroyed: a replay
+    // queued before this call may still read it), then the descriptors that point at it
+    void *p;
+    rc = s->alloc(&p, sizeof(double) * 2 * (size_t)(nsteps > 0 ? nsteps : 1));
+    if (rc) return rc;
+    if (nsteps) HIP_TRY(hipMemcpyAsync(p, current, sizeof(double) * 2 * nsteps, hipMemcpyHostToDevice, s->stream));
+    s->srcs[index].cur = (const double *)p;
+    s->srcs[index].ncur = nsteps;
+    rc = s->alloc(&p, sizeof(SrcDev) * s->srcs.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(p, s->srcs.data(), sizeof(SrcDev) * s->srcs.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->host.src = (const SrcDev *)p;
+    s->dirty = true;
+    return FDTD_OK;
+}
+
 static float *volume_by_name(fdtd_sim *s, const char *name) {
     SimDev &h = s->host;
     auto comp = [](char ch) { return ch == 'x' ? 0 : ch == 'y' ? 1 : ch == 'z' ? 2 : -1; };

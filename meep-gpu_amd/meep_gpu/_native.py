@@ -69,6 +69,8 @@ SIGNATURES = {
                                                   ctypes.c_size_t]),
     "fdtd_flux_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _DP, ctypes.c_size_t]),
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "fdtd_set_source_currents": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     "fdtd_set_fold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_conductivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,

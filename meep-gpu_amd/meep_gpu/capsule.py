@@ -546,6 +546,18 @@ class LiftedRun:
     def grid(self) -> Grid:
         return self.sim.grid
 
+    def engine_has(self, name: str) -> bool:
+        """Whether the engine holds a stored volume of this name: D, B and E always; H and
+        the layer auxiliaries under an absorber; f_cond where that component has a
+        conductivity under one."""
+        if name[:-1] in ("B", "D", "E"):
+            return True
+        if self.state.pml is None:
+            return False
+        if name.startswith("f_cond_"):
+            return name[len("f_cond_"):] in self.state.conductivity
+        return name[:-1] in ("H", "fu_B", "fu_D", "f_w_E", "f_w_H")
+
     @property
     def step_count(self) -> int:
         return self.sim.step_count

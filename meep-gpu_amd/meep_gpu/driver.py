@@ -342,12 +342,23 @@ class FdtdDriver:
     CONSULTS = ("step_B", "fill_B", "fill_folded_far_ghosts_B", "update_H", "step_D", "fill_D",
                 "fill_folded_far_ghosts_D", "update_E", "update_P")
 
+    def _extend_sources(self, steps: int):
+        """Source tables long enough for ``steps`` steps from step 0: a run past the planned
+        length extends them on the device (``fdtd_set_source_currents``), doubling the plan,
+        where the reference evaluates each current as it goes (sources.py:1844-1936)."""
+        if steps <= self._planned:
+            return
+        planned = max(int(steps), 2 * self._planned)
+        for q, (box, fn) in enumerate(self._sources):
+            self.sim.set_source_currents(q, current_table(fn, self.grid.dt, planned, box.magnetic,
+                                                          box.integrated))
+        self._planned = planned
+
     def step(self):
         """One time step through the dispatch seam: each consult must be answered by the plan
         (there is no array path to fall back on), the fused launches at step_B and step_D."""
         sim = self.freeze()
-        if sim.step_count >= self._planned:
-            raise RuntimeError("the run is past its planned length; set max_steps higher")
+        self._extend_sources(sim.step_count + 1)
         fast = self._fast_path
         for slot in self.CONSULTS:
             if not fast.dispatch(slot, sim):
@@ -359,8 +370,7 @@ class FdtdDriver:
         if until is not None:
             steps = int(math.ceil(until / self.grid.dt - 1e-9))
         sim = self.freeze(max(int(steps) + self.step_count, self.max_steps))
-        if sim.step_count + steps > self._planned:
-            raise RuntimeError("the run is longer than the source tables planned at freeze")
+        self._extend_sources(sim.step_count + int(steps))
         sim.run(int(steps))
 
     def synchronize(self):

@@ -218,6 +218,14 @@ class HipSimulation:
                                        iptr(lo_a), iptr(cnt_a), dptr(re), dptr(im), dptr(cur),
                                        int(cur.shape[0]), float(dt)), "fdtd_add_source")
 
+    def set_source_currents(self, index: int, current: numpy.ndarray):
+        """Replace source ``index``'s current table (``fdtd_set_source_currents``): (nsteps, 2)
+        float64 from step 0, e.g. a longer one for a run past the planned length."""
+        cur = numpy.ascontiguousarray(current, dtype=numpy.float64).reshape(-1, 2)
+        self._check(self.lib.fdtd_set_source_currents(self.handle, int(index), dptr(cur),
+                                                      int(cur.shape[0])),
+                    "fdtd_set_source_currents")
+
     # --- state -------------------------------------------------------------------------------
     def upload(self, name: str, array: numpy.ndarray):
         dev = to_device_layout(array)

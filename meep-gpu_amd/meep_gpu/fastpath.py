@@ -24,9 +24,9 @@ into one launch, so the plan has two fused units:
   update_P, in one launch, then the counter advance and the flux sampling; ``fill_D`` (and its
   far pass), ``update_E`` and ``update_P`` are its trailing consults.
 
-The symmetry fills are not carried by the engine (no mirror folding): a plan is refused by
-name for a configuration that needs them, so for every admitted configuration the fills are
-the no-op they are on the array path and their consults answer with the pair.
+The symmetry fills of a mirror-folded grid run inside the pairs' launches (the engine's fold
+kernel, after the sub-step's sources and before update_H / update_E, in the reference's
+order), so the fill consults and their far passes answer with their pair.
 """
 
 from __future__ import annotations

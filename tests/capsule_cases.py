@@ -141,3 +141,48 @@ def replay_reader(run, digests: capsule.DigestSet, steps=None) -> Dict[int, List
         run.advance(step - run.step_count)
         out[step] = capsule.compare_reader(run, digests, step)
     return out
+
+
+def live_driver(case: str, mode: str = "keep"):
+    """The capsule's object graph rebuilt as live Python objects (classes named by the
+    graph's class paths, attributes as decoded), each source given ``current(t, dt)`` from the
+    recorded currents table: a stand-in for a live reference driver at the seam."""
+    import json
+
+    folder = case_dir(case, mode)
+    root = capsule.read_capsule_document(os.path.join(folder, "capsule.capsule"))[1]
+    memo = {}
+    classes = {}
+
+    def build(value):
+        if isinstance(value, capsule.Node):
+            if id(value) in memo:
+                return memo[id(value)]
+            module, qual = value.cls.split(":")
+            kind = classes.get(value.cls)
+            if kind is None:
+                kind = type(qual.split(".")[-1], (), {})
+                kind.__module__, kind.__qualname__ = module, qual
+                classes[value.cls] = kind
+            obj = kind.__new__(kind)
+            memo[id(value)] = obj
+            for k, v in value.attrs.items():
+                setattr(obj, k, build(v))
+            return obj
+        if isinstance(value, dict):
+            return {k: build(v) for k, v in value.items()}
+        if isinstance(value, (list, tuple)) and not hasattr(value, "_fields"):
+            return type(value)(build(v) for v in value)
+        return value
+
+    driver = build(root)
+    with open(os.path.join(folder, "currents.json")) as fh:
+        doc = json.load(fh)
+    for src, rec in zip(driver._sources, doc["sources"]):
+        table = {}
+        for key, value in rec["calls"]["current"]:
+            t_text, dt_text = key.split("|")
+            table[(capsule._float_bits(t_text), capsule._float_bits(dt_text))] = complex(
+                capsule._float_bits(value["c"][0]), capsule._float_bits(value["c"][1]))
+        src.current = lambda t, dt, table=table: table[(t, dt)]
+    return driver

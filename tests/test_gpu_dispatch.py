@@ -31,3 +31,29 @@ def test_consult_by_consult_steps_match_run():
     assert rep["dispatched"] and rep["launches"] == 2 * steps
     a.close()
     b.close()
+
+
+def test_a_run_past_the_planned_length_extends_the_source_tables():
+    """Planned for 40 steps, then stepped and run to 300 (the tables extended on the device
+    twice): the same words as a run planned for 300 from the start, and the source still
+    drives after step 40."""
+    from meep_gpu.driver import FdtdDriver
+    from tests.workload_runs import small
+
+    a = FdtdDriver.from_workload(small("sphere"))
+    a.freeze(planned_steps=40)
+    for _ in range(45):
+        a.step()
+    a.run(steps=255)
+    a.synchronize()
+    b = FdtdDriver.from_workload(small("sphere"))
+    b.freeze(planned_steps=300)
+    b.run(steps=300)
+    b.synchronize()
+    assert a.step_count == b.step_count == 300
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Dz", "Bx"):
+        x, y = a.get_array(comp), b.get_array(comp)
+        assert x.view(numpy.uint32).tobytes() == y.view(numpy.uint32).tobytes(), comp
+    assert numpy.max(numpy.abs(b.get_array("Ez"))) > 0
+    a.close()
+    b.close()
