@@ -318,6 +318,15 @@ int fdtd_get_launch_shell(int32_t pair, int32_t *vec);
  * launches each sub-step). */
 int fdtd_set_graphs(int32_t enable);
 
+/* Where some plane decimates, the captured period is the least common multiple of the planes'
+ * decimations, repeated up to 128 steps (at most 1024 steps, with fewer than 127 samples of any
+ * plane in it): both pairs and the sampling launches, the transform passes run between replays
+ * before a ring would overflow; it replays while each plane's decimation phase is the one it
+ * was captured at. out[0] = graphs captured, out[1] = replays after their capture, over the
+ * simulation's (or batch's) life. No counterpart in the reference. */
+int fdtd_graph_stats(fdtd_sim *sim, int64_t out[2]);
+int fdtd_batch_graph_stats(fdtd_batch *batch, int64_t out[2]);
+
 /* mfma = 1 (the default), 2 or 4: the float64 flux transform runs as 16x16x4 float64 MFMA
  * products (phase words times ring samples), each wave taking that many 16-cell tiles; 0: the
  * vector-FMA kernel. The sums associate differently (both are float64; they agree to rounding).

@@ -39,6 +39,7 @@
 #include <string>
 #include <unordered_map>
 #include <utility>
+#include <numeric>
 #include <vector>
 
 #include "../../include/meep_fdtd.h"
@@ -53,6 +54,7 @@ constexpr int kLosslessPoles = 4;  // poles the lossless periodic instantiations
 constexpr int kRing = 128;      // flux samples buffered before one float64 transform pass
 constexpr int kMaxRuns = 2;     // runs of a layer membership line held in the descriptor
 constexpr int kFoldItem = 1024;  // ghost-plane cells per fold_kernel workgroup
+constexpr int kMaxPeriod = 1024;  // steps a decimated schedule's captured period may hold
 
 thread_local std::string g_error;
 
@@ -2315,6 +2317,7 @@ struct StepGraph {
     LaunchShape shape[2];
     int flux_mode = -1;  // the transform kernel the sequence was captured with
     int signature = -1;  // group_signature at capture
+    int64_t captures = 0, replays = 0;  // over the owner's life (fdtd_graph_stats)
     // Replays of the exec may still be queued on its stream: they finish before it is destroyed
     // (and before any buffer its launches read is freed, which callers do after a reset).
     void reset() {
@@ -2678,8 +2681,10 @@ int add_timing(Timing *tm, int nsteps) {
 
 // Flux bookkeeping after a step: one sampling launch for every plane that samples now, then a
 // transform pass for each ring that is full. With launch == false only the host mirrors move
-// (the step itself runs inside a replayed graph).
-int flux_after_step(const Stepper &k, bool launch, int adv = 0) {
+// (the step itself runs inside a replayed graph). With transforms == false no pass is due: the
+// caller has left every ring room for the samples ahead (a decimated period's graph, whose
+// passes run between its replays).
+int flux_after_step(const Stepper &k, bool launch, int adv = 0, bool transforms = true) {
     if (!k.nplanes) return FDTD_OK;
     bool any = false;
     for (int q = 0; q < k.nsims; ++q)
@@ -2696,7 +2701,7 @@ int flux_after_step(const Stepper &k, bool launch, int adv = 0) {
             PlaneHost &ph = k.sims[q]->planes[r];
             if (!samples_now(ph, k.sims[q]->host.step)) continue;
             ph.filled += 1;
-            if (ph.filled < kRing) continue;
+            if (ph.filled < kRing || !transforms) continue;
             if (!launch) {
                 ph.filled = 0;
                 continue;
@@ -2710,7 +2715,7 @@ int flux_after_step(const Stepper &k, bool launch, int adv = 0) {
 // One whole step. `t` < 0: a lone step (the counters advance after its D pair); else step t of a
 // chunk (eager or being captured) whose launches read the counters at offset t and whose
 // counters advance once, by the chunk's length, at its end (chunk_end).
-int step_once(const Stepper &k, bool launch, int t = -1) {
+int step_once(const Stepper &k, bool launch, int t = -1, bool transforms = true) {
     const int adv = t < 0 ? 0 : t;
     if (launch) {
         hipEvent_t *ev = (k.tm && t >= 0) ? k.tm->ev + 3 * t : nullptr;
@@ -2723,7 +2728,7 @@ int step_once(const Stepper &k, bool launch, int t = -1) {
         if (ev) HIP_TRY(hipEventRecord(ev[2], k.st));
     }
     for (int q = 0; q < k.nsims; ++q) mirror_advance(k.sims[q]);
-    return flux_after_step(k, launch, t < 0 ? 0 : t + 1);
+    return flux_after_step(k, launch, t < 0 ? 0 : t + 1, transforms);
 }
 
 // The counter advance that closes a chunk of n steps.
@@ -2766,16 +2771,19 @@ void restore_mirror(const Stepper &k, const Mirror &m) {
     }
 }
 
-// Capture kRing steps into k.graph and launch it once (the host mirrors advance as the
-// capture walks the steps). On failure nothing has run and the mirrors are put back.
-int capture_period(const Stepper &k, const std::vector<int> &phase) {
+// Capture `steps` steps into k.graph and launch it once (the host mirrors advance as the
+// capture walks the steps). transforms: the ring passes that fall due are in the sequence (a
+// kRing period of planes that sample every step), or none are (a decimated period). On failure
+// nothing has run and the mirrors are put back.
+int capture_period(const Stepper &k, const std::vector<int> &phase, int steps = kRing,
+                   bool transforms = true) {
     k.graph->reset();
     (void)num_cu();  // device query outside the capture
     const Mirror before = save_mirror(k);
     HIP_TRY(hipStreamBeginCapture(k.st, hipStreamCaptureModeThreadLocal));
     int rc = FDTD_OK;
-    for (int t = 0; t < kRing && rc == FDTD_OK; ++t) rc = step_once(k, true, t);
-    if (rc == FDTD_OK) rc = chunk_end(k, kRing);
+    for (int t = 0; t < steps && rc == FDTD_OK; ++t) rc = step_once(k, true, t, transforms);
+    if (rc == FDTD_OK) rc = chunk_end(k, steps);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(k.st, &graph);
     hipGraphExec_t exec = nullptr;
@@ -2797,21 +2805,81 @@ int capture_period(const Stepper &k, const std::vector<int> &phase) {
     k.graph->shape[1] = g_shape[1];
     k.graph->flux_mode = g_flux_mfma;
     k.graph->signature = group_signature(k.g);
+    k.graph->captures += 1;
     HIP_TRY(hipGraphLaunch(exec, k.st));
     return FDTD_OK;
 }
 
-// nsteps whole steps: replayed kRing-step graphs where every plane samples each step, eager
-// launches for the rest.
+// The period of a decimated schedule: the least common multiple of the planes' decimations,
+// taken as many times as fits in kRing steps, or 0 when no such period holds (it would exceed
+// kMaxPeriod steps, or a plane would take kRing - 1 or more samples in it).
+int decimated_period(const Stepper &k) {
+    long long l = 1;
+    int dmin = 1 << 30;
+    for (int q = 0; q < k.nsims; ++q)
+        for (auto &ph : k.sims[q]->planes) {
+            const long long d = ph.dev.dec;
+            l = l / std::gcd(l, d) * d;
+            if (l > kMaxPeriod) return 0;
+            dmin = std::min(dmin, ph.dev.dec);
+        }
+    const int p = l <= kRing ? (int)(l * (kRing / l)) : (int)l;
+    return p / dmin < kRing - 1 ? p : 0;
+}
+
+// Transform now every ring that the next `period` steps would fill (at most kRing - 1 samples
+// may stand in a ring when a decimated period's replay starts, so none fills inside it).
+int flush_before_period(const Stepper &k, int period) {
+    int pi = 0;
+    for (int q = 0; q < k.nsims; ++q)
+        for (size_t r = 0; r < k.sims[q]->planes.size(); ++r, ++pi) {
+            PlaneHost &ph = k.sims[q]->planes[r];
+            if (ph.filled + period / ph.dev.dec <= kRing - 1) continue;
+            int rc = flux_flush_plane(k.planes + pi, ph, k.st);
+            if (rc) return rc;
+        }
+    return FDTD_OK;
+}
+
+// nsteps whole steps: replayed graphs -- kRing-step periods with their transform passes where
+// every plane samples each step, or periods of the decimations' least common multiple with the
+// passes between replays where some plane decimates -- and eager launches for the rest.
 int run_steps(const Stepper &k, int nsteps) {
     int done = 0;
-    bool eligible = g_graphs && k.graph && !k.tm && nsteps >= kRing;
-    std::vector<int> phase;
-    for (int q = 0; q < k.nsims && eligible; ++q)
-        for (auto &ph : k.sims[q]->planes) {
-            eligible &= ph.dev.dec == 1;
-            phase.push_back(ph.filled);
+    bool every = true;
+    for (int q = 0; q < k.nsims; ++q)
+        for (auto &ph : k.sims[q]->planes) every &= ph.dev.dec == 1;
+    const int period = every ? kRing : decimated_period(k);
+    bool eligible = g_graphs && k.graph && !k.tm && period > 0 && nsteps >= period;
+    if (eligible && !every) {
+        // the graph holds the pairs and the sampling launches of one period; it replays while
+        // each plane's decimation phase is the one it was captured at
+        std::vector<int> phase{1, period};
+        for (int q = 0; q < k.nsims; ++q)
+            for (auto &ph : k.sims[q]->planes)
+                phase.push_back(((k.sims[q]->host.step - ph.dev.first_step) % ph.dev.dec +
+                                 ph.dev.dec) % ph.dev.dec);
+        if (!k.graph->matches(phase, k.g)) {
+            int rc = flush_before_period(k, period);
+            if (rc == FDTD_OK) rc = capture_period(k, phase, period, false);
+            if (rc) return rc;
+            done = period;
         }
+        for (; done + period <= nsteps; done += period) {
+            int rc = flush_before_period(k, period);
+            if (rc) return rc;
+            for (int t = 0; t < period; ++t) {
+                rc = step_once(k, false, -1, false);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipGraphLaunch(k.graph->exec, k.st));
+            k.graph->replays += 1;
+        }
+        eligible = false;
+    }
+    std::vector<int> phase{0};
+    for (int q = 0; q < k.nsims && eligible; ++q)
+        for (auto &ph : k.sims[q]->planes) phase.push_back(ph.filled);
     if (eligible) {
         if (!k.graph->matches(phase, k.g)) {
             int rc = capture_period(k, phase);
@@ -2824,6 +2892,7 @@ int run_steps(const Stepper &k, int nsteps) {
                 if (rc) return rc;
             }
             HIP_TRY(hipGraphLaunch(k.graph->exec, k.st));
+            k.graph->replays += 1;
         }
     }
     while (done < nsteps) {
@@ -3529,6 +3598,20 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
             s->host.src_hi[x.field][a] = std::max(s->host.src_hi[x.field][a], x.lo[a] + x.cnt[a]);
         }
     s->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_graph_stats(fdtd_sim *s, int64_t out[2]) {
+    if (!s || !out) return fail(FDTD_EINVAL, "null argument");
+    out[0] = s->graph.captures;
+    out[1] = s->graph.replays;
+    return FDTD_OK;
+}
+
+int fdtd_batch_graph_stats(fdtd_batch *b, int64_t out[2]) {
+    if (!b || !out) return fail(FDTD_EINVAL, "null argument");
+    out[0] = b->graph.captures;
+    out[1] = b->graph.replays;
     return FDTD_OK;
 }
 

@@ -71,6 +71,8 @@ SIGNATURES = {
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_source_currents": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
+    "fdtd_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "fdtd_batch_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "fdtd_set_fold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_conductivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,

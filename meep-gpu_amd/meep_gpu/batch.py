@@ -63,6 +63,13 @@ class BatchedRun:
     def synchronize(self):
         check(self.lib.fdtd_batch_synchronize(self.handle), "fdtd_batch_synchronize", self.lib)
 
+    def graph_stats(self):
+        """(graphs captured, replays after their capture) over the batch's life."""
+        out = (ctypes.c_int64 * 2)()
+        check(self.lib.fdtd_batch_graph_stats(self.handle, out), "fdtd_batch_graph_stats",
+              self.lib)
+        return int(out[0]), int(out[1])
+
     def flush(self):
         check(self.lib.fdtd_batch_flush(self.handle), "fdtd_batch_flush", self.lib)
 

@@ -28,7 +28,7 @@ from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
 from .grid import Grid
 from .pml import PML
 from .readback import meep_array
-from .sources import DepositBox, current_table, meep_volume_source
+from .sources import DepositBox, GaussianSource, current_table, meep_volume_source
 from .workloads import Workload
 
 _E = ("Ex", "Ey", "Ez")
@@ -54,6 +54,7 @@ class FdtdDriver:
         self.flux_monitors: List[FluxMonitor] = []
         self.dft_monitors: List[DftFieldsMonitor] = []
         self._default_dft_regions: List[DftFieldsMonitor] = []
+        self._automatic_monitors: list = []  # decimation 0: resolved at the freeze
         self.max_steps = int(max_steps)
         self.sim: Optional[HipSimulation] = None
         self._fast_path = None
@@ -199,7 +200,8 @@ class FdtdDriver:
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))
         mon = FluxMonitor(int(normal), tuple(lo), tuple(cnt), freqs, self.grid.dx, self.grid.dt,
-                          decimation)
+                          decimation or 1)
+        self._note_automatic(mon, decimation)
         self.flux_monitors.append(mon)
         return mon
 
@@ -213,7 +215,8 @@ class FdtdDriver:
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))
         mon = FluxMonitor.from_region(self.grid, tuple(center), tuple(size), freqs,
-                                      int(decimation))
+                                      int(decimation) or 1)
+        self._note_automatic(mon, decimation)
         self.flux_monitors.append(mon)
         return mon
 
@@ -236,14 +239,38 @@ class FdtdDriver:
             raise ValueError(f"monitor size must be non-negative, got {tuple(size)}")
         if center is not None:
             mon = DftFieldsMonitor.from_region(self.grid, tuple(components), tuple(center),
-                                               tuple(size), freqs, int(decimation), yee_grid)
+                                               tuple(size), freqs, int(decimation) or 1,
+                                               yee_grid)
         else:  # placed at freeze, when the absorber is known
             mon = DftFieldsMonitor(tuple(components), (0, 0, 0), tuple(self.grid.shape), freqs,
-                                   self.grid.dt, int(decimation), yee_grid,
+                                   self.grid.dt, int(decimation) or 1, yee_grid,
                                    invariant=tuple(self.grid.is_invariant(a) for a in range(3)))
             self._default_dft_regions.append(mon)
+        self._note_automatic(mon, decimation)
         self.dft_monitors.append(mon)
         return mon
+
+    def _note_automatic(self, mon, decimation) -> None:
+        if int(decimation) < 0:
+            raise ValueError(f"decimation must be >= 0 (0: automatic), got {decimation!r}")
+        if int(decimation) == 0:
+            self._automatic_monitors.append(mon)
+
+    def automatic_decimation(self, frequencies: Sequence[float]) -> int:
+        """MEEP's automatic decimation (``decimation_factor == 0``; reference
+        ``_automatic_decimation_factor`` driver.py:3101-3163, MEEP dft.cpp:195-216):
+        ``floor(1 / (dt * (freq_max + src_freq_max)))``, at least 1, where each Gaussian
+        source contributes ``|frequency| + fwidth/2`` and continuous or custom sources nothing
+        (1 when no Gaussian source or no nonzero monitor frequency). Resolved at the freeze,
+        when every source is known."""
+        src_max = 0.0
+        for _, fn in self._sources:
+            if isinstance(fn, GaussianSource):
+                src_max = max(src_max, abs(float(fn.frequency)) + 0.5 * float(fn.fwidth))
+        mon_max = max((abs(float(f)) for f in numpy.atleast_1d(frequencies)), default=0.0)
+        if mon_max <= 0.0 or src_max <= 0.0:
+            return 1
+        return max(1, int(math.floor(1.0 / (self.grid.dt * (mon_max + src_max)))))
 
     def _place_default_dft_regions(self, pml) -> None:
         """The absorber's interior: each face trimmed by the whole cells it grades."""
@@ -274,6 +301,7 @@ class FdtdDriver:
         drv.flux_monitors = []
         drv.dft_monitors = []
         drv._default_dft_regions = []
+        drv._automatic_monitors = []
         drv.max_steps = max(wl.steps, 1)
         drv.sim = None
         drv._fast_path = None
@@ -322,6 +350,8 @@ class FdtdDriver:
             axis = "xyz".index(box.component[1])
             sim.add_source(box.magnetic, axis, box.lo, box.cnt, box.amp_re, box.amp_im, table,
                            self.grid.dt, box.integrated)
+        for mon in self._automatic_monitors:
+            mon.decimation = self.automatic_decimation(mon.frequencies)
         for mon in self.flux_monitors:
             mon.attach(sim)
         self._place_default_dft_regions(pml)

@@ -265,6 +265,12 @@ class HipSimulation:
     def synchronize(self):
         self._check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
 
+    def graph_stats(self) -> Tuple[int, int]:
+        """(graphs captured, replays after their capture) over this simulation's life."""
+        out = (ctypes.c_int64 * 2)()
+        self._check(self.lib.fdtd_graph_stats(self.handle, out), "fdtd_graph_stats")
+        return int(out[0]), int(out[1])
+
     def flush(self):
         """Transform every plane's buffered samples now (``fdtd_flush``), in stream order."""
         self._check(self.lib.fdtd_flush(self.handle), "fdtd_flush")

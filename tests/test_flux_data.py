@@ -112,3 +112,22 @@ def test_an_unattached_monitor_refuses():
         mon.get_dft_data()
     with pytest.raises(RuntimeError):
         mon.scale_dfts(-1)
+
+
+def test_automatic_decimation_is_meep_s_rule():
+    """decimation=0 resolves at the freeze to floor(1/(dt*(f_max + src_max))) with a Gaussian
+    source's |f| + fwidth/2 (reference driver.py:3101-3163): the two recorded examples' nominal
+    values 49 and 36, and 1 without a Gaussian source or a nonzero monitor frequency."""
+    from meep_gpu.driver import FdtdDriver
+    from meep_gpu.sources import ContinuousSource, GaussianSource
+
+    def rule(cell, fcen, df, src_f, src_w, kind=GaussianSource):
+        drv = FdtdDriver(cell_size=cell, resolution=20.0, dimensions=2)
+        drv._sources.append((None, kind(src_f, src_w) if kind is GaussianSource else kind(src_f)))
+        freqs = numpy.linspace(fcen - df / 2, fcen + df / 2, 5)
+        return drv.automatic_decimation(freqs)
+
+    assert rule((10.0, 6.0, 0.0), 0.25, 0.3, 0.25, 0.3) == 49
+    assert rule((8.0, 6.0, 0.0), 0.4, 0.3, 0.4, 0.3) == 36
+    assert rule((8.0, 6.0, 0.0), 0.4, 0.3, 0.4, 0.3, ContinuousSource) == 1
+    assert rule((8.0, 6.0, 0.0), 0.0, 0.0, 0.4, 0.3) == 1
