@@ -217,10 +217,10 @@ def config_line(name: str, steps: int) -> dict:
     from meep_gpu.pml import PML
 
     wl = workloads.config(name)
-    shapes = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
-                           wl.poles, use_cache=False, inv=wl.inv)
     drv = FdtdDriver.from_workload(wl)
     sim = drv.freeze(planned_steps=steps + WARM + 1)
+    shapes = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
+                           wl.poles, use_cache=False, inv=wl.inv, target=sim)
     sim.run(WARM)  # the first step graph is captured (and replayed once) here
     sim.synchronize()
     t0 = time.perf_counter()
@@ -275,19 +275,20 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     batched_s = time.perf_counter() - t0
     batch.close()
     nbytes = sum(step_bytes(d.sim) for d in drivers)
-    # one at a time: each simulation alone, in turn, with its own autotuned shapes; first as
-    # the plain form launches a step (eagerly), then with the step graphs of the batched form
-    from meep_gpu.engine import set_graphs
+    # one at a time: each simulation alone, in turn, with its own autotuned shapes (its own
+    # launch configuration); first as the plain form launches a step (eagerly), then with the
+    # step graphs of the batched form
+    import dataclasses
 
     alone = {}
     for graphs in (False, True):
-        set_graphs(graphs and GRAPHS)
         alone_s = 0.0
         for q, d in enumerate(drivers):
             if q % 16 == 0:
                 print(f"bench: one at a time (graphs {graphs}), simulation {q}/{len(drivers)}",
                       file=sys.stderr, flush=True)
-            autotune.tune(d.grid, d.sim.pml, [], repeats=3)
+            autotune.tune(d.grid, d.sim.pml, [], repeats=3, target=d.sim)
+            d.sim.set_config(dataclasses.replace(d.sim.config(), graphs=graphs and GRAPHS))
             d.sim.flush()
             d.sim.run(WARM)
             d.sim.synchronize()
@@ -297,7 +298,6 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             d.sim.synchronize()
             alone_s += time.perf_counter() - t0
         alone[graphs] = cells * steps / alone_s / 1e6
-    set_graphs(GRAPHS)
     finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum()))
                  for d in drivers for m in d.flux_monitors)
     for d in drivers:

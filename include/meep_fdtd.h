@@ -318,6 +318,17 @@ int fdtd_get_launch_shell(int32_t pair, int32_t *vec);
  * launches each sub-step). */
 int fdtd_set_graphs(int32_t enable);
 
+/* A simulation's (or batch's) own launch configuration, in place of the process defaults the
+ * setters above change: cfg[0..4] the H pair's shape (vec, blocks_per_cu, xcd_remap, nt,
+ * shell vec), cfg[5..9] the D pair's, cfg[10] graphs (0/1), cfg[11] the flux transform (0, 1,
+ * 2 or 4, as fdtd_set_flux_transform). NULL: follow the process defaults again. The get calls
+ * write the configuration in force. Results are the same words under every configuration. No
+ * counterpart in the reference. */
+int fdtd_sim_set_config(fdtd_sim *sim, const int32_t *cfg);
+int fdtd_sim_get_config(fdtd_sim *sim, int32_t out[12]);
+int fdtd_batch_set_config(fdtd_batch *batch, const int32_t *cfg);
+int fdtd_batch_get_config(fdtd_batch *batch, int32_t out[12]);
+
 /* Where some plane decimates, the captured period is the least common multiple of the planes'
  * decimations, repeated up to 128 steps (at most 1024 steps, with fewer than 127 samples of any
  * plane in it): both pairs and the sampling launches, the transform passes run between replays

@@ -2077,7 +2077,17 @@ struct LaunchShape {
     int svec = 1;           // cells per thread of the shell's lean kernel (its layer forms hold
                             // more operands, so fewer cells a thread keep more waves resident)
 };
-LaunchShape g_shape[2];  // [0]: the H pair's launches, [1]: the D pair's (and its row path)
+// How one simulation's (or batch's) launches run: each fused pair's launch shape ([0] the H
+// pair's launches, [1] the D pair's and its row path), whether fdtd_run / fdtd_batch_run replay
+// captured graphs, and the flux transform kernel (1, 2 or 4: MFMA tiles per wave; 0: vector).
+struct LaunchConfig {
+    LaunchShape shape[2];
+    int graphs = 1;
+    int flux_mfma = 1;
+};
+// The process defaults (the process-wide setters): every object without a configuration of
+// its own (fdtd_sim_set_config, fdtd_batch_set_config) follows them, as they change.
+LaunchConfig g_defaults;
 int g_num_cu = 0;
 
 int num_cu() {
@@ -2126,6 +2136,7 @@ struct Group {
                            // kLosslessPoles poles: the pairs' general instantiation (the others
                            // keep the lossless periodic code)
     int inv_mode = 3;  // the dpair_kernel INV instantiation that serves every member
+    const LaunchConfig *cfg = &g_defaults;  // the owner's launch configuration
     const int4 *fold_items = nullptr;  // the folded members' ghost planes (fold_kernel)
     int nfold = 0;
 };
@@ -2188,7 +2199,7 @@ void launch_hbox(const Group &g, int G, const int4 *l, int n, int rm, int adv, h
 }
 
 int launch_h(const Group &g, hipStream_t st, int adv = 0) {
-    const LaunchShape &sh = g_shape[0];
+    const LaunchShape &sh = g.cfg->shape[0];
     const int V = sh.vec, SV = sh.svec;
     const int rm = sh.xcd_remap;
     if (!g.general) {  // the box's rows, then the shell's: two lean kernels
@@ -2246,7 +2257,7 @@ void launch_dlean(const Group &g, int G, const int4 *l, int n, int rm, int adv, 
 // ... then the counter advance by `advance_by` steps (0 inside a captured sequence, which
 // advances once at its end).
 int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
-    const LaunchShape &sh = g_shape[1];
+    const LaunchShape &sh = g.cfg->shape[1];
     const int V = sh.vec, SV = sh.svec;
     const int rm = sh.xcd_remap;
     const int nt = g.ntiles.at(V);
@@ -2299,8 +2310,8 @@ int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
 // replayed while the launch shape and every plane's ring
 // phase are what they were at capture. Plane rings fill once per kRing steps when every plane
 // samples each step, so one capture serves every later period.
-int g_graphs = 1;  // replay captured step sequences where eligible
-int g_flux_mfma = 1;  // the flux transform on the matrix cores with this many 16-cell tiles per
+// (g_defaults.graphs: replay captured step sequences where eligible; g_defaults.flux_mfma: the
+// flux transform on the matrix cores with this many 16-cell tiles per
                       // wave (1, 2 or 4), or 0: the vector FMA kernel
 
 // The kernel instantiations a group's launches select (its general/lossless forms, inverse
@@ -2330,8 +2341,8 @@ struct StepGraph {
         phase.clear();
     }
     bool matches(const std::vector<int> &ph, const Group &g) const {
-        return exec && ph == phase && same_shape(shape[0], g_shape[0]) &&
-               same_shape(shape[1], g_shape[1]) && flux_mode == g_flux_mfma &&
+        return exec && ph == phase && same_shape(shape[0], g.cfg->shape[0]) &&
+               same_shape(shape[1], g.cfg->shape[1]) && flux_mode == g.cfg->flux_mfma &&
                signature == group_signature(g);
     }
     ~StepGraph() { reset(); }
@@ -2365,6 +2376,9 @@ struct fdtd_sim {
     int nshellw[3] = {0, 0, 0};
     const int4 *intw[3] = {nullptr, nullptr, nullptr};    // the interior box's items alone
     int nintw[3] = {0, 0, 0};
+    LaunchConfig own_cfg;  // the simulation's own launch configuration, when has_cfg
+    bool has_cfg = false;
+    const LaunchConfig *config() const { return has_cfg ? &own_cfg : &g_defaults; }
     const int4 *foldw = nullptr;  // the ghost planes of the folded axes (fold_kernel items)
     int nfoldw = 0;
     std::vector<int4> fold_host;      // the same items, for a batch's concatenation
@@ -2440,6 +2454,7 @@ struct fdtd_sim {
         g.inv_mode = inv_mode_of(host);
         g.fold_items = foldw;
         g.nfold = nfoldw;
+        g.cfg = config();
         return g;
     }
     int push_planes() {
@@ -2595,15 +2610,18 @@ struct fdtd_batch {
     int max_plane_cells = 0;
     hipStream_t stream = nullptr;
     StepGraph graph;
+    LaunchConfig own_cfg;  // the batch's own launch configuration, when has_cfg
+    bool has_cfg = false;
+    const LaunchConfig *config() const { return has_cfg ? &own_cfg : &g_defaults; }
 };
 
 namespace {
-int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st) {
+int flux_flush_plane(PlaneDev *dev_plane, PlaneHost &ph, hipStream_t st, int mfma) {
     if (ph.filled == 0) return FDTD_OK;
     const PlaneDev &p = ph.dev;
     flux_phase_kernel<<<dim3((ph.filled * p.nf + 255) / 256, 1), 256, 0, st>>>(dev_plane, ph.filled);
-    if (g_flux_mfma) {
-        const int ct = g_flux_mfma;  // 16-cell tiles per wave: 1, 2 or 4
+    if (mfma) {
+        const int ct = mfma;  // 16-cell tiles per wave: 1, 2 or 4
         const int per_block = 4 * 16 * ct;
         const dim3 grid((p.cells + per_block - 1) / per_block, p.nslot,
                         (2 * p.nf + 16 * kMT - 1) / (16 * kMT));
@@ -2706,7 +2724,7 @@ int flux_after_step(const Stepper &k, bool launch, int adv = 0, bool transforms 
                 ph.filled = 0;
                 continue;
             }
-            int rc = flux_flush_plane(k.planes + pi, ph, k.st);
+            int rc = flux_flush_plane(k.planes + pi, ph, k.st, k.g.cfg->flux_mfma);
             if (rc) return rc;
         }
     return FDTD_OK;
@@ -2801,9 +2819,9 @@ int capture_period(const Stepper &k, const std::vector<int> &phase, int steps = 
     k.graph->exec = exec;
     k.graph->st = k.st;
     k.graph->phase = phase;
-    k.graph->shape[0] = g_shape[0];
-    k.graph->shape[1] = g_shape[1];
-    k.graph->flux_mode = g_flux_mfma;
+    k.graph->shape[0] = k.g.cfg->shape[0];
+    k.graph->shape[1] = k.g.cfg->shape[1];
+    k.graph->flux_mode = k.g.cfg->flux_mfma;
     k.graph->signature = group_signature(k.g);
     k.graph->captures += 1;
     HIP_TRY(hipGraphLaunch(exec, k.st));
@@ -2835,7 +2853,7 @@ int flush_before_period(const Stepper &k, int period) {
         for (size_t r = 0; r < k.sims[q]->planes.size(); ++r, ++pi) {
             PlaneHost &ph = k.sims[q]->planes[r];
             if (ph.filled + period / ph.dev.dec <= kRing - 1) continue;
-            int rc = flux_flush_plane(k.planes + pi, ph, k.st);
+            int rc = flux_flush_plane(k.planes + pi, ph, k.st, k.g.cfg->flux_mfma);
             if (rc) return rc;
         }
     return FDTD_OK;
@@ -2850,7 +2868,7 @@ int run_steps(const Stepper &k, int nsteps) {
     for (int q = 0; q < k.nsims; ++q)
         for (auto &ph : k.sims[q]->planes) every &= ph.dev.dec == 1;
     const int period = every ? kRing : decimated_period(k);
-    bool eligible = g_graphs && k.graph && !k.tm && period > 0 && nsteps >= period;
+    bool eligible = k.g.cfg->graphs && k.graph && !k.tm && period > 0 && nsteps >= period;
     if (eligible && !every) {
         // the graph holds the pairs and the sampling launches of one period; it replays while
         // each plane's decimation phase is the one it was captured at
@@ -2963,7 +2981,7 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
     if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", blocks_per_cu);
     if (xcd_remap < 0 || xcd_remap > 4096) return fail(FDTD_EINVAL, "xcd_remap %d", xcd_remap);
-    LaunchShape &sh = g_shape[pair];
+    LaunchShape &sh = g_defaults.shape[pair];
     sh.vec = vec;
     sh.blocks_per_cu = blocks_per_cu;
     sh.xcd_remap = xcd_remap;
@@ -2974,22 +2992,22 @@ int fdtd_set_launch_pair(int32_t pair, int32_t vec, int32_t blocks_per_cu, int32
 int fdtd_set_launch_shell(int32_t pair, int32_t vec) {
     if (pair != 0 && pair != 1) return fail(FDTD_EINVAL, "pair must be 0 (H) or 1 (D), got %d", pair);
     if (vec != 1 && vec != 2 && vec != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", vec);
-    g_shape[pair].svec = vec;
+    g_defaults.shape[pair].svec = vec;
     return FDTD_OK;
 }
 
 int fdtd_get_launch_shell(int32_t pair, int32_t *vec) {
     if ((pair != 0 && pair != 1) || !vec) return fail(FDTD_EINVAL, "bad argument");
-    *vec = g_shape[pair].svec;
+    *vec = g_defaults.shape[pair].svec;
     return FDTD_OK;
 }
 
 int fdtd_get_launch_pair(int32_t pair, int32_t out[4]) {
     if (!out || (pair != 0 && pair != 1)) return fail(FDTD_EINVAL, "bad argument");
-    out[0] = g_shape[pair].vec;
-    out[1] = g_shape[pair].blocks_per_cu;
-    out[2] = g_shape[pair].xcd_remap;
-    out[3] = g_shape[pair].nt;
+    out[0] = g_defaults.shape[pair].vec;
+    out[1] = g_defaults.shape[pair].blocks_per_cu;
+    out[2] = g_defaults.shape[pair].xcd_remap;
+    out[3] = g_defaults.shape[pair].nt;
     return FDTD_OK;
 }
 
@@ -3601,6 +3619,82 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
     return FDTD_OK;
 }
 
+// cfg[0..4] the H pair's shape (vec, blocks_per_cu, xcd_remap, nt, svec), [5..9] the D pair's,
+// [10] graphs, [11] flux transform; checked as the process-wide setters check them.
+static int parse_config(const int32_t *cfg, LaunchConfig *out) {
+    for (int pair = 0; pair < 2; ++pair) {
+        const int32_t *c = cfg + 5 * pair;
+        if (c[0] != 1 && c[0] != 2 && c[0] != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", c[0]);
+        if (c[1] < 0 || c[1] > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", c[1]);
+        if (c[2] < 0 || c[2] > 4096) return fail(FDTD_EINVAL, "xcd_remap %d", c[2]);
+        if (c[4] != 1 && c[4] != 2 && c[4] != 4) return fail(FDTD_EINVAL, "shell vec must be 1, 2 or 4, got %d", c[4]);
+        LaunchShape &sh = out->shape[pair];
+        sh.vec = c[0];
+        sh.blocks_per_cu = c[1];
+        sh.xcd_remap = c[2];
+        sh.nt = c[3] ? 1 : 0;
+        sh.svec = c[4];
+    }
+    if (cfg[11] != 0 && cfg[11] != 1 && cfg[11] != 2 && cfg[11] != 4)
+        return fail(FDTD_EINVAL, "flux transform must be 0, 1, 2 or 4, got %d", cfg[11]);
+    out->graphs = cfg[10] ? 1 : 0;
+    out->flux_mfma = cfg[11];
+    return FDTD_OK;
+}
+
+static void write_config(const LaunchConfig &c, int32_t out[12]) {
+    for (int pair = 0; pair < 2; ++pair) {
+        const LaunchShape &sh = c.shape[pair];
+        int32_t *o = out + 5 * pair;
+        o[0] = sh.vec;
+        o[1] = sh.blocks_per_cu;
+        o[2] = sh.xcd_remap;
+        o[3] = sh.nt;
+        o[4] = sh.svec;
+    }
+    out[10] = c.graphs;
+    out[11] = c.flux_mfma;
+}
+
+int fdtd_sim_set_config(fdtd_sim *s, const int32_t *cfg) {
+    if (!s) return fail(FDTD_EINVAL, "null simulation");
+    if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
+    LaunchConfig c;
+    if (cfg) {
+        int rc = parse_config(cfg, &c);
+        if (rc) return rc;
+    }
+    s->graph.reset();  // a captured sequence launched with the old shapes
+    s->own_cfg = c;
+    s->has_cfg = cfg != nullptr;
+    return FDTD_OK;
+}
+
+int fdtd_sim_get_config(fdtd_sim *s, int32_t out[12]) {
+    if (!s || !out) return fail(FDTD_EINVAL, "null argument");
+    write_config(*s->config(), out);
+    return FDTD_OK;
+}
+
+int fdtd_batch_set_config(fdtd_batch *b, const int32_t *cfg) {
+    if (!b) return fail(FDTD_EINVAL, "null batch");
+    LaunchConfig c;
+    if (cfg) {
+        int rc = parse_config(cfg, &c);
+        if (rc) return rc;
+    }
+    b->graph.reset();
+    b->own_cfg = c;
+    b->has_cfg = cfg != nullptr;
+    return FDTD_OK;
+}
+
+int fdtd_batch_get_config(fdtd_batch *b, int32_t out[12]) {
+    if (!b || !out) return fail(FDTD_EINVAL, "null argument");
+    write_config(*b->config(), out);
+    return FDTD_OK;
+}
+
 int fdtd_graph_stats(fdtd_sim *s, int64_t out[2]) {
     if (!s || !out) return fail(FDTD_EINVAL, "null argument");
     out[0] = s->graph.captures;
@@ -3786,7 +3880,7 @@ int fdtd_step_d(fdtd_sim *s) {
 // The transform pass over every plane's buffered samples (its ring then starts empty).
 static int flush_all(fdtd_sim *s) {
     for (size_t r = 0; r < s->planes.size(); ++r) {
-        int rc = flux_flush_plane(s->dev_planes + r, s->planes[r], s->stream);
+        int rc = flux_flush_plane(s->dev_planes + r, s->planes[r], s->stream, s->config()->flux_mfma);
         if (rc) return rc;
     }
     return FDTD_OK;
@@ -3809,12 +3903,12 @@ int fdtd_run(fdtd_sim *s, int32_t nsteps) {
 int fdtd_set_flux_transform(int32_t mfma) {
     if (mfma != 0 && mfma != 1 && mfma != 2 && mfma != 4)
         return fail(FDTD_EINVAL, "flux transform: 0 (vector), or 1, 2, 4 (matrix cores), got %d", mfma);
-    g_flux_mfma = mfma;
+    g_defaults.flux_mfma = mfma;
     return FDTD_OK;
 }
 
 int fdtd_set_graphs(int32_t enable) {
-    g_graphs = enable ? 1 : 0;
+    g_defaults.graphs = enable ? 1 : 0;
     return FDTD_OK;
 }
 
@@ -3917,7 +4011,7 @@ int fdtd_flux_read(fdtd_sim *s, int32_t plane, double *out, size_t count) {
     PlaneHost &ph = s->planes[plane];
     const size_t need = (size_t)2 * ph.dev.nslot * ph.dev.nf * ph.dev.cells;
     if (count != need) return fail(FDTD_EINVAL, "flux read needs %zu doubles, got %zu", need, count);
-    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream, s->config()->flux_mfma);
     if (rc) return rc;
     return copy_pinned(s, ph.dev.acc, out, sizeof(double) * need, false);
 }
@@ -3931,7 +4025,7 @@ static int flux_plane_for_edit(fdtd_sim *s, int32_t plane, size_t count, bool ch
     const size_t need = (size_t)2 * ph.dev.nslot * ph.dev.nf * ph.dev.cells;
     if (check_count && count != need)
         return fail(FDTD_EINVAL, "flux write needs %zu doubles, got %zu", need, count);
-    return flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    return flux_flush_plane(s->dev_planes + plane, ph, s->stream, s->config()->flux_mfma);
 }
 
 int fdtd_flux_write(fdtd_sim *s, int32_t plane, const double *in, size_t count) {
@@ -3980,7 +4074,7 @@ int fdtd_flux_spectrum(fdtd_sim *s, int32_t plane, const double *weights, double
     const PlaneDev &p = ph.dev;
     if (!is_flux_plane(p)) return fail(FDTD_EINVAL, "region %d is not a flux plane", plane);
     if (count != (size_t)p.nf) return fail(FDTD_EINVAL, "the plane has %d frequencies, got %zu", p.nf, count);
-    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    int rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream, s->config()->flux_mfma);
     if (rc) return rc;
     if (!ph.spec_partial) {
         rc = s->alloc((void **)&ph.spec_partial, sizeof(double) * (size_t)p.nf * kSpecBlocks);
@@ -4038,7 +4132,7 @@ int fdtd_flux_spectrum_noalloc(fdtd_sim *s, int32_t plane, const double *weights
     double *partial = nullptr;
     int rc = scratch_address(&partial);
     if (rc) return rc;
-    rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream);
+    rc = flux_flush_plane(s->dev_planes + plane, ph, s->stream, s->config()->flux_mfma);
     if (rc) return rc;
     // stage 1 writes every partial[f][b] stage 2 reads (b < the launch's G): nothing to clear
     return launch_spectrum(p, weights, partial, out, s->stream);
@@ -4295,7 +4389,7 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
             return cleanup(FDTD_EHIP, "batch plane upload");
     }
     int rc = FDTD_OK;
-    for (const LaunchShape &sh : g_shape) {  // the widths the launches take, boxes and shells
+    for (const LaunchShape &sh : b->config()->shape) {  // the widths the launches take
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
     }
@@ -4355,13 +4449,14 @@ void fdtd_batch_destroy(fdtd_batch *b) {
 int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     int rc = FDTD_OK;
-    for (const LaunchShape &sh : g_shape) {  // the widths the launches take, boxes and shells
+    for (const LaunchShape &sh : b->config()->shape) {  // the widths the launches take
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
     }
     if (rc) return rc;
     Stepper k;
     k.g = b->group;
+    k.g.cfg = b->config();
     k.sims = b->sims.data();
     k.nsims = (int)b->sims.size();
     k.planes = b->planes;
@@ -4377,7 +4472,7 @@ int fdtd_batch_flush(fdtd_batch *b) {
     int pi = 0;
     for (auto *s : b->sims)
         for (size_t r = 0; r < s->planes.size(); ++r, ++pi) {
-            int rc = flux_flush_plane(b->planes + pi, s->planes[r], b->stream);
+            int rc = flux_flush_plane(b->planes + pi, s->planes[r], b->stream, b->config()->flux_mfma);
             if (rc) return rc;
         }
     return FDTD_OK;

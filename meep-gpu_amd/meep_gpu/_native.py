@@ -71,6 +71,10 @@ SIGNATURES = {
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_source_currents": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
+    "fdtd_sim_set_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "fdtd_sim_get_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "fdtd_batch_set_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "fdtd_batch_get_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "fdtd_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "fdtd_batch_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "fdtd_set_fold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,

@@ -1,6 +1,6 @@
 """Launch-shape autotuner: pick the tile shape the fused pairs run fastest with on this GPU.
 
-The candidates are the engine's launch shapes (``fdtd_set_launch``): cells per thread along x
+The candidates are the engine's launch shapes (``engine.LaunchConfig``): cells per thread along x
 (1, 2 or 4 -- wider runs mean fewer index computations and wider loads, narrower ones fewer
 registers and more resident waves), one workgroup per tile or a persistent grid of 4/8
 workgroups per CU, the XCD-contiguous tile order on or off, and plain or non-temporal output
@@ -9,6 +9,9 @@ simulation with the same grid, absorber and poles as the run (no sources, no mon
 HIP events around every launch; the H pair and the D pair each get the shape they ran fastest
 with (the D pair streams more arrays than the H pair and their best shapes differ). Results
 are cached per (grid shape, absorber, poles, device) in a JSON file, so a run tunes once.
+Candidates are timed under the scratch simulation's (or the batch's) own configuration, so
+tuning changes no other simulation; the result is installed on ``target`` (a simulation or a
+batch) when one is given, else as the process defaults.
 """
 
 from __future__ import annotations
@@ -20,8 +23,8 @@ from typing import Dict, Iterable, List, Optional, Tuple
 
 import numpy
 
-from .engine import (HipSimulation, device_info, get_launch_pair, get_launch_shell,
-                     set_launch, set_launch_pair, set_launch_shell)
+from .engine import (HipSimulation, LaunchConfig, device_info, set_launch_pair,
+                     set_launch_shell)
 from .grid import Grid
 from .pml import PML
 
@@ -89,16 +92,14 @@ def measure(grid: Grid, pml: Optional[PML], poles: Iterable = (),
         sim.add_pole(term, sigma)
     if candidates is None:
         candidates = candidates_for(sim)
-    saved = (get_launch_pair(0), get_launch_pair(1))
     out = []
     try:
         for shape in candidates:
-            set_launch(*shape)
+            sim.set_config(LaunchConfig.of_shapes((shape, shape)))
             sim.time_pairs(2)  # first launch of a shape builds its tile walk
             h, d = sim.time_pairs(repeats)
             out.append((shape, h, d))
     finally:
-        install(saved)
         sim.close()
     return out
 
@@ -108,10 +109,12 @@ SHELL_WIDTHS = (1, 2)
 
 
 def tune_shell(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
-               inv: Optional[dict] = None) -> Tuple[int, int]:
-    """With the pairs' shapes installed, the fastest shell width of each pair (installed)."""
+               inv: Optional[dict] = None, shapes: Optional[PairShapes] = None
+               ) -> Tuple[int, int]:
+    """With the pairs' ``shapes``, the fastest shell width of each pair."""
     if pml is None:
-        return get_launch_shell(0), get_launch_shell(1)
+        return (1, 1)
+    shapes = shapes or ((1, 8, 0, False), (1, 8, 0, False))
     sim = HipSimulation(grid, pml)
     for c, vol in (inv or {}).items():
         sim.set_inverse_epsilon(c, vol)
@@ -120,22 +123,29 @@ def tune_shell(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: in
     times = []
     try:
         for sv in SHELL_WIDTHS:
-            set_launch_shell(0, sv)
-            set_launch_shell(1, sv)
+            sim.set_config(LaunchConfig.of_shapes(shapes, shell=(sv, sv)))
             sim.time_pairs(2)
             h, d = sim.time_pairs(repeats)
             times.append((sv, h, d))
     finally:
         sim.close()
-    best = (min(times, key=lambda r: r[1])[0], min(times, key=lambda r: r[2])[0])
-    set_launch_shell(0, best[0])
-    set_launch_shell(1, best[1])
-    return best
+    return (min(times, key=lambda r: r[1])[0], min(times, key=lambda r: r[2])[0])
 
 
-def install(shapes: PairShapes) -> None:
+def install(shapes: PairShapes, shell: Optional[Tuple[int, int]] = None, target=None) -> None:
+    """The pairs' shapes (and shell widths) as ``target``'s own configuration (a simulation or a
+    batch; its graph and transform modes kept), or as the process defaults."""
+    if target is not None:
+        now = target.config()
+        sh = shell or (now.h[4], now.d[4])
+        target.set_config(LaunchConfig.of_shapes(shapes, shell=sh, graphs=now.graphs,
+                                                 flux_mfma=now.flux_mfma))
+        return
     set_launch_pair(0, *shapes[0])
     set_launch_pair(1, *shapes[1])
+    if shell is not None:
+        set_launch_shell(0, shell[0])
+        set_launch_shell(1, shell[1])
 
 
 def best_pair_shapes(results) -> PairShapes:
@@ -146,28 +156,30 @@ def best_pair_shapes(results) -> PairShapes:
 
 
 def tune(grid: Grid, pml: Optional[PML], poles: Iterable = (), repeats: int = 10,
-         use_cache: bool = True, inv: Optional[dict] = None) -> PairShapes:
-    """Choose and install the fastest launch shape of each pair for this grid; returns them."""
+         use_cache: bool = True, inv: Optional[dict] = None, target=None) -> PairShapes:
+    """Choose the fastest launch shape of each pair (and shell widths) for this grid, install
+    them on ``target`` (a simulation or batch) or as the process defaults; returns the shapes."""
     poles = list(poles)
     key = _key(grid, pml, len(poles))
     cache = _load_cache() if use_cache else {}
     entry = cache.get(key)
-    if isinstance(entry, list) and len(entry) == 2 and all(len(x) == 4 for x in entry):
+    if isinstance(entry, list) and len(entry) == 3 and all(len(x) == 4 for x in entry[:2]):
         shapes = (tuple(entry[0]), tuple(entry[1]))
-        install(shapes)
+        install(shapes, tuple(entry[2]), target)
         return shapes  # type: ignore[return-value]
     shapes = best_pair_shapes(measure(grid, pml, poles, repeats=repeats, inv=inv))
-    install(shapes)
-    tune_shell(grid, pml, poles, repeats=repeats, inv=inv)
+    shell = tune_shell(grid, pml, poles, repeats=repeats, inv=inv, shapes=shapes)
+    install(shapes, shell, target)
     if use_cache:
-        cache[key] = [list(shapes[0]), list(shapes[1])]
+        cache[key] = [list(shapes[0]), list(shapes[1]), list(shell)]
         _store_cache(cache)
     return shapes
 
 
 def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16
                ) -> PairShapes:
-    """Choose and install launch shapes for a ``BatchedRun`` by stepping it.
+    """Choose launch shapes for a ``BatchedRun`` by stepping it; installed as the batch's own
+    configuration (no other simulation's launches change).
 
     Each candidate runs ``steps`` warm steps (its tile list is built on first use), then
     ``steps`` timed ones, for both pairs at once; then the best shape's D pair is held while
@@ -175,8 +187,11 @@ def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16
     cover 2 * steps * (2 * len(candidates)) steps.
     """
     candidates = list(candidates)
+    base = batch.config()
 
-    def timed() -> float:
+    def timed(shapes) -> float:
+        batch.set_config(LaunchConfig.of_shapes(shapes, shell=(base.h[4], base.d[4]),
+                                                graphs=base.graphs, flux_mfma=base.flux_mfma))
         batch.run(steps)
         batch.synchronize()
         t0 = time.perf_counter()
@@ -184,9 +199,8 @@ def tune_batch(batch, candidates: Iterable[Shape] = CANDIDATES, steps: int = 16
         batch.synchronize()
         return time.perf_counter() - t0
 
-    best = min(candidates, key=lambda sh: (set_launch(*sh), timed())[1])
-    set_launch(*best)
-    h_best = min(candidates, key=lambda sh: (set_launch_pair(0, *sh), timed())[1])
+    best = min(candidates, key=lambda sh: timed((sh, sh)))
+    h_best = min(candidates, key=lambda sh: timed((sh, best)))
     shapes = (tuple(h_best), tuple(best))
-    install(shapes)
+    install(shapes, target=batch)
     return shapes

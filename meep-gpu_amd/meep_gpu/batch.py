@@ -63,6 +63,21 @@ class BatchedRun:
     def synchronize(self):
         check(self.lib.fdtd_batch_synchronize(self.handle), "fdtd_batch_synchronize", self.lib)
 
+    def set_config(self, config):
+        """The batch's own launch configuration (``engine.LaunchConfig``); None: follow the
+        process defaults."""
+        words = config.words() if config is not None else None
+        check(self.lib.fdtd_batch_set_config(
+            self.handle, words.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            if words is not None else None), "fdtd_batch_set_config", self.lib)
+
+    def config(self):
+        from .engine import LaunchConfig
+
+        out = (ctypes.c_int32 * 12)()
+        check(self.lib.fdtd_batch_get_config(self.handle, out), "fdtd_batch_get_config", self.lib)
+        return LaunchConfig.from_words(list(out))
+
     def graph_stats(self):
         """(graphs captured, replays after their capture) over the batch's life."""
         out = (ctypes.c_int64 * 2)()

@@ -8,6 +8,7 @@ device stores x fastest, so every transfer transposes. Volumes are named as the 
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass
 from typing import Dict, Optional, Sequence, Tuple
 
 import numpy
@@ -265,6 +266,18 @@ class HipSimulation:
     def synchronize(self):
         self._check(self.lib.fdtd_synchronize(self.handle), "fdtd_synchronize")
 
+    def set_config(self, config: Optional["LaunchConfig"]):
+        """This simulation's own launch configuration; None: follow the process defaults."""
+        words = config.words() if config is not None else None
+        self._check(self.lib.fdtd_sim_set_config(self.handle, iptr(words) if words is not None
+                                                 else None), "fdtd_sim_set_config")
+
+    def config(self) -> "LaunchConfig":
+        """The launch configuration in force for this simulation."""
+        out = numpy.zeros(12, dtype=numpy.int32)
+        self._check(self.lib.fdtd_sim_get_config(self.handle, iptr(out)), "fdtd_sim_get_config")
+        return LaunchConfig.from_words(out)
+
     def graph_stats(self) -> Tuple[int, int]:
         """(graphs captured, replays after their capture) over this simulation's life."""
         out = (ctypes.c_int64 * 2)()
@@ -378,6 +391,36 @@ def copy_probe(nbytes: int, repeats: int = 20, mode: int = 0) -> float:
     check(_native.load().fdtd_copy_probe(int(nbytes), int(repeats), int(mode), fptr(out)),
           "fdtd_copy_probe")
     return float(out[0])
+
+
+@dataclass(frozen=True)
+class LaunchConfig:
+    """How one simulation's (or batch's) launches run: per fused pair (H, D) the shape
+    ``(cells per thread, workgroups per CU or 0, tile order, non-temporal stores, shell cells
+    per thread)``, graph replay on or off, and the flux transform (1, 2 or 4 MFMA tiles per
+    wave, 0 the vector kernel). ``fdtd_sim_set_config`` / ``fdtd_batch_set_config`` words."""
+    h: Tuple[int, int, int, bool, int] = (1, 8, 0, False, 1)
+    d: Tuple[int, int, int, bool, int] = (1, 8, 0, False, 1)
+    graphs: bool = True
+    flux_mfma: int = 1
+
+    def words(self) -> numpy.ndarray:
+        out = [int(v) for v in self.h] + [int(v) for v in self.d]
+        return numpy.array(out + [1 if self.graphs else 0, int(self.flux_mfma)], dtype=numpy.int32)
+
+    @classmethod
+    def from_words(cls, w) -> "LaunchConfig":
+        w = [int(v) for v in w]
+        h = (w[0], w[1], w[2], bool(w[3]), w[4])
+        d = (w[5], w[6], w[7], bool(w[8]), w[9])
+        return cls(h, d, bool(w[10]), w[11])
+
+    @classmethod
+    def of_shapes(cls, shapes, shell=(1, 1), graphs: bool = True, flux_mfma: int = 1):
+        """From (H shape, D shape) 4-tuples as the autotuner returns them."""
+        h = tuple(shapes[0][:4]) + (int(shell[0]),)
+        d = tuple(shapes[1][:4]) + (int(shell[1]),)
+        return cls(h, d, graphs, flux_mfma)
 
 
 def set_launch(vec: int, blocks_per_cu: int, xcd_remap: int, nt_stores: bool = False):
