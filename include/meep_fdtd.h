@@ -199,6 +199,20 @@ int fdtd_set_boundary(fdtd_sim *sim, int32_t axis, int32_t kind);
  * polarization (integrated) sources. */
 int fdtd_set_fold(fdtd_sim *sim, int32_t axis, int32_t kind, int32_t phase, int32_t far_row);
 
+/* Complex field storage as MEEP keeps it (two realnum parts, stepped with the same real
+ * coefficients; reference part_products.py:1-40): `re` and `im` are two simulations set up
+ * alike (grid, layer, materials, poles, folds), holding the real and the imaginary parts of
+ * every stored volume; a source's imaginary line is the real line of amplitudes (amp_im,
+ * -amp_re). kinds[axis] is the Bloch connection of a wrapping axis (0 none, 1 a copy, 2 a
+ * negation, 3 a product by the phase, whose parts are phase[2*axis], phase[2*axis+1] rounded
+ * to float32; boundaries.cpp:359-366): a curl or an off-diagonal row reading a neighbour's SAME
+ * stored index across the window edge (plane 0) reads the connected word -- conj(phase) below
+ * the window, phase above -- as (a*c) - (b*d), (a*d) + (b*c) with each operation rounded on
+ * its own (step.cpp:171-221; stepping._bloch_ghost_operand :2545-2586). Linked parts step only
+ * together, in one batch (fdtd_batch_create puts both in it). Refused with flux or DFT
+ * regions. */
+int fdtd_link_parts(fdtd_sim *re, fdtd_sim *im, const int32_t kinds[3], const float phase[6]);
+
 /* A conductivity: sigma (the reference's layout, float32, one volume) for component `axis` of
  * D (field 0, MEEP's D_conductivity) or B (field 1, B_conductivity); null or all zero removes it.
  * Stores condfac = 1 - sigma*dt2 (float32, dt2 = float32(dt)/2) and condinv = 1/(1 + sigma*dt/2)

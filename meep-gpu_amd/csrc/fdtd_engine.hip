@@ -134,6 +134,14 @@ struct SimDev {
     // a zero ghost past the top), 2 over periodic ones (one stored slot past the window); the
     // plane's phase (+1 even, -1 odd) and, for 2, the row the far ghost images
     int fold[3], fphase[3], frow[3];
+    // complex storage (MEEP's two realnum parts, stepped alike with the same real
+    // coefficients): which part this descriptor steps and the other part's descriptor in the
+    // same launch; per axis the Bloch connection of the word read across the window edge (0
+    // none, 1 copy, 2 negation, 3 a product by the phase ph, its parts rounded to float32)
+    int part;
+    const SimDev *other;
+    int pkind[3];
+    float ph[3][2];
     // The work split of every fused launch: the interior box [x0,x1) x [j0,j1) x [k0,k1) holds
     // cells on no layer line of either lattice (x0, x1 multiples of 4), where the layer forms
     // are exactly the lossless ones and every H read is B's word; every other run of V cells
@@ -582,6 +590,27 @@ __device__ inline bool ghost_at(const SimDev &s, const Run &q, int family, int c
     return g;
 }
 
+// MEEP's connection of the word a stencil reads at the SAME stored index across a Bloch-phased
+// axis's window edge (plane 0; the engine stores MEEP's owned window there): below the window
+// conj(eikna), above it eikna (stepping._bloch_ghost_operand :2545-2586). A phase within 1e-13
+// of +1 is a copy, of -1 a negation (boundaries.cpp:359-366); otherwise complex<realnum>(phase)
+// times the word, (a*c) - (b*d) for the real part and (a*d) + (b*c) for the imaginary one,
+// each operation rounded on its own (step.cpp:183-192): that reads the other part's word.
+__device__ inline float connect_value(const SimDev &s, int axis, bool below, float mine,
+                                      float other) {
+    const int k = s.pkind[axis];
+    if (k == 2) return -mine;
+    if (k != 3) return mine;
+    const float a = s.ph[axis][0], b = below ? -s.ph[axis][1] : s.ph[axis][1];
+    const float x = a * mine, y = b * other;
+    return s.part == 0 ? x - y : x + y;
+}
+
+__device__ inline float connect_word(const SimDev &s, int axis, bool below, float mine,
+                                     const float *other, int at) {
+    return connect_value(s, axis, below, mine, s.pkind[axis] == 3 ? gp(other)[at] : 0.0f);
+}
+
 // Past the top of a metallic axis, walled or folded, a +1 curl operand reads zero.
 __device__ inline bool zero_top(const SimDev &s, int axis) {
     return s.metal[axis] || s.fold[axis] == 1;
@@ -629,8 +658,15 @@ __device__ inline void hpair_component(const SimDev &s, const Clock &ck, const R
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], e[first].v[m],
-                             en[first][fa].v[m], e[second].v[m], en[second][sa].v[m]);
+        float g1 = e[first].v[m], g2 = e[second].v[m];
+        if constexpr (GEN) {  // the unshifted partners below a phased axis's window edge
+            if (s.pkind[fa] && coord_of(q, fa, m) == 0)
+                g1 = connect_word(s, fa, true, g1, s.other ? s.other->E[first] : nullptr, q.at + m);
+            if (s.pkind[sa] && coord_of(q, sa, m) == 0)
+                g2 = connect_word(s, sa, true, g2, s.other ? s.other->E[second] : nullptr, q.at + m);
+        }
+        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], g1, en[first][fa].v[m], g2,
+                             en[second][sa].v[m]);
         if constexpr (GEN) {
             if (on_wall(s, q, C, m) || ghost_at(s, q, 1, C, m)) curl = 0.0f;
             // a condinv word of 0 marks a cell whose chunk keeps no conductivity array: it
@@ -732,6 +768,12 @@ __global__ void __launch_bounds__(kThreads) hpair_kernel(const SimDev *__restric
 // ------------------------------------------------------------------------------------------
 // D pair: step_D -> electric deposit -> update_E -> update_P.
 // ------------------------------------------------------------------------------------------
+// The array the D curl reads component c of H from at lane m: H on c's own-axis layer rows, B
+// elsewhere (as ld_hb chooses).
+__device__ inline const float *hb_of(const SimDev &s, const Run &q, int c, int m) {
+    return member(s, 0, c, coord_of(q, c, m)) ? s.H[c] : s.B[c];
+}
+
 template <int V, int C, bool PML, bool GEN>
 __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const Run &q,
                                         const VF<V> (&h)[3], const VF<V> (&hn)[3][3], VF<V> d) {
@@ -740,8 +782,17 @@ __device__ inline VF<V> dcurl_component(const SimDev &s, const Clock &ck, const 
     constexpr int dsig = (C + 1) % 3, dsigu = (C + 2) % 3;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], h[first].v[m],
-                             hn[first][fa].v[m], h[second].v[m], hn[second][sa].v[m]);
+        float g1 = h[first].v[m], g2 = h[second].v[m];
+        if constexpr (GEN) {  // the unshifted partners above a phased axis's window edge
+            if (s.pkind[fa] && coord_of(q, fa, m) == 0)
+                g1 = connect_word(s, fa, false, g1, s.other ? hb_of(*s.other, q, first, m) : nullptr,
+                                  q.at + m);
+            if (s.pkind[sa] && coord_of(q, sa, m) == 0)
+                g2 = connect_word(s, sa, false, g2, s.other ? hb_of(*s.other, q, second, m) : nullptr,
+                                  q.at + m);
+        }
+        float curl = curl_of(s.dtdx, s.present[fa], s.present[sa], g1, hn[first][fa].v[m], g2,
+                             hn[second][sa].v[m]);
         if constexpr (GEN) {
             if (on_wall(s, q, (C + 1) % 3, m) || on_wall(s, q, (C + 2) % 3, m) ||
                 ghost_at(s, q, 0, C, m))
@@ -1444,7 +1495,33 @@ __device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, i
     return dir > 0 ? q.zp : q.zm;
 }
 
-// The row of stepping._offdiagonal_terms (stepping.py:1594-1687), summed in double.
+// The off-diagonal row's partner pair of one part at cell c3 (padded index `at`): D - P of
+// partner `pa` there, read across a phased partner axis's window edge as the word above it,
+// plus its lower neighbour along `pa` -- zero past a metallic wall, the mirror ghost on a
+// folded axis (the parity of D_pa about its own plane, -phase, times stored row 2), else the
+// periodic neighbour (stepping._offdiagonal_terms :1594-1687 with _shift_down :2485).
+__device__ inline float offdiag_pair(const SimDev &s, const SimDev *o, int ppar, int pa, int at,
+                                     const int c3[3], int down) {
+    float v = dmp_at(s, ppar, pa, at);
+    if (s.pkind[pa] && c3[pa] == 0)
+        v = connect_value(s, pa, false, v, (s.pkind[pa] == 3 && o) ? dmp_at(*o, ppar, pa, at) : 0.0f);
+    float lower;
+    if (s.metal[pa] && c3[pa] == 0) {
+        lower = 0.0f;
+    } else if (s.fold[pa] && c3[pa] == 0) {
+        const int stride = pa == 0 ? 1 : (pa == 1 ? s.pitch : s.pitch * s.n[1]);
+        const float m = dmp_at(s, ppar, pa, at + 2 * stride);
+        lower = s.fphase[pa] > 0 ? -m : m;
+    } else {
+        lower = dmp_at(s, ppar, pa, at + down);
+    }
+    return v + lower;
+}
+
+// The row of stepping._offdiagonal_terms (stepping.py:1594-1687), summed in double. On a folded
+// or metallic own axis the product above the top row reads zero (_shift_up's zero face); at
+// plane 0 of a phased own axis the product here is the word below the window, which for a
+// product connection is formed from both parts' products.
 template <int V, int C, bool PML>
 __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> p[kMaxPoles];
@@ -1461,34 +1538,34 @@ __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Ru
             continue;
         }
         double row = (double)(dmp.v[m] * iv.v[m]);
+        const int c3[3] = {i, q.j, q.k};
 #pragma unroll
         for (int o = 1; o <= 2; ++o) {
             const int pa = (C + o) % 3;
             const float *U = o == 1 ? u1 : u2;
             if (!U) continue;
             // u[r]*(g[r] + g[r - s_partner]) at r = here and r = here + s_own; a shift along
-            // the own axis leaves the coordinate along the partner's axis unchanged. Past a
-            // metallic wall the partner's lower neighbour and the product above read zero, and
-            // the summed term is zero on a wall plane of a metallic axis other than the own
-            // one (stepping.py:1594-1687, _mask_metallic_wall_coupling :1718-1745)
-            const bool low_pa = s.metal[pa] && coord_of(q, pa, m) == 0;
-            const bool top_c = s.metal[C] && coord_of(q, C, m) == s.n[C] - 1;
+            // the own axis leaves the coordinate along the partner's axis unchanged
+            const bool top_c = (s.metal[C] || s.fold[C]) && c3[C] == s.n[C] - 1;
             const int up = at + step_off(s, q, i, C, +1);
             const int iu = (C == 0) ? ((i + 1 < s.n[0]) ? i + 1 : 0) : i;
+            int u3[3] = {iu, q.j, q.k};
+            if (C == 1) u3[1] = (q.j + 1 < s.n[1]) ? q.j + 1 : 0;
+            if (C == 2) u3[2] = (q.k + 1 < s.n[2]) ? q.k + 1 : 0;
             const int down0 = step_off(s, q, i, pa, -1);
             const int down1 = step_off(s, q, iu, pa, -1);
-            const float pair0 = dmp_at(s, ck.ppar, pa, at) +
-                                (low_pa ? 0.0f : dmp_at(s, ck.ppar, pa, at + down0));
-            const float prod0 = gp(U)[at] * pair0;
-            float prod1 = 0.0f;
-            if (!top_c) {
-                const float pair1 = dmp_at(s, ck.ppar, pa, up) +
-                                    (low_pa ? 0.0f : dmp_at(s, ck.ppar, pa, up + down1));
-                prod1 = gp(U)[up] * pair1;
+            float prod0 = gp(U)[at] * offdiag_pair(s, s.other, ck.ppar, pa, at, c3, down0);
+            if (s.pkind[C] && c3[C] == 0) {
+                float other = 0.0f;
+                if (s.pkind[C] == 3 && s.other)
+                    other = gp(U)[at] * offdiag_pair(*s.other, &s, ck.ppar, pa, at, c3, down0);
+                prod0 = connect_value(s, C, true, prod0, other);
             }
+            float prod1 = 0.0f;
+            if (!top_c) prod1 = gp(U)[up] * offdiag_pair(s, s.other, ck.ppar, pa, up, u3, down1);
             float summed = prod0 + prod1;
-            if ((s.metal[(C + 1) % 3] && coord_of(q, (C + 1) % 3, m) == 0) ||
-                (s.metal[(C + 2) % 3] && coord_of(q, (C + 2) % 3, m) == 0))
+            if ((s.metal[(C + 1) % 3] && c3[(C + 1) % 3] == 0) ||
+                (s.metal[(C + 2) % 3] && c3[(C + 2) % 3] == 0))
                 summed = 0.0f;
             row = row + 0.25 * (double)summed;
         }
@@ -2145,7 +2222,8 @@ struct Group {
 // more than kLosslessPoles poles or a polarization source?
 bool needs_general(const SimDev &h) {
     return h.cond != 0 || h.metal[0] || h.metal[1] || h.metal[2] || h.npol > kLosslessPoles ||
-           h.npsrc > 0 || h.fold[0] || h.fold[1] || h.fold[2];
+           h.npsrc > 0 || h.fold[0] || h.fold[1] || h.fold[2] || h.pkind[0] || h.pkind[1] ||
+           h.pkind[2];
 }
 
 // The one inverse-epsilon storage all three components of a descriptor use, or 3 (mixed).
@@ -2379,6 +2457,7 @@ struct fdtd_sim {
     LaunchConfig own_cfg;  // the simulation's own launch configuration, when has_cfg
     bool has_cfg = false;
     const LaunchConfig *config() const { return has_cfg ? &own_cfg : &g_defaults; }
+    fdtd_sim *partner = nullptr;  // the other part of complex storage (fdtd_link_parts)
     const int4 *foldw = nullptr;  // the ghost planes of the folded axes (fold_kernel items)
     int nfoldw = 0;
     std::vector<int4> fold_host;      // the same items, for a batch's concatenation
@@ -3095,6 +3174,7 @@ void fdtd_sim_destroy(fdtd_sim *s) {
         (void)fail(FDTD_ESTATE, "simulation is stepped by a batch: released when the batch is destroyed");
         return;
     }
+    if (s->partner) s->partner->partner = nullptr;  // the other part is stepped alone no more
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->graph.reset();  // before the memory its launches read is freed
     for (hipEvent_t e : s->timed_ev) (void)hipEventDestroy(e);
@@ -3695,6 +3775,39 @@ int fdtd_batch_get_config(fdtd_batch *b, int32_t out[12]) {
     return FDTD_OK;
 }
 
+int fdtd_link_parts(fdtd_sim *re, fdtd_sim *im, const int32_t kinds[3], const float phase[6]) {
+    if (!re || !im || re == im || !kinds || !phase) return fail(FDTD_EINVAL, "fdtd_link_parts: bad argument");
+    if (re->in_batch || im->in_batch) return fail(FDTD_ESTATE, "a simulation is stepped by a batch");
+    if (re->partner || im->partner) return fail(FDTD_ESTATE, "a simulation is already linked");
+    for (int a = 0; a < 3; ++a) {
+        if (kinds[a] < 0 || kinds[a] > 3) return fail(FDTD_EINVAL, "connection kind %d on axis %d", kinds[a], a);
+        if (re->host.n[a] != im->host.n[a]) return fail(FDTD_EINVAL, "the parts' grids differ");
+        if (kinds[a] && (re->host.fold[a] || re->host.metal[a] || !re->host.present[a]))
+            return fail(FDTD_EINVAL, "axis %d carries a Bloch connection but does not wrap", a);
+    }
+    if (!re->planes.empty() || !im->planes.empty())
+        return fail(FDTD_EINVAL, "fdtd_link_parts: flux and DFT regions on complex storage are not "
+                                 "supported");
+    fdtd_sim *parts[2] = {re, im};
+    for (int p = 0; p < 2; ++p) {
+        int rc = parts[p]->pull_desc();
+        if (rc) return rc;
+        SimDev &h = parts[p]->host;
+        h.part = p;
+        h.other = nullptr;
+        for (int a = 0; a < 3; ++a) {
+            h.pkind[a] = kinds[a];
+            h.ph[a][0] = phase[2 * a];
+            h.ph[a][1] = phase[2 * a + 1];
+        }
+        parts[p]->graph.reset();
+        parts[p]->dirty = true;
+    }
+    re->partner = im;
+    im->partner = re;
+    return FDTD_OK;
+}
+
 int fdtd_graph_stats(fdtd_sim *s, int64_t out[2]) {
     if (!s || !out) return fail(FDTD_EINVAL, "null argument");
     out[0] = s->graph.captures;
@@ -3859,6 +3972,8 @@ int fdtd_prime_e(fdtd_sim *s) {
 }
 
 int fdtd_step_h(fdtd_sim *s) {
+    if (s && s->partner)
+        return fail(FDTD_ESTATE, "a part of complex storage steps only beside its other part, in a batch");
     if (!s) return fail(FDTD_EINVAL, "null argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->ready();
@@ -3867,6 +3982,8 @@ int fdtd_step_h(fdtd_sim *s) {
 }
 
 int fdtd_step_d(fdtd_sim *s) {
+    if (s && s->partner)
+        return fail(FDTD_ESTATE, "a part of complex storage steps only beside its other part, in a batch");
     if (!s) return fail(FDTD_EINVAL, "null argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->ready();
@@ -3893,6 +4010,8 @@ int fdtd_flush(fdtd_sim *s) {
 }
 
 int fdtd_run(fdtd_sim *s, int32_t nsteps) {
+    if (s && s->partner)
+        return fail(FDTD_ESTATE, "a part of complex storage steps only beside its other part, in a batch");
     if (!s || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->ready();
@@ -3926,6 +4045,8 @@ static int add_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], in
     if (s->host.fold[0] || s->host.fold[1] || s->host.fold[2])
         return fail(FDTD_EINVAL, "flux and DFT regions on a folded grid are not supported: the "
                                  "reference's read-out weighs a folded plane by its images");
+    if (s->partner)
+        return fail(FDTD_EINVAL, "flux and DFT regions on complex storage are not supported");
     s->graph.reset();  // a captured sequence samples the regions it was captured with
     int cells = 1;
     for (int a = 0; a < 3; ++a) {
@@ -4139,6 +4260,8 @@ int fdtd_flux_spectrum_noalloc(fdtd_sim *s, int32_t plane, const double *weights
 }
 
 int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
+    if (s && s->partner)
+        return fail(FDTD_ESTATE, "a part of complex storage steps only beside its other part, in a batch");
     if (!s || repeats < 1 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     if (!s->planes.empty()) return fail(FDTD_ESTATE, "fdtd_time_pairs steps a simulation without flux planes");
@@ -4175,6 +4298,8 @@ int fdtd_time_pairs(fdtd_sim *s, int32_t repeats, float out_ms[2]) {
 }
 
 int fdtd_run_timed(fdtd_sim *s, int32_t nsteps, float out_ms[3]) {
+    if (s && s->partner)
+        return fail(FDTD_ESTATE, "a part of complex storage steps only beside its other part, in a batch");
     if (!s || nsteps < 0 || !out_ms) return fail(FDTD_EINVAL, "bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
     int rc = s->ready();
@@ -4361,6 +4486,20 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     b->group.inv_mode = inv_mode_of(b->sims[0]->host);
     for (auto *s : b->sims)
         if (inv_mode_of(s->host) != b->group.inv_mode) b->group.inv_mode = 3;
+    // the parts of complex storage read each other's words: point each at its partner's copy
+    for (int q = 0; q < count; ++q) {
+        descs[q].other = nullptr;
+        fdtd_sim *mate = sims[q]->partner;
+        if (!mate) continue;
+        int r = 0;
+        while (r < count && sims[r] != mate) ++r;
+        if (r == count) {
+            delete b;
+            return fail(FDTD_EINVAL, "simulation %d is a part of complex storage whose other part "
+                                     "is not in the batch", q);
+        }
+        descs[q].other = (const SimDev *)(uintptr_t)r;  // an index until the copy exists
+    }
     b->group.nsims = count;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return fail(FDTD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
@@ -4370,6 +4509,8 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     };
     if (hipMalloc((void **)&b->group.dev, sizeof(SimDev) * count) != hipSuccess) return cleanup(FDTD_ENOMEM, "batch descriptors");
     b->allocs.push_back(b->group.dev);
+    for (int q = 0; q < count; ++q)
+        if (sims[q]->partner) descs[q].other = b->group.dev + (uintptr_t)descs[q].other;
     if (hipMemcpy(b->group.dev, descs.data(), sizeof(SimDev) * count, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(FDTD_EHIP, "batch descriptor upload");
     // every simulation's planes, re-pointed at the batch's descriptors

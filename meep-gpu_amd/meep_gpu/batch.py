@@ -41,17 +41,21 @@ class BatchedRun:
         if len(libs) > 1:
             raise ValueError("a batch steps simulations of one subnormal policy")
         self.lib = self.drivers[0].sim.lib if self.drivers[0].sim is not None else _native.load()
-        handles = (ctypes.c_void_p * len(self.drivers))()
-        for q, drv in enumerate(self.drivers):
+        # a run on complex storage brings both of its parts (``sims``), stepped side by side
+        self.members = []
+        for drv in self.drivers:
             if drv.sim is None:
                 raise ValueError("freeze every driver before batching it")
-            handles[q] = drv.sim.handle.value
+            self.members.extend(getattr(drv, "sims", None) or [drv.sim])
+        handles = (ctypes.c_void_p * len(self.members))()
+        for q, sim in enumerate(self.members):
+            handles[q] = sim.handle.value
         out = ctypes.c_void_p()
-        check(self.lib.fdtd_batch_create(handles, len(self.drivers), ctypes.byref(out)),
+        check(self.lib.fdtd_batch_create(handles, len(self.members), ctypes.byref(out)),
               "fdtd_batch_create", self.lib)
         self.handle = out
-        for drv in self.drivers:
-            drv.sim._batch = weakref.ref(self)
+        for sim in self.members:
+            sim._batch = weakref.ref(self)
 
     @property
     def cells(self) -> int:
@@ -93,9 +97,8 @@ class BatchedRun:
         if self.handle is not None and self.handle.value:
             self.lib.fdtd_batch_destroy(self.handle)
         self.handle = None
-        for drv in self.drivers:
-            if drv.sim is not None:
-                drv.sim._batch = None
+        for sim in getattr(self, "members", ()):
+            sim._batch = None
 
     def __del__(self):
         try:

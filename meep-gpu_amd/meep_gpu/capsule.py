@@ -21,8 +21,11 @@ What is read, and how:
   conductivity and the sources' point amplitudes go to the engine as the capsule holds them,
   and the PML membership lines come from the lifted chunk division (chunk_map.py:497-570).
 
-What is refused, by name (``CapsuleRefusal``): cylindrical and complex (Bloch) storage,
-chi2/chi3, off-diagonal chi1inv, monitors, a chunk division no membership line describes,
+Complex storage (a Bloch phase, ``force_complex_fields``) is taken as two real parts,
+``volumes`` and ``volumes_im``, with each axis's Bloch connection (``bloch_kinds``,
+``bloch_phase``: MEEP's copy, negation or product by the phase, boundaries.cpp:359-366).
+
+What is refused, by name (``CapsuleRefusal``): cylindrical storage, chi2/chi3, monitors, a chunk division no membership line describes,
 more poles than the engine holds, a source without point tables, a currents table whose
 times are not the driver's (electric sources at ``n*dt + 0.5*dt``, magnetic at ``n*dt``;
 driver.py:3536, :3565), and on a mirror-folded grid a conductivity and MEEP's ``get_array``
@@ -223,6 +226,15 @@ class LiftedState:
     #: mirror planes ((axis 0..2, phase +-1), ...) and the full (unfolded) cell counts
     symmetry: Tuple[Tuple[int, int], ...] = ()
     shape_full: Tuple[int, int, int] = None
+    #: complex storage: the imaginary parts of the stored volumes (``volumes`` holds the real
+    #: ones), and per axis the Bloch connection (0 none, 1 copy, 2 negate, 3 a product) and
+    #: the phase's two parts (float64; the engine rounds them to float32)
+    complex: bool = False
+    volumes_im: Dict[str, numpy.ndarray] = field(default_factory=dict)
+    bloch_kinds: Tuple[int, int, int] = (0, 0, 0)
+    bloch_phase: Tuple[Tuple[float, float], ...] = ((1.0, 0.0),) * 3
+    #: off-diagonal chi1inv rows, {"Ex": {"Ey": float32 volume}}
+    offdiag: Dict[str, Dict[str, numpy.ndarray]] = field(default_factory=dict)
 
     def grid(self) -> Grid:
         """The engine grid; refuses when its dx, dt or cell counts are not the lifted words."""
@@ -317,13 +329,21 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
                      if m is not None)
     if grid["cylindrical"]:
         raise CapsuleRefusal("cylindrical coordinates are not built")
-    if grid["has_bloch"] or fields["force_complex_fields"] or driver.get("force_complex_fields"):
-        raise CapsuleRefusal("complex (Bloch) field storage is not built")
+    complex_fields = bool(fields["force_complex_fields"] or driver.get("force_complex_fields")
+                          or grid["has_bloch"])
+    kinds, phases = [0, 0, 0], [(1.0, 0.0)] * 3
+    for axis, phase in enumerate(grid["bloch_phases"] or (None, None, None)):
+        if phase is None:
+            continue
+        phase = complex(phase)
+        kinds[axis], phases[axis] = bloch_connection(phase), (phase.real, phase.imag)
+        if kinds[axis] == 3 and not complex_fields:
+            raise CapsuleRefusal("a Bloch phase on real storage")
     if float(driver["_chi2"]) != 0.0 or float(driver["_chi3"]) != 0.0 \
             or fields["_chi2_components"] or fields["_chi3_components"]:
         raise CapsuleRefusal("chi2/chi3 nonlinearity is not built")
-    if fields["_chi1inv_offdiagonal"]:
-        raise CapsuleRefusal("off-diagonal chi1inv in a lifted run is not built")
+    offdiag = {row: {partner: _volume(vol) for partner, vol in partners.items()}
+               for row, partners in (fields["_chi1inv_offdiagonal"] or {}).items()}
     if driver["_flux_monitors"] or driver["_dft_monitors"]:
         raise CapsuleRefusal("a capsule carries no monitors")
     if any(fields.get(f"f_bfast_{c}") is not None for c in ("Bx", "By", "Bz", "Dx", "Dy", "Dz")):
@@ -388,17 +408,30 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
             raise CapsuleRefusal(f"source {src.cls} has points outside the grid")
         sources.append(LiftedSource(component, index, amps))
 
-    volumes = {}
+    volumes, volumes_im = {}, {}
+
+    def put(name, value):
+        array = numpy.asarray(value)
+        if array.dtype == numpy.complex64:
+            if not complex_fields:
+                raise CapsuleRefusal(f"{name} is complex on a real-storage run")
+            volumes[name] = numpy.ascontiguousarray(array.real)
+            volumes_im[name] = numpy.ascontiguousarray(array.imag)
+            return
+        volumes[name] = _volume(array)
+        if complex_fields:
+            raise CapsuleRefusal(f"{name} is {array.dtype} on a complex-storage run")
+
     for prefix in ("B", "D", "E", "H", "fu_B", "fu_D", "f_w_E", "f_w_H", "f_cond_D", "f_cond_B"):
         for a in _AXES:
             value = fields.get(prefix + a)
             if value is not None:
-                volumes[prefix + a] = _volume(value)
+                put(prefix + a, value)
     for k, state in enumerate(fields["polarizations"]):
         for c, value in state["P"].items():
-            volumes[f"P[{k}].{c}"] = _volume(value)
+            put(f"P[{k}].{c}", value)
         for c, value in state["P_prev"].items():
-            volumes[f"P_prev[{k}].{c}"] = _volume(value)
+            put(f"P_prev[{k}].{c}", value)
 
     division = None  # the per-chunk facts of a folded division are not built (no
     if not any(_folded_frame(frame) for frame in layout["frames"]):  # conductivity there)
@@ -419,7 +452,23 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         condinv=side(fields["_condinv"]), absorber_sigma=absorber_sigma, absorber_layers=layers,
         poles=poles, sources=sources, volumes=volumes, step=int(driver["step_count"]),
         h_separated=bool(fields["h_separated"]), absent=tuple(fields.get("absent_components") or ()),
-        symmetry=symmetry, shape_full=tuple(int(n) for n in grid["shape_full"]))
+        symmetry=symmetry, shape_full=tuple(int(n) for n in grid["shape_full"]),
+        complex=complex_fields, volumes_im=volumes_im, bloch_kinds=tuple(kinds),
+        bloch_phase=tuple(phases), offdiag=offdiag)
+
+
+#: ``phase_isclose`` (reference part_products.py CONNECT_TOLERANCE, boundaries.cpp:359-360)
+CONNECT_TOLERANCE = 1e-13
+
+
+def bloch_connection(phase: complex) -> int:
+    """MEEP's connection kind for a Bloch phase: 1 a copy (within 1e-13 of +1), 2 a negation
+    (of -1), 3 a product by the phase."""
+    if abs(phase.imag) < CONNECT_TOLERANCE and abs(phase.real - 1.0) < CONNECT_TOLERANCE:
+        return 1
+    if abs(phase.imag) < CONNECT_TOLERANCE and abs(phase.real + 1.0) < CONNECT_TOLERANCE:
+        return 2
+    return 3
 
 
 def read_capsule(path, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedState:
@@ -498,28 +547,39 @@ def _source_boxes(src: LiftedSource):
 
 class LiftedRun:
     """A lifted state resident on the GPU: every word pasted, stepped by the engine built with
-    the given subnormal policy ("keep" or "flush", the identity set's ``mode``)."""
+    the given subnormal policy ("keep" or "flush", the identity set's ``mode``). Complex
+    storage is two linked simulations, the real and the imaginary parts (``fdtd_link_parts``),
+    stepped together as a batch of two."""
 
     def __init__(self, state: LiftedState, currents: Sequence[numpy.ndarray],
                  subnormals: str = "keep"):
-        from .engine import HipSimulation
+        from .engine import HipSimulation, link_parts
 
         self.state = state
         grid = state.grid()
-        self.sim = HipSimulation(grid, state.pml, subnormals=subnormals)
+        self.sims = []
         try:
-            self._configure(state, currents)
+            for part in range(2 if state.complex else 1):
+                self.sims.append(HipSimulation(grid, state.pml, subnormals=subnormals))
+                self._configure(self.sims[-1], state, currents, part)
+            if state.complex:
+                link_parts(self.sims[0], self.sims[1], state.bloch_kinds, state.bloch_phase)
         except Exception:
-            self.sim.close()
+            for sim in self.sims:
+                sim.close()
             raise
+        self.sim = self.sims[0]
 
-    def _configure(self, state: LiftedState, currents):
-        sim = self.sim
+    @staticmethod
+    def _configure(sim, state: LiftedState, currents, part: int = 0):
         for axis in range(3):
             if state.metallic[axis] and sim.grid.folds[axis] is None:  # a fold carries its own
                 sim.set_boundary(axis, "metallic")
         for c, vol in state.inv_eps.items():
             sim.set_inverse_epsilon(c, vol)
+        for row, partners in state.offdiag.items():
+            for partner, vol in partners.items():
+                sim.set_offdiagonal(row, partner, vol)
         for pole in state.poles:
             sim.add_pole(None, pole.sigma, constants=pole.constants)
         for name, vol in state.conductivity.items():
@@ -527,10 +587,14 @@ class LiftedRun:
         for src, table in zip(state.sources, currents):
             axis = "xyz".index(src.component[1])
             for lo, amp in _source_boxes(src):
-                sim.add_source(src.magnetic, axis, lo, (1, 1, 1), numpy.array([amp.real]),
-                               numpy.array([amp.imag]), table, state.dt)
-        stores_e = state.pml is not None or bool(state.poles)
-        for name, vol in state.volumes.items():
+                # the imaginary part takes MEEP's second line, (amp_re*c_im) + (amp_im*c_re):
+                # the real line of the amplitude (amp_im, -amp_re) (step.cpp:309-314)
+                re, im = (amp.real, amp.imag) if part == 0 else (amp.imag, -amp.real)
+                sim.add_source(src.magnetic, axis, lo, (1, 1, 1), numpy.array([re]),
+                               numpy.array([im]), table, state.dt)
+        stores_e = state.pml is not None or bool(state.poles) or bool(state.offdiag)
+        volumes = state.volumes if part == 0 else state.volumes_im
+        for name, vol in volumes.items():
             if not numpy.any(vol):
                 continue  # the engine's volumes start at +0
             if name[0] == "H" and state.pml is None:
@@ -563,15 +627,28 @@ class LiftedRun:
         return self.sim.step_count
 
     def advance(self, steps: int):
-        if steps > 0:
+        if steps <= 0:
+            return
+        if len(self.sims) == 1:
             self.sim.run(int(steps))
+            return
+        import types
 
-    def words(self, component: str) -> numpy.ndarray:
-        """One component's stored words in the reference layout (H is B's word off the
-        layer's H rows, and everywhere without an absorber)."""
+        from .batch import BatchedRun
+
+        batch = BatchedRun([types.SimpleNamespace(sim=s, grid=s.grid) for s in self.sims])
+        try:
+            batch.run(int(steps))
+            batch.synchronize()
+        finally:
+            batch.close()
+
+    def words(self, component: str, part: int = 0) -> numpy.ndarray:
+        """One component's stored words (of one part) in the reference layout (H is B's word
+        off the layer's H rows, and everywhere without an absorber)."""
         if component[0] == "H" and self.state.pml is None:
             component = "B" + component[1]
-        return self.sim.download(component)
+        return self.sims[part].download(component)
 
     def meep_array(self, component: str) -> numpy.ndarray:
         """MEEP's whole-cell ``get_array(component)`` of the current words, with the lifted
@@ -598,7 +675,8 @@ class LiftedRun:
                               state.pml is not None, self.step_count)
 
     def close(self):
-        self.sim.close()
+        for sim in self.sims:
+            sim.close()
 
 
 def compare_reader(run, digests: DigestSet, step: int) -> List[str]:
@@ -628,12 +706,19 @@ def ghost_cells(grid: Grid, component: str) -> numpy.ndarray:
 
 def compared_words(run, component: str) -> numpy.ndarray:
     """One component's words on the cells MEEP stores: every stored cell but the mirror
-    ghosts, in C order (x, y, z)."""
-    words = run.words(component)
+    ghosts, in C order (x, y, z); a complex component's two parts interleaved per cell (as
+    complex64 lays them out, reference meep_identity.py:18-24)."""
     grid = run.grid
-    if not any(f is not None for f in grid.folds):
-        return words
-    return words[~ghost_cells(grid, component)]
+    folded = any(f is not None for f in grid.folds)
+    keep = ~ghost_cells(grid, component) if folded else None
+    parts = [run.words(component)]
+    if getattr(run.state, "complex", False):
+        parts.append(run.words(component, part=1))
+    if keep is not None:
+        parts = [p[keep] for p in parts]
+    if len(parts) == 1:
+        return parts[0]
+    return numpy.stack([p.reshape(-1) for p in parts], axis=-1).reshape(-1)
 
 
 def compare_step(run, digests: DigestSet, step: int) -> List[str]:

@@ -423,6 +423,18 @@ class LaunchConfig:
         return cls(h, d, graphs, flux_mfma)
 
 
+def link_parts(re: "HipSimulation", im: "HipSimulation", kinds: Sequence[int],
+               phases: Sequence[Tuple[float, float]]):
+    """Make two simulations the real and imaginary parts of one complex storage
+    (``fdtd_link_parts``): per axis the Bloch connection kind (0 none, 1 copy, 2 negate,
+    3 a product) and the phase's parts, rounded here to float32 (MEEP's complex<realnum>)."""
+    k = numpy.asarray([int(v) for v in kinds], dtype=numpy.int32)
+    ph = numpy.asarray([numpy.float32(x) for pair in phases for x in pair], dtype=numpy.float32)
+    check(re.lib.fdtd_link_parts(re.handle, im.handle, iptr(k),
+                                 ph.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+          "fdtd_link_parts", re.lib)
+
+
 def set_launch(vec: int, blocks_per_cu: int, xcd_remap: int, nt_stores: bool = False):
     """Process-wide launch shape: cells per thread, workgroups per CU (0 = one per tile), tile
     order over the XCDs (0 plain, 1 contiguous eighths, sb >= 2 super-blocks of sb tiles),

@@ -35,7 +35,9 @@ class _OGrid(ctypes.Structure):
                 ("pml", ctypes.c_int),
                 ("kms", (_FP * 3) * 2), ("sinv", (_FP * 3) * 2), ("kps", (_FP * 3) * 2),
                 ("mem", (_U8P * 3) * 2), ("metal", ctypes.c_int * 3),
-                ("fold", ctypes.c_int * 3)]
+                ("fold", ctypes.c_int * 3), ("fphase", ctypes.c_int * 3),
+                ("pkind", ctypes.c_int * 3), ("ph", (ctypes.c_float * 2) * 3),
+                ("part", ctypes.c_int)]
 
 
 MAX_POLES = 8  # ORACLE_MAX_POLES of the C restatement
@@ -150,6 +152,12 @@ class OracleSim:
         # per axis: None, or a mirror fold (phase, kind, far row): kind 1 over metallic
         # boundaries, 2 over periodic ones with the far ghost imaging stored row `far row`
         self.folds: Tuple[Optional[Tuple[int, int, Optional[int]]], ...] = (None, None, None)
+        # Bloch connections per axis (0 none, 1 copy, 2 negate, 3 a product by ph[axis]) and,
+        # for complex storage, which part this is and the other part's run
+        self.bloch_kinds = (0, 0, 0)
+        self.bloch_ph = ((1.0, 0.0),) * 3
+        self.part = 0
+        self.other: Optional["OracleSim"] = None
         self.condfac: Dict[str, numpy.ndarray] = {}  # "Dx".."Bz": MEEP's conductivity words
         self.condinv: Dict[str, numpy.ndarray] = {}
         # coefficient vectors and membership lines [half][axis]
@@ -354,6 +362,11 @@ class OracleSim:
         for a in range(3):
             g.metal[a] = 1 if self.metal[a] else 0
             g.fold[a] = 0 if self.folds[a] is None else self.folds[a][1]
+            g.fphase[a] = 0 if self.folds[a] is None else int(self.folds[a][0])
+            g.pkind[a] = int(self.bloch_kinds[a])
+            g.ph[a][0] = float(numpy.float32(self.bloch_ph[a][0]))
+            g.ph[a][1] = float(numpy.float32(self.bloch_ph[a][1]))
+        g.part = int(self.part)
         for half in (0, 1):
             for a in range(3):
                 kms, sinv, kps, mem = self._vec[(half, a)]
@@ -384,23 +397,31 @@ class OracleSim:
         return x
 
     # --- sub-steps ------------------------------------------------------------------------
+    def _e_sources(self):
+        if self.stores_E:
+            return self._triple("E"), []
+        arrays = [self.get_E("E" + c) for c in COMPONENTS]
+        return (_FP * 3)(*[_fp(a) for a in arrays]), arrays
+
     def step_B(self):
         L = lib()
         g = self._grid()
         fu = self._triple("fu_B") if self.pml else None
-        src = self._triple("E") if self.stores_E else None
-        if src is None:
-            src_arrays = [self.get_E("E" + c) for c in COMPONENTS]
-            src = (_FP * 3)(*[_fp(a) for a in src_arrays])
-        L.oracle_step_curl(ctypes.byref(g), 0, self._triple("B"), src, fu, *self._conductive("B"))
+        src, keep = self._e_sources()
+        other, keep_o = self.other._e_sources() if self.other is not None else (None, [])
+        L.oracle_step_curl(ctypes.byref(g), 0, self._triple("B"), src, fu, *self._conductive("B"),
+                           other)
 
     def step_D(self):
         L = lib()
         g = self._grid()
         fu = self._triple("fu_D") if self.pml else None
+        other = None
+        if self.other is not None:
+            other = self.other._triple("H") if self.other.pml else self.other._triple("B")
         L.oracle_step_curl(ctypes.byref(g), 1, self._triple("D"),
                            self._triple("H") if self.pml else self._triple("B"), fu,
-                           *self._conductive("D"))
+                           *self._conductive("D"), other)
 
     def update_H(self):
         if not self.pml:
@@ -417,8 +438,10 @@ class OracleSim:
         x = self._electric()
         inv = (_FP * 3)(*[_fp(self.inv["E" + c]) for c in COMPONENTS])
         fw = self._triple("f_w_E") if self.pml else None
+        xo = ctypes.byref(self.other._electric()) if self.other is not None else None
+        do = self.other._triple("D") if self.other is not None else None
         lib().oracle_update_e(ctypes.byref(g), ctypes.byref(x), self._triple("E"),
-                              self._triple("D"), inv, fw)
+                              self._triple("D"), inv, fw, xo, do)
 
     def update_P(self):
         g = self._grid()
@@ -513,3 +536,64 @@ class OracleSim:
         self.update_P()
         self.step_count += 1
         self.time = self.step_count * self.dt
+
+
+class ComplexOracle:
+    """Complex field storage as MEEP steps it (part_products.py:1-40): two real parts, each
+    stepped with the same real coefficients (MEEP's DOCMP loops), coupled only where a Bloch
+    connection forms a product by a phase that is neither +1 nor -1. ``re`` and ``im`` are
+    OracleSims set up alike (materials, layer, folds, poles); a source's imaginary line
+    ``(amp_re*c_im) + (amp_im*c_re)`` is the real line of amplitudes ``(amp_im, -amp_re)``
+    (sources.py:1357-1482, step.cpp:309-314), so ``add_deposit`` gives the imaginary part
+    those. Every sub-step runs for both parts before the next starts."""
+
+    def __init__(self, re: OracleSim, im: OracleSim, kinds, phases):
+        self.re, self.im = re, im
+        for part, (sim, other) in enumerate(((re, im), (im, re))):
+            sim.part = part
+            sim.other = other
+            sim.bloch_kinds = tuple(int(k) for k in kinds)
+            sim.bloch_ph = tuple((float(a), float(b)) for a, b in phases)
+
+    @property
+    def parts(self):
+        return (self.re, self.im)
+
+    @property
+    def step_count(self) -> int:
+        return self.re.step_count
+
+    def add_deposit(self, deposit: PointDeposit):
+        self.re.add_deposit(deposit)
+        self.im.add_deposit(PointDeposit(
+            deposit.component, deposit.index, numpy.asarray(deposit.amp_im, dtype=numpy.float64),
+            -numpy.asarray(deposit.amp_re, dtype=numpy.float64), deposit.current,
+            integrated=deposit.integrated))
+
+    def step(self):
+        re, im = self.re, self.im
+        if any(d.integrated for d in re.deposits):
+            raise NotImplementedError("a polarization source on complex storage")
+        for s in (re, im):
+            s.step_B()
+        for s in (re, im):
+            s.deposit(True, s.time)
+            s.fill_folds("B", far=False)
+            s.zero_metal("B")
+            s.fill_folds("B", far=True)
+        for s in (re, im):
+            s.update_H()
+        for s in (re, im):
+            s.step_D()
+        for s in (re, im):
+            s.deposit(False, s.time + 0.5 * s.dt)
+            s.fill_folds("D", far=False)
+            s.zero_metal("D")
+            s.fill_folds("D", far=True)
+        for s in (re, im):
+            s.update_E()
+        for s in (re, im):
+            s.update_P()
+        for s in (re, im):
+            s.step_count += 1
+            s.time = s.step_count * s.dt

@@ -55,7 +55,38 @@ typedef struct {
      * metallic boundaries (no stored far wall, a zero ghost past the top), 2 over periodic
      * ones (one stored slot past the window, the far ghost) */
     int fold[3];
+    int fphase[3];         /* a fold's declared phase, +1 or -1 */
+    /* Bloch connection per axis (boundaries.cpp:359-366, step.cpp:171-221): 0 none (a plain
+     * periodic wrap), 1 a copy (phase within 1e-13 of +1), 2 a negation (-1), 3 a product by
+     * the phase (ph: its two parts rounded to float32); on a phased axis the engine stores
+     * MEEP's owned window, and the read of a neighbour's SAME stored index across the window
+     * edge (plane 0) is the connected word (stepping._bloch_ghost_operand :2545-2586) */
+    int pkind[3];
+    float ph[3][2];
+    /* complex storage as two real parts stepped alike: which part this call steps (0 the real,
+     * 1 the imaginary); a product connection reads the other part's word */
+    int part;
 } ogrid;
+
+/* MEEP's connection of one word across a phased axis's window edge: `mine` this part's word,
+ * `other` the other part's; conj for the word BELOW the window (conj(eikna)), else above.
+ * A product: libc++'s (a*c) - (b*d), (a*d) + (b*c), each operation rounded on its own
+ * (part_products.connect_parts). */
+static inline float conn(const ogrid *g, int axis, int conj, float mine, float other) {
+    switch (g->pkind[axis]) {
+        case 2: return -mine;
+        case 3: {
+            const float a = g->ph[axis][0], b = conj ? -g->ph[axis][1] : g->ph[axis][1];
+            if (g->part == 0) {
+                const float x = a * mine, y = b * other;
+                return x - y;
+            }
+            const float x = a * mine, y = b * other;
+            return x + y;
+        }
+        default: return mine;
+    }
+}
 
 static inline int64_t nb(const ogrid *g, int64_t i, int64_t j, int64_t k, int axis, int step) {
     int64_t c[3] = {i, j, k};
@@ -82,7 +113,7 @@ static inline float rd(const ogrid *g, const float *G, int64_t i, int64_t j, int
  * conductivity's condfac, condinv and (under an absorber) f_cond volumes, or NULL (lossless). */
 void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *const src[3],
                       float *const fu[3], const float *const cf[3], const float *const ci[3],
-                      float *const fc[3]) {
+                      float *const fc[3], float *const src_other[3]) {
     const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
     const int step = family == 0 ? +1 : -1;
     const int half = family == 0 ? 1 : 0;
@@ -94,6 +125,10 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
         const int p1 = g->present[first_axis], p2 = g->present[second_axis];
         float *restrict F = f[c];
         const float *G1 = src[first], *G2 = src[second];
+        const float *O1 = src_other ? src_other[first] : 0, *O2 = src_other ? src_other[second] : 0;
+        /* B's forward differences read the unshifted partner half a cell BELOW the target
+         * (conj(eikna) at plane 0), D's backward ones half a cell ABOVE (eikna) */
+        const int below = family == 0;
         float *U = fu ? fu[c] : 0;
         const float *CF = cf ? cf[c] : 0, *CI = ci ? ci[c] : 0;
         float *FC = fc ? fc[c] : 0;
@@ -104,18 +139,23 @@ void oracle_step_curl(const ogrid *g, int family, float *const f[3], float *cons
                     const int64_t at = IDX(i, j, k);
                     const int ic[3] = {i, j, k};
                     float curl;
+                    float g1 = G1[at], g2 = G2[at];
+                    if (g->pkind[first_axis] && ic[first_axis] == 0)
+                        g1 = conn(g, first_axis, below, g1, O1 ? O1[at] : 0.0f);
+                    if (g->pkind[second_axis] && ic[second_axis] == 0)
+                        g2 = conn(g, second_axis, below, g2, O2 ? O2[at] : 0.0f);
                     if (p1 && p2) {
-                        float g1s = rd(g, G1, i, j, k, first_axis, step), g1 = G1[at];
-                        float g2 = G2[at], g2s = rd(g, G2, i, j, k, second_axis, step);
+                        float g1s = rd(g, G1, i, j, k, first_axis, step);
+                        float g2s = rd(g, G2, i, j, k, second_axis, step);
                         float t = g1s - g1;
                         t = t + g2;
                         t = t - g2s;
                         curl = dtdx * t;
                     } else if (p1) {
-                        float t = rd(g, G1, i, j, k, first_axis, step) - G1[at];
+                        float t = rd(g, G1, i, j, k, first_axis, step) - g1;
                         curl = dtdx * t;
                     } else if (p2) {
-                        float t = rd(g, G2, i, j, k, second_axis, step) - G2[at];
+                        float t = rd(g, G2, i, j, k, second_axis, step) - g2;
                         curl = (-dtdx) * t;
                     } else {
                         curl = 0.0f;
@@ -261,10 +301,46 @@ static inline float dmp(const oelectric *x, float *const d[3], int c, int64_t at
     return v;
 }
 
+/* A connection for an explicit part (the off-diagonal row forms the other part's product). */
+static inline float conn_part(const ogrid *g, int axis, int conj, int part, float mine, float other) {
+    if (g->pkind[axis] != 3) return g->pkind[axis] == 2 ? -mine : mine;
+    const float a = g->ph[axis][0], b = conj ? -g->ph[axis][1] : g->ph[axis][1];
+    const float x = a * mine, y = b * other;
+    return part == 0 ? x - y : x + y;
+}
+
+/* The off-diagonal row's partner pair at cell (i, j, k) for one part: (D - P) of partner `pa`
+ * there, read across a phased partner axis's window edge as the word above it, plus its lower
+ * neighbour along `pa` -- zero past a metallic wall, the mirror ghost on a folded axis (parity
+ * of D_pa about its own plane, -phase, times stored row 2), else the periodic neighbour
+ * (stepping._offdiagonal_terms :1594-1687 with _shift_down :2485). `xm`/`dm` are this part's
+ * polarizations and D, `xt`/`dt_` the other part's (NULL for real storage). */
+static float offdiag_pair(const ogrid *g, int part, const oelectric *xm, float *const dm[3],
+                          const oelectric *xt, float *const dt_[3], int pa, int i, int j, int k) {
+    const int ic[3] = {i, j, k};
+    const int64_t at = ((int64_t)i * g->n[1] + j) * g->n[2] + k;
+    float v = dmp(xm, dm, pa, at);
+    if (g->pkind[pa] && ic[pa] == 0) v = conn_part(g, pa, 0, part, v, xt ? dmp(xt, dt_, pa, at) : 0.0f);
+    float lower;
+    if (g->metal[pa] && ic[pa] == 0) {
+        lower = 0.0f;
+    } else if (g->fold[pa] && ic[pa] == 0) {
+        int r[3] = {i, j, k};
+        r[pa] = 2;
+        const float m = dmp(xm, dm, pa, ((int64_t)r[0] * g->n[1] + r[1]) * g->n[2] + r[2]);
+        lower = g->fphase[pa] > 0 ? -m : m;
+    } else {
+        lower = dmp(xm, dm, pa, nb(g, i, j, k, pa, -1));
+    }
+    return v + lower;
+}
+
 /* E from D: constitutive product (diagonal, or the off-diagonal row in double), then the
- * f_w accumulation on the own-axis layer's cells under an absorber. */
+ * f_w accumulation on the own-axis layer's cells under an absorber. For complex storage `xo`
+ * and `d_other` are the other part's polarizations and D (NULL for real storage). */
 void oracle_update_e(const ogrid *g, const oelectric *x, float *const e[3], float *const d[3],
-                     const float *const inv[3], float *const fw[3]) {
+                     const float *const inv[3], float *const fw[3], const oelectric *xo,
+                     float *const d_other[3]) {
     const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
     for (int c = 0; c < 3; ++c) {
         float *E = e[c];
@@ -293,21 +369,23 @@ void oracle_update_e(const ogrid *g, const oelectric *x, float *const e[3], floa
                              * wall plane of a metallic axis where this component's Yee shift
                              * is 0 (stepping.py:1594-1687, _mask_metallic_wall_coupling
                              * :1718-1745) */
-                            const int low_pa = g->metal[pa] && ic[pa] == 0;
-                            const int top_c = g->metal[c] && ic[c] == g->n[c] - 1;
+                            /* the product above reads zero past the top of a metallic or a
+                             * folded own axis (_shift_up's zero face); at plane 0 of a phased
+                             * own axis the product here is the word below the window */
+                            const int top_c = (g->metal[c] || g->fold[c]) && ic[c] == g->n[c] - 1;
                             const int64_t up = nb(g, i, j, k, c, +1);
-                            int64_t q0 = at, q1 = up;
-                            int64_t q0m = nb(g, i, j, k, pa, -1);
                             int iu[3] = {i, j, k};
                             iu[c] = (iu[c] + 1) % g->n[c];
-                            int64_t q1m = nb(g, iu[0], iu[1], iu[2], pa, -1);
-                            float pair0 = dmp(x, d, pa, q0) + (low_pa ? 0.0f : dmp(x, d, pa, q0m));
-                            float prod0 = U[q0] * pair0;
-                            float prod1 = 0.0f;
-                            if (!top_c) {
-                                float pair1 = dmp(x, d, pa, q1) + (low_pa ? 0.0f : dmp(x, d, pa, q1m));
-                                prod1 = U[q1] * pair1;
+                            float prod0 = U[at] * offdiag_pair(g, g->part, x, d, xo, d_other, pa, i, j, k);
+                            if (g->pkind[c] && ic[c] == 0) {
+                                float other = 0.0f;
+                                if (xo)
+                                    other = U[at] * offdiag_pair(g, 1 - g->part, xo, d_other, x, d, pa, i, j, k);
+                                prod0 = conn_part(g, c, 1, g->part, prod0, other);
                             }
+                            float prod1 = 0.0f;
+                            if (!top_c)
+                                prod1 = U[up] * offdiag_pair(g, g->part, x, d, xo, d_other, pa, iu[0], iu[1], iu[2]);
                             float summed = prod0 + prod1;
                             for (int a = 0; a < 3; ++a)
                                 if (a != c && g->metal[a] && ic[a] == 0) summed = 0.0f;

@@ -20,12 +20,14 @@ ROOT = os.path.join(HERE, "golden", "meep_identity")
 FIXTURES = os.path.join(ROOT, "keep")
 #: The reference's two subnormal policies: MEEP's words with subnormals kept, and flushed.
 MODES = ("keep", "flush")
-#: The cases the engine steps (the holey waveguide needs complex storage and the ring
-#: cylindrical storage; both are refused by name).
-CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d", "c", "c_prime")
+#: The holey waveguide of MEEP's test_holey_wvg_bands (complex storage, a Bloch phase of -1
+#: along x, an odd Y fold over periodic boundaries, an off-diagonal chi1inv, a PML on high y).
+HOLEY = "t.test_holey_wvg_bands.TestHoleyWvgBands.test_fields_at_kx"
+#: The cases the engine steps (the ring needs cylindrical storage, refused by name).
+CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d", "c", "c_prime", HOLEY)
 #: Those on a mirror-folded grid (Y and Z odd mirrors over metallic boundaries, an Hx point
 #: source on both planes): their get_array read-out is not built.
-FOLDED = ("c", "c_prime")
+FOLDED = ("c", "c_prime", HOLEY)
 
 
 def case_dir(case: str, mode: str = "keep") -> str:
@@ -46,9 +48,29 @@ class OracleRun:
     """A lifted state stepped by the CPU restatement, every word pasted as the engine pastes it."""
 
     def __init__(self, state: capsule.LiftedState, currents, subnormals: str = "keep"):
-        from oracle.engine import OracleSim, PointDeposit
+        from oracle.engine import ComplexOracle, OracleSim, PointDeposit
 
         self.flush = subnormals == "flush"
+        self.state = state
+        if state.complex:
+            parts = [self._part(state, currents, state.volumes),
+                     self._part(state, currents, state.volumes_im)]
+            self.complex = ComplexOracle(parts[0], parts[1], state.bloch_kinds,
+                                         state.bloch_phase)
+            # the imaginary part's deposits are the imaginary line of the same sources
+            deposits, parts[0].deposits, parts[1].deposits = parts[0].deposits, [], []
+            for dep in deposits:
+                self.complex.add_deposit(dep)
+            self.sim, self.sims = parts[0], parts
+            return
+        self.complex = None
+        self.sim = self._part(state, currents, state.volumes)
+        self.sims = [self.sim]
+
+    @staticmethod
+    def _part(state: capsule.LiftedState, currents, volumes):
+        from oracle.engine import OracleSim, PointDeposit
+
         faces = state.pml.faces if state.pml is not None else ((0, 0), (0, 0), (0, 0))
         sim = OracleSim(state.shape, state.dt, state.dx, invariant=state.invariant,
                         pml_faces=faces)
@@ -57,7 +79,7 @@ class OracleRun:
         sim.set_folding(folds)
         if state.pml is not None:
             sim.set_layer_tables(state.pml.vectors, state.pml.members)
-        sim.set_inverse_epsilon(dict(state.inv_eps))
+        sim.set_inverse_epsilon(dict(state.inv_eps), dict(state.offdiag) or None)
         for pole in state.poles:
             sigma = {c: pole.sigma.get(c, 0.0) for c in ("Ex", "Ey", "Ez")}
             added = sim.add_pole(pole.frequency, pole.gamma, pole.kind == "drude", sigma)
@@ -73,7 +95,9 @@ class OracleRun:
                 times[t] = complex(table[n, 0], table[n, 1])
             sim.add_deposit(PointDeposit(src.component, flat, src.amps.real.copy(),
                                          src.amps.imag.copy(), times.__getitem__))
-        for name, vol in state.volumes.items():
+        if state.offdiag or state.complex:
+            sim.enable_field_storage()
+        for name, vol in volumes.items():
             if not numpy.any(vol):
                 continue
             if name.startswith("P"):
@@ -82,8 +106,7 @@ class OracleRun:
         sim.step_count = state.step
         sim.time = state.step * state.dt
         sim.h_separated = state.h_separated
-        self.sim = sim
-        self.state = state
+        return sim
 
     @property
     def step_count(self) -> int:
@@ -98,10 +121,10 @@ class OracleRun:
 
         with flushing(self.flush):
             for _ in range(steps):
-                self.sim.step()
+                (self.complex or self.sim).step()
 
-    def words(self, component: str) -> numpy.ndarray:
-        sim = self.sim
+    def words(self, component: str, part: int = 0) -> numpy.ndarray:
+        sim = self.sims[part]
         if component[0] == "E":
             return sim.get_E(component)
         if component[0] == "H" and not sim.pml:
