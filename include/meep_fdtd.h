@@ -298,6 +298,13 @@ int fdtd_flux_spectrum_noalloc(fdtd_sim *sim, int32_t plane, const double *weigh
 int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out);
 void fdtd_batch_destroy(fdtd_batch *batch);
 int fdtd_batch_run(fdtd_batch *batch, int32_t nsteps);
+
+/* One half-step of every member: the H pairs (fdtd_batch_step_h), then the D pairs with the
+ * counter advance and the flux sampling (fdtd_batch_step_d) -- the batch's fdtd_step_h and
+ * fdtd_step_d, the driver's two fused consults for a run on complex storage, whose two parts
+ * step only together. */
+int fdtd_batch_step_h(fdtd_batch *batch);
+int fdtd_batch_step_d(fdtd_batch *batch);
 /* Run the float64 transform pass over every plane's buffered samples (before a read). */
 int fdtd_batch_flush(fdtd_batch *batch);
 int fdtd_batch_synchronize(fdtd_batch *batch);

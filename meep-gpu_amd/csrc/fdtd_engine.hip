@@ -4587,25 +4587,51 @@ void fdtd_batch_destroy(fdtd_batch *b) {
     for (auto *s : release) fdtd_sim_destroy(s);
 }
 
-int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
-    if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
+// The batch's stepper, with the tile lists of the widths its configuration launches.
+static int batch_stepper(fdtd_batch *b, Stepper *k) {
     int rc = FDTD_OK;
     for (const LaunchShape &sh : b->config()->shape) {  // the widths the launches take
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
     }
     if (rc) return rc;
+    k->g = b->group;
+    k->g.cfg = b->config();
+    k->sims = b->sims.data();
+    k->nsims = (int)b->sims.size();
+    k->planes = b->planes;
+    k->nplanes = b->nplanes;
+    k->max_plane_cells = b->max_plane_cells;
+    k->st = b->stream;
+    k->graph = &b->graph;
+    return FDTD_OK;
+}
+
+int fdtd_batch_run(fdtd_batch *b, int32_t nsteps) {
+    if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
     Stepper k;
-    k.g = b->group;
-    k.g.cfg = b->config();
-    k.sims = b->sims.data();
-    k.nsims = (int)b->sims.size();
-    k.planes = b->planes;
-    k.nplanes = b->nplanes;
-    k.max_plane_cells = b->max_plane_cells;
-    k.st = b->stream;
-    k.graph = &b->graph;
+    int rc = batch_stepper(b, &k);
+    if (rc) return rc;
     return run_steps(k, nsteps);
+}
+
+int fdtd_batch_step_h(fdtd_batch *b) {
+    if (!b) return fail(FDTD_EINVAL, "null argument");
+    Stepper k;
+    int rc = batch_stepper(b, &k);
+    if (rc) return rc;
+    return launch_h(k.g, k.st);
+}
+
+int fdtd_batch_step_d(fdtd_batch *b) {
+    if (!b) return fail(FDTD_EINVAL, "null argument");
+    Stepper k;
+    int rc = batch_stepper(b, &k);
+    if (rc) return rc;
+    rc = launch_d(k.g, k.st);
+    if (rc) return rc;
+    for (int q = 0; q < k.nsims; ++q) mirror_advance(k.sims[q]);
+    return flux_after_step(k, true);
 }
 
 int fdtd_batch_flush(fdtd_batch *b) {

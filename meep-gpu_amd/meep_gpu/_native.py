@@ -71,6 +71,8 @@ SIGNATURES = {
     "fdtd_set_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fdtd_set_source_currents": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
+    "fdtd_batch_step_h": (ctypes.c_int, [ctypes.c_void_p]),
+    "fdtd_batch_step_d": (ctypes.c_int, [ctypes.c_void_p]),
     "fdtd_link_parts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _IP,
                                        ctypes.POINTER(ctypes.c_float)]),
     "fdtd_sim_set_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),

@@ -67,6 +67,12 @@ class BatchedRun:
     def synchronize(self):
         check(self.lib.fdtd_batch_synchronize(self.handle), "fdtd_batch_synchronize", self.lib)
 
+    def step_h(self):
+        check(self.lib.fdtd_batch_step_h(self.handle), "fdtd_batch_step_h", self.lib)
+
+    def step_d(self):
+        check(self.lib.fdtd_batch_step_d(self.handle), "fdtd_batch_step_d", self.lib)
+
     def set_config(self, config):
         """The batch's own launch configuration (``engine.LaunchConfig``); None: follow the
         process defaults."""

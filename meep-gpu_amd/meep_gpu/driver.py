@@ -34,19 +34,105 @@ from .workloads import Workload
 _E = ("Ex", "Ey", "Ez")
 
 
+def _boundary_kinds(spec) -> Dict[int, str]:
+    """Per-axis boundaries from the reference's spellings (grid.py ``_normalize_boundaries``):
+    None, one kind for every axis, an ``{"x": kind}`` mapping, or a 3-sequence, each entry a
+    kind or a ``(low, high)`` pair whose sides agree."""
+    if spec is None or (not isinstance(spec, str) and len(spec) == 0):
+        return {}
+    if isinstance(spec, str):
+        items = {a: spec for a in range(3)}
+    elif isinstance(spec, dict):
+        items = {(k if isinstance(k, int) else "xyz".index(str(k).lower())): v
+                 for k, v in spec.items()}
+    else:
+        if len(spec) != 3:
+            raise ValueError(f"boundaries as a sequence take one entry per axis, got {spec!r}")
+        items = dict(enumerate(spec))
+    kinds = {}
+    for axis, kind in items.items():
+        if not isinstance(kind, str):
+            low, high = kind
+            if low != high:
+                raise ValueError(f"axis {'xyz'[axis]}: the two faces must agree, got {kind!r}")
+            kind = low
+        kinds[axis] = kind
+    return kinds
+
+
+def _mirror_entries(symmetry) -> Tuple[Tuple[str, int], ...]:
+    """``(axis, phase)`` per mirror plane: a bare axis name (an even plane), an ``(axis,
+    phase)`` pair, or an object with ``axis`` (or ``direction``) and ``phase`` attributes, as
+    ``mp.Mirror(direction, phase)`` and the reference's ``Mirror`` carry them."""
+    out = []
+    for entry in symmetry or ():
+        if isinstance(entry, (str, int)):
+            out.append((entry, 1))
+        elif isinstance(entry, (tuple, list)):
+            out.append((entry[0], int(entry[1])))
+        else:
+            axis = getattr(entry, "axis", getattr(entry, "direction", None))
+            out.append((axis, int(getattr(entry, "phase", 1))))
+    return tuple(out)
+
+
+def bloch_phase(k_value: float, n_full: int, resolution: float) -> Optional[complex]:
+    """The factor a field picks up one lattice vector up an axis, MEEP's ``eikna`` (reference
+    grid.py:1009-1030, boundaries.cpp ``use_bloch``): None at k = 0, exactly -1 at the zone
+    edge ``k * n == resolution / 2``, else ``exp(i * k * ((2*pi/a) * n))`` with the product
+    associated as MEEP forms it."""
+    if k_value == 0.0:
+        return None
+    if k_value * n_full == 0.5 * resolution:
+        return complex(-1.0, 0.0)
+    angle = k_value * ((2 * math.pi / resolution) * n_full)
+    return complex(math.cos(angle), math.sin(angle))
+
+
 class FdtdDriver:
     def __init__(self, cell_size: Sequence[float], resolution: float, courant: float = 0.5,
+                 force_complex_fields: bool = False, symmetry: Sequence = (),
+                 k_point: Sequence[float] = (0.0, 0.0, 0.0), boundaries=None,
                  dimensions: int = 3, prefer_gpu: bool = True, max_steps: int = 100000):
+        """The reference's arguments (driver.py:811-827) in its order. Two defaults differ on
+        purpose: real storage (``force_complex_fields=False``, MEEP's own default; the
+        reference defaults to complex), and the GPU (``prefer_gpu``; there is no array path).
+        A real run is the real part of the same run on complex storage, word for word.
+
+        ``symmetry``: mirror planes (``("X", -1)``, a bare ``"Y"``, or ``mp.Mirror``-like
+        objects); each halves its axis as MEEP does. ``k_point``: the Bloch wavevector in
+        MEEP's units; a nonzero component needs complex fields and is refused on a mirrored,
+        invariant, metallic or absorbing axis, as the reference refuses it
+        (driver.py:1094-1170, grid.py:960-1000). ``boundaries``: per axis "periodic" (the
+        default) or "metallic" (``set_boundaries``)."""
         if not prefer_gpu:
             raise ValueError("this package steps on the GPU only; prefer_gpu=False has no engine")
-        self.grid = Grid(resolution=resolution, cell_size=tuple(cell_size), courant=courant,
-                         dimensions=dimensions)
+        self._cell = tuple(float(v) for v in cell_size)
+        self._resolution, self._courant, self._dimensions = resolution, courant, dimensions
+        self.force_complex_fields = bool(force_complex_fields)
+        self.symmetry = _mirror_entries(symmetry)
+        k = tuple(float(v) for v in k_point)
+        if len(k) != 3 or not all(math.isfinite(v) for v in k):
+            raise ValueError(f"k_point takes three finite real components, got {k_point!r}")
+        self.k_point = k
+        self._boundaries: Dict[int, str] = {}
+        self.grid = self._make_grid()
+        bloch = [a for a in range(3) if k[a] != 0.0]
+        if bloch and not self.force_complex_fields:
+            raise ValueError(f"k_point={k} is nonzero on {['xyz'[a] for a in bloch]}: the "
+                             "boundary phase needs complex fields (force_complex_fields=True); "
+                             "MEEP refuses the same pairing")
+        for a in bloch:
+            if self.grid.is_invariant(a):
+                raise ValueError(f"k_point is nonzero on the invariant axis {'xyz'[a]}")
+            if self.grid.folds[a] is not None:
+                raise ValueError(f"k_point={k} is nonzero on the mirror-symmetric axis "
+                                 f"{'xyz'[a]}: inside the zone a mirror needs k = 0 there")
         shape = self.grid.shape
         self._inv = {c: numpy.ones(shape, dtype=numpy.float32) for c in _E}
         self._offdiag: Dict[str, Dict[str, numpy.ndarray]] = {}
         self._conductivity: Dict[str, numpy.ndarray] = {}
         self._absorber_layers: List[AbsorbingFace] = []
-        self._boundaries: Dict[int, str] = {}
         self._poles: List[Tuple[Susceptibility, Dict[str, object]]] = []
         self._pml_spec = None
         self._pml_kwargs = {}
@@ -59,6 +145,40 @@ class FdtdDriver:
         self.sim: Optional[HipSimulation] = None
         self._fast_path = None
         self._planned = 0
+        self._shaped = False  # a volume, source or monitor placed on the grid's cells
+        if boundaries is not None:
+            self.set_boundaries(boundaries)
+
+    def _make_grid(self) -> Grid:
+        return Grid(resolution=self._resolution, cell_size=self._cell, courant=self._courant,
+                    dimensions=self._dimensions, symmetry=self.symmetry,
+                    metallic=tuple(self._boundaries.get(a) == "metallic" for a in range(3)))
+
+    @property
+    def is_complex(self) -> bool:
+        return self.force_complex_fields
+
+    @property
+    def bloch_phases(self) -> Tuple[Optional[complex], ...]:
+        """Per axis the Bloch factor (``bloch_phase``) of the full cell, or None."""
+        return tuple(bloch_phase(self.k_point[a], self.grid.shape_full[a],
+                                 self.grid.resolution) for a in range(3))
+
+    @property
+    def sims(self):
+        """The parts a batch steps: both parts of complex storage, else None."""
+        return getattr(self.sim, "sims", None)
+
+    def _require_real(self, what: str):
+        if self.force_complex_fields:
+            raise ValueError(f"{what} on complex storage is not built: the per-cell transforms "
+                             "of the two parts are not combined (build the run with "
+                             "force_complex_fields=False)")
+
+    def _require_unfolded(self, what: str):
+        if any(f is not None for f in self.grid.folds):
+            raise ValueError(f"{what} on a mirror-folded grid is not built: MEEP's placement "
+                             "and weights under a symmetry are not restated here")
 
     # --- configuration (before the first step) ------------------------------------------------
     def _require_unfrozen(self, what: str):
@@ -68,6 +188,7 @@ class FdtdDriver:
     def set_epsilon_components(self, epsilon: Dict[str, object],
                                chi1inv_offdiagonal: Optional[Dict[str, Dict[str, object]]] = None):
         self._require_unfrozen("set_epsilon_components")
+        self._shaped = True
         if set(epsilon) != set(_E):
             raise ValueError("epsilon must map exactly Ex, Ey, Ez")
         for c in _E:
@@ -98,6 +219,7 @@ class FdtdDriver:
 
     def _set_conductivity_side(self, side: str, conductivity: Dict[str, object]):
         self._require_unfrozen("set_conductivity")
+        self._shaped = True
         for name in [n for n in self._conductivity if n[0] == side]:
             del self._conductivity[name]
         for name, value in (conductivity or {}).items():
@@ -120,14 +242,28 @@ class FdtdDriver:
         zero past a wall and couples nothing on its planes (stepping.py:1594-1745)."""
         self._require_unfrozen("set_boundaries")
         kinds = {}
-        for key, kind in (boundaries or {}).items():
-            axis = "xyz".index(str(key).lower())
+        for axis, kind in _boundary_kinds(boundaries).items():
+            key = "xyz"[axis]
             if kind not in ("periodic", "metallic"):
                 raise ValueError(f"boundary {key!r} must be 'periodic' or 'metallic', got {kind!r}")
             if kind == "metallic" and self.grid.is_invariant(axis):
                 raise ValueError(f"axis {key!r} is invariant: it has no wall")
+            if kind == "metallic" and self.k_point[axis] != 0.0:
+                raise ValueError(f"k_point is nonzero on {key!r}: a metallic wall gives the axis "
+                                 "no lattice vector for a Bloch phase")
             kinds[axis] = kind
+        previous = self._boundaries
         self._boundaries = kinds
+        grid = self._make_grid()
+        if grid.shape != self.grid.shape:
+            # a folded axis stores one cell fewer between walls than over a periodic wrap
+            if self._shaped:
+                self._boundaries = previous
+                raise ValueError("on a mirror-folded axis the boundaries set the stored cell "
+                                 "count: call set_boundaries before any material, source or "
+                                 "monitor")
+            self._inv = {c: numpy.ones(grid.shape, dtype=numpy.float32) for c in _E}
+        self.grid = grid
 
     @property
     def wraps(self) -> Tuple[bool, bool, bool]:
@@ -142,6 +278,7 @@ class FdtdDriver:
         :1596-1625), added to any material conductivity in double and rounded once. Every
         resolved axis when ``direction`` is None, both sides when ``side`` is None."""
         self._require_unfrozen("set_absorber")
+        self._shaped = True
         self._absorber_layers += faces_for(thickness, direction, side, R_asymptotic,
                                             [not self.grid.is_invariant(a) for a in range(3)])
 
@@ -156,11 +293,13 @@ class FdtdDriver:
 
     def set_inverse_epsilon(self, inv: Dict[str, numpy.ndarray]):
         self._require_unfrozen("set_inverse_epsilon")
+        self._shaped = True
         for c in _E:
             self._inv[c] = numpy.ascontiguousarray(inv[c], dtype=numpy.float32)
 
     def add_susceptibility(self, term: Susceptibility, sigma):
         self._require_unfrozen("add_susceptibility")
+        self._shaped = True
         if not term.stable(self.grid.dt):
             raise ValueError(f"{term} is unstable at dt={self.grid.dt}")
         if not isinstance(sigma, dict):
@@ -172,13 +311,34 @@ class FdtdDriver:
         probe = PML(self.grid, thickness_cells, order=order, R_asymptotic=R_asymptotic)
         if not probe.is_active:
             raise ValueError("a PML request that absorbs nowhere")
+        for a in range(3):
+            if self.k_point[a] != 0.0 and any(float(t) > 0 for t in probe.faces[a]):
+                raise ValueError(f"k_point is nonzero on {'xyz'[a]!r}, which the PML would "
+                                 "terminate: a Bloch phase needs the axis's lattice vector "
+                                 "(reference driver.py:2203-2209)")
+        self._shaped = True
         self._pml_spec = thickness_cells
         self._pml_kwargs = dict(order=order, R_asymptotic=R_asymptotic)
 
     def add_source(self, box, time_function):
         """A deposit box (or a list of them, deposited in order) driven by ``time_function``."""
         self._require_unfrozen("add_source")
-        for b in ([box] if isinstance(box, DepositBox) else list(box)):
+        boxes = [box] if isinstance(box, DepositBox) else list(box)
+        for b in boxes:
+            if self.force_complex_fields and b.integrated:
+                raise ValueError("a polarization (integrated) source on complex storage is "
+                                 "not built")
+            for a, phase in enumerate(self.bloch_phases):
+                # stored row 0 of a Yee-shift-0 component on a phased axis holds the window
+                # top, the point one lattice vector above the bottom face (reference
+                # grid.py:1054-1075): a deposit there would need the phase of its image
+                comp = "xyz".index(b.component[1])
+                shift0 = (a == comp) if b.magnetic else (a != comp)
+                if phase is not None and shift0 and b.lo[a] == 0:
+                    raise ValueError(f"a {b.component} source on stored row 0 of the Bloch-phased "
+                                     f"axis {'xyz'[a]} is not built: move it off the cell face")
+        self._shaped = True
+        for b in boxes:
             self._sources.append((b, time_function))
 
     def add_volume_source(self, component: str, center: Sequence[float], size: Sequence[float],
@@ -187,6 +347,14 @@ class FdtdDriver:
         """``mp.Source(src, component, center, size, amplitude, is_integrated)``: placed as MEEP
         places it on this grid and its boundaries (``meep_volume_source``); returns the boxes."""
         self._require_unfrozen("add_volume_source")
+        self._require_unfolded("add_volume_source")
+        for a, phase in enumerate(self.bloch_phases):
+            half = 0.5 * self.grid.shape_full[a] * self.grid.dx
+            lo, hi = center[a] - 0.5 * size[a], center[a] + 0.5 * size[a]
+            if phase is not None and (lo < -half + self.grid.dx or hi > half - self.grid.dx):
+                raise ValueError(f"a source reaching the Bloch-phased boundary of "
+                                 f"{'xyz'[a]} is not built: its wrapped part would carry the "
+                                 "phase")
         boxes = meep_volume_source(self.grid, component, center, size, amplitude,
                                    metallic=tuple(not w for w in self.wraps),
                                    integrated=is_integrated)
@@ -197,6 +365,9 @@ class FdtdDriver:
                          fcen: float = None, df: float = None, nfreq: int = None,
                          frequencies: Sequence[float] = None, decimation: int = 1) -> FluxMonitor:
         self._require_unfrozen("add_flux_monitor")
+        self._require_real("a flux monitor")
+        self._require_unfolded("a flux monitor")
+        self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))
         mon = FluxMonitor(int(normal), tuple(lo), tuple(cnt), freqs, self.grid.dx, self.grid.dt,
@@ -212,6 +383,9 @@ class FdtdDriver:
         weighted as MEEP places it (coordinates with their origin at the cell centre).
         ``decimation`` is the accumulation stride MEEP resolved for the monitor."""
         self._require_unfrozen("add_flux_region")
+        self._require_real("a flux region")
+        self._require_unfolded("a flux region")
+        self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))
         mon = FluxMonitor.from_region(self.grid, tuple(center), tuple(size), freqs,
@@ -231,6 +405,9 @@ class FdtdDriver:
         without an absorber), resolved when stepping starts. ``decimation`` is an explicit
         positive accumulation stride."""
         self._require_unfrozen("add_dft_monitor")
+        self._require_real("a DFT monitor")
+        self._require_unfolded("a DFT monitor")
+        self._shaped = True
         freqs = (numpy.atleast_1d(numpy.asarray(frequencies, dtype=numpy.float64))
                  if frequencies is not None else meep_frequency_span(fcen, df, nfreq))
         if (center is None) != (size is None):
@@ -289,6 +466,10 @@ class FdtdDriver:
     def from_workload(cls, wl: Workload) -> "FdtdDriver":
         drv = cls.__new__(cls)
         drv.grid = wl.grid
+        drv.force_complex_fields = False
+        drv.symmetry = ()
+        drv.k_point = (0.0, 0.0, 0.0)
+        drv._shaped = True
         drv._inv = dict(wl.inv)
         drv._offdiag = {}
         drv._conductivity = {}
@@ -320,45 +501,35 @@ class FdtdDriver:
         return self.step_count * self.grid.dt
 
     def freeze(self, planned_steps: Optional[int] = None):
-        """Create the device simulation and upload every table (the reference's first-step freeze)."""
+        """Create the device simulation and upload every table (the reference's first-step
+        freeze). On complex storage: two parts, linked by the Bloch connections."""
         if self.sim is not None:
             return self.sim
         steps = int(planned_steps or self.max_steps)
         pml = (PML(self.grid, self._pml_spec, wraps=self.wraps, **self._pml_kwargs)
                if self._pml_spec else None)
-        sim = HipSimulation(self.grid, pml)
-        try:
-            for axis, kind in self._boundaries.items():
-                sim.set_boundary(axis, kind)
-        except Exception:
-            sim.close()
-            raise
-        for c in _E:
-            sim.set_inverse_epsilon(c, self._inv[c])
-        for row, partners in self._offdiag.items():
-            for partner, vol in partners.items():
-                sim.set_offdiagonal(row, partner, vol)
-        for term, sigma in self._poles:
-            sim.add_pole(term, sigma)
-        division = ChunkDivision.one_rank(
-            self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)), self.wraps)
-        for name, vol in self.conductivity_volumes().items():
-            # MEEP keeps a chunk's conductivity only where it samples a nonzero sigma
-            sim.set_conductivity(name, vol, division.conductive_mask(vol, IYEE[name]))
-        for box, fn in self._sources:
-            table = current_table(fn, self.grid.dt, steps, box.magnetic, box.integrated)
-            axis = "xyz".index(box.component[1])
-            sim.add_source(box.magnetic, axis, box.lo, box.cnt, box.amp_re, box.amp_im, table,
-                           self.grid.dt, box.integrated)
         for mon in self._automatic_monitors:
             mon.decimation = self.automatic_decimation(mon.frequencies)
-        for mon in self.flux_monitors:
-            mon.attach(sim)
         self._place_default_dft_regions(pml)
-        for mon in self.dft_monitors:
-            mon.attach(sim)
-        if pml is None and not self._poles and not self._offdiag:
-            sim.prime_e()
+        parts = []
+        try:
+            for part in range(2 if self.force_complex_fields else 1):
+                parts.append(self._build_part(pml, steps, part))
+            if self.force_complex_fields:
+                from .capsule import bloch_connection
+                from .engine import ComplexSimulation, link_parts
+
+                phases = self.bloch_phases
+                kinds = [0 if z is None else bloch_connection(z) for z in phases]
+                link_parts(parts[0], parts[1], kinds,
+                           [(1.0, 0.0) if z is None else (z.real, z.imag) for z in phases])
+                sim = ComplexSimulation(parts[0], parts[1])
+            else:
+                sim = parts[0]
+        except Exception:
+            for p in parts:
+                p.close()
+            raise
         plan = plan_fast_path(sim, pml, self.grid, self._sources)
         if plan is None:
             sim.close()
@@ -366,6 +537,52 @@ class FdtdDriver:
         self.sim = sim
         self._fast_path = plan
         self._planned = steps
+        return sim
+
+    def _build_part(self, pml, steps: int, part: int) -> HipSimulation:
+        """One part's device simulation with every table; part 1 (the imaginary part) deposits
+        each source's imaginary line, the real line of amplitudes ``(amp_im, -amp_re)``
+        (reference sources.py:1357-1482)."""
+        sim = HipSimulation(self.grid, pml)
+        try:
+            for axis, kind in self._boundaries.items():
+                if self.grid.folds[axis] is None:  # a fold's walls are part of its storage
+                    sim.set_boundary(axis, kind)
+            for c in _E:
+                sim.set_inverse_epsilon(c, self._inv[c])
+            for row, partners in self._offdiag.items():
+                for partner, vol in partners.items():
+                    sim.set_offdiagonal(row, partner, vol)
+            for term, sigma in self._poles:
+                sim.add_pole(term, sigma)
+            if self._conductivity or self._absorber_layers:
+                if any(f is not None for f in self.grid.folds):
+                    raise ValueError("a conductivity on a mirror-folded grid is not built: "
+                                     "MEEP's chunk division of the halved cell is not restated")
+                division = ChunkDivision.one_rank(
+                    self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)),
+                    self.wraps)
+                for name, vol in self.conductivity_volumes().items():
+                    # MEEP keeps a chunk's conductivity only where it samples a nonzero sigma
+                    sim.set_conductivity(name, vol, division.conductive_mask(vol, IYEE[name]))
+            for box, fn in self._sources:
+                table = current_table(fn, self.grid.dt, steps, box.magnetic, box.integrated)
+                axis = "xyz".index(box.component[1])
+                amp_re, amp_im = box.amp_re, box.amp_im
+                if part == 1:
+                    amp_re, amp_im = (numpy.asarray(box.amp_im, dtype=numpy.float64),
+                                      -numpy.asarray(box.amp_re, dtype=numpy.float64))
+                sim.add_source(box.magnetic, axis, box.lo, box.cnt, amp_re, amp_im, table,
+                               self.grid.dt, box.integrated)
+            for mon in self.flux_monitors:
+                mon.attach(sim)
+            for mon in self.dft_monitors:
+                mon.attach(sim)
+            if pml is None and not self._poles and not self._offdiag:
+                sim.prime_e()
+        except Exception:
+            sim.close()
+            raise
         return sim
 
     #: The consults of one step in the reference driver's order (driver.py:3505-3596).
@@ -408,16 +625,21 @@ class FdtdDriver:
             self.sim.synchronize()
 
     def get_array(self, component: str) -> numpy.ndarray:
-        """A stored volume in the reference's layout (C-order (nx, ny, nz))."""
+        """A stored volume in the reference's layout (C-order (nx, ny, nz)): float32, or
+        complex64 on complex storage (the two parts' words, side by side)."""
         sim = self.freeze()
         if component[0] == "H" and sim.pml is None:
             component = "B" + component[1]
+        if self.force_complex_fields:
+            return sim.download_complex(component)
         return sim.download(component)
 
     def get_meep_array(self, component: str, center=None, size=None) -> numpy.ndarray:
         """MEEP's ``sim.get_array(component)`` over the whole cell or a ``center``/``size``
         box: the centred lattice, word for word (``readback.meep_array``), where ``get_array``
         gives the stored words."""
+        self._require_real("get_meep_array")
+        self._require_unfolded("get_meep_array")
         sim = self.freeze()
         pml = sim.pml
         division = ChunkDivision.one_rank(

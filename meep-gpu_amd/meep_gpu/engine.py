@@ -435,6 +435,83 @@ def link_parts(re: "HipSimulation", im: "HipSimulation", kinds: Sequence[int],
           "fdtd_link_parts", re.lib)
 
 
+class ComplexSimulation:
+    """Complex field storage: two linked ``HipSimulation`` parts, the real and the imaginary
+    (reference ``Fields(force_complex_fields=True)``, part_products.py:1-40). The parts step
+    only together, as a batch of two whose one launch per half-step serves both; any other
+    operation (a read, a table write) ends that batch first, after its work has drained."""
+
+    def __init__(self, re: HipSimulation, im: HipSimulation):
+        self.sims = (re, im)
+        self.grid, self.pml, self.lib = re.grid, re.pml, re.lib
+        self._run = None
+
+    @property
+    def handle(self):
+        return self.sims[0].handle
+
+    def _stepper(self):
+        if self._run is None:
+            import types
+
+            from .batch import BatchedRun
+
+            self._run = BatchedRun([types.SimpleNamespace(sim=s, grid=s.grid)
+                                    for s in self.sims])
+        return self._run
+
+    def _settle(self):
+        if self._run is not None:
+            try:
+                self._run.synchronize()
+            finally:
+                self._run.close()
+                self._run = None
+
+    def step_h(self):
+        self._stepper().step_h()
+
+    def step_d(self):
+        self._stepper().step_d()
+
+    def run(self, steps: int):
+        self._stepper().run(int(steps))
+
+    def synchronize(self):
+        if self._run is not None:
+            self._run.synchronize()
+        else:
+            for s in self.sims:
+                s.synchronize()
+
+    @property
+    def step_count(self) -> int:
+        return self.sims[0].step_count
+
+    def download(self, name: str, part: int = 0) -> numpy.ndarray:
+        self._settle()
+        return self.sims[part].download(name)
+
+    def download_complex(self, name: str) -> numpy.ndarray:
+        """complex64: the real part's words plus i times the imaginary part's."""
+        self._settle()
+        re, im = (s.download(name) for s in self.sims)
+        out = numpy.empty(re.shape, dtype=numpy.complex64)
+        out.real, out.imag = re, im
+        return out
+
+    def set_source_currents(self, index: int, currents: numpy.ndarray):
+        """The same current table on both parts (each keeps its own amplitudes)."""
+        self._settle()
+        for s in self.sims:
+            s.set_source_currents(index, currents)
+
+    def close(self):
+        self._settle()
+        for s in self.sims:
+            s.close()
+
+
 def set_launch(vec: int, blocks_per_cu: int, xcd_remap: int, nt_stores: bool = False):
     """Process-wide launch shape: cells per thread, workgroups per CU (0 = one per tile), tile
     order over the XCDs (0 plain, 1 contiguous eighths, sb >= 2 super-blocks of sb tiles),

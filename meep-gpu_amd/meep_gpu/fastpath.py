@@ -107,15 +107,15 @@ def plan_fast_path(fields: Any, pml: Any = None, grid: Any = None, sources: Any 
     """Build the plan for a frozen configuration, or return None with the named reason.
 
     The arguments are the reference's (``fastpath.py:6986``). ``fields`` is the device-resident
-    state (``HipSimulation``); the engine already holds the absorber, materials, poles and
+    state (``HipSimulation``, or ``ComplexSimulation``'s two linked parts); the engine already holds the absorber, materials, poles and
     sources uploaded at the freeze, so ``pml``, ``grid`` and ``sources`` are checked against
     it rather than re-read. ``chunk_layout`` must be the one-rank division (or None).
     """
-    from .engine import HipSimulation
+    from .engine import ComplexSimulation, HipSimulation
 
     reasons = []
-    if not isinstance(fields, HipSimulation):
-        reasons.append("fields are not device-resident (HipSimulation)")
+    if not isinstance(fields, (HipSimulation, ComplexSimulation)):
+        reasons.append("fields are not device-resident (HipSimulation or ComplexSimulation)")
     elif fields.handle is None:
         reasons.append("the simulation is closed")
     else:

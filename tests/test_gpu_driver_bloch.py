@@ -1,0 +1,57 @@
+"""The driver on complex storage on the GPU: ``FdtdDriver(force_complex_fields=True,
+k_point=..., symmetry=..., boundaries=...)`` stepped consult by consult (``step``) and by
+captured graphs (``run``), every stored word of both parts against the oracle's two parts;
+and a real run equal to the real part of the same run on complex storage."""
+
+import numpy
+import pytest
+
+from tests import driver_bloch as db
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_parts(drv, ref, label):
+    for part, osim in enumerate(ref.parts):
+        for key, want in osim.stored_volumes().items():
+            got = drv.sim.download(key, part)
+            assert got.view(numpy.uint32).tobytes() == want.view(numpy.uint32).tobytes(), \
+                (label, part, key)
+
+
+@pytest.mark.parametrize("steps", [24, 300])
+@pytest.mark.parametrize("name", sorted(db.RUNS))
+def test_driver_complex_run_matches_the_oracle(name, steps):
+    print(f"driver complex {name} x{steps}", flush=True)
+    drv = db.driver(name)
+    if steps < 128:
+        for _ in range(steps):
+            drv.step()
+    else:
+        drv.run(steps=steps)
+    drv.synchronize()
+    ref = db.oracle(drv, steps)
+    for _ in range(steps):
+        ref.step()
+    _same_parts(drv, ref, name)
+    ez = drv.get_array("Ez")
+    assert ez.dtype == numpy.complex64
+    assert numpy.array_equal(ez.real, ref.re.v["Ez"]) and numpy.array_equal(ez.imag, ref.im.v["Ez"])
+    assert numpy.any(ez.imag != 0)
+    drv.close()
+
+
+def test_a_real_run_is_the_real_part_of_the_complex_run():
+    steps = 150
+    print(f"driver real vs complex x{steps}", flush=True)
+    cplx = db.driver("xy_products", k=(0.0, 0.0, 0.0), pml=2)
+    real = db.driver("xy_products", complex_fields=False, k=(0.0, 0.0, 0.0), pml=2)
+    cplx.run(steps=steps)
+    real.run(steps=steps)
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Dx", "By"):
+        got = cplx.get_array(name)
+        want = real.get_array(name)
+        assert got.real.view(numpy.uint32).tobytes() == want.view(numpy.uint32).tobytes(), name
+    assert numpy.any(cplx.get_array("Ez").imag != 0)
+    cplx.close()
+    real.close()
