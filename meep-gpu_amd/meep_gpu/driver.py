@@ -171,8 +171,7 @@ class FdtdDriver:
 
     def _require_real(self, what: str):
         if self.force_complex_fields:
-            raise ValueError(f"{what} on complex storage is not built: the per-cell transforms "
-                             "of the two parts are not combined (build the run with "
+            raise ValueError(f"{what} on complex storage is not built (build the run with "
                              "force_complex_fields=False)")
 
     def _require_unfolded(self, what: str):
@@ -365,7 +364,6 @@ class FdtdDriver:
                          fcen: float = None, df: float = None, nfreq: int = None,
                          frequencies: Sequence[float] = None, decimation: int = 1) -> FluxMonitor:
         self._require_unfrozen("add_flux_monitor")
-        self._require_real("a flux monitor")
         self._require_unfolded("a flux monitor")
         self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
@@ -383,7 +381,6 @@ class FdtdDriver:
         weighted as MEEP places it (coordinates with their origin at the cell centre).
         ``decimation`` is the accumulation stride MEEP resolved for the monitor."""
         self._require_unfrozen("add_flux_region")
-        self._require_real("a flux region")
         self._require_unfolded("a flux region")
         self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
@@ -405,7 +402,6 @@ class FdtdDriver:
         without an absorber), resolved when stepping starts. ``decimation`` is an explicit
         positive accumulation stride."""
         self._require_unfrozen("add_dft_monitor")
-        self._require_real("a DFT monitor")
         self._require_unfolded("a DFT monitor")
         self._shaped = True
         freqs = (numpy.atleast_1d(numpy.asarray(frequencies, dtype=numpy.float64))
@@ -526,6 +522,10 @@ class FdtdDriver:
                 sim = ComplexSimulation(parts[0], parts[1])
             else:
                 sim = parts[0]
+            for mon in self.flux_monitors:
+                mon.attach(sim)
+            for mon in self.dft_monitors:
+                mon.attach(sim)
         except Exception:
             for p in parts:
                 p.close()
@@ -574,10 +574,6 @@ class FdtdDriver:
                                       -numpy.asarray(box.amp_re, dtype=numpy.float64))
                 sim.add_source(box.magnetic, axis, box.lo, box.cnt, amp_re, amp_im, table,
                                self.grid.dt, box.integrated)
-            for mon in self.flux_monitors:
-                mon.attach(sim)
-            for mon in self.dft_monitors:
-                mon.attach(sim)
             if pml is None and not self._poles and not self._offdiag:
                 sim.prime_e()
         except Exception:

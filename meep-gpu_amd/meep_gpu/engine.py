@@ -500,6 +500,56 @@ class ComplexSimulation:
         out.real, out.imag = re, im
         return out
 
+    # --- flux and DFT regions: one region on each part, read as one complex transform ---------
+    @property
+    def planes(self):
+        return self.sims[0].planes
+
+    def add_flux_plane(self, *args, **kw) -> int:
+        self._settle()
+        ids = [s.add_flux_plane(*args, **kw) for s in self.sims]
+        assert ids[0] == ids[1]
+        return ids[0]
+
+    def add_dft_region(self, *args, **kw) -> int:
+        self._settle()
+        ids = [s.add_dft_region(*args, **kw) for s in self.sims]
+        assert ids[0] == ids[1]
+        return ids[0]
+
+    def flux_read(self, plane: int) -> numpy.ndarray:
+        """The transforms of the complex field: the real part's plus i times the imaginary
+        part's (the transform is linear, and each part's is complex itself)."""
+        self._settle()
+        re, im = (s.flux_read(plane) for s in self.sims)
+        return re + 1j * im
+
+    def flux_spectrum(self, plane: int, cell_weights: Optional[numpy.ndarray] = None
+                      ) -> numpy.ndarray:
+        """sum over the plane of w * Re(E1 conj(H2) - E2 conj(H1)) of the complex field's
+        transforms, on the host: the two parts' cross terms need both planes' cells."""
+        from .dft import flux_spectrum
+
+        return flux_spectrum(self.flux_read(plane), 1.0, 1.0, cell_weights)
+
+    def flux_write(self, plane: int, transforms: numpy.ndarray) -> None:
+        """The whole transform into the real part and zeros into the imaginary part: their
+        combination is the transform written, and later samples add to both linearly."""
+        self._settle()
+        data = numpy.asarray(transforms)
+        self.sims[0].flux_write(plane, data)
+        self.sims[1].flux_write(plane, numpy.zeros_like(data))
+
+    def flux_scale(self, plane: int, scale: complex) -> None:
+        self._settle()
+        for s in self.sims:
+            s.flux_scale(plane, scale)
+
+    def flush(self):
+        self._settle()
+        for s in self.sims:
+            s.flush()
+
     def set_source_currents(self, index: int, currents: numpy.ndarray):
         """The same current table on both parts (each keeps its own amplitudes)."""
         self._settle()

@@ -58,6 +58,9 @@ def driver(name: str, seed: int = 11, complex_fields: bool = True, **overrides) 
         if spec["symmetry"]:
             continue  # placement under a mirror is refused; boxes go in below
         drv.add_volume_source(comp, center, size, GaussianSource(f, fw), amplitude=amp)
+    if spec.get("flux"):  # a z-normal flux plane between the sources and the absorber
+        drv.add_flux_monitor(2, (0, 0, shape[2] - 3), (shape[0], shape[1], 1),
+                             frequencies=[0.9, 1.1, 1.3])
     if spec["symmetry"]:
         from meep_gpu.sources import DepositBox
 
@@ -105,3 +108,25 @@ def oracle(drv: FdtdDriver, steps: int) -> oe.ComplexOracle:
     kinds = [0 if z is None else bloch_connection(z) for z in phases]
     return oe.ComplexOracle(parts[0], parts[1], kinds,
                             [(1.0, 0.0) if z is None else (z.real, z.imag) for z in phases])
+
+
+def oracle_planes(drv: FdtdDriver):
+    """Per flux monitor of the driver, the oracle's plane on each part (real, imaginary)."""
+    from oracle import workload as ow
+
+    return [tuple(ow.OraclePlane(m.normal, m.lo, m.cnt, m.omegas, m.scale) for _ in range(2))
+            for m in drv.flux_monitors]
+
+
+def step_oracle(ref: oe.ComplexOracle, planes, steps: int) -> None:
+    for _ in range(steps):
+        ref.step()
+        for pair in planes:
+            for plane, part in zip(pair, ref.parts):
+                plane.update(part, part.time, reference_precision=False)
+
+
+def complex_flux(pair, dx: float) -> numpy.ndarray:
+    """The flux of the complex field from the two parts' transforms (each complex itself)."""
+    e1, e2, h1, h2 = pair[0].acc64 + 1j * pair[1].acc64
+    return numpy.sum(numpy.real(e1 * numpy.conj(h2) - e2 * numpy.conj(h1)), axis=1) * dx * dx

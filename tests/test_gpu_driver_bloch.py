@@ -55,3 +55,27 @@ def test_a_real_run_is_the_real_part_of_the_complex_run():
     assert numpy.any(cplx.get_array("Ez").imag != 0)
     cplx.close()
     real.close()
+
+
+@pytest.mark.parametrize("steps", [40, 300])
+def test_flux_of_the_complex_field_matches_the_oracle(steps):
+    """A flux plane on complex storage: each part's transform on its own part, read as the
+    complex field's transform (real part's + i times imaginary part's); the spectrum holds the
+    two parts' cross terms. 1e-9 of the peak: both transforms sum in float64, in other orders."""
+    print(f"driver complex flux x{steps}", flush=True)
+    drv = db.driver("xy_products", flux=True)
+    drv.run(steps=steps)
+    drv.synchronize()
+    got = drv.flux_monitors[0].get_flux_spectrum()
+    ref = db.oracle(drv, steps)
+    planes = db.oracle_planes(drv)
+    db.step_oracle(ref, planes, steps)
+    _same_parts(drv, ref, "flux run")
+    want = db.complex_flux(planes[0], drv.grid.dx)
+    peak = numpy.max(numpy.abs(want))
+    assert peak > 0
+    assert numpy.max(numpy.abs(got - want)) <= 1e-9 * peak
+    # the cross terms matter: the two parts' own fluxes do not add up to it
+    own = sum(p.flux(drv.grid.dx) for p in planes[0])
+    assert numpy.max(numpy.abs(own - want)) > 1e-3 * peak
+    drv.close()
