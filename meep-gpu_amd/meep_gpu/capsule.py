@@ -48,7 +48,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy
 
 from . import _native
-from .chunks import IYEE, ChunkDivision, from_boxes
+from .chunks import IYEE, BoxDivision, ChunkDivision, from_boxes
 from .grid import Grid
 
 CAPSULE_SCHEMA = ("meep_gpu.capsule", 1)
@@ -233,6 +233,11 @@ class LiftedState:
     volumes_im: Dict[str, numpy.ndarray] = field(default_factory=dict)
     bloch_kinds: Tuple[int, int, int] = (0, 0, 0)
     bloch_phase: Tuple[Tuple[float, float], ...] = ((1.0, 0.0),) * 3
+
+    def read_phases(self) -> Tuple[Optional[complex], ...]:
+        """Per axis MEEP's Bloch factor in double (``eikna``), None where the axis has none."""
+        return tuple(None if k == 0 else complex(*p)
+                     for k, p in zip(self.bloch_kinds, self.bloch_phase))
     #: off-diagonal chi1inv rows, {"Ex": {"Ey": float32 volume}}
     offdiag: Dict[str, Dict[str, numpy.ndarray]] = field(default_factory=dict)
 
@@ -433,17 +438,18 @@ def lift_state(driver: Node, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedS
         for c, value in state["P_prev"].items():
             put(f"P_prev[{k}].{c}", value)
 
-    division = None  # the per-chunk facts of a folded division are not built (no
-    if not any(_folded_frame(frame) for frame in layout["frames"]):  # conductivity there)
+    frames = [_axis_frame(frame) for frame in layout["frames"]]
+    boxes = [(tuple(int(v) for v in lo), tuple(int(v) for v in hi)) for lo, hi in layout["boxes"]]
+    flags = [tuple(bool(f) for f in fl) for fl in layout["pml_flags"]]
+    if not any(_folded_frame(frame) for frame in layout["frames"]):
         try:
-            division = from_boxes(
-                [_axis_frame(frame) for frame in layout["frames"]],
-                [(tuple(lo), tuple(hi)) for lo, hi in layout["boxes"]],
-                [tuple(bool(f) for f in flags) for flags in layout["pml_flags"]])
+            division = from_boxes(frames, boxes, flags)
         except ValueError as err:
             raise CapsuleRefusal(f"the lifted chunk division: {err}") from None
-    elif conductivity:
+    elif conductivity:  # the per-chunk conductivity of a folded division is not built
         raise CapsuleRefusal("a conductivity on a mirror-folded grid is not built")
+    else:  # the boxes answer the read-out's question (which chunk owns a word)
+        division = BoxDivision(tuple(frames), tuple(boxes), tuple(flags))
     return LiftedState(
         division=division, shape=shape, dimensions=int(grid["dimensions"]), resolution=float(grid["resolution"]),
         courant=float(grid["courant"]), dx=float(grid["dx"]), dt=float(grid["dt"]),
@@ -476,6 +482,14 @@ def read_capsule(path, max_poles: int = _native.FDTD_MAX_POLES) -> LiftedState:
 
 
 # --- currents and digests -------------------------------------------------------------------
+def recording_runtime(path) -> str:
+    """The C++ runtime MEEP ran with where a currents table was recorded (its provenance's
+    ``system``): libc++ on Darwin, libstdc++ elsewhere."""
+    doc = json.loads(Path(path).read_text())
+    system = str(doc.get("provenance", {}).get("system", ""))
+    return "libc++" if system == "Darwin" else "libstdc++"
+
+
 def read_currents(path, state: LiftedState, steps: int) -> List[numpy.ndarray]:
     """Each source's current per step, ``(steps, 2)`` float64 (re, im), from a recorded
     currents table; its times must be the ones the driver asks for at each step."""
@@ -534,8 +548,10 @@ def read_digests(path) -> DigestSet:
 
 
 def word_digest(words: numpy.ndarray) -> str:
-    """SHA-256 of one component's words: little-endian float32 in C order (x, y, z)."""
-    return hashlib.sha256(numpy.ascontiguousarray(words, dtype="<f4").tobytes()).hexdigest()
+    """SHA-256 of one component's words: little-endian float32 in C order (x, y, z); complex
+    words as interleaved (real, imaginary) float32 pairs."""
+    kind = "<c8" if numpy.asarray(words).dtype.kind == "c" else "<f4"
+    return hashlib.sha256(numpy.ascontiguousarray(words, dtype=kind).tobytes()).hexdigest()
 
 
 # --- stepping a lifted state on the GPU -----------------------------------------------------
@@ -650,20 +666,27 @@ class LiftedRun:
             component = "B" + component[1]
         return self.sims[part].download(component)
 
-    def meep_array(self, component: str) -> numpy.ndarray:
+    #: the C++ runtime of MEEP's complex powers in the read-out (None: this host's); a
+    #: recorded run's is its recording host's (``recording_runtime``)
+    runtime: Optional[str] = None
+
+    def meep_array(self, component: str, runtime: Optional[str] = None) -> numpy.ndarray:
         """MEEP's whole-cell ``get_array(component)`` of the current words, with the lifted
-        chunk division's H/B alias rule (``readback.meep_array``)."""
+        chunk division's H/B alias rule (``readback.meep_array``): float32, or complex64 from
+        complex storage."""
         from .readback import meep_array
 
         state = self.state
-        if state.symmetry:
-            raise CapsuleRefusal("MEEP's get_array on a mirror-folded grid (the images "
-                                 "unfolded) is not built")
         words = {}  # downloaded before the host arithmetic changes mode
 
         def read(name):
             if name not in words:
-                words[name] = self.words(name)
+                parts = [self.words(name, p) for p in range(len(self.sims))]
+                if len(parts) == 1:
+                    words[name] = parts[0]
+                else:
+                    words[name] = numpy.empty(parts[0].shape, dtype=numpy.complex64)
+                    words[name].real, words[name].imag = parts
             return words[name]
 
         read(component)
@@ -672,7 +695,8 @@ class LiftedRun:
         with _native.host_flush(self.sim.subnormals == "flush"):
             return meep_array(read, component, state.grid(),
                               tuple(not m for m in state.metallic), state.division,
-                              state.pml is not None, self.step_count)
+                              state.pml is not None, self.step_count,
+                              phases=state.read_phases(), runtime=runtime or self.runtime)
 
     def close(self):
         for sim in self.sims:

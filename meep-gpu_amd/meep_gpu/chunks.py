@@ -118,10 +118,24 @@ class ChunkDivision:
 
     @classmethod
     def one_rank(cls, grid: Grid, faces, wraps: Sequence[bool] = (True, True, True)):
-        """``faces`` per axis: (low, high) layer thickness in cells."""
-        return cls(tuple(AxisCuts.flat() if grid.is_invariant(a) else
-                         AxisCuts.effort(grid.shape[a], *faces[a], wraps[a])
-                         for a in range(3)))
+        """``faces`` per axis: (low, high) layer thickness in cells. On a mirror-folded axis the
+        cut is the same rule over MEEP's owned window of the halved axis (its low face, the
+        mirror plane, carries no layer), and a periodic fold's far ghost row, past the window,
+        joins the last piece."""
+        axes = []
+        for a in range(3):
+            if grid.is_invariant(a):
+                axes.append(AxisCuts.flat())
+                continue
+            folds = getattr(grid, "folds", (None, None, None))
+            if folds[a] is None:
+                axes.append(AxisCuts.effort(grid.shape[a], *faces[a], wraps[a]))
+                continue
+            owned = grid.owned(a)
+            cut = AxisCuts.effort(owned, *faces[a], False)
+            axes.append(AxisCuts(grid.shape[a], cut.cuts[:-1] + (grid.shape[a],), cut.layer,
+                                 False))
+        return cls(tuple(axes))
 
     @property
     def count(self) -> int:
@@ -152,6 +166,90 @@ class ChunkDivision:
     def conductive_mask(self, sigma: numpy.ndarray, iyee: Sequence[int]) -> numpy.ndarray:
         """Per stored cell: does the chunk owning it keep this component's conductivity?"""
         return self.keeps(sigma, iyee)[self.owner_volume(iyee)]
+
+    def reader_masks(self, component: str, axes: Sequence[int], shape, first_step: bool):
+        """Per word of a B component's centred sum, words ordered (f, n1[, n2, n12]): (take
+        the H word, take a zero).
+
+        The chunk that reads a cell centre is the one owning the centre, row by row the piece
+        holding the centre's row on each axis. Along an integer-lattice axis the word above
+        the centre is always that chunk's own; the word below is another chunk's exactly at
+        the first row of the reader's piece (its low face). A word the reader does not own is
+        what MEEP's boundary exchange left in the reader's array there: where the reader holds
+        H as B's alias (no layer on the component's own axis) that is the owner's H word, and
+        where the reader keeps an H of its own it is +0 when the run has taken exactly one step
+        (``first_step``). MEEP array_slice.cpp over the chunk arrays, update_eh.cpp:162-172;
+        the reference states the same rule as HAliasRule.masks (readback.py:491-546)."""
+        own = "xyz".index(component[1])
+
+        def along(axis, rows):
+            view = [1, 1, 1]
+            view[axis] = shape[axis]
+            return numpy.asarray(rows).reshape(view)
+
+        aliasing = ~along(own, self.axes[own].layer_rows())
+        below = [along(axis, self.axes[axis].first_rows()) for axis in axes]
+        take_h, take_zero = [], []
+        for corner in range(1 << len(axes)):  # bit q set: the word above along axes[q]
+            notowned = numpy.zeros((1, 1, 1), dtype=bool)
+            for q in range(len(axes)):
+                if not corner >> q & 1:
+                    notowned = notowned | below[q]
+            take_h.append(numpy.broadcast_to(aliasing & notowned, shape))
+            stale = notowned & ~aliasing if first_step else numpy.zeros((1, 1, 1), dtype=bool)
+            take_zero.append(numpy.broadcast_to(stale, shape))
+        return take_h, take_zero
+
+
+@dataclass(frozen=True)
+class BoxDivision:
+    """A lifted division held as MEEP's chunk boxes ``(lo, hi]`` in doubled coordinates with
+    their per-axis layer flags, over per-axis frames ``(origin, stored, owned, wraps,
+    invariant)``: any division, the halved grid of a mirror fold included, for the one
+    question the read-out asks of it (``reader_masks``)."""
+    frames: Tuple[Tuple[int, int, int, bool, bool], ...]
+    boxes: Tuple[Tuple[Tuple[int, int, int], Tuple[int, int, int]], ...]
+    flags: Tuple[Tuple[bool, bool, bool], ...]
+
+    def _contains(self, box: int, coords) -> numpy.ndarray:
+        lo, hi = self.boxes[box]
+        inside = numpy.ones((1, 1, 1), dtype=bool)
+        for a in range(3):
+            if self.frames[a][4]:  # an invariant axis bounds nothing
+                continue
+            inside = inside & (coords[a] > lo[a]) & (coords[a] <= hi[a])
+        return inside
+
+    def reader_masks(self, component: str, axes: Sequence[int], shape, first_step: bool):
+        """The rule of ``ChunkDivision.reader_masks``, answered by box containment: the
+        reader is the box holding the centre (doubled ``origin + 2i + 1``), a word at
+        ``centre -/+ 1`` along the centred axes is the reader's when its box holds it."""
+        own = "xyz".index(component[1])
+        centre = []
+        for a in range(3):
+            view = [1, 1, 1]
+            view[a] = shape[a]
+            centre.append((self.frames[a][0] + 2 * numpy.arange(shape[a]) + 1).reshape(view))
+        reader = numpy.full(tuple(shape), -1, dtype=numpy.int64)
+        for k in range(len(self.boxes)):
+            reader = numpy.where((reader < 0) & self._contains(k, centre), k, reader)
+        aliasing = numpy.zeros(tuple(shape), dtype=bool)
+        for k in range(len(self.boxes)):
+            if not self.flags[k][own]:
+                aliasing |= reader == k
+        take_h, take_zero = [], []
+        for corner in range(1 << len(axes)):
+            coords = list(centre)
+            for q, axis in enumerate(axes):
+                coords[axis] = centre[axis] + (1 if corner >> q & 1 else -1)
+            owned = numpy.zeros(tuple(shape), dtype=bool)
+            for k in range(len(self.boxes)):
+                owned |= (reader == k) & self._contains(k, coords)
+            notowned = ~owned & (reader >= 0)
+            take_h.append(aliasing & notowned)
+            take_zero.append(notowned & ~aliasing if first_step
+                             else numpy.zeros(tuple(shape), dtype=bool))
+        return take_h, take_zero
 
 
 def from_boxes(frames: Sequence[Tuple[int, int, int, bool, bool]], boxes, layer_flags

@@ -169,11 +169,6 @@ class FdtdDriver:
         """The parts a batch steps: both parts of complex storage, else None."""
         return getattr(self.sim, "sims", None)
 
-    def _require_real(self, what: str):
-        if self.force_complex_fields:
-            raise ValueError(f"{what} on complex storage is not built (build the run with "
-                             "force_complex_fields=False)")
-
     def _require_unfolded(self, what: str):
         if any(f is not None for f in self.grid.folds):
             raise ValueError(f"{what} on a mirror-folded grid is not built: MEEP's placement "
@@ -633,15 +628,16 @@ class FdtdDriver:
     def get_meep_array(self, component: str, center=None, size=None) -> numpy.ndarray:
         """MEEP's ``sim.get_array(component)`` over the whole cell or a ``center``/``size``
         box: the centred lattice, word for word (``readback.meep_array``), where ``get_array``
-        gives the stored words."""
-        self._require_real("get_meep_array")
-        self._require_unfolded("get_meep_array")
+        gives the stored words. Complex64 on complex storage (with the Bloch window's
+        phases); a mirror-folded grid's images unfolded over the whole cell with their
+        parities."""
         sim = self.freeze()
         pml = sim.pml
         division = ChunkDivision.one_rank(
             self.grid, pml.faces if pml is not None else ((0, 0), (0, 0), (0, 0)), self.wraps)
         return meep_array(self.get_array, component, self.grid, self.wraps, division,
-                          pml is not None, sim.step_count, center=center, size=size)
+                          pml is not None, sim.step_count, center=center, size=size,
+                          phases=self.bloch_phases)
 
     def fast_path_report(self) -> Dict[str, object]:
         from .engine import device_info

@@ -26,12 +26,17 @@ HOLEY = "t.test_holey_wvg_bands.TestHoleyWvgBands.test_fields_at_kx"
 #: The cases the engine steps (the ring needs cylindrical storage, refused by name).
 CASES = ("a", "b", "ex.material-dispersion", "ex.absorber-1d", "c", "c_prime", HOLEY)
 #: Those on a mirror-folded grid (Y and Z odd mirrors over metallic boundaries, an Hx point
-#: source on both planes): their get_array read-out is not built.
+#: source on both planes; the holey waveguide's odd Y mirror over periodic boundaries).
 FOLDED = ("c", "c_prime", HOLEY)
 
 
 def case_dir(case: str, mode: str = "keep") -> str:
     return os.path.join(ROOT, mode, case)
+
+
+def runtime(case: str, mode: str = "keep") -> str:
+    """The C++ runtime MEEP ran with where the case was recorded."""
+    return capsule.recording_runtime(os.path.join(case_dir(case, mode), "currents.json"))
 
 
 def load(case: str, max_poles: int = 8, mode: str = "keep"):
@@ -131,6 +136,16 @@ class OracleRun:
             return sim.v["B" + component[1]]
         return sim.v[component]
 
+    runtime = None  # the C++ runtime of MEEP's complex powers (recording_runtime)
+
+    def read(self, name: str) -> numpy.ndarray:
+        """A stored volume: float32, or complex64 from the two parts."""
+        if len(self.sims) == 1:
+            return self.words(name)
+        out = numpy.empty(self.state.shape, dtype=numpy.complex64)
+        out.real, out.imag = self.words(name, 0), self.words(name, 1)
+        return out
+
     def meep_array(self, component: str) -> numpy.ndarray:
         """The product's readback over the oracle's words, under the run's subnormal policy."""
         from meep_gpu.readback import meep_array
@@ -138,9 +153,10 @@ class OracleRun:
 
         state = self.state
         with flushing(self.flush):
-            return meep_array(self.words, component, state.grid(),
+            return meep_array(self.read, component, state.grid(),
                               tuple(not m for m in state.metallic), state.division,
-                              state.pml is not None, self.step_count)
+                              state.pml is not None, self.step_count,
+                              phases=state.read_phases(), runtime=self.runtime)
 
     def close(self):
         pass

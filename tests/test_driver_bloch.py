@@ -73,8 +73,6 @@ def test_refused_pairings():
     with pytest.raises(ValueError, match="integrated"):
         drv.add_source(DepositBox("Ey", (3, 2, 2), (1, 1, 1), numpy.ones(1), numpy.zeros(1),
                                   integrated=True), GaussianSource(1.0, 0.5))
-    with pytest.raises(ValueError, match="get_meep_array"):
-        drv.get_meep_array("Ez")
 
 
 def test_boundaries_set_a_fold_s_stored_count():

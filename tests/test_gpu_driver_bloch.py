@@ -38,6 +38,24 @@ def test_driver_complex_run_matches_the_oracle(name, steps):
     assert ez.dtype == numpy.complex64
     assert numpy.array_equal(ez.real, ref.re.v["Ez"]) and numpy.array_equal(ez.imag, ref.im.v["Ez"])
     assert numpy.any(ez.imag != 0)
+    # MEEP's get_array of the same words: the Bloch window's phases, a fold's images
+    from meep_gpu.chunks import ChunkDivision
+    from meep_gpu.readback import meep_array
+
+    def oracle_read(name):
+        out = numpy.empty(drv.grid.shape, dtype=numpy.complex64)
+        out.real, out.imag = ref.re.stored_volumes()[name], ref.im.stored_volumes()[name]
+        return out
+
+    division = ChunkDivision.one_rank(drv.grid, drv.sim.pml.faces, drv.wraps)
+    for comp in ("Ez", "Hx", "By"):
+        got = drv.get_meep_array(comp)
+        want = meep_array(oracle_read, comp, drv.grid, drv.wraps, division, True, steps,
+                          phases=drv.bloch_phases)
+        assert got.dtype == numpy.complex64 and got.shape == want.shape
+        assert got.view(numpy.uint32).tobytes() == want.view(numpy.uint32).tobytes(), comp
+        if drv.grid.folds[1] is not None:  # the whole cell along the fold, images unfolded
+            assert got.shape[1] == drv.grid.shape_full[1]
     drv.close()
 
 

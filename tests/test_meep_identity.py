@@ -8,6 +8,7 @@ float32 words of one component at one step (steps 0-20, then every tenth of the 
 
 import hashlib
 import json
+import math
 import os
 
 import numpy
@@ -212,16 +213,12 @@ def test_a_planted_pole_word_is_caught():
     assert "Ez" in got[40]
 
 
-def test_a_folded_capsule_s_conductivity_and_read_out_are_refused_by_name():
+def test_a_folded_capsule_s_conductivity_is_refused_by_name():
     driver = capsule.read_capsule_document(os.path.join(cc.case_dir("c"), "capsule.capsule"))[1]
     sigma = numpy.ones(tuple(driver["fields"]["grid"]["shape"]), dtype=numpy.float32)
     driver["fields"].attrs["_conductivity"] = {"Dx": sigma}
     with pytest.raises(capsule.CapsuleRefusal, match="mirror-folded"):
         capsule.lift_state(driver)
-    state, digests, currents = cc.load("c")
-    run = cc.OracleRun(state, currents)
-    with pytest.raises(capsule.CapsuleRefusal, match="mirror-folded"):
-        capsule.LiftedRun.meep_array(run, "Ez")
 
 
 @pytest.mark.parametrize("planted", ["even_parity", "no_fill", "no_zero_top"])
@@ -253,14 +250,18 @@ def test_the_folded_replay_catches_planted_mistakes(planted, monkeypatch):
 
 
 @pytest.mark.parametrize("mode", cc.MODES)
-@pytest.mark.parametrize("case", [c for c in cc.CASES if c not in cc.FOLDED])
+@pytest.mark.parametrize("case", cc.CASES)
 def test_get_array_is_meep_s_on_the_oracle_s_words(case, mode):
     """readback.meep_array over the oracle's words gives MEEP's own sim.get_array words (the
     identity sets' reader digests) at every compared step: the centred lattice, MEEP's
-    collapse of a one-cell axis, and under a PML the chunks' H/B alias rule."""
+    collapse of a one-cell axis, under a PML the chunks' H/B alias rule, on a folded grid the
+    mirror images with their parities, and on complex storage with a Bloch phase the window's
+    connected words and the duplicate plane's phase."""
     state, digests, currents = cc.load(case, mode=mode)
     assert digests.reader
-    got = cc.replay_reader(cc.OracleRun(state, currents, subnormals=mode), digests)
+    run = cc.OracleRun(state, currents, subnormals=mode)
+    run.runtime = cc.runtime(case, mode)
+    got = cc.replay_reader(run, digests)
     assert {step: d for step, d in got.items() if d} == {}
 
 
@@ -277,8 +278,8 @@ def test_get_array_planted_mistakes_are_caught(planted, monkeypatch):
     if planted == "no_alias_rule":
         original = readback.centred_sums
         monkeypatch.setattr(readback, "centred_sums",
-                            lambda read, comp, inv, metal, div, pml, first_step:
-                            original(read, comp, inv, metal, div, False, first_step))
+                            lambda read, comp, inv, metal, div, pml, first_step, frame=None:
+                            original(read, comp, inv, metal, div, False, first_step, frame))
     else:
         run.state.metallic = (True, True, False)  # no duplicate plane on x and y
     got = cc.replay_reader(run, digests, steps=[1, 200])
@@ -316,3 +317,81 @@ def test_get_array_over_a_box_is_a_slice_of_the_whole_cell():
     assert numpy.array_equal(line, want)
     assert numpy.array_equal(
         meep_array(*args, center=(-0.025, -0.025, 0.0), size=(9.95, 9.95, 0.0)), whole)
+
+
+@pytest.mark.parametrize("planted", ["even_mirror", "no_reflection_sign", "no_window",
+                                     "unphased_duplicate"])
+def test_folded_and_complex_read_out_catches_planted_mistakes(planted, monkeypatch):
+    """MEEP's get_array words of ``c`` (odd Y and Z folds) catch a mirror ghost read with the
+    even sign and images unfolded without their parity; those of the holey waveguide (complex,
+    a Bloch factor of -1 on x) catch the window's word below read as stored row 0 and the
+    duplicate plane read without pow(eikna, -1)."""
+    from meep_gpu import readback
+
+    case = "c" if planted in ("even_mirror", "no_reflection_sign") else cc.HOLEY
+    if planted == "even_mirror":
+        monkeypatch.setattr(readback.ReadFrame, "parity", lambda self, comp, axis: 1)
+    elif planted == "no_reflection_sign":
+        monkeypatch.setattr(readback, "phase_shift", lambda frame, comp, mask: (1.0, 0.0))
+    elif planted == "no_window":
+        monkeypatch.setattr(readback.ReadFrame, "window_axes", lambda self, comp: ())
+    else:
+        monkeypatch.setattr(readback, "shift_phase", lambda frame, ishift: (1.0, 0.0))
+    state, digests, currents = cc.load(case)
+    run = cc.OracleRun(state, currents)
+    run.runtime = cc.runtime(case)
+    got = cc.replay_reader(run, digests, steps=[digests.steps[-1]])
+    assert any(got.values())
+
+
+def test_complex_powers_of_both_runtimes():
+    """pow(complex<double>, int) as libstdc++ forms it (repeated squaring; 1 over the power by
+    libgcc's division) and as libc++ does (exp(n log x)), against closed forms: a unit phase's
+    inverse is its conjugate, its square its product with itself, and the Bloch factor MEEP
+    recorded for the holey waveguide inverts to (-1, -residue) under libgcc's division."""
+    import cmath
+    import random
+
+    from meep_gpu.readback import _divide, cmul, cpow_int
+
+    rng = random.Random(7)
+    for _ in range(200):
+        z = cmath.exp(1j * rng.uniform(-math.pi, math.pi))
+        x = (z.real, z.imag)
+        for runtime in ("libstdc++", "libc++"):
+            inv = cpow_int(x, -1, runtime)
+            assert abs(complex(*inv) - z.conjugate()) < 4e-16
+            sq = cpow_int(x, 2, runtime)
+            assert abs(complex(*sq) - complex(*cmul(x, x))) < 4e-16
+            assert cpow_int(x, 0, runtime) == (1.0, 0.0)
+        a, b, c, d = (rng.uniform(-3, 3) for _ in range(4))
+        q = complex(a, b) / complex(c, d)
+        got = complex(*_divide(a, b, c, d))
+        assert abs(got - q) <= 4e-16 * abs(q)
+    eikna = (-1.0, 8.572527594031472e-16)
+    assert cpow_int(eikna, -1, "libstdc++") == (-1.0, -8.572527594031472e-16)
+    assert cpow_int((1.0, 0.0), -1, "libstdc++") == (1.0, 0.0)
+    assert math.copysign(1.0, cpow_int((1.0, 0.0), -1, "libc++")[1]) > 0
+
+
+@pytest.mark.parametrize("case", cc.FOLDED)
+def test_the_folded_division_rule_answers_as_meep_s_boxes(case):
+    """The one-rank division of a folded grid (MEEP's effort cut over the halved axis's owned
+    window, a periodic fold's far ghost row in the last piece) answers the read-out's question
+    -- which chunk owns each word of a B component's centred sum -- as the lifted chunk boxes
+    of MEEP's own run do, for every B component, at the first step and after. (A periodic
+    fold's far ghost row lies past every box and the read-out never takes its sums.)"""
+    from meep_gpu.chunks import ChunkDivision, IYEE
+
+    state, _, _ = cc.load(case)
+    grid = state.grid()
+    ours = ChunkDivision.one_rank(grid, state.pml.faces, tuple(not m for m in state.metallic))
+    window = tuple(slice(0, grid.owned(a)) for a in range(3))
+    for comp in ("Bx", "By", "Bz"):
+        axes = [a for a in range(3) if not state.invariant[a] and IYEE[comp][a] == 0]
+        for first in (True, False):
+            got = ours.reader_masks(comp, axes, state.shape, first)
+            want = state.division.reader_masks(comp, axes, state.shape, first)
+            for g, w in zip(got[0] + got[1], want[0] + want[1]):
+                assert numpy.array_equal(numpy.broadcast_to(g, state.shape)[window],
+                                         numpy.broadcast_to(w, state.shape)[window]), (comp, first)
