@@ -158,16 +158,17 @@ def measured_traffic(pair: str, kernels: str):
 def pair_kernels(sim, shape) -> dict:
     """The kernels each fused pair launches for this workload under ``shape`` ((H, D) of
     (cells per thread, workgroups per CU, tile order, non-temporal stores, shell cells per
-    thread)): the box's rows and the shell's rows as two lean kernels where the simulation runs
-    the lossless forms, which the benchmark workloads do."""
-    (hv, _, _, hnt, hsv), (dv, _, _, dnt, dsv) = shape
+    thread, the D pair's halo tile rows)): the box's rows and the shell's rows as two lean
+    kernels where the simulation runs the lossless forms, which the benchmark workloads do."""
+    (hv, _, _, hnt, hsv, _), (dv, _, _, dnt, dsv, tile) = shape
     widths = set(sim.inverse_epsilon_bytes())
     inv = 1 if widths == {1} else 0
     pole = "true" if len(sim.poles) == 1 else "false"
     b = lambda x: "true" if x else "false"
+    box = (f"dtile_kernel<{tile},{b(dnt)},{inv},{pole}>" if tile and dv == 1 else
+           f"dshell_kernel<{dv},{b(dnt)},{inv},{pole},true>")
     return {"h_pair": f"hshell_kernel<{hv},{b(hnt)},true>+hshell_kernel<{hsv},{b(hnt)},false>",
-            "d_pair": (f"dshell_kernel<{dv},{b(dnt)},{inv},{pole},true>"
-                       f"+dshell_kernel<{dsv},{b(dnt)},{inv},{pole},false>")}
+            "d_pair": f"{box}+dshell_kernel<{dsv},{b(dnt)},{inv},{pole},false>"}
 
 
 def compulsory_bytes(sim) -> dict:
@@ -229,12 +230,14 @@ def config_line(name: str, steps: int) -> dict:
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     finite = all(numpy.all(numpy.isfinite(m.get_flux_spectrum())) for m in drv.flux_monitors)
+    tile = sim.config().d[5]
     drv.close()
     return {"workload": f"{wl.name}: {wl.grid.total_cells} cells, {len(wl.poles)} Lorentz pole(s), "
                         f"{len(wl.planes)} flux planes x {wl.planes[0].frequencies.size} freqs",
             "steps_timed": steps, "value": round(wl.grid.total_cells * steps / elapsed / 1e6, 1),
             "unit": "Mcell-steps/s", "ms_per_step": round(elapsed * 1e3 / steps, 4),
-            "launch_shape": {"h_pair": list(shapes[0]), "d_pair": list(shapes[1])},
+            "launch_shape": {"h_pair": list(shapes[0]), "d_pair": list(shapes[1]),
+                             "d_halo_tile_rows": tile},
             "spectra_finite": bool(finite)}
 
 
@@ -259,12 +262,14 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
     drivers = []
     for wl in wls:
         drv = FdtdDriver.from_workload(wl)
-        drv.freeze(planned_steps=3 * steps + 3 * WARM + 4 * 16 * len(autotune.CANDIDATES) + 400)
+        drv.freeze(planned_steps=3 * steps + 3 * WARM
+                    + 2 * 16 * (2 * len(autotune.CANDIDATES) + len(autotune.TILE_ROWS)) + 400)
         drivers.append(drv)
     build_s = time.perf_counter() - t_build
     cells = sum(d.grid.total_cells for d in drivers)
     batch = BatchedRun(drivers)
     shape = autotune.tune_batch(batch, steps=16)
+    tile = batch.config().d[5]
     batch.flush()
     batch.run(WARM)  # warm: the first capture of the step graph, rings empty after it
     batch.synchronize()
@@ -317,7 +322,9 @@ def batched_line(count: int, steps: int, seed: int) -> dict:
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "note": "algorithmic bytes of both fused pairs per step, all members"},
-            "launch_shape": {"h_pair": list(shape[0]), "d_pair": list(shape[1])}, "graphs": GRAPHS, "spectra_finite": bool(finite),
+            "launch_shape": {"h_pair": list(shape[0]), "d_pair": list(shape[1]),
+                             "d_halo_tile_rows": tile},
+            "graphs": GRAPHS, "spectra_finite": bool(finite),
             "host_build_s": round(build_s, 2)}
 
 
@@ -368,7 +375,8 @@ def main(argv=None) -> int:
                     help="a second stream whose arrivals outpace the device (capacity); 0: none")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step eagerly, in every leg (for kernel tracing)")
-    ap.add_argument("--launch", default="", metavar="V,BPC,REMAP,NT[,SV][/V,BPC,REMAP,NT[,SV]]",
+    ap.add_argument("--launch", default="",
+                    metavar="V,BPC,REMAP,NT[,SV][/V,BPC,REMAP,NT[,SV[,TILE]]]",
                     help="fixed launch shapes (H pair[/D pair], SV the shell's cells per "
                          "thread) instead of the autotuned ones (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -401,22 +409,26 @@ def main(argv=None) -> int:
         GRAPHS = False
         set_graphs(False)
     wl = workloads.config(args.workload)
-    from meep_gpu.engine import get_launch_shell, set_launch_shell
+    from meep_gpu.engine import get_launch_shell, get_launch_tile, set_launch_shell
 
     if args.launch:
         parts = args.launch.split("/")
         shapes = []
+        tile = 0
         for pair, part in enumerate((parts[0], parts[-1])):
             fields = [int(x) for x in part.split(",")]
             v, bpc, remap, nt = fields[:4]
             set_launch_shell(pair, fields[4] if len(fields) > 4 else 1)
+            if pair == 1 and len(fields) > 5:
+                tile = fields[5]
             shapes.append((v, bpc, int(remap), bool(nt)))
         shape = tuple(shapes)
-        autotune.install(shape)
+        autotune.install(shape, tile=tile)
     else:
         shape = autotune.tune(wl.grid, PML(wl.grid, wl.pml_faces) if wl.pml_faces else None,
                               wl.poles, use_cache=False, inv=wl.inv)
-    shape = tuple(tuple(sh) + (get_launch_shell(pair),) for pair, sh in enumerate(shape))
+    shape = tuple(tuple(sh) + (get_launch_shell(pair), get_launch_tile() if pair else 0)
+                  for pair, sh in enumerate(shape))
     drv = FdtdDriver.from_workload(wl)
     sim = drv.freeze(planned_steps=args.warmup + args.steps + 1)
     print(f"bench: {wl.name} {wl.grid.shape} on {device_info()}", file=sys.stderr, flush=True)
@@ -487,7 +499,7 @@ def main(argv=None) -> int:
         "spectra_finite": bool(spectra_ok),
         "launch_shape": {name: {"cells_per_thread": sh[0], "blocks_per_cu": sh[1],
                                 "xcd_remap": int(sh[2]), "nt_stores": bool(sh[3]),
-                                "shell_cells_per_thread": sh[4]}
+                                "shell_cells_per_thread": sh[4], "halo_tile_rows": sh[5]}
                          for name, sh in (("h_pair", shape[0]), ("d_pair", shape[1]))},
     }
     drv.close()

@@ -331,6 +331,13 @@ int fdtd_get_launch_pair(int32_t pair, int32_t out[4]);
 int fdtd_set_launch_shell(int32_t pair, int32_t vec);
 int fdtd_get_launch_shell(int32_t pair, int32_t *vec);
 
+/* The D pair's box rows over LDS-staged halo tiles of `rows` rows (2, 4 or 8; a tile is
+ * 256 / rows cells along x, one cell a thread, with the row below and the column left of it
+ * staged in LDS), or 0: runs of vec cells; used where the D pair's vec is 1. Process default,
+ * as fdtd_set_launch_pair. No counterpart in the reference. */
+int fdtd_set_launch_tile(int32_t rows);
+int fdtd_get_launch_tile(int32_t *rows);
+
 /* enable = 1 (the default): fdtd_run and fdtd_batch_run replay a captured hipGraph of 128 whole
  * steps (both fused pairs, flux sampling and the transform passes that fall due, each launch
  * reading the step counters at its step's offset, and one counter advance at the end) wherever
@@ -340,15 +347,16 @@ int fdtd_get_launch_shell(int32_t pair, int32_t *vec);
 int fdtd_set_graphs(int32_t enable);
 
 /* A simulation's (or batch's) own launch configuration, in place of the process defaults the
- * setters above change: cfg[0..4] the H pair's shape (vec, blocks_per_cu, xcd_remap, nt,
- * shell vec), cfg[5..9] the D pair's, cfg[10] graphs (0/1), cfg[11] the flux transform (0, 1,
- * 2 or 4, as fdtd_set_flux_transform). NULL: follow the process defaults again. The get calls
- * write the configuration in force. Results are the same words under every configuration. No
- * counterpart in the reference. */
+ * setters above change: cfg[0..5] the H pair's shape (vec, blocks_per_cu, xcd_remap, nt,
+ * shell vec, halo tile rows), cfg[6..11] the D pair's, cfg[12] graphs (0/1), cfg[13] the flux
+ * transform (0, 1, 2 or 4, as fdtd_set_flux_transform). Halo tile rows: 0, or 2, 4 or 8 for
+ * the D pair's box rows over LDS-staged halo tiles of that many rows (vec 1 only). NULL:
+ * follow the process defaults again. The get calls write the configuration in force. Results
+ * are the same words under every configuration. No counterpart in the reference. */
 int fdtd_sim_set_config(fdtd_sim *sim, const int32_t *cfg);
-int fdtd_sim_get_config(fdtd_sim *sim, int32_t out[12]);
+int fdtd_sim_get_config(fdtd_sim *sim, int32_t out[14]);
 int fdtd_batch_set_config(fdtd_batch *batch, const int32_t *cfg);
-int fdtd_batch_get_config(fdtd_batch *batch, int32_t out[12]);
+int fdtd_batch_get_config(fdtd_batch *batch, int32_t out[14]);
 
 /* Where some plane decimates, the captured period is the least common multiple of the planes'
  * decimations, repeated up to 128 steps (at most 1024 steps, with fewer than 127 samples of any

@@ -1388,6 +1388,122 @@ __global__ void __launch_bounds__(kThreads) dshell_kernel(const SimDev *__restri
     }
 }
 
+// The box's D pair over LDS-staged halo tiles: a workgroup takes a tile of TX x TY cells of
+// one plane (one cell a thread, TX = kThreads / TY) and first stages the H words the tile's
+// curls read within the plane -- Hx and Hz with the row below the tile, Hy and Hz with the
+// column left of it -- so each word is loaded from memory once per tile, and the j-1 and i-1
+// neighbours come from LDS; the k-1 plane is read from memory (the caches hold it). The words
+// are those load_h_box reads, so the arithmetic (dshell_run's, with XO) is the same.
+template <int TY>
+struct DTileStage {
+    static constexpr int TX = kThreads / TY;
+    float hx[TY + 1][TX];      // row -1 .. TY-1
+    float hz[TY + 1][TX + 1];  // row -1 .. TY-1, column -1 .. TX-1
+    float hy[TY][TX + 1];      // column -1 .. TX-1
+};
+
+// Work item (simulation, 2, tile, 0): tiles numbered x fastest, then y, then z over the box.
+template <int TY>
+__device__ inline bool dtile_cell(const SimDev &s, int tile, int tx, int ty, int &i, int &j, int &k,
+                                  int &x0, int &y0) {
+    constexpr int TX = kThreads / TY;
+    const int bxn = (s.ibox[1] - s.ibox[0] + TX - 1) / TX;
+    const int byn = (s.ibox[3] - s.ibox[2] + TY - 1) / TY;
+    const int bx = tile % bxn, r = tile / bxn;
+    const int by = r % byn, bz = r / byn;
+    x0 = s.ibox[0] + bx * TX;
+    y0 = s.ibox[2] + by * TY;
+    i = x0 + tx;
+    j = y0 + ty;
+    k = s.ibox[4] + bz;
+    return i < s.ibox[1] && j < s.ibox[3];
+}
+
+template <int TY, bool NT, int INV, bool POLE>
+__global__ void __launch_bounds__(kThreads) dtile_kernel(const SimDev *__restrict__ sims,
+                                                         const int4 *__restrict__ items,
+                                                         int nitems, int xcd_remap, int adv) {
+    constexpr int TX = kThreads / TY;
+    __shared__ DTileStage<TY> st;
+    const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
+    const TileWalk w = tile_walk(nitems, xcd_remap);
+    for (int g = w.next; g < w.end; g += w.stride) {
+        const int t = tile_at(w, g);
+        if (t >= nitems) continue;  // the same for every thread of the workgroup
+        const int4 it = items[t];
+        const SimDev &s = sims[it.x];
+        int i, j, k, x0, y0;
+        const bool live = dtile_cell<TY>(s, it.z, tx, ty, i, j, k, x0, y0);
+        // every memory operand is requested before the stage's barrier, so the barrier waits
+        // while they are in flight
+        Run q;
+        Clock ck;
+        InvWords iw[3];
+        VF<1> h[3], hn[3][3], d[3];
+        DLayer<1, 0> l0;
+        DLayer<1, 1> l1;
+        DLayer<1, 2> l2;
+        bool active = false;
+        float hx = 0.f, hy = 0.f, hz = 0.f, hxm = 0.f, hzm = 0.f, hyl = 0.f, hzl = 0.f;
+        if (live) {
+            q = locate_ijk(s, i, j, k);
+            ck = clock_of(s, adv);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) iw[c] = load_inv_words<1, INV, NT>(s, c, q.at);
+            hx = ld_hb<1>(s, q, 0, 0).v[0];
+            hy = gp(s.B[1])[q.at];
+            hz = gp(s.B[2])[q.at];
+            if (ty == 0) {  // the row below the tile (periodic)
+                const Run qm = locate_ijk(s, i, y0 > 0 ? y0 - 1 : s.n[1] - 1, k);
+                hxm = ld_hb<1>(s, qm, 0, 0).v[0];
+                hzm = gp(s.B[2])[qm.at];
+            }
+            if (tx == 0) {  // the column left of the tile (the wrap within the stored row)
+                const int at = q.row + (x0 > 0 ? x0 - 1 : s.n[0] - 1);
+                hyl = gp(s.B[1])[at];
+                hzl = gp(s.B[2])[at];
+            }
+            hn[0][2] = ld_hb<1>(s, q, 0, q.zm);
+            hn[1][2] = ldv<1>(s.B[1], q.at + q.zm);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) d[c] = NT ? ldv_once<1>(s.D[c], q.at) : ldv<1>(s.D[c], q.at);
+            active = POLE && pole_run_active(s, q, 1);
+            load_dlayer<1, 0, POLE, true>(s, ck, q, active, l0);
+            load_dlayer<1, 1, POLE, true>(s, ck, q, active, l1);
+            load_dlayer<1, 2, POLE, true>(s, ck, q, active, l2);
+        }
+        __syncthreads();  // the previous tile's reads of the stage are done
+        if (live) {
+            st.hx[ty + 1][tx] = hx;
+            st.hy[ty][tx + 1] = hy;
+            st.hz[ty + 1][tx + 1] = hz;
+            if (ty == 0) {
+                st.hx[0][tx] = hxm;
+                st.hz[0][tx + 1] = hzm;
+            }
+            if (tx == 0) {
+                st.hy[ty][0] = hyl;
+                st.hz[ty + 1][0] = hzl;
+            }
+        }
+        __syncthreads();
+        if (!live) continue;
+        h[0].v[0] = hx;
+        h[1].v[0] = hy;
+        h[2].v[0] = hz;
+        hn[0][1].v[0] = st.hx[ty][tx];
+        hn[2][1].v[0] = st.hz[ty][tx + 1];
+        hn[1][0].v[0] = st.hy[ty][tx];
+        hn[2][0].v[0] = st.hz[ty + 1][tx];
+        VF<1> iv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) iv[c] = inv_values<1, INV>(s, c, q.at, iw[c]);
+        dlayer_component<1, 0, NT, POLE>(s, ck, q, h, hn, d[0], iv[0], active, l0);
+        dlayer_component<1, 1, NT, POLE>(s, ck, q, h, hn, d[1], iv[1], active, l1);
+        dlayer_component<1, 2, NT, POLE>(s, ck, q, h, hn, d[2], iv[2], active, l2);
+    }
+}
+
 template <int V, bool NT>
 __global__ void __launch_bounds__(kThreads) hint_kernel(const SimDev *__restrict__ sims,
                                                         const int4 *__restrict__ items,
@@ -2153,10 +2269,16 @@ struct LaunchShape {
     int nt = 0;             // non-temporal stores of the fused pairs' outputs
     int svec = 1;           // cells per thread of the shell's lean kernel (its layer forms hold
                             // more operands, so fewer cells a thread keep more waves resident)
+    int tile = 0;           // the D pair's box rows over LDS-staged halo tiles of this many rows
+                            // (2, 4 or 8; one cell a thread), or 0: runs of vec cells
 };
 // How one simulation's (or batch's) launches run: each fused pair's launch shape ([0] the H
 // pair's launches, [1] the D pair's and its row path), whether fdtd_run / fdtd_batch_run replay
 // captured graphs, and the flux transform kernel (1, 2 or 4: MFMA tiles per wave; 0: vector).
+// The words of a configuration: per pair (vec, blocks_per_cu, xcd_remap, nt, svec, tile),
+// then graphs and the flux transform.
+constexpr int kShapeWords = 6;
+constexpr int kConfigWords = 2 * kShapeWords + 2;
 struct LaunchConfig {
     LaunchShape shape[2];
     int graphs = 1;
@@ -2187,7 +2309,7 @@ int grid_for(const LaunchShape &sh, int ntiles) {
 
 bool same_shape(const LaunchShape &a, const LaunchShape &b) {
     return a.vec == b.vec && a.blocks_per_cu == b.blocks_per_cu && a.xcd_remap == b.xcd_remap &&
-           a.nt == b.nt && a.svec == b.svec;
+           a.nt == b.nt && a.svec == b.svec && a.tile == b.tile;
 }
 
 struct PlaneHost {
@@ -2206,6 +2328,8 @@ struct Group {
     std::map<int, int> ntiles;
     std::map<int, const int4 *> shell_tiles, int_tiles;  // the shell's items, the box's items
     std::map<int, int> nshell_tiles, nint_tiles;
+    std::map<int, const int4 *> lds_tiles;  // the box's LDS halo tiles, per tile row count
+    std::map<int, int> nlds_tiles;
     int max_npol = 0;     // the most poles a member has
     bool any_pml = false;  // some member has an absorber (its box rows hold x-layer cells)
     bool any_offdiag = false, all_offdiag = false;
@@ -2332,6 +2456,19 @@ void launch_dlean(const Group &g, int G, const int4 *l, int n, int rm, int adv, 
     else launch_dlean_inv<V, NT, 0, BOX>(g, G, l, n, rm, adv, st);
 }
 
+template <int TY>
+void launch_dtile(const Group &g, int G, const int4 *l, int n, int rm, int adv, hipStream_t st,
+                  bool nt) {
+    const bool pole = g.max_npol == 1;
+    if (g.inv_mode == 1) {
+        if (nt) { if (pole) dtile_kernel<TY, true, 1, true><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); else dtile_kernel<TY, true, 1, false><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); }
+        else { if (pole) dtile_kernel<TY, false, 1, true><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); else dtile_kernel<TY, false, 1, false><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); }
+    } else {
+        if (nt) { if (pole) dtile_kernel<TY, true, 0, true><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); else dtile_kernel<TY, true, 0, false><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); }
+        else { if (pole) dtile_kernel<TY, false, 0, true><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); else dtile_kernel<TY, false, 0, false><<<G, kThreads, 0, st>>>(g.dev, l, n, rm, adv); }
+    }
+}
+
 // ... then the counter advance by `advance_by` steps (0 inside a captured sequence, which
 // advances once at its end).
 int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
@@ -2345,7 +2482,14 @@ int launch_d(const Group &g, hipStream_t st, int adv = 0, int advance_by = 1) {
         const int ni = g.nint_tiles.at(V);
         const int4 *il = g.int_tiles.at(V);
         const int GI = grid_for(sh, ni);
-        if (ni) {
+        const int nl = sh.tile && V == 1 && g.any_pml ? g.nlds_tiles.at(sh.tile) : 0;
+        if (nl) {  // the box over LDS-staged halo tiles
+            const int4 *ll = g.lds_tiles.at(sh.tile);
+            const int GL = grid_for(sh, nl);
+            if (sh.tile == 2) launch_dtile<2>(g, GL, ll, nl, rm, adv, st, sh.nt);
+            else if (sh.tile == 4) launch_dtile<4>(g, GL, ll, nl, rm, adv, st, sh.nt);
+            else launch_dtile<8>(g, GL, ll, nl, rm, adv, st, sh.nt);
+        } else if (ni) {
             if (V == 4) { if (sh.nt) launch_dlean<4, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<4, false, true>(g, GI, il, ni, rm, adv, st); }
             else if (V == 2) { if (sh.nt) launch_dlean<2, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<2, false, true>(g, GI, il, ni, rm, adv, st); }
             else { if (sh.nt) launch_dlean<1, true, true>(g, GI, il, ni, rm, adv, st); else launch_dlean<1, false, true>(g, GI, il, ni, rm, adv, st); }
@@ -2452,6 +2596,8 @@ struct fdtd_sim {
     int nwork[3] = {0, 0, 0};
     const int4 *shellw[3] = {nullptr, nullptr, nullptr};  // the shell's items alone
     int nshellw[3] = {0, 0, 0};
+    const int4 *ldsw[3] = {nullptr, nullptr, nullptr};    // the box's halo tiles of 2, 4, 8 rows
+    int nldsw[3] = {0, 0, 0};
     const int4 *intw[3] = {nullptr, nullptr, nullptr};    // the interior box's items alone
     int nintw[3] = {0, 0, 0};
     LaunchConfig own_cfg;  // the simulation's own launch configuration, when has_cfg
@@ -2525,6 +2671,8 @@ struct fdtd_sim {
             g.nshell_tiles[1 << vi] = nshellw[vi];
             g.int_tiles[1 << vi] = intw[vi];
             g.nint_tiles[1 << vi] = nintw[vi];
+            g.lds_tiles[2 << vi] = ldsw[vi];
+            g.nlds_tiles[2 << vi] = nldsw[vi];
         }
         g.max_npol = host.npol;
         g.any_pml = host.pml != 0;
@@ -2656,6 +2804,24 @@ int fdtd_sim::prepare_work() {
             if (which) { intw[vi] = (const int4 *)p; nintw[vi] = (int)v.size(); }
             else { shellw[vi] = (const int4 *)p; nshellw[vi] = (int)v.size(); }
         }
+    }
+    // the box's LDS halo tiles of TY = 2, 4, 8 rows (TX = kThreads / TY cells along x)
+    for (int ti = 0; ti < 3; ++ti) {
+        const int TY = 2 << ti, TX = kThreads / TY;
+        int64_t n = 0;
+        if (h.pml && !empty)
+            n = (int64_t)((box[1] - box[0] + TX - 1) / TX) * ((box[3] - box[2] + TY - 1) / TY) *
+                (box[5] - box[4]);
+        if (n >= ((int64_t)1 << 31)) return fail(FDTD_EINVAL, "grid too large for the work lists");
+        std::vector<int4> items((size_t)n);
+        for (int64_t t = 0; t < n; ++t) items[(size_t)t] = make_int4(0, 2, (int)t, 0);
+        void *p = nullptr;
+        int rc = work_alloc(&p, sizeof(int4) * std::max<size_t>(1, items.size()));
+        if (rc) return rc;
+        if (!items.empty())
+            HIP_TRY(hipMemcpy(p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+        ldsw[ti] = (const int4 *)p;
+        nldsw[ti] = (int)n;
     }
     // the folded axes' ghost planes, in pieces of up to kFoldItem cells
     fold_host.clear();
@@ -3078,6 +3244,19 @@ int fdtd_set_launch_shell(int32_t pair, int32_t vec) {
 int fdtd_get_launch_shell(int32_t pair, int32_t *vec) {
     if ((pair != 0 && pair != 1) || !vec) return fail(FDTD_EINVAL, "bad argument");
     *vec = g_defaults.shape[pair].svec;
+    return FDTD_OK;
+}
+
+int fdtd_set_launch_tile(int32_t rows) {
+    if (rows != 0 && rows != 2 && rows != 4 && rows != 8)
+        return fail(FDTD_EINVAL, "halo tile rows must be 0, 2, 4 or 8, got %d", rows);
+    g_defaults.shape[1].tile = rows;
+    return FDTD_OK;
+}
+
+int fdtd_get_launch_tile(int32_t *rows) {
+    if (!rows) return fail(FDTD_EINVAL, "null argument");
+    *rows = g_defaults.shape[1].tile;
     return FDTD_OK;
 }
 
@@ -3703,7 +3882,7 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
 // [10] graphs, [11] flux transform; checked as the process-wide setters check them.
 static int parse_config(const int32_t *cfg, LaunchConfig *out) {
     for (int pair = 0; pair < 2; ++pair) {
-        const int32_t *c = cfg + 5 * pair;
+        const int32_t *c = cfg + kShapeWords * pair;
         if (c[0] != 1 && c[0] != 2 && c[0] != 4) return fail(FDTD_EINVAL, "vec must be 1, 2 or 4, got %d", c[0]);
         if (c[1] < 0 || c[1] > 64) return fail(FDTD_EINVAL, "blocks_per_cu %d", c[1]);
         if (c[2] < 0 || c[2] > 4096) return fail(FDTD_EINVAL, "xcd_remap %d", c[2]);
@@ -3714,26 +3893,33 @@ static int parse_config(const int32_t *cfg, LaunchConfig *out) {
         sh.xcd_remap = c[2];
         sh.nt = c[3] ? 1 : 0;
         sh.svec = c[4];
+        if (c[5] != 0 && c[5] != 2 && c[5] != 4 && c[5] != 8)
+            return fail(FDTD_EINVAL, "halo tile rows must be 0, 2, 4 or 8, got %d", c[5]);
+        if (c[5] && (pair != 1 || c[0] != 1))
+            return fail(FDTD_EINVAL, "halo tiles serve the D pair at one cell a thread");
+        sh.tile = c[5];
     }
-    if (cfg[11] != 0 && cfg[11] != 1 && cfg[11] != 2 && cfg[11] != 4)
-        return fail(FDTD_EINVAL, "flux transform must be 0, 1, 2 or 4, got %d", cfg[11]);
-    out->graphs = cfg[10] ? 1 : 0;
-    out->flux_mfma = cfg[11];
+    const int32_t *tail = cfg + 2 * kShapeWords;
+    if (tail[1] != 0 && tail[1] != 1 && tail[1] != 2 && tail[1] != 4)
+        return fail(FDTD_EINVAL, "flux transform must be 0, 1, 2 or 4, got %d", tail[1]);
+    out->graphs = tail[0] ? 1 : 0;
+    out->flux_mfma = tail[1];
     return FDTD_OK;
 }
 
-static void write_config(const LaunchConfig &c, int32_t out[12]) {
+static void write_config(const LaunchConfig &c, int32_t out[kConfigWords]) {
     for (int pair = 0; pair < 2; ++pair) {
         const LaunchShape &sh = c.shape[pair];
-        int32_t *o = out + 5 * pair;
+        int32_t *o = out + kShapeWords * pair;
         o[0] = sh.vec;
         o[1] = sh.blocks_per_cu;
         o[2] = sh.xcd_remap;
         o[3] = sh.nt;
         o[4] = sh.svec;
+        o[5] = sh.tile;
     }
-    out[10] = c.graphs;
-    out[11] = c.flux_mfma;
+    out[2 * kShapeWords] = c.graphs;
+    out[2 * kShapeWords + 1] = c.flux_mfma;
 }
 
 int fdtd_sim_set_config(fdtd_sim *s, const int32_t *cfg) {
@@ -3750,7 +3936,7 @@ int fdtd_sim_set_config(fdtd_sim *s, const int32_t *cfg) {
     return FDTD_OK;
 }
 
-int fdtd_sim_get_config(fdtd_sim *s, int32_t out[12]) {
+int fdtd_sim_get_config(fdtd_sim *s, int32_t out[kConfigWords]) {
     if (!s || !out) return fail(FDTD_EINVAL, "null argument");
     write_config(*s->config(), out);
     return FDTD_OK;
@@ -3769,7 +3955,7 @@ int fdtd_batch_set_config(fdtd_batch *b, const int32_t *cfg) {
     return FDTD_OK;
 }
 
-int fdtd_batch_get_config(fdtd_batch *b, int32_t out[12]) {
+int fdtd_batch_get_config(fdtd_batch *b, int32_t out[kConfigWords]) {
     if (!b || !out) return fail(FDTD_EINVAL, "null argument");
     write_config(*b->config(), out);
     return FDTD_OK;
@@ -4412,8 +4598,10 @@ static int batch_list(fdtd_batch *b, int vi, int which, const int4 **out, int *n
     std::vector<int4> items;
     for (int q = 0; q < (int)b->sims.size(); ++q) {
         fdtd_sim *s = b->sims[q];
-        const int4 *list = which == 0 ? s->work[vi] : which == 1 ? s->shellw[vi] : s->intw[vi];
-        const int n = which == 0 ? s->nwork[vi] : which == 1 ? s->nshellw[vi] : s->nintw[vi];
+        const int4 *list = which == 0 ? s->work[vi] : which == 1 ? s->shellw[vi]
+                           : which == 2 ? s->intw[vi] : s->ldsw[vi];
+        const int n = which == 0 ? s->nwork[vi] : which == 1 ? s->nshellw[vi]
+                      : which == 2 ? s->nintw[vi] : s->nldsw[vi];
         std::vector<int4> own(std::max(1, n));
         if (n) HIP_TRY(hipMemcpy(own.data(), list, sizeof(int4) * n, hipMemcpyDeviceToHost));
         for (int t = 0; t < n; ++t) {
@@ -4452,6 +4640,10 @@ static int batch_tiles(fdtd_batch *b, int V) {
     if (rc) return rc;
     g.int_tiles[V] = p;
     g.nint_tiles[V] = n;
+    rc = batch_list(b, vi, 3, &p, &n);  // (the halo tiles of 2 << vi rows)
+    if (rc) return rc;
+    g.lds_tiles[2 << vi] = p;
+    g.nlds_tiles[2 << vi] = n;
     return FDTD_OK;
 }
 
@@ -4531,6 +4723,7 @@ int fdtd_batch_create(fdtd_sim *const *sims, int32_t count, fdtd_batch **out) {
     for (const LaunchShape &sh : b->config()->shape) {  // the widths the launches take
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
+        if (rc == FDTD_OK && sh.tile) rc = batch_tiles(b, sh.tile / 2);  // its halo tiles
     }
     if (rc) { fdtd_batch_destroy(b); return rc; }
     // the folded members' ghost planes, re-numbered to their descriptors
@@ -4591,6 +4784,7 @@ static int batch_stepper(fdtd_batch *b, Stepper *k) {
     for (const LaunchShape &sh : b->config()->shape) {  // the widths the launches take
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.vec);
         if (rc == FDTD_OK) rc = batch_tiles(b, sh.svec);
+        if (rc == FDTD_OK && sh.tile) rc = batch_tiles(b, sh.tile / 2);  // its halo tiles
     }
     if (rc) return rc;
     k->g = b->group;

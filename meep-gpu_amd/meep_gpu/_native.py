@@ -105,6 +105,8 @@ SIGNATURES = {
     "fdtd_get_launch_pair": (ctypes.c_int, [ctypes.c_int32, _IP]),
     "fdtd_set_launch_shell": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
     "fdtd_get_launch_shell": (ctypes.c_int, [ctypes.c_int32, _IP]),
+    "fdtd_set_launch_tile": (ctypes.c_int, [ctypes.c_int32]),
+    "fdtd_get_launch_tile": (ctypes.c_int, [_IP]),
     "fdtd_set_graphs": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_set_flux_transform": (ctypes.c_int, [ctypes.c_int32]),
     "fdtd_batch_flush": (ctypes.c_int, [ctypes.c_void_p]),

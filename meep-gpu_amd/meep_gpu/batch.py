@@ -84,7 +84,7 @@ class BatchedRun:
     def config(self):
         from .engine import LaunchConfig
 
-        out = (ctypes.c_int32 * 12)()
+        out = (ctypes.c_int32 * LaunchConfig.WORDS)()
         check(self.lib.fdtd_batch_get_config(self.handle, out), "fdtd_batch_get_config", self.lib)
         return LaunchConfig.from_words(list(out))
 

@@ -274,7 +274,7 @@ class HipSimulation:
 
     def config(self) -> "LaunchConfig":
         """The launch configuration in force for this simulation."""
-        out = numpy.zeros(12, dtype=numpy.int32)
+        out = numpy.zeros(LaunchConfig.WORDS, dtype=numpy.int32)
         self._check(self.lib.fdtd_sim_get_config(self.handle, iptr(out)), "fdtd_sim_get_config")
         return LaunchConfig.from_words(out)
 
@@ -397,30 +397,46 @@ def copy_probe(nbytes: int, repeats: int = 20, mode: int = 0) -> float:
 class LaunchConfig:
     """How one simulation's (or batch's) launches run: per fused pair (H, D) the shape
     ``(cells per thread, workgroups per CU or 0, tile order, non-temporal stores, shell cells
-    per thread)``, graph replay on or off, and the flux transform (1, 2 or 4 MFMA tiles per
-    wave, 0 the vector kernel). ``fdtd_sim_set_config`` / ``fdtd_batch_set_config`` words."""
-    h: Tuple[int, int, int, bool, int] = (1, 8, 0, False, 1)
-    d: Tuple[int, int, int, bool, int] = (1, 8, 0, False, 1)
+    per thread, halo tile rows)``, graph replay on or off, and the flux transform (1, 2 or 4
+    MFMA tiles per wave, 0 the vector kernel). Halo tile rows (the D pair only, one cell a
+    thread): 0 for runs of cells, or 2, 4, 8 for the box's rows over LDS-staged tiles of
+    ``256 / rows`` x ``rows`` cells. ``fdtd_sim_set_config`` / ``fdtd_batch_set_config``
+    words."""
+    h: Tuple[int, int, int, bool, int, int] = (1, 8, 0, False, 1, 0)
+    d: Tuple[int, int, int, bool, int, int] = (1, 8, 0, False, 1, 0)
     graphs: bool = True
     flux_mfma: int = 1
 
+    #: words per pair's shape, and in all
+    SHAPE_WORDS = 6
+    WORDS = 14
+
     def words(self) -> numpy.ndarray:
-        out = [int(v) for v in self.h] + [int(v) for v in self.d]
+        out = [int(v) for v in _full(self.h)] + [int(v) for v in _full(self.d)]
         return numpy.array(out + [1 if self.graphs else 0, int(self.flux_mfma)], dtype=numpy.int32)
 
     @classmethod
     def from_words(cls, w) -> "LaunchConfig":
         w = [int(v) for v in w]
-        h = (w[0], w[1], w[2], bool(w[3]), w[4])
-        d = (w[5], w[6], w[7], bool(w[8]), w[9])
-        return cls(h, d, bool(w[10]), w[11])
+        h = (w[0], w[1], w[2], bool(w[3]), w[4], w[5])
+        d = (w[6], w[7], w[8], bool(w[9]), w[10], w[11])
+        return cls(h, d, bool(w[12]), w[13])
 
     @classmethod
-    def of_shapes(cls, shapes, shell=(1, 1), graphs: bool = True, flux_mfma: int = 1):
-        """From (H shape, D shape) 4-tuples as the autotuner returns them."""
-        h = tuple(shapes[0][:4]) + (int(shell[0]),)
-        d = tuple(shapes[1][:4]) + (int(shell[1]),)
+    def of_shapes(cls, shapes, shell=(1, 1), graphs: bool = True, flux_mfma: int = 1,
+                  tile: int = 0):
+        """From (H shape, D shape) 4-tuples as the autotuner returns them (a D shape may carry
+        its halo tile rows as a fifth entry; ``tile`` otherwise)."""
+        h = tuple(shapes[0][:4]) + (int(shell[0]), 0)
+        dt = int(shapes[1][4]) if len(shapes[1]) > 4 else int(tile)
+        d = tuple(shapes[1][:4]) + (int(shell[1]), dt)
         return cls(h, d, graphs, flux_mfma)
+
+
+def _full(shape) -> Tuple:
+    """A pair's shape with its halo tile rows (0 when a 5-entry shape leaves them out)."""
+    shape = tuple(shape)
+    return shape + (0,) * (6 - len(shape))
 
 
 def link_parts(re: "HipSimulation", im: "HipSimulation", kinds: Sequence[int],
@@ -586,6 +602,17 @@ def set_launch_pair(pair: int, vec: int, blocks_per_cu: int, xcd_remap: int,
 def set_launch_shell(pair: int, vec: int):
     """Cells per thread of one pair's shell kernel (the layer forms outside the interior box)."""
     check(_native.load().fdtd_set_launch_shell(int(pair), int(vec)), "fdtd_set_launch_shell")
+
+
+def set_launch_tile(rows: int):
+    """The D pair's box rows over LDS-staged halo tiles of ``rows`` rows (2, 4, 8), or 0."""
+    check(_native.load().fdtd_set_launch_tile(int(rows)), "fdtd_set_launch_tile")
+
+
+def get_launch_tile() -> int:
+    out = numpy.zeros(1, dtype=numpy.int32)
+    check(_native.load().fdtd_get_launch_tile(iptr(out)), "fdtd_get_launch_tile")
+    return int(out[0])
 
 
 def get_launch_shell(pair: int) -> int:

@@ -35,6 +35,61 @@ def test_every_launch_shape_reproduces_the_recorded_digests(shapes, golden_diges
         install(saved)
 
 
+@pytest.mark.parametrize("rows", [2, 4, 8])
+def test_the_d_pair_s_halo_tiles_reproduce_the_recorded_digests(rows, golden_digests):
+    """The D pair's box rows over LDS-staged halo tiles (the row below and the column left of
+    each 256 / rows x rows tile staged once) step the same words as runs of cells: the
+    recorded digests of the lossless cases with an absorber (3-D, and 2-D with a pole)."""
+    from meep_gpu.autotune import install
+    from meep_gpu.engine import get_launch_pair, get_launch_tile
+    from oracle import cases
+    from tests import hip_cases
+
+    saved = (get_launch_pair(0), get_launch_pair(1))
+    install(((1, 0, 17, True), (1, 0, 17, True)), tile=rows)
+    try:
+        assert get_launch_tile() == rows
+        for name in ("pml_3d", "dispersive_pml_2d", "pml_2d"):
+            print(f"halo tiles of {rows} rows, case {name}", flush=True)
+            digest, _ = hip_cases.hip_case_digest(cases.build_state(name))
+            assert digest == golden_digests["case_sha256"][name], (rows, name)
+    finally:
+        install(saved, tile=0)
+
+
+def test_halo_tiles_in_a_batch_step_each_member_as_runs_do():
+    """The synthetic members (edges 28-44, half with a Lorentz pole) stepped by one batch
+    whose D pair runs 32 x 8 halo tiles, replayed from graphs, against each alone over runs."""
+    import dataclasses
+
+    from meep_gpu.batch import BatchedRun
+    from meep_gpu.driver import FdtdDriver
+
+    steps = 300
+    alone, together = [], []
+    for wl in _members():
+        a = FdtdDriver.from_workload(wl)
+        a.freeze(planned_steps=steps + 1)
+        a.run(steps=steps)
+        a.synchronize()
+        alone.append(a)
+        b = FdtdDriver.from_workload(wl)
+        b.freeze(planned_steps=steps + 1)
+        together.append(b)
+    print(f"batch with halo tiles x{steps}", flush=True)
+    batch = BatchedRun(together)
+    cfg = batch.config()
+    batch.set_config(dataclasses.replace(cfg, d=(1, 0, 17, True, 1, 8)))
+    assert batch.config().d[5] == 8
+    batch.run(steps)
+    batch.synchronize()
+    batch.close()
+    for a, b in zip(alone, together):
+        _same_words(a, b)
+    for d in alone + together:
+        d.close()
+
+
 def _members():
     from meep_gpu.batch import synthetic_batch
 

@@ -72,7 +72,8 @@ def test_each_simulation_keeps_its_own_launch_configuration():
     b = FdtdDriver.from_workload(small("sphere"))
     a.freeze(planned_steps=steps)
     b.freeze(planned_steps=steps)
-    own = LaunchConfig(h=(4, 0, 1, True, 2), d=(2, 4, 0, False, 1), graphs=False, flux_mfma=0)
+    own = LaunchConfig(h=(4, 0, 1, True, 2, 0), d=(1, 4, 0, False, 1, 4), graphs=False,
+                       flux_mfma=0)
     a.sim.set_config(own)
     saved = get_launch_pair(0)
     try:
