@@ -4228,9 +4228,6 @@ int fdtd_synchronize(fdtd_sim *s) {
 static int add_region(fdtd_sim *s, const int32_t lo[3], const int32_t cnt[3], int nslot,
                       const int32_t *comps, int yee, int32_t nf, const double *omega,
                       double scale, int32_t dec, int32_t *plane) {
-    if (s->host.fold[0] || s->host.fold[1] || s->host.fold[2])
-        return fail(FDTD_EINVAL, "flux and DFT regions on a folded grid are not supported: the "
-                                 "reference's read-out weighs a folded plane by its images");
     s->graph.reset();  // a captured sequence samples the regions it was captured with
     int cells = 1;
     for (int a = 0; a < 3; ++a) {

@@ -93,6 +93,31 @@ def ladder_weight(index: int, count: int, s0: float, s1: float, e0: float, e1: f
     return 1.0
 
 
+def folded_axis_cells(grid, axis: int, where_min: float, where_max: float, sign: int = 1
+                      ) -> Tuple[int, numpy.ndarray]:
+    """(first stored row, per-row weights) of one direction of a monitor volume on a
+    mirror-folded axis: the ladder of the request over the full cell (its origin at the
+    mirror plane), each cell centre served where MEEP's symmetry images serve it
+    (``Grid.fold_site``: the site at doubled -1 directly, those below it at their mirror
+    images); a row reached through an odd number of reflections adds its weight times
+    ``sign`` (the image's flux sign), and two sites meeting on a row add."""
+    a = 1.0 / grid.dx
+    is_d, ie_d, (s0, s1, e0, e1) = boundary_weight_ladder(where_min, where_max, a)
+    count = (ie_d - is_d) // 2 + 1
+    rows = {}
+    for q in range(count):
+        site = grid.fold_site(axis, is_d + 2 * q, 1)
+        if site is None:
+            raise ValueError(f"monitor extent [{where_min}, {where_max}] leaves the grid")
+        w = ladder_weight(q, count, s0, s1, e0, e1) * (sign if site[1] % 2 else 1)
+        rows[site[0]] = rows.get(site[0], 0.0) + w
+    first, last = min(rows), max(rows)
+    out = numpy.zeros(last - first + 1)
+    for row, w in rows.items():
+        out[row - first] = w
+    return first, out
+
+
 def axis_cells(n: int, dx: float, where_min: float, where_max: float) -> Tuple[int, numpy.ndarray]:
     """(first cell index, per-cell weights) of one resolved direction of a monitor volume whose
     coordinates have their origin at the cell centre (the grid's cell centres at
@@ -216,13 +241,18 @@ class FluxMonitor:
     @classmethod
     def from_region(cls, grid, center, size, frequencies, decimation: int = 1) -> "FluxMonitor":
         """MEEP's ``sim.add_flux(..., mp.FluxRegion(center, size))`` on this grid: exactly one
-        axis of ``size`` is zero (the normal); coordinates have their origin at the cell centre."""
+        axis of ``size`` is zero (the normal); coordinates have their origin at the cell centre.
+        On a mirror-folded axis the region is laid on the full cell and folded onto the stored
+        rows (``folded_axis_cells``): a mirror image counts with the sign its flux takes, -1
+        through a plane normal to the monitor's normal (the image points the other way), +1
+        through one along it (E_t1 H_t2 and E_t2 H_t1 both keep their sign there)."""
         flat = [a for a in range(3) if not grid.is_invariant(a) and size[a] == 0]
         if len(flat) != 1:
             raise ValueError("a flux region is flat along exactly one resolved axis")
         normal = flat[0]
         lo, cnt, weights = [0, 0, 0], [1, 1, 1], []
         planes = None
+        folds = getattr(grid, "folds", (None, None, None))
         for a in range(3):
             n = grid.shape[a]
             if grid.is_invariant(a):
@@ -230,7 +260,10 @@ class FluxMonitor:
                 continue
             lo_w = center[a] - 0.5 * size[a]
             hi_w = center[a] + 0.5 * size[a]
-            first, w = axis_cells(n, grid.dx, lo_w, hi_w)
+            if folds[a] is not None:
+                first, w = folded_axis_cells(grid, a, lo_w, hi_w, -1 if a == normal else 1)
+            else:
+                first, w = axis_cells(n, grid.dx, lo_w, hi_w)
             if a == normal:  # a plane of weight zero contributes nothing: not registered
                 planes = [(first + q, float(w[q])) for q in range(w.size) if w[q] != 0.0]
                 lo[a], cnt[a] = first, 1

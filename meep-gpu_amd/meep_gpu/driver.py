@@ -171,8 +171,8 @@ class FdtdDriver:
 
     def _require_unfolded(self, what: str):
         if any(f is not None for f in self.grid.folds):
-            raise ValueError(f"{what} on a mirror-folded grid is not built: MEEP's placement "
-                             "and weights under a symmetry are not restated here")
+            raise ValueError(f"{what} on a mirror-folded grid is not built: the region's "
+                             "mirror images are not unfolded into its arrays")
 
     # --- configuration (before the first step) ------------------------------------------------
     def _require_unfrozen(self, what: str):
@@ -341,7 +341,6 @@ class FdtdDriver:
         """``mp.Source(src, component, center, size, amplitude, is_integrated)``: placed as MEEP
         places it on this grid and its boundaries (``meep_volume_source``); returns the boxes."""
         self._require_unfrozen("add_volume_source")
-        self._require_unfolded("add_volume_source")
         for a, phase in enumerate(self.bloch_phases):
             half = 0.5 * self.grid.shape_full[a] * self.grid.dx
             lo, hi = center[a] - 0.5 * size[a], center[a] + 0.5 * size[a]
@@ -359,7 +358,6 @@ class FdtdDriver:
                          fcen: float = None, df: float = None, nfreq: int = None,
                          frequencies: Sequence[float] = None, decimation: int = 1) -> FluxMonitor:
         self._require_unfrozen("add_flux_monitor")
-        self._require_unfolded("a flux monitor")
         self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))
@@ -376,7 +374,6 @@ class FdtdDriver:
         weighted as MEEP places it (coordinates with their origin at the cell centre).
         ``decimation`` is the accumulation stride MEEP resolved for the monitor."""
         self._require_unfrozen("add_flux_region")
-        self._require_unfolded("a flux region")
         self._shaped = True
         freqs = (numpy.asarray(frequencies, dtype=numpy.float64) if frequencies is not None
                  else meep_frequency_span(fcen, df, nfreq))

@@ -90,6 +90,41 @@ class Grid:
         fold = self.folds[axis]
         return self.shape[axis] - (1 if fold is not None and fold[1] == 2 else 0)
 
+    def fold_site(self, axis: int, doubled: int, parity: int):
+        """Where MEEP serves a requested site of a folded axis (``loop_in_chunks``' symmetry
+        images, the reference's ``folded_axis_sites`` dft.py:1004-1100): the site at doubled
+        full-cell coordinate ``doubled`` of a component with Yee shift ``parity`` on the axis
+        is first brought into the full cell's owned window by a lattice vector (periodic
+        boundaries; past a metallic face nothing serves it), then, below the plane (under
+        ``-parity``), reflected to ``-doubled``; on a periodic fold a site past the stored
+        window is reflected again through the second mirror at doubled ``N``. Returns
+        ``(stored row, reflections)`` -- row ``j`` sits at doubled ``2 j - 2 + parity`` -- or
+        None. Each reflection multiplies the value by the component's parity."""
+        fold = self.folds[axis]
+        full = self.shape_full[axis]
+        lattice = 2 * full
+        little = -(full - full % 2) + 2 - parity
+        big = little + lattice - 2
+        if doubled < little or doubled > big:
+            if fold[1] == 1:  # metallic: no lattice vector; the wall holds zero
+                return None
+            doubled += lattice if doubled < little else -lattice
+            if doubled < little or doubled > big:
+                return None
+        flips = 0
+        if doubled < -parity:
+            doubled, flips = -doubled, 1
+        top = self.owned(axis) - 1  # a periodic fold's far ghost row is not owned
+        row = (doubled + 2 - parity) // 2
+        if row > top:
+            if fold[1] == 1:
+                return None
+            doubled, flips = lattice - doubled, flips + 1
+            row = (doubled + 2 - parity) // 2
+            if row < 0 or row > top or doubled < -parity:
+                return None
+        return row, flips
+
     def is_invariant(self, axis: int) -> bool:
         """MEEP's assignment: z at 2-D, x and y at 1-D."""
         return (self.dimensions == 2 and axis == 2) or (self.dimensions == 1 and axis < 2)

@@ -231,6 +231,51 @@ def _axis_passes(n: int, a: float, iyee: int, where_min: float, where_max: float
     return out
 
 
+def _fold_parity(grid, component: str, axis: int) -> int:
+    """The component's sign under the fold of ``axis`` (MEEP's phase_shift): a vector odd
+    about the plane its own axis is normal to, a pseudovector even there and odd about the
+    others, times the plane's declared phase."""
+    own = "xyz".index(component[1]) == axis
+    declared = grid.folds[axis][0]
+    return -declared if own != (component[0] in "HB") else declared
+
+
+def _folded_axis_passes(grid, component: str, d: int, a: float, iyee: int, where_min: float,
+                        where_max: float):
+    """A mirror-folded direction of a source request: MEEP's two symmetry passes over it
+    (``loop_in_chunks`` with the images, sources.cpp:456-489). The request's samples are laid
+    on the full cell (its origin at the mirror plane); pass 0 takes those at and above the
+    component's first owned sample (doubled ``-iyee``) directly, pass 1 those below it through
+    the reflection, each clipped run with MEEP's end weights (``_end_weights``), and each
+    sample lands where ``Grid.fold_site`` serves it. Returns {pass: [(first_row, weights)]},
+    each weight times the component's parity once per reflection."""
+    parity = _fold_parity(grid, component, d)
+    is_d, ie_d, ends = boundary_weight_ladder(where_min, where_max, a, 1 - iyee)
+    out = {}
+    for bit, (start, end) in ((0, (max(is_d, -iyee), ie_d)), (1, (is_d, min(ie_d, -iyee - 2)))):
+        if start > end:
+            continue
+        ws = _end_weights(start, end, is_d, ie_d, ends)
+        count = (end - start) // 2 + 1
+        served = {}
+        for q in range(count):
+            site = grid.fold_site(d, start + 2 * q, iyee)
+            if site is not None:
+                served[site[0]] = ladder_weight(q, count, *ws) * (parity ** site[1])
+        runs, rows, weights = [], [], []
+        for row in sorted(served):
+            if rows and row != rows[-1] + 1:
+                runs.append((rows[0], numpy.array(weights)))
+                rows, weights = [], []
+            rows.append(row)
+            weights.append(served[row])
+        if rows:
+            runs.append((rows[0], numpy.array(weights)))
+        if runs:
+            out[bit] = runs
+    return out
+
+
 def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequence[float],
                        amplitude: complex = 1.0, metallic: Sequence[bool] = (False, False, False),
                        integrated: bool = False) -> List[DepositBox]:
@@ -251,6 +296,9 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
     The point weight is ``w_z * (w_y * w_x)`` (vec.hpp:381-383, exact in any grouping of the
     two-factor products) and the amplitude meets it, and the shift phase 1, as complex
     products. ``integrated`` marks the boxes as MEEP's polarization source (electric only).
+    On a mirror-folded axis the request is laid on the full cell and each sample served as
+    MEEP's symmetry passes serve it (``_folded_axis_passes``): a sample below the plane
+    deposits at its mirror image with the component's parity.
     """
     if integrated and component[0] in "HB":
         raise ValueError("a magnetic integrated source is not supported: the engine keeps H "
@@ -259,7 +307,9 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
     a = float(grid.resolution)
     inva = 1.0 / a
     amp = (float(complex(amplitude).real), float(complex(amplitude).imag))
-    lengths = tuple(grid.shape[d] * grid.dx for d in range(3))
+    folds = tuple(getattr(grid, "folds", (None, None, None)))
+    full = tuple(getattr(grid, "shape_full", grid.shape))
+    lengths = tuple(full[d] * grid.dx for d in range(3))
     per_axis = []
     for d in range(3):
         n = grid.shape[d]
@@ -279,21 +329,34 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
             hi_w = lo_w + lengths[d]
         if hi_w - lo_w == 0.0:
             amp = (amp[0] * a, amp[1] * a)
-        per_axis.append(_axis_passes(n, a, iyee[d], lo_w, hi_w, not metallic[d]))
+        if folds[d] is not None:
+            per_axis.append(_folded_axis_passes(grid, component, d, a, iyee[d], lo_w, hi_w))
+        else:
+            per_axis.append(_axis_passes(n, a, iyee[d], lo_w, hi_w, not metallic[d]))
     pass_amp = _cmul(amp, (1.0, -0.0))
     boxes: List[DepositBox] = []
-    # MEEP's shift odometer runs its first direction fastest
-    for kz in sorted(per_axis[2]):
-        for ky in sorted(per_axis[1]):
-            for kx in sorted(per_axis[0]):
-                for x0, wx in per_axis[0][kx]:
-                    for y0, wy in per_axis[1][ky]:
-                        for z0, wz in per_axis[2][kz]:
-                            w = (wx[:, None, None] * wy[None, :, None]) * wz[None, None, :]
-                            re = w * pass_amp[0]
-                            im = w * pass_amp[1]
-                            # times the amplitude function's value, one: (re*1 - im*0, re*0 + im*1)
-                            boxes.append(DepositBox(component, (x0, y0, z0), w.shape,
-                                                    re * 1.0 - im * 0.0, re * 0.0 + im * 1.0,
-                                                    integrated))
+    folded = [d for d in range(3) if folds[d] is not None]
+    # the symmetry passes outermost (their images in chain order); within each, MEEP's shift
+    # odometer runs its first direction fastest
+    for sn in range(1 << len(folded)):
+        bit = {d: sn >> q & 1 for q, d in enumerate(folded)}
+
+        def keys(d):
+            if d in bit:
+                return [bit[d]] if bit[d] in per_axis[d] else []
+            return sorted(per_axis[d])
+
+        for kz in keys(2):
+            for ky in keys(1):
+                for kx in keys(0):
+                    for x0, wx in per_axis[0][kx]:
+                        for y0, wy in per_axis[1][ky]:
+                            for z0, wz in per_axis[2][kz]:
+                                w = (wx[:, None, None] * wy[None, :, None]) * wz[None, None, :]
+                                re = w * pass_amp[0]
+                                im = w * pass_amp[1]
+                                # times the amplitude function's value, one
+                                boxes.append(DepositBox(component, (x0, y0, z0), w.shape,
+                                                        re * 1.0 - im * 0.0, re * 0.0 + im * 1.0,
+                                                        integrated))
     return boxes

@@ -86,8 +86,10 @@ def test_boundaries_set_a_fold_s_stored_count():
     assert drv.grid.shape == (8, 7, 8)
     assert FdtdDriver((1.6, 2.4, 1.6), 5.0, symmetry=("Y",),
                       boundaries="metallic").grid.shape == (8, 7, 8)
+    boxes = drv.add_volume_source("Ez", (0, 0.2, 0), (0, 0, 0), GaussianSource(1.0, 0.5))
+    assert boxes and all(0 < b.lo[1] < drv.grid.shape[1] for b in boxes)
     with pytest.raises(ValueError, match="mirror-folded"):
-        drv.add_volume_source("Ez", (0, 0.2, 0), (0, 0, 0), GaussianSource(1.0, 0.5))
+        drv.add_dft_monitor([1.0], ("Ez",))
 
 
 def test_boundary_spellings_agree():
