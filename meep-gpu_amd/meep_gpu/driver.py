@@ -454,8 +454,10 @@ class FdtdDriver:
     def from_workload(cls, wl: Workload) -> "FdtdDriver":
         drv = cls.__new__(cls)
         drv.grid = wl.grid
+        drv._cell, drv._resolution = tuple(wl.grid.cell_size), wl.grid.resolution
+        drv._courant, drv._dimensions = wl.grid.courant, wl.grid.dimensions
         drv.force_complex_fields = False
-        drv.symmetry = ()
+        drv.symmetry = _mirror_entries(wl.grid.symmetry)
         drv.k_point = (0.0, 0.0, 0.0)
         drv._shaped = True
         drv._inv = dict(wl.inv)

@@ -102,3 +102,14 @@ def test_boundary_spellings_agree():
         {0: "periodic", 1: "metallic", 2: "periodic"}
     with pytest.raises(ValueError, match="agree"):
         _boundary_kinds({"x": ("metallic", "periodic")})
+
+
+def test_a_workload_driver_takes_boundaries():
+    """``from_workload`` keeps what ``set_boundaries`` rebuilds the grid from."""
+    from meep_gpu import workloads
+
+    wl = workloads.sphere(edge=24, resolution=4.0, pml_cells=4, radius_cells=4.0, eps=2.25,
+                          fcen=0.15, df=0.1, nfreq=2, planes="none", steps=10)
+    drv = FdtdDriver.from_workload(wl)
+    drv.set_boundaries({"x": "metallic"})
+    assert drv.wraps == (False, True, True) and drv.grid.shape == wl.grid.shape
