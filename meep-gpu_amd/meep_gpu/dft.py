@@ -118,6 +118,24 @@ def folded_axis_cells(grid, axis: int, where_min: float, where_max: float, sign:
     return first, out
 
 
+def _folded_centres(grid, axis: int, where_min: float, where_max: float):
+    """The ladder's cell centres of a request over a folded axis's full cell: per centre, the
+    stored row serving it, its reflections and its weight (``Grid.fold_site``)."""
+    a = 1.0 / grid.dx
+    is_d, ie_d, ends = boundary_weight_ladder(where_min, where_max, a)
+    count = (ie_d - is_d) // 2 + 1
+    rows, flips, weights = [], [], []
+    for q in range(count):
+        site = grid.fold_site(axis, is_d + 2 * q, 1)
+        if site is None:
+            raise ValueError(f"monitor extent [{where_min}, {where_max}] leaves the grid")
+        rows.append(site[0])
+        flips.append(site[1])
+        weights.append(ladder_weight(q, count, *ends))
+    return (numpy.asarray(rows, dtype=numpy.int64), numpy.asarray(flips, dtype=numpy.int64),
+            numpy.asarray(weights))
+
+
 def axis_cells(n: int, dx: float, where_min: float, where_max: float) -> Tuple[int, numpy.ndarray]:
     """(first cell index, per-cell weights) of one resolved direction of a monitor volume whose
     coordinates have their origin at the cell centre (the grid's cell centres at
@@ -395,6 +413,11 @@ class DftFieldsMonitor:
     ``get_dft_array`` sums that axis away with them, and drops invariant axes, as MEEP's
     ``get_dft_array`` does (process_dft_component and collapse_array, reference
     from_meep.py:886-955).
+
+    ``folded`` maps a mirror-folded axis to its requested cell centres over the full cell:
+    ``(stored row offsets from lo, reflections, declared phase)``; ``get_dft_array`` reads each
+    centre from the row serving it, times the component's parity per reflection (MEEP's
+    symmetry images, ``Grid.fold_site``), and a flat folded axis's weights are per centre.
     """
     components: Tuple[str, ...]
     lo: Tuple[int, int, int]
@@ -405,6 +428,7 @@ class DftFieldsMonitor:
     yee_grid: bool = False
     empty_axes: dict = field(default_factory=dict)
     invariant: Tuple[bool, bool, bool] = (False, False, False)
+    folded: dict = field(default_factory=dict)
     _sim: object = field(default=None, repr=False)
     _slots: dict = field(default_factory=dict, repr=False)
 
@@ -424,8 +448,9 @@ class DftFieldsMonitor:
         self.lo = tuple(int(v) for v in self.lo)
         self.cnt = tuple(int(v) for v in self.cnt)
         for axis, w in self.empty_axes.items():
-            if len(w) != self.cnt[axis]:
-                raise ValueError(f"axis {axis}: {len(w)} weights for {self.cnt[axis]} cells")
+            sites = len(self.folded[axis][0]) if axis in self.folded else self.cnt[axis]
+            if len(w) != sites:
+                raise ValueError(f"axis {axis}: {len(w)} weights for {sites} cells")
 
     @classmethod
     def from_region(cls, grid, components, center, size, frequencies, decimation: int = 1,
@@ -435,11 +460,25 @@ class DftFieldsMonitor:
         the ladder's cells, a flat one the one or two cell-centre planes bracketing it, with
         their interpolation weights (reference ``set_region_from_volume`` /
         ``_register_volume``, dft.py:604-706)."""
-        lo, cnt, empty = [0, 0, 0], [1, 1, 1], {}
+        lo, cnt, empty, folded = [0, 0, 0], [1, 1, 1], {}, {}
+        folds = getattr(grid, "folds", (None, None, None))
         for a in range(3):
             if grid.is_invariant(a):
                 if size[a] != 0:
                     raise ValueError(f"axis {a} is invariant: the region must be flat along it")
+                continue
+            if folds[a] is not None:
+                if yee_grid:
+                    raise ValueError("a yee_grid region on a mirror-folded axis is not built: "
+                                     "each component's own lattice folds differently")
+                rows, flips, w = _folded_centres(grid, a, center[a] - 0.5 * size[a],
+                                                 center[a] + 0.5 * size[a])
+                if size[a] == 0:
+                    keep = [q for q in range(w.size) if w[q] != 0.0] or [0]
+                    rows, flips, w = rows[keep], flips[keep], w[keep]
+                    empty[a] = tuple(float(x) for x in w)
+                lo[a], cnt[a] = int(rows.min()), int(rows.max() - rows.min() + 1)
+                folded[a] = (rows - rows.min(), flips, int(folds[a][0]))
                 continue
             first, w = axis_cells(grid.shape[a], grid.dx, center[a] - 0.5 * size[a],
                                   center[a] + 0.5 * size[a])
@@ -453,7 +492,7 @@ class DftFieldsMonitor:
                 empty[a] = tuple(float(x) for x in w)
             lo[a], cnt[a] = first, w.size
         return cls(tuple(components), tuple(lo), tuple(cnt), frequencies, grid.dt, decimation,
-                   yee_grid, empty, tuple(grid.is_invariant(a) for a in range(3)))
+                   yee_grid, empty, tuple(grid.is_invariant(a) for a in range(3)), folded)
 
     @property
     def omegas(self) -> numpy.ndarray:
@@ -494,6 +533,13 @@ class DftFieldsMonitor:
         """``sim.get_dft_array(dft_fields, component, freq_index)``: the block with every flat
         axis summed away under its interpolation weights and every invariant axis dropped."""
         block = self.get_dft(component, freq_index)
+        for axis, (rows, flips, declared) in self.folded.items():  # the full cell's centres
+            own = "xyz".index(component[1]) == axis
+            parity = -declared if own != (component[0] == "H") else declared
+            sign = numpy.where(flips % 2 == 1, float(parity), 1.0)
+            view = [1, 1, 1]
+            view[axis] = sign.size
+            block = block.take(rows, axis=axis) * sign.reshape(view)
         for axis in (2, 1, 0):
             if axis in self.empty_axes:
                 w = numpy.asarray(self.empty_axes[axis], dtype=numpy.float64)

@@ -171,8 +171,8 @@ class FdtdDriver:
 
     def _require_unfolded(self, what: str):
         if any(f is not None for f in self.grid.folds):
-            raise ValueError(f"{what} on a mirror-folded grid is not built: the region's "
-                             "mirror images are not unfolded into its arrays")
+            raise ValueError(f"{what} on a mirror-folded grid is not built: give the region "
+                             "a center and size (the whole cell's images are not placed)")
 
     # --- configuration (before the first step) ------------------------------------------------
     def _require_unfrozen(self, what: str):
@@ -394,7 +394,8 @@ class FdtdDriver:
         without an absorber), resolved when stepping starts. ``decimation`` is an explicit
         positive accumulation stride."""
         self._require_unfrozen("add_dft_monitor")
-        self._require_unfolded("a DFT monitor")
+        if center is None:
+            self._require_unfolded("a DFT monitor without a region")
         self._shaped = True
         freqs = (numpy.atleast_1d(numpy.asarray(frequencies, dtype=numpy.float64))
                  if frequencies is not None else meep_frequency_span(fcen, df, nfreq))

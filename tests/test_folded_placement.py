@@ -67,3 +67,32 @@ def test_flux_region_rows_and_signed_weights():
     below = FluxMonitor.from_region(g, (0.0, -0.6, 0.0), (2.0, 0.0, 1.6), f)
     assert [r for r, _ in above.normal_planes] == [r for r, _ in below.normal_planes]
     assert [w for _, w in below.normal_planes] == [-w for _, w in above.normal_planes]
+
+
+def test_a_folded_dft_region_reads_its_images_with_the_parity():
+    """A DFT region over a folded axis registers the stored rows its centres land on, and its
+    array reads every centre of the full region from its row, the images times the parity."""
+    from meep_gpu.dft import DftFieldsMonitor
+
+    g = Grid(resolution=10, cell_size=(1.6, 3.2, 1.2), symmetry=(("Y", -1),))
+    m = DftFieldsMonitor.from_region(g, ("Ez", "Ey"), (0.0, -0.1, 0.0), (0.4, 1.0, 0.0),
+                                     numpy.array([1.0]), 1.0)
+    rows, flips, declared = m.folded[1]
+    # y in [-0.6, 0.4]: the ladder's centres at doubled -13 .. 9; -13 .. -3 are the images
+    # of 13 .. 3 (rows 7 .. 2), -1 is the owned pixel (row 0), 1 .. 9 rows 1 .. 5
+    assert list(flips) == [1] * 6 + [0] * 6 and declared == -1
+    assert list(m.lo[1] + rows) == [7, 6, 5, 4, 3, 2, 0, 1, 2, 3, 4, 5]
+
+    class Fake:
+        def flux_read(self, rid):
+            nz = m.cnt
+            data = numpy.zeros((2, 1) + nz, dtype=numpy.complex128)
+            data[0, 0] = numpy.arange(nz[1])[None, :, None] + 1.0  # Ez: the row number + 1
+            data[1, 0] = numpy.arange(nz[1])[None, :, None] + 1.0
+            return data
+
+    m._sim, m._slots = Fake(), {"Ez": (0, 0), "Ey": (0, 1)}
+    ez = m.get_dft_array("Ez")[0]   # Ez under an odd Y mirror: parity -1 (even x declared -1)
+    ey = m.get_dft_array("Ey")[0]   # Ey: its own axis, parity +1
+    assert list(ez.real) == [-(r + 1.0) if f else r + 1.0 for r, f in zip(rows, flips)]
+    assert list(ey.real) == [r + 1.0 for r in rows]
