@@ -319,9 +319,6 @@ class FdtdDriver:
         self._require_unfrozen("add_source")
         boxes = [box] if isinstance(box, DepositBox) else list(box)
         for b in boxes:
-            if self.force_complex_fields and b.integrated:
-                raise ValueError("a polarization (integrated) source on complex storage is "
-                                 "not built")
             for a, phase in enumerate(self.bloch_phases):
                 # stored row 0 of a Yee-shift-0 component on a phased axis holds the window
                 # top, the point one lattice vector above the bottom face (reference

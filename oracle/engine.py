@@ -571,9 +571,14 @@ class ComplexOracle:
             integrated=deposit.integrated))
 
     def step(self):
+        """One step of both parts. An electric polarization (integrated) source writes each
+        part's ``f_minus_p`` around update_E (reference sources.py:1697-1747: the real line
+        from the real part, the imaginary line from the imaginary part, as a deposit splits)."""
         re, im = self.re, self.im
-        if any(d.integrated for d in re.deposits):
-            raise NotImplementedError("a polarization source on complex storage")
+        if any(d.integrated and d.magnetic for d in re.deposits):
+            raise NotImplementedError("a magnetic polarization source on complex storage")
+        if any(d.integrated for d in re.deposits) and any(f is not None for f in re.folds):
+            raise NotImplementedError("a polarization source on a folded grid")
         for s in (re, im):
             s.step_B()
         for s in (re, im):
@@ -590,8 +595,11 @@ class ComplexOracle:
             s.fill_folds("D", far=False)
             s.zero_metal("D")
             s.fill_folds("D", far=True)
+        held = [s.f_minus_p_begin(False, s.time + s.dt) for s in (re, im)]
         for s in (re, im):
             s.update_E()
+        for s, h in zip((re, im), held):
+            s.f_minus_p_end(h)
         for s in (re, im):
             s.update_P()
         for s in (re, im):

@@ -21,6 +21,11 @@ RUNS = {
     "x_edge_metallic_y": dict(cell=(1.6, 1.2, 1.6), k=(0.5 / 1.6, 0.0, 0.0), pml={"z": 2},
                               offdiag=False, pole=True, symmetry=(),
                               boundaries={"y": "metallic"}),
+    # products on x and y with an electric polarization (integrated) source as well: each
+    # part subtracts its own line of the dipole before update_E
+    "xy_products_integrated": dict(cell=(1.6, 1.2, 2.0), k=(0.3, -0.45, 0.0), pml={"z": 2},
+                                   offdiag=False, pole=False, symmetry=(), boundaries=None,
+                                   integrated=True),
     # a product on x beside an odd mirror on y (the fold and the connection together)
     "x_product_odd_y_mirror": dict(cell=(1.6, 2.4, 1.6), k=(0.7, 0.0, 0.0), pml={"z": 2},
                                    offdiag=False, pole=False, symmetry=(("Y", -1),),
@@ -58,6 +63,9 @@ def driver(name: str, seed: int = 11, complex_fields: bool = True, **overrides) 
         if spec["symmetry"]:
             continue  # placement under a mirror is refused; boxes go in below
         drv.add_volume_source(comp, center, size, GaussianSource(f, fw), amplitude=amp)
+    if spec.get("integrated"):  # an electric polarization source off every cell face
+        drv.add_volume_source("Ey", (0.13, -0.07, 0.21), (0.0, 0.0, 0.0),
+                              GaussianSource(1.2, 0.5), amplitude=0.6 - 0.8j, is_integrated=True)
     if spec.get("flux"):  # a z-normal flux plane between the sources and the absorber
         drv.add_flux_monitor(2, (0, 0, shape[2] - 3), (shape[0], shape[1], 1),
                              frequencies=[0.9, 1.1, 1.3])
@@ -96,11 +104,14 @@ def oracle(drv: FdtdDriver, steps: int) -> oe.ComplexOracle:
                 n = int(numpy.floor(t / dt + 0.25))
                 return complex(table[n, 0], table[n, 1]) if 0 <= n < len(table) else 0j
 
+            if box.integrated:  # the dipole, evaluated at the oracle's own time
+                cur = fn.dipole
+
             re = numpy.asarray(box.amp_re, dtype=numpy.float64).reshape(-1)
             im = numpy.asarray(box.amp_im, dtype=numpy.float64).reshape(-1)
             if part == 1:
                 re, im = im, -re
-            sim.add_deposit(oe.PointDeposit(box.component, flat, re, im, cur, False))
+            sim.add_deposit(oe.PointDeposit(box.component, flat, re, im, cur, box.integrated))
         parts.append(sim)
     from meep_gpu.capsule import bloch_connection
 

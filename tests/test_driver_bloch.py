@@ -70,9 +70,9 @@ def test_refused_pairings():
     # Ex is Yee-shift 1 on x: row 0 is an owned cell there
     drv.add_source(DepositBox("Ex", (0, 2, 2), (1, 1, 1), numpy.ones(1), numpy.zeros(1)),
                    GaussianSource(1.0, 0.5))
-    with pytest.raises(ValueError, match="integrated"):
-        drv.add_source(DepositBox("Ey", (3, 2, 2), (1, 1, 1), numpy.ones(1), numpy.zeros(1),
-                                  integrated=True), GaussianSource(1.0, 0.5))
+    # a polarization (integrated) source is taken on complex storage, as each part's line
+    drv.add_source(DepositBox("Ey", (3, 2, 2), (1, 1, 1), numpy.ones(1), numpy.zeros(1),
+                              integrated=True), GaussianSource(1.0, 0.5))
 
 
 def test_boundaries_set_a_fold_s_stored_count():
@@ -113,3 +113,28 @@ def test_a_workload_driver_takes_boundaries():
     drv = FdtdDriver.from_workload(wl)
     drv.set_boundaries({"x": "metallic"})
     assert drv.wraps == (False, True, True) and drv.grid.shape == wl.grid.shape
+
+
+def test_complex_oracle_polarization_source_is_two_real_runs():
+    """With no Bloch phase the two parts of the complex oracle never meet: each part is the
+    real oracle run of its own line of the sources (the real line; the imaginary line as the
+    amplitudes (amp_im, -amp_re)), the polarization source's dipole included. A dropped or
+    misplaced ``f_minus_p`` around the complex step's update_E moves one of them."""
+    from tests import driver_bloch as db
+
+    steps = 30
+    drv = db.driver("xy_products_integrated", k=(0.0, 0.0, 0.0))
+    assert drv.bloch_phases == [None, None, None] or all(z is None for z in drv.bloch_phases)
+    assert any(box.integrated for box, _ in drv._sources)
+    linked = db.oracle(drv, steps)
+    alone = db.oracle(drv, steps)
+    for _ in range(steps):
+        linked.step()
+        for part in alone.parts:
+            part.step()
+    for a, b in zip(linked.parts, alone.parts):
+        for key, want in b.stored_volumes().items():
+            got = a.stored_volumes()[key]
+            assert got.view(numpy.uint32).tobytes() == want.view(numpy.uint32).tobytes(), key
+    assert numpy.any(linked.im.v["Ey"] != 0)
+    drv.close()

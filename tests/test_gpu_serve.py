@@ -40,6 +40,44 @@ def test_each_solve_is_the_same_alone_and_in_the_mix():
     assert s["p99_latency_s"] >= s["p50_latency_s"] > 0
 
 
+def test_served_results_match_the_oracle():
+    """Four solves (one with a Lorentz pole in its sphere) arriving together into two slots, so
+    the batch regroups as they complete: each result read out of the mix against the oracle's
+    run of the same request. Fields: the final E and H words (their sha256, as the server
+    records them) equal the oracle's. Spectra: within 1e-9 of the oracle's float64 peak (the
+    transform sums the same float64 products blocked over 128 steps)."""
+    import hashlib
+
+    from meep_gpu.serve import SolveRequest, SolveServer, request_workload
+    from tests import workload_runs as wr
+
+    reqs = [SolveRequest(rid=0, arrival=0.0, edge=30, steps=150, dispersive=True, nplanes=3,
+                         nfreq=6),
+            SolveRequest(rid=1, arrival=0.0, edge=34, steps=110, nplanes=2, nfreq=5),
+            SolveRequest(rid=2, arrival=0.0, edge=28, steps=190, nplanes=1, nfreq=7),
+            SolveRequest(rid=3, arrival=0.0, edge=32, steps=70, nplanes=4, nfreq=4,
+                         dispersive=True)]
+    srv = SolveServer(max_slots=2, quantum=48, checksums=True)
+    rep = srv.run(reqs)
+    assert rep.peak_resident == 2
+    for req, res in zip(reqs, rep.results):
+        print(f"request {req.rid} edge {req.edge} steps {req.steps} against the oracle",
+              flush=True)
+        sim, planes = wr.oracle_run(request_workload(req), req.steps,
+                                    reference_precision=False)
+        stored = sim.stored_volumes()
+        h = hashlib.sha256()
+        for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+            h.update(numpy.ascontiguousarray(stored[comp], dtype=numpy.float32).tobytes())
+        assert res.checksum == h.hexdigest(), req.rid
+        assert len(res.spectra) == len(planes) == req.nplanes
+        for got, plane in zip(res.spectra, planes):
+            want = plane.flux(request_workload(req).grid.dx, "c128")
+            peak = numpy.max(numpy.abs(want))
+            assert peak > 0, req.rid
+            assert numpy.max(numpy.abs(got - want)) <= 1e-9 * peak, req.rid
+
+
 def test_the_admission_footprint_bounds_the_allocation():
     from meep_gpu.serve import HipBackend, request_footprint, request_workload
 
