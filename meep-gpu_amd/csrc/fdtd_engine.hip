@@ -1642,7 +1642,11 @@ template <int V, int C, bool PML>
 __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Run &q) {
     VF<V> p[kMaxPoles];
     const bool active = pole_run_active(s, q, V);
-    const VF<V> dmp = minus_poles<V, C, kMaxPoles>(s, ck, q, ldv<V>(s.D[C], q.at), p, active);
+    VF<V> d = ldv<V>(s.D[C], q.at);
+    // the polarization sources' dipole (on a folded grid, whose ghost rows carry their images'
+    // boxes; a real off-diagonal row refuses such sources)
+    if (s.npsrc) polarize_run<V>(s, ck.step, C, q, d);
+    const VF<V> dmp = minus_poles<V, C, kMaxPoles>(s, ck, q, d, p, active);
     const VF<V> iv = ldv<V>(s.inv[C], q.at);
     VF<V> cst;
     const float *u1 = s.u[C][(C + 1) % 3], *u2 = s.u[C][(C + 2) % 3];
@@ -3504,9 +3508,6 @@ int fdtd_set_fold(fdtd_sim *s, int32_t axis, int32_t kind, int32_t phase, int32_
         if (h.n[axis] < 4) return fail(FDTD_EINVAL, "a folded axis needs 4 stored cells, axis %d has %d", axis, h.n[axis]);
         if (kind == 2 && (far_row < 3 || far_row >= h.n[axis] - 1))
             return fail(FDTD_EINVAL, "far ghost row %d is not an interior row of axis %d", (int)far_row, axis);
-        if (h.npsrc)
-            return fail(FDTD_EINVAL, "fdtd_set_fold: a polarization (integrated) source on a "
-                                     "folded grid is not supported");
         if (!s->planes.empty())
             return fail(FDTD_EINVAL, "fdtd_set_fold: flux and DFT regions on a folded grid are "
                                      "not supported (fold before adding none)");
@@ -3822,10 +3823,10 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
                                  "not supported: H is kept apart from B only on the layer rows, "
                                  "and such a source makes H differ from B at its points");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    if (field == 2 && (s->host.fold[0] || s->host.fold[1] || s->host.fold[2]))
-        return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source on a "
-                                 "folded grid is not supported");
-    if (field == 2 && s->host.offdiag)
+    bool rows = false;  // a real off-diagonal chi1inv row (a fold alone also takes that path)
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) rows |= s->host.u[r][q] != nullptr;
+    if (field == 2 && rows)
         return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source with an "
                                  "off-diagonal chi1inv is not supported: the off-diagonal rows "
                                  "read D at neighbouring cells, where MEEP reads f_minus_p");

@@ -28,7 +28,8 @@ from .fastpath import FastPathError, last_dispatch_report, plan_fast_path
 from .grid import Grid
 from .pml import PML
 from .readback import meep_array
-from .sources import DepositBox, GaussianSource, current_table, meep_volume_source
+from .sources import (DepositBox, GaussianSource, current_table, meep_volume_source,
+                      polarization_boxes)
 from .workloads import Workload
 
 _E = ("Ex", "Ey", "Ez")
@@ -145,6 +146,7 @@ class FdtdDriver:
         self.sim: Optional[HipSimulation] = None
         self._fast_path = None
         self._planned = 0
+        self._source_slots: List[List[int]] = []  # engine source indices of each source
         self._shaped = False  # a volume, source or monitor placed on the grid's cells
         if boundaries is not None:
             self.set_boundaries(boundaries)
@@ -557,15 +559,25 @@ class FdtdDriver:
                 for name, vol in self.conductivity_volumes().items():
                     # MEEP keeps a chunk's conductivity only where it samples a nonzero sigma
                     sim.set_conductivity(name, vol, division.conductive_mask(vol, IYEE[name]))
+            # a polarization box goes in as its pieces (off ghost and wall rows, plus its
+            # images on the ghost rows); each source's pieces share its current table
+            slots, count = [], 0
+            metallic = tuple(not w for w in self.wraps)
             for box, fn in self._sources:
                 table = current_table(fn, self.grid.dt, steps, box.magnetic, box.integrated)
                 axis = "xyz".index(box.component[1])
-                amp_re, amp_im = box.amp_re, box.amp_im
-                if part == 1:
-                    amp_re, amp_im = (numpy.asarray(box.amp_im, dtype=numpy.float64),
-                                      -numpy.asarray(box.amp_re, dtype=numpy.float64))
-                sim.add_source(box.magnetic, axis, box.lo, box.cnt, amp_re, amp_im, table,
-                               self.grid.dt, box.integrated)
+                mine = []
+                for piece in polarization_boxes(self.grid, box, metallic):
+                    amp_re, amp_im = piece.amp_re, piece.amp_im
+                    if part == 1:
+                        amp_re, amp_im = (numpy.asarray(piece.amp_im, dtype=numpy.float64),
+                                          -numpy.asarray(piece.amp_re, dtype=numpy.float64))
+                    mine.append(count)
+                    count += 1
+                    sim.add_source(piece.magnetic, axis, piece.lo, piece.cnt, amp_re, amp_im,
+                                   table, self.grid.dt, piece.integrated)
+                slots.append(mine)
+            self._source_slots = slots
             if pml is None and not self._poles and not self._offdiag:
                 sim.prime_e()
         except Exception:
@@ -584,9 +596,10 @@ class FdtdDriver:
         if steps <= self._planned:
             return
         planned = max(int(steps), 2 * self._planned)
-        for q, (box, fn) in enumerate(self._sources):
-            self.sim.set_source_currents(q, current_table(fn, self.grid.dt, planned, box.magnetic,
-                                                          box.integrated))
+        for slots, (box, fn) in zip(self._source_slots, self._sources):
+            table = current_table(fn, self.grid.dt, planned, box.magnetic, box.integrated)
+            for q in slots:
+                self.sim.set_source_currents(q, table)
         self._planned = planned
 
     def step(self):

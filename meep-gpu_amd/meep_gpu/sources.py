@@ -360,3 +360,73 @@ def meep_volume_source(grid, component: str, center: Sequence[float], size: Sequ
                                                         re * 1.0 - im * 0.0, re * 0.0 + im * 1.0,
                                                         integrated))
     return boxes
+
+
+def polarization_boxes(grid, box: DepositBox, metallic: Sequence[bool]) -> List[DepositBox]:
+    """An electric polarization (integrated) box as the engine's update_E subtracts it, box by
+    box at each box's own cells.
+
+    MEEP writes ``f_minus_p`` at the box's cells and then runs the D boundary passes again
+    (reference sources.py:1697-1747, ``_boundary_passes``): a ghost row of a folded axis takes
+    its image row's ``f_minus_p`` times the component's parity, and a wall row of a metallic
+    axis is cleared. So the box is returned without its ghost and wall rows, followed by one
+    image box per set of ghost planes its cells are images of: on those planes' ghost rows, the
+    amplitudes times the product of the planes' parities. A parity is a sign, so at a ghost the
+    engine's ``float(double(parity * D) - parity * r)`` is exactly ``parity`` times the image's
+    word. Ghost rows (the engine's fold fill): row 0 for a component whose lattice puts a
+    sample on the plane, the far row of a periodic fold for one whose lattice does not."""
+    if not box.integrated or box.magnetic:
+        return [box]
+    comp = "xyz".index(box.component[1])
+    n = grid.shape
+    re = numpy.asarray(box.amp_re, dtype=numpy.float64).reshape(box.cnt)
+    im = numpy.asarray(box.amp_im, dtype=numpy.float64).reshape(box.cnt)
+    lo = list(box.lo)
+    hi = [l + c for l, c in zip(box.lo, box.cnt)]
+    ghosts = {}  # axis -> (ghost row, image row, parity)
+    for a in range(3):
+        fold = grid.folds[a]
+        on_plane = a != comp  # an electric component's integer lattice on this axis
+        cut = []
+        if fold is not None:
+            if on_plane:
+                ghosts[a] = (0, 2, _fold_parity(grid, box.component, a))
+            elif fold[1] == 2:
+                ghosts[a] = (n[a] - 1, fold[2], _fold_parity(grid, box.component, a))
+            if a in ghosts:
+                cut.append(ghosts[a][0])
+        elif metallic[a] and on_plane:
+            cut.append(0)
+        for row in cut:
+            if lo[a] < hi[a] and row == lo[a]:
+                lo[a] += 1
+            elif lo[a] < hi[a] and row == hi[a] - 1:
+                hi[a] -= 1
+
+    def piece(rows, sign: float, at):
+        sl = tuple(slice(r0 - box.lo[a], r1 - box.lo[a]) for a, (r0, r1) in enumerate(rows))
+        return DepositBox(box.component, tuple(at), tuple(r1 - r0 for r0, r1 in rows),
+                          re[sl] * sign, im[sl] * sign, True)
+
+    out = []
+    if all(l < h for l, h in zip(lo, hi)):
+        out.append(piece(list(zip(lo, hi)), 1.0, lo))
+    axes = sorted(ghosts)
+    for mask in range(1, 1 << len(axes)):
+        rows, at, sign = [], [], 1.0
+        for a in range(3):
+            if a in axes and mask >> axes.index(a) & 1:
+                ghost, image, parity = ghosts[a]
+                if not lo[a] <= image < hi[a]:
+                    break
+                rows.append((image, image + 1))
+                at.append(ghost)
+                sign *= parity
+            else:
+                if lo[a] >= hi[a]:
+                    break
+                rows.append((lo[a], hi[a]))
+                at.append(lo[a])
+        else:
+            out.append(piece(rows, sign, at))
+    return out

@@ -502,15 +502,23 @@ class OracleSim:
                 flat = numpy.unique(numpy.concatenate(cells))
                 saved.append((name + c, flat, self.v[name + c].reshape(-1)[flat].copy()))
         self.deposit(magnetic, time_value, integrated=True)
-        self.zero_metal(name)
+        self._boundary_passes(name)
         return saved
+
+    def _boundary_passes(self, family: str):
+        """The family's boundary passes run again over ``f_minus_p`` (reference
+        sources.py:1749 ``_boundary_passes``): the mirror ghosts from their images, the walls
+        cleared, the far ghosts of periodic folds (each a no-op without folds or walls)."""
+        self.fill_folds(family, far=False)
+        self.zero_metal(family)
+        self.fill_folds(family, far=True)
 
     def f_minus_p_end(self, saved):
         if saved is None:
             return
         for name, flat, words in saved:
             self.v[name].reshape(-1)[flat] = words
-        self.zero_metal(name[0])
+        self._boundary_passes(name[0])
 
     def step(self):
         """The reference driver's step (driver.py:3524-3590)."""
@@ -577,8 +585,6 @@ class ComplexOracle:
         re, im = self.re, self.im
         if any(d.integrated and d.magnetic for d in re.deposits):
             raise NotImplementedError("a magnetic polarization source on complex storage")
-        if any(d.integrated for d in re.deposits) and any(f is not None for f in re.folds):
-            raise NotImplementedError("a polarization source on a folded grid")
         for s in (re, im):
             s.step_B()
         for s in (re, im):

@@ -26,6 +26,14 @@ RUNS = {
     "xy_products_integrated": dict(cell=(1.6, 1.2, 2.0), k=(0.3, -0.45, 0.0), pml={"z": 2},
                                    offdiag=False, pole=False, symmetry=(), boundaries=None,
                                    integrated=True),
+    # polarization sources on a folded axis: Ez one cell above the odd Y mirror (its image
+    # row feeds the near ghost row) and Ey below the periodic fold's second mirror (the far
+    # ghost row)
+    "odd_y_mirror_integrated": dict(cell=(1.6, 2.4, 1.6), k=(0.7, 0.0, 0.0), pml={"z": 2},
+                                    offdiag=False, pole=False, symmetry=(("Y", -1),),
+                                    boundaries=None,
+                                    integrated=(("Ez", (0.13, 0.2, 0.21)),
+                                                ("Ey", (-0.31, 1.1, -0.17)))),
     # a product on x beside an odd mirror on y (the fold and the connection together)
     "x_product_odd_y_mirror": dict(cell=(1.6, 2.4, 1.6), k=(0.7, 0.0, 0.0), pml={"z": 2},
                                    offdiag=False, pole=False, symmetry=(("Y", -1),),
@@ -63,9 +71,12 @@ def driver(name: str, seed: int = 11, complex_fields: bool = True, **overrides) 
         if spec["symmetry"]:
             continue  # placement under a mirror is refused; boxes go in below
         drv.add_volume_source(comp, center, size, GaussianSource(f, fw), amplitude=amp)
-    if spec.get("integrated"):  # an electric polarization source off every cell face
-        drv.add_volume_source("Ey", (0.13, -0.07, 0.21), (0.0, 0.0, 0.0),
-                              GaussianSource(1.2, 0.5), amplitude=0.6 - 0.8j, is_integrated=True)
+    integrated = spec.get("integrated")
+    if integrated is True:  # an electric polarization source off every cell face
+        integrated = (("Ey", (0.13, -0.07, 0.21)),)
+    for comp, center in integrated or ():
+        drv.add_volume_source(comp, center, (0.0, 0.0, 0.0), GaussianSource(1.2, 0.5),
+                              amplitude=0.6 - 0.8j, is_integrated=True)
     if spec.get("flux"):  # a z-normal flux plane between the sources and the absorber
         drv.add_flux_monitor(2, (0, 0, shape[2] - 3), (shape[0], shape[1], 1),
                              frequencies=[0.9, 1.1, 1.3])
