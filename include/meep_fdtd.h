@@ -116,8 +116,9 @@ int fdtd_add_pole(fdtd_sim *sim, const float consts[4], const float *const sigma
  * field 2 is an electric polarization (MEEP is_integrated) source, which replaces
  * sources.f_minus_p_begin/_end (sources.py:1697-1760): nothing is deposited into D; the table
  * is the dipole P at (n+1)*dt, and update_E reads float(double(D) - (amp_re*P_re -
- * amp_im*P_im)) at the box's cells (no dt, no condinv) while D keeps its word. field 3 (a
- * magnetic polarization source) and field 2 with an off-diagonal chi1inv return FDTD_EINVAL.
+ * amp_im*P_im)) at the box's cells (no dt, no condinv) while D keeps its word; an off-diagonal
+ * chi1inv row reads the same replaced words at its neighbouring cells. field 3 (a magnetic
+ * polarization source) returns FDTD_EINVAL.
  * The subtraction is at the box's own cells only: on a folded grid the caller passes a box
  * without its ghost rows plus its images on them (amplitudes times the fold parity), and none
  * on a metallic wall row, as MEEP's boundary passes over f_minus_p leave them. */

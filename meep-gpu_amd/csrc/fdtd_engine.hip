@@ -1601,6 +1601,45 @@ __device__ inline float dmp_at(const SimDev &s, int ppar, int comp, int at) {
     return v;
 }
 
+// The electric polarization sources of component `comp` subtracted from the word x at padded
+// index `at`, as polarize_run does for a run (the same products and roundings, source by source
+// in list order): MEEP's f_minus_p at a cell the off-diagonal rows read beside their own.
+__device__ inline float polarize_at(const SimDev &s, int step, int comp, int at, float x) {
+    const int i = at % s.pitch, r = at / s.pitch, j = r % s.n[1], k = r / s.n[1];
+    if (i < s.src_lo[2][0] || i >= s.src_hi[2][0] || j < s.src_lo[2][1] || j >= s.src_hi[2][1] ||
+        k < s.src_lo[2][2] || k >= s.src_hi[2][2])
+        return x;
+    for (int n = 0; n < s.nsrc; ++n) {
+        const GMEM SrcDev &d = gp(s.src)[n];
+        if (d.field != 2 || d.comp != comp) continue;
+        const int a = i - d.lo[0], b = j - d.lo[1], e = k - d.lo[2];
+        if (a < 0 || b < 0 || e < 0 || a >= d.cnt[0] || b >= d.cnt[1] || e >= d.cnt[2]) continue;
+        if (step >= d.ncur) continue;
+        const double pre = gp(d.cur)[2 * step], pim = gp(d.cur)[2 * step + 1];
+        if (plus_zero(pre) && plus_zero(pim)) continue;
+        const int p = (e * d.cnt[1] + b) * d.cnt[0] + a;
+        double rr = gp(d.amp_re)[p] * pre;
+        rr = rr - gp(d.amp_im)[p] * pim;
+        double w = (double)x;
+        w = w - rr;
+        x = (float)w;
+    }
+    return x;
+}
+
+// f_minus_p less the poles' P at padded index `at`: (D - dipole) - P, the order of the
+// constitutive update's own read.
+__device__ inline float fmp_at(const SimDev &s, int step, int ppar, int comp, int at) {
+    float v = gp(s.D[comp])[at];
+    if (s.npsrc) v = polarize_at(s, step, comp, at, v);
+#pragma unroll
+    for (int pq = 0; pq < kMaxPoles; ++pq) {
+        if (pq >= s.npol) break;
+        if (s.pdriven[pq][comp]) v = v - gp(s.pbuf[pq][comp][ppar])[at];
+    }
+    return v;
+}
+
 // Padded-index offset of the +1 / -1 neighbour of cell (i, j, k) along `axis` (periodic).
 __device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, int dir) {
     if (axis == 0) {
@@ -1616,20 +1655,21 @@ __device__ inline int step_off(const SimDev &s, const Run &q, int i, int axis, i
 // plus its lower neighbour along `pa` -- zero past a metallic wall, the mirror ghost on a
 // folded axis (the parity of D_pa about its own plane, -phase, times stored row 2), else the
 // periodic neighbour (stepping._offdiagonal_terms :1594-1687 with _shift_down :2485).
-__device__ inline float offdiag_pair(const SimDev &s, const SimDev *o, int ppar, int pa, int at,
-                                     const int c3[3], int down) {
-    float v = dmp_at(s, ppar, pa, at);
+__device__ inline float offdiag_pair(const SimDev &s, const SimDev *o, int step, int ppar, int pa,
+                                     int at, const int c3[3], int down) {
+    float v = fmp_at(s, step, ppar, pa, at);
     if (s.pkind[pa] && c3[pa] == 0)
-        v = connect_value(s, pa, false, v, (s.pkind[pa] == 3 && o) ? dmp_at(*o, ppar, pa, at) : 0.0f);
+        v = connect_value(s, pa, false, v,
+                          (s.pkind[pa] == 3 && o) ? fmp_at(*o, step, ppar, pa, at) : 0.0f);
     float lower;
     if (s.metal[pa] && c3[pa] == 0) {
         lower = 0.0f;
     } else if (s.fold[pa] && c3[pa] == 0) {
         const int stride = pa == 0 ? 1 : (pa == 1 ? s.pitch : s.pitch * s.n[1]);
-        const float m = dmp_at(s, ppar, pa, at + 2 * stride);
+        const float m = fmp_at(s, step, ppar, pa, at + 2 * stride);
         lower = s.fphase[pa] > 0 ? -m : m;
     } else {
-        lower = dmp_at(s, ppar, pa, at + down);
+        lower = fmp_at(s, step, ppar, pa, at + down);
     }
     return v + lower;
 }
@@ -1674,15 +1714,15 @@ __device__ inline void erow_component(const SimDev &s, const Clock &ck, const Ru
             if (C == 2) u3[2] = (q.k + 1 < s.n[2]) ? q.k + 1 : 0;
             const int down0 = step_off(s, q, i, pa, -1);
             const int down1 = step_off(s, q, iu, pa, -1);
-            float prod0 = gp(U)[at] * offdiag_pair(s, s.other, ck.ppar, pa, at, c3, down0);
+            float prod0 = gp(U)[at] * offdiag_pair(s, s.other, ck.step, ck.ppar, pa, at, c3, down0);
             if (s.pkind[C] && c3[C] == 0) {
                 float other = 0.0f;
                 if (s.pkind[C] == 3 && s.other)
-                    other = gp(U)[at] * offdiag_pair(*s.other, &s, ck.ppar, pa, at, c3, down0);
+                    other = gp(U)[at] * offdiag_pair(*s.other, &s, ck.step, ck.ppar, pa, at, c3, down0);
                 prod0 = connect_value(s, C, true, prod0, other);
             }
             float prod1 = 0.0f;
-            if (!top_c) prod1 = gp(U)[up] * offdiag_pair(s, s.other, ck.ppar, pa, up, u3, down1);
+            if (!top_c) prod1 = gp(U)[up] * offdiag_pair(s, s.other, ck.step, ck.ppar, pa, up, u3, down1);
             float summed = prod0 + prod1;
             if ((s.metal[(C + 1) % 3] && c3[(C + 1) % 3] == 0) ||
                 (s.metal[(C + 2) % 3] && c3[(C + 2) % 3] == 0))
@@ -3706,9 +3746,6 @@ int fdtd_set_offdiagonal(fdtd_sim *s, int32_t row, int32_t partner, const float 
     if (!s || row < 0 || row > 2 || partner < 0 || partner > 2 || row == partner || !vol)
         return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: bad argument");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    if (s->host.npsrc)
-        return fail(FDTD_EINVAL, "fdtd_set_offdiagonal: an off-diagonal chi1inv with a "
-                                 "polarization (integrated) source is not supported");
     s->graph.reset();  // the off-diagonal path launches other kernels
     int rc = s->pull_desc();
     if (rc) return rc;
@@ -3823,13 +3860,6 @@ int fdtd_add_source(fdtd_sim *s, int32_t field, int32_t comp, const int32_t lo[3
                                  "not supported: H is kept apart from B only on the layer rows, "
                                  "and such a source makes H differ from B at its points");
     if (s->in_batch) return fail(FDTD_ESTATE, "the simulation is stepped by a batch");
-    bool rows = false;  // a real off-diagonal chi1inv row (a fold alone also takes that path)
-    for (int r = 0; r < 3; ++r)
-        for (int q = 0; q < 3; ++q) rows |= s->host.u[r][q] != nullptr;
-    if (field == 2 && rows)
-        return fail(FDTD_EINVAL, "fdtd_add_source: a polarization (integrated) source with an "
-                                 "off-diagonal chi1inv is not supported: the off-diagonal rows "
-                                 "read D at neighbouring cells, where MEEP reads f_minus_p");
     int64_t npts = 1;
     for (int a = 0; a < 3; ++a) {
         if (cnt[a] < 1 || lo[a] < 0 || lo[a] + cnt[a] > s->host.n[a])

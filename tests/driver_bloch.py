@@ -26,6 +26,13 @@ RUNS = {
     "xy_products_integrated": dict(cell=(1.6, 1.2, 2.0), k=(0.3, -0.45, 0.0), pml={"z": 2},
                                    offdiag=False, pole=False, symmetry=(), boundaries=None,
                                    integrated=True),
+    # products on x and y, an off-diagonal Ex-Ey row and a polarization source: the row's
+    # neighbour reads take the dipole-less words too
+    "xy_products_offdiag_integrated": dict(cell=(1.6, 1.2, 2.0), k=(0.3, -0.45, 0.0),
+                                           pml={"z": 2}, offdiag=True, pole=True, symmetry=(),
+                                           boundaries=None,
+                                           integrated=(("Ey", (0.13, -0.07, 0.21)),
+                                                       ("Ex", (-0.27, 0.31, -0.13)))),
     # polarization sources on a folded axis: Ez one cell above the odd Y mirror (its image
     # row feeds the near ghost row) and Ey below the periodic fold's second mirror (the far
     # ghost row)

@@ -1,8 +1,9 @@
-"""Polarization (integrated) sources on the GPU where the D boundary passes touch them: an odd
-Y mirror over periodic boundaries with sources on the images of its near and far ghost rows,
-and a metallic x wall with a box straddling Ey's wall row. Every stored word against the
-oracle (which subtracts the dipole and runs MEEP's boundary passes again), consult by consult
-and replayed from captured graphs; and the folded run against the unfolded one."""
+"""Polarization (integrated) sources on the GPU where their replaced words are read beyond
+their own cells: an odd Y mirror over periodic boundaries with sources on the images of its
+near and far ghost rows, a metallic x wall with a box straddling Ey's wall row, and an
+off-diagonal chi1inv row (whose neighbour reads take the replaced words). Every stored word
+against the oracle (which replaces D in place and runs MEEP's boundary passes again), consult
+by consult and replayed from captured graphs; and the folded run against the unfolded one."""
 
 import numpy
 import pytest
@@ -21,11 +22,22 @@ def _run(drv, steps: int):
     drv.synchronize()
 
 
+def _driver(case: str, steps: int):
+    if case == "mirror":
+        return fp.mirror_driver(True, steps)
+    if case == "wall":
+        return fp.wall_driver(steps)
+    # real storage with an off-diagonal Ex-Ey row, a pole and two polarization sources
+    from tests import driver_bloch as db
+
+    return db.driver("xy_products_offdiag_integrated", complex_fields=False, k=(0.0, 0.0, 0.0))
+
+
 @pytest.mark.parametrize("steps", [30, 300])
-@pytest.mark.parametrize("case", ["mirror", "wall"])
+@pytest.mark.parametrize("case", ["mirror", "wall", "offdiag"])
 def test_polarization_sources_match_the_oracle(case, steps):
     print(f"polarization sources, {case} x{steps}", flush=True)
-    drv = fp.mirror_driver(True, steps) if case == "mirror" else fp.wall_driver(steps)
+    drv = _driver(case, steps)
     _run(drv, steps)
     ref = fp.oracle_of(drv, steps)
     for _ in range(steps):
